@@ -1,0 +1,1341 @@
+/*
+FP64 CPU oracle for the PAMHD ideal-MHD finite-volume step.
+
+TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the CPU
+baseline of bench.py as the checker; never linked into the product.
+
+Build with -O2 -ffp-contract=off (no fused multiply-add) so that every
+expression rounds exactly as the reference's x86-64 build does.  The
+operation order of each expression follows the cited reference line;
+where the reference writes pamhd::add(a, pamhd::neg(b)) this file writes
+a - b (IEEE subtraction is defined as a + (-b)), and pamhd::mul(s, v) is
+v[i] * s (multiplication commutes exactly).
+*/
+#include "pamhd_oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+/* std::max / std::min semantics (the first argument wins ties and NaNs) */
+static inline double smax(double a, double b) { return (a < b) ? b : a; }
+static inline double smin(double a, double b) { return (b < a) ? b : a; }
+
+enum { RHO = 0, M0 = 1, M1 = 2, M2 = 3, NRJ = 4, B0 = 5, B1 = 6, B2 = 7 };
+
+/* pamhd::dot / norm2 of common_functions.hpp:143-163: (a0b0 + a1b1) + a2b2 */
+static inline double dot3(const double* a, const double* b)
+{
+	return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
+}
+
+/* ---------------------------------------------------------------------- */
+/* physics helpers                                                        */
+/* ---------------------------------------------------------------------- */
+
+/* source/mhd/common.hpp:76-115 */
+int or_get_pressure(const double U[8], double gamma, double mu0, double* p)
+{
+	if (U[RHO] <= 0) return OR_ERR_DENSITY;
+	const double
+		kinetic = 0.5 / U[RHO] * (U[M0] * U[M0] + U[M1] * U[M1] + U[M2] * U[M2]),
+		magnetic = 0.5 / mu0 * (U[B0] * U[B0] + U[B1] * U[B1] + U[B2] * U[B2]);
+	*p = (U[NRJ] - kinetic - magnetic) * (gamma - 1);
+	return OR_OK;
+}
+
+/* source/mhd/common.hpp:254-311 (argument checks made by callers) */
+double or_total_energy(double rho, const double v[3], double p, const double B[3],
+	double gamma, double mu0)
+{
+	const double
+		kinetic = 0.5 * rho * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]),
+		magnetic = 0.5 / mu0 * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+	return p / (gamma - 1) + kinetic + magnetic;
+}
+
+/* source/mhd/common.hpp:324-368 */
+static int sound_speed(const double U[8], double gamma, double mu0, double* cs)
+{
+	if (!isnormal(U[RHO]) || U[RHO] < 0) return OR_ERR_DENSITY;
+	double p;
+	int e = or_get_pressure(U, gamma, mu0, &p);
+	if (e) return e;
+	if (!isnormal(p) || p < 0) return OR_ERR_PRESSURE;
+	*cs = sqrt(gamma * p / U[RHO]);
+	return OR_OK;
+}
+
+/* source/mhd/common.hpp:415-491 with get_alfven_speed of :378-405 */
+int or_fast_speed(const double U[8], const double bg[3], double gamma, double mu0,
+	double* cf)
+{
+	const double t[3] = {U[B0] + bg[0], U[B1] + bg[1], U[B2] + bg[2]};
+	if (!isfinite(t[0])) return OR_ERR_FIELD;
+	const double mag = sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+	double sound;
+	int e = sound_speed(U, gamma, mu0, &sound);
+	if (e) return e;
+	if (mag <= 0) {
+		*cf = sound;
+		return OR_OK;
+	}
+	/* get_alfven_speed: |B + B0| / sqrt(mu0 rho); its own density check
+	   cannot fire here because sound_speed made the same check */
+	const double alfven = mag / sqrt(mu0 * U[RHO]);
+	const double
+		sound2 = sound * sound,
+		alfven2 = alfven * alfven,
+		speeds2 = sound2 + alfven2;
+	if (!isnormal(speeds2) || speeds2 < 0) return OR_ERR_SPEED;
+	const double ratio = t[0] / mag;
+	const double to_sqrt
+		= sound2 * sound2 + alfven2 * alfven2
+		+ 2 * sound2 * alfven2 * (1 - 2 * (ratio * ratio));
+	if (!isnormal(to_sqrt) || to_sqrt < 0) return OR_ERR_SPEED;
+	*cf = sqrt(0.5 * (speeds2 + sqrt(to_sqrt)));
+	return OR_OK;
+}
+
+/* get_velocity, source/mhd/common.hpp:118-130 */
+static inline void velocity(const double U[8], double v[3])
+{
+	if (U[RHO] > 0) {
+		const double inv = 1 / U[RHO];
+		v[0] = U[M0] * inv;
+		v[1] = U[M1] * inv;
+		v[2] = U[M2] * inv;
+	} else {
+		v[0] = v[1] = v[2] = 0;
+	}
+}
+
+/* ideal-MHD x-flux with background-field splitting, source/mhd/common.hpp:138-244 */
+int or_physical_flux(const double U[8], const double bg[3], double gamma,
+	double mu0, double F[8])
+{
+	if (!isnormal(U[RHO]) || U[RHO] <= 0) return OR_ERR_DENSITY;
+	if (!isfinite(U[M0])) return OR_ERR_FLUX;
+
+	const double* B = U + B0;
+	const double t[3] = {B[0] + bg[0], B[1] + bg[1], B[2] + bg[2]};
+	const double
+		inv = 1.0 / mu0,
+		pB0 = 0.5 * inv * (bg[0] * bg[0] + bg[1] * bg[1] + bg[2] * bg[2]),
+		pB1 = 0.5 * inv * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]),
+		pBt = 0.5 * inv * (t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
+	double pth;
+	int e = or_get_pressure(U, gamma, mu0, &pth);
+	if (e) return e;
+	double v[3];
+	velocity(U, v);
+
+	F[RHO] = U[M0];
+	if (!isfinite(F[RHO])) return OR_ERR_FLUX;
+
+	/* add(mul(m, v0), mul(-inv, add(mul(t0, t), mul(-bg0, bg)))) */
+	for (int i = 0; i < 3; i++) {
+		F[M0 + i] = U[M0 + i] * v[0] + (t[i] * t[0] + bg[i] * (-bg[0])) * (-inv);
+	}
+	F[M0] += pth + pBt - pB0;
+
+	F[NRJ]
+		= v[0] * (U[NRJ] + pth + pB1)
+		- B[0] * dot3(v, B) * inv
+		+ inv * (
+			B[1] * (v[0] * bg[1] - v[1] * bg[0])
+			+ B[2] * (v[0] * bg[2] - v[2] * bg[0]));
+
+	/* add(mul(v0, t), mul(-t0, v)) */
+	for (int i = 0; i < 3; i++) {
+		F[B0 + i] = t[i] * v[0] + v[i] * (-t[0]);
+	}
+	F[B0] = 0;
+	return OR_OK;
+}
+
+/* ---------------------------------------------------------------------- */
+/* Riemann solvers                                                        */
+/* ---------------------------------------------------------------------- */
+
+/* source/mhd/rusanov.hpp:59-214 */
+int or_flux_rusanov(const double L[8], const double R[8], const double bg[3],
+	double gamma, double mu0, double F[8], double* s_out)
+{
+	if (!isnormal(L[RHO]) || L[RHO] < 0) return OR_ERR_DENSITY;
+	if (!isnormal(R[RHO]) || R[RHO] < 0) return OR_ERR_DENSITY;
+	double cfL, cfR;
+	int e = or_fast_speed(L, bg, gamma, mu0, &cfL);
+	if (e) return e;
+	e = or_fast_speed(R, bg, gamma, mu0, &cfR);
+	if (e) return e;
+	const double
+		max_fast = smax(cfL, cfR),
+		s = smax(fabs(L[M0] / L[RHO]) + max_fast, fabs(R[M0] / R[RHO]) + max_fast);
+	if (!isfinite(max_fast) || !isfinite(s)) return OR_ERR_SPEED;
+
+	double FL[8], FR[8];
+	e = or_physical_flux(L, bg, gamma, mu0, FL);
+	if (e) return e;
+	e = or_physical_flux(R, bg, gamma, mu0, FR);
+	if (e) return e;
+	if (!isfinite(FL[RHO]) || !isfinite(FR[RHO])) return OR_ERR_FLUX;
+
+	F[RHO] = (FL[RHO] + FR[RHO]) / 2 - (R[RHO] - L[RHO]) * (s / 2);
+	F[NRJ] = (FL[NRJ] + FR[NRJ]) / 2 - (R[NRJ] - L[NRJ]) * (s / 2);
+	const double hs = -s / 2;
+	for (int i = 0; i < 3; i++) {
+		F[M0 + i] = (FL[M0 + i] + FR[M0 + i]) * 0.5 + (R[M0 + i] - L[M0 + i]) * hs;
+		F[B0 + i] = (FL[B0 + i] + FR[B0 + i]) * 0.5 + (R[B0 + i] - L[B0 + i]) * hs;
+	}
+	if (!isfinite(F[RHO])) return OR_ERR_FLUX;
+	*s_out = s;
+	return OR_OK;
+}
+
+/* source/mhd/hll_athena.hpp:57-275 */
+int or_flux_hll(const double L[8], const double R[8], const double bg[3],
+	double gamma, double mu0, double F[8], double* s_out)
+{
+	double vL[3], vR[3];
+	velocity(L, vL);
+	velocity(R, vR);
+	double pL, pR;
+	int e = or_get_pressure(L, gamma, mu0, &pL);
+	if (e) return e;
+	e = or_get_pressure(R, gamma, mu0, &pR);
+	if (e) return e;
+	const double
+		pmL = dot3(L + B0, L + B0) / (2 * mu0),
+		pmR = dot3(R + B0, R + B0) / (2 * mu0);
+	double cfL, cfR;
+	e = or_fast_speed(L, bg, gamma, mu0, &cfL);
+	if (e) return e;
+	e = or_fast_speed(R, bg, gamma, mu0, &cfR);
+	if (e) return e;
+	const double
+		ms = smax(cfL, cfR),
+		msn = (vL[0] <= vR[0]) ? vL[0] - ms : vR[0] - ms,
+		msp = (vL[0] <= vR[0]) ? vR[0] + ms : vL[0] + ms,
+		bm = smin(msn, 0.0),
+		bp = smax(msp, 0.0);
+
+	if (!isnormal(pL) || pL < 0) return OR_ERR_PRESSURE;
+	if (!isfinite(pmL) || pmL < 0) return OR_ERR_PRESSURE;
+	if (!isfinite(msn)) return OR_ERR_SPEED;
+	if (!isnormal(pR) || pR < 0) return OR_ERR_PRESSURE;
+	if (!isfinite(pmR) || pmR < 0) return OR_ERR_PRESSURE;
+	if (!isfinite(msp)) return OR_ERR_SPEED;
+
+	if (!isnormal(bp - bm) || bp - bm < 0) {
+		for (int i = 0; i < 8; i++) F[i] = 0;
+		*s_out = 0;
+		return OR_OK;
+	}
+
+	double FL[8], FR[8];
+	e = or_physical_flux(L, bg, gamma, mu0, FL);
+	if (e) return e;
+	e = or_physical_flux(R, bg, gamma, mu0, FR);
+	if (e) return e;
+
+	FL[RHO] -= L[RHO] * bm;
+	FL[NRJ] -= L[NRJ] * bm;
+	FR[RHO] -= R[RHO] * bp;
+	FR[NRJ] -= R[NRJ] * bp;
+	for (int i = 0; i < 3; i++) {
+		FL[M0 + i] = FL[M0 + i] + L[M0 + i] * (-bm);
+		FL[B0 + i] = FL[B0 + i] + L[B0 + i] * (-bm);
+		FR[M0 + i] = FR[M0 + i] + R[M0 + i] * (-bp);
+		FR[B0 + i] = FR[B0 + i] + R[B0 + i] * (-bp);
+	}
+	FR[B0] = FL[B0] = 0;
+
+	F[RHO] = (FL[RHO] + FR[RHO]) / 2 + (FL[RHO] - FR[RHO]) * (bp + bm) / (bp - bm) / 2.0;
+	F[NRJ] = (FL[NRJ] + FR[NRJ]) / 2 + (FL[NRJ] - FR[NRJ]) * (bp + bm) / (bp - bm) / 2.0;
+	const double w = (bp + bm) / (bp - bm) / 2.0;
+	for (int i = 0; i < 3; i++) {
+		F[M0 + i] = (FL[M0 + i] + FR[M0 + i]) * 0.5 + (FL[M0 + i] - FR[M0 + i]) * w;
+		F[B0 + i] = (FL[B0 + i] + FR[B0 + i]) * 0.5 + (FL[B0 + i] - FR[B0 + i]) * w;
+	}
+	*s_out = smax(fabs(bp), fabs(bm));
+	return OR_OK;
+}
+
+/* One side's star state of source/mhd/hlld_athena.hpp:316-401 (eqns 43-48). */
+static void hlld_star(const double U[8], const double v[3], double p, double nm,
+	double sfd, double smd, double sm, double ds, double Bx, double Bx2,
+	double inv, double ptst, double S[8], double* vdbs)
+{
+	const double eps = DBL_EPSILON;
+	const double tmp = U[RHO] * sfd * smd - Bx2 * inv;
+	if (fabs(tmp) < eps * ptst) {
+		for (int i = 0; i < 3; i++) {
+			S[M0 + i] = v[i] * ds;
+			S[B0 + i] = U[B0 + i];
+		}
+	} else {
+		const double tmp1 = Bx * (sm - v[0]) * inv / tmp;
+		for (int i = 0; i < 3; i++) {
+			S[M0 + i] = (v[i] + U[B0 + i] * (-tmp1)) * ds;
+		}
+		const double tmp2 = (U[RHO] * (sfd * sfd) - Bx2 * inv) / tmp;
+		for (int i = 0; i < 3; i++) {
+			S[B0 + i] = U[B0 + i] * tmp2;
+		}
+	}
+	S[RHO] = ds;
+	S[M0] = ds * sm;
+	S[B0] = Bx;
+	*vdbs = dot3(S + M0, S + B0) / ds;
+	S[NRJ]
+		= (
+			sfd * U[NRJ]
+			- (p + nm) * v[0]
+			+ ptst * sm
+			+ Bx * inv * (dot3(v, U + B0) - *vdbs)
+		) / smd;
+}
+
+/* source/mhd/hlld_athena.hpp:54-559 (ignores the background field, :77) */
+int or_flux_hlld(const double L[8], const double R[8], const double bg_unused[3],
+	double gamma, double mu0, double F[8], double* s_out)
+{
+	(void)bg_unused;
+	const double zero[3] = {0, 0, 0};
+	const double eps = DBL_EPSILON;
+
+	if (L[RHO] <= 0) return OR_ERR_DENSITY;
+	if (R[RHO] <= 0) return OR_ERR_DENSITY;
+	double pL, pR;
+	int e = or_get_pressure(L, gamma, mu0, &pL);
+	if (e) return e;
+	if (pL <= 0) return OR_ERR_PRESSURE;
+	e = or_get_pressure(R, gamma, mu0, &pR);
+	if (e) return e;
+	if (pR <= 0) return OR_ERR_PRESSURE;
+
+	const double
+		inv = 1.0 / mu0,
+		inv_sqrt = 1.0 / sqrt(mu0),
+		Bx = (L[B0] + R[B0]) / 2,
+		Bx2 = Bx * Bx;
+	double cfL, cfR;
+	e = or_fast_speed(L, zero, gamma, mu0, &cfL);
+	if (e) return e;
+	e = or_fast_speed(R, zero, gamma, mu0, &cfR);
+	if (e) return e;
+	const double ms = smax(cfL, cfR);
+
+	double vL[3], vR[3];
+	{
+		const double iL = 1 / L[RHO], iR = 1 / R[RHO];
+		for (int i = 0; i < 3; i++) {
+			vL[i] = L[M0 + i] * iL;
+			vR[i] = R[M0 + i] * iR;
+		}
+	}
+	const double
+		msn = (vL[0] <= vR[0]) ? vL[0] - ms : vR[0] - ms,
+		msp = (vL[0] <= vR[0]) ? vR[0] + ms : vL[0] + ms;
+
+	double FL[8], FR[8];
+	e = or_physical_flux(L, zero, gamma, mu0, FL);
+	if (e) return e;
+	e = or_physical_flux(R, zero, gamma, mu0, FR);
+	if (e) return e;
+
+	if (msn >= 0.0) {
+		memcpy(F, FL, sizeof(FL));
+		*s_out = fabs(msp);
+		return OR_OK;
+	} else if (msp <= 0.0) {
+		memcpy(F, FR, sizeof(FR));
+		*s_out = fabs(msn);
+		return OR_OK;
+	}
+
+	const double
+		sfdn = msn - vL[0],
+		sfdp = msp - vR[0],
+		nmn = dot3(L + B0, L + B0) * (inv / 2),
+		nmp = dot3(R + B0, R + B0) * (inv / 2),
+		/* eqn 38 */
+		sm
+			= (
+				sfdp * R[M0]
+				- sfdn * L[M0]
+				- (pR + nmp)
+				+ (pL + nmn)
+			) / (
+				sfdp * R[RHO]
+				- sfdn * L[RHO]
+			),
+		smdn = msn - sm,
+		smdp = msp - sm,
+		/* eqn 43 */
+		dsn = L[RHO] * sfdn / smdn,
+		dssn = sqrt(dsn),
+		dsp = R[RHO] * sfdp / smdp,
+		dssp = sqrt(dsp),
+		/* eqn 51 */
+		ssn = sm - fabs(Bx) / sqrt(dsn * mu0),
+		ssp = sm + fabs(Bx) / sqrt(dsp * mu0),
+		ptst = pL + nmn + L[RHO] * sfdn * (sm - vL[0]);
+
+	double Sn[8], Sp[8], vdbsn, vdbsp;
+	hlld_star(L, vL, pL, nmn, sfdn, smdn, sm, dsn, Bx, Bx2, inv, ptst, Sn, &vdbsn);
+	hlld_star(R, vR, pR, nmp, sfdp, smdp, sm, dsp, Bx, Bx2, inv, ptst, Sp, &vdbsp);
+
+	double vsn[3], vsp[3];
+	{
+		const double in = 1 / dsn, ip = 1 / dsp;
+		for (int i = 0; i < 3; i++) {
+			vsn[i] = Sn[M0 + i] * in;
+			vsp[i] = Sp[M0 + i] * ip;
+		}
+	}
+
+	double S2n[8], S2p[8];
+	if (Bx2 / 2 < eps * ptst) {
+		memcpy(S2n, Sn, sizeof(Sn));
+		memcpy(S2p, Sp, sizeof(Sp));
+	} else {
+		const double isds = 1.0 / (dssn + dssp);
+		const double sgn = (Bx > 0) ? 1.0 : -1.0;
+		S2n[RHO] = Sn[RHO];
+		S2p[RHO] = Sp[RHO];
+		double v2[3];
+		const double c1 = sgn * inv_sqrt;
+		for (int i = 0; i < 3; i++) {
+			v2[i] = ((vsn[i] * dssn + vsp[i] * dssp) + (Sp[B0 + i] - Sn[B0 + i]) * c1) * isds;
+		}
+		v2[0] = sm;
+		/* eqs 59 & 60 */
+		for (int i = 0; i < 3; i++) {
+			S2n[M0 + i] = v2[i] * Sn[RHO];
+			S2p[M0 + i] = v2[i] * Sp[RHO];
+		}
+		S2n[M0] = Sn[M0];
+		S2p[M0] = Sp[M0];
+		/* eqs 61 & 62 */
+		const double c2 = sgn * dssn * dssp * sqrt(mu0);
+		for (int i = 0; i < 3; i++) {
+			const double b = ((Sp[B0 + i] * dssn + Sn[B0 + i] * dssp) + (vsp[i] - vsn[i]) * c2) * isds;
+			S2n[B0 + i] = b;
+			S2p[B0 + i] = b;
+		}
+		S2n[B0] = S2p[B0] = Bx;
+		/* eqn 63 */
+		const double vdb2 = dot3(v2, S2n + B0);
+		S2n[NRJ] = Sn[NRJ] - dssn * sgn * (vdbsn - vdb2) * inv_sqrt;
+		S2p[NRJ] = Sp[NRJ] + dssp * sgn * (vdbsp - vdb2) * inv_sqrt;
+	}
+
+	if (ssn >= 0) {
+		F[RHO] = FL[RHO] + (Sn[RHO] - L[RHO]) * msn;
+		F[NRJ] = FL[NRJ] + (Sn[NRJ] - L[NRJ]) * msn;
+		for (int i = 0; i < 3; i++) {
+			F[M0 + i] = FL[M0 + i] + (Sn[M0 + i] - L[M0 + i]) * msn;
+			F[B0 + i] = FL[B0 + i] + (Sn[B0 + i] - L[B0 + i]) * msn;
+		}
+	} else if (sm >= 0) {
+		const double d = ssn - msn;
+		for (int q = 0; q < 8; q++) {
+			F[q] = FL[q] + S2n[q] * ssn - Sn[q] * d - L[q] * msn;
+		}
+	} else if (ssp > 0) {
+		const double d = ssp - msp;
+		for (int q = 0; q < 8; q++) {
+			F[q] = FR[q] + S2p[q] * ssp - Sp[q] * d - R[q] * msp;
+		}
+	} else {
+		F[RHO] = FR[RHO] + (Sp[RHO] - R[RHO]) * msp;
+		F[NRJ] = FR[NRJ] + (Sp[NRJ] - R[NRJ]) * msp;
+		for (int i = 0; i < 3; i++) {
+			F[M0 + i] = FR[M0 + i] + (Sp[M0 + i] - R[M0 + i]) * msp;
+			F[B0 + i] = FR[B0 + i] + (Sp[B0 + i] - R[B0 + i]) * msp;
+		}
+	}
+	F[B0] = 0;
+	*s_out = smax(fabs(msn), fabs(msp));
+	return OR_OK;
+}
+
+/* Roe eigensystem, source/mhd/roe_athena.hpp:56-296 (eqs. B17-B29 of the
+   Athena method paper); rem holds right eigenvectors as columns, lem left
+   eigenvectors as rows. */
+void or_roe_eigensystem(double d, double v1, double v2, double v3, double h,
+	double b1, double b2, double b3, double x, double y, double gamma,
+	double ev[7], double rem[7][7], double lem[7][7])
+{
+	const double tiny = DBL_EPSILON;
+	const double g1 = gamma - 1, g2 = gamma - 2;
+
+	const double di = 1.0 / d;
+	const double vsq = v1 * v1 + v2 * v2 + v3 * v3;
+	const double btsq = b2 * b2 + b3 * b3;
+	const double bt_starsq = (g1 - g2 * y) * btsq;
+	const double vaxsq = b1 * b1 * di;
+	const double hp = h - (vaxsq + btsq * di);
+	const double twid_asq = smax((g1 * (hp - 0.5 * vsq) - g2 * x), tiny);
+
+	/* fast and slow magnetosonic speeds, eq. B18 */
+	const double ct2 = bt_starsq * di;
+	const double tsum = vaxsq + ct2 + twid_asq;
+	const double tdif = vaxsq + ct2 - twid_asq;
+	const double cf2_cs2 = sqrt(tdif * tdif + 4.0 * twid_asq * ct2);
+	const double cfsq = 0.5 * (tsum + cf2_cs2);
+	const double cf = sqrt(cfsq);
+	const double cssq = twid_asq * vaxsq / cfsq;
+	const double cs = sqrt(cssq);
+
+	/* beta(s), eqs. A17, B20, B28 */
+	const double bt = sqrt(btsq);
+	const double bt_star = sqrt(bt_starsq);
+	double bet2, bet3;
+	if (bt == 0.0) {
+		bet2 = 1.0;
+		bet3 = 0.0;
+	} else {
+		bet2 = b2 / bt;
+		bet3 = b3 / bt;
+	}
+	const double bet2_star = bet2 / sqrt(g1 - g2 * y);
+	const double bet3_star = bet3 / sqrt(g1 - g2 * y);
+	const double bet_starsq = bet2_star * bet2_star + bet3_star * bet3_star;
+	const double vbet = v2 * bet2_star + v3 * bet3_star;
+
+	/* alpha(s), eq. A16 */
+	double alpha_f, alpha_s;
+	if ((cfsq - cssq) == 0.0) {
+		alpha_f = 1.0;
+		alpha_s = 0.0;
+	} else if ((twid_asq - cssq) <= 0.0) {
+		alpha_f = 0.0;
+		alpha_s = 1.0;
+	} else if ((cfsq - twid_asq) <= 0.0) {
+		alpha_f = 1.0;
+		alpha_s = 0.0;
+	} else {
+		alpha_f = sqrt((twid_asq - cssq) / (cfsq - cssq));
+		alpha_s = sqrt((cfsq - twid_asq) / (cfsq - cssq));
+	}
+
+	/* Q(s) and A(s), eqs. A14-15 */
+	const double sqrtd = sqrt(d);
+	const double isqrtd = 1.0 / sqrtd;
+	const double s = (b1 > 0) ? 1 : -1;
+	const double twid_a = sqrt(twid_asq);
+	double qf = cf * alpha_f * s;
+	double qs = cs * alpha_s * s;
+	const double af_prime = twid_a * alpha_f * isqrtd;
+	const double as_prime = twid_a * alpha_s * isqrtd;
+	const double afpbb = af_prime * bt_star * bet_starsq;
+	const double aspbb = as_prime * bt_star * bet_starsq;
+
+	/* eigenvalues, eq. B17 */
+	const double vax = sqrt(vaxsq);
+	ev[0] = v1 - cf;
+	ev[1] = v1 - vax;
+	ev[2] = v1 - cs;
+	ev[3] = v1;
+	ev[4] = v1 + cs;
+	ev[5] = v1 + vax;
+	ev[6] = v1 + cf;
+
+	/* right eigenvectors as columns, eq. B21 */
+	rem[0][0] = alpha_f;
+	rem[0][1] = 0.0;
+	rem[0][2] = alpha_s;
+	rem[0][3] = 1.0;
+	rem[0][4] = alpha_s;
+	rem[0][5] = 0.0;
+	rem[0][6] = alpha_f;
+
+	rem[1][0] = alpha_f * ev[0];
+	rem[1][1] = 0.0;
+	rem[1][2] = alpha_s * ev[2];
+	rem[1][3] = v1;
+	rem[1][4] = alpha_s * ev[4];
+	rem[1][5] = 0.0;
+	rem[1][6] = alpha_f * ev[6];
+
+	double qa = alpha_f * v2, qb = alpha_s * v2, qc = qs * bet2_star, qd = qf * bet2_star;
+	rem[2][0] = qa + qc;
+	rem[2][1] = -bet3;
+	rem[2][2] = qb - qd;
+	rem[2][3] = v2;
+	rem[2][4] = qb + qd;
+	rem[2][5] = bet3;
+	rem[2][6] = qa - qc;
+
+	qa = alpha_f * v3;
+	qb = alpha_s * v3;
+	qc = qs * bet3_star;
+	qd = qf * bet3_star;
+	rem[3][0] = qa + qc;
+	rem[3][1] = bet2;
+	rem[3][2] = qb - qd;
+	rem[3][3] = v3;
+	rem[3][4] = qb + qd;
+	rem[3][5] = -bet2;
+	rem[3][6] = qa - qc;
+
+	rem[4][0] = alpha_f * (hp - v1 * cf) + qs * vbet + aspbb;
+	rem[4][1] = -(v2 * bet3 - v3 * bet2);
+	rem[4][2] = alpha_s * (hp - v1 * cs) - qf * vbet - afpbb;
+	rem[4][3] = 0.5 * vsq + g2 * x / g1;
+	rem[4][4] = alpha_s * (hp + v1 * cs) + qf * vbet - afpbb;
+	rem[4][5] = -rem[4][1];
+	rem[4][6] = alpha_f * (hp + v1 * cf) - qs * vbet + aspbb;
+
+	rem[5][0] = as_prime * bet2_star;
+	rem[5][1] = -bet3 * s * isqrtd;
+	rem[5][2] = -af_prime * bet2_star;
+	rem[5][3] = 0.0;
+	rem[5][4] = rem[5][2];
+	rem[5][5] = rem[5][1];
+	rem[5][6] = rem[5][0];
+
+	rem[6][0] = as_prime * bet3_star;
+	rem[6][1] = bet2 * s * isqrtd;
+	rem[6][2] = -af_prime * bet3_star;
+	rem[6][3] = 0.0;
+	rem[6][4] = rem[6][2];
+	rem[6][5] = rem[6][1];
+	rem[6][6] = rem[6][0];
+
+	/* left eigenvectors as rows, eq. B29; normalised by 1/2a^2 */
+	double norm = 0.5 / twid_asq;
+	const double cff = norm * alpha_f * cf;
+	const double css = norm * alpha_s * cs;
+	qf *= norm;
+	qs *= norm;
+	const double af = norm * af_prime * d;
+	const double as = norm * as_prime * d;
+	const double afpb = norm * af_prime * bt_star;
+	const double aspb = norm * as_prime * bt_star;
+
+	/* normalised by (gamma-1)/2a^2 */
+	norm *= g1;
+	alpha_f *= norm;
+	alpha_s *= norm;
+	const double q2_star = bet2_star / bet_starsq;
+	const double q3_star = bet3_star / bet_starsq;
+	const double vqstr = (v2 * q2_star + v3 * q3_star);
+	norm *= 2.0;
+
+	lem[0][0] = alpha_f * (vsq - hp) + cff * (cf + v1) - qs * vqstr - aspb;
+	lem[0][1] = -alpha_f * v1 - cff;
+	lem[0][2] = -alpha_f * v2 + qs * q2_star;
+	lem[0][3] = -alpha_f * v3 + qs * q3_star;
+	lem[0][4] = alpha_f;
+	lem[0][5] = as * q2_star - alpha_f * b2;
+	lem[0][6] = as * q3_star - alpha_f * b3;
+
+	lem[1][0] = 0.5 * (v2 * bet3 - v3 * bet2);
+	lem[1][1] = 0.0;
+	lem[1][2] = -0.5 * bet3;
+	lem[1][3] = 0.5 * bet2;
+	lem[1][4] = 0.0;
+	lem[1][5] = -0.5 * sqrtd * bet3 * s;
+	lem[1][6] = 0.5 * sqrtd * bet2 * s;
+
+	lem[2][0] = alpha_s * (vsq - hp) + css * (cs + v1) + qf * vqstr + afpb;
+	lem[2][1] = -alpha_s * v1 - css;
+	lem[2][2] = -alpha_s * v2 - qf * q2_star;
+	lem[2][3] = -alpha_s * v3 - qf * q3_star;
+	lem[2][4] = alpha_s;
+	lem[2][5] = -af * q2_star - alpha_s * b2;
+	lem[2][6] = -af * q3_star - alpha_s * b3;
+
+	lem[3][0] = 1.0 - norm * (0.5 * vsq - g2 * x / g1);
+	lem[3][1] = norm * v1;
+	lem[3][2] = norm * v2;
+	lem[3][3] = norm * v3;
+	lem[3][4] = -norm;
+	lem[3][5] = norm * b2;
+	lem[3][6] = norm * b3;
+
+	lem[4][0] = alpha_s * (vsq - hp) + css * (cs - v1) - qf * vqstr + afpb;
+	lem[4][1] = -alpha_s * v1 + css;
+	lem[4][2] = -alpha_s * v2 + qf * q2_star;
+	lem[4][3] = -alpha_s * v3 + qf * q3_star;
+	lem[4][4] = alpha_s;
+	lem[4][5] = lem[2][5];
+	lem[4][6] = lem[2][6];
+
+	lem[5][0] = -lem[1][0];
+	lem[5][1] = 0.0;
+	lem[5][2] = -lem[1][2];
+	lem[5][3] = -lem[1][3];
+	lem[5][4] = 0.0;
+	lem[5][5] = lem[1][5];
+	lem[5][6] = lem[1][6];
+
+	lem[6][0] = alpha_f * (vsq - hp) + cff * (cf - v1) + qs * vqstr - aspb;
+	lem[6][1] = -alpha_f * v1 + cff;
+	lem[6][2] = -alpha_f * v2 - qs * q2_star;
+	lem[6][3] = -alpha_f * v3 - qs * q3_star;
+	lem[6][4] = alpha_f;
+	lem[6][5] = lem[0][5];
+	lem[6][6] = lem[0][6];
+}
+
+/* 1-D Roe flux in scaled units, source/mhd/roe_athena.hpp:341-559.
+   Ul/Ur = {d, Mx, My, Mz, E, By, Bz}, Wl/Wr = {d, Vx, Vy, Vz, P, By, Bz}. */
+static int roe_fluxes(const double Ul[7], const double Ur[7], const double Wl[7],
+	const double Wr[7], double Bxi, double gamma, double flux[7])
+{
+	enum { D = 0, X = 1, Y = 2, Z = 3, E = 4, BY = 5, BZ = 6 };
+	double ev[7], rem[7][7], lem[7][7], a[7], dU[7], u[7];
+
+	const double sqrtdl = sqrt(Wl[D]);
+	const double sqrtdr = sqrt(Wr[D]);
+	const double isdlpdr = 1.0 / (sqrtdl + sqrtdr);
+	const double droe = sqrtdl * sqrtdr;
+	const double v1roe = (sqrtdl * Wl[X] + sqrtdr * Wr[X]) * isdlpdr;
+	const double v2roe = (sqrtdl * Wl[Y] + sqrtdr * Wr[Y]) * isdlpdr;
+	const double v3roe = (sqrtdl * Wl[Z] + sqrtdr * Wr[Z]) * isdlpdr;
+	/* the Roe average of the magnetic field is defined differently */
+	const double b2roe = (sqrtdr * Wl[BY] + sqrtdl * Wr[BY]) * isdlpdr;
+	const double b3roe = (sqrtdr * Wl[BZ] + sqrtdl * Wr[BZ]) * isdlpdr;
+	const double dby = Wl[BY] - Wr[BY], dbz = Wl[BZ] - Wr[BZ], sd = sqrtdl + sqrtdr;
+	const double x = 0.5 * (dby * dby + dbz * dbz) / (sd * sd);
+	const double y = 0.5 * (Wl[D] + Wr[D]) / droe;
+	const double pbl = 0.5 * (Bxi * Bxi + Wl[BY] * Wl[BY] + Wl[BZ] * Wl[BZ]);
+	const double pbr = 0.5 * (Bxi * Bxi + Wr[BY] * Wr[BY] + Wr[BZ] * Wr[BZ]);
+	/* enthalpy average, Roe (1981) */
+	const double hroe = ((Ul[E] + Wl[E] + pbl) / sqrtdl + (Ur[E] + Wr[E] + pbr) / sqrtdr) * isdlpdr;
+
+	or_roe_eigensystem(droe, v1roe, v2roe, v3roe, hroe, Bxi, b2roe, b3roe, x, y,
+		gamma, ev, rem, lem);
+
+	double Fl[7], Fr[7];
+	Fl[D] = Ul[X];
+	Fr[D] = Ur[X];
+	Fl[X] = Ul[X] * Wl[X];
+	Fr[X] = Ur[X] * Wr[X];
+	Fl[Y] = Ul[X] * Wl[Y];
+	Fr[Y] = Ur[X] * Wr[Y];
+	Fl[Z] = Ul[X] * Wl[Z];
+	Fr[Z] = Ur[X] * Wr[Z];
+	Fl[X] += Wl[E];
+	Fr[X] += Wr[E];
+	Fl[E] = (Ul[E] + Wl[E]) * Wl[X];
+	Fr[E] = (Ur[E] + Wr[E]) * Wr[X];
+	Fl[X] -= 0.5 * (Bxi * Bxi - Wl[BY] * Wl[BY] - Wl[BZ] * Wl[BZ]);
+	Fr[X] -= 0.5 * (Bxi * Bxi - Wr[BY] * Wr[BY] - Wr[BZ] * Wr[BZ]);
+	Fl[Y] -= Bxi * Wl[BY];
+	Fr[Y] -= Bxi * Wr[BY];
+	Fl[Z] -= Bxi * Wl[BZ];
+	Fr[Z] -= Bxi * Wr[BZ];
+	Fl[E] += pbl * Wl[X] - Bxi * (Bxi * Wl[X] + Wl[BY] * Wl[Y] + Wl[BZ] * Wl[Z]);
+	Fr[E] += pbr * Wr[X] - Bxi * (Bxi * Wr[X] + Wr[BY] * Wr[Y] + Wr[BZ] * Wr[Z]);
+	Fl[BY] = Wl[BY] * Wl[X] - Bxi * Wl[Y];
+	Fr[BY] = Wr[BY] * Wr[X] - Bxi * Wr[Y];
+	Fl[BZ] = Wl[BZ] * Wl[X] - Bxi * Wl[Z];
+	Fr[BZ] = Wr[BZ] * Wr[X] - Bxi * Wr[Z];
+
+	/* upwind flux if the flow is supersonic */
+	if (ev[0] >= 0.0) {
+		memcpy(flux, Fl, sizeof(Fl));
+		return OR_OK;
+	}
+	if (ev[6] <= 0.0) {
+		memcpy(flux, Fr, sizeof(Fr));
+		return OR_OK;
+	}
+
+	/* projection of dU onto the left eigenvectors */
+	for (int m = 0; m < 7; m++) dU[m] = Ur[m] - Ul[m];
+	for (int n = 0; n < 7; n++) {
+		a[n] = 0.0;
+		for (int m = 0; m < 7; m++) a[n] += lem[n][m] * dU[m];
+	}
+
+	/* positivity of the intermediate states */
+	for (int m = 0; m < 7; m++) u[m] = Ul[m];
+	for (int n = 0; n < 6; n++) {
+		for (int m = 0; m < 7; m++) u[m] += a[n] * rem[m][n];
+		if (ev[n + 1] > ev[n]) {
+			if (u[0] <= 0.0) return OR_ERR_ROE_DENSITY;
+			double p = u[4] - 0.5 * (u[1] * u[1] + u[2] * u[2] + u[3] * u[3]) / u[0];
+			p -= 0.5 * (u[5] * u[5] + u[6] * u[6] + Bxi * Bxi);
+			if (p < 0.0) return OR_ERR_ROE_PRESSURE;
+		}
+	}
+
+	double coeff[7];
+	for (int m = 0; m < 7; m++) coeff[m] = 0.5 * a[m] * smax(fabs(ev[m]), 0.0);
+	for (int q = 0; q < 7; q++) flux[q] = 0.5 * (Fl[q] + Fr[q]);
+	for (int m = 0; m < 7; m++) {
+		for (int q = 0; q < 7; q++) flux[q] -= coeff[m] * rem[q][m];
+	}
+	return OR_OK;
+}
+
+/* source/mhd/roe_athena.hpp:567-818 (ignores the background field, :584) */
+int or_flux_roe(const double L[8], const double R[8], const double bg_unused[3],
+	double gamma, double mu0, double F[8], double* s_out)
+{
+	(void)bg_unused;
+	const double zero[3] = {0, 0, 0};
+	if (L[RHO] <= 0) return OR_ERR_DENSITY;
+	if (R[RHO] <= 0) return OR_ERR_DENSITY;
+	double pL, pR;
+	int e = or_get_pressure(L, gamma, mu0, &pL);
+	if (e) return e;
+	if (pL <= 0) return OR_ERR_PRESSURE;
+	e = or_get_pressure(R, gamma, mu0, &pR);
+	if (e) return e;
+	if (pR <= 0) return OR_ERR_PRESSURE;
+
+	const double Bxi = (L[B0] + R[B0]) / (2.0 * sqrt(mu0));
+	const double Ul[7] = {L[RHO], L[M0], L[M1], L[M2], L[NRJ],
+		L[B1] / sqrt(mu0), L[B2] / sqrt(mu0)};
+	const double Ur[7] = {R[RHO], R[M0], R[M1], R[M2], R[NRJ],
+		R[B1] / sqrt(mu0), R[B2] / sqrt(mu0)};
+
+	/* get_primitive with vacuum permeability 1 on the unscaled field
+	   (roe_athena.hpp:705-721) */
+	double pl1, pr1, vL[3], vR[3];
+	e = or_get_pressure(L, gamma, 1.0, &pl1);
+	if (e) return e;
+	e = or_get_pressure(R, gamma, 1.0, &pr1);
+	if (e) return e;
+	velocity(L, vL);
+	velocity(R, vR);
+	const double Wl[7] = {L[RHO], vL[0], vL[1], vL[2], pl1, L[B1] / sqrt(mu0), L[B2] / sqrt(mu0)};
+	const double Wr[7] = {R[RHO], vR[0], vR[1], vR[2], pr1, R[B1] / sqrt(mu0), R[B2] / sqrt(mu0)};
+
+	double f7[7];
+	e = roe_fluxes(Ul, Ur, Wl, Wr, Bxi, gamma, f7);
+	if (e) return e;
+
+	F[RHO] = f7[0];
+	F[M0] = f7[1];
+	F[M1] = f7[2];
+	F[M2] = f7[3];
+	F[NRJ] = f7[4];
+	F[B0] = 0;
+	F[B1] = f7[5] * sqrt(mu0);
+	F[B2] = f7[6] * sqrt(mu0);
+
+	double cfL, cfR;
+	e = or_fast_speed(L, zero, gamma, mu0, &cfL);
+	if (e) return e;
+	e = or_fast_speed(R, zero, gamma, mu0, &cfR);
+	if (e) return e;
+	const double ms = smax(cfL, cfR);
+	const double iL = 1 / L[RHO], iR = 1 / R[RHO];
+	const double vL0 = L[M0] * iL, vR0 = R[M0] * iR;
+	const double
+		msn = (vL0 <= vR0) ? vL0 - ms : vR0 - ms,
+		msp = (vL0 <= vR0) ? vR0 + ms : vL0 + ms;
+	if (msn >= 0.0) {
+		*s_out = fabs(msp);
+	} else if (msp <= 0.0) {
+		*s_out = fabs(msn);
+	} else {
+		*s_out = smax(fabs(msn), fabs(msp));
+	}
+	return OR_OK;
+}
+
+int or_flux(int solver, const double L[8], const double R[8], const double bg[3],
+	double gamma, double mu0, double F[8], double* s)
+{
+	switch (solver) {
+	case OR_RUSANOV: return or_flux_rusanov(L, R, bg, gamma, mu0, F, s);
+	case OR_HLL_ATHENA: return or_flux_hll(L, R, bg, gamma, mu0, F, s);
+	case OR_HLLD_ATHENA: return or_flux_hlld(L, R, bg, gamma, mu0, F, s);
+	case OR_ROE_ATHENA: return or_flux_roe(L, R, bg, gamma, mu0, F, s);
+	default: return OR_ERR_SOLVER;
+	}
+}
+
+size_t or_face_flux_batch(int solver, size_t n, const double* L, const double* R,
+	const double* bg, double gamma, double mu0, double* F, double* smax_out,
+	int* status)
+{
+	size_t bad = 0;
+	for (size_t f = 0; f < n; f++) {
+		double l[8], r[8], b[3] = {0, 0, 0}, out[8] = {0, 0, 0, 0, 0, 0, 0, 0}, s = 0;
+		for (int q = 0; q < 8; q++) {
+			l[q] = L[q * n + f];
+			r[q] = R[q * n + f];
+		}
+		if (bg) for (int q = 0; q < 3; q++) b[q] = bg[q * n + f];
+		const int e = or_flux(solver, l, r, b, gamma, mu0, out, &s);
+		if (e) {
+			bad++;
+			for (int q = 0; q < 8; q++) out[q] = 0;
+			s = 0;
+		}
+		for (int q = 0; q < 8; q++) F[q * n + f] = out[q];
+		smax_out[f] = s;
+		if (status) status[f] = e;
+	}
+	return bad;
+}
+
+/* ---------------------------------------------------------------------- */
+/* uniform periodic staggered grid                                        */
+/* ---------------------------------------------------------------------- */
+
+/*
+get_rotated_vector, source/mhd/common.hpp:612-648: a d-face (d = 0, 1, 2 for
+x, y, z) sees the global vector v as
+    d = 0: (v0, v1, v2),  d = 1: (v1, v2, v0),  d = 2: (v2, v0, v1)
+*/
+static inline void rot_to_face(int d, const double* v, double* r)
+{
+	if (d == 0) { r[0] = v[0]; r[1] = v[1]; r[2] = v[2]; }
+	else if (d == 1) { r[0] = v[1]; r[1] = v[2]; r[2] = v[0]; }
+	else { r[0] = v[2]; r[1] = v[0]; r[2] = v[1]; }
+}
+static inline void rot_to_global(int d, const double* r, double* v)
+{
+	if (d == 0) { v[0] = r[0]; v[1] = r[1]; v[2] = r[2]; }
+	else if (d == 1) { v[1] = r[0]; v[2] = r[1]; v[0] = r[2]; }
+	else { v[2] = r[0]; v[0] = r[1]; v[1] = r[2]; }
+}
+
+typedef struct {
+	long nx, ny, nz, n;
+} dims_t;
+
+static inline long wrap(long i, long n) { return ((i % n) + n) % n; }
+
+static inline long cid(const dims_t* D, long i, long j, long k)
+{
+	return wrap(i, D->nx) + D->nx * (wrap(j, D->ny) + D->ny * wrap(k, D->nz));
+}
+
+/* cell-centred B = average of the two faces, source/mhd/solve.hpp:1661-1663 */
+static inline void vol_b(const dims_t* D, const double* s, long i, long j, long k,
+	double B[3])
+{
+	const long n = D->n, c = cid(D, i, j, k);
+	B[0] = 0.5 * (s[5 * n + c] + s[5 * n + cid(D, i + 1, j, k)]);
+	B[1] = 0.5 * (s[6 * n + c] + s[6 * n + cid(D, i, j + 1, k)]);
+	B[2] = 0.5 * (s[7 * n + c] + s[7 * n + cid(D, i, j, k + 1)]);
+}
+
+/* cell state rotated into the frame of a d-face, source/mhd/solve.hpp:106-118 */
+static inline void face_state(const dims_t* D, const double* s, int d, long i,
+	long j, long k, double U[8])
+{
+	const long n = D->n, c = cid(D, i, j, k);
+	const double m[3] = {s[1 * n + c], s[2 * n + c], s[3 * n + c]};
+	double B[3];
+	vol_b(D, s, i, j, k, B);
+	U[RHO] = s[c];
+	U[NRJ] = s[4 * n + c];
+	rot_to_face(d, m, U + M0);
+	rot_to_face(d, B, U + B0);
+}
+
+/*
+Constant-thermal-pressure energy refit of update_vol_B,
+source/mhd/solve.hpp:1646-1672: pressure from (rho, m, E, vb_old), then
+E = p/(gamma-1) + rho v^2/2 + vb_new^2/(2 mu0).  Cells with rho <= 0 are
+skipped as in the reference.
+*/
+static void refit_energy(double rho, const double m[3], double* E,
+	const double vb_old[3], const double vb_new[3], double gamma, double mu0)
+{
+	if (rho <= 0) return;
+	const double U[8] = {rho, m[0], m[1], m[2], *E, vb_old[0], vb_old[1], vb_old[2]};
+	double p;
+	or_get_pressure(U, gamma, mu0, &p);
+	const double inv = 1 / rho;
+	const double v[3] = {m[0] * inv, m[1] * inv, m[2] * inv};
+	*E = or_total_energy(rho, v, p, vb_new, gamma, mu0);
+}
+
+int or_step_periodic(const or_grid* g, int solver, double dt, int refits,
+	const double* in, double* out, double smax_dim[3], long zero_dim[3],
+	long* bad_face)
+{
+	const dims_t D = {g->nx, g->ny, g->nz, (long)g->nx * g->ny * g->nz};
+	const long n = D.n;
+	const double len[3] = {g->dx, g->dy, g->dz};
+	/* P[(d*8 + q)*n + c] = dt * F_q (global frame) of the d-face owned by
+	   cell c, i.e. the face on the cell's negative d side */
+	double* P = (double*)malloc(sizeof(double) * 24 * n);
+	if (!P) return OR_ERR_SOLVER;
+	int err = OR_OK;
+	double sm[3] = {0, 0, 0};
+	long zc[3] = {0, 0, 0};
+
+	/* flux pass of get_fluxes, source/mhd/solve.hpp:304-393 with get_flux :68-188 */
+	for (int d = 0; d < 3 && !err; d++) {
+		for (long k = 0; k < D.nz && !err; k++)
+		for (long j = 0; j < D.ny && !err; j++)
+		for (long i = 0; i < D.nx; i++) {
+			const long c = cid(&D, i, j, k);
+			double L[8], R[8], bgr[3] = {0, 0, 0}, F[8], s = 0;
+			face_state(&D, in, d, i - (d == 0), j - (d == 1), k - (d == 2), L);
+			face_state(&D, in, d, i, j, k, R);
+			if (g->bg) {
+				const double b[3] = {
+					g->bg[(3 * d + 0) * n + c],
+					g->bg[(3 * d + 1) * n + c],
+					g->bg[(3 * d + 2) * n + c]};
+				rot_to_face(d, b, bgr);
+			}
+			const int e = or_flux(solver, L, R, bgr, g->gamma, g->mu0, F, &s);
+			if (e) {
+				err = e;
+				if (bad_face) *bad_face = (long)d * n + c;
+				break;
+			}
+			double Fg[8];
+			Fg[RHO] = F[RHO];
+			Fg[NRJ] = F[NRJ];
+			rot_to_global(d, F + M0, Fg + M0);
+			rot_to_global(d, F + B0, Fg + B0);
+			for (int q = 0; q < 8; q++) P[(d * 8 + q) * n + c] = dt * Fg[q];
+			if (s <= 0) zc[d]++;
+			sm[d] = smax(sm[d], s);
+		}
+	}
+	if (smax_dim) for (int d = 0; d < 3; d++) smax_dim[d] = sm[d];
+	if (zero_dim) for (int d = 0; d < 3; d++) zero_dim[d] = zc[d];
+	if (err) {
+		free(P);
+		return err;
+	}
+
+	const double dx = g->dx, dy = g->dy, dz = g->dz;
+#define PF(d, q, i, j, k) P[((d) * 8 + (q)) * n + cid(&D, (i), (j), (k))]
+	/* edge terms of assign_face_dBs_f{x,y,z}, source/mhd/solve.hpp:610-613,
+	   :926-928, :1223-1225, written from dt*F of the face owned by (i,j,k):
+	     x-face: Ez = +F_By dt dz/4, Ey = -F_Bz dt dy/4
+	     y-face: Ex = +F_Bz dt dx/4, Ez = -F_Bx dt dz/4
+	     z-face: Ex = -F_By dt dx/4, Ey = +F_Bx dt dy/4 */
+#define EZX(i, j, k) (+(PF(0, 6, i, j, k) * dz / 4))
+#define EYX(i, j, k) (-(PF(0, 7, i, j, k) * dy / 4))
+#define EXY(i, j, k) (+(PF(1, 7, i, j, k) * dx / 4))
+#define EZY(i, j, k) (-(PF(1, 5, i, j, k) * dz / 4))
+#define EXZ(i, j, k) (-(PF(2, 6, i, j, k) * dx / 4))
+#define EYZ(i, j, k) (+(PF(2, 5, i, j, k) * dy / 4))
+
+	for (long k = 0; k < D.nz; k++)
+	for (long j = 0; j < D.ny; j++)
+	for (long i = 0; i < D.nx; i++) {
+		const long c = cid(&D, i, j, k);
+
+		/* face B += Face_dB / area, source/mhd/solve.hpp:1589-1593; each
+		   face sums the 12 edge terms of the faces that share its edges */
+		double dBx = 0, dBy = 0, dBz = 0;
+		dBx += EZX(i, j + 1, k);
+		dBx -= EZX(i, j - 1, k);
+		dBx -= EYX(i, j, k + 1);
+		dBx += EYX(i, j, k - 1);
+		dBx += EZY(i, j + 1, k);
+		dBx += EZY(i - 1, j + 1, k);
+		dBx -= EZY(i, j, k);
+		dBx -= EZY(i - 1, j, k);
+		dBx -= EYZ(i, j, k + 1);
+		dBx -= EYZ(i - 1, j, k + 1);
+		dBx += EYZ(i, j, k);
+		dBx += EYZ(i - 1, j, k);
+
+		dBy += EXY(i, j, k + 1);
+		dBy -= EXY(i, j, k - 1);
+		dBy -= EZY(i + 1, j, k);
+		dBy += EZY(i - 1, j, k);
+		dBy += EXZ(i, j, k + 1);
+		dBy += EXZ(i, j - 1, k + 1);
+		dBy -= EXZ(i, j, k);
+		dBy -= EXZ(i, j - 1, k);
+		dBy -= EZX(i + 1, j, k);
+		dBy += EZX(i, j, k);
+		dBy -= EZX(i + 1, j - 1, k);
+		dBy += EZX(i, j - 1, k);
+
+		dBz += EYZ(i + 1, j, k);
+		dBz -= EYZ(i - 1, j, k);
+		dBz -= EXZ(i, j + 1, k);
+		dBz += EXZ(i, j - 1, k);
+		dBz += EYX(i + 1, j, k);
+		dBz += EYX(i + 1, j, k - 1);
+		dBz -= EYX(i, j, k);
+		dBz -= EYX(i, j, k - 1);
+		dBz -= EXY(i, j + 1, k);
+		dBz -= EXY(i, j + 1, k - 1);
+		dBz += EXY(i, j, k);
+		dBz += EXY(i, j, k - 1);
+
+		out[5 * n + c] = in[5 * n + c] + dBx / (dy * dz);
+		out[6 * n + c] = in[6 * n + c] + dBy / (dx * dz);
+		out[7 * n + c] = in[7 * n + c] + dBz / (dx * dy);
+	}
+
+	/* conserved update of update_mhd_state, source/mhd/solve.hpp:1551-1587;
+	   flux slots hold 0 + dt*F (:375-392) */
+	for (long k = 0; k < D.nz; k++)
+	for (long j = 0; j < D.ny; j++)
+	for (long i = 0; i < D.nx; i++) {
+		const long c = cid(&D, i, j, k);
+		double lo[3][8], hi[3][8];
+		for (int q = 0; q < 8; q++) {
+			lo[0][q] = 0.0 + PF(0, q, i, j, k);
+			hi[0][q] = 0.0 + PF(0, q, i + 1, j, k);
+			lo[1][q] = 0.0 + PF(1, q, i, j, k);
+			hi[1][q] = 0.0 + PF(1, q, i, j + 1, k);
+			lo[2][q] = 0.0 + PF(2, q, i, j, k);
+			hi[2][q] = 0.0 + PF(2, q, i, j, k + 1);
+		}
+		double rho = in[c];
+		double m[3] = {in[n + c], in[2 * n + c], in[3 * n + c]};
+		double E = in[4 * n + c], vb[3];
+		vol_b(&D, in, i, j, k, vb);
+		for (int d = 0; d < 3; d++) rho += (lo[d][RHO] - hi[d][RHO]) / len[d];
+		for (int d = 0; d < 3; d++) {
+			const double il = 1 / len[d];
+			for (int q = 0; q < 3; q++) m[q] = m[q] + (lo[d][M0 + q] - hi[d][M0 + q]) * il;
+		}
+		for (int d = 0; d < 3; d++) E += (lo[d][NRJ] - hi[d][NRJ]) / len[d];
+		for (int d = 0; d < 3; d++) {
+			const double il = 1 / len[d];
+			for (int q = 0; q < 3; q++) vb[q] = vb[q] + (lo[d][B0 + q] - hi[d][B0 + q]) * il;
+		}
+
+		/* energy refits: the first uses the flux-updated volume field
+		   (timestep, :1904-1908), the later ones the new face average
+		   (:1913-1916 and apply_boundaries, mhd/boundaries.hpp:1289-1292) */
+		double vb_new[3];
+		vol_b(&D, out, i, j, k, vb_new);
+		for (int r = 0; r < refits; r++) {
+			refit_energy(rho, m, &E, r == 0 ? vb : vb_new, vb_new, g->gamma, g->mu0);
+		}
+		out[c] = rho;
+		out[n + c] = m[0];
+		out[2 * n + c] = m[1];
+		out[3 * n + c] = m[2];
+		out[4 * n + c] = E;
+	}
+#undef PF
+#undef EZX
+#undef EYX
+#undef EXY
+#undef EZY
+#undef EXZ
+#undef EYZ
+	free(P);
+	return OR_OK;
+}
+
+/* minimize_timestep + set_minmax_substepping_period with one substep,
+   source/mhd/solve.hpp:1961-2006, substepping.hpp:303-347 */
+double or_cfl_dt(const or_grid* g, double cfl, const double smax_dim[3],
+	const long zero_dim[3])
+{
+	const double len[3] = {g->dx, g->dy, g->dz};
+	double dt = DBL_MAX;
+	for (int d = 0; d < 3; d++) {
+		if (zero_dim[d] > 0) return 0;
+		dt = smin(dt, cfl * len[d] / smax_dim[d]);
+	}
+	return dt;
+}
+
+/* source/math/nabla.hpp:58-113 */
+double or_div_face2volume(const or_grid* g, const double* s)
+{
+	const dims_t D = {g->nx, g->ny, g->nz, (long)g->nx * g->ny * g->nz};
+	const long n = D.n;
+	double total = 0;
+	for (long k = 0; k < D.nz; k++)
+	for (long j = 0; j < D.ny; j++)
+	for (long i = 0; i < D.nx; i++) {
+		const long c = cid(&D, i, j, k);
+		const double div
+			= (s[5 * n + cid(&D, i + 1, j, k)] - s[5 * n + c]) / g->dx
+			+ (s[6 * n + cid(&D, i, j + 1, k)] - s[6 * n + c]) / g->dy
+			+ (s[7 * n + cid(&D, i, j, k + 1)] - s[7 * n + c]) / g->dz;
+		total += fabs(div);
+	}
+	return total / (double)n;
+}
+
+int or_run_periodic(const or_grid* g, int solver, double cfl, double t_start,
+	double t_end, long max_steps, double* state, long* steps_taken,
+	double* t_final)
+{
+	const long n = (long)g->nx * g->ny * g->nz;
+	const dims_t D = {g->nx, g->ny, g->nz, n};
+	double* tmp = (double*)malloc(sizeof(double) * 8 * n);
+	if (!tmp) return OR_ERR_SOLVER;
+
+	/* apply_boundaries before the final init, tests/mhd/test.cpp:446-453:
+	   one refit with the face-average field */
+	for (long k = 0; k < D.nz; k++)
+	for (long j = 0; j < D.ny; j++)
+	for (long i = 0; i < D.nx; i++) {
+		const long c = cid(&D, i, j, k);
+		double vb[3];
+		vol_b(&D, state, i, j, k, vb);
+		const double m[3] = {state[n + c], state[2 * n + c], state[3 * n + c]};
+		refit_energy(state[c], m, &state[4 * n + c], vb, vb, g->gamma, g->mu0);
+	}
+
+	/* final init with a zero time step, test.cpp:462-470 */
+	double sm[3];
+	long zc[3];
+	int e = or_step_periodic(g, solver, 0.0, 2, state, tmp, sm, zc, NULL);
+	if (e) { free(tmp); return e; }
+	memcpy(state, tmp, sizeof(double) * 8 * n);
+
+	double t = t_start;
+	long steps = 0;
+	while (t < t_end && steps < max_steps) {
+		const double until_end = t_end - t;
+		const double cfl_dt = or_cfl_dt(g, cfl, sm, zc);
+		/* substepping.hpp:331-338: max_dt <= smallest -> max_dt */
+		const double dt = (until_end <= cfl_dt) ? until_end : cfl_dt;
+		e = or_step_periodic(g, solver, dt, 3, state, tmp, sm, zc, NULL);
+		if (e) { free(tmp); return e; }
+		memcpy(state, tmp, sizeof(double) * 8 * n);
+		t += dt;
+		steps++;
+	}
+	free(tmp);
+	if (steps_taken) *steps_taken = steps;
+	if (t_final) *t_final = t;
+	return OR_OK;
+}
+
+/* ---------------------------------------------------------------------- */
+/* 1-D plain method of tests/mhd/reference.cpp                            */
+/* ---------------------------------------------------------------------- */
+
+/* solve_mhd, tests/mhd/reference.cpp:284-440; returns the next max dt */
+static int plain1d_step(const or_plain1d* p, int solver, double dt, double* s,
+	double* max_dt_out)
+{
+	const size_t N = p->ncells;
+	const double zero[3] = {0, 0, 0};
+	double* fl = (double*)calloc(8 * N, sizeof(double)); /* Mas_f, Mom_f, Nrj_f, Mag_f */
+	double* ele = (double*)calloc(3 * N, sizeof(double));
+	if (!fl || !ele) { free(fl); free(ele); return OR_ERR_SOLVER; }
+	double max_dt = DBL_MAX;
+#define S(q, c) s[(q) * N + (c)]
+#define FL(q, c) fl[(q) * N + (c)]
+#define EL(q, c) ele[(q) * N + (c)]
+	for (size_t c = 0; c + 1 < N; c++) {
+		double L[8], R[8], F[8], vel = -1;
+		for (int q = 0; q < 8; q++) {
+			L[q] = S(q, c);
+			R[q] = S(q, c + 1);
+		}
+		const int e = or_flux(solver, L, R, zero, p->gamma, p->mu0, F, &vel);
+		if (e) { free(fl); free(ele); return e; }
+		const double fa = 1.0 * dt; /* face_area * dt */
+		F[RHO] *= fa;
+		for (int q = 0; q < 3; q++) F[M0 + q] = F[M0 + q] * fa;
+		F[NRJ] *= fa;
+		max_dt = smin(max_dt, p->cell_size / vel);
+		if (c > 0) {
+			FL(RHO, c) -= F[RHO];
+			for (int q = 0; q < 3; q++) FL(M0 + q, c) = FL(M0 + q, c) - F[M0 + q];
+			FL(NRJ, c) -= F[NRJ];
+		}
+		if (c < N - 2) {
+			FL(RHO, c + 1) += F[RHO];
+			for (int q = 0; q < 3; q++) FL(M0 + q, c + 1) = FL(M0 + q, c + 1) + F[M0 + q];
+			FL(NRJ, c + 1) += F[NRJ];
+		}
+		/* upwind E = -v x B on the cell's +x edge, :393-404 */
+		const double ic = 1 / S(RHO, c), in = 1 / S(RHO, c + 1);
+		const double vc[3] = {S(M0, c) * ic, S(M1, c) * ic, S(M2, c) * ic};
+		const double vn[3] = {S(M0, c + 1) * in, S(M1, c + 1) * in, S(M2, c + 1) * in};
+		EL(0, c) = 0;
+		if (vc[0] + vn[0] > 0) {
+			EL(1, c) = vc[0] * S(B2, c) - vc[2] * S(B0, c);
+			EL(2, c) = vc[1] * S(B0, c) - vc[0] * S(B1, c);
+		} else {
+			EL(1, c) = vn[0] * S(B2, c + 1) - vn[2] * S(B0, c);
+			EL(2, c) = vn[1] * S(B0, c) - vn[0] * S(B1, c + 1);
+		}
+		FL(B0, c) = 0;
+		FL(B1, c) = EL(2, c);
+		FL(B2, c) = -EL(1, c);
+		if (c > 0) {
+			FL(B1, c) -= EL(2, c - 1);
+			FL(B2, c) += EL(1, c - 1);
+		}
+		for (int q = 0; q < 3; q++) FL(B0 + q, c) = FL(B0 + q, c) * dt;
+	}
+	/* apply_fluxes, source/mhd/common.hpp:707-743 */
+	const double inverse_volume = 1.0 / (1.0 * p->cell_size);
+	int err = OR_OK;
+	for (size_t c = 0; c < N; c++) {
+		S(RHO, c) += FL(RHO, c) * inverse_volume;
+		for (int q = 0; q < 3; q++) S(M0 + q, c) = S(M0 + q, c) + FL(M0 + q, c) * inverse_volume;
+		S(NRJ, c) += FL(NRJ, c) * inverse_volume;
+		for (int q = 0; q < 3; q++) S(B0 + q, c) = S(B0 + q, c) + FL(B0 + q, c) * inverse_volume;
+		if (S(RHO, c) <= 0) { err = OR_ERR_DENSITY; break; }
+		double U[8], pr;
+		for (int q = 0; q < 8; q++) U[q] = S(q, c);
+		or_get_pressure(U, p->gamma, p->mu0, &pr);
+		if (pr <= 0) { err = OR_ERR_PRESSURE; break; }
+	}
+#undef S
+#undef FL
+#undef EL
+	free(fl);
+	free(ele);
+	*max_dt_out = max_dt;
+	return err;
+}
+
+/* main loop of tests/mhd/reference.cpp:830-877 */
+int or_plain1d_run(const or_plain1d* p, int solver, double t_end, double* state,
+	long* steps_taken)
+{
+	double max_dt = 0, t = 0;
+	long steps = 0;
+	while (t < t_end) {
+		const double until_end = t_end - t;
+		const double time_step = smin(p->cfl * max_dt, until_end);
+		double next;
+		const int e = plain1d_step(p, solver, time_step, state, &next);
+		if (e) return e;
+		max_dt = smin(DBL_MAX, next);
+		t += time_step;
+		steps++;
+	}
+	if (steps_taken) *steps_taken = steps;
+	return OR_OK;
+}
+
+/* 7-point Gauss-Legendre rule (textbook nodes and weights), mapped to
+   [a, b] and divided by b - a */
+double or_gl7_average(double (*f)(double, void*), void* ctx, double a, double b)
+{
+	static const double x[4] = {
+		0.0,
+		0.405845151377397166906606412076961,
+		0.741531185599394439863864773280788,
+		0.949107912342758524526189684047851};
+	static const double w[4] = {
+		0.417959183673469387755102040816327,
+		0.381830050505118944950369775488975,
+		0.279705391489276667901467771423780,
+		0.129484966168869693270611432679082};
+	const double avg = (a + b) * 0.5, half = (b - a) * 0.5;
+	double r = f(avg, ctx) * w[0];
+	for (int i = 1; i < 4; i++) {
+		const double fp = f(avg + half * x[i], ctx);
+		const double fm = f(avg - half * x[i], ctx);
+		r += (fp + fm) * w[i];
+	}
+	return r * half / (b - a);
+}

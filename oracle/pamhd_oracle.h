@@ -1,0 +1,158 @@
+/*
+FP64 CPU oracle for the PAMHD ideal-MHD finite-volume step.
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product path (the HIP library,
+its C ABI, the Python package) may link, load or call this code; only the
+tests, the smoke check and the CPU-baseline leg of the benchmark use it,
+and only as the checker.
+
+Every function restates the reference's arithmetic in the same operation
+order (no FMA contraction: build with -ffp-contract=off) so that the GPU's
+FP64 path can be compared to it bit for bit.  Each function cites the
+reference file:line it follows (paths relative to the reference checkout).
+
+State vectors in the face-normal frame are 8 doubles:
+    {rho, m0, m1, m2, E, B0, B1, B2}   (index 0 is the face normal)
+*/
+#ifndef PAMHD_ORACLE_H
+#define PAMHD_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* solver numbering of source/mhd/common.hpp:54-61 */
+enum {
+	OR_RUSANOV = 0,
+	OR_RUSANOV_STAGGERED = 1,
+	OR_HLL_ATHENA = 2,
+	OR_HLLD_ATHENA = 3,
+	OR_ROE_ATHENA = 4,
+	OR_HYBRID = 5
+};
+
+/* error classes of the std::domain_error throw sites of the solvers */
+enum {
+	OR_OK = 0,
+	OR_ERR_DENSITY = 1,      /* non-positive / non-normal mass density */
+	OR_ERR_PRESSURE = 2,     /* non-positive / non-normal thermal pressure */
+	OR_ERR_SPEED = 3,        /* invalid wave speed, squared speeds or to_sqrt */
+	OR_ERR_FLUX = 4,         /* non-finite momentum or flux */
+	OR_ERR_FIELD = 5,        /* non-finite total magnetic field */
+	OR_ERR_ROE_DENSITY = 6,  /* Roe intermediate state density */
+	OR_ERR_ROE_PRESSURE = 7, /* Roe intermediate state pressure */
+	OR_ERR_SOLVER = 8        /* unknown solver */
+};
+
+/* ---- physics helpers, source/mhd/common.hpp ---- */
+int or_get_pressure(const double U[8], double gamma, double mu0, double* p);
+double or_total_energy(double rho, const double v[3], double p, const double B[3],
+	double gamma, double mu0);
+int or_fast_speed(const double U[8], const double bg[3], double gamma, double mu0,
+	double* cf);
+int or_physical_flux(const double U[8], const double bg[3], double gamma,
+	double mu0, double F[8]);
+
+/* ---- Riemann solvers: L = negative side, R = positive side ---- */
+int or_flux_rusanov(const double L[8], const double R[8], const double bg[3],
+	double gamma, double mu0, double F[8], double* smax);
+int or_flux_hll(const double L[8], const double R[8], const double bg[3],
+	double gamma, double mu0, double F[8], double* smax);
+int or_flux_hlld(const double L[8], const double R[8], const double bg[3],
+	double gamma, double mu0, double F[8], double* smax);
+int or_flux_roe(const double L[8], const double R[8], const double bg[3],
+	double gamma, double mu0, double F[8], double* smax);
+int or_flux(int solver, const double L[8], const double R[8], const double bg[3],
+	double gamma, double mu0, double F[8], double* smax);
+
+/* Roe eigensystem, exposed for the L*R = I check */
+void or_roe_eigensystem(double d, double v1, double v2, double v3, double h,
+	double b1, double b2, double b3, double x, double y, double gamma,
+	double ev[7], double rem[7][7], double lem[7][7]);
+
+/*
+Batched face flux in structure-of-arrays form: L, R are 8*n (component-major),
+bg is 3*n or NULL, F is 8*n, smax and status are n.  Returns number of faces
+whose status is non-zero.
+*/
+size_t or_face_flux_batch(int solver, size_t n, const double* L, const double* R,
+	const double* bg, double gamma, double mu0, double* F, double* smax,
+	int* status);
+
+/* ---- uniform periodic staggered grid (Appendix A of the restatement) ---- */
+typedef struct {
+	int nx, ny, nz;
+	double dx, dy, dz;
+	double gamma, mu0;
+	/* background field per face owned by the cell, 9*n (face x comps, face y
+	   comps, face z comps; each component-major) or NULL for zero */
+	const double* bg;
+} or_grid;
+
+/*
+Cell arrays, 8*n component-major:
+    rho, mx, my, mz, E, Bx(i-1/2,j,k), By(i,j-1/2,k), Bz(i,j,k-1/2)
+with cell index i + nx*(j + ny*k).
+*/
+
+/*
+One time step of length dt on a fully periodic grid of normal cells:
+face fluxes, flux slots, constrained-transport face dB, conserved update,
+and `refits` constant-pressure energy refits (the reference makes three per
+main-loop step).  Writes the new state to out (may not alias in).
+
+smax_dim[d] receives the largest signal speed over d-faces and zero_dim[d]
+the number of d-faces whose speed is <= 0 (either may be NULL).
+Returns 0 or the first error code; *bad_face gets its flat face index
+(dir*n + cell) when not NULL.
+*/
+int or_step_periodic(const or_grid* g, int solver, double dt, int refits,
+	const double* in, double* out, double smax_dim[3], long zero_dim[3],
+	long* bad_face);
+
+/* lagged CFL time step: min over dims of cfl*len/smax, 0 if any speed <= 0 */
+double or_cfl_dt(const or_grid* g, double cfl, const double smax_dim[3],
+	const long zero_dim[3]);
+
+/* volume-averaged |div B| over all cells, source/math/nabla.hpp:58-113 */
+double or_div_face2volume(const or_grid* g, const double* state);
+
+/*
+Program driver equivalent to tests/mhd/test.cpp:446-610 for a periodic grid:
+one refit for apply_boundaries, the zero-dt final-init step, then steps of
+dt = min(lagged CFL dt, t_end - t) until t >= t_end or max_steps steps.
+state is updated in place.  Returns 0 or an error code; *steps_taken and
+*t_final receive the step count and time.
+*/
+int or_run_periodic(const or_grid* g, int solver, double cfl, double t_start,
+	double t_end, long max_steps, double* state, long* steps_taken,
+	double* t_final);
+
+/* ---- 1-D plain method of tests/mhd/reference.cpp ---- */
+typedef struct {
+	size_t ncells;           /* including the two end cells */
+	double cell_size;
+	double gamma, mu0;
+	double cfl;              /* 0.5 in the reference */
+} or_plain1d;
+
+/*
+state: 8*ncells component-major {rho, m0, m1, m2, E, B0, B1, B2}.
+Runs from t = 0 until t >= t_end with the reference's lagged dt.
+*/
+int or_plain1d_run(const or_plain1d* p, int solver, double t_end, double* state,
+	long* steps_taken);
+
+/* ---- initial conditions used by the goldens and the benchmark configs ---- */
+
+/* 7-point Gauss-Legendre average over [a, b] of f (textbook nodes/weights) */
+double or_gl7_average(double (*f)(double, void*), void* ctx, double a, double b);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,210 @@
+"""ctypes access to the FP64 CPU oracle (oracle/liboracle.so) and, when it was
+built, to the reference's compiled flux headers (oracle/_ref/libref_flux.so).
+
+Test infrastructure: only tests, the smoke check and the CPU-baseline leg of
+bench.py import this module.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+REF_SO = os.path.join(ORACLE_DIR, "_ref", "libref_flux.so")
+
+RUSANOV, RUSANOV_STAGGERED, HLL, HLLD, ROE, HYBRID = range(6)
+SOLVERS = {"rusanov": RUSANOV, "hll_athena": HLL, "hlld_athena": HLLD,
+           "roe_athena": ROE}
+
+_dp = ctypes.POINTER(ctypes.c_double)
+_lp = ctypes.POINTER(ctypes.c_long)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+class OrGrid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double),
+                ("dz", ctypes.c_double), ("gamma", ctypes.c_double),
+                ("mu0", ctypes.c_double), ("bg", _dp)]
+
+
+class OrPlain1d(ctypes.Structure):
+    _fields_ = [("ncells", ctypes.c_size_t), ("cell_size", ctypes.c_double),
+                ("gamma", ctypes.c_double), ("mu0", ctypes.c_double),
+                ("cfl", ctypes.c_double)]
+
+
+_lib = None
+
+
+def lib():
+    """Load (building if needed) the oracle library."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(ORACLE_SO):
+        subprocess.run(["make", "-C", ORACLE_DIR], check=True,
+                       stdout=subprocess.DEVNULL)
+    L = ctypes.CDLL(ORACLE_SO)
+    for name in ("or_flux_rusanov", "or_flux_hll", "or_flux_hlld", "or_flux_roe"):
+        f = getattr(L, name)
+        f.argtypes = [_dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        f.restype = ctypes.c_int
+    L.or_flux.argtypes = [ctypes.c_int, _dp, _dp, _dp, ctypes.c_double,
+                          ctypes.c_double, _dp, _dp]
+    L.or_flux.restype = ctypes.c_int
+    L.or_face_flux_batch.argtypes = [ctypes.c_int, ctypes.c_size_t, _dp, _dp, _dp,
+                                     ctypes.c_double, ctypes.c_double, _dp, _dp,
+                                     ctypes.POINTER(ctypes.c_int)]
+    L.or_face_flux_batch.restype = ctypes.c_size_t
+    L.or_physical_flux.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, _dp]
+    L.or_physical_flux.restype = ctypes.c_int
+    L.or_fast_speed.argtypes = [_dp, _dp, ctypes.c_double, ctypes.c_double, _dp]
+    L.or_fast_speed.restype = ctypes.c_int
+    L.or_get_pressure.argtypes = [_dp, ctypes.c_double, ctypes.c_double, _dp]
+    L.or_get_pressure.restype = ctypes.c_int
+    L.or_roe_eigensystem.argtypes = [ctypes.c_double] * 11 + [_dp, _dp, _dp]
+    L.or_roe_eigensystem.restype = None
+    L.or_step_periodic.argtypes = [ctypes.POINTER(OrGrid), ctypes.c_int,
+                                   ctypes.c_double, ctypes.c_int, _dp, _dp, _dp,
+                                   _lp, _lp]
+    L.or_step_periodic.restype = ctypes.c_int
+    L.or_cfl_dt.argtypes = [ctypes.POINTER(OrGrid), ctypes.c_double, _dp, _lp]
+    L.or_cfl_dt.restype = ctypes.c_double
+    L.or_div_face2volume.argtypes = [ctypes.POINTER(OrGrid), _dp]
+    L.or_div_face2volume.restype = ctypes.c_double
+    L.or_run_periodic.argtypes = [ctypes.POINTER(OrGrid), ctypes.c_int,
+                                  ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_long, _dp, _lp, _dp]
+    L.or_run_periodic.restype = ctypes.c_int
+    L.or_plain1d_run.argtypes = [ctypes.POINTER(OrPlain1d), ctypes.c_int,
+                                 ctypes.c_double, _dp, _lp]
+    L.or_plain1d_run.restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def flux(solver, L, R, bg=(0.0, 0.0, 0.0), gamma=5.0 / 3.0, mu0=1.0):
+    """One face flux; returns (status, F[8], smax)."""
+    l = np.ascontiguousarray(L, dtype=np.float64)
+    r = np.ascontiguousarray(R, dtype=np.float64)
+    b = np.ascontiguousarray(bg, dtype=np.float64)
+    F = np.zeros(8)
+    s = np.zeros(1)
+    st = lib().or_flux(solver, _ptr(l), _ptr(r), _ptr(b), gamma, mu0, _ptr(F), _ptr(s))
+    return st, F, float(s[0])
+
+
+def physical_flux(U, bg=(0.0, 0.0, 0.0), gamma=5.0 / 3.0, mu0=1.0):
+    u = np.ascontiguousarray(U, dtype=np.float64)
+    b = np.ascontiguousarray(bg, dtype=np.float64)
+    F = np.zeros(8)
+    st = lib().or_physical_flux(_ptr(u), _ptr(b), gamma, mu0, _ptr(F))
+    return st, F
+
+
+def flux_batch(solver, L, R, bg=None, gamma=5.0 / 3.0, mu0=1.0):
+    """SoA batch: L, R (8, n); bg (3, n) or None.  Returns F, smax, status."""
+    L = np.ascontiguousarray(L, dtype=np.float64)
+    R = np.ascontiguousarray(R, dtype=np.float64)
+    n = L.shape[1]
+    F = np.zeros((8, n))
+    s = np.zeros(n)
+    st = np.zeros(n, dtype=np.int32)
+    bgp = None
+    if bg is not None:
+        bg = np.ascontiguousarray(bg, dtype=np.float64)
+        bgp = _ptr(bg)
+    lib().or_face_flux_batch(solver, n, _ptr(L), _ptr(R), bgp, gamma, mu0,
+                             _ptr(F), _ptr(s),
+                             st.ctypes.data_as(ctypes.POINTER(ctypes.c_int)))
+    return F, s, st
+
+
+def make_grid(shape, lengths, gamma, mu0, bg=None):
+    nx, ny, nz = shape
+    g = OrGrid(nx, ny, nz, lengths[0], lengths[1], lengths[2], gamma, mu0, None)
+    if bg is not None:
+        g.bg = _ptr(bg)
+    return g
+
+
+def step(grid, solver, dt, state, refits=3):
+    """One periodic step; state is (8, n).  Returns (status, new, smax3, zero3)."""
+    state = np.ascontiguousarray(state, dtype=np.float64)
+    out = np.zeros_like(state)
+    sm = np.zeros(3)
+    zc = np.zeros(3, dtype=np.int64)
+    bad = ctypes.c_long(-1)
+    st = lib().or_step_periodic(ctypes.byref(grid), solver, dt, refits, _ptr(state),
+                                _ptr(out), _ptr(sm), zc.ctypes.data_as(_lp),
+                                ctypes.byref(bad))
+    return st, out, sm, zc
+
+
+def cfl_dt(grid, cfl, sm, zc):
+    sm = np.ascontiguousarray(sm, dtype=np.float64)
+    zc = np.ascontiguousarray(zc, dtype=np.int64)
+    return lib().or_cfl_dt(ctypes.byref(grid), cfl, _ptr(sm), zc.ctypes.data_as(_lp))
+
+
+def div_face2volume(grid, state):
+    state = np.ascontiguousarray(state, dtype=np.float64)
+    return lib().or_div_face2volume(ctypes.byref(grid), _ptr(state))
+
+
+def run_periodic(grid, solver, cfl, t_start, t_end, state, max_steps=1 << 40):
+    """Program driver of tests/mhd/test.cpp for a periodic grid (in place)."""
+    assert state.flags.c_contiguous and state.dtype == np.float64
+    steps = ctypes.c_long(0)
+    tf = ctypes.c_double(0)
+    st = lib().or_run_periodic(ctypes.byref(grid), solver, cfl, t_start, t_end,
+                               max_steps, _ptr(state), ctypes.byref(steps),
+                               ctypes.byref(tf))
+    return st, steps.value, tf.value
+
+
+def plain1d_run(solver, state, cell_size, gamma, mu0, t_end, cfl=0.5):
+    assert state.flags.c_contiguous and state.dtype == np.float64
+    p = OrPlain1d(state.shape[1], cell_size, gamma, mu0, cfl)
+    steps = ctypes.c_long(0)
+    st = lib().or_plain1d_run(ctypes.byref(p), solver, t_end, _ptr(state),
+                              ctypes.byref(steps))
+    return st, steps.value
+
+
+_ref = None
+
+
+def ref_lib():
+    """The reference's compiled Rusanov/HLL (None when not built here)."""
+    global _ref
+    if _ref is not None:
+        return _ref
+    if not os.path.exists(REF_SO):
+        return None
+    R = ctypes.CDLL(REF_SO)
+    for name in ("ref_flux_rusanov", "ref_flux_hll"):
+        f = getattr(R, name)
+        f.argtypes = [_dp, _dp, _dp, ctypes.c_double, ctypes.c_double, _dp, _dp]
+        f.restype = ctypes.c_int
+    _ref = R
+    return R
+
+
+def ref_flux(name, L, R, bg=(0.0, 0.0, 0.0), gamma=5.0 / 3.0, mu0=1.0):
+    lib_ = ref_lib()
+    l = np.ascontiguousarray(L, dtype=np.float64)
+    r = np.ascontiguousarray(R, dtype=np.float64)
+    b = np.ascontiguousarray(bg, dtype=np.float64)
+    F = np.zeros(8)
+    s = np.zeros(1)
+    st = getattr(lib_, "ref_flux_" + name)(_ptr(l), _ptr(r), _ptr(b), gamma, mu0,
+                                           _ptr(F), _ptr(s))
+    return st, F, float(s[0])
