@@ -1,0 +1,179 @@
+/*
+C ABI of the MI355X-native PAMHD ideal-MHD finite-volume step.
+
+The library owns a uniform Cartesian grid of cells in HBM in structure-of-
+arrays form and advances it with hand-written HIP kernels for gfx950.  It is
+a drop-in for the reference's MHD step:
+
+  reference entry point                          replaced by
+  ---------------------------------------------  --------------------------------
+  pamhd::mhd::get_flux_rusanov / athena::         pamhd_face_flux_batch
+    get_flux_hll / get_flux_hlld / get_flux_roe
+    (source/mhd/rusanov.hpp:59-73,
+     hll_athena.hpp:57-71, hlld_athena.hpp:54-68,
+     roe_athena.hpp:567-581)
+  pamhd::mhd::timestep                            pamhd_timestep
+    (source/mhd/solve.hpp:1837-1924)
+  timestep + apply_boundaries + sync_magnetic_    pamhd_step
+    field of the main loop
+    (tests/mhd/test.cpp:514-568)
+  pamhd::mhd::apply_boundaries                    pamhd_apply_boundaries
+    (source/mhd/boundaries.hpp:1249-1297)
+  pamhd::math::get_divergence_face2volume         pamhd_div_face2volume
+    (source/math/nabla.hpp:58-113)
+
+Conventions
+  * All functions return 0 on success or a PAMHD_E* code; no exception
+    crosses the ABI.  pamhd_last_error() gives a message for the last failure
+    of a handle (physics errors name the face and the failed check, like the
+    diagnostic of source/mhd/solve.hpp:156-181).
+  * State arrays exchanged with the host are component-major double arrays of
+    8 * n values, n = nx * ny * nz, cell index i + nx * (j + ny * k):
+        rho, m_x, m_y, m_z, E, B_x(i-1/2,j,k), B_y(i,j-1/2,k), B_z(i,j,k-1/2)
+    i.e. the conserved variables of the cell (Mass_Density, Momentum_Density,
+    Total_Energy_Density) and the face magnetic field of the cell's three
+    negative-side faces (the reference's Face_Magnetic_Field(-dir), stored
+    once per face).  The cell-centred field is the average of the two faces,
+    as update_vol_B computes it (source/mhd/solve.hpp:1661-1663).
+  * `stream` is a hipStream_t (NULL = the default stream).  Calls are
+    stream-ordered; only calls that return host values synchronise.
+*/
+#ifndef PAMHD_MI355X_H
+#define PAMHD_MI355X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Riemann solver numbering of source/mhd/common.hpp:54-61 */
+enum pamhd_solver {
+	PAMHD_RUSANOV = 0,
+	PAMHD_RUSANOV_STAGGERED = 1, /* not a separate solver in the reference's dispatch */
+	PAMHD_HLL_ATHENA = 2,
+	PAMHD_HLLD_ATHENA = 3,
+	PAMHD_ROE_ATHENA = 4,
+	PAMHD_HYBRID = 5             /* MHD solution not used: fluxes are skipped */
+};
+
+/* arithmetic of the face fluxes and wave speeds; the cell state is FP64 in both */
+enum pamhd_precision {
+	PAMHD_FP64 = 0,   /* bit-compatible with the FP64 CPU path */
+	PAMHD_MIXED = 1   /* FP32 fluxes/speeds, FP64 state and pressure recovery */
+};
+
+enum pamhd_status {
+	PAMHD_OK = 0,
+	PAMHD_E_DENSITY = 1,       /* non-positive or non-normal mass density */
+	PAMHD_E_PRESSURE = 2,      /* non-positive or non-normal thermal pressure */
+	PAMHD_E_SPEED = 3,         /* invalid wave speed */
+	PAMHD_E_FLUX = 4,          /* non-finite momentum or flux */
+	PAMHD_E_FIELD = 5,         /* non-finite total magnetic field */
+	PAMHD_E_ROE_DENSITY = 6,   /* Roe intermediate-state density */
+	PAMHD_E_ROE_PRESSURE = 7,  /* Roe intermediate-state pressure */
+	PAMHD_E_SOLVER = 8,        /* unknown solver */
+	PAMHD_E_ARG = 16,          /* invalid argument */
+	PAMHD_E_UNSUPPORTED = 17,  /* valid request this build does not implement */
+	PAMHD_E_DEVICE = 18,       /* HIP runtime failure (allocation, launch) */
+	PAMHD_E_NOLIB = 19
+};
+
+typedef struct pamhd_grid_desc {
+	int32_t cells[3];        /* nx, ny, nz (grid-options "cells") */
+	double start[3];         /* grid-options "start" */
+	double cell_length[3];   /* volume / cells, tests/mhd/test.cpp:339-346 */
+	int32_t periodic[3];     /* grid-options "periodic" */
+} pamhd_grid_desc;
+
+typedef struct pamhd_physics {
+	double adiabatic_index;      /* "adiabatic-index" */
+	double vacuum_permeability;  /* "vacuum-permeability" */
+	double proton_mass;          /* "proton-mass" */
+} pamhd_physics;
+
+typedef struct pamhd_handle pamhd_handle;
+
+/* Library/ABI version, for loaders: returns 1. */
+int pamhd_abi_version(void);
+
+/* Allocates the device state for a grid.  Fully periodic grids only in this
+   version (PAMHD_E_UNSUPPORTED otherwise). */
+int pamhd_create(const pamhd_grid_desc* grid, const pamhd_physics* physics,
+	int precision, pamhd_handle** out);
+int pamhd_destroy(pamhd_handle* h);
+
+/* Host <-> device copies of the 8 * n state described above (synchronous). */
+int pamhd_upload_state(pamhd_handle* h, const double* state);
+int pamhd_download_state(pamhd_handle* h, double* state);
+
+/* Device pointer of the current 8 * n state (valid until the next step). */
+int pamhd_state_device_ptr(pamhd_handle* h, double** ptr);
+
+/* Background field on each cell's positive-side faces (the reference's
+   Bg_B(cell)(+dir), source/common_variables.hpp:553-559), 9 * n host doubles:
+   x-face components x,y,z, then y-face, then z-face; NULL clears it. */
+int pamhd_set_background_faces(pamhd_handle* h, const double* bg);
+
+/*
+One step of pamhd::mhd::timestep with global time stepping: the step length
+is min(lagged CFL dt, max_time_step) following
+source/substepping.hpp:331-338, where the lagged CFL dt is
+time_step_factor * cell length / max signal speed of the previous flux pass
+(0 before the first pass, or if any face speed was <= 0).  Computes all face
+fluxes, the constrained-transport face-field update, the conserved update and
+the two constant-pressure energy refits of the reference's timestep.
+
+dt_taken (may be NULL) receives the step length; when non-NULL the call
+synchronises and reports device-side physics errors.
+*/
+int pamhd_timestep(pamhd_handle* h, int solver, double max_time_step,
+	double time_step_factor, double* dt_taken, void* stream);
+
+/* Boundary update of pamhd::mhd::apply_boundaries: for a periodic grid the
+   constant-pressure energy refit of all cells. */
+int pamhd_apply_boundaries(pamhd_handle* h, double simulation_time, void* stream);
+
+/* Fused main-loop step: pamhd_timestep + pamhd_apply_boundaries in one pass
+   (three energy refits), as tests/mhd/test.cpp:523-568 does per step. */
+int pamhd_step(pamhd_handle* h, int solver, double max_time_step,
+	double time_step_factor, double* dt_taken, void* stream);
+
+/*
+Program loop of tests/mhd/test.cpp:446-610 on the device-resident state:
+apply_boundaries, the zero-length "final init" timestep, then steps of
+min(lagged CFL dt, t_end - t) until t >= t_end or max_steps steps.
+*/
+int pamhd_run(pamhd_handle* h, int solver, double time_step_factor,
+	double t_start, double t_end, long max_steps, long* steps_taken,
+	double* t_final, void* stream);
+
+/* Synchronises and returns the first device-side physics error, if any. */
+int pamhd_check(pamhd_handle* h);
+
+/* Mean |div B| of the face field over all cells (synchronous). */
+int pamhd_div_face2volume(pamhd_handle* h, double* avg_div);
+
+/* Largest face signal speed per dimension of the last flux pass (synchronous). */
+int pamhd_max_signal_speeds(pamhd_handle* h, double smax[3]);
+
+/*
+Batched Riemann solver on device buffers: L, R (8 * n), bg (3 * n or NULL),
+F (8 * n), smax (n), status (n, may be NULL), component-major, states already
+rotated into the face frame.  precision selects FP64 or FP32 arithmetic.
+Faces with a physics error get status != 0 and zero flux.
+*/
+int pamhd_face_flux_batch(int solver, int precision, size_t n, const double* L,
+	const double* R, const double* bg, double adiabatic_index,
+	double vacuum_permeability, double* F, double* smax, int* status,
+	void* stream);
+
+/* Copies the last error message of h (or of the library when h is NULL). */
+int pamhd_last_error(pamhd_handle* h, char* buf, size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

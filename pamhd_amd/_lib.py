@@ -1,0 +1,95 @@
+"""ctypes binding of the C ABI in include/pamhd_mi355x.h.
+
+The library is built in-tree (pamhd_amd/lib/libpamhd_mi355x.so) by
+``python -m pamhd_amd.build``.  There is no CPU fallback: every entry point
+runs the HIP kernels, and loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "lib", "libpamhd_mi355x.so")
+
+# reference solver numbering, source/mhd/common.hpp:54-61
+RUSANOV, RUSANOV_STAGGERED, HLL_ATHENA, HLLD_ATHENA, ROE_ATHENA, HYBRID = range(6)
+FP64, MIXED = 0, 1
+
+EXPORTS = (
+    "pamhd_abi_version", "pamhd_create", "pamhd_destroy", "pamhd_upload_state",
+    "pamhd_download_state", "pamhd_state_device_ptr", "pamhd_set_background_faces",
+    "pamhd_timestep", "pamhd_apply_boundaries", "pamhd_step", "pamhd_run",
+    "pamhd_check", "pamhd_div_face2volume", "pamhd_max_signal_speeds",
+    "pamhd_face_flux_batch", "pamhd_last_error",
+)
+
+
+class GridDesc(ctypes.Structure):
+    _fields_ = [("cells", ctypes.c_int32 * 3), ("start", ctypes.c_double * 3),
+                ("cell_length", ctypes.c_double * 3), ("periodic", ctypes.c_int32 * 3)]
+
+
+class Physics(ctypes.Structure):
+    _fields_ = [("adiabatic_index", ctypes.c_double),
+                ("vacuum_permeability", ctypes.c_double),
+                ("proton_mass", ctypes.c_double)]
+
+
+class PamhdError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__("pamhd error %d: %s" % (code, message))
+        self.code = code
+
+
+_lib = None
+_dp = ctypes.POINTER(ctypes.c_double)
+_vp = ctypes.c_void_p
+
+
+def load():
+    """Load the HIP library; raises if it was not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("pamhd_amd HIP library not built: %s "
+                          "(run python -m pamhd_amd.build)" % LIB_PATH)
+    L = ctypes.CDLL(LIB_PATH)
+    H = ctypes.c_void_p
+    L.pamhd_abi_version.restype = ctypes.c_int
+    L.pamhd_create.argtypes = [ctypes.POINTER(GridDesc), ctypes.POINTER(Physics),
+                               ctypes.c_int, ctypes.POINTER(H)]
+    L.pamhd_destroy.argtypes = [H]
+    L.pamhd_upload_state.argtypes = [H, _vp]
+    L.pamhd_download_state.argtypes = [H, _vp]
+    L.pamhd_state_device_ptr.argtypes = [H, ctypes.POINTER(_vp)]
+    L.pamhd_set_background_faces.argtypes = [H, _vp]
+    for name in ("pamhd_timestep", "pamhd_step"):
+        getattr(L, name).argtypes = [H, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     _dp, _vp]
+    L.pamhd_apply_boundaries.argtypes = [H, ctypes.c_double, _vp]
+    L.pamhd_run.argtypes = [H, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                            ctypes.c_double, ctypes.c_long,
+                            ctypes.POINTER(ctypes.c_long), _dp, _vp]
+    L.pamhd_check.argtypes = [H]
+    L.pamhd_div_face2volume.argtypes = [H, _dp]
+    L.pamhd_max_signal_speeds.argtypes = [H, _dp]
+    L.pamhd_face_flux_batch.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
+                                        _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
+                                        _vp, _vp, _vp, _vp]
+    L.pamhd_last_error.argtypes = [H, ctypes.c_char_p, ctypes.c_size_t]
+    for name in EXPORTS:
+        getattr(L, name).restype = ctypes.c_int
+    _lib = L
+    return L
+
+
+def last_error(handle=None):
+    buf = ctypes.create_string_buffer(512)
+    load().pamhd_last_error(handle, buf, 512)
+    return buf.value.decode()
+
+
+def check(code, handle=None):
+    if code != 0:
+        raise PamhdError(code, last_error(handle))
+    return code

@@ -1,0 +1,797 @@
+/*
+MI355X (gfx950) implementation of the PAMHD ideal-MHD finite-volume step
+behind the C ABI of include/pamhd_mi355x.h.
+
+Data layout in HBM (structure of arrays, FP64, component-major):
+  state[2]  ping-pong buffers of 8 * n doubles:
+            rho, m_x, m_y, m_z, E, B_x(i-1/2), B_y(j-1/2), B_z(k-1/2)
+  flux      24 * n doubles: dt * F_q of the d-face owned by each cell
+            (d-major, then component, then cell), written by the flux pass
+            and read by the update pass
+  bg        optional 9 * n doubles of background field on the positive faces
+  status[2] per-pass reductions (max signal speed per dimension, count of
+            faces with speed <= 0, first physics error) feeding the lagged
+            CFL time step of the next step
+
+One step = dt_kernel (1 thread: lagged CFL step from the previous pass) ->
+flux_pass_kernel (one thread per face, all faces of one dimension in a
+z-slab of blocks) -> update_kernel (one thread per cell: constrained-
+transport face update, conserved update, constant-pressure energy refits).
+*/
+#include <hip/hip_runtime.h>
+
+#include <cfloat>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+
+#include "pamhd_mi355x.h"
+#include "riemann.hpp"
+
+using namespace pamhd_dev;
+
+namespace {
+
+struct GridDev {
+	int nx, ny, nz;
+	long long n;
+	double dx, dy, dz;
+	double gamma, mu0;
+};
+
+struct StepStatus {
+	unsigned long long smax_bits[3];
+	unsigned int zero[3];
+	int err;
+	long long err_face;
+};
+
+constexpr int BX = 64, BY = 4;
+
+__device__ __forceinline__ int wrap_dec(int i, int n) { return i == 0 ? n - 1 : i - 1; }
+__device__ __forceinline__ int wrap_inc(int i, int n) { return i + 1 == n ? 0 : i + 1; }
+
+/* cell state rotated into the frame of a d-face (get_rotated_vector,
+   source/mhd/common.hpp:612-648; vol B of update_vol_B, solve.hpp:1661) */
+__device__ __forceinline__ void load_face_state(const double* __restrict__ s, long long n,
+	const GridDev& g, int d, int i, int j, int k, double U[8])
+{
+	const long long c = i + (long long)g.nx * (j + (long long)g.ny * k);
+	const long long cx = wrap_inc(i, g.nx) + (long long)g.nx * (j + (long long)g.ny * k);
+	const long long cy = i + (long long)g.nx * (wrap_inc(j, g.ny) + (long long)g.ny * k);
+	const long long cz = i + (long long)g.nx * (j + (long long)g.ny * wrap_inc(k, g.nz));
+	const double m[3] = {s[n + c], s[2 * n + c], s[3 * n + c]};
+	const double B[3] = {
+		0.5 * (s[5 * n + c] + s[5 * n + cx]),
+		0.5 * (s[6 * n + c] + s[6 * n + cy]),
+		0.5 * (s[7 * n + c] + s[7 * n + cz])};
+	U[RHO] = s[c];
+	U[NRJ] = s[4 * n + c];
+	const int a = d, b = (d + 1) % 3, e = (d + 2) % 3;
+	U[M0] = m[a];
+	U[M1] = m[b];
+	U[M2] = m[e];
+	U[B0] = B[a];
+	U[B1] = B[b];
+	U[B2] = B[e];
+}
+
+__device__ __forceinline__ void record_error(StepStatus* st, int code, long long face)
+{
+	if (atomicCAS(&st->err, 0, code) == 0) {
+		st->err_face = face;
+	}
+}
+
+/* block-wide max of s and count of s <= 0, one atomic per block */
+__device__ __forceinline__ void reduce_speed(StepStatus* st, int d, double s, unsigned zero)
+{
+	__shared__ double ms[BX * BY / 64];
+	__shared__ unsigned mz[BX * BY / 64];
+	for (int off = 32; off > 0; off >>= 1) {
+		const double o = __shfl_xor(s, off, 64);
+		s = smax(s, o);
+		zero += __shfl_xor(zero, off, 64);
+	}
+	const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+	const int wave = tid / 64, lane = tid % 64;
+	if (lane == 0) {
+		ms[wave] = s;
+		mz[wave] = zero;
+	}
+	__syncthreads();
+	if (tid == 0) {
+		double m = ms[0];
+		unsigned z = mz[0];
+		for (int w = 1; w < (BX * BY) / 64; w++) {
+			m = smax(m, ms[w]);
+			z += mz[w];
+		}
+		const unsigned long long bits = m > 0 ? (unsigned long long)__double_as_longlong(m) : 0ull;
+		atomicMax(&st->smax_bits[d], bits);
+		if (z) atomicAdd(&st->zero[d], z);
+	}
+}
+
+/*
+Flux pass: get_fluxes + get_flux of source/mhd/solve.hpp:68-188, :304-393.
+Thread (i, j, k) of z-slab d computes the d-face on the cell's negative side
+between L = cell - e_d and R = cell, stores dt * F in the global frame.
+*/
+template <int SOLVER, class T>
+__global__ __launch_bounds__(BX * BY) void flux_pass_kernel(GridDev g,
+	const double* __restrict__ s, const double* __restrict__ bg,
+	double* __restrict__ P, const double* __restrict__ dtp, StepStatus* st)
+{
+	const int i = blockIdx.x * BX + threadIdx.x;
+	const int j = blockIdx.y * BY + threadIdx.y;
+	const int d = blockIdx.z / g.nz;
+	const int k = blockIdx.z - d * g.nz;
+	const long long n = g.n;
+	double sp = 0;
+	unsigned zero = 0;
+	if (i < g.nx && j < g.ny) {
+		const long long c = i + (long long)g.nx * (j + (long long)g.ny * k);
+		const int il = d == 0 ? wrap_dec(i, g.nx) : i;
+		const int jl = d == 1 ? wrap_dec(j, g.ny) : j;
+		const int kl = d == 2 ? wrap_dec(k, g.nz) : k;
+		double L[8], R[8];
+		load_face_state(s, n, g, d, il, jl, kl, L);
+		load_face_state(s, n, g, d, i, j, k, R);
+		T b[3] = {0, 0, 0};
+		if (bg) {
+			const double bgl[3] = {bg[(3 * d + 0) * n + c], bg[(3 * d + 1) * n + c], bg[(3 * d + 2) * n + c]};
+			b[0] = (T)bgl[d];
+			b[1] = (T)bgl[(d + 1) % 3];
+			b[2] = (T)bgl[(d + 2) % 3];
+		}
+		T F[8], sT = 0;
+		const int e = riemann<SOLVER, T>(L, R, b, g.gamma, g.mu0, F, sT);
+		double Fg[8];
+		if (e) {
+			record_error(st, e, (long long)d * n + c);
+			for (int q = 0; q < 8; q++) Fg[q] = 0;
+			sT = 0;
+		} else {
+			Fg[RHO] = (double)F[RHO];
+			Fg[NRJ] = (double)F[NRJ];
+			/* rotate back to the global frame */
+			Fg[M0 + d] = (double)F[M0];
+			Fg[M0 + (d + 1) % 3] = (double)F[M1];
+			Fg[M0 + (d + 2) % 3] = (double)F[M2];
+			Fg[B0 + d] = (double)F[B0];
+			Fg[B0 + (d + 1) % 3] = (double)F[B1];
+			Fg[B0 + (d + 2) % 3] = (double)F[B2];
+		}
+		const double dt = *dtp;
+		double* Pd = P + (size_t)d * 8 * n;
+		for (int q = 0; q < 8; q++) Pd[(size_t)q * n + c] = dt * Fg[q];
+		sp = (double)sT;
+		zero = sp <= 0 ? 1u : 0u;
+	}
+	reduce_speed(st, d, sp, zero);
+}
+
+/* constant-thermal-pressure energy refit of update_vol_B,
+   source/mhd/solve.hpp:1646-1672 */
+__device__ __forceinline__ void refit_energy(double rho, const double m[3], double& E,
+	const double vb_old[3], const double vb_new[3], double gamma, double mu0)
+{
+	if (rho <= 0) return;
+	const double U[8] = {rho, m[0], m[1], m[2], E, vb_old[0], vb_old[1], vb_old[2]};
+	const double p = pressure64(U, gamma, mu0);
+	const double inv = 1 / rho;
+	const double v[3] = {m[0] * inv, m[1] * inv, m[2] * inv};
+	const double
+		kinetic = 0.5 * rho * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]),
+		magnetic = 0.5 / mu0 * (vb_new[0] * vb_new[0] + vb_new[1] * vb_new[1] + vb_new[2] * vb_new[2]);
+	E = p / (gamma - 1) + kinetic + magnetic;
+}
+
+/*
+Update pass: update_mhd_state + update_vol_B of source/mhd/solve.hpp:1503-1683.
+Face dB of each face sums the 12 constrained-transport edge terms of
+assign_face_dBs_f{x,y,z} (:610-613, :926-928, :1223-1225) in the fixed order
+of the oracle; the face field is recomputed for the cell's positive-side
+faces (owned by the +x, +y, +z neighbours) for the energy refit.
+*/
+__global__ __launch_bounds__(BX * BY) void update_kernel(GridDev g,
+	const double* __restrict__ s, double* __restrict__ out,
+	const double* __restrict__ P, int refits)
+{
+	const int i = blockIdx.x * BX + threadIdx.x;
+	const int j = blockIdx.y * BY + threadIdx.y;
+	const int k = blockIdx.z;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	const int xs[3] = {wrap_dec(i, g.nx), i, wrap_inc(i, g.nx)};
+	const int ys[3] = {wrap_dec(j, g.ny), j, wrap_inc(j, g.ny)};
+	const int zs[3] = {wrap_dec(k, g.nz), k, wrap_inc(k, g.nz)};
+	auto idx = [&](int a, int b, int c) -> long long {
+		return xs[a] + (long long)g.nx * (ys[b] + (long long)g.ny * zs[c]);
+	};
+	auto pf = [&](int d, int q, int a, int b, int c) -> double {
+		return P[((size_t)d * 8 + q) * n + idx(a, b, c)];
+	};
+	const double dx = g.dx, dy = g.dy, dz = g.dz;
+	auto EZX = [&](int a, int b, int c) { return +(pf(0, 6, a, b, c) * dz / 4); };
+	auto EYX = [&](int a, int b, int c) { return -(pf(0, 7, a, b, c) * dy / 4); };
+	auto EXY = [&](int a, int b, int c) { return +(pf(1, 7, a, b, c) * dx / 4); };
+	auto EZY = [&](int a, int b, int c) { return -(pf(1, 5, a, b, c) * dz / 4); };
+	auto EXZ = [&](int a, int b, int c) { return -(pf(2, 6, a, b, c) * dx / 4); };
+	auto EYZ = [&](int a, int b, int c) { return +(pf(2, 5, a, b, c) * dy / 4); };
+
+	auto dBx_at = [&](int a, int b, int c) {
+		double r = 0;
+		r += EZX(a, b + 1, c);
+		r -= EZX(a, b - 1, c);
+		r -= EYX(a, b, c + 1);
+		r += EYX(a, b, c - 1);
+		r += EZY(a, b + 1, c);
+		r += EZY(a - 1, b + 1, c);
+		r -= EZY(a, b, c);
+		r -= EZY(a - 1, b, c);
+		r -= EYZ(a, b, c + 1);
+		r -= EYZ(a - 1, b, c + 1);
+		r += EYZ(a, b, c);
+		r += EYZ(a - 1, b, c);
+		return r;
+	};
+	auto dBy_at = [&](int a, int b, int c) {
+		double r = 0;
+		r += EXY(a, b, c + 1);
+		r -= EXY(a, b, c - 1);
+		r -= EZY(a + 1, b, c);
+		r += EZY(a - 1, b, c);
+		r += EXZ(a, b, c + 1);
+		r += EXZ(a, b - 1, c + 1);
+		r -= EXZ(a, b, c);
+		r -= EXZ(a, b - 1, c);
+		r -= EZX(a + 1, b, c);
+		r += EZX(a, b, c);
+		r -= EZX(a + 1, b - 1, c);
+		r += EZX(a, b - 1, c);
+		return r;
+	};
+	auto dBz_at = [&](int a, int b, int c) {
+		double r = 0;
+		r += EYZ(a + 1, b, c);
+		r -= EYZ(a - 1, b, c);
+		r -= EXZ(a, b + 1, c);
+		r += EXZ(a, b - 1, c);
+		r += EYX(a + 1, b, c);
+		r += EYX(a + 1, b, c - 1);
+		r -= EYX(a, b, c);
+		r -= EYX(a, b, c - 1);
+		r -= EXY(a, b + 1, c);
+		r -= EXY(a, b + 1, c - 1);
+		r += EXY(a, b, c);
+		r += EXY(a, b, c - 1);
+		return r;
+	};
+
+	const long long c0 = idx(1, 1, 1), cxp = idx(2, 1, 1), cyp = idx(1, 2, 1), czp = idx(1, 1, 2);
+	const double axy = dx * dy, axz = dx * dz, ayz = dy * dz;
+	const double bx = s[5 * n + c0] + dBx_at(1, 1, 1) / ayz;
+	const double by = s[6 * n + c0] + dBy_at(1, 1, 1) / axz;
+	const double bz = s[7 * n + c0] + dBz_at(1, 1, 1) / axy;
+	const double bxp = s[5 * n + cxp] + dBx_at(2, 1, 1) / ayz;
+	const double byp = s[6 * n + cyp] + dBy_at(1, 2, 1) / axz;
+	const double bzp = s[7 * n + czp] + dBz_at(1, 1, 2) / axy;
+	out[5 * n + c0] = bx;
+	out[6 * n + c0] = by;
+	out[7 * n + c0] = bz;
+
+	const double len[3] = {dx, dy, dz};
+	const long long cp[3] = {cxp, cyp, czp};
+	double rho = s[c0];
+	double m[3] = {s[n + c0], s[2 * n + c0], s[3 * n + c0]};
+	double E = s[4 * n + c0];
+	double vb[3] = {
+		0.5 * (s[5 * n + c0] + s[5 * n + cxp]),
+		0.5 * (s[6 * n + c0] + s[6 * n + cyp]),
+		0.5 * (s[7 * n + c0] + s[7 * n + czp])};
+	for (int d = 0; d < 3; d++) {
+		const double* Pd = P + (size_t)d * 8 * n;
+		rho += ((0.0 + Pd[c0]) - (0.0 + Pd[cp[d]])) / len[d];
+	}
+	for (int d = 0; d < 3; d++) {
+		const double* Pd = P + (size_t)d * 8 * n;
+		const double il = 1 / len[d];
+		for (int q = 0; q < 3; q++) {
+			m[q] = m[q] + ((0.0 + Pd[(size_t)(M0 + q) * n + c0]) - (0.0 + Pd[(size_t)(M0 + q) * n + cp[d]])) * il;
+		}
+	}
+	for (int d = 0; d < 3; d++) {
+		const double* Pd = P + (size_t)d * 8 * n;
+		E += ((0.0 + Pd[(size_t)NRJ * n + c0]) - (0.0 + Pd[(size_t)NRJ * n + cp[d]])) / len[d];
+	}
+	for (int d = 0; d < 3; d++) {
+		const double* Pd = P + (size_t)d * 8 * n;
+		const double il = 1 / len[d];
+		for (int q = 0; q < 3; q++) {
+			vb[q] = vb[q] + ((0.0 + Pd[(size_t)(B0 + q) * n + c0]) - (0.0 + Pd[(size_t)(B0 + q) * n + cp[d]])) * il;
+		}
+	}
+	const double vb_new[3] = {0.5 * (bx + bxp), 0.5 * (by + byp), 0.5 * (bz + bzp)};
+	for (int r = 0; r < refits; r++) {
+		refit_energy(rho, m, E, r == 0 ? vb : vb_new, vb_new, g.gamma, g.mu0);
+	}
+	out[c0] = rho;
+	out[n + c0] = m[0];
+	out[2 * n + c0] = m[1];
+	out[3 * n + c0] = m[2];
+	out[4 * n + c0] = E;
+}
+
+/* lagged CFL step of minimize_timestep + set_minmax_substepping_period with a
+   single substep (source/mhd/solve.hpp:1961-2006, substepping.hpp:303-338) */
+__global__ void dt_kernel(const StepStatus* __restrict__ prev, StepStatus* __restrict__ next,
+	double* __restrict__ dt_out, double cfl, double max_dt, double dx, double dy, double dz)
+{
+	const double len[3] = {dx, dy, dz};
+	double cfl_dt = DBL_MAX;
+	for (int d = 0; d < 3; d++) {
+		if (prev->zero[d] > 0) {
+			cfl_dt = 0;
+			break;
+		}
+		cfl_dt = smin(cfl_dt, cfl * len[d] / __longlong_as_double((long long)prev->smax_bits[d]));
+	}
+	*dt_out = (max_dt <= cfl_dt) ? max_dt : cfl_dt;
+	for (int d = 0; d < 3; d++) {
+		next->smax_bits[d] = 0;
+		next->zero[d] = 0;
+	}
+	next->err = 0;
+	next->err_face = -1;
+}
+
+/* energy refit of apply_boundaries for a periodic grid (update_vol_B with
+   constant pressure over all cells, source/mhd/boundaries.hpp:1289-1292) */
+__global__ __launch_bounds__(BX * BY) void refit_kernel(GridDev g, double* __restrict__ s)
+{
+	const int i = blockIdx.x * BX + threadIdx.x;
+	const int j = blockIdx.y * BY + threadIdx.y;
+	const int k = blockIdx.z;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	const long long c = i + (long long)g.nx * (j + (long long)g.ny * k);
+	const long long cx = wrap_inc(i, g.nx) + (long long)g.nx * (j + (long long)g.ny * k);
+	const long long cy = i + (long long)g.nx * (wrap_inc(j, g.ny) + (long long)g.ny * k);
+	const long long cz = i + (long long)g.nx * (j + (long long)g.ny * wrap_inc(k, g.nz));
+	const double vb[3] = {
+		0.5 * (s[5 * n + c] + s[5 * n + cx]),
+		0.5 * (s[6 * n + c] + s[6 * n + cy]),
+		0.5 * (s[7 * n + c] + s[7 * n + cz])};
+	const double m[3] = {s[n + c], s[2 * n + c], s[3 * n + c]};
+	double E = s[4 * n + c];
+	refit_energy(s[c], m, E, vb, vb, g.gamma, g.mu0);
+	s[4 * n + c] = E;
+}
+
+/* sum of |div B| per block, source/math/nabla.hpp:64-88 */
+__global__ __launch_bounds__(BX * BY) void div_kernel(GridDev g, const double* __restrict__ s,
+	double* __restrict__ partial)
+{
+	const int i = blockIdx.x * BX + threadIdx.x;
+	const int j = blockIdx.y * BY + threadIdx.y;
+	const int k = blockIdx.z;
+	double v = 0;
+	if (i < g.nx && j < g.ny) {
+		const long long n = g.n;
+		const long long c = i + (long long)g.nx * (j + (long long)g.ny * k);
+		const long long cx = wrap_inc(i, g.nx) + (long long)g.nx * (j + (long long)g.ny * k);
+		const long long cy = i + (long long)g.nx * (wrap_inc(j, g.ny) + (long long)g.ny * k);
+		const long long cz = i + (long long)g.nx * (j + (long long)g.ny * wrap_inc(k, g.nz));
+		v = fabs((s[5 * n + cx] - s[5 * n + c]) / g.dx
+			+ (s[6 * n + cy] - s[6 * n + c]) / g.dy
+			+ (s[7 * n + cz] - s[7 * n + c]) / g.dz);
+	}
+	for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+	__shared__ double w[BX * BY / 64];
+	const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+	if (tid % 64 == 0) w[tid / 64] = v;
+	__syncthreads();
+	if (tid == 0) {
+		double t = 0;
+		for (int q = 0; q < BX * BY / 64; q++) t += w[q];
+		partial[blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * (size_t)blockIdx.z)] = t;
+	}
+}
+
+/* batched Riemann solver for the pamhd_face_flux_batch unit-test surface */
+template <int SOLVER, class T>
+__global__ __launch_bounds__(256) void face_batch_kernel(size_t n, const double* __restrict__ Lp,
+	const double* __restrict__ Rp, const double* __restrict__ bg, double gamma, double mu0,
+	double* __restrict__ Fp, double* __restrict__ sp, int* __restrict__ status)
+{
+	const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+	if (f >= n) return;
+	double L[8], R[8];
+	for (int q = 0; q < 8; q++) {
+		L[q] = Lp[q * n + f];
+		R[q] = Rp[q * n + f];
+	}
+	T b[3] = {0, 0, 0};
+	if (bg) for (int q = 0; q < 3; q++) b[q] = (T)bg[q * n + f];
+	T F[8], s = 0;
+	const int e = riemann<SOLVER, T>(L, R, b, gamma, mu0, F, s);
+	for (int q = 0; q < 8; q++) Fp[q * n + f] = e ? 0.0 : (double)F[q];
+	sp[f] = e ? 0.0 : (double)s;
+	if (status) status[f] = e;
+}
+
+/* ---------------------------------------------------------------------- */
+/* host side                                                              */
+/* ---------------------------------------------------------------------- */
+
+std::mutex g_err_mutex;
+std::string g_err;
+
+void set_global_error(const std::string& m)
+{
+	std::lock_guard<std::mutex> lock(g_err_mutex);
+	g_err = m;
+}
+
+const char* error_name(int e)
+{
+	switch (e) {
+	case PAMHD_E_DENSITY: return "non-positive or invalid mass density";
+	case PAMHD_E_PRESSURE: return "non-positive or invalid thermal pressure";
+	case PAMHD_E_SPEED: return "invalid signal speed";
+	case PAMHD_E_FLUX: return "non-finite momentum or flux";
+	case PAMHD_E_FIELD: return "non-finite total magnetic field";
+	case PAMHD_E_ROE_DENSITY: return "non-physical density in Roe intermediate state";
+	case PAMHD_E_ROE_PRESSURE: return "non-physical pressure in Roe intermediate state";
+	case PAMHD_E_SOLVER: return "invalid solver";
+	default: return "error";
+	}
+}
+
+} // namespace
+
+struct pamhd_handle {
+	GridDev g;
+	pamhd_grid_desc desc;
+	pamhd_physics phys;
+	int precision = PAMHD_FP64;
+	double* state[2] = {nullptr, nullptr};
+	double* flux = nullptr;
+	double* bg = nullptr;
+	double* dt = nullptr;
+	double* scratch = nullptr;
+	size_t scratch_len = 0;
+	StepStatus* status = nullptr; /* 2 entries */
+	double* host_dt = nullptr;    /* pinned */
+	StepStatus* host_status = nullptr;
+	int cur = 0;    /* state buffer holding the current state */
+	int last = 0;   /* status entry of the last flux pass */
+	std::string err;
+};
+
+namespace {
+
+int fail(pamhd_handle* h, int code, const std::string& m)
+{
+	if (h) h->err = m;
+	else set_global_error(m);
+	return code;
+}
+
+int hip_fail(pamhd_handle* h, hipError_t e, const char* what)
+{
+	return fail(h, PAMHD_E_DEVICE, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+#define HIP_TRY(h, expr) do { hipError_t e_ = (expr); if (e_ != hipSuccess) return hip_fail((h), e_, #expr); } while (0)
+
+dim3 cell_grid(const GridDev& g, int slabs = 1)
+{
+	return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, g.nz * slabs);
+}
+
+template <class T>
+int launch_flux_pass(pamhd_handle* h, int solver, hipStream_t st, StepStatus* stat)
+{
+	const double* s = h->state[h->cur];
+	const dim3 grid = cell_grid(h->g, 3), block(BX, BY, 1);
+	switch (solver) {
+	case RUSANOV:
+		flux_pass_kernel<RUSANOV, T><<<grid, block, 0, st>>>(h->g, s, h->bg, h->flux, h->dt, stat);
+		break;
+	case HLL_ATHENA:
+		flux_pass_kernel<HLL_ATHENA, T><<<grid, block, 0, st>>>(h->g, s, h->bg, h->flux, h->dt, stat);
+		break;
+	case HLLD_ATHENA:
+		flux_pass_kernel<HLLD_ATHENA, T><<<grid, block, 0, st>>>(h->g, s, h->bg, h->flux, h->dt, stat);
+		break;
+	case ROE_ATHENA:
+		flux_pass_kernel<ROE_ATHENA, T><<<grid, block, 0, st>>>(h->g, s, h->bg, h->flux, h->dt, stat);
+		break;
+	default:
+		return fail(h, PAMHD_E_SOLVER, "invalid solver " + std::to_string(solver));
+	}
+	HIP_TRY(h, hipGetLastError());
+	return PAMHD_OK;
+}
+
+int report_status(pamhd_handle* h, const StepStatus& st)
+{
+	if (st.err == 0) return PAMHD_OK;
+	const long long n = h->g.n;
+	const long long face = st.err_face;
+	const int d = (int)(face / n);
+	const long long c = face - (long long)d * n;
+	const int i = (int)(c % h->g.nx), j = (int)((c / h->g.nx) % h->g.ny), k = (int)(c / ((long long)h->g.nx * h->g.ny));
+	char buf[256];
+	snprintf(buf, sizeof(buf),
+		"solution failed between cells (%d,%d,%d) and its -%c neighbour in direction %d: %s",
+		i, j, k, "xyz"[d], d + 1, error_name(st.err));
+	return fail(h, st.err, buf);
+}
+
+int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
+	double* dt_taken, void* stream_)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (solver == HYBRID) return fail(h, PAMHD_E_UNSUPPORTED, "hybrid solver: MHD fluxes are not computed");
+	if (solver != RUSANOV && solver != HLL_ATHENA && solver != HLLD_ATHENA && solver != ROE_ATHENA) {
+		return fail(h, PAMHD_E_SOLVER, "invalid solver " + std::to_string(solver));
+	}
+	if (!(cfl >= 0)) return fail(h, PAMHD_E_ARG, "time step factor must be >= 0");
+	hipStream_t st = (hipStream_t)stream_;
+	StepStatus* prev = h->status + h->last;
+	StepStatus* next = h->status + (1 - h->last);
+	dt_kernel<<<1, 1, 0, st>>>(prev, next, h->dt, cfl, max_dt, h->g.dx, h->g.dy, h->g.dz);
+	HIP_TRY(h, hipGetLastError());
+	const int e = h->precision == PAMHD_MIXED
+		? launch_flux_pass<float>(h, solver, st, next)
+		: launch_flux_pass<double>(h, solver, st, next);
+	if (e) return e;
+	update_kernel<<<cell_grid(h->g), dim3(BX, BY, 1), 0, st>>>(
+		h->g, h->state[h->cur], h->state[1 - h->cur], h->flux, refits);
+	HIP_TRY(h, hipGetLastError());
+	h->cur = 1 - h->cur;
+	h->last = 1 - h->last;
+	if (dt_taken) {
+		HIP_TRY(h, hipMemcpyAsync(h->host_dt, h->dt, sizeof(double), hipMemcpyDeviceToHost, st));
+		HIP_TRY(h, hipMemcpyAsync(h->host_status, next, sizeof(StepStatus), hipMemcpyDeviceToHost, st));
+		HIP_TRY(h, hipStreamSynchronize(st));
+		*dt_taken = *h->host_dt;
+		return report_status(h, *h->host_status);
+	}
+	return PAMHD_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+int pamhd_abi_version(void) { return 1; }
+
+int pamhd_create(const pamhd_grid_desc* grid, const pamhd_physics* physics, int precision,
+	pamhd_handle** out)
+{
+	if (!grid || !physics || !out) return fail(nullptr, PAMHD_E_ARG, "null argument");
+	*out = nullptr;
+	for (int d = 0; d < 3; d++) {
+		if (grid->cells[d] < 1) return fail(nullptr, PAMHD_E_ARG, "cells must be >= 1");
+		if (!(grid->cell_length[d] > 0)) return fail(nullptr, PAMHD_E_ARG, "cell length must be > 0");
+		if (!grid->periodic[d]) {
+			return fail(nullptr, PAMHD_E_UNSUPPORTED, "non-periodic grids need boundary cells; not supported");
+		}
+	}
+	/* get_total_energy_density's checks, source/mhd/common.hpp:275-289 */
+	if (!(physics->adiabatic_index > 1)) return fail(nullptr, PAMHD_E_ARG, "adiabatic index must be > 1");
+	if (!(physics->vacuum_permeability > 0)) return fail(nullptr, PAMHD_E_ARG, "vacuum permeability must be > 0");
+	if (precision != PAMHD_FP64 && precision != PAMHD_MIXED) return fail(nullptr, PAMHD_E_ARG, "invalid precision");
+	const long long n = (long long)grid->cells[0] * grid->cells[1] * grid->cells[2];
+	if ((long long)grid->cells[2] * 3 > 65535) return fail(nullptr, PAMHD_E_UNSUPPORTED, "nz too large");
+
+	pamhd_handle* h = new pamhd_handle();
+	h->desc = *grid;
+	h->phys = *physics;
+	h->precision = precision;
+	h->g = GridDev{grid->cells[0], grid->cells[1], grid->cells[2], n,
+		grid->cell_length[0], grid->cell_length[1], grid->cell_length[2],
+		physics->adiabatic_index, physics->vacuum_permeability};
+	auto cleanup = [&](int code, const char* what, hipError_t e) {
+		std::string m = std::string(what) + ": " + hipGetErrorString(e);
+		pamhd_destroy(h);
+		return fail(nullptr, code, m);
+	};
+	hipError_t e;
+	if ((e = hipMalloc(&h->state[0], sizeof(double) * 8 * n)) != hipSuccess) return cleanup(PAMHD_E_DEVICE, "state", e);
+	if ((e = hipMalloc(&h->state[1], sizeof(double) * 8 * n)) != hipSuccess) return cleanup(PAMHD_E_DEVICE, "state", e);
+	if ((e = hipMalloc(&h->flux, sizeof(double) * 24 * n)) != hipSuccess) return cleanup(PAMHD_E_DEVICE, "flux", e);
+	if ((e = hipMalloc(&h->dt, sizeof(double))) != hipSuccess) return cleanup(PAMHD_E_DEVICE, "dt", e);
+	if ((e = hipMalloc(&h->status, 2 * sizeof(StepStatus))) != hipSuccess) return cleanup(PAMHD_E_DEVICE, "status", e);
+	if ((e = hipHostMalloc(&h->host_dt, sizeof(double))) != hipSuccess) return cleanup(PAMHD_E_DEVICE, "pinned", e);
+	if ((e = hipHostMalloc(&h->host_status, sizeof(StepStatus))) != hipSuccess) return cleanup(PAMHD_E_DEVICE, "pinned", e);
+	/* before the first flux pass every face's Max_v is -1
+	   (tests/mhd/test.cpp:383), so the first lagged step is 0 */
+	StepStatus init[2];
+	for (auto& s : init) {
+		for (int d = 0; d < 3; d++) {
+			s.smax_bits[d] = 0;
+			s.zero[d] = 1;
+		}
+		s.err = 0;
+		s.err_face = -1;
+	}
+	if ((e = hipMemcpy(h->status, init, sizeof(init), hipMemcpyHostToDevice)) != hipSuccess) return cleanup(PAMHD_E_DEVICE, "status", e);
+	if ((e = hipMemset(h->state[0], 0, sizeof(double) * 8 * n)) != hipSuccess) return cleanup(PAMHD_E_DEVICE, "state", e);
+	*out = h;
+	return PAMHD_OK;
+}
+
+int pamhd_destroy(pamhd_handle* h)
+{
+	if (!h) return PAMHD_OK;
+	hipFree(h->state[0]);
+	hipFree(h->state[1]);
+	hipFree(h->flux);
+	hipFree(h->bg);
+	hipFree(h->dt);
+	hipFree(h->status);
+	hipFree(h->scratch);
+	hipHostFree(h->host_dt);
+	hipHostFree(h->host_status);
+	delete h;
+	return PAMHD_OK;
+}
+
+int pamhd_upload_state(pamhd_handle* h, const double* state)
+{
+	if (!h || !state) return fail(h, PAMHD_E_ARG, "null argument");
+	HIP_TRY(h, hipMemcpy(h->state[h->cur], state, sizeof(double) * 8 * h->g.n, hipMemcpyHostToDevice));
+	return PAMHD_OK;
+}
+
+int pamhd_download_state(pamhd_handle* h, double* state)
+{
+	if (!h || !state) return fail(h, PAMHD_E_ARG, "null argument");
+	HIP_TRY(h, hipMemcpy(state, h->state[h->cur], sizeof(double) * 8 * h->g.n, hipMemcpyDeviceToHost));
+	return PAMHD_OK;
+}
+
+int pamhd_state_device_ptr(pamhd_handle* h, double** ptr)
+{
+	if (!h || !ptr) return fail(h, PAMHD_E_ARG, "null argument");
+	*ptr = h->state[h->cur];
+	return PAMHD_OK;
+}
+
+int pamhd_set_background_faces(pamhd_handle* h, const double* bg)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (!bg) {
+		hipFree(h->bg);
+		h->bg = nullptr;
+		return PAMHD_OK;
+	}
+	if (!h->bg) HIP_TRY(h, hipMalloc(&h->bg, sizeof(double) * 9 * h->g.n));
+	HIP_TRY(h, hipMemcpy(h->bg, bg, sizeof(double) * 9 * h->g.n, hipMemcpyHostToDevice));
+	return PAMHD_OK;
+}
+
+int pamhd_timestep(pamhd_handle* h, int solver, double max_time_step, double time_step_factor,
+	double* dt_taken, void* stream)
+{
+	return do_step(h, solver, max_time_step, time_step_factor, 2, dt_taken, stream);
+}
+
+int pamhd_step(pamhd_handle* h, int solver, double max_time_step, double time_step_factor,
+	double* dt_taken, void* stream)
+{
+	return do_step(h, solver, max_time_step, time_step_factor, 3, dt_taken, stream);
+}
+
+int pamhd_apply_boundaries(pamhd_handle* h, double /*simulation_time*/, void* stream)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	refit_kernel<<<cell_grid(h->g), dim3(BX, BY, 1), 0, (hipStream_t)stream>>>(h->g, h->state[h->cur]);
+	HIP_TRY(h, hipGetLastError());
+	return PAMHD_OK;
+}
+
+int pamhd_check(pamhd_handle* h)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	HIP_TRY(h, hipDeviceSynchronize());
+	HIP_TRY(h, hipMemcpy(h->host_status, h->status + h->last, sizeof(StepStatus), hipMemcpyDeviceToHost));
+	return report_status(h, *h->host_status);
+}
+
+int pamhd_max_signal_speeds(pamhd_handle* h, double smax_out[3])
+{
+	if (!h || !smax_out) return fail(h, PAMHD_E_ARG, "null argument");
+	HIP_TRY(h, hipDeviceSynchronize());
+	StepStatus st;
+	HIP_TRY(h, hipMemcpy(&st, h->status + h->last, sizeof(StepStatus), hipMemcpyDeviceToHost));
+	for (int d = 0; d < 3; d++) {
+		double v;
+		memcpy(&v, &st.smax_bits[d], sizeof(v));
+		smax_out[d] = v;
+	}
+	return PAMHD_OK;
+}
+
+int pamhd_div_face2volume(pamhd_handle* h, double* avg)
+{
+	if (!h || !avg) return fail(h, PAMHD_E_ARG, "null argument");
+	const dim3 grid = cell_grid(h->g);
+	const size_t nb = (size_t)grid.x * grid.y * grid.z;
+	if (h->scratch_len < nb) {
+		hipFree(h->scratch);
+		h->scratch = nullptr;
+		HIP_TRY(h, hipMalloc(&h->scratch, sizeof(double) * nb));
+		h->scratch_len = nb;
+	}
+	div_kernel<<<grid, dim3(BX, BY, 1)>>>(h->g, h->state[h->cur], h->scratch);
+	HIP_TRY(h, hipGetLastError());
+	std::string tmp;
+	double* host = new double[nb];
+	const hipError_t e = hipMemcpy(host, h->scratch, sizeof(double) * nb, hipMemcpyDeviceToHost);
+	if (e != hipSuccess) {
+		delete[] host;
+		return hip_fail(h, e, "div");
+	}
+	double total = 0;
+	for (size_t b = 0; b < nb; b++) total += host[b];
+	delete[] host;
+	*avg = total / (double)h->g.n;
+	return PAMHD_OK;
+}
+
+int pamhd_face_flux_batch(int solver, int precision, size_t n, const double* L, const double* R,
+	const double* bg, double gamma, double mu0, double* F, double* smax_out, int* status,
+	void* stream)
+{
+	if (n == 0) return PAMHD_OK;
+	if (!L || !R || !F || !smax_out) return fail(nullptr, PAMHD_E_ARG, "null argument");
+	if (precision != PAMHD_FP64 && precision != PAMHD_MIXED) return fail(nullptr, PAMHD_E_ARG, "invalid precision");
+	hipStream_t st = (hipStream_t)stream;
+	const dim3 grid((unsigned)((n + 255) / 256)), block(256);
+#define BATCH(S, T) face_batch_kernel<S, T><<<grid, block, 0, st>>>(n, L, R, bg, gamma, mu0, F, smax_out, status)
+	if (precision == PAMHD_FP64) {
+		switch (solver) {
+		case RUSANOV: BATCH(RUSANOV, double); break;
+		case HLL_ATHENA: BATCH(HLL_ATHENA, double); break;
+		case HLLD_ATHENA: BATCH(HLLD_ATHENA, double); break;
+		case ROE_ATHENA: BATCH(ROE_ATHENA, double); break;
+		default: return fail(nullptr, PAMHD_E_SOLVER, "invalid solver");
+		}
+	} else {
+		switch (solver) {
+		case RUSANOV: BATCH(RUSANOV, float); break;
+		case HLL_ATHENA: BATCH(HLL_ATHENA, float); break;
+		case HLLD_ATHENA: BATCH(HLLD_ATHENA, float); break;
+		case ROE_ATHENA: BATCH(ROE_ATHENA, float); break;
+		default: return fail(nullptr, PAMHD_E_SOLVER, "invalid solver");
+		}
+	}
+#undef BATCH
+	const hipError_t e = hipGetLastError();
+	if (e != hipSuccess) return hip_fail(nullptr, e, "face_batch_kernel");
+	return PAMHD_OK;
+}
+
+int pamhd_last_error(pamhd_handle* h, char* buf, size_t n)
+{
+	if (!buf || n == 0) return PAMHD_E_ARG;
+	std::string m;
+	if (h) {
+		m = h->err;
+	} else {
+		std::lock_guard<std::mutex> lock(g_err_mutex);
+		m = g_err;
+	}
+	snprintf(buf, n, "%s", m.c_str());
+	return PAMHD_OK;
+}
+
+} // extern "C"

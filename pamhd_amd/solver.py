@@ -1,0 +1,182 @@
+"""Host-side mirror of the reference's MHD solver interface over the C ABI.
+
+The reference drives its step through
+    pamhd::mhd::timestep(solver, grid, options, t, max_dt, cfl, gamma, mu0, ...)
+        -> (dt, max_substep, flux_calcs)          source/mhd/solve.hpp:1837-1924
+    pamhd::mhd::apply_boundaries(...)            source/mhd/boundaries.hpp:1249
+    pamhd::math::get_divergence_face2volume(...) source/math/nabla.hpp:58-113
+selecting the Riemann solver by the "solver-mhd" string of
+source/mhd/options.hpp:97-117 / tests/mhd/test.cpp:294-316.  Grid here
+wraps one device-resident periodic grid (include/pamhd_mi355x.h).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+# "solver-mhd" strings accepted by tests/mhd/test.cpp:294-316
+SOLVER_NAMES = {
+    "rusanov": _lib.RUSANOV,
+    "hll-athena": _lib.HLL_ATHENA,
+    "hlld-athena": _lib.HLLD_ATHENA,
+    "roe-athena": _lib.ROE_ATHENA,
+}
+
+
+def solver_id(solver):
+    if isinstance(solver, str):
+        try:
+            return SOLVER_NAMES[solver]
+        except KeyError:
+            raise ValueError("Invalid solver: %s, use rusanov, hll-athena, "
+                             "hlld-athena or roe-athena" % solver) from None
+    return int(solver)
+
+
+class Grid:
+    """A periodic nx*ny*nz grid of MHD cells resident on the GPU.
+
+    State layout (host side): float64 array of shape (8, n), n = nx*ny*nz,
+    rows rho, m_x, m_y, m_z, E, B_x(i-1/2), B_y(j-1/2), B_z(k-1/2), cell
+    index i + nx*(j + ny*k).
+    """
+
+    def __init__(self, cells, cell_length, adiabatic_index=5.0 / 3.0,
+                 vacuum_permeability=1.0, proton_mass=1.0, precision="fp64",
+                 start=(0.0, 0.0, 0.0)):
+        L = _lib.load()
+        self.cells = tuple(int(c) for c in cells)
+        self.n = self.cells[0] * self.cells[1] * self.cells[2]
+        self.cell_length = tuple(float(x) for x in cell_length)
+        self.gamma = float(adiabatic_index)
+        self.mu0 = float(vacuum_permeability)
+        self.precision = {"fp64": _lib.FP64, "mixed": _lib.MIXED}[precision]
+        desc = _lib.GridDesc((ctypes.c_int32 * 3)(*self.cells),
+                             (ctypes.c_double * 3)(*start),
+                             (ctypes.c_double * 3)(*self.cell_length),
+                             (ctypes.c_int32 * 3)(1, 1, 1))
+        phys = _lib.Physics(self.gamma, self.mu0, float(proton_mass))
+        h = ctypes.c_void_p()
+        _lib.check(L.pamhd_create(ctypes.byref(desc), ctypes.byref(phys),
+                                  self.precision, ctypes.byref(h)))
+        self._h = h
+        self._L = L
+        self.time = 0.0
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.pamhd_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _check(self, code):
+        return _lib.check(code, self._h)
+
+    # --- state ----------------------------------------------------------
+    def upload(self, state):
+        state = np.ascontiguousarray(state, dtype=np.float64)
+        if state.shape != (8, self.n):
+            raise ValueError("state must have shape (8, %d)" % self.n)
+        self._check(self._L.pamhd_upload_state(self._h, state.ctypes.data))
+
+    def download(self):
+        out = np.empty((8, self.n))
+        self._check(self._L.pamhd_download_state(self._h, out.ctypes.data))
+        return out
+
+    def device_ptr(self):
+        p = ctypes.c_void_p()
+        self._check(self._L.pamhd_state_device_ptr(self._h, ctypes.byref(p)))
+        return p.value
+
+    def set_background_faces(self, bg):
+        if bg is None:
+            self._check(self._L.pamhd_set_background_faces(self._h, None))
+            return
+        bg = np.ascontiguousarray(bg, dtype=np.float64)
+        if bg.shape != (9, self.n):
+            raise ValueError("background field must have shape (9, %d)" % self.n)
+        self._check(self._L.pamhd_set_background_faces(self._h, bg.ctypes.data))
+
+    # --- the step ---------------------------------------------------------
+    def timestep(self, solver, max_time_step, time_step_factor, sync=True, stream=None):
+        """pamhd::mhd::timestep: returns the step length (None when sync=False)."""
+        dt = ctypes.c_double(0)
+        self._check(self._L.pamhd_timestep(self._h, solver_id(solver), max_time_step,
+                                           time_step_factor,
+                                           ctypes.byref(dt) if sync else None, stream))
+        return dt.value if sync else None
+
+    def apply_boundaries(self, simulation_time=0.0, stream=None):
+        self._check(self._L.pamhd_apply_boundaries(self._h, simulation_time, stream))
+
+    def step(self, solver, max_time_step, time_step_factor, sync=True, stream=None):
+        """timestep + apply_boundaries of one main-loop iteration."""
+        dt = ctypes.c_double(0)
+        self._check(self._L.pamhd_step(self._h, solver_id(solver), max_time_step,
+                                       time_step_factor,
+                                       ctypes.byref(dt) if sync else None, stream))
+        return dt.value if sync else None
+
+    def run(self, solver, time_step_factor, t_start, t_end, max_steps=1 << 62, stream=None):
+        """The program loop of tests/mhd/test.cpp; returns (steps, t_final)."""
+        steps = ctypes.c_long(0)
+        tf = ctypes.c_double(0)
+        self._check(self._L.pamhd_run(self._h, solver_id(solver), time_step_factor,
+                                      t_start, t_end, max_steps, ctypes.byref(steps),
+                                      ctypes.byref(tf), stream))
+        self.time = tf.value
+        return steps.value, tf.value
+
+    def check(self):
+        self._check(self._L.pamhd_check(self._h))
+
+    def div_face2volume(self):
+        v = ctypes.c_double(0)
+        self._check(self._L.pamhd_div_face2volume(self._h, ctypes.byref(v)))
+        return v.value
+
+    def max_signal_speeds(self):
+        v = (ctypes.c_double * 3)()
+        self._check(self._L.pamhd_max_signal_speeds(self._h, v))
+        return tuple(v)
+
+
+def face_flux_batch(solver, L, R, bg=None, adiabatic_index=5.0 / 3.0,
+                    vacuum_permeability=1.0, precision="fp64"):
+    """Batched Riemann solver on torch CUDA tensors (8, n) in the face frame.
+
+    Returns (F (8, n), smax (n,), status (n,)) as torch tensors on the same
+    device.
+    """
+    import torch
+    Lib = _lib.load()
+    L = L.contiguous().to(torch.float64)
+    R = R.contiguous().to(torch.float64)
+    n = L.shape[1]
+    F = torch.empty_like(L)
+    s = torch.empty(n, dtype=torch.float64, device=L.device)
+    st = torch.empty(n, dtype=torch.int32, device=L.device)
+    bgp = None
+    if bg is not None:
+        bg = bg.contiguous().to(torch.float64)
+        bgp = bg.data_ptr()
+    prec = {"fp64": _lib.FP64, "mixed": _lib.MIXED}[precision]
+    stream = torch.cuda.current_stream(L.device).cuda_stream
+    _lib.check(Lib.pamhd_face_flux_batch(solver_id(solver), prec, n, L.data_ptr(),
+                                         R.data_ptr(), bgp, adiabatic_index,
+                                         vacuum_permeability, F.data_ptr(),
+                                         s.data_ptr(), st.data_ptr(), stream))
+    return F, s, st
