@@ -1,0 +1,197 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X ideal-MHD step: cell-updates/s of the 256^3
+periodic smooth-vortex workload (gamma 5/3, +-1% seeded density noise) with
+the HLLD solver on one GPU, FP64 and mixed precision, with the fraction of
+the HBM roofline and the oracle's CPU rate on the host beside it.
+
+    python bench.py [--gpus 1] [--steps 100] [--warmup 5] [--size 256]
+                    [--solver hlld-athena] [--cfl 0.3]
+
+Prints one JSON line on stdout; progress goes to stderr.  A step is one
+main-loop iteration (timestep + apply_boundaries, pamhd_step) over the whole
+grid with the state resident in HBM.  The time-step factor defaults to 0.3:
+with 0.5 per dimension the unsplit first-order scheme's 3-D Courant sum
+reaches ~1.3 on this state and grid-scale noise grows.
+"""
+import argparse
+import json
+import math
+import os
+import platform
+import subprocess
+import sys
+import time
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, AMD specification
+BYTES_PER_CELL_UPDATE = 128    # 8 FP64 read + 8 FP64 written per cell-update
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        out = subprocess.run(["lscpu"], capture_output=True, text=True).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name"):
+                return line.split(":", 1)[1].strip()
+    except Exception:
+        pass
+    return platform.processor() or "unknown"
+
+
+def cpu_baseline(solver_name, cfl, seconds, threads):
+    """The FP64 oracle (a port of the reference's step) on a 64^3 sample of
+    the same workload, stepped until about `seconds` of wall time."""
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+    import oracle_lib as O
+    from pamhd_amd import configs
+    cells = (64, 64, 64)
+    state, length, gamma, mu0 = configs.orszag_tang_vortex(cells, noise=0.01, seed=2024)
+    sid = {"rusanov": O.RUSANOV, "hll-athena": O.HLL, "hlld-athena": O.HLLD,
+           "roe-athena": O.ROE}[solver_name]
+    used = O.lib().or_set_threads(threads)
+    g = O.make_grid(cells, length, gamma, mu0)
+    st, steps, _ = O.run_periodic(g, sid, cfl, 0.0, 1e30, state, max_steps=0)
+    # final init (not timed), then timed steps
+    st, new, sm, zc = O.step(g, sid, 0.0, state, refits=2)
+    state = new
+    n = cells[0] * cells[1] * cells[2]
+    done = 0
+    t0 = time.perf_counter()
+    while True:
+        dt = O.cfl_dt(g, cfl, sm, zc)
+        st, new, sm, zc = O.step(g, sid, dt, state, refits=3)
+        if st != 0:
+            raise RuntimeError("oracle step failed with %d" % st)
+        state = new
+        done += 1
+        el = time.perf_counter() - t0
+        if el >= seconds:
+            break
+    return {"value": n * done / el, "unit": "cell-updates/s", "cores": used,
+            "kind": "port",
+            "sample": "%d steps of the %dx%dx%d periodic vortex, %s, FP64 oracle "
+                      "(oracle/pamhd_oracle.c, -O2, no FMA), %d thread(s), %s"
+                      % (done, *cells, solver_name, used, cpu_model())}
+
+
+def run_gpu(prec, state, cells, length, gamma, mu0, solver, cfl, steps, warmup):
+    import torch
+    from pamhd_amd.solver import Grid
+    n = cells[0] * cells[1] * cells[2]
+    with Grid(cells, length, gamma, mu0, precision=prec) as g:
+        g.upload(state)
+        stream = torch.cuda.current_stream().cuda_stream
+        g.apply_boundaries(0.0, stream=stream)
+        g.timestep(solver, 0.0, cfl, sync=True, stream=stream)   # final init, dt = 0
+        for _ in range(warmup):
+            g.step(solver, 1e300, cfl, sync=False, stream=stream)
+        torch.cuda.synchronize()
+        g.check()
+        g.enable_timing(True)
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        start.record()
+        for k in range(steps):
+            g.step(solver, 1e300, cfl, sync=False, stream=stream)
+        stop.record()
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop)
+        g.check()
+        kt = {k: g.kernel_time(k) for k in ("dt", "flux_pass", "update")}
+        speeds = g.max_signal_speeds()
+    return {"ms": ms, "value": n * steps / (ms / 1e3), "kernels": kt, "speeds": speeds}
+
+
+def roofline(res, n):
+    """Step roofline: algorithmic 128 B per cell-update over the device time
+    of one step's kernels (flux pass + update), from HIP events."""
+    kt = res["kernels"]
+    flux_ms, flux_n = kt["flux_pass"]
+    upd_ms, upd_n = kt["update"]
+    step_ms = flux_ms / flux_n + upd_ms / upd_n
+    achieved = BYTES_PER_CELL_UPDATE * n / (step_ms / 1e3) / 1e9
+    return {"bound": "hbm", "kernel": "flux_pass+update",
+            "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel_ms": {"flux_pass": round(flux_ms / flux_n, 4),
+                          "update": round(upd_ms / upd_n, 4),
+                          "dt": round(kt["dt"][0] / max(kt["dt"][1], 1), 4)}}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--size", type=int, default=256)
+    ap.add_argument("--solver", default="hlld-athena")
+    ap.add_argument("--cfl", type=float, default=0.3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path "
+              "(requested --gpus %d)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from pamhd_amd import configs
+    if not torch.cuda.is_available():
+        print("no GPU visible", file=sys.stderr)
+        sys.exit(1)
+    cells = (args.size,) * 3
+    n = args.size ** 3
+    log("building %d^3 vortex state" % args.size)
+    state, length, gamma, mu0 = configs.orszag_tang_vortex(cells, noise=0.01, seed=2024)
+
+    results = {}
+    for prec in ("fp64", "mixed"):
+        log("gpu run: %s, %s, %d warmup + %d timed steps" % (prec, args.solver, args.warmup, args.steps))
+        results[prec] = run_gpu(prec, state, cells, length, gamma, mu0, args.solver,
+                                args.cfl, args.steps, args.warmup)
+        log("  %s: %.3f ms/step, %.3e cell-updates/s" % (prec, results[prec]["ms"] / args.steps,
+                                                          results[prec]["value"]))
+    del state
+
+    base = None
+    if not args.no_cpu_baseline:
+        log("cpu baseline: oracle, 1 thread, ~%.0f s" % args.cpu_seconds)
+        base = cpu_baseline(args.solver, args.cfl, args.cpu_seconds, 1)
+        log("  %.3e cell-updates/s" % base["value"])
+
+    f64 = results["fp64"]
+    mix = results["mixed"]
+    out = {
+        "metric": "MHD cell-updates/s at 256^3 HLLD (FP64 & mixed) on 1 GPU; % of HBM roofline",
+        "value": f64["value"],
+        "unit": "cell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": f64["ms"] / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "periodic Orszag-Tang smooth vortex with +-1%% density "
+                               "noise, %d^3 cells, %s, time-step factor %.2f"
+                               % (args.size, args.solver, args.cfl),
+                   "cells": list(cells), "solver": args.solver, "cfl": args.cfl,
+                   "gamma": gamma, "precision": "fp64"},
+        "roofline": roofline(f64, n),
+        "cpu_baseline": base,
+        "mixed": {"value": mix["value"], "ms_per_step": mix["ms"] / args.steps,
+                  "dtype": "f32 flux/wave speeds, f64 state",
+                  "speedup_vs_fp64": mix["value"] / f64["value"],
+                  "roofline": roofline(mix, n)},
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

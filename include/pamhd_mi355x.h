@@ -169,6 +169,18 @@ int pamhd_face_flux_batch(int solver, int precision, size_t n, const double* L,
 	double vacuum_permeability, double* F, double* smax, int* status,
 	void* stream);
 
+/*
+Per-kernel device-time accounting: when enabled, each launch of the handle is
+bracketed by HIP events on its stream.  pamhd_kernel_time synchronises and
+returns the accumulated milliseconds and launch count of one kernel:
+"dt", "flux_pass", "update" or "refit".  Enabling resets the totals.
+(No counterpart in the reference, which has no timers; used by the
+benchmark for the roofline figure.)
+*/
+int pamhd_enable_timing(pamhd_handle* h, int enable);
+int pamhd_kernel_time(pamhd_handle* h, const char* kernel, double* total_ms,
+	long* launches);
+
 /* Copies the last error message of h (or of the library when h is NULL). */
 int pamhd_last_error(pamhd_handle* h, char* buf, size_t n);
 

@@ -15,6 +15,9 @@ v[i] * s (multiplication commutes exactly).
 
 #include <float.h>
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -973,9 +976,13 @@ int or_step_periodic(const or_grid* g, int solver, double dt, int refits,
 	long zc[3] = {0, 0, 0};
 
 	/* flux pass of get_fluxes, source/mhd/solve.hpp:304-393 with get_flux :68-188 */
+	long bad = -1;
 	for (int d = 0; d < 3 && !err; d++) {
-		for (long k = 0; k < D.nz && !err; k++)
-		for (long j = 0; j < D.ny && !err; j++)
+		double smd = 0;
+		long zcd = 0;
+		#pragma omp parallel for collapse(2) schedule(static) reduction(max: smd) reduction(+: zcd)
+		for (long k = 0; k < D.nz; k++)
+		for (long j = 0; j < D.ny; j++)
 		for (long i = 0; i < D.nx; i++) {
 			const long c = cid(&D, i, j, k);
 			double L[8], R[8], bgr[3] = {0, 0, 0}, F[8], s = 0;
@@ -990,9 +997,14 @@ int or_step_periodic(const or_grid* g, int solver, double dt, int refits,
 			}
 			const int e = or_flux(solver, L, R, bgr, g->gamma, g->mu0, F, &s);
 			if (e) {
-				err = e;
-				if (bad_face) *bad_face = (long)d * n + c;
-				break;
+				#pragma omp critical
+				{
+					if (!err || (long)d * n + c < bad) {
+						err = e;
+						bad = (long)d * n + c;
+					}
+				}
+				continue;
 			}
 			double Fg[8];
 			Fg[RHO] = F[RHO];
@@ -1000,10 +1012,13 @@ int or_step_periodic(const or_grid* g, int solver, double dt, int refits,
 			rot_to_global(d, F + M0, Fg + M0);
 			rot_to_global(d, F + B0, Fg + B0);
 			for (int q = 0; q < 8; q++) P[(d * 8 + q) * n + c] = dt * Fg[q];
-			if (s <= 0) zc[d]++;
-			sm[d] = smax(sm[d], s);
+			if (s <= 0) zcd++;
+			if (s > smd) smd = s;
 		}
+		sm[d] = smd;
+		zc[d] = zcd;
 	}
+	if (err && bad_face) *bad_face = bad;
 	if (smax_dim) for (int d = 0; d < 3; d++) smax_dim[d] = sm[d];
 	if (zero_dim) for (int d = 0; d < 3; d++) zero_dim[d] = zc[d];
 	if (err) {
@@ -1025,6 +1040,7 @@ int or_step_periodic(const or_grid* g, int solver, double dt, int refits,
 #define EXZ(i, j, k) (-(PF(2, 6, i, j, k) * dx / 4))
 #define EYZ(i, j, k) (+(PF(2, 5, i, j, k) * dy / 4))
 
+	#pragma omp parallel for collapse(2) schedule(static)
 	for (long k = 0; k < D.nz; k++)
 	for (long j = 0; j < D.ny; j++)
 	for (long i = 0; i < D.nx; i++) {
@@ -1079,6 +1095,7 @@ int or_step_periodic(const or_grid* g, int solver, double dt, int refits,
 
 	/* conserved update of update_mhd_state, source/mhd/solve.hpp:1551-1587;
 	   flux slots hold 0 + dt*F (:375-392) */
+	#pragma omp parallel for collapse(2) schedule(static)
 	for (long k = 0; k < D.nz; k++)
 	for (long j = 0; j < D.ny; j++)
 	for (long i = 0; i < D.nx; i++) {
@@ -1338,4 +1355,15 @@ double or_gl7_average(double (*f)(double, void*), void* ctx, double a, double b)
 		r += (fp + fm) * w[i];
 	}
 	return r * half / (b - a);
+}
+
+int or_set_threads(int n)
+{
+#ifdef _OPENMP
+	if (n > 0) omp_set_num_threads(n);
+	return omp_get_max_threads();
+#else
+	(void)n;
+	return 1;
+#endif
 }

@@ -146,6 +146,9 @@ Runs from t = 0 until t >= t_end with the reference's lagged dt.
 int or_plain1d_run(const or_plain1d* p, int solver, double t_end, double* state,
 	long* steps_taken);
 
+/* OpenMP threads of the grid step (n > 0 sets it); returns the count in use */
+int or_set_threads(int n);
+
 /* ---- initial conditions used by the goldens and the benchmark configs ---- */
 
 /* 7-point Gauss-Legendre average over [a, b] of f (textbook nodes/weights) */

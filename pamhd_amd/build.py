@@ -1,10 +1,8 @@
-"""Builds the HIP library (gfx950) and the oracle in-tree.
+"""Builds the HIP library for gfx950 in-tree:
 
-    python -m pamhd_amd.build          # library + oracle (+ reference harness
-                                       # when the reference checkout exists)
+    python -m pamhd_amd.build     ->  pamhd_amd/lib/libpamhd_mi355x.so
 """
 import os
-import shutil
 import subprocess
 import sys
 
@@ -43,16 +41,8 @@ def build_library(verbose=False):
     return LIB
 
 
-def build_oracle():
-    subprocess.run(["make", "-C", os.path.join(ROOT, "oracle"), "-s"], check=True)
-    ref = os.path.join(ROOT, "oracle", "build_ref.sh")
-    if os.path.isdir("/root/reference/source"):
-        subprocess.run(["bash", ref], check=True, stdout=subprocess.DEVNULL)
-
-
 def build(verbose=False):
-    build_library(verbose)
-    build_oracle()
+    return build_library(verbose)
 
 
 if __name__ == "__main__":

@@ -25,6 +25,7 @@ transport face update, conserved update, constant-pressure energy refits).
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <vector>
 
 #include "pamhd_mi355x.h"
 #include "riemann.hpp"
@@ -192,8 +193,8 @@ __device__ __forceinline__ void refit_energy(double rho, const double m[3], doub
 /*
 Update pass: update_mhd_state + update_vol_B of source/mhd/solve.hpp:1503-1683.
 Face dB of each face sums the 12 constrained-transport edge terms of
-assign_face_dBs_f{x,y,z} (:610-613, :926-928, :1223-1225) in the fixed order
-of the oracle; the face field is recomputed for the cell's positive-side
+assign_face_dBs_f{x,y,z} (:610-613, :926-928, :1223-1225) in a fixed order
+(the face's own direction, then the next two cyclically); the face field is recomputed for the cell's positive-side
 faces (owned by the +x, +y, +z neighbours) for the energy refit.
 */
 __global__ __launch_bounds__(BX * BY) void update_kernel(GridDev g,
@@ -467,12 +468,59 @@ struct pamhd_handle {
 	StepStatus* status = nullptr; /* 2 entries */
 	double* host_dt = nullptr;    /* pinned */
 	StepStatus* host_status = nullptr;
+	/* optional per-kernel device time accounting with HIP events */
+	bool timing = false;
+	struct Timed { int kernel; hipEvent_t start, stop; };
+	std::vector<Timed> timed;
+	double kernel_ms[4] = {0, 0, 0, 0};
+	long kernel_launches[4] = {0, 0, 0, 0};
 	int cur = 0;    /* state buffer holding the current state */
 	int last = 0;   /* status entry of the last flux pass */
 	std::string err;
 };
 
 namespace {
+
+enum { K_DT = 0, K_FLUX = 1, K_UPDATE = 2, K_REFIT = 3 };
+const char* const kernel_names[4] = {"dt", "flux_pass", "update", "refit"};
+
+/* brackets a launch with events when timing is enabled */
+struct TimedLaunch {
+	pamhd_handle* h;
+	hipStream_t st;
+	int k;
+	hipEvent_t a = nullptr, b = nullptr;
+	TimedLaunch(pamhd_handle* h_, hipStream_t st_, int k_) : h(h_), st(st_), k(k_)
+	{
+		if (h->timing && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
+			hipEventRecord(a, st);
+		} else {
+			a = b = nullptr;
+		}
+	}
+	~TimedLaunch()
+	{
+		if (a && b) {
+			hipEventRecord(b, st);
+			h->timed.push_back({k, a, b});
+		}
+	}
+};
+
+void collect_times(pamhd_handle* h)
+{
+	for (auto& t : h->timed) {
+		hipEventSynchronize(t.stop);
+		float ms = 0;
+		if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
+			h->kernel_ms[t.kernel] += ms;
+			h->kernel_launches[t.kernel]++;
+		}
+		hipEventDestroy(t.start);
+		hipEventDestroy(t.stop);
+	}
+	h->timed.clear();
+}
 
 int fail(pamhd_handle* h, int code, const std::string& m)
 {
@@ -545,14 +593,24 @@ int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
 	hipStream_t st = (hipStream_t)stream_;
 	StepStatus* prev = h->status + h->last;
 	StepStatus* next = h->status + (1 - h->last);
-	dt_kernel<<<1, 1, 0, st>>>(prev, next, h->dt, cfl, max_dt, h->g.dx, h->g.dy, h->g.dz);
+	{
+		TimedLaunch t(h, st, K_DT);
+		dt_kernel<<<1, 1, 0, st>>>(prev, next, h->dt, cfl, max_dt, h->g.dx, h->g.dy, h->g.dz);
+	}
 	HIP_TRY(h, hipGetLastError());
-	const int e = h->precision == PAMHD_MIXED
-		? launch_flux_pass<float>(h, solver, st, next)
-		: launch_flux_pass<double>(h, solver, st, next);
+	int e;
+	{
+		TimedLaunch t(h, st, K_FLUX);
+		e = h->precision == PAMHD_MIXED
+			? launch_flux_pass<float>(h, solver, st, next)
+			: launch_flux_pass<double>(h, solver, st, next);
+	}
 	if (e) return e;
-	update_kernel<<<cell_grid(h->g), dim3(BX, BY, 1), 0, st>>>(
-		h->g, h->state[h->cur], h->state[1 - h->cur], h->flux, refits);
+	{
+		TimedLaunch t(h, st, K_UPDATE);
+		update_kernel<<<cell_grid(h->g), dim3(BX, BY, 1), 0, st>>>(
+			h->g, h->state[h->cur], h->state[1 - h->cur], h->flux, refits);
+	}
 	HIP_TRY(h, hipGetLastError());
 	h->cur = 1 - h->cur;
 	h->last = 1 - h->last;
@@ -631,6 +689,7 @@ int pamhd_create(const pamhd_grid_desc* grid, const pamhd_physics* physics, int 
 int pamhd_destroy(pamhd_handle* h)
 {
 	if (!h) return PAMHD_OK;
+	collect_times(h);
 	hipFree(h->state[0]);
 	hipFree(h->state[1]);
 	hipFree(h->flux);
@@ -693,7 +752,10 @@ int pamhd_step(pamhd_handle* h, int solver, double max_time_step, double time_st
 int pamhd_apply_boundaries(pamhd_handle* h, double /*simulation_time*/, void* stream)
 {
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
-	refit_kernel<<<cell_grid(h->g), dim3(BX, BY, 1), 0, (hipStream_t)stream>>>(h->g, h->state[h->cur]);
+	{
+		TimedLaunch t(h, (hipStream_t)stream, K_REFIT);
+		refit_kernel<<<cell_grid(h->g), dim3(BX, BY, 1), 0, (hipStream_t)stream>>>(h->g, h->state[h->cur]);
+	}
 	HIP_TRY(h, hipGetLastError());
 	return PAMHD_OK;
 }
@@ -778,6 +840,32 @@ int pamhd_face_flux_batch(int solver, int precision, size_t n, const double* L, 
 	const hipError_t e = hipGetLastError();
 	if (e != hipSuccess) return hip_fail(nullptr, e, "face_batch_kernel");
 	return PAMHD_OK;
+}
+
+int pamhd_enable_timing(pamhd_handle* h, int enable)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	collect_times(h);
+	h->timing = enable != 0;
+	for (int k = 0; k < 4; k++) {
+		h->kernel_ms[k] = 0;
+		h->kernel_launches[k] = 0;
+	}
+	return PAMHD_OK;
+}
+
+int pamhd_kernel_time(pamhd_handle* h, const char* kernel, double* total_ms, long* launches)
+{
+	if (!h || !kernel || !total_ms || !launches) return fail(h, PAMHD_E_ARG, "null argument");
+	collect_times(h);
+	for (int k = 0; k < 4; k++) {
+		if (strcmp(kernel, kernel_names[k]) == 0) {
+			*total_ms = h->kernel_ms[k];
+			*launches = h->kernel_launches[k];
+			return PAMHD_OK;
+		}
+	}
+	return fail(h, PAMHD_E_ARG, std::string("unknown kernel ") + kernel);
 }
 
 int pamhd_last_error(pamhd_handle* h, char* buf, size_t n)

@@ -19,7 +19,7 @@ Flux precision T is a template parameter:
     hardware reciprocal / square-root instructions (about 1 ulp).
 
 A domain error (the reference throws std::domain_error) is reported through
-the returned status code instead; codes match the oracle's.
+the returned status code instead (PAMHD_E_* of include/pamhd_mi355x.h).
 */
 #pragma once
 

@@ -143,6 +143,17 @@ class Grid:
     def check(self):
         self._check(self._L.pamhd_check(self._h))
 
+    def enable_timing(self, enable=True):
+        self._check(self._L.pamhd_enable_timing(self._h, 1 if enable else 0))
+
+    def kernel_time(self, kernel):
+        """(total milliseconds, launches) of one kernel since timing was enabled."""
+        ms = ctypes.c_double(0)
+        n = ctypes.c_long(0)
+        self._check(self._L.pamhd_kernel_time(self._h, kernel.encode(), ctypes.byref(ms),
+                                              ctypes.byref(n)))
+        return ms.value, n.value
+
     def div_face2volume(self):
         v = ctypes.c_double(0)
         self._check(self._L.pamhd_div_face2volume(self._h, ctypes.byref(v)))

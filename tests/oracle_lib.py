@@ -86,6 +86,8 @@ def lib():
     L.or_plain1d_run.argtypes = [ctypes.POINTER(OrPlain1d), ctypes.c_int,
                                  ctypes.c_double, _dp, _lp]
     L.or_plain1d_run.restype = ctypes.c_int
+    L.or_set_threads.argtypes = [ctypes.c_int]
+    L.or_set_threads.restype = ctypes.c_int
     _lib = L
     return L
 
