@@ -120,8 +120,10 @@ def test_goldens_on_gpu(gpu, case):
 
 def test_mixed_precision_solution_error(gpu):
     """HLLD, 32^3 vortex, 60 steps at time-step factor 0.3: relative L2 error
-    of each conserved variable of the mixed run against the FP64 run is at
-    most 1e-6."""
+    of each conserved variable (rho, momentum vector, E, face-field vector) of
+    the mixed run against the FP64 run is at most 1e-6.  Vector variables are
+    normalised by the vector's norm: m_z and B_z of this quasi-2-D state are
+    roundoff-sized in the FP64 run."""
     from pamhd_amd import solver as S
     cells = (32, 32, 32)
     state, length, gamma, mu0 = _vortex(cells, seed=5)
@@ -132,9 +134,11 @@ def test_mixed_precision_solution_error(gpu):
             g.run("hlld-athena", 0.3, 0.0, 1e9, max_steps=60)
             outs[prec] = g.download()
     a, b = outs["mixed"], outs["fp64"]
-    rel = np.sqrt(((a - b) ** 2).sum(1)) / np.sqrt((b ** 2).sum(1)).clip(1e-300)
+    rel = {}
+    for name, rows in (("rho", [0]), ("m", [1, 2, 3]), ("E", [4]), ("B", [5, 6, 7])):
+        rel[name] = float(np.sqrt(((a[rows] - b[rows]) ** 2).sum()) / np.sqrt((b[rows] ** 2).sum()))
     print("relative L2 error per variable", rel, flush=True)
-    assert rel.max() <= 1e-6
+    assert max(rel.values()) <= 1e-6
 
 
 def test_conservation_and_divergence(gpu):
