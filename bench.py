@@ -132,6 +132,9 @@ def main():
     ap.add_argument("--cfl", type=float, default=0.3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (scales the "
+                         "FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path "
@@ -143,6 +146,14 @@ def main():
     if not torch.cuda.is_available():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
+    if args.copy_calibration:
+        a = torch.ones(1 << 27, dtype=torch.float64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copied 3 x 1 GiB")
+        return
     cells = (args.size,) * 3
     n = args.size ** 3
     log("building %d^3 vortex state" % args.size)
