@@ -5,9 +5,9 @@ behind the C ABI of include/pamhd_mi355x.h.
 Data layout in HBM (structure of arrays, FP64, component-major):
   state[2]  ping-pong buffers of 8 * n doubles:
             rho, m_x, m_y, m_z, E, B_x(i-1/2), B_y(j-1/2), B_z(k-1/2)
-  flux      24 * n doubles: dt * F_q of the d-face owned by each cell
+  flux      24 * n values: dt * F_q of the d-face owned by each cell
             (d-major, then component, then cell), written by the flux pass
-            and read by the update pass
+            and read by the update pass (FP64, or FP32 in mixed precision)
   bg        optional 9 * n doubles of background field on the positive faces
   status[2] per-pass reductions (max signal speed per dimension, count of
             faces with speed <= 0, first physics error) feeding the lagged
@@ -15,8 +15,9 @@ Data layout in HBM (structure of arrays, FP64, component-major):
 
 One step = dt_kernel (1 thread: lagged CFL step from the previous pass) ->
 flux_pass_kernel (one thread per face, all faces of one dimension in a
-z-slab of blocks) -> update_kernel (one thread per cell: constrained-
-transport face update, conserved update, constant-pressure energy refits).
+z-slab of blocks) -> update_zmarch_kernel (column tiles marching in z:
+constrained-transport face update, conserved update, constant-pressure
+energy refits).  In mixed precision the flux array holds FP32 values.
 */
 #include <hip/hip_runtime.h>
 
@@ -38,7 +39,7 @@ struct GridDev {
 	int nx, ny, nz;
 	long long n;
 	double dx, dy, dz;
-	double gamma, mu0;
+	Phys ph;
 };
 
 struct StepStatus {
@@ -48,10 +49,25 @@ struct StepStatus {
 	long long err_face;
 };
 
-constexpr int BX = 64, BY = 4;
+constexpr int BX = 64, BY = 4;      /* flux pass and cell kernels */
+constexpr int UBX = 64, UBY = 4;    /* z-marching update */
+constexpr int NXCD = 8;             /* MI355X: 8 XCDs, blocks dispatched round-robin */
 
 __device__ __forceinline__ int wrap_dec(int i, int n) { return i == 0 ? n - 1 : i - 1; }
 __device__ __forceinline__ int wrap_inc(int i, int n) { return i + 1 == n ? 0 : i + 1; }
+
+/*
+XCD-aware block order: the hardware hands consecutive block ids to the 8 XCDs
+in turn, each with its own L2.  Remapping id -> (id % 8) * (nblocks / 8) +
+id / 8 gives every XCD one contiguous range of logical blocks (x fastest,
+then y, then z), so the neighbouring planes a block reads were touched by the
+same XCD shortly before.
+*/
+__device__ __forceinline__ unsigned logical_block(unsigned id, unsigned nblocks)
+{
+	if (nblocks % NXCD) return id;
+	return (id % NXCD) * (nblocks / NXCD) + id / NXCD;
+}
 
 /* cell state rotated into the frame of a d-face (get_rotated_vector,
    source/mhd/common.hpp:612-648; vol B of update_vol_B, solve.hpp:1661) */
@@ -117,18 +133,22 @@ __device__ __forceinline__ void reduce_speed(StepStatus* st, int d, double s, un
 
 /*
 Flux pass: get_fluxes + get_flux of source/mhd/solve.hpp:68-188, :304-393.
-Thread (i, j, k) of z-slab d computes the d-face on the cell's negative side
-between L = cell - e_d and R = cell, stores dt * F in the global frame.
+One thread per face: logical block (bx, by, slab) with slab = d * nz + k; the
+thread's cell (i, j, k) owns the d-face on its negative side, between
+L = cell - e_d and R = cell.  Stores dt * F (global frame) as PT.
 */
-template <int SOLVER, class T>
+template <int SOLVER, class T, class PT>
 __global__ __launch_bounds__(BX * BY) void flux_pass_kernel(GridDev g,
 	const double* __restrict__ s, const double* __restrict__ bg,
-	double* __restrict__ P, const double* __restrict__ dtp, StepStatus* st)
+	PT* __restrict__ P, const double* __restrict__ dtp, StepStatus* st,
+	unsigned gx, unsigned gy)
 {
-	const int i = blockIdx.x * BX + threadIdx.x;
-	const int j = blockIdx.y * BY + threadIdx.y;
-	const int d = blockIdx.z / g.nz;
-	const int k = blockIdx.z - d * g.nz;
+	const unsigned lb = logical_block(blockIdx.x, gridDim.x);
+	const unsigned bx = lb % gx, by = (lb / gx) % gy, slab = lb / (gx * gy);
+	const int i = bx * BX + threadIdx.x;
+	const int j = by * BY + threadIdx.y;
+	const int d = slab / g.nz;
+	const int k = slab - d * g.nz;
 	const long long n = g.n;
 	double sp = 0;
 	unsigned zero = 0;
@@ -148,7 +168,7 @@ __global__ __launch_bounds__(BX * BY) void flux_pass_kernel(GridDev g,
 			b[2] = (T)bgl[(d + 2) % 3];
 		}
 		T F[8], sT = 0;
-		const int e = riemann<SOLVER, T>(L, R, b, g.gamma, g.mu0, F, sT);
+		const int e = riemann<SOLVER, T>(L, R, b, g.ph, F, sT);
 		double Fg[8];
 		if (e) {
 			record_error(st, e, (long long)d * n + c);
@@ -166,8 +186,8 @@ __global__ __launch_bounds__(BX * BY) void flux_pass_kernel(GridDev g,
 			Fg[B0 + (d + 2) % 3] = (double)F[B2];
 		}
 		const double dt = *dtp;
-		double* Pd = P + (size_t)d * 8 * n;
-		for (int q = 0; q < 8; q++) Pd[(size_t)q * n + c] = dt * Fg[q];
+		PT* Pd = P + (size_t)d * 8 * n;
+		for (int q = 0; q < 8; q++) Pd[(size_t)q * n + c] = (PT)(dt * Fg[q]);
 		sp = (double)sT;
 		zero = sp <= 0 ? 1u : 0u;
 	}
@@ -177,153 +197,267 @@ __global__ __launch_bounds__(BX * BY) void flux_pass_kernel(GridDev g,
 /* constant-thermal-pressure energy refit of update_vol_B,
    source/mhd/solve.hpp:1646-1672 */
 __device__ __forceinline__ void refit_energy(double rho, const double m[3], double& E,
-	const double vb_old[3], const double vb_new[3], double gamma, double mu0)
+	const double vb_old[3], const double vb_new[3], const Phys& ph)
 {
 	if (rho <= 0) return;
 	const double U[8] = {rho, m[0], m[1], m[2], E, vb_old[0], vb_old[1], vb_old[2]};
-	const double p = pressure64(U, gamma, mu0);
+	const double p = pressure64(U, ph.gm1, ph.half_inv_mu0);
 	const double inv = 1 / rho;
 	const double v[3] = {m[0] * inv, m[1] * inv, m[2] * inv};
 	const double
 		kinetic = 0.5 * rho * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]),
-		magnetic = 0.5 / mu0 * (vb_new[0] * vb_new[0] + vb_new[1] * vb_new[1] + vb_new[2] * vb_new[2]);
-	E = p / (gamma - 1) + kinetic + magnetic;
+		magnetic = ph.half_inv_mu0 * (vb_new[0] * vb_new[0] + vb_new[1] * vb_new[1] + vb_new[2] * vb_new[2]);
+	E = p / ph.gm1 + kinetic + magnetic;
 }
 
 /*
-Update pass: update_mhd_state + update_vol_B of source/mhd/solve.hpp:1503-1683.
-Face dB of each face sums the 12 constrained-transport edge terms of
-assign_face_dBs_f{x,y,z} (:610-613, :926-928, :1223-1225) in a fixed order
-(the face's own direction, then the next two cyclically); the face field is recomputed for the cell's positive-side
-faces (owned by the +x, +y, +z neighbours) for the energy refit.
+Constrained-transport face changes.  Each face sums the 12 edge terms of
+assign_face_dBs_f{x,y,z} (source/mhd/solve.hpp:610-613, :926-928,
+:1223-1225) that touch its edges, in a fixed order (the face's own
+direction, then the next two cyclically), from dt * F of the faces owned by
+the cells around it:
+  x-face: Ez = +F_By dt dz/4, Ey = -F_Bz dt dy/4
+  y-face: Ex = +F_Bz dt dx/4, Ez = -F_Bx dt dz/4
+  z-face: Ex = -F_By dt dx/4, Ey = +F_Bx dt dy/4
 */
-__global__ __launch_bounds__(BX * BY) void update_kernel(GridDev g,
-	const double* __restrict__ s, double* __restrict__ out,
-	const double* __restrict__ P, int refits)
+template <class PT> struct EdgeTerms {
+	const PT* P;
+	long long n;
+	double dx, dy, dz;
+	__device__ __forceinline__ double ezx(long long c) const { return +((double)P[6 * n + c] * dz / 4); }
+	__device__ __forceinline__ double eyx(long long c) const { return -((double)P[7 * n + c] * dy / 4); }
+	__device__ __forceinline__ double exy(long long c) const { return +((double)P[15 * n + c] * dx / 4); }
+	__device__ __forceinline__ double ezy(long long c) const { return -((double)P[13 * n + c] * dz / 4); }
+	__device__ __forceinline__ double exz(long long c) const { return -((double)P[22 * n + c] * dx / 4); }
+	__device__ __forceinline__ double eyz(long long c) const { return +((double)P[21 * n + c] * dy / 4); }
+};
+
+struct Nb {
+	int nx, ny;
+	__device__ __forceinline__ long long operator()(int i, int j, int k) const
+	{
+		return i + (long long)nx * (j + (long long)ny * k);
+	}
+};
+
+/* dB of the x-face owned by (i, j, k); im = i-1, jm/jp = j-+1, km/kp = k-+1 */
+template <class PT> __device__ __forceinline__ double dBx_at(const EdgeTerms<PT>& E, const Nb& L,
+	int i, int im, int j, int jm, int jp, int k, int km, int kp)
 {
-	const int i = blockIdx.x * BX + threadIdx.x;
-	const int j = blockIdx.y * BY + threadIdx.y;
-	const int k = blockIdx.z;
-	if (i >= g.nx || j >= g.ny) return;
+	double r = 0;
+	r += E.ezx(L(i, jp, k));
+	r -= E.ezx(L(i, jm, k));
+	r -= E.eyx(L(i, j, kp));
+	r += E.eyx(L(i, j, km));
+	r += E.ezy(L(i, jp, k));
+	r += E.ezy(L(im, jp, k));
+	r -= E.ezy(L(i, j, k));
+	r -= E.ezy(L(im, j, k));
+	r -= E.eyz(L(i, j, kp));
+	r -= E.eyz(L(im, j, kp));
+	r += E.eyz(L(i, j, k));
+	r += E.eyz(L(im, j, k));
+	return r;
+}
+
+template <class PT> __device__ __forceinline__ double dBy_at(const EdgeTerms<PT>& E, const Nb& L,
+	int i, int im, int ip, int j, int jm, int k, int km, int kp)
+{
+	double r = 0;
+	r += E.exy(L(i, j, kp));
+	r -= E.exy(L(i, j, km));
+	r -= E.ezy(L(ip, j, k));
+	r += E.ezy(L(im, j, k));
+	r += E.exz(L(i, j, kp));
+	r += E.exz(L(i, jm, kp));
+	r -= E.exz(L(i, j, k));
+	r -= E.exz(L(i, jm, k));
+	r -= E.ezx(L(ip, j, k));
+	r += E.ezx(L(i, j, k));
+	r -= E.ezx(L(ip, jm, k));
+	r += E.ezx(L(i, jm, k));
+	return r;
+}
+
+template <class PT> __device__ __forceinline__ double dBz_at(const EdgeTerms<PT>& E, const Nb& L,
+	int i, int im, int ip, int j, int jm, int jp, int k, int km)
+{
+	double r = 0;
+	r += E.eyz(L(ip, j, k));
+	r -= E.eyz(L(im, j, k));
+	r -= E.exz(L(i, jp, k));
+	r += E.exz(L(i, jm, k));
+	r += E.eyx(L(ip, j, k));
+	r += E.eyx(L(ip, j, km));
+	r -= E.eyx(L(i, j, k));
+	r -= E.eyx(L(i, j, km));
+	r -= E.exy(L(i, jp, k));
+	r -= E.exy(L(i, jp, km));
+	r += E.exy(L(i, j, k));
+	r += E.exy(L(i, j, km));
+	return r;
+}
+
+/*
+Update pass: update_mhd_state + update_vol_B of source/mhd/solve.hpp:1503-1683,
+fused into one z-marching pass.  A block owns a UBX x UBY column tile and a
+chunk of kz planes; thread (i, j) walks k.  At plane k it
+  * computes the new face fields of its three faces (the z-face first, which
+    also completes the cell below),
+  * gets the new +x / +y faces from its neighbours through LDS (threads on
+    the tile's +x / +y edge recompute them),
+  * applies the x and y flux differences to its cell and carries the
+    partial state; the z difference and the energy refits follow at plane
+    k + 1, once the cell's +z face field is known.
+Every flux value is loaded once per plane by its owner and reused through
+L1 by the neighbours; the order of every sum is that of the reference
+(x, then y, then z per conserved variable).
+*/
+template <class PT>
+__global__ __launch_bounds__(UBX * UBY) void update_zmarch_kernel(GridDev g,
+	const double* __restrict__ s, double* __restrict__ out,
+	const PT* __restrict__ P, int refits, int kz, unsigned gx, unsigned gy)
+{
+	__shared__ double sbx[2][UBY][UBX];
+	__shared__ double sby[2][UBY][UBX];
+	const unsigned lb = logical_block(blockIdx.x, gridDim.x);
+	const unsigned bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
+	const int tx = threadIdx.x, ty = threadIdx.y;
+	const int i = bx * UBX + tx;
+	const int j = by * UBY + ty;
+	const bool active = i < g.nx && j < g.ny;
+	const int ic = active ? i : 0, jc = active ? j : 0;
+	const int im = wrap_dec(ic, g.nx), ip = wrap_inc(ic, g.nx), ipp = wrap_inc(ip, g.nx);
+	const int jm = wrap_dec(jc, g.ny), jp = wrap_inc(jc, g.ny), jpp = wrap_inc(jp, g.ny);
+	const bool own_xp = tx + 1 < UBX && i + 1 < g.nx;   /* +x neighbour in this block */
+	const bool own_yp = ty + 1 < UBY && j + 1 < g.ny;
 	const long long n = g.n;
-	const int xs[3] = {wrap_dec(i, g.nx), i, wrap_inc(i, g.nx)};
-	const int ys[3] = {wrap_dec(j, g.ny), j, wrap_inc(j, g.ny)};
-	const int zs[3] = {wrap_dec(k, g.nz), k, wrap_inc(k, g.nz)};
-	auto idx = [&](int a, int b, int c) -> long long {
-		return xs[a] + (long long)g.nx * (ys[b] + (long long)g.ny * zs[c]);
-	};
-	auto pf = [&](int d, int q, int a, int b, int c) -> double {
-		return P[((size_t)d * 8 + q) * n + idx(a, b, c)];
-	};
-	const double dx = g.dx, dy = g.dy, dz = g.dz;
-	auto EZX = [&](int a, int b, int c) { return +(pf(0, 6, a, b, c) * dz / 4); };
-	auto EYX = [&](int a, int b, int c) { return -(pf(0, 7, a, b, c) * dy / 4); };
-	auto EXY = [&](int a, int b, int c) { return +(pf(1, 7, a, b, c) * dx / 4); };
-	auto EZY = [&](int a, int b, int c) { return -(pf(1, 5, a, b, c) * dz / 4); };
-	auto EXZ = [&](int a, int b, int c) { return -(pf(2, 6, a, b, c) * dx / 4); };
-	auto EYZ = [&](int a, int b, int c) { return +(pf(2, 5, a, b, c) * dy / 4); };
+	const Nb L{g.nx, g.ny};
+	const EdgeTerms<PT> E{P, n, g.dx, g.dy, g.dz};
+	const double axy = g.dx * g.dy, axz = g.dx * g.dz, ayz = g.dy * g.dz;
+	const double len[3] = {g.dx, g.dy, g.dz};
+	const PT* P0 = P;
+	const PT* P1 = P + 8 * n;
+	const PT* P2 = P + 16 * n;
+	const int k0 = bz * kz;
+	const int k1 = min(k0 + kz, g.nz);
 
-	auto dBx_at = [&](int a, int b, int c) {
-		double r = 0;
-		r += EZX(a, b + 1, c);
-		r -= EZX(a, b - 1, c);
-		r -= EYX(a, b, c + 1);
-		r += EYX(a, b, c - 1);
-		r += EZY(a, b + 1, c);
-		r += EZY(a - 1, b + 1, c);
-		r -= EZY(a, b, c);
-		r -= EZY(a - 1, b, c);
-		r -= EYZ(a, b, c + 1);
-		r -= EYZ(a - 1, b, c + 1);
-		r += EYZ(a, b, c);
-		r += EYZ(a - 1, b, c);
-		return r;
-	};
-	auto dBy_at = [&](int a, int b, int c) {
-		double r = 0;
-		r += EXY(a, b, c + 1);
-		r -= EXY(a, b, c - 1);
-		r -= EZY(a + 1, b, c);
-		r += EZY(a - 1, b, c);
-		r += EXZ(a, b, c + 1);
-		r += EXZ(a, b - 1, c + 1);
-		r -= EXZ(a, b, c);
-		r -= EXZ(a, b - 1, c);
-		r -= EZX(a + 1, b, c);
-		r += EZX(a, b, c);
-		r -= EZX(a + 1, b - 1, c);
-		r += EZX(a, b - 1, c);
-		return r;
-	};
-	auto dBz_at = [&](int a, int b, int c) {
-		double r = 0;
-		r += EYZ(a + 1, b, c);
-		r -= EYZ(a - 1, b, c);
-		r -= EXZ(a, b + 1, c);
-		r += EXZ(a, b - 1, c);
-		r += EYX(a + 1, b, c);
-		r += EYX(a + 1, b, c - 1);
-		r -= EYX(a, b, c);
-		r -= EYX(a, b, c - 1);
-		r -= EXY(a, b + 1, c);
-		r -= EXY(a, b + 1, c - 1);
-		r += EXY(a, b, c);
-		r += EXY(a, b, c - 1);
-		return r;
-	};
+	/* carried from plane k - 1 */
+	double c_rho = 0, c_m[3] = {0, 0, 0}, c_E = 0, c_vb[3] = {0, 0, 0};
+	double c_bx = 0, c_bxp = 0, c_by = 0, c_byp = 0, c_bz = 0;
+	double lo_z[8];
+	for (int q = 0; q < 8; q++) lo_z[q] = 0;
 
-	const long long c0 = idx(1, 1, 1), cxp = idx(2, 1, 1), cyp = idx(1, 2, 1), czp = idx(1, 1, 2);
-	const double axy = dx * dy, axz = dx * dz, ayz = dy * dz;
-	const double bx = s[5 * n + c0] + dBx_at(1, 1, 1) / ayz;
-	const double by = s[6 * n + c0] + dBy_at(1, 1, 1) / axz;
-	const double bz = s[7 * n + c0] + dBz_at(1, 1, 1) / axy;
-	const double bxp = s[5 * n + cxp] + dBx_at(2, 1, 1) / ayz;
-	const double byp = s[6 * n + cyp] + dBy_at(1, 2, 1) / axz;
-	const double bzp = s[7 * n + czp] + dBz_at(1, 1, 2) / axy;
-	out[5 * n + c0] = bx;
-	out[6 * n + c0] = by;
-	out[7 * n + c0] = bz;
+	for (int k = k0; k <= k1; k++) {
+		const int kk = k < g.nz ? k : k - g.nz;
+		const int km = wrap_dec(kk, g.nz), kp = wrap_inc(kk, g.nz);
+		const long long c = L(ic, jc, kk);
+		double bz_new = 0;
+		double hi_z[8];
+		if (active) {
+			bz_new = s[7 * n + c] + dBz_at(E, L, ic, im, ip, jc, jm, jp, kk, km) / axy;
+			for (int q = 0; q < 8; q++) hi_z[q] = (double)P2[(long long)q * n + c];
+		}
+		if (active && k > k0) {
+			/* finish the cell below: z flux difference, new cell field, refits */
+			const double ilz = 1 / len[2];
+			c_rho += ((0.0 + lo_z[RHO]) - (0.0 + hi_z[RHO])) / len[2];
+			for (int q = 0; q < 3; q++) {
+				c_m[q] = c_m[q] + ((0.0 + lo_z[M0 + q]) - (0.0 + hi_z[M0 + q])) * ilz;
+			}
+			c_E += ((0.0 + lo_z[NRJ]) - (0.0 + hi_z[NRJ])) / len[2];
+			for (int q = 0; q < 3; q++) {
+				c_vb[q] = c_vb[q] + ((0.0 + lo_z[B0 + q]) - (0.0 + hi_z[B0 + q])) * ilz;
+			}
+			const double vb_new[3] = {0.5 * (c_bx + c_bxp), 0.5 * (c_by + c_byp), 0.5 * (c_bz + bz_new)};
+			double Ecur = c_E;
+			for (int r = 0; r < refits; r++) {
+				refit_energy(c_rho, c_m, Ecur, r == 0 ? c_vb : vb_new, vb_new, g.ph);
+			}
+			const long long cb = L(ic, jc, wrap_dec(kk, g.nz));
+			out[cb] = c_rho;
+			out[n + cb] = c_m[0];
+			out[2 * n + cb] = c_m[1];
+			out[3 * n + cb] = c_m[2];
+			out[4 * n + cb] = Ecur;
+		}
+		if (k == k1) break;
 
-	const double len[3] = {dx, dy, dz};
-	const long long cp[3] = {cxp, cyp, czp};
-	double rho = s[c0];
-	double m[3] = {s[n + c0], s[2 * n + c0], s[3 * n + c0]};
-	double E = s[4 * n + c0];
-	double vb[3] = {
-		0.5 * (s[5 * n + c0] + s[5 * n + cxp]),
-		0.5 * (s[6 * n + c0] + s[6 * n + cyp]),
-		0.5 * (s[7 * n + c0] + s[7 * n + czp])};
-	for (int d = 0; d < 3; d++) {
-		const double* Pd = P + (size_t)d * 8 * n;
-		rho += ((0.0 + Pd[c0]) - (0.0 + Pd[cp[d]])) / len[d];
-	}
-	for (int d = 0; d < 3; d++) {
-		const double* Pd = P + (size_t)d * 8 * n;
-		const double il = 1 / len[d];
-		for (int q = 0; q < 3; q++) {
-			m[q] = m[q] + ((0.0 + Pd[(size_t)(M0 + q) * n + c0]) - (0.0 + Pd[(size_t)(M0 + q) * n + cp[d]])) * il;
+		/* new x- and y-face fields of this plane */
+		double bx_new = 0, by_new = 0;
+		if (active) {
+			bx_new = s[5 * n + c] + dBx_at(E, L, ic, im, jc, jm, jp, kk, km, kp) / ayz;
+			by_new = s[6 * n + c] + dBy_at(E, L, ic, im, ip, jc, jm, kk, km, kp) / axz;
+			out[5 * n + c] = bx_new;
+			out[6 * n + c] = by_new;
+			out[7 * n + c] = bz_new;
+		}
+		const int par = k & 1;
+		sbx[par][ty][tx] = bx_new;
+		sby[par][ty][tx] = by_new;
+		__syncthreads();
+		if (active) {
+			double bxp_new, byp_new;
+			const long long cxp = L(ip, jc, kk), cyp = L(ic, jp, kk);
+			if (own_xp) {
+				bxp_new = sbx[par][ty][tx + 1];
+			} else {
+				bxp_new = s[5 * n + cxp] + dBx_at(E, L, ip, ic, jc, jm, jp, kk, km, kp) / ayz;
+			}
+			if (own_yp) {
+				byp_new = sby[par][ty + 1][tx];
+			} else {
+				byp_new = s[6 * n + cyp] + dBy_at(E, L, ic, im, ip, jp, jc, kk, km, kp) / axz;
+			}
+
+			/* x and y flux differences of this plane's cell */
+			double rho = s[c];
+			double m[3] = {s[n + c], s[2 * n + c], s[3 * n + c]};
+			double En = s[4 * n + c];
+			const long long czp = L(ic, jc, kp);
+			double vb[3] = {
+				0.5 * (s[5 * n + c] + s[5 * n + cxp]),
+				0.5 * (s[6 * n + c] + s[6 * n + cyp]),
+				0.5 * (s[7 * n + c] + s[7 * n + czp])};
+			const PT* Pd[2] = {P0, P1};
+			const long long chi[2] = {cxp, cyp};
+			for (int d = 0; d < 2; d++) {
+				rho += ((0.0 + (double)Pd[d][c]) - (0.0 + (double)Pd[d][chi[d]])) / len[d];
+			}
+			for (int d = 0; d < 2; d++) {
+				const double il = 1 / len[d];
+				for (int q = 0; q < 3; q++) {
+					const long long o = (long long)(M0 + q) * n;
+					m[q] = m[q] + ((0.0 + (double)Pd[d][o + c]) - (0.0 + (double)Pd[d][o + chi[d]])) * il;
+				}
+			}
+			for (int d = 0; d < 2; d++) {
+				const long long o = (long long)NRJ * n;
+				En += ((0.0 + (double)Pd[d][o + c]) - (0.0 + (double)Pd[d][o + chi[d]])) / len[d];
+			}
+			for (int d = 0; d < 2; d++) {
+				const double il = 1 / len[d];
+				for (int q = 0; q < 3; q++) {
+					const long long o = (long long)(B0 + q) * n;
+					vb[q] = vb[q] + ((0.0 + (double)Pd[d][o + c]) - (0.0 + (double)Pd[d][o + chi[d]])) * il;
+				}
+			}
+			c_rho = rho;
+			for (int q = 0; q < 3; q++) {
+				c_m[q] = m[q];
+				c_vb[q] = vb[q];
+			}
+			c_E = En;
+			c_bx = bx_new;
+			c_bxp = bxp_new;
+			c_by = by_new;
+			c_byp = byp_new;
+			c_bz = bz_new;
+			for (int q = 0; q < 8; q++) lo_z[q] = hi_z[q];
 		}
 	}
-	for (int d = 0; d < 3; d++) {
-		const double* Pd = P + (size_t)d * 8 * n;
-		E += ((0.0 + Pd[(size_t)NRJ * n + c0]) - (0.0 + Pd[(size_t)NRJ * n + cp[d]])) / len[d];
-	}
-	for (int d = 0; d < 3; d++) {
-		const double* Pd = P + (size_t)d * 8 * n;
-		const double il = 1 / len[d];
-		for (int q = 0; q < 3; q++) {
-			vb[q] = vb[q] + ((0.0 + Pd[(size_t)(B0 + q) * n + c0]) - (0.0 + Pd[(size_t)(B0 + q) * n + cp[d]])) * il;
-		}
-	}
-	const double vb_new[3] = {0.5 * (bx + bxp), 0.5 * (by + byp), 0.5 * (bz + bzp)};
-	for (int r = 0; r < refits; r++) {
-		refit_energy(rho, m, E, r == 0 ? vb : vb_new, vb_new, g.gamma, g.mu0);
-	}
-	out[c0] = rho;
-	out[n + c0] = m[0];
-	out[2 * n + c0] = m[1];
-	out[3 * n + c0] = m[2];
-	out[4 * n + c0] = E;
+	(void)ipp;
+	(void)jpp;
 }
 
 /* lagged CFL step of minimize_timestep + set_minmax_substepping_period with a
@@ -368,7 +502,7 @@ __global__ __launch_bounds__(BX * BY) void refit_kernel(GridDev g, double* __res
 		0.5 * (s[7 * n + c] + s[7 * n + cz])};
 	const double m[3] = {s[n + c], s[2 * n + c], s[3 * n + c]};
 	double E = s[4 * n + c];
-	refit_energy(s[c], m, E, vb, vb, g.gamma, g.mu0);
+	refit_energy(s[c], m, E, vb, vb, g.ph);
 	s[4 * n + c] = E;
 }
 
@@ -405,7 +539,7 @@ __global__ __launch_bounds__(BX * BY) void div_kernel(GridDev g, const double* _
 /* batched Riemann solver for the pamhd_face_flux_batch unit-test surface */
 template <int SOLVER, class T>
 __global__ __launch_bounds__(256) void face_batch_kernel(size_t n, const double* __restrict__ Lp,
-	const double* __restrict__ Rp, const double* __restrict__ bg, double gamma, double mu0,
+	const double* __restrict__ Rp, const double* __restrict__ bg, Phys ph,
 	double* __restrict__ Fp, double* __restrict__ sp, int* __restrict__ status)
 {
 	const size_t f = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -418,7 +552,7 @@ __global__ __launch_bounds__(256) void face_batch_kernel(size_t n, const double*
 	T b[3] = {0, 0, 0};
 	if (bg) for (int q = 0; q < 3; q++) b[q] = (T)bg[q * n + f];
 	T F[8], s = 0;
-	const int e = riemann<SOLVER, T>(L, R, b, gamma, mu0, F, s);
+	const int e = riemann<SOLVER, T>(L, R, b, ph, F, s);
 	for (int q = 0; q < 8; q++) Fp[q * n + f] = e ? 0.0 : (double)F[q];
 	sp[f] = e ? 0.0 : (double)s;
 	if (status) status[f] = e;
@@ -541,29 +675,52 @@ dim3 cell_grid(const GridDev& g, int slabs = 1)
 	return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, g.nz * slabs);
 }
 
-template <class T>
+template <class T, class PT>
 int launch_flux_pass(pamhd_handle* h, int solver, hipStream_t st, StepStatus* stat)
 {
 	const double* s = h->state[h->cur];
-	const dim3 grid = cell_grid(h->g, 3), block(BX, BY, 1);
+	PT* P = reinterpret_cast<PT*>(h->flux);
+	const unsigned gx = (h->g.nx + BX - 1) / BX, gy = (h->g.ny + BY - 1) / BY;
+	const unsigned nblocks = gx * gy * (unsigned)h->g.nz * 3;
+	const dim3 grid(nblocks), block(BX, BY, 1);
 	switch (solver) {
 	case RUSANOV:
-		flux_pass_kernel<RUSANOV, T><<<grid, block, 0, st>>>(h->g, s, h->bg, h->flux, h->dt, stat);
+		flux_pass_kernel<RUSANOV, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy);
 		break;
 	case HLL_ATHENA:
-		flux_pass_kernel<HLL_ATHENA, T><<<grid, block, 0, st>>>(h->g, s, h->bg, h->flux, h->dt, stat);
+		flux_pass_kernel<HLL_ATHENA, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy);
 		break;
 	case HLLD_ATHENA:
-		flux_pass_kernel<HLLD_ATHENA, T><<<grid, block, 0, st>>>(h->g, s, h->bg, h->flux, h->dt, stat);
+		flux_pass_kernel<HLLD_ATHENA, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy);
 		break;
 	case ROE_ATHENA:
-		flux_pass_kernel<ROE_ATHENA, T><<<grid, block, 0, st>>>(h->g, s, h->bg, h->flux, h->dt, stat);
+		flux_pass_kernel<ROE_ATHENA, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy);
 		break;
 	default:
 		return fail(h, PAMHD_E_SOLVER, "invalid solver " + std::to_string(solver));
 	}
 	HIP_TRY(h, hipGetLastError());
 	return PAMHD_OK;
+}
+
+/* planes per block of the z-marching update: enough blocks to fill the chip */
+int update_chunk(const GridDev& g)
+{
+	const long long tiles = (long long)((g.nx + UBX - 1) / UBX) * ((g.ny + UBY - 1) / UBY);
+	int kz = 32;
+	while (kz > 4 && tiles * ((g.nz + kz - 1) / kz) < 2048) kz /= 2;
+	return kz < g.nz ? kz : g.nz;
+}
+
+template <class PT>
+void launch_update(pamhd_handle* h, hipStream_t st, int refits)
+{
+	const unsigned gx = (h->g.nx + UBX - 1) / UBX, gy = (h->g.ny + UBY - 1) / UBY;
+	const int kz = update_chunk(h->g);
+	const unsigned gz = (h->g.nz + kz - 1) / kz;
+	update_zmarch_kernel<PT><<<dim3(gx * gy * gz), dim3(UBX, UBY, 1), 0, st>>>(
+		h->g, h->state[h->cur], h->state[1 - h->cur], reinterpret_cast<const PT*>(h->flux),
+		refits, kz, gx, gy);
 }
 
 int report_status(pamhd_handle* h, const StepStatus& st)
@@ -602,14 +759,14 @@ int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
 	{
 		TimedLaunch t(h, st, K_FLUX);
 		e = h->precision == PAMHD_MIXED
-			? launch_flux_pass<float>(h, solver, st, next)
-			: launch_flux_pass<double>(h, solver, st, next);
+			? launch_flux_pass<float, float>(h, solver, st, next)
+			: launch_flux_pass<double, double>(h, solver, st, next);
 	}
 	if (e) return e;
 	{
 		TimedLaunch t(h, st, K_UPDATE);
-		update_kernel<<<cell_grid(h->g), dim3(BX, BY, 1), 0, st>>>(
-			h->g, h->state[h->cur], h->state[1 - h->cur], h->flux, refits);
+		if (h->precision == PAMHD_MIXED) launch_update<float>(h, st, refits);
+		else launch_update<double>(h, st, refits);
 	}
 	HIP_TRY(h, hipGetLastError());
 	h->cur = 1 - h->cur;
@@ -655,7 +812,7 @@ int pamhd_create(const pamhd_grid_desc* grid, const pamhd_physics* physics, int 
 	h->precision = precision;
 	h->g = GridDev{grid->cells[0], grid->cells[1], grid->cells[2], n,
 		grid->cell_length[0], grid->cell_length[1], grid->cell_length[2],
-		physics->adiabatic_index, physics->vacuum_permeability};
+		make_phys(physics->adiabatic_index, physics->vacuum_permeability)};
 	auto cleanup = [&](int code, const char* what, hipError_t e) {
 		std::string m = std::string(what) + ": " + hipGetErrorString(e);
 		pamhd_destroy(h);
@@ -818,7 +975,8 @@ int pamhd_face_flux_batch(int solver, int precision, size_t n, const double* L, 
 	if (precision != PAMHD_FP64 && precision != PAMHD_MIXED) return fail(nullptr, PAMHD_E_ARG, "invalid precision");
 	hipStream_t st = (hipStream_t)stream;
 	const dim3 grid((unsigned)((n + 255) / 256)), block(256);
-#define BATCH(S, T) face_batch_kernel<S, T><<<grid, block, 0, st>>>(n, L, R, bg, gamma, mu0, F, smax_out, status)
+	const Phys ph = make_phys(gamma, mu0);
+#define BATCH(S, T) face_batch_kernel<S, T><<<grid, block, 0, st>>>(n, L, R, bg, ph, F, smax_out, status)
 	if (precision == PAMHD_FP64) {
 		switch (solver) {
 		case RUSANOV: BATCH(RUSANOV, double); break;

@@ -78,18 +78,34 @@ template <> struct Arith<float> {
 	static constexpr float eps = FLT_EPSILON;
 };
 
+/*
+Physical constants and the per-run values derived from them, computed once on
+the host (IEEE division and square root round identically there):
+0.5/mu0 and 1/mu0 of get_pressure / get_flux, sqrt(mu0) and 1/sqrt(mu0) of the
+HLLD and Roe solvers.
+*/
+struct Phys {
+	double gamma, mu0;
+	double gm1;           /* gamma - 1 */
+	double inv_mu0;       /* 1.0 / mu0 */
+	double half_inv_mu0;  /* 0.5 / mu0 */
+	double sqrt_mu0;      /* sqrt(mu0) */
+	double inv_sqrt_mu0;  /* 1.0 / sqrt(mu0) */
+	double two_mu0;       /* 2 * mu0 */
+};
+
 template <class T> __device__ __forceinline__ T dot3(const T* a, const T* b)
 {
 	return a[0] * b[0] + a[1] * b[1] + a[2] * b[2];
 }
 
-/* FP64 thermal pressure, source/mhd/common.hpp:76-115 */
-__device__ __forceinline__ double pressure64(const double U[8], double gamma, double mu0)
+/* FP64 thermal pressure, source/mhd/common.hpp:76-115; half_inv_mu0 = 0.5/mu0 */
+__device__ __forceinline__ double pressure64(const double U[8], double gm1, double half_inv_mu0)
 {
 	const double
 		kinetic = 0.5 / U[RHO] * (U[M0] * U[M0] + U[M1] * U[M1] + U[M2] * U[M2]),
-		magnetic = 0.5 / mu0 * (U[B0] * U[B0] + U[B1] * U[B1] + U[B2] * U[B2]);
-	return (U[NRJ] - kinetic - magnetic) * (gamma - 1);
+		magnetic = half_inv_mu0 * (U[B0] * U[B0] + U[B1] * U[B1] + U[B2] * U[B2]);
+	return (U[NRJ] - kinetic - magnetic) * gm1;
 }
 
 /*
@@ -105,12 +121,12 @@ template <class T> struct Side {
 	bool rho_pos;
 };
 
-template <class T> __device__ __forceinline__ void make_side(const double U[8], double gamma,
-	double mu0, Side<T>& s)
+template <class T> __device__ __forceinline__ void make_side(const double U[8], const Phys& ph,
+	Side<T>& s)
 {
 	for (int q = 0; q < 8; q++) s.u[q] = (T)U[q];
 	s.rho_pos = U[RHO] > 0;
-	s.p64 = s.rho_pos ? pressure64(U, gamma, mu0) : 0.0;
+	s.p64 = s.rho_pos ? pressure64(U, ph.gm1, ph.half_inv_mu0) : 0.0;
 	s.p = (T)s.p64;
 	if (s.rho_pos) {
 		const T inv = Arith<T>::rcp(s.u[RHO]);
@@ -156,14 +172,14 @@ template <class T> __device__ __forceinline__ int fast_speed(const Side<T>& s,
 
 /* ideal-MHD x-flux with background-field splitting, source/mhd/common.hpp:138-244 */
 template <class T> __device__ __forceinline__ int physical_flux(const Side<T>& s,
-	const double U[8], const T bg[3], T mu0, T F[8])
+	const double U[8], const T bg[3], T inv_mu0, T F[8])
 {
 	if (!__builtin_isnormal(U[RHO]) || U[RHO] <= 0) return ERR_DENSITY;
 	if (!__builtin_isfinite(U[M0])) return ERR_FLUX;
 	const T* B = s.u + B0;
 	const T t[3] = {B[0] + bg[0], B[1] + bg[1], B[2] + bg[2]};
 	const T
-		inv = Arith<T>::rcp(mu0),
+		inv = inv_mu0,
 		pB0 = T(0.5) * inv * (bg[0] * bg[0] + bg[1] * bg[1] + bg[2] * bg[2]),
 		pB1 = T(0.5) * inv * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]),
 		pBt = T(0.5) * inv * (t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
@@ -188,15 +204,15 @@ template <class T> __device__ __forceinline__ int physical_flux(const Side<T>& s
 
 /* source/mhd/rusanov.hpp:59-214 */
 template <class T> __device__ __forceinline__ int flux_rusanov(const double Ld[8],
-	const double Rd[8], const T bg[3], double gamma64, double mu064, T F[8], T& s_out)
+	const double Rd[8], const T bg[3], const Phys& ph, T F[8], T& s_out)
 {
 	using A = Arith<T>;
-	const T gamma = (T)gamma64, mu0 = (T)mu064;
+	const T gamma = (T)ph.gamma, mu0 = (T)ph.mu0, inv_mu0 = (T)ph.inv_mu0;
 	if (!__builtin_isnormal(Ld[RHO]) || Ld[RHO] < 0) return ERR_DENSITY;
 	if (!__builtin_isnormal(Rd[RHO]) || Rd[RHO] < 0) return ERR_DENSITY;
 	Side<T> L, R;
-	make_side<T>(Ld, gamma64, mu064, L);
-	make_side<T>(Rd, gamma64, mu064, R);
+	make_side<T>(Ld, ph, L);
+	make_side<T>(Rd, ph, R);
 	T cfL, cfR;
 	int e = fast_speed<T>(L, Ld, bg, gamma, mu0, cfL);
 	if (e) return e;
@@ -208,9 +224,9 @@ template <class T> __device__ __forceinline__ int flux_rusanov(const double Ld[8
 			fabs(A::div(R.u[M0], R.u[RHO])) + max_fast);
 	if (!A::isfinite(max_fast) || !A::isfinite(s)) return ERR_SPEED;
 	T FL[8], FR[8];
-	e = physical_flux<T>(L, Ld, bg, mu0, FL);
+	e = physical_flux<T>(L, Ld, bg, inv_mu0, FL);
 	if (e) return e;
-	e = physical_flux<T>(R, Rd, bg, mu0, FR);
+	e = physical_flux<T>(R, Rd, bg, inv_mu0, FR);
 	if (e) return e;
 	if (!A::isfinite(FL[RHO]) || !A::isfinite(FR[RHO])) return ERR_FLUX;
 
@@ -228,17 +244,17 @@ template <class T> __device__ __forceinline__ int flux_rusanov(const double Ld[8
 
 /* source/mhd/hll_athena.hpp:57-275 */
 template <class T> __device__ __forceinline__ int flux_hll(const double Ld[8],
-	const double Rd[8], const T bg[3], double gamma64, double mu064, T F[8], T& s_out)
+	const double Rd[8], const T bg[3], const Phys& ph, T F[8], T& s_out)
 {
 	using A = Arith<T>;
-	const T gamma = (T)gamma64, mu0 = (T)mu064;
+	const T gamma = (T)ph.gamma, mu0 = (T)ph.mu0, inv_mu0 = (T)ph.inv_mu0;
 	if (Ld[RHO] <= 0 || Rd[RHO] <= 0) return ERR_DENSITY; /* get_pressure throws */
 	Side<T> L, R;
-	make_side<T>(Ld, gamma64, mu064, L);
-	make_side<T>(Rd, gamma64, mu064, R);
+	make_side<T>(Ld, ph, L);
+	make_side<T>(Rd, ph, R);
 	const T
-		pmL = A::div(dot3(L.u + B0, L.u + B0), T(2) * mu0),
-		pmR = A::div(dot3(R.u + B0, R.u + B0), T(2) * mu0);
+		pmL = A::div(dot3(L.u + B0, L.u + B0), (T)ph.two_mu0),
+		pmR = A::div(dot3(R.u + B0, R.u + B0), (T)ph.two_mu0);
 	T cfL, cfR;
 	int e = fast_speed<T>(L, Ld, bg, gamma, mu0, cfL);
 	if (e) return e;
@@ -263,9 +279,9 @@ template <class T> __device__ __forceinline__ int flux_hll(const double Ld[8],
 		return ERR_OK;
 	}
 	T FL[8], FR[8];
-	e = physical_flux<T>(L, Ld, bg, mu0, FL);
+	e = physical_flux<T>(L, Ld, bg, inv_mu0, FL);
 	if (e) return e;
-	e = physical_flux<T>(R, Rd, bg, mu0, FR);
+	e = physical_flux<T>(R, Rd, bg, inv_mu0, FR);
 	if (e) return e;
 	FL[RHO] -= L.u[RHO] * bm;
 	FL[NRJ] -= L.u[NRJ] * bm;
@@ -320,22 +336,22 @@ template <class T> __device__ __forceinline__ void hlld_star(const Side<T>& s, T
 
 /* source/mhd/hlld_athena.hpp:54-559 (ignores the background field, :77) */
 template <class T> __device__ __forceinline__ int flux_hlld(const double Ld[8],
-	const double Rd[8], const T* /*bg*/, double gamma64, double mu064, T F[8], T& s_out)
+	const double Rd[8], const T* /*bg*/, const Phys& ph, T F[8], T& s_out)
 {
 	using A = Arith<T>;
-	const T gamma = (T)gamma64, mu0 = (T)mu064;
+	const T gamma = (T)ph.gamma, mu0 = (T)ph.mu0, inv_mu0 = (T)ph.inv_mu0;
 	const T zero[3] = {0, 0, 0};
 	if (Ld[RHO] <= 0) return ERR_DENSITY;
 	if (Rd[RHO] <= 0) return ERR_DENSITY;
 	Side<T> L, R;
-	make_side<T>(Ld, gamma64, mu064, L);
-	make_side<T>(Rd, gamma64, mu064, R);
+	make_side<T>(Ld, ph, L);
+	make_side<T>(Rd, ph, R);
 	if (L.p64 <= 0) return ERR_PRESSURE;
 	if (R.p64 <= 0) return ERR_PRESSURE;
 
 	const T
-		inv = A::rcp(mu0),
-		inv_sqrt = A::rcp(A::sqrt(mu0)),
+		inv = inv_mu0,
+		inv_sqrt = (T)ph.inv_sqrt_mu0,
 		Bx = (L.u[B0] + R.u[B0]) / T(2),
 		Bx2 = Bx * Bx;
 	T cfL, cfR;
@@ -351,9 +367,9 @@ template <class T> __device__ __forceinline__ int flux_hlld(const double Ld[8],
 		msp = (vL[0] <= vR[0]) ? vR[0] + ms : vL[0] + ms;
 
 	T FL[8], FR[8];
-	e = physical_flux<T>(L, Ld, zero, mu0, FL);
+	e = physical_flux<T>(L, Ld, zero, inv_mu0, FL);
 	if (e) return e;
-	e = physical_flux<T>(R, Rd, zero, mu0, FR);
+	e = physical_flux<T>(R, Rd, zero, inv_mu0, FR);
 	if (e) return e;
 
 	if (msn >= T(0)) {
@@ -424,7 +440,7 @@ template <class T> __device__ __forceinline__ int flux_hlld(const double Ld[8],
 		}
 		S2n[M0] = Sn[M0];
 		S2p[M0] = Sp[M0];
-		const T c2 = sgn * dssn * dssp * A::sqrt(mu0);
+		const T c2 = sgn * dssn * dssp * (T)ph.sqrt_mu0;
 		for (int i = 0; i < 3; i++) {
 			const T b = ((Sp[B0 + i] * dssn + Sn[B0 + i] * dssp) + (vsp[i] - vsn[i]) * c2) * isds;
 			S2n[B0 + i] = b;
@@ -785,20 +801,20 @@ template <class T> __device__ __forceinline__ int roe_fluxes(const T Ul[7], cons
 
 /* source/mhd/roe_athena.hpp:567-818 (ignores the background field, :584) */
 template <class T> __device__ __forceinline__ int flux_roe(const double Ld[8],
-	const double Rd[8], const T* /*bg*/, double gamma64, double mu064, T F[8], T& s_out)
+	const double Rd[8], const T* /*bg*/, const Phys& ph, T F[8], T& s_out)
 {
 	using A = Arith<T>;
-	const T gamma = (T)gamma64, mu0 = (T)mu064;
+	const T gamma = (T)ph.gamma, mu0 = (T)ph.mu0, inv_mu0 = (T)ph.inv_mu0;
 	const T zero[3] = {0, 0, 0};
 	if (Ld[RHO] <= 0) return ERR_DENSITY;
 	if (Rd[RHO] <= 0) return ERR_DENSITY;
 	Side<T> L, R;
-	make_side<T>(Ld, gamma64, mu064, L);
-	make_side<T>(Rd, gamma64, mu064, R);
+	make_side<T>(Ld, ph, L);
+	make_side<T>(Rd, ph, R);
 	if (L.p64 <= 0) return ERR_PRESSURE;
 	if (R.p64 <= 0) return ERR_PRESSURE;
 
-	const T sq = A::sqrt(mu0);
+	const T sq = (T)ph.sqrt_mu0;
 	const T Bxi = A::div(L.u[B0] + R.u[B0], T(2.0) * sq);
 	const T Ul[7] = {L.u[RHO], L.u[M0], L.u[M1], L.u[M2], L.u[NRJ],
 		A::div(L.u[B1], sq), A::div(L.u[B2], sq)};
@@ -806,8 +822,8 @@ template <class T> __device__ __forceinline__ int flux_roe(const double Ld[8],
 		A::div(R.u[B1], sq), A::div(R.u[B2], sq)};
 	/* pressure with vacuum permeability 1 on the unscaled field
 	   (roe_athena.hpp:705-721), recovered in FP64 */
-	const T pl1 = (T)pressure64(Ld, gamma64, 1.0);
-	const T pr1 = (T)pressure64(Rd, gamma64, 1.0);
+	const T pl1 = (T)pressure64(Ld, ph.gm1, 0.5);
+	const T pr1 = (T)pressure64(Rd, ph.gm1, 0.5);
 	const T Wl[7] = {L.u[RHO], L.v[0], L.v[1], L.v[2], pl1, Ul[5], Ul[6]};
 	const T Wr[7] = {R.u[RHO], R.v[0], R.v[1], R.v[2], pr1, Ur[5], Ur[6]};
 
@@ -843,13 +859,27 @@ template <class T> __device__ __forceinline__ int flux_roe(const double Ld[8],
 }
 
 template <int SOLVER, class T> __device__ __forceinline__ int riemann(const double L[8],
-	const double R[8], const T bg[3], double gamma, double mu0, T F[8], T& s)
+	const double R[8], const T bg[3], const Phys& ph, T F[8], T& s)
 {
-	if constexpr (SOLVER == RUSANOV) return flux_rusanov<T>(L, R, bg, gamma, mu0, F, s);
-	else if constexpr (SOLVER == HLL_ATHENA) return flux_hll<T>(L, R, bg, gamma, mu0, F, s);
-	else if constexpr (SOLVER == HLLD_ATHENA) return flux_hlld<T>(L, R, bg, gamma, mu0, F, s);
-	else if constexpr (SOLVER == ROE_ATHENA) return flux_roe<T>(L, R, bg, gamma, mu0, F, s);
+	if constexpr (SOLVER == RUSANOV) return flux_rusanov<T>(L, R, bg, ph, F, s);
+	else if constexpr (SOLVER == HLL_ATHENA) return flux_hll<T>(L, R, bg, ph, F, s);
+	else if constexpr (SOLVER == HLLD_ATHENA) return flux_hlld<T>(L, R, bg, ph, F, s);
+	else if constexpr (SOLVER == ROE_ATHENA) return flux_roe<T>(L, R, bg, ph, F, s);
 	else return ERR_SOLVER;
+}
+
+inline Phys make_phys(double gamma, double mu0)
+{
+	Phys p;
+	p.gamma = gamma;
+	p.mu0 = mu0;
+	p.gm1 = gamma - 1;
+	p.inv_mu0 = 1.0 / mu0;
+	p.half_inv_mu0 = 0.5 / mu0;
+	p.sqrt_mu0 = std::sqrt(mu0);
+	p.inv_sqrt_mu0 = 1.0 / std::sqrt(mu0);
+	p.two_mu0 = 2 * mu0;
+	return p;
 }
 
 } // namespace pamhd_dev
