@@ -86,6 +86,8 @@ def run_gpu(prec, state, cells, length, gamma, mu0, solver, cfl, steps, warmup):
         stream = torch.cuda.current_stream().cuda_stream
         g.apply_boundaries(0.0, stream=stream)
         g.timestep(solver, 0.0, cfl, sync=True, stream=stream)   # final init, dt = 0
+        tuning = g.autotune(solver, stream=stream)
+        log("  autotune: " + tuning)
         for _ in range(warmup):
             g.step(solver, 1e300, cfl, sync=False, stream=stream)
         torch.cuda.synchronize()
@@ -103,7 +105,8 @@ def run_gpu(prec, state, cells, length, gamma, mu0, solver, cfl, steps, warmup):
         g.check()
         kt = {k: g.kernel_time(k) for k in ("dt", "flux_pass", "update")}
         speeds = g.max_signal_speeds()
-    return {"ms": ms, "value": n * steps / (ms / 1e3), "kernels": kt, "speeds": speeds}
+    return {"ms": ms, "value": n * steps / (ms / 1e3), "kernels": kt, "speeds": speeds,
+            "tuning": tuning}
 
 
 def roofline(res, n):
@@ -117,6 +120,7 @@ def roofline(res, n):
     return {"bound": "hbm", "kernel": "flux_pass+update",
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "tuning": res["tuning"].split("; chosen ")[-1],
             "kernel_ms": {"flux_pass": round(flux_ms / flux_n, 4),
                           "update": round(upd_ms / upd_n, 4),
                           "dt": round(kt["dt"][0] / max(kt["dt"][1], 1), 4)}}

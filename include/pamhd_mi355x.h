@@ -170,6 +170,22 @@ int pamhd_face_flux_batch(int solver, int precision, size_t n, const double* L,
 	void* stream);
 
 /*
+Autotuner: times the launch variants of the flux and update passes that give
+bit-identical results (block order, per-cell vs z-marching update and its
+planes per block) on the handle's current state with the given solver, keeps
+the fastest for later steps and writes a report to buf (may be NULL).  The
+state and the lagged time step are not changed.  (No counterpart in the
+reference.)
+*/
+int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen,
+	void* stream);
+
+/* Sets the launch variants directly: XCD-aware block order of the flux pass
+   and of the update pass (0/1), update_kz = 0 for the per-cell update or the
+   planes per block of the z-marching update.  All give identical results. */
+int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_remap);
+
+/*
 Per-kernel device-time accounting: when enabled, each launch of the handle is
 bracketed by HIP events on its stream.  pamhd_kernel_time synchronises and
 returns the accumulated milliseconds and launch count of one kernel:

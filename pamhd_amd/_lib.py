@@ -19,7 +19,7 @@ EXPORTS = (
     "pamhd_download_state", "pamhd_state_device_ptr", "pamhd_set_background_faces",
     "pamhd_timestep", "pamhd_apply_boundaries", "pamhd_step", "pamhd_run",
     "pamhd_check", "pamhd_div_face2volume", "pamhd_max_signal_speeds",
-    "pamhd_face_flux_batch", "pamhd_enable_timing", "pamhd_kernel_time",
+    "pamhd_face_flux_batch", "pamhd_autotune", "pamhd_set_launch", "pamhd_enable_timing", "pamhd_kernel_time",
     "pamhd_last_error",
 )
 
@@ -77,6 +77,8 @@ def load():
     L.pamhd_face_flux_batch.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
                                         _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,
                                         _vp, _vp, _vp, _vp]
+    L.pamhd_autotune.argtypes = [H, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, _vp]
+    L.pamhd_set_launch.argtypes = [H, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.pamhd_enable_timing.argtypes = [H, ctypes.c_int]
     L.pamhd_kernel_time.argtypes = [H, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_long)]
     L.pamhd_last_error.argtypes = [H, ctypes.c_char_p, ctypes.c_size_t]

@@ -141,9 +141,9 @@ template <int SOLVER, class T, class PT>
 __global__ __launch_bounds__(BX * BY) void flux_pass_kernel(GridDev g,
 	const double* __restrict__ s, const double* __restrict__ bg,
 	PT* __restrict__ P, const double* __restrict__ dtp, StepStatus* st,
-	unsigned gx, unsigned gy)
+	unsigned gx, unsigned gy, int remap)
 {
-	const unsigned lb = logical_block(blockIdx.x, gridDim.x);
+	const unsigned lb = remap ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
 	const unsigned bx = lb % gx, by = (lb / gx) % gy, slab = lb / (gx * gy);
 	const int i = bx * BX + threadIdx.x;
 	const int j = by * BY + threadIdx.y;
@@ -316,11 +316,11 @@ L1 by the neighbours; the order of every sum is that of the reference
 template <class PT>
 __global__ __launch_bounds__(UBX * UBY) void update_zmarch_kernel(GridDev g,
 	const double* __restrict__ s, double* __restrict__ out,
-	const PT* __restrict__ P, int refits, int kz, unsigned gx, unsigned gy)
+	const PT* __restrict__ P, int refits, int kz, unsigned gx, unsigned gy, int remap)
 {
 	__shared__ double sbx[2][UBY][UBX];
 	__shared__ double sby[2][UBY][UBX];
-	const unsigned lb = logical_block(blockIdx.x, gridDim.x);
+	const unsigned lb = remap ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
 	const unsigned bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
 	const int tx = threadIdx.x, ty = threadIdx.y;
 	const int i = bx * UBX + tx;
@@ -458,6 +458,85 @@ __global__ __launch_bounds__(UBX * UBY) void update_zmarch_kernel(GridDev g,
 	}
 	(void)ipp;
 	(void)jpp;
+}
+
+/*
+Per-cell update variant (one thread per cell, no marching): the same sums in
+the same order as update_zmarch_kernel, recomputing the positive-side faces'
+dB instead of exchanging them.  Kept as an autotuning candidate.
+*/
+template <class PT>
+__global__ __launch_bounds__(BX * BY) void update_cell_kernel(GridDev g,
+	const double* __restrict__ s, double* __restrict__ out,
+	const PT* __restrict__ P, int refits, unsigned gx, unsigned gy, int remap)
+{
+	const unsigned lb = remap ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+	const unsigned bx = lb % gx, by = (lb / gx) % gy, k = lb / (gx * gy);
+	const int i = bx * BX + threadIdx.x;
+	const int j = by * BY + threadIdx.y;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	const int im = wrap_dec(i, g.nx), ip = wrap_inc(i, g.nx);
+	const int jm = wrap_dec(j, g.ny), jp = wrap_inc(j, g.ny);
+	const int kk = (int)k, km = wrap_dec(kk, g.nz), kp = wrap_inc(kk, g.nz), kpp = wrap_inc(kp, g.nz);
+	const Nb L{g.nx, g.ny};
+	const EdgeTerms<PT> E{P, n, g.dx, g.dy, g.dz};
+	const double axy = g.dx * g.dy, axz = g.dx * g.dz, ayz = g.dy * g.dz;
+	const double len[3] = {g.dx, g.dy, g.dz};
+	const long long c = L(i, j, kk), cxp = L(ip, j, kk), cyp = L(i, jp, kk), czp = L(i, j, kp);
+	const double bx_new = s[5 * n + c] + dBx_at(E, L, i, im, j, jm, jp, kk, km, kp) / ayz;
+	const double by_new = s[6 * n + c] + dBy_at(E, L, i, im, ip, j, jm, kk, km, kp) / axz;
+	const double bz_new = s[7 * n + c] + dBz_at(E, L, i, im, ip, j, jm, jp, kk, km) / axy;
+	const double bxp_new = s[5 * n + cxp] + dBx_at(E, L, ip, i, j, jm, jp, kk, km, kp) / ayz;
+	const double byp_new = s[6 * n + cyp] + dBy_at(E, L, i, im, ip, jp, j, kk, km, kp) / axz;
+	const double bzp_new = s[7 * n + czp] + dBz_at(E, L, i, im, ip, j, jm, jp, kp, kk) / axy;
+	out[5 * n + c] = bx_new;
+	out[6 * n + c] = by_new;
+	out[7 * n + c] = bz_new;
+	(void)kpp;
+
+	const long long cp[3] = {cxp, cyp, czp};
+	double rho = s[c];
+	double m[3] = {s[n + c], s[2 * n + c], s[3 * n + c]};
+	double En = s[4 * n + c];
+	double vb[3] = {
+		0.5 * (s[5 * n + c] + s[5 * n + cxp]),
+		0.5 * (s[6 * n + c] + s[6 * n + cyp]),
+		0.5 * (s[7 * n + c] + s[7 * n + czp])};
+	for (int d = 0; d < 3; d++) {
+		const PT* Pd = P + (long long)d * 8 * n;
+		rho += ((0.0 + (double)Pd[c]) - (0.0 + (double)Pd[cp[d]])) / len[d];
+	}
+	for (int d = 0; d < 3; d++) {
+		const PT* Pd = P + (long long)d * 8 * n;
+		const double il = 1 / len[d];
+		for (int q = 0; q < 3; q++) {
+			const long long o = (long long)(M0 + q) * n;
+			m[q] = m[q] + ((0.0 + (double)Pd[o + c]) - (0.0 + (double)Pd[o + cp[d]])) * il;
+		}
+	}
+	for (int d = 0; d < 3; d++) {
+		const PT* Pd = P + (long long)d * 8 * n;
+		const long long o = (long long)NRJ * n;
+		En += ((0.0 + (double)Pd[o + c]) - (0.0 + (double)Pd[o + cp[d]])) / len[d];
+	}
+	for (int d = 0; d < 3; d++) {
+		const PT* Pd = P + (long long)d * 8 * n;
+		const double il = 1 / len[d];
+		for (int q = 0; q < 3; q++) {
+			const long long o = (long long)(B0 + q) * n;
+			vb[q] = vb[q] + ((0.0 + (double)Pd[o + c]) - (0.0 + (double)Pd[o + cp[d]])) * il;
+		}
+	}
+	const double vb_new[3] = {0.5 * (bx_new + bxp_new), 0.5 * (by_new + byp_new), 0.5 * (bz_new + bzp_new)};
+	for (int r = 0; r < refits; r++) {
+		refit_energy(rho, m, En, r == 0 ? vb : vb_new, vb_new, g.ph);
+	}
+	out[c] = rho;
+	out[n + c] = m[0];
+	out[2 * n + c] = m[1];
+	out[3 * n + c] = m[2];
+	out[4 * n + c] = En;
 }
 
 /* lagged CFL step of minimize_timestep + set_minmax_substepping_period with a
@@ -602,6 +681,12 @@ struct pamhd_handle {
 	StepStatus* status = nullptr; /* 2 entries */
 	double* host_dt = nullptr;    /* pinned */
 	StepStatus* host_status = nullptr;
+	/* launch choices, set by pamhd_autotune (defaults: per-cell update) */
+	struct Tune {
+		int flux_remap = 0;
+		int update_kz = 0;
+		int update_remap = 0;
+	} tune;
 	/* optional per-kernel device time accounting with HIP events */
 	bool timing = false;
 	struct Timed { int kernel; hipEvent_t start, stop; };
@@ -685,16 +770,16 @@ int launch_flux_pass(pamhd_handle* h, int solver, hipStream_t st, StepStatus* st
 	const dim3 grid(nblocks), block(BX, BY, 1);
 	switch (solver) {
 	case RUSANOV:
-		flux_pass_kernel<RUSANOV, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy);
+		flux_pass_kernel<RUSANOV, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy, h->tune.flux_remap);
 		break;
 	case HLL_ATHENA:
-		flux_pass_kernel<HLL_ATHENA, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy);
+		flux_pass_kernel<HLL_ATHENA, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy, h->tune.flux_remap);
 		break;
 	case HLLD_ATHENA:
-		flux_pass_kernel<HLLD_ATHENA, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy);
+		flux_pass_kernel<HLLD_ATHENA, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy, h->tune.flux_remap);
 		break;
 	case ROE_ATHENA:
-		flux_pass_kernel<ROE_ATHENA, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy);
+		flux_pass_kernel<ROE_ATHENA, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy, h->tune.flux_remap);
 		break;
 	default:
 		return fail(h, PAMHD_E_SOLVER, "invalid solver " + std::to_string(solver));
@@ -703,24 +788,23 @@ int launch_flux_pass(pamhd_handle* h, int solver, hipStream_t st, StepStatus* st
 	return PAMHD_OK;
 }
 
-/* planes per block of the z-marching update: enough blocks to fill the chip */
-int update_chunk(const GridDev& g)
-{
-	const long long tiles = (long long)((g.nx + UBX - 1) / UBX) * ((g.ny + UBY - 1) / UBY);
-	int kz = 32;
-	while (kz > 4 && tiles * ((g.nz + kz - 1) / kz) < 2048) kz /= 2;
-	return kz < g.nz ? kz : g.nz;
-}
-
+/* update_kz = 0 selects the per-cell update kernel, > 0 the z-marching one
+   with that many planes per block */
 template <class PT>
 void launch_update(pamhd_handle* h, hipStream_t st, int refits)
 {
+	const PT* P = reinterpret_cast<const PT*>(h->flux);
+	const int kz = h->tune.update_kz < h->g.nz ? h->tune.update_kz : h->g.nz;
+	if (kz <= 0) {
+		const unsigned gx = (h->g.nx + BX - 1) / BX, gy = (h->g.ny + BY - 1) / BY;
+		update_cell_kernel<PT><<<dim3(gx * gy * h->g.nz), dim3(BX, BY, 1), 0, st>>>(
+			h->g, h->state[h->cur], h->state[1 - h->cur], P, refits, gx, gy, h->tune.update_remap);
+		return;
+	}
 	const unsigned gx = (h->g.nx + UBX - 1) / UBX, gy = (h->g.ny + UBY - 1) / UBY;
-	const int kz = update_chunk(h->g);
 	const unsigned gz = (h->g.nz + kz - 1) / kz;
 	update_zmarch_kernel<PT><<<dim3(gx * gy * gz), dim3(UBX, UBY, 1), 0, st>>>(
-		h->g, h->state[h->cur], h->state[1 - h->cur], reinterpret_cast<const PT*>(h->flux),
-		refits, kz, gx, gy);
+		h->g, h->state[h->cur], h->state[1 - h->cur], P, refits, kz, gx, gy, h->tune.update_remap);
 }
 
 int report_status(pamhd_handle* h, const StepStatus& st)
@@ -997,6 +1081,95 @@ int pamhd_face_flux_batch(int solver, int precision, size_t n, const double* L, 
 #undef BATCH
 	const hipError_t e = hipGetLastError();
 	if (e != hipSuccess) return hip_fail(nullptr, e, "face_batch_kernel");
+	return PAMHD_OK;
+}
+
+int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* stream_)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (solver != RUSANOV && solver != HLL_ATHENA && solver != HLLD_ATHENA && solver != ROE_ATHENA) {
+		return fail(h, PAMHD_E_SOLVER, "invalid solver " + std::to_string(solver));
+	}
+	hipStream_t st = (hipStream_t)stream_;
+	const bool timing = h->timing;
+	h->timing = false;
+	StepStatus* prev = h->status + h->last;
+	StepStatus* next = h->status + (1 - h->last);
+	hipEvent_t a, b;
+	HIP_TRY(h, hipEventCreate(&a));
+	HIP_TRY(h, hipEventCreate(&b));
+	/* average of 3 timed launches after one warm-up; neither the current
+	   state nor the status of the last flux pass is modified */
+	auto time_it = [&](auto&& launch) -> float {
+		launch();
+		float best = 0;
+		hipEventRecord(a, st);
+		for (int r = 0; r < 3; r++) launch();
+		hipEventRecord(b, st);
+		hipEventSynchronize(b);
+		hipEventElapsedTime(&best, a, b);
+		return best / 3;
+	};
+	auto flux = [&]() {
+		dt_kernel<<<1, 1, 0, st>>>(prev, next, h->dt, 0.0, 0.0, h->g.dx, h->g.dy, h->g.dz);
+		if (h->precision == PAMHD_MIXED) launch_flux_pass<float, float>(h, solver, st, next);
+		else launch_flux_pass<double, double>(h, solver, st, next);
+	};
+	std::string report = "flux_pass:";
+	float best_flux = 1e30f;
+	int best_remap = 0;
+	for (int remap = 0; remap < 2; remap++) {
+		h->tune.flux_remap = remap;
+		const float ms = time_it(flux);
+		report += " remap" + std::to_string(remap) + "=" + std::to_string(ms) + "ms";
+		if (ms < best_flux) {
+			best_flux = ms;
+			best_remap = remap;
+		}
+	}
+	h->tune.flux_remap = best_remap;
+	flux();
+	report += "; update:";
+	const int kzs[] = {0, 4, 8, 16, 32};
+	float best_upd = 1e30f;
+	pamhd_handle::Tune best = h->tune;
+	for (int kz : kzs) {
+		if (kz > 0 && kz > h->g.nz) continue;
+		for (int remap = 0; remap < 2; remap++) {
+			h->tune.update_kz = kz;
+			h->tune.update_remap = remap;
+			const float ms = time_it([&]() {
+				if (h->precision == PAMHD_MIXED) launch_update<float>(h, st, 3);
+				else launch_update<double>(h, st, 3);
+			});
+			report += " " + std::string(kz ? "zmarch" + std::to_string(kz) : "cell") +
+				"/remap" + std::to_string(remap) + "=" + std::to_string(ms) + "ms";
+			if (ms < best_upd) {
+				best_upd = ms;
+				best = h->tune;
+			}
+		}
+	}
+	h->tune = best;
+	hipEventDestroy(a);
+	hipEventDestroy(b);
+	HIP_TRY(h, hipGetLastError());
+	HIP_TRY(h, hipStreamSynchronize(st));
+	h->timing = timing;
+	report += "; chosen flux_remap=" + std::to_string(h->tune.flux_remap) +
+		" update=" + (h->tune.update_kz ? "zmarch" + std::to_string(h->tune.update_kz) : std::string("cell")) +
+		" update_remap=" + std::to_string(h->tune.update_remap);
+	if (buf && buflen) snprintf(buf, buflen, "%s", report.c_str());
+	return PAMHD_OK;
+}
+
+int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_remap)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (update_kz < 0) return fail(h, PAMHD_E_ARG, "update_kz must be >= 0");
+	h->tune.flux_remap = flux_remap != 0;
+	h->tune.update_kz = update_kz;
+	h->tune.update_remap = update_remap != 0;
 	return PAMHD_OK;
 }
 

@@ -143,6 +143,15 @@ class Grid:
     def check(self):
         self._check(self._L.pamhd_check(self._h))
 
+    def autotune(self, solver, stream=None):
+        """Pick the fastest bit-identical launch variants; returns the report."""
+        buf = ctypes.create_string_buffer(4096)
+        self._check(self._L.pamhd_autotune(self._h, solver_id(solver), buf, 4096, stream))
+        return buf.value.decode()
+
+    def set_launch(self, flux_remap=0, update_kz=0, update_remap=0):
+        self._check(self._L.pamhd_set_launch(self._h, flux_remap, update_kz, update_remap))
+
     def enable_timing(self, enable=True):
         self._check(self._L.pamhd_enable_timing(self._h, 1 if enable else 0))
 

@@ -90,6 +90,43 @@ def test_grid_steps_fp64_bit_exact(gpu, name, sid):
     np.testing.assert_array_equal(out, ref)
 
 
+@pytest.mark.parametrize("variant", [(0, 0, 0), (1, 0, 1), (0, 4, 0), (1, 8, 1), (0, 32, 1)])
+def test_launch_variants_bit_exact(gpu, variant):
+    """Every launch variant the autotuner may pick (block order, per-cell or
+    z-marching update with its planes per block) gives the oracle's bits;
+    uneven sizes exercise partial tiles and chunk ends."""
+    from pamhd_amd import solver as S
+    cells = (70, 9, 13)
+    state, length, gamma, mu0 = _vortex(cells, seed=11)
+    og = O.make_grid(cells, length, gamma, mu0)
+    ref = state.copy()
+    st, steps_o, t_o = O.run_periodic(og, O.HLL, 0.3, 0.0, 1e9, ref, max_steps=4)
+    assert st == 0
+    with S.Grid(cells, length, gamma, mu0) as g:
+        g.upload(state)
+        g.set_launch(*variant)
+        g.run("hll-athena", 0.3, 0.0, 1e9, max_steps=4)
+        out = g.download()
+    np.testing.assert_array_equal(out, ref)
+
+
+def test_autotune_keeps_results(gpu):
+    from pamhd_amd import solver as S
+    cells = (32, 24, 16)
+    state, length, gamma, mu0 = _vortex(cells, seed=12)
+    og = O.make_grid(cells, length, gamma, mu0)
+    ref = state.copy()
+    st, steps_o, t_o = O.run_periodic(og, O.HLLD, 0.3, 0.0, 1e9, ref, max_steps=5)
+    with S.Grid(cells, length, gamma, mu0) as g:
+        g.upload(state)
+        report = g.autotune("hlld-athena")
+        print(report, flush=True)
+        assert "chosen" in report
+        g.run("hlld-athena", 0.3, 0.0, 1e9, max_steps=5)
+        out = g.download()
+    np.testing.assert_array_equal(out, ref)
+
+
 @pytest.mark.parametrize("case", sorted(G.GOLDENS))
 def test_goldens_on_gpu(gpu, case):
     """The reference's uniform-grid regression runs, on the GPU: bit-identical
