@@ -133,65 +133,91 @@ __device__ __forceinline__ void reduce_speed(StepStatus* st, int d, double s, un
 
 /*
 Flux pass: get_fluxes + get_flux of source/mhd/solve.hpp:68-188, :304-393.
-One thread per face: logical block (bx, by, slab) with slab = d * nz + k; the
-thread's cell (i, j, k) owns the d-face on its negative side, between
-L = cell - e_d and R = cell.  Stores dt * F (global frame) as PT.
+One launch per face direction D (compile-time, so the rotation of
+get_rotated_vector is register renaming); one thread per face: the thread's
+cell (i, j, k) owns the D-face on its negative side, between L = cell - e_D
+and R = cell.  Stores dt * F (global frame) as PT.  Offsets are 32-bit
+(n < 2^28 cells) from per-array base pointers.
 */
-template <int SOLVER, class T, class PT>
-__global__ __launch_bounds__(BX * BY) void flux_pass_kernel(GridDev g,
-	const double* __restrict__ s, const double* __restrict__ bg,
-	PT* __restrict__ P, const double* __restrict__ dtp, StepStatus* st,
-	unsigned gx, unsigned gy, int remap)
+struct StateArrays {
+	const double* q[8];
+};
+
+template <int D>
+__device__ __forceinline__ void load_face_state_d(const StateArrays& S, const GridDev& g,
+	int i, int j, int k, double U[8])
+{
+	const int c = i + g.nx * (j + g.ny * k);
+	const int cx = wrap_inc(i, g.nx) + g.nx * (j + g.ny * k);
+	const int cy = i + g.nx * (wrap_inc(j, g.ny) + g.ny * k);
+	const int cz = i + g.nx * (j + g.ny * wrap_inc(k, g.nz));
+	const double m[3] = {S.q[1][c], S.q[2][c], S.q[3][c]};
+	const double B[3] = {
+		0.5 * (S.q[5][c] + S.q[5][cx]),
+		0.5 * (S.q[6][c] + S.q[6][cy]),
+		0.5 * (S.q[7][c] + S.q[7][cz])};
+	U[RHO] = S.q[0][c];
+	U[NRJ] = S.q[4][c];
+	U[M0] = m[D];
+	U[M1] = m[(D + 1) % 3];
+	U[M2] = m[(D + 2) % 3];
+	U[B0] = B[D];
+	U[B1] = B[(D + 1) % 3];
+	U[B2] = B[(D + 2) % 3];
+}
+
+template <int SOLVER, class T, class PT, int D>
+__global__ __launch_bounds__(BX * BY) void flux_pass_kernel(GridDev g, StateArrays S,
+	const double* __restrict__ bg, PT* __restrict__ P, const double* __restrict__ dtp,
+	StepStatus* st, unsigned gx, unsigned gy, int remap)
 {
 	const unsigned lb = remap ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
-	const unsigned bx = lb % gx, by = (lb / gx) % gy, slab = lb / (gx * gy);
+	const unsigned bx = lb % gx, by = (lb / gx) % gy, k = lb / (gx * gy);
 	const int i = bx * BX + threadIdx.x;
 	const int j = by * BY + threadIdx.y;
-	const int d = slab / g.nz;
-	const int k = slab - d * g.nz;
-	const long long n = g.n;
+	const int n = (int)g.n;
 	double sp = 0;
 	unsigned zero = 0;
 	if (i < g.nx && j < g.ny) {
-		const long long c = i + (long long)g.nx * (j + (long long)g.ny * k);
-		const int il = d == 0 ? wrap_dec(i, g.nx) : i;
-		const int jl = d == 1 ? wrap_dec(j, g.ny) : j;
-		const int kl = d == 2 ? wrap_dec(k, g.nz) : k;
+		const int c = i + g.nx * (j + g.ny * (int)k);
 		double L[8], R[8];
-		load_face_state(s, n, g, d, il, jl, kl, L);
-		load_face_state(s, n, g, d, i, j, k, R);
+		load_face_state_d<D>(S, g, D == 0 ? wrap_dec(i, g.nx) : i, D == 1 ? wrap_dec(j, g.ny) : j,
+			D == 2 ? wrap_dec((int)k, g.nz) : (int)k, L);
+		load_face_state_d<D>(S, g, i, j, (int)k, R);
 		T b[3] = {0, 0, 0};
 		if (bg) {
-			const double bgl[3] = {bg[(3 * d + 0) * n + c], bg[(3 * d + 1) * n + c], bg[(3 * d + 2) * n + c]};
-			b[0] = (T)bgl[d];
-			b[1] = (T)bgl[(d + 1) % 3];
-			b[2] = (T)bgl[(d + 2) % 3];
+			const double* bgd = bg + (size_t)(3 * D) * g.n;
+			b[0] = (T)bgd[(size_t)D * g.n + c];
+			b[1] = (T)bgd[(size_t)((D + 1) % 3) * g.n + c];
+			b[2] = (T)bgd[(size_t)((D + 2) % 3) * g.n + c];
 		}
 		T F[8], sT = 0;
 		const int e = riemann<SOLVER, T>(L, R, b, g.ph, F, sT);
 		double Fg[8];
 		if (e) {
-			record_error(st, e, (long long)d * n + c);
+			record_error(st, e, (long long)D * g.n + c);
 			for (int q = 0; q < 8; q++) Fg[q] = 0;
 			sT = 0;
 		} else {
 			Fg[RHO] = (double)F[RHO];
 			Fg[NRJ] = (double)F[NRJ];
 			/* rotate back to the global frame */
-			Fg[M0 + d] = (double)F[M0];
-			Fg[M0 + (d + 1) % 3] = (double)F[M1];
-			Fg[M0 + (d + 2) % 3] = (double)F[M2];
-			Fg[B0 + d] = (double)F[B0];
-			Fg[B0 + (d + 1) % 3] = (double)F[B1];
-			Fg[B0 + (d + 2) % 3] = (double)F[B2];
+			Fg[M0 + D] = (double)F[M0];
+			Fg[M0 + (D + 1) % 3] = (double)F[M1];
+			Fg[M0 + (D + 2) % 3] = (double)F[M2];
+			Fg[B0 + D] = (double)F[B0];
+			Fg[B0 + (D + 1) % 3] = (double)F[B1];
+			Fg[B0 + (D + 2) % 3] = (double)F[B2];
 		}
 		const double dt = *dtp;
-		PT* Pd = P + (size_t)d * 8 * n;
-		for (int q = 0; q < 8; q++) Pd[(size_t)q * n + c] = (PT)(dt * Fg[q]);
+		PT* Pd = P + (size_t)D * 8 * g.n;
+#pragma unroll
+		for (int q = 0; q < 8; q++) (Pd + (size_t)q * g.n)[c] = (PT)(dt * Fg[q]);
 		sp = (double)sT;
 		zero = sp <= 0 ? 1u : 0u;
+		(void)n;
 	}
-	reduce_speed(st, d, sp, zero);
+	reduce_speed(st, D, sp, zero);
 }
 
 /* constant-thermal-pressure energy refit of update_vol_B,
@@ -760,27 +786,29 @@ dim3 cell_grid(const GridDev& g, int slabs = 1)
 	return dim3((g.nx + BX - 1) / BX, (g.ny + BY - 1) / BY, g.nz * slabs);
 }
 
+template <int SOLVER, class T, class PT>
+void launch_flux_dirs(pamhd_handle* h, hipStream_t st, StepStatus* stat)
+{
+	const double* s = h->state[h->cur];
+	StateArrays S;
+	for (int q = 0; q < 8; q++) S.q[q] = s + (size_t)q * h->g.n;
+	PT* P = reinterpret_cast<PT*>(h->flux);
+	const unsigned gx = (h->g.nx + BX - 1) / BX, gy = (h->g.ny + BY - 1) / BY;
+	const dim3 grid(gx * gy * (unsigned)h->g.nz), block(BX, BY, 1);
+	const int r = h->tune.flux_remap;
+	flux_pass_kernel<SOLVER, T, PT, 0><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r);
+	flux_pass_kernel<SOLVER, T, PT, 1><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r);
+	flux_pass_kernel<SOLVER, T, PT, 2><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r);
+}
+
 template <class T, class PT>
 int launch_flux_pass(pamhd_handle* h, int solver, hipStream_t st, StepStatus* stat)
 {
-	const double* s = h->state[h->cur];
-	PT* P = reinterpret_cast<PT*>(h->flux);
-	const unsigned gx = (h->g.nx + BX - 1) / BX, gy = (h->g.ny + BY - 1) / BY;
-	const unsigned nblocks = gx * gy * (unsigned)h->g.nz * 3;
-	const dim3 grid(nblocks), block(BX, BY, 1);
 	switch (solver) {
-	case RUSANOV:
-		flux_pass_kernel<RUSANOV, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy, h->tune.flux_remap);
-		break;
-	case HLL_ATHENA:
-		flux_pass_kernel<HLL_ATHENA, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy, h->tune.flux_remap);
-		break;
-	case HLLD_ATHENA:
-		flux_pass_kernel<HLLD_ATHENA, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy, h->tune.flux_remap);
-		break;
-	case ROE_ATHENA:
-		flux_pass_kernel<ROE_ATHENA, T, PT><<<grid, block, 0, st>>>(h->g, s, h->bg, P, h->dt, stat, gx, gy, h->tune.flux_remap);
-		break;
+	case RUSANOV: launch_flux_dirs<RUSANOV, T, PT>(h, st, stat); break;
+	case HLL_ATHENA: launch_flux_dirs<HLL_ATHENA, T, PT>(h, st, stat); break;
+	case HLLD_ATHENA: launch_flux_dirs<HLLD_ATHENA, T, PT>(h, st, stat); break;
+	case ROE_ATHENA: launch_flux_dirs<ROE_ATHENA, T, PT>(h, st, stat); break;
 	default:
 		return fail(h, PAMHD_E_SOLVER, "invalid solver " + std::to_string(solver));
 	}
@@ -888,7 +916,7 @@ int pamhd_create(const pamhd_grid_desc* grid, const pamhd_physics* physics, int 
 	if (!(physics->vacuum_permeability > 0)) return fail(nullptr, PAMHD_E_ARG, "vacuum permeability must be > 0");
 	if (precision != PAMHD_FP64 && precision != PAMHD_MIXED) return fail(nullptr, PAMHD_E_ARG, "invalid precision");
 	const long long n = (long long)grid->cells[0] * grid->cells[1] * grid->cells[2];
-	if ((long long)grid->cells[2] * 3 > 65535) return fail(nullptr, PAMHD_E_UNSUPPORTED, "nz too large");
+	if (n >= (1ll << 28)) return fail(nullptr, PAMHD_E_UNSUPPORTED, "more than 2^28 cells per grid");
 
 	pamhd_handle* h = new pamhd_handle();
 	h->desc = *grid;
