@@ -1367,3 +1367,315 @@ int or_set_threads(int n)
 	return 1;
 #endif
 }
+
+/* ---------------------------------------------------------------------- */
+/* divergence cleaning: periodic Poisson projection of the face field     */
+/* ---------------------------------------------------------------------- */
+
+/*
+Replaces divergence::remove (source/divergence/remove.hpp:609-769, whose
+Poisson solve is dccrg's, not present in the reference checkout) for the
+staggered field: rhs = face divergence per cell, solve the periodic
+7-point problem L phi = rhs by geometric multigrid, then
+B_face -= face gradient of phi, so the new face divergence is the solver's
+residual.  Levels halve every dimension that is even and >= 4; smoothing is
+red-black Gauss-Seidel when every dimension of the level is even (colours
+are then consistent across the periodic wrap), damped Jacobi (omega 0.8)
+otherwise; restriction averages the children, prolongation is trilinear
+in the coarsened dimensions.  Only dimensions whose spacing is within 1.5x
+of the level's finest are coarsened (semi-coarsening for anisotropic cells).
+*/
+typedef struct {
+	long n[3];
+	double h[3];
+	long ncell;
+	double* phi;
+	double* f;
+	double* r;
+} mg_level;
+
+static inline long mg_idx(const mg_level* L, long i, long j, long k)
+{
+	return wrap(i, L->n[0]) + L->n[0] * (wrap(j, L->n[1]) + L->n[1] * wrap(k, L->n[2]));
+}
+
+/* (L phi)(i,j,k); dimensions of length 1 contribute nothing */
+static double mg_apply(const mg_level* L, const double* phi, long i, long j, long k)
+{
+	const double c = phi[mg_idx(L, i, j, k)];
+	double s = 0;
+	if (L->n[0] > 1) s += (phi[mg_idx(L, i + 1, j, k)] - 2 * c + phi[mg_idx(L, i - 1, j, k)]) / (L->h[0] * L->h[0]);
+	if (L->n[1] > 1) s += (phi[mg_idx(L, i, j + 1, k)] - 2 * c + phi[mg_idx(L, i, j - 1, k)]) / (L->h[1] * L->h[1]);
+	if (L->n[2] > 1) s += (phi[mg_idx(L, i, j, k + 1)] - 2 * c + phi[mg_idx(L, i, j, k - 1)]) / (L->h[2] * L->h[2]);
+	return s;
+}
+
+/* new value of phi at (i,j,k) from its neighbours: (sum nb/h^2 - f) / diag */
+static double mg_relax_value(const mg_level* L, const double* phi, long i, long j, long k)
+{
+	double s = 0, diag = 0;
+	if (L->n[0] > 1) {
+		s += (phi[mg_idx(L, i + 1, j, k)] + phi[mg_idx(L, i - 1, j, k)]) / (L->h[0] * L->h[0]);
+		diag += 2 / (L->h[0] * L->h[0]);
+	}
+	if (L->n[1] > 1) {
+		s += (phi[mg_idx(L, i, j + 1, k)] + phi[mg_idx(L, i, j - 1, k)]) / (L->h[1] * L->h[1]);
+		diag += 2 / (L->h[1] * L->h[1]);
+	}
+	if (L->n[2] > 1) {
+		s += (phi[mg_idx(L, i, j, k + 1)] + phi[mg_idx(L, i, j, k - 1)]) / (L->h[2] * L->h[2]);
+		diag += 2 / (L->h[2] * L->h[2]);
+	}
+	if (diag == 0) return phi[mg_idx(L, i, j, k)];
+	return (s - L->f[mg_idx(L, i, j, k)]) / diag;
+}
+
+static int mg_rb_ok(const mg_level* L)
+{
+	for (int d = 0; d < 3; d++) if (L->n[d] > 1 && L->n[d] % 2) return 0;
+	return 1;
+}
+
+static void mg_smooth(mg_level* L, int sweeps)
+{
+	if (mg_rb_ok(L)) {
+		for (int s = 0; s < sweeps; s++) {
+			for (int color = 0; color < 2; color++) {
+				#pragma omp parallel for collapse(2) schedule(static)
+				for (long k = 0; k < L->n[2]; k++)
+				for (long j = 0; j < L->n[1]; j++)
+				for (long i = 0; i < L->n[0]; i++) {
+					if (((i + j + k) & 1) != color) continue;
+					L->phi[mg_idx(L, i, j, k)] = mg_relax_value(L, L->phi, i, j, k);
+				}
+			}
+		}
+	} else {
+		for (int s = 0; s < sweeps; s++) {
+			#pragma omp parallel for collapse(2) schedule(static)
+			for (long k = 0; k < L->n[2]; k++)
+			for (long j = 0; j < L->n[1]; j++)
+			for (long i = 0; i < L->n[0]; i++) {
+				const long c = mg_idx(L, i, j, k);
+				L->r[c] = L->phi[c] + 0.8 * (mg_relax_value(L, L->phi, i, j, k) - L->phi[c]);
+			}
+			memcpy(L->phi, L->r, sizeof(double) * L->ncell);
+		}
+	}
+}
+
+static void mg_residual(mg_level* L)
+{
+	#pragma omp parallel for collapse(2) schedule(static)
+	for (long k = 0; k < L->n[2]; k++)
+	for (long j = 0; j < L->n[1]; j++)
+	for (long i = 0; i < L->n[0]; i++) {
+		const long c = mg_idx(L, i, j, k);
+		L->r[c] = L->f[c] - mg_apply(L, L->phi, i, j, k);
+	}
+}
+
+/* coarse f = average of the fine residual over the children */
+static void mg_restrict(const mg_level* F, mg_level* C)
+{
+	long ratio[3];
+	for (int d = 0; d < 3; d++) ratio[d] = F->n[d] / C->n[d];
+	const double w = 1.0 / (double)(ratio[0] * ratio[1] * ratio[2]);
+	#pragma omp parallel for collapse(2) schedule(static)
+	for (long k = 0; k < C->n[2]; k++)
+	for (long j = 0; j < C->n[1]; j++)
+	for (long i = 0; i < C->n[0]; i++) {
+		double s = 0;
+		for (long c = 0; c < ratio[2]; c++)
+		for (long b = 0; b < ratio[1]; b++)
+		for (long a = 0; a < ratio[0]; a++) {
+			s += F->r[mg_idx(F, i * ratio[0] + a, j * ratio[1] + b, k * ratio[2] + c)];
+		}
+		C->f[mg_idx(C, i, j, k)] = s * w;
+		C->phi[mg_idx(C, i, j, k)] = 0;
+	}
+}
+
+/* fine phi += trilinear interpolation of the coarse phi; in a coarsened
+   dimension a child at fine index 2I + a takes 3/4 of coarse cell I and 1/4
+   of its neighbour on the child's side */
+static void mg_prolong_add(mg_level* F, const mg_level* C)
+{
+	long ratio[3];
+	for (int d = 0; d < 3; d++) ratio[d] = F->n[d] / C->n[d];
+	#pragma omp parallel for collapse(2) schedule(static)
+	for (long k = 0; k < F->n[2]; k++)
+	for (long j = 0; j < F->n[1]; j++)
+	for (long i = 0; i < F->n[0]; i++) {
+		const long idx[3] = {i, j, k};
+		long ci[3][2];
+		double cw[3][2];
+		int cnt[3];
+		for (int d = 0; d < 3; d++) {
+			if (ratio[d] == 2) {
+				const long I = idx[d] / 2, side = (idx[d] % 2) ? 1 : -1;
+				ci[d][0] = I;
+				cw[d][0] = 0.75;
+				ci[d][1] = I + side;
+				cw[d][1] = 0.25;
+				cnt[d] = 2;
+			} else {
+				ci[d][0] = idx[d];
+				cw[d][0] = 1.0;
+				cnt[d] = 1;
+			}
+		}
+		double s = 0;
+		for (int c = 0; c < cnt[2]; c++)
+		for (int b = 0; b < cnt[1]; b++)
+		for (int a = 0; a < cnt[0]; a++) {
+			s += cw[0][a] * cw[1][b] * cw[2][c] * C->phi[mg_idx(C, ci[0][a], ci[1][b], ci[2][c])];
+		}
+		F->phi[mg_idx(F, i, j, k)] += s;
+	}
+}
+
+static void mg_vcycle(mg_level* lv, int l, int nlev)
+{
+	if (l == nlev - 1) {
+		mg_smooth(&lv[l], 60);
+		return;
+	}
+	mg_smooth(&lv[l], 2);
+	mg_residual(&lv[l]);
+	mg_restrict(&lv[l], &lv[l + 1]);
+	mg_vcycle(lv, l + 1, nlev);
+	mg_prolong_add(&lv[l], &lv[l + 1]);
+	mg_smooth(&lv[l], 2);
+}
+
+/* 2-norm of a - mean(a): the mean lies in the periodic Laplacian's null
+   space, so it is excluded from the convergence test */
+static double l2_centred(const double* a, long n)
+{
+	double m = 0;
+	for (long c = 0; c < n; c++) m += a[c];
+	m /= (double)n;
+	double s = 0;
+	for (long c = 0; c < n; c++) s += (a[c] - m) * (a[c] - m);
+	return sqrt(s);
+}
+
+int or_poisson_periodic(const int dims[3], const double h[3], const double* rhs,
+	double* phi, double tol, double atol, int max_cycles, double* rel_residual,
+	int* cycles)
+{
+	mg_level lv[32];
+	int nlev = 0;
+	long n[3] = {dims[0], dims[1], dims[2]};
+	double hh[3] = {h[0], h[1], h[2]};
+	for (;;) {
+		mg_level* L = &lv[nlev++];
+		for (int d = 0; d < 3; d++) {
+			L->n[d] = n[d];
+			L->h[d] = hh[d];
+		}
+		L->ncell = n[0] * n[1] * n[2];
+		L->phi = (double*)calloc(L->ncell, sizeof(double));
+		L->f = (double*)calloc(L->ncell, sizeof(double));
+		L->r = (double*)calloc(L->ncell, sizeof(double));
+		/* semi-coarsening: halve the even dimensions whose spacing is within
+		   1.5x of the finest active spacing, so coarse levels tend to
+		   isotropy and point smoothing keeps working */
+		double hmin = 1e300;
+		for (int d = 0; d < 3; d++) if (n[d] > 1 && hh[d] < hmin) hmin = hh[d];
+		int coarsened = 0;
+		for (int d = 0; d < 3; d++) {
+			if (n[d] % 2 == 0 && n[d] >= 4 && hh[d] <= 1.5 * hmin) {
+				n[d] /= 2;
+				hh[d] *= 2;
+				coarsened = 1;
+			}
+		}
+		if (!coarsened || L->ncell <= 64 || nlev == 32) break;
+	}
+	mg_level* F = &lv[0];
+	/* the face divergence of a periodic field sums to zero by telescoping;
+	   its roundoff-sized mean is left in the data (it only drifts phi by a
+	   constant) and excluded from the norms of the convergence test */
+	for (long c = 0; c < F->ncell; c++) F->f[c] = rhs[c];
+	const double fnorm = l2_centred(F->f, F->ncell);
+	const double target = tol * fnorm > atol ? tol * fnorm : atol;
+	double rel = 0, rnorm = fnorm;
+	int it = 0;
+	if (fnorm > target) {
+		for (it = 1; it <= max_cycles; it++) {
+			mg_vcycle(lv, 0, nlev);
+			mg_residual(F);
+			rnorm = l2_centred(F->r, F->ncell);
+			if (rnorm <= target) break;
+		}
+		if (it > max_cycles) it = max_cycles;
+	}
+	rel = fnorm > 0 ? rnorm / fnorm : 0;
+	memcpy(phi, F->phi, sizeof(double) * F->ncell);
+	for (int l = 0; l < nlev; l++) {
+		free(lv[l].phi);
+		free(lv[l].f);
+		free(lv[l].r);
+	}
+	if (rel_residual) *rel_residual = rel;
+	if (cycles) *cycles = it;
+	return rnorm > target ? OR_ERR_SOLVER : OR_OK;
+}
+
+int or_clean_divergence(const or_grid* g, double* s, double tol, int max_cycles,
+	double* rel_residual, int* cycles)
+{
+	const dims_t D = {g->nx, g->ny, g->nz, (long)g->nx * g->ny * g->nz};
+	const long n = D.n;
+	double* rhs = (double*)calloc(n, sizeof(double));
+	double* phi = (double*)calloc(n, sizeof(double));
+	double* vb_old = (double*)calloc(3 * n, sizeof(double));
+	for (long k = 0; k < D.nz; k++)
+	for (long j = 0; j < D.ny; j++)
+	for (long i = 0; i < D.nx; i++) {
+		const long c = cid(&D, i, j, k);
+		rhs[c] = (s[5 * n + cid(&D, i + 1, j, k)] - s[5 * n + c]) / g->dx
+			+ (s[6 * n + cid(&D, i, j + 1, k)] - s[6 * n + c]) / g->dy
+			+ (s[7 * n + cid(&D, i, j, k + 1)] - s[7 * n + c]) / g->dz;
+		double vb[3];
+		vol_b(&D, s, i, j, k, vb);
+		for (int q = 0; q < 3; q++) vb_old[q * n + c] = vb[q];
+	}
+	const int dims[3] = {g->nx, g->ny, g->nz};
+	const double h[3] = {g->dx, g->dy, g->dz};
+	/* roundoff floor of the face divergence: 8 eps || sum_d 2|B_d| / h_d ||_2 */
+	double fl = 0;
+	for (long c = 0; c < n; c++) {
+		const double t = 2 * fabs(s[5 * n + c]) / g->dx + 2 * fabs(s[6 * n + c]) / g->dy
+			+ 2 * fabs(s[7 * n + c]) / g->dz;
+		fl += t * t;
+	}
+	const double atol = 8 * DBL_EPSILON * sqrt(fl);
+	const int e = or_poisson_periodic(dims, h, rhs, phi, tol, atol, max_cycles, rel_residual, cycles);
+	/* B_face -= grad phi on faces; dimensions of length 1 have no gradient */
+	for (long k = 0; k < D.nz; k++)
+	for (long j = 0; j < D.ny; j++)
+	for (long i = 0; i < D.nx; i++) {
+		const long c = cid(&D, i, j, k);
+		if (D.nx > 1) s[5 * n + c] -= (phi[c] - phi[cid(&D, i - 1, j, k)]) / g->dx;
+		if (D.ny > 1) s[6 * n + c] -= (phi[c] - phi[cid(&D, i, j - 1, k)]) / g->dy;
+		if (D.nz > 1) s[7 * n + c] -= (phi[c] - phi[cid(&D, i, j, k - 1)]) / g->dz;
+	}
+	/* constant-thermal-pressure energy refit for the new cell field */
+	for (long k = 0; k < D.nz; k++)
+	for (long j = 0; j < D.ny; j++)
+	for (long i = 0; i < D.nx; i++) {
+		const long c = cid(&D, i, j, k);
+		double vb_new[3];
+		vol_b(&D, s, i, j, k, vb_new);
+		const double vo[3] = {vb_old[c], vb_old[n + c], vb_old[2 * n + c]};
+		const double m[3] = {s[n + c], s[2 * n + c], s[3 * n + c]};
+		refit_energy(s[c], m, &s[4 * n + c], vo, vb_new, g->gamma, g->mu0);
+	}
+	free(rhs);
+	free(phi);
+	free(vb_old);
+	return e;
+}

@@ -146,6 +146,25 @@ Runs from t = 0 until t >= t_end with the reference's lagged dt.
 int or_plain1d_run(const or_plain1d* p, int solver, double t_end, double* state,
 	long* steps_taken);
 
+/*
+Periodic 7-point Poisson solve L phi = rhs by geometric multigrid V-cycles
+until ||residual - mean||_2 <= max(tol ||rhs - mean||_2, atol) (or
+max_cycles); rhs must have zero mean up to roundoff (a face divergence does).
+Divergence cleaning passes atol = 8 eps ||sum_d 2|B_d|/h_d||_2, the roundoff
+floor of the face divergence.
+Returns 0, or OR_ERR_SOLVER when the tolerance was not reached.
+*/
+int or_poisson_periodic(const int dims[3], const double h[3], const double* rhs,
+	double* phi, double tol, double atol, int max_cycles, double* rel_residual,
+	int* cycles);
+
+/*
+Divergence cleaning of the face field (in place): Poisson solve on the face
+divergence, face-gradient projection, constant-pressure energy refit.
+*/
+int or_clean_divergence(const or_grid* g, double* state, double tol, int max_cycles,
+	double* rel_residual, int* cycles);
+
 /* OpenMP threads of the grid step (n > 0 sets it); returns the count in use */
 int or_set_threads(int n);
 

@@ -86,6 +86,13 @@ def lib():
     L.or_plain1d_run.argtypes = [ctypes.POINTER(OrPlain1d), ctypes.c_int,
                                  ctypes.c_double, _dp, _lp]
     L.or_plain1d_run.restype = ctypes.c_int
+    L.or_poisson_periodic.argtypes = [ctypes.POINTER(ctypes.c_int), _dp, _dp, _dp,
+                                      ctypes.c_double, ctypes.c_double, ctypes.c_int, _dp,
+                                      ctypes.POINTER(ctypes.c_int)]
+    L.or_poisson_periodic.restype = ctypes.c_int
+    L.or_clean_divergence.argtypes = [ctypes.POINTER(OrGrid), _dp, ctypes.c_double,
+                                      ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int)]
+    L.or_clean_divergence.restype = ctypes.c_int
     L.or_set_threads.argtypes = [ctypes.c_int]
     L.or_set_threads.restype = ctypes.c_int
     _lib = L
@@ -210,3 +217,25 @@ def ref_flux(name, L, R, bg=(0.0, 0.0, 0.0), gamma=5.0 / 3.0, mu0=1.0):
     st = getattr(lib_, "ref_flux_" + name)(_ptr(l), _ptr(r), _ptr(b), gamma, mu0,
                                            _ptr(F), _ptr(s))
     return st, F, float(s[0])
+
+
+def poisson(dims, h, rhs, tol=1e-10, max_cycles=60, atol=0.0):
+    rhs = np.ascontiguousarray(rhs, dtype=np.float64)
+    phi = np.zeros_like(rhs)
+    d = (ctypes.c_int * 3)(*dims)
+    hh = np.ascontiguousarray(h, dtype=np.float64)
+    res = ctypes.c_double(0)
+    cyc = ctypes.c_int(0)
+    st = lib().or_poisson_periodic(d, _ptr(hh), _ptr(rhs), _ptr(phi), tol, atol, max_cycles,
+                                   ctypes.byref(res), ctypes.byref(cyc))
+    return st, phi, res.value, cyc.value
+
+
+def clean_divergence(grid, state, tol=1e-10, max_cycles=60):
+    """In place on state (8, n); returns (status, relative residual, cycles)."""
+    assert state.flags.c_contiguous and state.dtype == np.float64
+    res = ctypes.c_double(0)
+    cyc = ctypes.c_int(0)
+    st = lib().or_clean_divergence(ctypes.byref(grid), _ptr(state), tol, max_cycles,
+                                   ctypes.byref(res), ctypes.byref(cyc))
+    return st, res.value, cyc.value
