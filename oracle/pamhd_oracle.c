@@ -989,10 +989,12 @@ int or_step_periodic(const or_grid* g, int solver, double dt, int refits,
 			face_state(&D, in, d, i - (d == 0), j - (d == 1), k - (d == 2), L);
 			face_state(&D, in, d, i, j, k, R);
 			if (g->bg) {
+				/* Bg_B(L)(+d): the face is the positive d-face of L */
+				const long cl = cid(&D, i - (d == 0), j - (d == 1), k - (d == 2));
 				const double b[3] = {
-					g->bg[(3 * d + 0) * n + c],
-					g->bg[(3 * d + 1) * n + c],
-					g->bg[(3 * d + 2) * n + c]};
+					g->bg[(3 * d + 0) * n + cl],
+					g->bg[(3 * d + 1) * n + cl],
+					g->bg[(3 * d + 2) * n + cl]};
 				rot_to_face(d, b, bgr);
 			}
 			const int e = or_flux(solver, L, R, bgr, g->gamma, g->mu0, F, &s);

@@ -186,10 +186,14 @@ __global__ __launch_bounds__(BX * BY) void flux_pass_kernel(GridDev g, StateArra
 		load_face_state_d<D>(S, g, i, j, (int)k, R);
 		T b[3] = {0, 0, 0};
 		if (bg) {
+			/* Bg_B(L)(+D): the face is the positive D-face of L */
+			const int cl = D == 0 ? wrap_dec(i, g.nx) + g.nx * (j + g.ny * (int)k)
+				: D == 1 ? i + g.nx * (wrap_dec(j, g.ny) + g.ny * (int)k)
+				: i + g.nx * (j + g.ny * wrap_dec((int)k, g.nz));
 			const double* bgd = bg + (size_t)(3 * D) * g.n;
-			b[0] = (T)bgd[(size_t)D * g.n + c];
-			b[1] = (T)bgd[(size_t)((D + 1) % 3) * g.n + c];
-			b[2] = (T)bgd[(size_t)((D + 2) % 3) * g.n + c];
+			b[0] = (T)bgd[(size_t)D * g.n + cl];
+			b[1] = (T)bgd[(size_t)((D + 1) % 3) * g.n + cl];
+			b[2] = (T)bgd[(size_t)((D + 2) % 3) * g.n + cl];
 		}
 		T F[8], sT = 0;
 		const int e = riemann<SOLVER, T>(L, R, b, g.ph, F, sT);

@@ -90,6 +90,28 @@ def test_grid_steps_fp64_bit_exact(gpu, name, sid):
     np.testing.assert_array_equal(out, ref)
 
 
+@pytest.mark.parametrize("name,sid", SOLVERS[:2])
+def test_grid_steps_face_varying_background(gpu, name, sid):
+    """A random background field that differs on every face (Rusanov and HLL
+    use it): each face reads Bg_B of its negative-side cell's +d face."""
+    from pamhd_amd import solver as S
+    cells = (12, 10, 8)
+    state, length, gamma, mu0 = _vortex(cells, seed=21)
+    n = cells[0] * cells[1] * cells[2]
+    bg = np.random.default_rng(5).uniform(-0.2, 0.2, size=(9, n))
+    og = O.make_grid(cells, length, gamma, mu0, bg)
+    ref = state.copy()
+    st, steps_o, t_o = O.run_periodic(og, sid, 0.3, 0.0, 1e9, ref, max_steps=4)
+    assert st == 0
+    with S.Grid(cells, length, gamma, mu0) as g:
+        g.upload(state)
+        g.set_background_faces(bg)
+        steps_g, t_g = g.run(name, 0.3, 0.0, 1e9, max_steps=4)
+        out = g.download()
+    assert steps_g == steps_o and t_g == t_o
+    np.testing.assert_array_equal(out, ref)
+
+
 @pytest.mark.parametrize("variant", [(0, 0, 0), (1, 0, 1), (0, 4, 0), (1, 8, 1), (0, 32, 1)])
 def test_launch_variants_bit_exact(gpu, variant):
     """Every launch variant the autotuner may pick (block order, per-cell or

@@ -385,3 +385,46 @@ def test_step_invariant_under_axis_cycle(sid):
     scale = np.abs(a2).max(1, keepdims=True)
     assert np.abs(b - a2).max() < 1e-12 * scale.max()
     assert abs(t - t2) < 1e-14 * t
+
+
+@pytest.mark.parametrize("sid", [O.RUSANOV, O.HLL])
+def test_step_matches_literal_face_loop(sid):
+    """The grid step against a literal per-face loop of get_fluxes
+    (source/mhd/solve.hpp:68-188, :304-393): L = cell, R = its +d neighbour,
+    bg = Bg_B(L)(+d), states rotated into the face frame, fluxes rotated back,
+    then U += dt (F(-d face) - F(+d face)) / len_d for rho, m, E.  A random,
+    face-varying background field pins which cell's faces it is read from."""
+    cells = (4, 3, 5)
+    nx, ny, nz = cells
+    n = nx * ny * nz
+    state, length, gamma, mu0 = _smooth_state(cells, 3)
+    rng = np.random.default_rng(11)
+    bg = rng.uniform(-0.3, 0.3, size=(9, n))
+    g = O.make_grid(cells, length, gamma, mu0, bg)
+    dt = 1e-3
+    st, new, sm, zc = O.step(g, sid, dt, state, refits=0)
+    assert st == 0
+    s = state.reshape(8, nz, ny, nx)
+    vb = np.stack([0.5 * (s[5] + np.roll(s[5], -1, 2)),
+                   0.5 * (s[6] + np.roll(s[6], -1, 1)),
+                   0.5 * (s[7] + np.roll(s[7], -1, 0))])
+    expect = state[:5].reshape(5, nz, ny, nx).copy()
+    smax = []
+    for d in range(3):
+        rot = [d, (d + 1) % 3, (d + 2) % 3]
+        ax = 2 - d                      # numpy axis of dimension d
+        U = np.concatenate([s[0:1], s[1:4][rot], s[4:5], vb[rot]])
+        L = U.reshape(8, n)
+        R = np.roll(U, -1, ax + 1).reshape(8, n)
+        b = bg[3 * d:3 * d + 3][rot]
+        F, sp, stt = O.flux_batch(sid, L, R, b, gamma, mu0)
+        assert not stt.any()
+        smax.append(sp.max())
+        Fg = np.empty((5, n))
+        Fg[0], Fg[4] = F[0], F[4]
+        for q in range(3):
+            Fg[1 + rot[q]] = F[1 + q]
+        Fg = Fg.reshape(5, nz, ny, nx)
+        expect += dt * (np.roll(Fg, 1, ax + 1) - Fg) / length[d]
+    np.testing.assert_allclose(new[:5], expect.reshape(5, n), rtol=1e-13, atol=1e-15)
+    np.testing.assert_array_equal(sm, smax)
