@@ -21,6 +21,8 @@ a drop-in for the reference's MHD step:
     (source/mhd/boundaries.hpp:1249-1297)
   pamhd::math::get_divergence_face2volume         pamhd_div_face2volume
     (source/math/nabla.hpp:58-113)
+  pamhd::divergence::remove                       pamhd_clean_divergence
+    (source/divergence/remove.hpp:609-769)
 
 Conventions
   * All functions return 0 on success or a PAMHD_E* code; no exception
@@ -74,6 +76,7 @@ enum pamhd_status {
 	PAMHD_E_ROE_DENSITY = 6,   /* Roe intermediate-state density */
 	PAMHD_E_ROE_PRESSURE = 7,  /* Roe intermediate-state pressure */
 	PAMHD_E_SOLVER = 8,        /* unknown solver */
+	PAMHD_E_POISSON = 9,       /* divergence cleaning: tolerance not reached */
 	PAMHD_E_ARG = 16,          /* invalid argument */
 	PAMHD_E_UNSUPPORTED = 17,  /* valid request this build does not implement */
 	PAMHD_E_DEVICE = 18,       /* HIP runtime failure (allocation, launch) */
@@ -154,6 +157,31 @@ int pamhd_check(pamhd_handle* h);
 
 /* Mean |div B| of the face field over all cells (synchronous). */
 int pamhd_div_face2volume(pamhd_handle* h, double* avg_div);
+
+/*
+Divergence cleaning of the face field, the role of pamhd::divergence::remove
+(source/divergence/remove.hpp:609-769) on a periodic grid: solves the
+periodic 7-point Poisson problem L phi = div(B_face) by multigrid V-cycles
+until the residual's 2-norm (mean removed) is <= tol times the right-hand
+side's, or at the roundoff floor 8 eps ||sum_d 2|B_d|/len_d||_2, or
+max_cycles cycles; then B_face -= face gradient of phi, so the new face
+divergence is the Poisson residual, and E is refit at constant thermal
+pressure (update_vol_B, source/mhd/solve.hpp:1646-1672).  Outputs (each may
+be NULL): div_before = mean |div B| before cleaning (what remove() returns),
+rel_residual, cycles.  Returns PAMHD_E_POISSON when the tolerance was not
+reached (the field is still projected).  Synchronises the stream.
+The reference computes the divergence of the cell-centred field with central
+differences and delegates the solve to dccrg's Poisson solver; this build's
+state holds face fields, so it projects those (the constrained-transport
+step then keeps the face divergence at roundoff).
+*/
+int pamhd_clean_divergence(pamhd_handle* h, double tol, int max_cycles,
+	double* div_before, double* rel_residual, int* cycles, void* stream);
+
+/* Smoother of the Poisson solve: 1 = LDS-tiled fused red-black sweeps where
+   every dimension of a level is even and >= 4 (default), 0 = one launch per
+   colour.  Both give identical results.  (No counterpart in the reference.) */
+int pamhd_set_poisson_smoother(pamhd_handle* h, int fused);
 
 /* Largest face signal speed per dimension of the last flux pass (synchronous). */
 int pamhd_max_signal_speeds(pamhd_handle* h, double smax[3]);

@@ -13,12 +13,14 @@ LIB_PATH = os.path.join(PKG, "lib", "libpamhd_mi355x.so")
 # reference solver numbering, source/mhd/common.hpp:54-61
 RUSANOV, RUSANOV_STAGGERED, HLL_ATHENA, HLLD_ATHENA, ROE_ATHENA, HYBRID = range(6)
 FP64, MIXED = 0, 1
+PAMHD_E_POISSON = 9
 
 EXPORTS = (
     "pamhd_abi_version", "pamhd_create", "pamhd_destroy", "pamhd_upload_state",
     "pamhd_download_state", "pamhd_state_device_ptr", "pamhd_set_background_faces",
     "pamhd_timestep", "pamhd_apply_boundaries", "pamhd_step", "pamhd_run",
-    "pamhd_check", "pamhd_div_face2volume", "pamhd_max_signal_speeds",
+    "pamhd_check", "pamhd_div_face2volume", "pamhd_clean_divergence",
+    "pamhd_set_poisson_smoother", "pamhd_max_signal_speeds",
     "pamhd_face_flux_batch", "pamhd_autotune", "pamhd_set_launch", "pamhd_enable_timing", "pamhd_kernel_time",
     "pamhd_last_error",
 )
@@ -73,6 +75,9 @@ def load():
                             ctypes.POINTER(ctypes.c_long), _dp, _vp]
     L.pamhd_check.argtypes = [H]
     L.pamhd_div_face2volume.argtypes = [H, _dp]
+    L.pamhd_clean_divergence.argtypes = [H, ctypes.c_double, ctypes.c_int, _dp, _dp,
+                                         ctypes.POINTER(ctypes.c_int), _vp]
+    L.pamhd_set_poisson_smoother.argtypes = [H, ctypes.c_int]
     L.pamhd_max_signal_speeds.argtypes = [H, _dp]
     L.pamhd_face_flux_batch.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_size_t,
                                         _vp, _vp, _vp, ctypes.c_double, ctypes.c_double,

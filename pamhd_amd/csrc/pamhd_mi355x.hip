@@ -22,6 +22,7 @@ energy refits).  In mixed precision the flux array holds FP32 values.
 #include <hip/hip_runtime.h>
 
 #include <cfloat>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -29,6 +30,7 @@ energy refits).  In mixed precision the flux array holds FP32 values.
 #include <vector>
 
 #include "pamhd_mi355x.h"
+#include "poisson.hpp"
 #include "riemann.hpp"
 
 using namespace pamhd_dev;
@@ -645,6 +647,75 @@ __global__ __launch_bounds__(BX * BY) void div_kernel(GridDev g, const double* _
 	}
 }
 
+/* right-hand side of the cleaning Poisson problem: the face divergence of
+   each cell, and the squared roundoff scale sum_d 2|B_d|/len_d of its faces */
+__global__ __launch_bounds__(BX * BY) void clean_rhs_kernel(GridDev g, const double* __restrict__ s,
+	double* __restrict__ rhs, double* __restrict__ scale2)
+{
+	const int i = blockIdx.x * BX + threadIdx.x;
+	const int j = blockIdx.y * BY + threadIdx.y;
+	const int k = blockIdx.z;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	const long long c = i + (long long)g.nx * (j + (long long)g.ny * k);
+	const long long cx = wrap_inc(i, g.nx) + (long long)g.nx * (j + (long long)g.ny * k);
+	const long long cy = i + (long long)g.nx * (wrap_inc(j, g.ny) + (long long)g.ny * k);
+	const long long cz = i + (long long)g.nx * (j + (long long)g.ny * wrap_inc(k, g.nz));
+	rhs[c] = (s[5 * n + cx] - s[5 * n + c]) / g.dx
+		+ (s[6 * n + cy] - s[6 * n + c]) / g.dy
+		+ (s[7 * n + cz] - s[7 * n + c]) / g.dz;
+	const double t = 2 * fabs(s[5 * n + c]) / g.dx + 2 * fabs(s[6 * n + c]) / g.dy
+		+ 2 * fabs(s[7 * n + c]) / g.dz;
+	scale2[c] = t * t;
+}
+
+/* projection B_face -= grad phi (a dimension of length 1 has no gradient)
+   and constant-pressure energy refit of each cell, from the old state into
+   the new one; the refit needs the new field of the cell's positive faces,
+   which is recomputed here from phi */
+__global__ __launch_bounds__(BX * BY) void project_kernel(GridDev g, const double* __restrict__ s,
+	double* __restrict__ o, const double* __restrict__ phi)
+{
+	const int i = blockIdx.x * BX + threadIdx.x;
+	const int j = blockIdx.y * BY + threadIdx.y;
+	const int k = blockIdx.z;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	auto id = [&](int a, int b, int e) { return a + (long long)g.nx * (b + (long long)g.ny * e); };
+	const long long c = id(i, j, k);
+	const int ip = wrap_inc(i, g.nx), jp = wrap_inc(j, g.ny), kp = wrap_inc(k, g.nz);
+	const int im = wrap_dec(i, g.nx), jm = wrap_dec(j, g.ny), km = wrap_dec(k, g.nz);
+	const double pc = phi[c];
+	double b[3] = {s[5 * n + c], s[6 * n + c], s[7 * n + c]};
+	double bp[3] = {s[5 * n + id(ip, j, k)], s[6 * n + id(i, jp, k)], s[7 * n + id(i, j, kp)]};
+	const double vb_old[3] = {0.5 * (b[0] + bp[0]), 0.5 * (b[1] + bp[1]), 0.5 * (b[2] + bp[2])};
+	if (g.nx > 1) {
+		b[0] -= (pc - phi[id(im, j, k)]) / g.dx;
+		bp[0] -= (phi[id(ip, j, k)] - pc) / g.dx;
+	}
+	if (g.ny > 1) {
+		b[1] -= (pc - phi[id(i, jm, k)]) / g.dy;
+		bp[1] -= (phi[id(i, jp, k)] - pc) / g.dy;
+	}
+	if (g.nz > 1) {
+		b[2] -= (pc - phi[id(i, j, km)]) / g.dz;
+		bp[2] -= (phi[id(i, j, kp)] - pc) / g.dz;
+	}
+	const double vb_new[3] = {0.5 * (b[0] + bp[0]), 0.5 * (b[1] + bp[1]), 0.5 * (b[2] + bp[2])};
+	const double rho = s[c];
+	const double m[3] = {s[n + c], s[2 * n + c], s[3 * n + c]};
+	double E = s[4 * n + c];
+	refit_energy(rho, m, E, vb_old, vb_new, g.ph);
+	o[c] = rho;
+	o[n + c] = m[0];
+	o[2 * n + c] = m[1];
+	o[3 * n + c] = m[2];
+	o[4 * n + c] = E;
+	o[5 * n + c] = b[0];
+	o[6 * n + c] = b[1];
+	o[7 * n + c] = b[2];
+}
+
 /* batched Riemann solver for the pamhd_face_flux_batch unit-test surface */
 template <int SOLVER, class T>
 __global__ __launch_bounds__(256) void face_batch_kernel(size_t n, const double* __restrict__ Lp,
@@ -723,6 +794,8 @@ struct pamhd_handle {
 	std::vector<Timed> timed;
 	double kernel_ms[4] = {0, 0, 0, 0};
 	long kernel_launches[4] = {0, 0, 0, 0};
+	PoissonMG* mg = nullptr;  /* multigrid hierarchy, allocated on first cleaning */
+	int mg_fused = 1;
 	int cur = 0;    /* state buffer holding the current state */
 	int last = 0;   /* status entry of the last flux pass */
 	std::string err;
@@ -970,6 +1043,10 @@ int pamhd_destroy(pamhd_handle* h)
 	hipFree(h->dt);
 	hipFree(h->status);
 	hipFree(h->scratch);
+	if (h->mg) {
+		mg_destroy(*h->mg);
+		delete h->mg;
+	}
 	hipHostFree(h->host_dt);
 	hipHostFree(h->host_status);
 	delete h;
@@ -1079,6 +1156,60 @@ int pamhd_div_face2volume(pamhd_handle* h, double* avg)
 	for (size_t b = 0; b < nb; b++) total += host[b];
 	delete[] host;
 	*avg = total / (double)h->g.n;
+	return PAMHD_OK;
+}
+
+int pamhd_clean_divergence(pamhd_handle* h, double tol, int max_cycles, double* div_before,
+	double* rel_residual, int* cycles, void* stream_)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (!(tol >= 0) || max_cycles < 0) return fail(h, PAMHD_E_ARG, "tol and max_cycles must be >= 0");
+	hipStream_t st = (hipStream_t)stream_;
+	if (!h->mg) {
+		h->mg = new PoissonMG();
+		const int dims[3] = {h->g.nx, h->g.ny, h->g.nz};
+		const double len[3] = {h->g.dx, h->g.dy, h->g.dz};
+		const hipError_t e = mg_create(*h->mg, dims, len);
+		if (e != hipSuccess) {
+			mg_destroy(*h->mg);
+			delete h->mg;
+			h->mg = nullptr;
+			return hip_fail(h, e, "multigrid levels");
+		}
+	}
+	PoissonMG& mg = *h->mg;
+	mg.fused_smoother = h->mg_fused;
+	MgLevel& F = mg.lv[0];
+	const int n = (int)h->g.n;
+	clean_rhs_kernel<<<cell_grid(h->g), dim3(BX, BY, 1), 0, st>>>(h->g, h->state[h->cur], F.f, F.r);
+	HIP_TRY(h, hipGetLastError());
+	double fl, dsum;
+	HIP_TRY(h, mg_sum(mg, st, F.r, n, 0, 0.0, &fl));
+	HIP_TRY(h, mg_sum(mg, st, F.f, n, 2, 0.0, &dsum));
+	const double atol = 8 * DBL_EPSILON * std::sqrt(fl);
+	double rel = 0;
+	int it = 0;
+	bool ok = true;
+	HIP_TRY(h, mg_solve(mg, st, tol, atol, max_cycles, &rel, &it, &ok));
+	project_kernel<<<cell_grid(h->g), dim3(BX, BY, 1), 0, st>>>(h->g, h->state[h->cur],
+		h->state[1 - h->cur], F.phi);
+	HIP_TRY(h, hipGetLastError());
+	HIP_TRY(h, hipStreamSynchronize(st));
+	h->cur = 1 - h->cur;
+	if (div_before) *div_before = dsum / (double)n;
+	if (rel_residual) *rel_residual = rel;
+	if (cycles) *cycles = it;
+	if (!ok) {
+		return fail(h, PAMHD_E_POISSON, "divergence cleaning: relative residual " + std::to_string(rel) +
+			" after " + std::to_string(it) + " V-cycles");
+	}
+	return PAMHD_OK;
+}
+
+int pamhd_set_poisson_smoother(pamhd_handle* h, int fused)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	h->mg_fused = fused != 0;
 	return PAMHD_OK;
 }
 

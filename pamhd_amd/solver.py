@@ -84,6 +84,9 @@ class Grid:
     def _check(self, code):
         return _lib.check(code, self._h)
 
+    def last_error(self):
+        return _lib.last_error(self._h)
+
     # --- state ----------------------------------------------------------
     def upload(self, state):
         state = np.ascontiguousarray(state, dtype=np.float64)
@@ -167,6 +170,22 @@ class Grid:
         v = ctypes.c_double(0)
         self._check(self._L.pamhd_div_face2volume(self._h, ctypes.byref(v)))
         return v.value
+
+    def clean_divergence(self, tol=1e-10, max_cycles=50, stream=None, check=True):
+        """Multigrid divergence cleaning of the face field (the role of
+        pamhd::divergence::remove, source/divergence/remove.hpp:609-769).
+        Returns (mean |div B| before, relative residual, V-cycles, status);
+        with check=True a tolerance not reached raises PamhdError."""
+        before, rel, cyc = ctypes.c_double(0), ctypes.c_double(0), ctypes.c_int(0)
+        st = self._L.pamhd_clean_divergence(self._h, tol, max_cycles, ctypes.byref(before),
+                                            ctypes.byref(rel), ctypes.byref(cyc),
+                                            stream)
+        if check:
+            self._check(st)
+        return before.value, rel.value, cyc.value, st
+
+    def set_poisson_smoother(self, fused):
+        self._check(self._L.pamhd_set_poisson_smoother(self._h, 1 if fused else 0))
 
     def max_signal_speeds(self):
         v = (ctypes.c_double * 3)()
