@@ -81,6 +81,7 @@ def run_gpu(prec, state, cells, length, gamma, mu0, solver, cfl, steps, warmup):
     import torch
     from pamhd_amd.solver import Grid
     n = cells[0] * cells[1] * cells[2]
+    final = None
     with Grid(cells, length, gamma, mu0, precision=prec) as g:
         g.upload(state)
         stream = torch.cuda.current_stream().cuda_stream
@@ -105,8 +106,107 @@ def run_gpu(prec, state, cells, length, gamma, mu0, solver, cfl, steps, warmup):
         g.check()
         kt = {k: g.kernel_time(k) for k in ("dt", "flux_pass", "update")}
         speeds = g.max_signal_speeds()
+        final = g.download()
     return {"ms": ms, "value": n * steps / (ms / 1e3), "kernels": kt, "speeds": speeds,
-            "tuning": tuning}
+            "tuning": tuning, "final": final}
+
+
+def rel_l2(a, b):
+    """Relative L2 error per conserved variable of a against b; momentum and
+    face field as vectors (their z components are roundoff-sized in this
+    quasi-2-D state)."""
+    import numpy as np
+    out = {}
+    for name, rows in (("rho", [0]), ("m", [1, 2, 3]), ("E", [4]), ("B", [5, 6, 7])):
+        out[name] = float(np.linalg.norm(a[rows] - b[rows]) / np.linalg.norm(b[rows]))
+    return out
+
+
+def divergence_cleaning(cells, cycles_max=50):
+    """Times one multigrid divergence cleaning of the vortex state with a
+    seeded Gaussian (sigma 1e-3) divergence-carrying face-field perturbation:
+    V(2,2) cycles to a 1e-10 relative residual, projection and energy refit."""
+    import torch
+    from pamhd_amd import configs
+    from pamhd_amd.solver import Grid
+    state, length, gamma, mu0 = configs.orszag_tang_vortex(cells, noise=0.01, seed=2024,
+                                                           divergence_sigma=1e-3)
+    res = {}
+    with Grid(cells, length, gamma, mu0) as g:
+        for fused in (True, False):
+            g.upload(state)
+            g.set_poisson_smoother(fused)
+            g.clean_divergence(tol=1e-10, max_cycles=cycles_max)     # warm-up, allocation
+            g.upload(state)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            before, rel, cyc, st = g.clean_divergence(tol=1e-10, max_cycles=cycles_max)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3
+            after = g.div_face2volume()
+            res["fused" if fused else "per_colour"] = {
+                "ms": round(ms, 3), "v_cycles": cyc, "rel_residual": rel,
+                "div_before": before, "div_after": after}
+    return res
+
+
+def ensemble_run(nbox, steps, solver="hll-athena", cfl=0.3, seed=2024):
+    """Config of the ensemble workload: `nbox` solar-wind boxes with seeded
+    edges in {32, 48, 64, 96, 128}, HLL, `steps` main-loop steps after the
+    final-init step.  Times the ensemble (one launch per stage for all boxes,
+    steps replayed from a captured HIP graph, autotuned tile shape) against a
+    loop over the boxes with one handle each (plain launches), on the same
+    stream; results of the two are compared box by box."""
+    import numpy as np
+    import torch
+    from pamhd_amd import configs
+    from pamhd_amd.swbox import BoxGrid, Ensemble
+    boxes = configs.solar_wind_ensemble(nbox=nbox, seed=seed)
+    stream = torch.cuda.current_stream().cuda_stream
+    t0 = time.perf_counter()
+    with Ensemble(boxes) as e:
+        log("  ensemble of %d boxes, %d cells, set up in %.1f s"
+            % (nbox, e.cells, time.perf_counter() - t0))
+        e.timestep(solver, 0.0, cfl, stream=stream)
+        tune = e.autotune(solver, cfl, stream=stream)
+        log("  " + tune)
+        e.run(solver, cfl, 2, use_graph=False, stream=stream)     # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e.run(solver, cfl, steps, use_graph=True, stream=stream)
+        torch.cuda.synchronize()
+        ens_s = time.perf_counter() - t0
+        dts = e.check(stream=stream)
+        probe = {q: e.download(q) for q in (0, nbox - 1)}
+    normal = 0
+    grids = []
+    for sb in boxes:
+        g = BoxGrid(sb)
+        ct, _ = g.cell_types()
+        normal += int((ct == 1).sum())
+        grids.append(g)
+    for g in grids:
+        g.timestep(solver, 0.0, cfl, stream=stream)
+        for _ in range(2):
+            g.step(solver, 1e300, cfl, sync=False, stream=stream)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        for g in grids:
+            g.step(solver, 1e300, cfl, sync=False, stream=stream)
+    torch.cuda.synchronize()
+    loop_s = time.perf_counter() - t0
+    same = all(np.array_equal(grids[q].download(), probe[q]) for q in probe)
+    for g in grids:
+        g.close()
+    return {"boxes": nbox, "cells": int(sum(b.n for b in boxes)), "normal_cells": normal,
+            "steps": steps, "solver": solver, "cfl": cfl,
+            "ensemble_s": round(ens_s, 4), "per_box_loop_s": round(loop_s, 4),
+            "ensemble_cell_updates_per_s": normal * steps / ens_s,
+            "per_box_loop_cell_updates_per_s": normal * steps / loop_s,
+            "speedup_vs_per_box_loop": loop_s / ens_s,
+            "tuning": tune.split("; chosen ")[-1], "graph": True,
+            "results_identical": bool(same), "min_dt": float(dts.min())}
 
 
 def roofline(res, n):
@@ -136,6 +236,9 @@ def main():
     ap.add_argument("--cfl", type=float, default=0.3)
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--ensemble-boxes", type=int, default=64)
+    ap.add_argument("--ensemble-steps", type=int, default=500,
+                    help="0 skips the ensemble workload")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
@@ -171,6 +274,19 @@ def main():
         log("  %s: %.3f ms/step, %.3e cell-updates/s" % (prec, results[prec]["ms"] / args.steps,
                                                           results[prec]["value"]))
     del state
+    err = rel_l2(results["mixed"].pop("final"), results["fp64"].pop("final"))
+    log("  mixed vs fp64 relative L2 after %d steps: %s" % (args.warmup + args.steps, err))
+    log("divergence cleaning, %d^3" % args.size)
+    clean = divergence_cleaning(cells)
+    log("  %s" % clean)
+
+    ens = None
+    if args.ensemble_steps > 0:
+        log("ensemble: %d solar-wind boxes, %d steps" % (args.ensemble_boxes, args.ensemble_steps))
+        ens = ensemble_run(args.ensemble_boxes, args.ensemble_steps)
+        log("  ensemble %.3e cell-updates/s, per-box loop %.3e, x%.2f, identical %s"
+            % (ens["ensemble_cell_updates_per_s"], ens["per_box_loop_cell_updates_per_s"],
+               ens["speedup_vs_per_box_loop"], ens["results_identical"]))
 
     base = None
     if not args.no_cpu_baseline:
@@ -203,7 +319,10 @@ def main():
         "mixed": {"value": mix["value"], "ms_per_step": mix["ms"] / args.steps,
                   "dtype": "f32 flux/wave speeds, f64 state",
                   "speedup_vs_fp64": mix["value"] / f64["value"],
+                  "rel_l2_vs_fp64": err,
                   "roofline": roofline(mix, n)},
+        "divergence_cleaning": clean,
+        "ensemble": ens,
     }
     print(json.dumps(out), flush=True)
 

@@ -96,7 +96,33 @@ typedef struct pamhd_physics {
 	double proton_mass;          /* "proton-mass" */
 } pamhd_physics;
 
+/*
+Solar-wind box description, the options of tests/mhd/solar_wind_box.cpp
+(source/solar_wind_box_options.hpp:50-390 and the "background-magnetic-field"
+object of source/background_magnetic_field.hpp): one solar-wind state, the
+inner (planet) boundary and a constant field plus point dipoles.
+*/
+#define PAMHD_MAX_DIPOLES 8
+typedef struct pamhd_solar_wind_desc {
+	int32_t direction;             /* "direction": -1 "-x", +1 "+x", -2, +2, -3, +3 */
+	double number_density;         /* "number-density" (1/m^3) */
+	double velocity[3];            /* "velocity" (m/s) */
+	double pressure;               /* "pressure" (Pa) */
+	double magnetic_field[3];      /* "magnetic-field" (T); component along the
+	                                  inflow direction must be 0 */
+	double inner_radius;           /* "inner-boundary": "radius" (m) */
+	double inner_number_density;
+	double inner_velocity[3];
+	double inner_pressure;
+	double bg_value[3];            /* "background-magnetic-field": "value" */
+	double bg_min_distance;        /* "min-distance" */
+	int32_t n_dipoles;             /* <= PAMHD_MAX_DIPOLES */
+	double dipole_moment[PAMHD_MAX_DIPOLES][3];
+	double dipole_position[PAMHD_MAX_DIPOLES][3];
+} pamhd_solar_wind_desc;
+
 typedef struct pamhd_handle pamhd_handle;
+typedef struct pamhd_ensemble pamhd_ensemble;
 
 /* Library/ABI version, for loaders: returns 1. */
 int pamhd_abi_version(void);
@@ -224,6 +250,71 @@ benchmark for the roofline figure.)
 int pamhd_enable_timing(pamhd_handle* h, int enable);
 int pamhd_kernel_time(pamhd_handle* h, const char* kernel, double* total_ms,
 	long* launches);
+
+/* ---- solar-wind box (tests/mhd/solar_wind_box.cpp) ---- */
+
+/*
+Creates a non-periodic uniform grid (grid->periodic all 0, >= 4 cells per
+dimension, the outer layer of cells being the boundary) set up as the
+reference's solar-wind program does before its final-init step: cell
+classification (prepare_grid, source/grid/solar_wind_box.hpp:193-406),
+background field at face centres, initialize_plasma
+(source/mhd/solar_wind_box.hpp:62-205), update_vol_B at constant pressure,
+apply_boundaries_sw_box and sync_magnetic_field
+(tests/mhd/solar_wind_box.cpp:430-461).  FP64.  On the handle:
+  pamhd_timestep        timestep() (source/mhd/solve.hpp:1837-1924)
+  pamhd_step            timestep() + apply_boundaries_sw_box + sync, the
+                        main-loop step of tests/mhd/solar_wind_box.cpp:514-553
+  pamhd_apply_boundaries  apply_boundaries_sw_box + sync
+  pamhd_run             final-init timestep(0) then pamhd_step to t_end
+  pamhd_upload/download_state  14 * n doubles: rho, m (3), E, Vol_B (3),
+                        the cell's own copies of its -x, +x, -y, +y, -z, +z
+                        face B (the reference keeps one copy per cell)
+The time step is global ("substep-max": 0).
+*/
+int pamhd_create_solar_wind_box(const pamhd_grid_desc* grid, const pamhd_physics* physics,
+	const pamhd_solar_wind_desc* solar_wind, pamhd_handle** out);
+
+/* values per cell of pamhd_upload/download_state: 8 (periodic) or 14 (box) */
+int pamhd_state_components(pamhd_handle* h, int* components);
+
+/* Cell_Type (1 normal, 0 boundary, -1 not solved) and the boundary rule of
+   each cell (1..27: copy from the neighbour at offset (o0, o1, o2) with code
+   1 + (o0+1) + 3 (o1+1) + 9 (o2+1); 28 planet; 29 inflow); either may be NULL */
+int pamhd_box_cell_types(pamhd_handle* h, signed char* cell_type, unsigned char* rule);
+
+/* Bg_B(cell)(+d) of a box, 9 * n doubles as pamhd_set_background_faces */
+int pamhd_box_background(pamhd_handle* h, double* bg);
+
+/*
+Ensemble of solar-wind boxes of different sizes advanced together: every
+stage of a step is one launch over all boxes, each box with its own lagged
+time step.  (No counterpart in the reference, which runs one box per
+program.)
+  create       as pamhd_create_solar_wind_box for each box
+  timestep     timestep() of every box (the final init uses max_dt = 0)
+  run          `steps` main-loop steps of every box (no time limit); with
+               use_graph the steps are replayed from a captured HIP graph
+  check        synchronises; per-box time step of the last step and the
+               first physics error
+  autotune     picks the fastest tile shape of the cell kernels (64x4, 32x8,
+               16x16) on a copy of the state; set_tile sets it
+*/
+int pamhd_ensemble_create(int nbox, const pamhd_grid_desc* grids, const pamhd_physics* physics,
+	const pamhd_solar_wind_desc* solar_winds, pamhd_ensemble** out);
+int pamhd_ensemble_destroy(pamhd_ensemble* e);
+int pamhd_ensemble_timestep(pamhd_ensemble* e, int solver, double max_time_step,
+	double time_step_factor, void* stream);
+int pamhd_ensemble_run(pamhd_ensemble* e, int solver, double time_step_factor, long steps,
+	int use_graph, void* stream);
+int pamhd_ensemble_check(pamhd_ensemble* e, double* dt_per_box, void* stream);
+int pamhd_ensemble_download(pamhd_ensemble* e, int box, double* state);
+int pamhd_ensemble_cells(pamhd_ensemble* e, int box, long long* cells);
+int pamhd_ensemble_set_tile(pamhd_ensemble* e, int tile);
+int pamhd_ensemble_autotune(pamhd_ensemble* e, int solver, double time_step_factor,
+	char* buf, size_t buflen, void* stream);
+/* last error of e, or of the last failed pamhd_ensemble_create when e is NULL */
+int pamhd_ensemble_last_error(pamhd_ensemble* e, char* buf, size_t n);
 
 /* Copies the last error message of h (or of the library when h is NULL). */
 int pamhd_last_error(pamhd_handle* h, char* buf, size_t n);

@@ -1681,3 +1681,532 @@ int or_clean_divergence(const or_grid* g, double* s, double tol, int max_cycles,
 	free(vb_old);
 	return e;
 }
+
+/* ====================================================================== */
+/* Solar-wind box: non-periodic uniform grid with a one-cell boundary     */
+/* layer, inflow wall, outflow copies, planet cells and a dipole field    */
+/* ====================================================================== */
+
+/*
+State of a box cell, 14 doubles (component-major, 14 * n), the reference's
+per-cell record for this path (source/mhd/variables.hpp:212-234):
+    0 rho, 1-3 m, 4 E, 5-7 Vol_B, 8-13 Face_B(-x, +x, -y, +y, -z, +z)
+Each cell keeps its own copy of each of its six faces, as the reference
+does, so the averaging of sync_magnetic_field is restated copy by copy.
+*/
+#define BQ_RHO 0
+#define BQ_M 1
+#define BQ_E 4
+#define BQ_VB 5
+#define BQ_F 8
+
+static inline long bid(const or_box* b, long i, long j, long k)
+{
+	return i + b->nx * (j + (long)b->ny * k);
+}
+
+static inline int in_box(const or_box* b, long i, long j, long k)
+{
+	return i >= 0 && j >= 0 && k >= 0 && i < b->nx && j < b->ny && k < b->nz;
+}
+
+/* cell min / max / centre coordinates of a level-0 cell */
+static inline double cmin(const or_box* b, int d, long idx) { return b->start[d] + (double)idx * b->len[d]; }
+static inline double cmax(const or_box* b, int d, long idx) { return b->start[d] + (double)(idx + 1) * b->len[d]; }
+static inline double ccen(const or_box* b, int d, long idx) { return b->start[d] + ((double)idx + 0.5) * b->len[d]; }
+
+/* at_inner_boundary, source/grid/solar_wind_box.hpp:66-117: (inside, outside) */
+static void at_inner_boundary(const or_box* b, long i, long j, long k, int* inside, int* outside)
+{
+	const double sx = cmin(b, 0, i), sy = cmin(b, 1, j), sz = cmin(b, 2, k);
+	const double ex = cmax(b, 0, i), ey = cmax(b, 1, j), ez = cmax(b, 2, k);
+	const double r[8] = {
+		sqrt(sx * sx + sy * sy + sz * sz), sqrt(sx * sx + sy * sy + ez * ez),
+		sqrt(sx * sx + ey * ey + sz * sz), sqrt(sx * sx + ey * ey + ez * ez),
+		sqrt(ex * ex + sy * sy + sz * sz), sqrt(ex * ex + sy * sy + ez * ez),
+		sqrt(ex * ex + ey * ey + sz * sz), sqrt(ex * ex + ey * ey + ez * ez)};
+	const double R = b->inner_radius;
+	int in = 0, out = 0;
+	for (int q = 0; q < 8; q++) {
+		if (r[q] <= R) in = 1;
+		if (r[q] > R) out = 1;
+	}
+	if (sx <= 0 && ex >= 0 && sy <= 0 && ey >= 0 && sz <= 0 && ez >= 0) in = 1;
+	*inside = in;
+	*outside = out;
+}
+
+int or_box_classify(const or_box* b, signed char* ctype, unsigned char* rule)
+{
+	const long n = (long)b->nx * b->ny * b->nz;
+	const int sw_dim = abs(b->sw_dir) - 1;
+	const long dims[3] = {b->nx, b->ny, b->nz};
+	if (b->sw_dir == 0 || sw_dim > 2) return OR_ERR_SOLVER;
+	for (long c = 0; c < n; c++) {
+		ctype[c] = 1;
+		rule[c] = OR_RULE_NONE;
+	}
+	/* prepare_grid, source/grid/solar_wind_box.hpp:262-367: inflow wall,
+	   then face / edge / vertex cells of the outflow walls, each copying
+	   from its interior neighbour (offset opposite to its wall sides) */
+	for (long k = 0; k < b->nz; k++)
+	for (long j = 0; j < b->ny; j++)
+	for (long i = 0; i < b->nx; i++) {
+		const long idx[3] = {i, j, k};
+		const long c = bid(b, i, j, k);
+		int side[3];
+		for (int d = 0; d < 3; d++) {
+			side[d] = idx[d] == 0 ? -1 : (idx[d] == dims[d] - 1 ? +1 : 0);
+			if (dims[d] == 1) side[d] = 0;
+		}
+		if (side[0] || side[1] || side[2]) {
+			ctype[c] = 0;
+			rule[c] = (unsigned char)(1 + (1 - side[0]) + 3 * (1 - side[1]) + 9 * (1 - side[2]));
+		}
+		const int sws = b->sw_dir < 0 ? 0 : (int)(dims[sw_dim] - 1);
+		if (idx[sw_dim] == sws) {
+			ctype[c] = 0;
+			rule[c] = OR_RULE_SW;
+		}
+	}
+	/* planet cells, :369-406 */
+	for (long k = 0; k < b->nz; k++)
+	for (long j = 0; j < b->ny; j++)
+	for (long i = 0; i < b->nx; i++) {
+		int in, out;
+		at_inner_boundary(b, i, j, k, &in, &out);
+		if (in) {
+			ctype[bid(b, i, j, k)] = 0;
+			rule[bid(b, i, j, k)] = OR_RULE_PLANET;
+		}
+	}
+	for (long k = 0; k < b->nz; k++)
+	for (long j = 0; j < b->ny; j++)
+	for (long i = 0; i < b->nx; i++) {
+		const long c = bid(b, i, j, k);
+		if (rule[c] != OR_RULE_PLANET) continue;
+		int have_normal = 0;
+		/* face and edge neighbours: offsets with one or two non-zero parts */
+		for (int dz = -1; dz <= 1; dz++)
+		for (int dy = -1; dy <= 1; dy++)
+		for (int dx = -1; dx <= 1; dx++) {
+			const int nz_ = (dx != 0) + (dy != 0) + (dz != 0);
+			if (nz_ == 0 || nz_ == 3) continue;
+			if (!in_box(b, i + dx, j + dy, k + dz)) continue;
+			if (ctype[bid(b, i + dx, j + dy, k + dz)] == 1) have_normal = 1;
+		}
+		if (!have_normal) {
+			ctype[c] = -1;
+			rule[c] = OR_RULE_NONE;
+		}
+	}
+	return OR_OK;
+}
+
+/* Background_Magnetic_Field::get_background_field for a constant field and
+   point dipoles, source/background_magnetic_field.hpp:331-494 */
+static void bg_field(const double pos[3], const double constant[3], double min_distance,
+	int ndip, const double* moment, const double* dpos, double mu0, double out[3])
+{
+	double r3[3] = {constant[0], constant[1], constant[2]};
+	for (int q = 0; q < ndip; q++) {
+		const double* m = moment + 3 * q;
+		const double r[3] = {pos[0] + (-dpos[3 * q]), pos[1] + (-dpos[3 * q + 1]), pos[2] + (-dpos[3 * q + 2])};
+		const double r1 = sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+		if (r1 < min_distance) {
+			const double f = mu0 / (2 * M_PI * pow(min_distance, 3));
+			for (int d = 0; d < 3; d++) r3[d] = r3[d] + m[d] * f;
+		} else {
+			const double inv = 1 / r1;
+			const double u[3] = {r[0] * inv, r[1] * inv, r[2] * inv};
+			const double dp = m[0] * u[0] + m[1] * u[1] + m[2] * u[2];
+			const double f = mu0 / (4 * M_PI) / (r1 * r1 * r1);
+			for (int d = 0; d < 3; d++) {
+				const double proj = dp * u[d];
+				r3[d] = r3[d] + (3 * proj + (-m[d])) * f;
+			}
+		}
+	}
+	out[0] = r3[0];
+	out[1] = r3[1];
+	out[2] = r3[2];
+}
+
+/* Bg_B(cell)(+d) at the centre of each cell's positive faces,
+   source/mhd/solar_wind_box.hpp:120-171 */
+void or_box_background(const or_box* b, const double constant[3], double min_distance,
+	int ndip, const double* moment, const double* dpos, double* bg)
+{
+	const long n = (long)b->nx * b->ny * b->nz;
+	for (long k = 0; k < b->nz; k++)
+	for (long j = 0; j < b->ny; j++)
+	for (long i = 0; i < b->nx; i++) {
+		const long c = bid(b, i, j, k);
+		const long idx[3] = {i, j, k};
+		for (int d = 0; d < 3; d++) {
+			double p[3];
+			for (int e = 0; e < 3; e++) p[e] = ccen(b, e, idx[e]);
+			p[d] = cmax(b, d, idx[d]);
+			double v[3];
+			bg_field(p, constant, min_distance, ndip, moment, dpos, b->mu0, v);
+			for (int e = 0; e < 3; e++) bg[(3 * d + e) * n + c] = v[e];
+		}
+	}
+}
+
+static int box_pressure(const or_box* b, const double* s, long n, long c, double* p)
+{
+	const double U[8] = {s[c], s[n + c], s[2 * n + c], s[3 * n + c], s[4 * n + c],
+		s[5 * n + c], s[6 * n + c], s[7 * n + c]};
+	return or_get_pressure(U, b->gamma, b->mu0, p);
+}
+
+/* update_vol_B with constant thermal pressure for one cell,
+   source/mhd/solve.hpp:1646-1672 */
+static void box_refit(const or_box* b, double* s, long n, long c)
+{
+	if (s[c] <= 0) return;
+	double p;
+	box_pressure(b, s, n, c, &p);
+	for (int d = 0; d < 3; d++) {
+		s[(BQ_VB + d) * n + c] = 0.5 * (s[(BQ_F + 2 * d) * n + c] + s[(BQ_F + 2 * d + 1) * n + c]);
+	}
+	const double inv = 1 / s[c];
+	const double v[3] = {s[n + c] * inv, s[2 * n + c] * inv, s[3 * n + c] * inv};
+	const double B[3] = {s[5 * n + c], s[6 * n + c], s[7 * n + c]};
+	s[4 * n + c] = or_total_energy(s[c], v, p, B, b->gamma, b->mu0);
+}
+
+/* sync_magnetic_field, source/common_functions.hpp:184-269: the two copies
+   of each face between cells of type >= 0 are averaged, visited first from
+   the cell with the smaller index, then from the other */
+void or_box_sync(const or_box* b, double* s)
+{
+	const long n = (long)b->nx * b->ny * b->nz;
+	for (long k = 0; k < b->nz; k++)
+	for (long j = 0; j < b->ny; j++)
+	for (long i = 0; i < b->nx; i++) {
+		const long c = bid(b, i, j, k);
+		if (b->ctype[c] < 0) continue;
+		for (int d = 0; d < 3; d++) {
+			const long ii = i + (d == 0), jj = j + (d == 1), kk = k + (d == 2);
+			if (!in_box(b, ii, jj, kk)) continue;
+			const long nb = bid(b, ii, jj, kk);
+			if (b->ctype[nb] < 0) continue;
+			double* a = &s[(BQ_F + 2 * d + 1) * n + c];
+			double* o = &s[(BQ_F + 2 * d) * n + nb];
+			const double diff = *a - *o;
+			*a -= diff / 2;
+			*o += diff / 2;
+			const double diff2 = *o - *a;
+			*o -= diff2 / 2;
+			*a += diff2 / 2;
+		}
+	}
+}
+
+/* apply_boundaries_sw_box + apply_solar_wind_boundaries,
+   source/mhd/solar_wind_box.hpp:219-577 (one solar-wind state) */
+int or_box_boundaries(const or_box* b, double* s)
+{
+	const long n = (long)b->nx * b->ny * b->nz;
+	const int sw_dim = abs(b->sw_dir) - 1;
+	/* outflow copies (face, edge and vertex cells): they read only normal
+	   cells, so their order does not matter */
+	for (long k = 0; k < b->nz; k++)
+	for (long j = 0; j < b->ny; j++)
+	for (long i = 0; i < b->nx; i++) {
+		const long c = bid(b, i, j, k);
+		const int r = b->rule[c];
+		if (r < 1 || r > 27) continue;
+		const int o[3] = {(r - 1) % 3 - 1, ((r - 1) / 3) % 3 - 1, (r - 1) / 9 - 1};
+		const long src = bid(b, i + o[0], j + o[1], k + o[2]);
+		if (b->ctype[src] != 1) return OR_ERR_SOLVER;
+		double p;
+		const int e = box_pressure(b, s, n, src, &p);
+		if (e) return e;
+		s[c] = s[src];
+		for (int q = 1; q <= 3; q++) s[q * n + c] = s[q * n + src];
+		for (int d = 0; d < 3; d++) {
+			const double lo = s[(BQ_F + 2 * d) * n + src], hi = s[(BQ_F + 2 * d + 1) * n + src];
+			/* mirrored across the walls the cell touches, copied along the others */
+			s[(BQ_F + 2 * d) * n + c] = o[d] ? hi : lo;
+			s[(BQ_F + 2 * d + 1) * n + c] = o[d] ? lo : hi;
+		}
+		for (int d = 0; d < 3; d++) {
+			s[(BQ_VB + d) * n + c] = 0.5 * (s[(BQ_F + 2 * d) * n + c] + s[(BQ_F + 2 * d + 1) * n + c]);
+		}
+		const double inv = 1 / s[c];
+		const double v[3] = {s[n + c] * inv, s[2 * n + c] * inv, s[3 * n + c] * inv};
+		const double B[3] = {s[5 * n + c], s[6 * n + c], s[7 * n + c]};
+		s[4 * n + c] = or_total_energy(s[c], v, p, B, b->gamma, b->mu0);
+	}
+	/* planet cells, :520-541; the energy is computed with the momentum in
+	   place of the velocity, as the reference does */
+	for (long c = 0; c < n; c++) {
+		if (b->rule[c] != OR_RULE_PLANET) continue;
+		const double mass = b->inner_n * b->mp;
+		for (int q = 0; q < 6; q++) s[(BQ_F + q) * n + c] = 0;
+		for (int d = 0; d < 3; d++) s[(BQ_VB + d) * n + c] = 0;
+		s[c] = mass;
+		for (int d = 0; d < 3; d++) s[(1 + d) * n + c] = mass * b->inner_v[d];
+		const double m[3] = {s[n + c], s[2 * n + c], s[3 * n + c]};
+		const double B[3] = {0, 0, 0};
+		s[4 * n + c] = or_total_energy(mass, m, b->inner_p, B, b->gamma, b->mu0);
+	}
+	/* inflow cells and the face each shares with its interior neighbour,
+	   :219-299 */
+	const double mass = b->sw_n * b->mp;
+	for (long k = 0; k < b->nz; k++)
+	for (long j = 0; j < b->ny; j++)
+	for (long i = 0; i < b->nx; i++) {
+		const long c = bid(b, i, j, k);
+		if (b->rule[c] != OR_RULE_SW) continue;
+		s[c] = mass;
+		for (int d = 0; d < 3; d++) s[(1 + d) * n + c] = mass * b->sw_v[d];
+		for (int d = 0; d < 3; d++) {
+			s[(BQ_VB + d) * n + c] = b->sw_B[d];
+			s[(BQ_F + 2 * d) * n + c] = b->sw_B[d];
+			s[(BQ_F + 2 * d + 1) * n + c] = b->sw_B[d];
+		}
+		s[4 * n + c] = or_total_energy(mass, b->sw_v, b->sw_p, b->sw_B, b->gamma, b->mu0);
+
+		const int sg = b->sw_dir > 0 ? 1 : -1;
+		const long ni = i - sg * (sw_dim == 0), nj = j - sg * (sw_dim == 1), nk = k - sg * (sw_dim == 2);
+		if (!in_box(b, ni, nj, nk)) continue;
+		const long nb = bid(b, ni, nj, nk);
+		const double inv = 1 / s[nb];
+		const double v[3] = {s[n + nb] * inv, s[2 * n + nb] * inv, s[3 * n + nb] * inv};
+		double p;
+		const int e = box_pressure(b, s, n, nb, &p);
+		if (e) return e;
+		const int toward = BQ_F + 2 * sw_dim + (sg > 0), away = BQ_F + 2 * sw_dim + (sg < 0);
+		s[toward * n + nb] = b->sw_B[sw_dim];
+		s[(BQ_VB + sw_dim) * n + nb] = s[toward * n + nb] / 2 + s[away * n + nb] / 2;
+		const double B[3] = {s[5 * n + nb], s[6 * n + nb], s[7 * n + nb]};
+		s[4 * n + nb] = or_total_energy(s[nb], v, p, B, b->gamma, b->mu0);
+	}
+	return OR_OK;
+}
+
+int or_box_init(const or_box* b, double* s)
+{
+	const long n = (long)b->nx * b->ny * b->nz;
+	/* initialize_plasma, source/mhd/solar_wind_box.hpp:62-205 */
+	const double end_x = b->start[0] + (double)b->nx * b->len[0];
+	const double mass = b->sw_n * b->mp;
+	for (long k = 0; k < b->nz; k++)
+	for (long j = 0; j < b->ny; j++)
+	for (long i = 0; i < b->nx; i++) {
+		const long c = bid(b, i, j, k);
+		const double ratio = ccen(b, 0, i) / end_x;
+		const double vf = (0.0 < ratio) ? ratio : 0.0;
+		const double v[3] = {vf * b->sw_v[0], vf * b->sw_v[1], vf * b->sw_v[2]};
+		s[c] = mass;
+		for (int d = 0; d < 3; d++) s[(1 + d) * n + c] = mass * v[d];
+		for (int d = 0; d < 3; d++) {
+			s[(BQ_VB + d) * n + c] = b->sw_B[d];
+			s[(BQ_F + 2 * d) * n + c] = b->sw_B[d];
+			s[(BQ_F + 2 * d + 1) * n + c] = b->sw_B[d];
+		}
+		s[4 * n + c] = or_total_energy(mass, v, b->sw_p, b->sw_B, b->gamma, b->mu0);
+	}
+	/* update_vol_B(constant pressure) over cells of type >= 0,
+	   tests/mhd/solar_wind_box.cpp:441-446 */
+	for (long c = 0; c < n; c++) if (b->ctype[c] >= 0) box_refit(b, s, n, c);
+	const int e = or_box_boundaries(b, s);
+	if (e) return e;
+	or_box_sync(b, s);
+	return OR_OK;
+}
+
+int or_box_step(const or_box* b, int solver, double dt, double* s, double smax_dim[3],
+	long zero_dim[3], long* bad_face)
+{
+	const long n = (long)b->nx * b->ny * b->nz;
+	const double len[3] = {b->len[0], b->len[1], b->len[2]};
+	double* P = (double*)calloc(24 * n, sizeof(double));
+	if (!P) return OR_ERR_SOLVER;
+	int err = OR_OK;
+	long bad = -1;
+	double sm[3] = {0, 0, 0};
+	long zc[3] = {0, 0, 0};
+	/* flux pass over faces whose both cells have type >= 0
+	   (source/mhd/solve.hpp:262-393); the face is owned by its positive-side
+	   cell c, between L = c - e_d and R = c */
+	for (int d = 0; d < 3 && !err; d++) {
+		for (long k = 0; k < b->nz; k++)
+		for (long j = 0; j < b->ny; j++)
+		for (long i = 0; i < b->nx; i++) {
+			const long li = i - (d == 0), lj = j - (d == 1), lk = k - (d == 2);
+			if (!in_box(b, li, lj, lk)) continue;
+			const long c = bid(b, i, j, k), l = bid(b, li, lj, lk);
+			if (b->ctype[c] < 0 || b->ctype[l] < 0) continue;
+			double L[8], R[8], bgr[3] = {0, 0, 0}, F[8], sp = 0;
+			const long cc[2] = {l, c};
+			double* U[2] = {L, R};
+			for (int q = 0; q < 2; q++) {
+				const double m[3] = {s[n + cc[q]], s[2 * n + cc[q]], s[3 * n + cc[q]]};
+				const double B[3] = {s[5 * n + cc[q]], s[6 * n + cc[q]], s[7 * n + cc[q]]};
+				U[q][RHO] = s[cc[q]];
+				U[q][NRJ] = s[4 * n + cc[q]];
+				rot_to_face(d, m, U[q] + M0);
+				rot_to_face(d, B, U[q] + B0);
+			}
+			if (b->bg) {
+				const double bb[3] = {b->bg[(3 * d) * n + l], b->bg[(3 * d + 1) * n + l], b->bg[(3 * d + 2) * n + l]};
+				rot_to_face(d, bb, bgr);
+			}
+			const int e = or_flux(solver, L, R, bgr, b->gamma, b->mu0, F, &sp);
+			if (e) {
+				if (!err || d * n + c < bad) {
+					err = e;
+					bad = d * n + c;
+				}
+				continue;
+			}
+			double Fg[8];
+			Fg[RHO] = F[RHO];
+			Fg[NRJ] = F[NRJ];
+			rot_to_global(d, F + M0, Fg + M0);
+			rot_to_global(d, F + B0, Fg + B0);
+			for (int q = 0; q < 8; q++) P[(d * 8 + q) * n + c] = dt * Fg[q];
+			if (sp <= 0) zc[d]++;
+			if (sp > sm[d]) sm[d] = sp;
+		}
+	}
+	if (bad_face) *bad_face = err ? bad : -1;
+	if (smax_dim) for (int d = 0; d < 3; d++) smax_dim[d] = sm[d];
+	if (zero_dim) for (int d = 0; d < 3; d++) zero_dim[d] = zc[d];
+	if (err) {
+		free(P);
+		return err;
+	}
+	const double dx = len[0], dy = len[1], dz = len[2];
+	/* flux of a face that was not computed (outside the box or next to a
+	   cell of type -1) is zero */
+#define PB(d, q, i, j, k) (in_box(b, (i), (j), (k)) ? P[((d) * 8 + (q)) * n + bid(b, (i), (j), (k))] : 0.0)
+#define EZX(i, j, k) (+(PB(0, 6, i, j, k) * dz / 4))
+#define EYX(i, j, k) (-(PB(0, 7, i, j, k) * dy / 4))
+#define EXY(i, j, k) (+(PB(1, 7, i, j, k) * dx / 4))
+#define EZY(i, j, k) (-(PB(1, 5, i, j, k) * dz / 4))
+#define EXZ(i, j, k) (-(PB(2, 6, i, j, k) * dx / 4))
+#define EYZ(i, j, k) (+(PB(2, 5, i, j, k) * dy / 4))
+	double* out = (double*)malloc(sizeof(double) * 14 * n);
+	if (!out) {
+		free(P);
+		return OR_ERR_SOLVER;
+	}
+	memcpy(out, s, sizeof(double) * 14 * n);
+	for (long k = 0; k < b->nz; k++)
+	for (long j = 0; j < b->ny; j++)
+	for (long i = 0; i < b->nx; i++) {
+		const long c = bid(b, i, j, k);
+		if (b->ctype[c] <= 0) continue;
+		/* Face_dB of the cell's six face copies (the 12 edge terms of each
+		   face, in the order of the periodic step), update_mhd_state's
+		   face change, :1589-1593 */
+		double dB[3][2];
+		for (int side = 0; side < 2; side++) {
+			const long I = i + side, J = j + side, K = k + side;
+			double r = 0;
+			r += EZX(I, j + 1, k); r -= EZX(I, j - 1, k); r -= EYX(I, j, k + 1); r += EYX(I, j, k - 1);
+			r += EZY(I, j + 1, k); r += EZY(I - 1, j + 1, k); r -= EZY(I, j, k); r -= EZY(I - 1, j, k);
+			r -= EYZ(I, j, k + 1); r -= EYZ(I - 1, j, k + 1); r += EYZ(I, j, k); r += EYZ(I - 1, j, k);
+			dB[0][side] = r;
+			r = 0;
+			r += EXY(i, J, k + 1); r -= EXY(i, J, k - 1); r -= EZY(i + 1, J, k); r += EZY(i - 1, J, k);
+			r += EXZ(i, J, k + 1); r += EXZ(i, J - 1, k + 1); r -= EXZ(i, J, k); r -= EXZ(i, J - 1, k);
+			r -= EZX(i + 1, J, k); r += EZX(i, J, k); r -= EZX(i + 1, J - 1, k); r += EZX(i, J - 1, k);
+			dB[1][side] = r;
+			r = 0;
+			r += EYZ(i + 1, j, K); r -= EYZ(i - 1, j, K); r -= EXZ(i, j + 1, K); r += EXZ(i, j - 1, K);
+			r += EYX(i + 1, j, K); r += EYX(i + 1, j, K - 1); r -= EYX(i, j, K); r -= EYX(i, j, K - 1);
+			r -= EXY(i, j + 1, K); r -= EXY(i, j + 1, K - 1); r += EXY(i, j, K); r += EXY(i, j, K - 1);
+			dB[2][side] = r;
+		}
+		const double area[3] = {dy * dz, dx * dz, dx * dy};
+		for (int d = 0; d < 3; d++)
+		for (int side = 0; side < 2; side++) {
+			out[(BQ_F + 2 * d + side) * n + c] = s[(BQ_F + 2 * d + side) * n + c] + dB[d][side] / area[d];
+		}
+		/* conserved update, :1551-1587 */
+		double lo[3][8], hi[3][8];
+		for (int q = 0; q < 8; q++) {
+			lo[0][q] = 0.0 + PB(0, q, i, j, k);
+			hi[0][q] = 0.0 + PB(0, q, i + 1, j, k);
+			lo[1][q] = 0.0 + PB(1, q, i, j, k);
+			hi[1][q] = 0.0 + PB(1, q, i, j + 1, k);
+			lo[2][q] = 0.0 + PB(2, q, i, j, k);
+			hi[2][q] = 0.0 + PB(2, q, i, j, k + 1);
+		}
+		double rho = s[c];
+		double m[3] = {s[n + c], s[2 * n + c], s[3 * n + c]};
+		double E = s[4 * n + c];
+		double vb[3] = {s[5 * n + c], s[6 * n + c], s[7 * n + c]};
+		for (int d = 0; d < 3; d++) rho += (lo[d][RHO] - hi[d][RHO]) / len[d];
+		for (int d = 0; d < 3; d++) {
+			const double il = 1 / len[d];
+			for (int q = 0; q < 3; q++) m[q] = m[q] + (lo[d][M0 + q] - hi[d][M0 + q]) * il;
+		}
+		for (int d = 0; d < 3; d++) E += (lo[d][NRJ] - hi[d][NRJ]) / len[d];
+		for (int d = 0; d < 3; d++) {
+			const double il = 1 / len[d];
+			for (int q = 0; q < 3; q++) vb[q] = vb[q] + (lo[d][B0 + q] - hi[d][B0 + q]) * il;
+		}
+		out[c] = rho;
+		for (int q = 0; q < 3; q++) out[(1 + q) * n + c] = m[q];
+		out[4 * n + c] = E;
+		for (int q = 0; q < 3; q++) out[(BQ_VB + q) * n + c] = vb[q];
+	}
+#undef PB
+#undef EZX
+#undef EYX
+#undef EXY
+#undef EZY
+#undef EXZ
+#undef EYZ
+	free(P);
+	/* update_vol_B(substep), sync_magnetic_field, update_vol_B(0):
+	   source/mhd/solve.hpp:1904-1916 */
+	for (long c = 0; c < n; c++) if (b->ctype[c] >= 0) box_refit(b, out, n, c);
+	or_box_sync(b, out);
+	for (long c = 0; c < n; c++) if (b->ctype[c] >= 0) box_refit(b, out, n, c);
+	memcpy(s, out, sizeof(double) * 14 * n);
+	free(out);
+	return OR_OK;
+}
+
+int or_box_run(const or_box* b, int solver, double cfl, double t_start, double t_end,
+	long max_steps, double* s, long* steps_taken, double* t_final)
+{
+	double sm[3], dt;
+	long zc[3], bad;
+	/* final init with a zero time step, tests/mhd/solar_wind_box.cpp:464-473 */
+	int e = or_box_step(b, solver, 0.0, s, sm, zc, &bad);
+	double t = t_start;
+	long steps = 0;
+	while (!e && t < t_end && steps < max_steps) {
+		/* lagged CFL step, as or_cfl_dt */
+		double cdt = DBL_MAX;
+		for (int d = 0; d < 3; d++) {
+			if (zc[d] > 0) {
+				cdt = 0;
+				break;
+			}
+			cdt = smin(cdt, cfl * b->len[d] / sm[d]);
+		}
+		const double until_end = t_end - t;
+		dt = (until_end <= cdt) ? until_end : cdt;
+		e = or_box_step(b, solver, dt, s, sm, zc, &bad);
+		if (e) break;
+		t += dt;
+		steps++;
+		e = or_box_boundaries(b, s);
+		if (e) break;
+		or_box_sync(b, s);
+	}
+	if (steps_taken) *steps_taken = steps;
+	if (t_final) *t_final = t;
+	return e;
+}

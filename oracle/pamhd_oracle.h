@@ -165,6 +165,59 @@ divergence, face-gradient projection, constant-pressure energy refit.
 int or_clean_divergence(const or_grid* g, double* state, double tol, int max_cycles,
 	double* rel_residual, int* cycles);
 
+/* ---- solar-wind box (tests/mhd/solar_wind_box.cpp with a uniform grid) ---- */
+
+/* boundary rule of a cell: 1..27 = copy from the neighbour at offset
+   (o0, o1, o2), code 1 + (o0 + 1) + 3 (o1 + 1) + 9 (o2 + 1), mirrored across
+   the walls with a non-zero offset (face, edge and vertex cells of
+   source/mhd/solar_wind_box.hpp:386-520); planet and inflow cells */
+enum { OR_RULE_NONE = 0, OR_RULE_PLANET = 28, OR_RULE_SW = 29 };
+
+typedef struct {
+	int nx, ny, nz;             /* cells, including the one-cell boundary layer */
+	double start[3], len[3];    /* grid start and cell size */
+	double gamma, mu0, mp;
+	int sw_dir;                 /* wall of the inflow cells: +-1, +-2, +-3 */
+	double sw_n, sw_v[3], sw_p, sw_B[3];           /* solar-wind state */
+	double inner_radius, inner_n, inner_v[3], inner_p;
+	const double* bg;           /* 9 * n Bg_B(cell)(+d) or NULL */
+	const signed char* ctype;   /* n: 1 normal, 0 boundary, -1 not solved */
+	const unsigned char* rule;  /* n: OR_RULE_* or a copy code */
+} or_box;
+
+/*
+Box cell state, 14 * n component-major: rho, m (3), E, Vol_B (3), then the
+cell's own copies of its faces' B: -x, +x, -y, +y, -z, +z.
+*/
+
+/* prepare_grid's classification, source/grid/solar_wind_box.hpp:262-406 */
+int or_box_classify(const or_box* b, signed char* ctype, unsigned char* rule);
+
+/* Bg_B(cell)(+d) at face centres from a constant field and point dipoles
+   (moment, position: 3 * ndip each), background_magnetic_field.hpp:331-494 */
+void or_box_background(const or_box* b, const double constant[3], double min_distance,
+	int ndip, const double* moment, const double* position, double* bg);
+
+/* initialize_plasma + update_vol_B + apply_boundaries_sw_box +
+   sync_magnetic_field, tests/mhd/solar_wind_box.cpp:430-461 */
+int or_box_init(const or_box* b, double* state);
+
+/* one timestep() body of length dt (source/mhd/solve.hpp:1837-1924, global
+   time step): fluxes, update of normal cells, refit, face sync, refit */
+int or_box_step(const or_box* b, int solver, double dt, double* state,
+	double smax_dim[3], long zero_dim[3], long* bad_face);
+
+/* apply_boundaries_sw_box, source/mhd/solar_wind_box.hpp:306-577 */
+int or_box_boundaries(const or_box* b, double* state);
+
+/* sync_magnetic_field, source/common_functions.hpp:184-269 */
+void or_box_sync(const or_box* b, double* state);
+
+/* final-init step then main loop of tests/mhd/solar_wind_box.cpp:464-560
+   (state after or_box_init) */
+int or_box_run(const or_box* b, int solver, double cfl, double t_start, double t_end,
+	long max_steps, double* state, long* steps_taken, double* t_final);
+
 /* OpenMP threads of the grid step (n > 0 sets it); returns the count in use */
 int or_set_threads(int n);
 

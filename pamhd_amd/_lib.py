@@ -22,7 +22,11 @@ EXPORTS = (
     "pamhd_check", "pamhd_div_face2volume", "pamhd_clean_divergence",
     "pamhd_set_poisson_smoother", "pamhd_max_signal_speeds",
     "pamhd_face_flux_batch", "pamhd_autotune", "pamhd_set_launch", "pamhd_enable_timing", "pamhd_kernel_time",
-    "pamhd_last_error",
+    "pamhd_last_error", "pamhd_create_solar_wind_box", "pamhd_state_components",
+    "pamhd_box_cell_types", "pamhd_box_background", "pamhd_ensemble_create",
+    "pamhd_ensemble_destroy", "pamhd_ensemble_timestep", "pamhd_ensemble_run",
+    "pamhd_ensemble_check", "pamhd_ensemble_download", "pamhd_ensemble_cells",
+    "pamhd_ensemble_set_tile", "pamhd_ensemble_autotune", "pamhd_ensemble_last_error",
 )
 
 
@@ -35,6 +39,27 @@ class Physics(ctypes.Structure):
     _fields_ = [("adiabatic_index", ctypes.c_double),
                 ("vacuum_permeability", ctypes.c_double),
                 ("proton_mass", ctypes.c_double)]
+
+
+MAX_DIPOLES = 8
+
+
+class SolarWindDesc(ctypes.Structure):
+    """pamhd_solar_wind_desc of include/pamhd_mi355x.h"""
+    _fields_ = [("direction", ctypes.c_int32),
+                ("number_density", ctypes.c_double),
+                ("velocity", ctypes.c_double * 3),
+                ("pressure", ctypes.c_double),
+                ("magnetic_field", ctypes.c_double * 3),
+                ("inner_radius", ctypes.c_double),
+                ("inner_number_density", ctypes.c_double),
+                ("inner_velocity", ctypes.c_double * 3),
+                ("inner_pressure", ctypes.c_double),
+                ("bg_value", ctypes.c_double * 3),
+                ("bg_min_distance", ctypes.c_double),
+                ("n_dipoles", ctypes.c_int32),
+                ("dipole_moment", (ctypes.c_double * 3) * MAX_DIPOLES),
+                ("dipole_position", (ctypes.c_double * 3) * MAX_DIPOLES)]
 
 
 class PamhdError(RuntimeError):
@@ -87,6 +112,26 @@ def load():
     L.pamhd_enable_timing.argtypes = [H, ctypes.c_int]
     L.pamhd_kernel_time.argtypes = [H, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_long)]
     L.pamhd_last_error.argtypes = [H, ctypes.c_char_p, ctypes.c_size_t]
+    L.pamhd_create_solar_wind_box.argtypes = [ctypes.POINTER(GridDesc), ctypes.POINTER(Physics),
+                                              ctypes.POINTER(SolarWindDesc), ctypes.POINTER(H)]
+    L.pamhd_state_components.argtypes = [H, ctypes.POINTER(ctypes.c_int)]
+    L.pamhd_box_cell_types.argtypes = [H, _vp, _vp]
+    L.pamhd_box_background.argtypes = [H, _vp]
+    E = ctypes.c_void_p
+    L.pamhd_ensemble_create.argtypes = [ctypes.c_int, ctypes.POINTER(GridDesc),
+                                        ctypes.POINTER(Physics), ctypes.POINTER(SolarWindDesc),
+                                        ctypes.POINTER(E)]
+    L.pamhd_ensemble_destroy.argtypes = [E]
+    L.pamhd_ensemble_timestep.argtypes = [E, ctypes.c_int, ctypes.c_double, ctypes.c_double, _vp]
+    L.pamhd_ensemble_run.argtypes = [E, ctypes.c_int, ctypes.c_double, ctypes.c_long,
+                                     ctypes.c_int, _vp]
+    L.pamhd_ensemble_check.argtypes = [E, _vp, _vp]
+    L.pamhd_ensemble_download.argtypes = [E, ctypes.c_int, _vp]
+    L.pamhd_ensemble_cells.argtypes = [E, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)]
+    L.pamhd_ensemble_set_tile.argtypes = [E, ctypes.c_int]
+    L.pamhd_ensemble_autotune.argtypes = [E, ctypes.c_int, ctypes.c_double, ctypes.c_char_p,
+                                          ctypes.c_size_t, _vp]
+    L.pamhd_ensemble_last_error.argtypes = [E, ctypes.c_char_p, ctypes.c_size_t]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
     _lib = L
@@ -102,4 +147,16 @@ def last_error(handle=None):
 def check(code, handle=None):
     if code != 0:
         raise PamhdError(code, last_error(handle))
+    return code
+
+
+def ensemble_error(ens=None):
+    buf = ctypes.create_string_buffer(512)
+    load().pamhd_ensemble_last_error(ens, buf, 512)
+    return buf.value.decode()
+
+
+def check_ensemble(code, ens=None):
+    if code != 0:
+        raise PamhdError(code, ensemble_error(ens))
     return code

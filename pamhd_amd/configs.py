@@ -59,3 +59,48 @@ def orszag_tang_vortex(cells, noise=0.01, seed=12345, divergence_sigma=0.0):
     state[6] = By
     state[7] = Bz
     return state.reshape(8, n), (dx, dy, dz), gamma, mu0
+
+
+# ---------------------------------------------------------------- solar wind
+
+K_BOLTZMANN = 1.380649e-23          # source/simulation_options.hpp:63
+PROTON_MASS = 1.67262192595e-27
+MU0_SI = 1.2566370614359173e-06     # "vacuum-permeability" of the SI configs
+MERCURY_RADIUS = 244e4
+
+
+def solar_wind_box(edge, number_density, speed, by, bz, temperature=1e5,
+                   box_length=32 * 4.88e6, dipole_moment=(0.0, 0.0, -225e16)):
+    """A cubic solar-wind box of `edge` cells (boundary layer included)
+    centred on a Mercury-sized planet, with the wind entering through the +x
+    wall moving along -x ("direction": "+x", velocity (-speed, 0, 0)),
+    thermal pressure n k_B T, IMF (0, By, Bz), an inner boundary ten times
+    denser and hotter than the wind at rest, and a dipole at the centre whose
+    field is clamped within one cell (min-distance)."""
+    from .swbox import SolarWindBox
+    h = box_length / edge
+    p = number_density * K_BOLTZMANN * temperature
+    return SolarWindBox(
+        cells=(edge, edge, edge), start=(-box_length / 2,) * 3, cell_length=(h, h, h),
+        direction="+x", number_density=number_density, velocity=(-speed, 0.0, 0.0),
+        pressure=p, magnetic_field=(0.0, by, bz),
+        inner_radius=MERCURY_RADIUS, inner_number_density=10 * number_density,
+        inner_velocity=(0.0, 0.0, 0.0), inner_pressure=10 * p,
+        bg_value=(0.0, 0.0, 0.0), bg_min_distance=h,
+        dipoles=[(tuple(dipole_moment), (0.0, 0.0, 0.0))],
+        adiabatic_index=5.0 / 3.0, vacuum_permeability=MU0_SI, proton_mass=PROTON_MASS)
+
+
+def solar_wind_ensemble(nbox=64, seed=2024, edges=(32, 48, 64, 96, 128)):
+    """Seeded ensemble of solar-wind boxes: edge drawn from `edges`, number
+    density log-uniform in 1-20 cm^-3, speed uniform in 300-800 km/s,
+    By and Bz uniform in [-10, 10] nT, T = 1e5 K."""
+    rng = np.random.default_rng(seed)
+    boxes = []
+    for _ in range(nbox):
+        edge = int(rng.choice(edges))
+        n = 1e6 * math.exp(rng.uniform(math.log(1.0), math.log(20.0)))
+        v = rng.uniform(3e5, 8e5)
+        by, bz = rng.uniform(-1e-8, 1e-8, size=2)
+        boxes.append(solar_wind_box(edge, n, v, float(by), float(bz)))
+    return boxes

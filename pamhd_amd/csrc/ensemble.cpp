@@ -1,0 +1,179 @@
+/*
+Ensemble of solar-wind boxes behind the C ABI: every box of the set is
+advanced by the same launches (one per stage of the step), and a run of
+main-loop steps replays a HIP graph holding two consecutive steps (the
+lagged-step status alternates between two buffers, so two steps bring the
+launch arguments back to where they started).
+*/
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <string>
+
+#include "pamhd_mi355x.h"
+#include "swbox.hpp"
+
+using namespace pamhd_dev;
+
+struct pamhd_ensemble {
+	BoxSet* set = nullptr;
+	hipStream_t own = nullptr;
+	std::string err;
+};
+
+namespace {
+
+int efail(pamhd_ensemble* e, int code, const std::string& m)
+{
+	if (e) e->err = m;
+	return code;
+}
+
+std::string& create_error()
+{
+	static thread_local std::string m;
+	return m;
+}
+
+hipStream_t pick(pamhd_ensemble* e, void* stream)
+{
+	return stream ? (hipStream_t)stream : e->own;
+}
+
+} // namespace
+
+extern "C" {
+
+int pamhd_ensemble_create(int nbox, const pamhd_grid_desc* grids, const pamhd_physics* physics,
+	const pamhd_solar_wind_desc* sw, pamhd_ensemble** out)
+{
+	if (!out) return PAMHD_E_ARG;
+	*out = nullptr;
+	pamhd_ensemble* e = new pamhd_ensemble();
+	std::string err;
+	const int rc = boxset_create(&e->set, nbox, grids, physics, sw, err);
+	if (rc) {
+		delete e;
+		create_error() = err;
+		return rc;
+	}
+	if (hipStreamCreate(&e->own) != hipSuccess) {
+		boxset_destroy(e->set);
+		delete e;
+		return PAMHD_E_DEVICE;
+	}
+	*out = e;
+	return PAMHD_OK;
+}
+
+int pamhd_ensemble_destroy(pamhd_ensemble* e)
+{
+	if (!e) return PAMHD_OK;
+	if (e->own) {
+		hipStreamSynchronize(e->own);
+		hipStreamDestroy(e->own);
+	}
+	boxset_destroy(e->set);
+	delete e;
+	return PAMHD_OK;
+}
+
+int pamhd_ensemble_timestep(pamhd_ensemble* e, int solver, double max_time_step,
+	double time_step_factor, void* stream)
+{
+	if (!e) return PAMHD_E_ARG;
+	const int rc = boxset_launch_step(e->set, solver, max_time_step, time_step_factor, false,
+		pick(e, stream), e->err);
+	return rc ? efail(e, rc, e->err) : PAMHD_OK;
+}
+
+int pamhd_ensemble_run(pamhd_ensemble* e, int solver, double time_step_factor, long steps,
+	int use_graph, void* stream)
+{
+	if (!e) return PAMHD_E_ARG;
+	if (steps < 0) return efail(e, PAMHD_E_ARG, "steps must be >= 0");
+	hipStream_t st = pick(e, stream);
+	int rc = PAMHD_OK;
+	if (!use_graph || steps < 4) {
+		for (long s = 0; s < steps && rc == PAMHD_OK; s++) {
+			rc = boxset_launch_step(e->set, solver, 1e300, time_step_factor, true, st, e->err);
+		}
+		return rc;
+	}
+	if (steps % 2) rc = boxset_launch_step(e->set, solver, 1e300, time_step_factor, true, st, e->err);
+	if (rc) return rc;
+	/* capture two steps (st is never the default stream here) */
+	hipStream_t cap = st;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	hipError_t he = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
+	if (he != hipSuccess) return efail(e, PAMHD_E_DEVICE, std::string("begin capture: ") + hipGetErrorString(he));
+	int r1 = boxset_launch_step(e->set, solver, 1e300, time_step_factor, true, cap, e->err);
+	int r2 = r1 ? r1 : boxset_launch_step(e->set, solver, 1e300, time_step_factor, true, cap, e->err);
+	he = hipStreamEndCapture(cap, &graph);
+	if (r2 || he != hipSuccess) {
+		if (graph) hipGraphDestroy(graph);
+		return efail(e, r2 ? r2 : PAMHD_E_DEVICE, r2 ? e->err : std::string("end capture: ") + hipGetErrorString(he));
+	}
+	he = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+	if (he != hipSuccess) {
+		hipGraphDestroy(graph);
+		return efail(e, PAMHD_E_DEVICE, std::string("instantiate: ") + hipGetErrorString(he));
+	}
+	const long reps = (steps - steps % 2) / 2;
+	for (long r = 0; r < reps && he == hipSuccess; r++) he = hipGraphLaunch(exec, cap);
+	/* the executable graph may be destroyed once its launches complete */
+	hipStreamSynchronize(cap);
+	hipGraphExecDestroy(exec);
+	hipGraphDestroy(graph);
+	if (he != hipSuccess) return efail(e, PAMHD_E_DEVICE, std::string("graph launch: ") + hipGetErrorString(he));
+	return PAMHD_OK;
+}
+
+int pamhd_ensemble_check(pamhd_ensemble* e, double* dt_per_box, void* stream)
+{
+	if (!e) return PAMHD_E_ARG;
+	const int rc = boxset_readback(e->set, pick(e, stream), dt_per_box, e->err);
+	return rc ? efail(e, rc, e->err) : PAMHD_OK;
+}
+
+int pamhd_ensemble_download(pamhd_ensemble* e, int box, double* state)
+{
+	if (!e || !state || box < 0 || box >= boxset_count(e->set)) return efail(e, PAMHD_E_ARG, "invalid box or null state");
+	hipStreamSynchronize(e->own);
+	const hipError_t he = boxset_download(e->set, box, state);
+	return he == hipSuccess ? PAMHD_OK : efail(e, PAMHD_E_DEVICE, hipGetErrorString(he));
+}
+
+int pamhd_ensemble_cells(pamhd_ensemble* e, int box, long long* cells)
+{
+	if (!e || !cells || box < 0 || box >= boxset_count(e->set)) return efail(e, PAMHD_E_ARG, "invalid box");
+	*cells = boxset_cells(e->set, box);
+	return PAMHD_OK;
+}
+
+int pamhd_ensemble_set_tile(pamhd_ensemble* e, int tile)
+{
+	if (!e) return PAMHD_E_ARG;
+	const int rc = boxset_set_tile(e->set, tile);
+	return rc ? efail(e, rc, "invalid tile " + std::to_string(tile)) : PAMHD_OK;
+}
+
+int pamhd_ensemble_autotune(pamhd_ensemble* e, int solver, double time_step_factor, char* buf,
+	size_t buflen, void* stream)
+{
+	if (!e) return PAMHD_E_ARG;
+	std::string report;
+	const int rc = boxset_autotune(e->set, solver, time_step_factor, 3, pick(e, stream), report, e->err);
+	if (buf && buflen) snprintf(buf, buflen, "%s", report.c_str());
+	return rc ? efail(e, rc, e->err) : PAMHD_OK;
+}
+
+int pamhd_ensemble_last_error(pamhd_ensemble* e, char* buf, size_t n)
+{
+	if (!buf || n == 0) return PAMHD_E_ARG;
+	snprintf(buf, n, "%s", e ? e->err.c_str() : create_error().c_str());
+	return PAMHD_OK;
+}
+
+} // extern "C"

@@ -32,6 +32,7 @@ energy refits).  In mixed precision the flux array holds FP32 values.
 #include "pamhd_mi355x.h"
 #include "poisson.hpp"
 #include "riemann.hpp"
+#include "swbox.hpp"
 
 using namespace pamhd_dev;
 
@@ -794,6 +795,7 @@ struct pamhd_handle {
 	std::vector<Timed> timed;
 	double kernel_ms[4] = {0, 0, 0, 0};
 	long kernel_launches[4] = {0, 0, 0, 0};
+	BoxSet* box = nullptr;    /* solar-wind box (non-periodic) handles only */
 	PoissonMG* mg = nullptr;  /* multigrid hierarchy, allocated on first cleaning */
 	int mg_fused = 1;
 	int cur = 0;    /* state buffer holding the current state */
@@ -937,6 +939,18 @@ int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
 	}
 	if (!(cfl >= 0)) return fail(h, PAMHD_E_ARG, "time step factor must be >= 0");
 	hipStream_t st = (hipStream_t)stream_;
+	if (h->box) {
+		/* timestep() alone, or with the main loop's apply_boundaries_sw_box
+		   and sync_magnetic_field (tests/mhd/solar_wind_box.cpp:522-553) */
+		std::string err;
+		int e = boxset_launch_step(h->box, solver, max_dt, cfl, refits == 3, st, err);
+		if (e) return fail(h, e, err);
+		if (dt_taken) {
+			e = boxset_readback(h->box, st, dt_taken, err);
+			if (e) return fail(h, e, err);
+		}
+		return PAMHD_OK;
+	}
 	StepStatus* prev = h->status + h->last;
 	StepStatus* next = h->status + (1 - h->last);
 	{
@@ -1032,6 +1046,43 @@ int pamhd_create(const pamhd_grid_desc* grid, const pamhd_physics* physics, int 
 	return PAMHD_OK;
 }
 
+int pamhd_create_solar_wind_box(const pamhd_grid_desc* grid, const pamhd_physics* physics,
+	const pamhd_solar_wind_desc* sw, pamhd_handle** out)
+{
+	if (!grid || !physics || !sw || !out) return fail(nullptr, PAMHD_E_ARG, "null argument");
+	*out = nullptr;
+	BoxSet* set = nullptr;
+	std::string err;
+	const int e = boxset_create(&set, 1, grid, physics, sw, err);
+	if (e) return fail(nullptr, e, err);
+	pamhd_handle* h = new pamhd_handle();
+	h->desc = *grid;
+	h->phys = *physics;
+	h->box = set;
+	h->g = GridDev{grid->cells[0], grid->cells[1], grid->cells[2],
+		(long long)grid->cells[0] * grid->cells[1] * grid->cells[2],
+		grid->cell_length[0], grid->cell_length[1], grid->cell_length[2],
+		make_phys(physics->adiabatic_index, physics->vacuum_permeability)};
+	*out = h;
+	return PAMHD_OK;
+}
+
+int pamhd_box_cell_types(pamhd_handle* h, signed char* cell_type, unsigned char* rule)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (!h->box) return fail(h, PAMHD_E_ARG, "not a solar-wind box");
+	HIP_TRY(h, boxset_cell_types(h->box, 0, cell_type, rule));
+	return PAMHD_OK;
+}
+
+int pamhd_box_background(pamhd_handle* h, double* bg)
+{
+	if (!h || !bg) return fail(h, PAMHD_E_ARG, "null argument");
+	if (!h->box) return fail(h, PAMHD_E_ARG, "not a solar-wind box");
+	HIP_TRY(h, boxset_background(h->box, 0, bg));
+	return PAMHD_OK;
+}
+
 int pamhd_destroy(pamhd_handle* h)
 {
 	if (!h) return PAMHD_OK;
@@ -1047,6 +1098,7 @@ int pamhd_destroy(pamhd_handle* h)
 		mg_destroy(*h->mg);
 		delete h->mg;
 	}
+	boxset_destroy(h->box);
 	hipHostFree(h->host_dt);
 	hipHostFree(h->host_status);
 	delete h;
@@ -1056,6 +1108,10 @@ int pamhd_destroy(pamhd_handle* h)
 int pamhd_upload_state(pamhd_handle* h, const double* state)
 {
 	if (!h || !state) return fail(h, PAMHD_E_ARG, "null argument");
+	if (h->box) {
+		HIP_TRY(h, boxset_upload(h->box, 0, state));
+		return PAMHD_OK;
+	}
 	HIP_TRY(h, hipMemcpy(h->state[h->cur], state, sizeof(double) * 8 * h->g.n, hipMemcpyHostToDevice));
 	return PAMHD_OK;
 }
@@ -1063,13 +1119,25 @@ int pamhd_upload_state(pamhd_handle* h, const double* state)
 int pamhd_download_state(pamhd_handle* h, double* state)
 {
 	if (!h || !state) return fail(h, PAMHD_E_ARG, "null argument");
+	if (h->box) {
+		HIP_TRY(h, boxset_download(h->box, 0, state));
+		return PAMHD_OK;
+	}
 	HIP_TRY(h, hipMemcpy(state, h->state[h->cur], sizeof(double) * 8 * h->g.n, hipMemcpyDeviceToHost));
+	return PAMHD_OK;
+}
+
+int pamhd_state_components(pamhd_handle* h, int* components)
+{
+	if (!h || !components) return fail(h, PAMHD_E_ARG, "null argument");
+	*components = h->box ? 14 : 8;
 	return PAMHD_OK;
 }
 
 int pamhd_state_device_ptr(pamhd_handle* h, double** ptr)
 {
 	if (!h || !ptr) return fail(h, PAMHD_E_ARG, "null argument");
+	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "solar-wind box state is not a flat array");
 	*ptr = h->state[h->cur];
 	return PAMHD_OK;
 }
@@ -1077,6 +1145,7 @@ int pamhd_state_device_ptr(pamhd_handle* h, double** ptr)
 int pamhd_set_background_faces(pamhd_handle* h, const double* bg)
 {
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "solar-wind boxes compute their background field");
 	if (!bg) {
 		hipFree(h->bg);
 		h->bg = nullptr;
@@ -1102,6 +1171,10 @@ int pamhd_step(pamhd_handle* h, int solver, double max_time_step, double time_st
 int pamhd_apply_boundaries(pamhd_handle* h, double /*simulation_time*/, void* stream)
 {
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (h->box) {
+		if (boxset_launch_boundaries(h->box, (hipStream_t)stream)) return fail(h, PAMHD_E_DEVICE, "boundary launch");
+		return PAMHD_OK;
+	}
 	{
 		TimedLaunch t(h, (hipStream_t)stream, K_REFIT);
 		refit_kernel<<<cell_grid(h->g), dim3(BX, BY, 1), 0, (hipStream_t)stream>>>(h->g, h->state[h->cur]);
@@ -1113,6 +1186,11 @@ int pamhd_apply_boundaries(pamhd_handle* h, double /*simulation_time*/, void* st
 int pamhd_check(pamhd_handle* h)
 {
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (h->box) {
+		std::string err;
+		const int e = boxset_readback(h->box, nullptr, nullptr, err);
+		return e ? fail(h, e, err) : PAMHD_OK;
+	}
 	HIP_TRY(h, hipDeviceSynchronize());
 	HIP_TRY(h, hipMemcpy(h->host_status, h->status + h->last, sizeof(StepStatus), hipMemcpyDeviceToHost));
 	return report_status(h, *h->host_status);
@@ -1121,6 +1199,7 @@ int pamhd_check(pamhd_handle* h)
 int pamhd_max_signal_speeds(pamhd_handle* h, double smax_out[3])
 {
 	if (!h || !smax_out) return fail(h, PAMHD_E_ARG, "null argument");
+	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "per-box speeds: use the ensemble interface");
 	HIP_TRY(h, hipDeviceSynchronize());
 	StepStatus st;
 	HIP_TRY(h, hipMemcpy(&st, h->status + h->last, sizeof(StepStatus), hipMemcpyDeviceToHost));
@@ -1135,6 +1214,10 @@ int pamhd_max_signal_speeds(pamhd_handle* h, double smax_out[3])
 int pamhd_div_face2volume(pamhd_handle* h, double* avg)
 {
 	if (!h || !avg) return fail(h, PAMHD_E_ARG, "null argument");
+	if (h->box) {
+		HIP_TRY(h, boxset_div_face2volume(h->box, 0, avg));
+		return PAMHD_OK;
+	}
 	const dim3 grid = cell_grid(h->g);
 	const size_t nb = (size_t)grid.x * grid.y * grid.z;
 	if (h->scratch_len < nb) {
@@ -1163,6 +1246,7 @@ int pamhd_clean_divergence(pamhd_handle* h, double tol, int max_cycles, double* 
 	double* rel_residual, int* cycles, void* stream_)
 {
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "divergence cleaning needs a periodic grid");
 	if (!(tol >= 0) || max_cycles < 0) return fail(h, PAMHD_E_ARG, "tol and max_cycles must be >= 0");
 	hipStream_t st = (hipStream_t)stream_;
 	if (!h->mg) {
@@ -1250,6 +1334,7 @@ int pamhd_face_flux_batch(int solver, int precision, size_t n, const double* L, 
 int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* stream_)
 {
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "solar-wind box: use the ensemble autotuner");
 	if (solver != RUSANOV && solver != HLL_ATHENA && solver != HLLD_ATHENA && solver != ROE_ATHENA) {
 		return fail(h, PAMHD_E_SOLVER, "invalid solver " + std::to_string(solver));
 	}

@@ -100,12 +100,23 @@ template <class T> __device__ __forceinline__ T dot3(const T* a, const T* b)
 }
 
 /* FP64 thermal pressure, source/mhd/common.hpp:76-115; half_inv_mu0 = 0.5/mu0 */
-__device__ __forceinline__ double pressure64(const double U[8], double gm1, double half_inv_mu0)
+__host__ __device__ __forceinline__ double pressure64(const double U[8], double gm1, double half_inv_mu0)
 {
 	const double
 		kinetic = 0.5 / U[RHO] * (U[M0] * U[M0] + U[M1] * U[M1] + U[M2] * U[M2]),
 		magnetic = half_inv_mu0 * (U[B0] * U[B0] + U[B1] * U[B1] + U[B2] * U[B2]);
 	return (U[NRJ] - kinetic - magnetic) * gm1;
+}
+
+/* FP64 total energy density from rho, v, p and B,
+   source/mhd/common.hpp:254-311 (argument checks made by callers) */
+__host__ __device__ __forceinline__ double total_energy64(double rho, const double v[3], double p,
+	const double B[3], double gm1, double half_inv_mu0)
+{
+	const double
+		kinetic = 0.5 * rho * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]),
+		magnetic = half_inv_mu0 * (B[0] * B[0] + B[1] * B[1] + B[2] * B[2]);
+	return p / gm1 + kinetic + magnetic;
 }
 
 /*

@@ -239,3 +239,109 @@ def clean_divergence(grid, state, tol=1e-10, max_cycles=60):
     st = lib().or_clean_divergence(ctypes.byref(grid), _ptr(state), tol, max_cycles,
                                    ctypes.byref(res), ctypes.byref(cyc))
     return st, res.value, cyc.value
+
+
+# ---------------------------------------------------------------- solar wind
+
+class OrBox(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("start", ctypes.c_double * 3), ("len", ctypes.c_double * 3),
+                ("gamma", ctypes.c_double), ("mu0", ctypes.c_double), ("mp", ctypes.c_double),
+                ("sw_dir", ctypes.c_int), ("sw_n", ctypes.c_double),
+                ("sw_v", ctypes.c_double * 3), ("sw_p", ctypes.c_double),
+                ("sw_B", ctypes.c_double * 3), ("inner_radius", ctypes.c_double),
+                ("inner_n", ctypes.c_double), ("inner_v", ctypes.c_double * 3),
+                ("inner_p", ctypes.c_double), ("bg", _dp),
+                ("ctype", ctypes.c_void_p), ("rule", ctypes.c_void_p)]
+
+
+RULE_PLANET, RULE_SW = 28, 29
+_box_ready = False
+
+
+def _box_lib():
+    global _box_ready
+    L = lib()
+    if not _box_ready:
+        B = ctypes.POINTER(OrBox)
+        L.or_box_classify.argtypes = [B, ctypes.c_void_p, ctypes.c_void_p]
+        L.or_box_background.argtypes = [B, _dp, ctypes.c_double, ctypes.c_int, _dp, _dp, _dp]
+        L.or_box_background.restype = None
+        L.or_box_init.argtypes = [B, _dp]
+        L.or_box_step.argtypes = [B, ctypes.c_int, ctypes.c_double, _dp, _dp, _lp, _lp]
+        L.or_box_boundaries.argtypes = [B, _dp]
+        L.or_box_sync.argtypes = [B, _dp]
+        L.or_box_sync.restype = None
+        L.or_box_run.argtypes = [B, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                 ctypes.c_double, ctypes.c_long, _dp, _lp, _dp]
+        _box_ready = True
+    return L
+
+
+class Box:
+    """Oracle view of a pamhd_amd.swbox.SolarWindBox: classification and
+    background field are computed by the oracle on construction."""
+
+    def __init__(self, sb):
+        from pamhd_amd.swbox import DIRECTIONS
+        L = _box_lib()
+        self.sb = sb
+        self.n = sb.n
+        b = OrBox()
+        b.nx, b.ny, b.nz = sb.cells
+        b.start[:] = sb.start
+        b.len[:] = sb.cell_length
+        b.gamma, b.mu0, b.mp = sb.adiabatic_index, sb.vacuum_permeability, sb.proton_mass
+        b.sw_dir = DIRECTIONS[sb.direction]
+        b.sw_n = sb.number_density
+        b.sw_v[:] = sb.velocity
+        b.sw_p = sb.pressure
+        b.sw_B[:] = sb.magnetic_field
+        b.inner_radius = sb.inner_radius
+        b.inner_n = sb.inner_number_density
+        b.inner_v[:] = sb.inner_velocity
+        b.inner_p = sb.inner_pressure
+        self.ctype = np.zeros(self.n, dtype=np.int8)
+        self.rule = np.zeros(self.n, dtype=np.uint8)
+        assert L.or_box_classify(ctypes.byref(b), self.ctype.ctypes.data, self.rule.ctypes.data) == 0
+        b.ctype = self.ctype.ctypes.data
+        b.rule = self.rule.ctypes.data
+        self.bg = np.zeros((9, self.n))
+        nd = len(sb.dipoles)
+        mom = np.array([m for m, _ in sb.dipoles] or [[0.0, 0.0, 0.0]], dtype=np.float64).reshape(-1)
+        pos = np.array([p for _, p in sb.dipoles] or [[0.0, 0.0, 0.0]], dtype=np.float64).reshape(-1)
+        const = np.array(sb.bg_value, dtype=np.float64)
+        L.or_box_background(ctypes.byref(b), _ptr(const), sb.bg_min_distance, nd, _ptr(mom),
+                            _ptr(pos), _ptr(self.bg))
+        b.bg = _ptr(self.bg)
+        self.b = b
+        self._keep = (mom, pos, const)
+
+    def init(self):
+        s = np.zeros((14, self.n))
+        st = _box_lib().or_box_init(ctypes.byref(self.b), _ptr(s))
+        assert st == 0, st
+        return s
+
+    def step(self, solver, dt, state):
+        """timestep() body in place; returns (status, smax3, zero3)."""
+        assert state.flags.c_contiguous and state.dtype == np.float64
+        sm = np.zeros(3)
+        zc = np.zeros(3, dtype=np.int64)
+        bad = ctypes.c_long(-1)
+        st = _box_lib().or_box_step(ctypes.byref(self.b), solver, dt, _ptr(state), _ptr(sm),
+                                    zc.ctypes.data_as(_lp), ctypes.byref(bad))
+        return st, sm, zc
+
+    def boundaries(self, state):
+        return _box_lib().or_box_boundaries(ctypes.byref(self.b), _ptr(state))
+
+    def sync(self, state):
+        _box_lib().or_box_sync(ctypes.byref(self.b), _ptr(state))
+
+    def run(self, solver, cfl, t_start, t_end, state, max_steps=1 << 40):
+        steps = ctypes.c_long(0)
+        tf = ctypes.c_double(0)
+        st = _box_lib().or_box_run(ctypes.byref(self.b), solver, cfl, t_start, t_end, max_steps,
+                                   _ptr(state), ctypes.byref(steps), ctypes.byref(tf))
+        return st, steps.value, tf.value
