@@ -251,6 +251,28 @@ int pamhd_enable_timing(pamhd_handle* h, int enable);
 int pamhd_kernel_time(pamhd_handle* h, const char* kernel, double* total_ms,
 	long* launches);
 
+/* ---- frozen-time-step stepping and its adjoint ---- */
+
+/* One main-loop step (as pamhd_step) of exactly dt, without the CFL
+   limit: the frozen time-step schedule a gradient is taken along.
+   (No counterpart in the reference.) */
+int pamhd_step_fixed(pamhd_handle* h, int solver, double dt, void* stream);
+
+/* Stream-ordered device-to-device copies of the 8 * n state. */
+int pamhd_set_state_device(pamhd_handle* h, const double* state, void* stream);
+int pamhd_get_state_device(pamhd_handle* h, double* state, void* stream);
+
+/*
+Reverse step of pamhd_step_fixed: given the state U^n (device, 8 * n) the
+forward step of length dt started from and lambda = dL/dU^{n+1} (device,
+8 * n, same layout), overwrites lambda with dL/dU^n.  HLL fluxes, FP64,
+periodic grid without background field.  Uses 108 * n doubles of scratch
+allocated on first use.  (No counterpart in the reference, which has no
+derivatives.)
+*/
+int pamhd_adjoint_step(pamhd_handle* h, int solver, double dt, const double* state,
+	double* lambda, void* stream);
+
 /* ---- solar-wind box (tests/mhd/solar_wind_box.cpp) ---- */
 
 /*

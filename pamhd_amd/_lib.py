@@ -27,6 +27,8 @@ EXPORTS = (
     "pamhd_ensemble_destroy", "pamhd_ensemble_timestep", "pamhd_ensemble_run",
     "pamhd_ensemble_check", "pamhd_ensemble_download", "pamhd_ensemble_cells",
     "pamhd_ensemble_set_tile", "pamhd_ensemble_autotune", "pamhd_ensemble_last_error",
+    "pamhd_step_fixed", "pamhd_set_state_device", "pamhd_get_state_device",
+    "pamhd_adjoint_step",
 )
 
 
@@ -132,6 +134,10 @@ def load():
     L.pamhd_ensemble_autotune.argtypes = [E, ctypes.c_int, ctypes.c_double, ctypes.c_char_p,
                                           ctypes.c_size_t, _vp]
     L.pamhd_ensemble_last_error.argtypes = [E, ctypes.c_char_p, ctypes.c_size_t]
+    L.pamhd_step_fixed.argtypes = [H, ctypes.c_int, ctypes.c_double, _vp]
+    L.pamhd_set_state_device.argtypes = [H, _vp, _vp]
+    L.pamhd_get_state_device.argtypes = [H, _vp, _vp]
+    L.pamhd_adjoint_step.argtypes = [H, ctypes.c_int, ctypes.c_double, _vp, _vp, _vp]
     for name in EXPORTS:
         getattr(L, name).restype = ctypes.c_int
     _lib = L

@@ -104,3 +104,93 @@ def solar_wind_ensemble(nbox=64, seed=2024, edges=(32, 48, 64, 96, 128)):
         by, bz = rng.uniform(-1e-8, 1e-8, size=2)
         boxes.append(solar_wind_box(edge, n, v, float(by), float(bz)))
     return boxes
+
+
+# ------------------------------------------------------- 2-D adjoint workload
+
+def _fourier_field(n, kmax, rng):
+    """Random periodic field on the n x n cell centres (and its values at the
+    cell corners), sum over 0 < |k|_inf <= kmax of Gaussian-weighted cos and
+    sin modes, normalised to unit RMS at the centres."""
+    ks = [(kx, ky) for kx in range(-kmax, kmax + 1) for ky in range(0, kmax + 1)
+          if (ky > 0 or kx > 0)]
+    a = rng.normal(size=len(ks))
+    b = rng.normal(size=len(ks))
+    xc = (np.arange(n) + 0.5) / n
+    xk = np.arange(n) / n
+    out = []
+    for x in (xc, xk):
+        X, Y = np.meshgrid(x, x, indexing="xy")       # [j, i] = (y_j, x_i)
+        f = np.zeros((n, n))
+        for (kx, ky), ca, cb in zip(ks, a, b):
+            ph = 2 * math.pi * (kx * X + ky * Y)
+            f += ca * np.cos(ph) + cb * np.sin(ph)
+        out.append(f)
+    s = np.sqrt((out[0] ** 2).mean())
+    return out[0] / s, out[1] / s
+
+
+def adjoint_vortex_2d(n=512, seed=5, kmax=8):
+    """Fixed parts of the 2-D periodic adjoint workload on [0,1]^2 (n x n x 1
+    cells, gamma 5/3, mu0 1): the smooth vortex (rho, p, v, and the vector
+    potential A_z of B = (-sin 2 pi y, sin 4 pi x, 0) / sqrt(4 pi) at cell
+    corners) and seeded unit-RMS random-Fourier perturbations (|k| <= kmax)
+    of rho, v_x, v_y, p and A_z.  Returns a dict of float64 numpy arrays
+    indexed [j, i]."""
+    rng = np.random.default_rng(seed)
+    xc = (np.arange(n) + 0.5) / n
+    xk = np.arange(n) / n
+    Xc, Yc = np.meshgrid(xc, xc, indexing="xy")
+    Xk, Yk = np.meshgrid(xk, xk, indexing="xy")
+    s4 = 1.0 / math.sqrt(4 * math.pi)
+    base = {
+        "rho": np.full((n, n), 25.0 / (36.0 * math.pi)),
+        "p": np.full((n, n), 5.0 / (12.0 * math.pi)),
+        "vx": -np.sin(2 * math.pi * Yc),
+        "vy": np.sin(2 * math.pi * Xc),
+        "A": s4 * (np.cos(2 * math.pi * Yk) / (2 * math.pi) + np.cos(4 * math.pi * Xk) / (4 * math.pi)),
+    }
+    pert = {}
+    for name in ("rho", "vx", "vy", "p"):
+        pert[name] = _fourier_field(n, kmax, rng)[0]
+    pert["A"] = _fourier_field(n, kmax, rng)[1]
+    return {"n": n, "base": base, "pert": pert, "length": (1.0 / n, 1.0 / n, 1.0),
+            "gamma": 5.0 / 3.0, "mu0": 1.0}
+
+
+def adjoint_initial_state(w, amplitudes):
+    """(8, n*n) float64 torch state of the workload `w` for the 4 amplitudes
+    (rho, v, p, A_z; a torch tensor, differentiable): rho = rho0 (1 + a0 d_rho),
+    v = v0 + a1 d_v, p = p0 (1 + a2 d_p), A_z = A0 + a3 0.05 d_A; face B from
+    corner differences of A_z (discretely divergence-free), m = rho v and
+    E = p/(gamma-1) + rho v^2/2 + |face-average B|^2/(2 mu0)."""
+    import torch
+    dev, dt = amplitudes.device, torch.float64
+    T = lambda a: torch.as_tensor(a, dtype=dt, device=dev)
+    n = w["n"]
+    b, d = w["base"], w["pert"]
+    h = 1.0 / n
+    rho = T(b["rho"]) * (1 + amplitudes[0] * T(d["rho"]))
+    vx = T(b["vx"]) + amplitudes[1] * T(d["vx"])
+    vy = T(b["vy"]) + amplitudes[1] * T(d["vy"])
+    p = T(b["p"]) * (1 + amplitudes[2] * T(d["p"]))
+    A = T(b["A"]) + amplitudes[3] * 0.05 * T(d["A"])        # corners (x_i - h/2, y_j - h/2)
+    Bx = (torch.roll(A, -1, 0) - A) / h                     # face (i - 1/2, j)
+    By = -(torch.roll(A, -1, 1) - A) / h                    # face (i, j - 1/2)
+    Bz = torch.zeros_like(Bx)
+    vbx = 0.5 * (Bx + torch.roll(Bx, -1, 1))
+    vby = 0.5 * (By + torch.roll(By, -1, 0))
+    E = p / (w["gamma"] - 1) + 0.5 * rho * (vx * vx + vy * vy) + 0.5 / w["mu0"] * (vbx * vbx + vby * vby)
+    U = torch.stack([rho, rho * vx, rho * vy, torch.zeros_like(rho), E, Bx, By, Bz])
+    return U.reshape(8, n * n)
+
+
+def window_magnetic_energy(U, n, lo=224, size=64, mu0=1.0):
+    """sum over a size x size window of cells of |B_cell|^2 / (2 mu0) * dA,
+    B_cell the face-average field of the final state (torch, differentiable)."""
+    s = U.reshape(8, n, n)
+    bx = 0.5 * (s[5] + s[5].roll(-1, 1))
+    by = 0.5 * (s[6] + s[6].roll(-1, 0))
+    bz = s[7]
+    e = 0.5 / mu0 * (bx * bx + by * by + bz * bz) / (n * n)
+    return e[lo:lo + size, lo:lo + size].sum()

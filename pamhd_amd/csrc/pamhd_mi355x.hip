@@ -34,6 +34,12 @@ energy refits).  In mixed precision the flux array holds FP32 values.
 #include "riemann.hpp"
 #include "swbox.hpp"
 
+namespace pamhd_dev {
+size_t adjoint_scratch_doubles(long long n);
+void adjoint_launch(int nx, int ny, int nz, const double len[3], double dt, const Phys& ph,
+	const double* state, double* lam, double* scratch, hipStream_t st);
+}
+
 using namespace pamhd_dev;
 
 namespace {
@@ -575,18 +581,19 @@ __global__ __launch_bounds__(BX * BY) void update_cell_kernel(GridDev g,
 /* lagged CFL step of minimize_timestep + set_minmax_substepping_period with a
    single substep (source/mhd/solve.hpp:1961-2006, substepping.hpp:303-338) */
 __global__ void dt_kernel(const StepStatus* __restrict__ prev, StepStatus* __restrict__ next,
-	double* __restrict__ dt_out, double cfl, double max_dt, double dx, double dy, double dz)
+	double* __restrict__ dt_out, double cfl, double max_dt, double dx, double dy, double dz,
+	int fixed)
 {
 	const double len[3] = {dx, dy, dz};
 	double cfl_dt = DBL_MAX;
-	for (int d = 0; d < 3; d++) {
+	for (int d = 0; d < 3 && !fixed; d++) {
 		if (prev->zero[d] > 0) {
 			cfl_dt = 0;
 			break;
 		}
 		cfl_dt = smin(cfl_dt, cfl * len[d] / __longlong_as_double((long long)prev->smax_bits[d]));
 	}
-	*dt_out = (max_dt <= cfl_dt) ? max_dt : cfl_dt;
+	*dt_out = (fixed || max_dt <= cfl_dt) ? max_dt : cfl_dt;
 	for (int d = 0; d < 3; d++) {
 		next->smax_bits[d] = 0;
 		next->zero[d] = 0;
@@ -796,6 +803,7 @@ struct pamhd_handle {
 	double kernel_ms[4] = {0, 0, 0, 0};
 	long kernel_launches[4] = {0, 0, 0, 0};
 	BoxSet* box = nullptr;    /* solar-wind box (non-periodic) handles only */
+	double* adj = nullptr;    /* adjoint scratch, allocated on first use */
 	PoissonMG* mg = nullptr;  /* multigrid hierarchy, allocated on first cleaning */
 	int mg_fused = 1;
 	int cur = 0;    /* state buffer holding the current state */
@@ -930,7 +938,7 @@ int report_status(pamhd_handle* h, const StepStatus& st)
 }
 
 int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
-	double* dt_taken, void* stream_)
+	double* dt_taken, void* stream_, int fixed_dt = 0)
 {
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (solver == HYBRID) return fail(h, PAMHD_E_UNSUPPORTED, "hybrid solver: MHD fluxes are not computed");
@@ -955,7 +963,7 @@ int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
 	StepStatus* next = h->status + (1 - h->last);
 	{
 		TimedLaunch t(h, st, K_DT);
-		dt_kernel<<<1, 1, 0, st>>>(prev, next, h->dt, cfl, max_dt, h->g.dx, h->g.dy, h->g.dz);
+		dt_kernel<<<1, 1, 0, st>>>(prev, next, h->dt, cfl, max_dt, h->g.dx, h->g.dy, h->g.dz, fixed_dt);
 	}
 	HIP_TRY(h, hipGetLastError());
 	int e;
@@ -1099,6 +1107,7 @@ int pamhd_destroy(pamhd_handle* h)
 		delete h->mg;
 	}
 	boxset_destroy(h->box);
+	hipFree(h->adj);
 	hipHostFree(h->host_dt);
 	hipHostFree(h->host_status);
 	delete h;
@@ -1290,6 +1299,48 @@ int pamhd_clean_divergence(pamhd_handle* h, double tol, int max_cycles, double* 
 	return PAMHD_OK;
 }
 
+int pamhd_step_fixed(pamhd_handle* h, int solver, double dt, void* stream)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "fixed steps need a periodic grid");
+	if (!(dt >= 0)) return fail(h, PAMHD_E_ARG, "dt must be >= 0");
+	return do_step(h, solver, dt, 0.0, 3, nullptr, stream, 1);
+}
+
+int pamhd_set_state_device(pamhd_handle* h, const double* state, void* stream)
+{
+	if (!h || !state) return fail(h, PAMHD_E_ARG, "null argument");
+	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "solar-wind box state is not a flat array");
+	HIP_TRY(h, hipMemcpyAsync(h->state[h->cur], state, sizeof(double) * 8 * h->g.n,
+		hipMemcpyDeviceToDevice, (hipStream_t)stream));
+	return PAMHD_OK;
+}
+
+int pamhd_get_state_device(pamhd_handle* h, double* state, void* stream)
+{
+	if (!h || !state) return fail(h, PAMHD_E_ARG, "null argument");
+	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "solar-wind box state is not a flat array");
+	HIP_TRY(h, hipMemcpyAsync(state, h->state[h->cur], sizeof(double) * 8 * h->g.n,
+		hipMemcpyDeviceToDevice, (hipStream_t)stream));
+	return PAMHD_OK;
+}
+
+int pamhd_adjoint_step(pamhd_handle* h, int solver, double dt, const double* state, double* lambda,
+	void* stream)
+{
+	if (!h || !state || !lambda) return fail(h, PAMHD_E_ARG, "null argument");
+	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint needs a periodic grid");
+	if (solver != HLL_ATHENA) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint is implemented for hll-athena");
+	if (h->precision != PAMHD_FP64) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint is FP64");
+	if (h->bg) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint has no background field");
+	if (!h->adj) HIP_TRY(h, hipMalloc(&h->adj, sizeof(double) * adjoint_scratch_doubles(h->g.n)));
+	const double len[3] = {h->g.dx, h->g.dy, h->g.dz};
+	adjoint_launch(h->g.nx, h->g.ny, h->g.nz, len, dt, h->g.ph, state, lambda, h->adj,
+		(hipStream_t)stream);
+	HIP_TRY(h, hipGetLastError());
+	return PAMHD_OK;
+}
+
 int pamhd_set_poisson_smoother(pamhd_handle* h, int fused)
 {
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
@@ -1359,7 +1410,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 		return best / 3;
 	};
 	auto flux = [&]() {
-		dt_kernel<<<1, 1, 0, st>>>(prev, next, h->dt, 0.0, 0.0, h->g.dx, h->g.dy, h->g.dz);
+		dt_kernel<<<1, 1, 0, st>>>(prev, next, h->dt, 0.0, 0.0, h->g.dx, h->g.dy, h->g.dz, 0);
 		if (h->precision == PAMHD_MIXED) launch_flux_pass<float, float>(h, solver, st, next);
 		else launch_flux_pass<double, double>(h, solver, st, next);
 	};

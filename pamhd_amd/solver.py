@@ -184,6 +184,25 @@ class Grid:
             self._check(st)
         return before.value, rel.value, cyc.value, st
 
+    # --- frozen time-step schedule and its adjoint ------------------------
+    def step_fixed(self, solver, dt, stream=None):
+        """One main-loop step of exactly dt (no CFL limit)."""
+        self._check(self._L.pamhd_step_fixed(self._h, solver_id(solver), dt, stream))
+
+    def set_state_device(self, ptr, stream=None):
+        """Copies 8 * n doubles from a device pointer into the grid."""
+        self._check(self._L.pamhd_set_state_device(self._h, ptr, stream))
+
+    def get_state_device(self, ptr, stream=None):
+        """Copies the grid's 8 * n doubles to a device pointer."""
+        self._check(self._L.pamhd_get_state_device(self._h, ptr, stream))
+
+    def adjoint_step(self, solver, dt, state_ptr, lambda_ptr, stream=None):
+        """lambda (device, 8 * n) <- dL/dU^n from dL/dU^{n+1}, for the step of
+        length dt that started from the state at state_ptr."""
+        self._check(self._L.pamhd_adjoint_step(self._h, solver_id(solver), dt, state_ptr,
+                                               lambda_ptr, stream))
+
     def set_poisson_smoother(self, fused):
         self._check(self._L.pamhd_set_poisson_smoother(self._h, 1 if fused else 0))
 

@@ -1,0 +1,118 @@
+"""The hand-written adjoint of the periodic MHD step (HLL, frozen time-step
+schedule) behind torch.autograd, against FP64 central finite differences of
+the FP64 forward, on a small 2-D version of the adjoint workload.
+
+The forward of the rollout is the library's FP64 step, so it equals an
+ordinary run with the same time steps bit for bit.  Gradients are checked
+along random directions of the initial state and for the four perturbation
+amplitudes (whose gradient flows through torch's autograd of the initial
+state construction into the adjoint).  Tolerance: central differences with
+h = 1e-6 have truncation error O(h^2) and roundoff O(eps L / h); 1e-5
+relative leaves room for HLL's max/min kinks crossed by a perturbation.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+N, STEPS, EVERY, LO, SIZE = 48, 30, 10, 16, 16
+
+
+def _setup(gpu):
+    import torch
+    from pamhd_amd import configs
+    from pamhd_amd.solver import Grid
+    w = configs.adjoint_vortex_2d(n=N, seed=5)
+    amps = torch.tensor([1e-2] * 4, dtype=torch.float64, device=gpu)
+    U0 = configs.adjoint_initial_state(w, amps).contiguous()
+    g = Grid((N, N, 1), w["length"], w["gamma"], w["mu0"])
+    # time-step schedule of an ordinary CFL-limited run from U0
+    g.upload(U0.cpu().numpy())
+    g.timestep("hll-athena", 0.0, 0.3)
+    dts = [g.step("hll-athena", 1e300, 0.3) for _ in range(STEPS)]
+    return w, amps, U0, g, dts
+
+
+def _loss(U):
+    from pamhd_amd import configs
+    return configs.window_magnetic_energy(U, N, lo=LO, size=SIZE)
+
+
+def test_rollout_forward_is_the_fp64_step(gpu):
+    import torch
+    from pamhd_amd.autograd import rollout
+    w, amps, U0, g, dts = _setup(gpu)
+    with g:
+        UT = rollout(U0, g, "hll-athena", dts, every=EVERY)
+        g.upload(U0.cpu().numpy())
+        for dt in dts:
+            g.step_fixed("hll-athena", dt)
+        ref = g.download()
+    np.testing.assert_array_equal(UT.cpu().numpy(), ref)
+    assert torch.isfinite(UT).all()
+
+
+def test_adjoint_matches_finite_differences(gpu):
+    import torch
+    from pamhd_amd.autograd import rollout
+    w, amps, U0, g, dts = _setup(gpu)
+    with g:
+        x = U0.clone().requires_grad_(True)
+        L = _loss(rollout(x, g, "hll-athena", dts, every=EVERY))
+        L.backward()
+        lam0 = x.grad.detach()
+        assert torch.isfinite(lam0).all()
+        rng = np.random.default_rng(17)
+        scale = U0.abs().mean(1, keepdim=True).clamp_min(1e-3)
+        h = 1e-6
+        errs = []
+        for q in range(8):
+            d = torch.as_tensor(rng.normal(size=U0.shape), device=gpu) * scale
+            with torch.no_grad():
+                lp = _loss(rollout(U0 + h * d, g, "hll-athena", dts, every=EVERY)).item()
+                lm = _loss(rollout(U0 - h * d, g, "hll-athena", dts, every=EVERY)).item()
+            fd = (lp - lm) / (2 * h)
+            ad = float((lam0 * d).sum())
+            err = abs(fd - ad) / max(abs(fd), abs(ad), 1e-300)
+            print("direction", q, "fd", fd, "adjoint", ad, "rel err", err, flush=True)
+            errs.append(err)
+        assert max(errs) < 1e-5
+
+
+def test_amplitude_gradients(gpu):
+    import torch
+    from pamhd_amd import configs
+    from pamhd_amd.autograd import rollout
+    w, amps, U0, g, dts = _setup(gpu)
+    with g:
+        a = amps.clone().requires_grad_(True)
+        L = _loss(rollout(configs.adjoint_initial_state(w, a), g, "hll-athena", dts, every=EVERY))
+        L.backward()
+        grad = a.grad.detach().cpu().numpy()
+        h = 1e-6
+        for q in range(4):
+            e = torch.zeros(4, dtype=torch.float64, device=gpu)
+            e[q] = h
+            with torch.no_grad():
+                lp = _loss(rollout(configs.adjoint_initial_state(w, amps + e), g, "hll-athena", dts,
+                                   every=EVERY)).item()
+                lm = _loss(rollout(configs.adjoint_initial_state(w, amps - e), g, "hll-athena", dts,
+                                   every=EVERY)).item()
+            fd = (lp - lm) / (2 * h)
+            err = abs(fd - grad[q]) / max(abs(fd), abs(grad[q]), 1e-300)
+            print("amplitude", q, "fd", fd, "adjoint", grad[q], "rel err", err, flush=True)
+            assert err < 1e-5
+
+
+def test_checkpoint_interval_does_not_change_gradient(gpu):
+    import torch
+    from pamhd_amd.autograd import rollout
+    w, amps, U0, g, dts = _setup(gpu)
+    with g:
+        grads = []
+        for every in (1, 7, STEPS):
+            x = U0.clone().requires_grad_(True)
+            _loss(rollout(x, g, "hll-athena", dts, every=every)).backward()
+            grads.append(x.grad.detach().cpu().numpy())
+    np.testing.assert_array_equal(grads[0], grads[1])
+    np.testing.assert_array_equal(grads[0], grads[2])
