@@ -239,6 +239,12 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen,
    planes per block of the z-marching update.  All give identical results. */
 int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_remap);
 
+/* Pipelines the flux pass and the per-cell update over `slabs` z-slabs on
+   two streams (1 = no pipelining; capped at nz / 2; ignored while kernel
+   timing is on or with the z-marching update).  Identical results.
+   (No counterpart in the reference.) */
+int pamhd_set_slabs(pamhd_handle* h, int slabs);
+
 /*
 Per-kernel device-time accounting: when enabled, each launch of the handle is
 bracketed by HIP events on its stream.  pamhd_kernel_time synchronises and

@@ -28,7 +28,7 @@ EXPORTS = (
     "pamhd_ensemble_check", "pamhd_ensemble_download", "pamhd_ensemble_cells",
     "pamhd_ensemble_set_tile", "pamhd_ensemble_autotune", "pamhd_ensemble_last_error",
     "pamhd_step_fixed", "pamhd_set_state_device", "pamhd_get_state_device",
-    "pamhd_adjoint_step",
+    "pamhd_adjoint_step", "pamhd_set_slabs",
 )
 
 
@@ -111,6 +111,7 @@ def load():
                                         _vp, _vp, _vp, _vp]
     L.pamhd_autotune.argtypes = [H, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, _vp]
     L.pamhd_set_launch.argtypes = [H, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.pamhd_set_slabs.argtypes = [H, ctypes.c_int]
     L.pamhd_enable_timing.argtypes = [H, ctypes.c_int]
     L.pamhd_kernel_time.argtypes = [H, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_long)]
     L.pamhd_last_error.argtypes = [H, ctypes.c_char_p, ctypes.c_size_t]

@@ -178,10 +178,10 @@ __device__ __forceinline__ void load_face_state_d(const StateArrays& S, const Gr
 template <int SOLVER, class T, class PT, int D>
 __global__ __launch_bounds__(BX * BY) void flux_pass_kernel(GridDev g, StateArrays S,
 	const double* __restrict__ bg, PT* __restrict__ P, const double* __restrict__ dtp,
-	StepStatus* st, unsigned gx, unsigned gy, int remap)
+	StepStatus* st, unsigned gx, unsigned gy, int remap, int k0)
 {
 	const unsigned lb = remap ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
-	const unsigned bx = lb % gx, by = (lb / gx) % gy, k = lb / (gx * gy);
+	const unsigned bx = lb % gx, by = (lb / gx) % gy, k = k0 + lb / (gx * gy);
 	const int i = bx * BX + threadIdx.x;
 	const int j = by * BY + threadIdx.y;
 	const int n = (int)g.n;
@@ -206,26 +206,27 @@ __global__ __launch_bounds__(BX * BY) void flux_pass_kernel(GridDev g, StateArra
 		}
 		T F[8], sT = 0;
 		const int e = riemann<SOLVER, T>(L, R, b, g.ph, F, sT);
-		double Fg[8];
+		/* dt * F rotated back to the global frame, in the flux precision
+		   (FP64: dt * F of the reference's flux slots) */
+		T Fg[8];
 		if (e) {
 			record_error(st, e, (long long)D * g.n + c);
 			for (int q = 0; q < 8; q++) Fg[q] = 0;
 			sT = 0;
 		} else {
-			Fg[RHO] = (double)F[RHO];
-			Fg[NRJ] = (double)F[NRJ];
-			/* rotate back to the global frame */
-			Fg[M0 + D] = (double)F[M0];
-			Fg[M0 + (D + 1) % 3] = (double)F[M1];
-			Fg[M0 + (D + 2) % 3] = (double)F[M2];
-			Fg[B0 + D] = (double)F[B0];
-			Fg[B0 + (D + 1) % 3] = (double)F[B1];
-			Fg[B0 + (D + 2) % 3] = (double)F[B2];
+			const T dt = (T)*dtp;
+			Fg[RHO] = dt * F[RHO];
+			Fg[NRJ] = dt * F[NRJ];
+			Fg[M0 + D] = dt * F[M0];
+			Fg[M0 + (D + 1) % 3] = dt * F[M1];
+			Fg[M0 + (D + 2) % 3] = dt * F[M2];
+			Fg[B0 + D] = dt * F[B0];
+			Fg[B0 + (D + 1) % 3] = dt * F[B1];
+			Fg[B0 + (D + 2) % 3] = dt * F[B2];
 		}
-		const double dt = *dtp;
 		PT* Pd = P + (size_t)D * 8 * g.n;
 #pragma unroll
-		for (int q = 0; q < 8; q++) (Pd + (size_t)q * g.n)[c] = (PT)(dt * Fg[q]);
+		for (int q = 0; q < 8; q++) (Pd + (size_t)q * g.n)[c] = (PT)Fg[q];
 		sp = (double)sT;
 		zero = sp <= 0 ? 1u : 0u;
 		(void)n;
@@ -507,10 +508,10 @@ dB instead of exchanging them.  Kept as an autotuning candidate.
 template <class PT>
 __global__ __launch_bounds__(BX * BY) void update_cell_kernel(GridDev g,
 	const double* __restrict__ s, double* __restrict__ out,
-	const PT* __restrict__ P, int refits, unsigned gx, unsigned gy, int remap)
+	const PT* __restrict__ P, int refits, unsigned gx, unsigned gy, int remap, int k0)
 {
 	const unsigned lb = remap ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
-	const unsigned bx = lb % gx, by = (lb / gx) % gy, k = lb / (gx * gy);
+	const unsigned bx = lb % gx, by = (lb / gx) % gy, k = k0 + lb / (gx * gy);
 	const int i = bx * BX + threadIdx.x;
 	const int j = by * BY + threadIdx.y;
 	if (i >= g.nx || j >= g.ny) return;
@@ -795,7 +796,11 @@ struct pamhd_handle {
 		int flux_remap = 0;
 		int update_kz = 0;
 		int update_remap = 0;
+		int slabs = 1;   /* > 1: flux pass and update pipelined over z-slabs */
 	} tune;
+	/* second stream and events of the slab pipeline, created on first use */
+	hipStream_t aux = nullptr;
+	std::vector<hipEvent_t> ev;
 	/* optional per-kernel device time accounting with HIP events */
 	bool timing = false;
 	struct Timed { int kernel; hipEvent_t start, stop; };
@@ -874,28 +879,31 @@ dim3 cell_grid(const GridDev& g, int slabs = 1)
 }
 
 template <int SOLVER, class T, class PT>
-void launch_flux_dirs(pamhd_handle* h, hipStream_t st, StepStatus* stat)
+void launch_flux_dirs(pamhd_handle* h, hipStream_t st, StepStatus* stat, int k0, int nk)
 {
 	const double* s = h->state[h->cur];
 	StateArrays S;
 	for (int q = 0; q < 8; q++) S.q[q] = s + (size_t)q * h->g.n;
 	PT* P = reinterpret_cast<PT*>(h->flux);
 	const unsigned gx = (h->g.nx + BX - 1) / BX, gy = (h->g.ny + BY - 1) / BY;
-	const dim3 grid(gx * gy * (unsigned)h->g.nz), block(BX, BY, 1);
+	const dim3 grid(gx * gy * (unsigned)nk), block(BX, BY, 1);
 	const int r = h->tune.flux_remap;
-	flux_pass_kernel<SOLVER, T, PT, 0><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r);
-	flux_pass_kernel<SOLVER, T, PT, 1><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r);
-	flux_pass_kernel<SOLVER, T, PT, 2><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r);
+	flux_pass_kernel<SOLVER, T, PT, 0><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
+	flux_pass_kernel<SOLVER, T, PT, 1><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
+	flux_pass_kernel<SOLVER, T, PT, 2><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
 }
 
+/* faces owned by the cells of planes [k0, k0 + nk), all three directions */
 template <class T, class PT>
-int launch_flux_pass(pamhd_handle* h, int solver, hipStream_t st, StepStatus* stat)
+int launch_flux_pass(pamhd_handle* h, int solver, hipStream_t st, StepStatus* stat, int k0 = 0,
+	int nk = -1)
 {
+	if (nk < 0) nk = h->g.nz;
 	switch (solver) {
-	case RUSANOV: launch_flux_dirs<RUSANOV, T, PT>(h, st, stat); break;
-	case HLL_ATHENA: launch_flux_dirs<HLL_ATHENA, T, PT>(h, st, stat); break;
-	case HLLD_ATHENA: launch_flux_dirs<HLLD_ATHENA, T, PT>(h, st, stat); break;
-	case ROE_ATHENA: launch_flux_dirs<ROE_ATHENA, T, PT>(h, st, stat); break;
+	case RUSANOV: launch_flux_dirs<RUSANOV, T, PT>(h, st, stat, k0, nk); break;
+	case HLL_ATHENA: launch_flux_dirs<HLL_ATHENA, T, PT>(h, st, stat, k0, nk); break;
+	case HLLD_ATHENA: launch_flux_dirs<HLLD_ATHENA, T, PT>(h, st, stat, k0, nk); break;
+	case ROE_ATHENA: launch_flux_dirs<ROE_ATHENA, T, PT>(h, st, stat, k0, nk); break;
 	default:
 		return fail(h, PAMHD_E_SOLVER, "invalid solver " + std::to_string(solver));
 	}
@@ -906,14 +914,15 @@ int launch_flux_pass(pamhd_handle* h, int solver, hipStream_t st, StepStatus* st
 /* update_kz = 0 selects the per-cell update kernel, > 0 the z-marching one
    with that many planes per block */
 template <class PT>
-void launch_update(pamhd_handle* h, hipStream_t st, int refits)
+void launch_update(pamhd_handle* h, hipStream_t st, int refits, int k0 = 0, int nk = -1)
 {
 	const PT* P = reinterpret_cast<const PT*>(h->flux);
 	const int kz = h->tune.update_kz < h->g.nz ? h->tune.update_kz : h->g.nz;
-	if (kz <= 0) {
+	if (kz <= 0 || nk >= 0) {
+		if (nk < 0) nk = h->g.nz;
 		const unsigned gx = (h->g.nx + BX - 1) / BX, gy = (h->g.ny + BY - 1) / BY;
-		update_cell_kernel<PT><<<dim3(gx * gy * h->g.nz), dim3(BX, BY, 1), 0, st>>>(
-			h->g, h->state[h->cur], h->state[1 - h->cur], P, refits, gx, gy, h->tune.update_remap);
+		update_cell_kernel<PT><<<dim3(gx * gy * nk), dim3(BX, BY, 1), 0, st>>>(
+			h->g, h->state[h->cur], h->state[1 - h->cur], P, refits, gx, gy, h->tune.update_remap, k0);
 		return;
 	}
 	const unsigned gx = (h->g.nx + UBX - 1) / UBX, gy = (h->g.ny + UBY - 1) / UBY;
@@ -935,6 +944,59 @@ int report_status(pamhd_handle* h, const StepStatus& st)
 		"solution failed between cells (%d,%d,%d) and its -%c neighbour in direction %d: %s",
 		i, j, k, "xyz"[d], d + 1, error_name(st.err));
 	return fail(h, st.err, buf);
+}
+
+/*
+Flux pass and update of one step.  With tune.slabs = S > 1 the grid is cut
+into S z-slabs: the flux pass of slab s runs on the caller's stream, and the
+update of slab s (which reads the fluxes of slabs s-1, s and s+1) runs on a
+second stream as soon as the flux pass of slab s+1 is done, so the
+memory-bound update overlaps the compute-bound flux pass of later slabs.
+Per-cell arithmetic is unchanged, so results are identical for every S.
+*/
+template <class T, class PT>
+int launch_step_body(pamhd_handle* h, int solver, hipStream_t st, StepStatus* next, int refits)
+{
+	int S = h->tune.slabs;
+	if (S > h->g.nz / 2) S = h->g.nz / 2;
+	if (h->timing || h->tune.update_kz > 0) S = 1;
+	if (S <= 1) {
+		int e;
+		{
+			TimedLaunch t(h, st, K_FLUX);
+			e = launch_flux_pass<T, PT>(h, solver, st, next);
+		}
+		if (e) return e;
+		TimedLaunch t(h, st, K_UPDATE);
+		launch_update<PT>(h, st, refits);
+		return PAMHD_OK;
+	}
+	if (!h->aux) HIP_TRY(h, hipStreamCreateWithFlags(&h->aux, hipStreamNonBlocking));
+	while ((int)h->ev.size() < S + 2) {
+		hipEvent_t e;
+		HIP_TRY(h, hipEventCreateWithFlags(&e, hipEventDisableTiming));
+		h->ev.push_back(e);
+	}
+	hipEvent_t start = h->ev[S], join = h->ev[S + 1];
+	HIP_TRY(h, hipEventRecord(start, st));
+	HIP_TRY(h, hipStreamWaitEvent(h->aux, start, 0));
+	auto k0 = [&](int q) { return (int)((long long)q * h->g.nz / S); };
+	for (int q = 0; q < S; q++) {
+		const int e = launch_flux_pass<T, PT>(h, solver, st, next, k0(q), k0(q + 1) - k0(q));
+		if (e) return e;
+		HIP_TRY(h, hipEventRecord(h->ev[q], st));
+		if (q >= 2) {
+			HIP_TRY(h, hipStreamWaitEvent(h->aux, h->ev[q], 0));
+			launch_update<PT>(h, h->aux, refits, k0(q - 1), k0(q) - k0(q - 1));
+		}
+	}
+	if (S == 2) HIP_TRY(h, hipStreamWaitEvent(h->aux, h->ev[1], 0));
+	launch_update<PT>(h, h->aux, refits, k0(S - 1), k0(S) - k0(S - 1));
+	launch_update<PT>(h, h->aux, refits, k0(0), k0(1) - k0(0));
+	HIP_TRY(h, hipGetLastError());
+	HIP_TRY(h, hipEventRecord(join, h->aux));
+	HIP_TRY(h, hipStreamWaitEvent(st, join, 0));
+	return PAMHD_OK;
 }
 
 int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
@@ -966,19 +1028,10 @@ int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
 		dt_kernel<<<1, 1, 0, st>>>(prev, next, h->dt, cfl, max_dt, h->g.dx, h->g.dy, h->g.dz, fixed_dt);
 	}
 	HIP_TRY(h, hipGetLastError());
-	int e;
-	{
-		TimedLaunch t(h, st, K_FLUX);
-		e = h->precision == PAMHD_MIXED
-			? launch_flux_pass<float, float>(h, solver, st, next)
-			: launch_flux_pass<double, double>(h, solver, st, next);
-	}
+	const int e = h->precision == PAMHD_MIXED
+		? launch_step_body<float, float>(h, solver, st, next, refits)
+		: launch_step_body<double, double>(h, solver, st, next, refits);
 	if (e) return e;
-	{
-		TimedLaunch t(h, st, K_UPDATE);
-		if (h->precision == PAMHD_MIXED) launch_update<float>(h, st, refits);
-		else launch_update<double>(h, st, refits);
-	}
 	HIP_TRY(h, hipGetLastError());
 	h->cur = 1 - h->cur;
 	h->last = 1 - h->last;
@@ -1108,6 +1161,8 @@ int pamhd_destroy(pamhd_handle* h)
 	}
 	boxset_destroy(h->box);
 	hipFree(h->adj);
+	for (auto e : h->ev) hipEventDestroy(e);
+	if (h->aux) hipStreamDestroy(h->aux);
 	hipHostFree(h->host_dt);
 	hipHostFree(h->host_status);
 	delete h;
@@ -1450,6 +1505,28 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 		}
 	}
 	h->tune = best;
+	/* z-slab pipelining of flux pass and per-cell update (whole step) */
+	if (h->tune.update_kz == 0) {
+		report += "; step:";
+		float best_step = 1e30f;
+		int best_slabs = 1;
+		const int slabs[] = {1, 4, 8, 16};
+		for (int S : slabs) {
+			if (S > 1 && S > h->g.nz / 2) continue;
+			h->tune.slabs = S;
+			const float ms = time_it([&]() {
+				dt_kernel<<<1, 1, 0, st>>>(prev, next, h->dt, 0.0, 0.0, h->g.dx, h->g.dy, h->g.dz, 0);
+				if (h->precision == PAMHD_MIXED) launch_step_body<float, float>(h, solver, st, next, 3);
+				else launch_step_body<double, double>(h, solver, st, next, 3);
+			});
+			report += " slabs" + std::to_string(S) + "=" + std::to_string(ms) + "ms";
+			if (ms < best_step) {
+				best_step = ms;
+				best_slabs = S;
+			}
+		}
+		h->tune.slabs = best_slabs;
+	}
 	hipEventDestroy(a);
 	hipEventDestroy(b);
 	HIP_TRY(h, hipGetLastError());
@@ -1457,7 +1534,8 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	h->timing = timing;
 	report += "; chosen flux_remap=" + std::to_string(h->tune.flux_remap) +
 		" update=" + (h->tune.update_kz ? "zmarch" + std::to_string(h->tune.update_kz) : std::string("cell")) +
-		" update_remap=" + std::to_string(h->tune.update_remap);
+		" update_remap=" + std::to_string(h->tune.update_remap) +
+		" slabs=" + std::to_string(h->tune.slabs);
 	if (buf && buflen) snprintf(buf, buflen, "%s", report.c_str());
 	return PAMHD_OK;
 }
@@ -1469,6 +1547,14 @@ int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_
 	h->tune.flux_remap = flux_remap != 0;
 	h->tune.update_kz = update_kz;
 	h->tune.update_remap = update_remap != 0;
+	return PAMHD_OK;
+}
+
+int pamhd_set_slabs(pamhd_handle* h, int slabs)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (slabs < 1) return fail(h, PAMHD_E_ARG, "slabs must be >= 1");
+	h->tune.slabs = slabs;
 	return PAMHD_OK;
 }
 

@@ -26,6 +26,7 @@ the returned status code instead (PAMHD_E_* of include/pamhd_mi355x.h).
 #include <hip/hip_runtime.h>
 #include <cfloat>
 #include <cmath>
+#include <type_traits>
 
 namespace pamhd_dev {
 
@@ -128,6 +129,7 @@ template <class T> struct Side {
 	T u[8];
 	T p;
 	T v[3];
+	T ir;        /* 1 / rho (0 when rho <= 0) */
 	double p64;
 	bool rho_pos;
 };
@@ -141,12 +143,45 @@ template <class T> __device__ __forceinline__ void make_side(const double U[8], 
 	s.p = (T)s.p64;
 	if (s.rho_pos) {
 		const T inv = Arith<T>::rcp(s.u[RHO]);
+		s.ir = inv;
 		s.v[0] = s.u[M0] * inv;
 		s.v[1] = s.u[M1] * inv;
 		s.v[2] = s.u[M2] * inv;
 	} else {
+		s.ir = 0;
 		s.v[0] = s.v[1] = s.v[2] = 0;
 	}
+}
+
+/*
+FP32 fast magnetosonic speed for the mixed path, from the squared speeds
+with the cell's 1/rho: c_s^2 = gamma p / rho, v_A^2 = |B|^2 / (mu0 rho),
+v_An^2 = B_n^2 / (mu0 rho), c_f^2 = (c_s^2 + v_A^2 + sqrt((c_s^2 + v_A^2)^2
+- 4 c_s^2 v_An^2)) / 2: the same quantity as get_fast_magnetosonic_speed
+(source/mhd/common.hpp:415-491; its to_sqrt is this discriminant written
+with ratio^2 = v_An^2 / v_A^2) with two square roots and no division.
+*/
+__device__ __forceinline__ int fast_speed32(const Side<float>& s, const double U[8], const float bg[3],
+	float gamma, float inv_mu0, float& cf)
+{
+	using A = Arith<float>;
+	const float t0 = s.u[B0] + bg[0], t1 = s.u[B1] + bg[1], t2 = s.u[B2] + bg[2];
+	if (!A::isfinite(t0)) return ERR_FIELD;
+	if (!__builtin_isnormal(U[RHO]) || U[RHO] < 0) return ERR_DENSITY;
+	if (!__builtin_isnormal(s.p64) || s.p64 < 0) return ERR_PRESSURE;
+	const float im = s.ir * inv_mu0;
+	const float cs2 = gamma * s.p * s.ir;
+	const float va2 = (t0 * t0 + t1 * t1 + t2 * t2) * im;
+	if (!(va2 > 0)) {
+		cf = A::sqrt(cs2);
+		return ERR_OK;
+	}
+	const float sum = cs2 + va2;
+	if (!A::isnormal(sum) || sum < 0) return ERR_SPEED;
+	const float disc = sum * sum - 4.0f * cs2 * (t0 * t0 * im);
+	if (!A::isnormal(disc) || disc < 0) return ERR_SPEED;
+	cf = A::sqrt(0.5f * (sum + A::sqrt(disc)));
+	return ERR_OK;
 }
 
 /* fast magnetosonic speed with get_sound_speed / get_alfven_speed,
@@ -155,6 +190,7 @@ template <class T> __device__ __forceinline__ int fast_speed(const Side<T>& s,
 	const double U[8], const T bg[3], T gamma, T mu0, T& cf)
 {
 	using A = Arith<T>;
+	if constexpr (std::is_same_v<T, float>) return fast_speed32(s, U, bg, gamma, A::rcp(mu0), cf);
 	const T t[3] = {s.u[B0] + bg[0], s.u[B1] + bg[1], s.u[B2] + bg[2]};
 	if (!A::isfinite(t[0])) return ERR_FIELD;
 	const T mag = A::sqrt(t[0] * t[0] + t[1] * t[1] + t[2] * t[2]);
@@ -345,11 +381,173 @@ template <class T> __device__ __forceinline__ void hlld_star(const Side<T>& s, T
 		+ Bx * inv * (dot3(s.v, s.u + B0) - vdbs), smd);
 }
 
+/*
+HLLD of source/mhd/hlld_athena.hpp:54-559 for the mixed path: the same
+states, branches and degenerate-case tests in FP32, arranged so that each
+denominator is inverted once (1/rho, 1/(S_M* - S_M), 1/sqrt(rho*), the star
+states' shared denominator, 1/rho*) and the fast speeds come from
+fast_speed32: 12 reciprocals and 6 square roots per face instead of the
+FP64 order's 24 divisions and 14 square roots.  Pressures and the
+positivity checks are FP64 (make_side).
+*/
+__device__ __forceinline__ int flux_hlld32(const double Ld[8], const double Rd[8], const Phys& ph,
+	float F[8], float& s_out)
+{
+	using A = Arith<float>;
+	if (Ld[RHO] <= 0) return ERR_DENSITY;
+	if (Rd[RHO] <= 0) return ERR_DENSITY;
+	Side<float> L, R;
+	make_side<float>(Ld, ph, L);
+	make_side<float>(Rd, ph, R);
+	if (L.p64 <= 0) return ERR_PRESSURE;
+	if (R.p64 <= 0) return ERR_PRESSURE;
+	const float gamma = (float)ph.gamma, inv = (float)ph.inv_mu0, inv_sqrt = (float)ph.inv_sqrt_mu0;
+	const float zero[3] = {0, 0, 0};
+	float cfL, cfR;
+	int e = fast_speed32(L, Ld, zero, gamma, inv, cfL);
+	if (e) return e;
+	e = fast_speed32(R, Rd, zero, gamma, inv, cfR);
+	if (e) return e;
+	const float ms = smax(cfL, cfR);
+	const float *vL = L.v, *vR = R.v;
+	const float
+		msn = (vL[0] <= vR[0]) ? vL[0] - ms : vR[0] - ms,
+		msp = (vL[0] <= vR[0]) ? vR[0] + ms : vL[0] + ms;
+	/* physical fluxes without background field, total pressure p + B^2/2mu0 */
+	const float nmn = dot3(L.u + B0, L.u + B0) * (0.5f * inv), nmp = dot3(R.u + B0, R.u + B0) * (0.5f * inv);
+	const float ptL = L.p + nmn, ptR = R.p + nmp;
+	float FL[8], FR[8];
+	{
+		const Side<float>* sd[2] = {&L, &R};
+		const float pt[2] = {ptL, ptR};
+		float* FF[2] = {FL, FR};
+		for (int q = 0; q < 2; q++) {
+			const Side<float>& S = *sd[q];
+			const float* B = S.u + B0;
+			float* f = FF[q];
+			const float bin = B[0] * inv;
+			f[RHO] = S.u[M0];
+			for (int i = 0; i < 3; i++) f[M0 + i] = S.u[M0 + i] * S.v[0] - B[i] * bin;
+			f[M0] += pt[q];
+			f[NRJ] = S.v[0] * (S.u[NRJ] + pt[q]) - bin * dot3(S.v, B);
+			f[B0] = 0;
+			for (int i = 1; i < 3; i++) f[B0 + i] = B[i] * S.v[0] - S.v[i] * B[0];
+		}
+	}
+	if (!A::isfinite(FL[RHO]) || !A::isfinite(FR[RHO])) return ERR_FLUX;
+	if (msn >= 0.0f) {
+		for (int q = 0; q < 8; q++) F[q] = FL[q];
+		s_out = fabsf(msp);
+		return ERR_OK;
+	} else if (msp <= 0.0f) {
+		for (int q = 0; q < 8; q++) F[q] = FR[q];
+		s_out = fabsf(msn);
+		return ERR_OK;
+	}
+	const float Bx = (L.u[B0] + R.u[B0]) * 0.5f, Bx2 = Bx * Bx, Bx2i = Bx2 * inv;
+	const float sfdn = msn - vL[0], sfdp = msp - vR[0];
+	const float rsn = L.u[RHO] * sfdn, rsp = R.u[RHO] * sfdp;
+	const float sm = (sfdp * R.u[M0] - sfdn * L.u[M0] - ptR + ptL) * A::rcp(rsp - rsn);
+	const float smdn = msn - sm, smdp = msp - sm;
+	const float ismdn = A::rcp(smdn), ismdp = A::rcp(smdp);
+	const float dsn = rsn * ismdn, dsp = rsp * ismdp;
+	const float dssn = A::sqrt(dsn), dssp = A::sqrt(dsp);
+	const float idssn = A::rcp(dssn), idssp = A::rcp(dssp);
+	const float ssn = sm - fabsf(Bx) * inv_sqrt * idssn, ssp = sm + fabsf(Bx) * inv_sqrt * idssp;
+	const float ptst = ptL + rsn * (sm - vL[0]);
+
+	/* single-star states, eqns 43-48 of the HLLD method */
+	float Sn[8], Sp[8], vsn[3], vsp[3], vdbsn, vdbsp;
+	{
+		const Side<float>* sd[2] = {&L, &R};
+		const float sfd[2] = {sfdn, sfdp}, smd[2] = {smdn, smdp}, ism[2] = {ismdn, ismdp};
+		const float ds[2] = {dsn, dsp}, nm[2] = {nmn, nmp};
+		float* SS[2] = {Sn, Sp};
+		float* vs[2] = {vsn, vsp};
+		float* vdb[2] = {&vdbsn, &vdbsp};
+		for (int q = 0; q < 2; q++) {
+			const Side<float>& S = *sd[q];
+			float* st = SS[q];
+			const float tmp = S.u[RHO] * sfd[q] * smd[q] - Bx2i;
+			if (fabsf(tmp) < A::eps * ptst) {
+				for (int i = 1; i < 3; i++) {
+					st[M0 + i] = S.v[i] * ds[q];
+					st[B0 + i] = S.u[B0 + i];
+				}
+			} else {
+				const float it = A::rcp(tmp);
+				const float tmp1 = Bx * (sm - S.v[0]) * inv * it;
+				const float tmp2 = (S.u[RHO] * (sfd[q] * sfd[q]) - Bx2i) * it;
+				for (int i = 1; i < 3; i++) {
+					st[M0 + i] = (S.v[i] - S.u[B0 + i] * tmp1) * ds[q];
+					st[B0 + i] = S.u[B0 + i] * tmp2;
+				}
+			}
+			st[RHO] = ds[q];
+			st[M0] = ds[q] * sm;
+			st[B0] = Bx;
+			const float ids = A::rcp(ds[q]);
+			for (int i = 0; i < 3; i++) vs[q][i] = st[M0 + i] * ids;
+			*vdb[q] = dot3(vs[q], st + B0);
+			st[NRJ] = (sfd[q] * S.u[NRJ] - (S.p + nm[q]) * S.v[0] + ptst * sm
+				+ Bx * inv * (dot3(S.v, S.u + B0) - *vdb[q])) * ism[q];
+		}
+	}
+	/* double-star states, eqns 49-52 */
+	float S2n[8], S2p[8];
+	if (Bx2 * 0.5f < A::eps * ptst) {
+		for (int q = 0; q < 8; q++) {
+			S2n[q] = Sn[q];
+			S2p[q] = Sp[q];
+		}
+	} else {
+		const float isds = A::rcp(dssn + dssp);
+		const float sgn = (Bx > 0) ? 1.0f : -1.0f;
+		float v2[3];
+		const float c1 = sgn * inv_sqrt;
+		for (int i = 0; i < 3; i++) v2[i] = ((vsn[i] * dssn + vsp[i] * dssp) + (Sp[B0 + i] - Sn[B0 + i]) * c1) * isds;
+		v2[0] = sm;
+		S2n[RHO] = Sn[RHO];
+		S2p[RHO] = Sp[RHO];
+		S2n[M0] = Sn[M0];
+		S2p[M0] = Sp[M0];
+		for (int i = 1; i < 3; i++) {
+			S2n[M0 + i] = v2[i] * Sn[RHO];
+			S2p[M0 + i] = v2[i] * Sp[RHO];
+		}
+		const float c2 = sgn * dssn * dssp * (float)ph.sqrt_mu0;
+		S2n[B0] = S2p[B0] = Bx;
+		for (int i = 1; i < 3; i++) {
+			const float b = ((Sp[B0 + i] * dssn + Sn[B0 + i] * dssp) + (vsp[i] - vsn[i]) * c2) * isds;
+			S2n[B0 + i] = b;
+			S2p[B0 + i] = b;
+		}
+		const float vdb2 = dot3(v2, S2n + B0);
+		S2n[NRJ] = Sn[NRJ] - dssn * sgn * (vdbsn - vdb2) * inv_sqrt;
+		S2p[NRJ] = Sp[NRJ] + dssp * sgn * (vdbsp - vdb2) * inv_sqrt;
+	}
+	if (ssn >= 0.0f) {
+		for (int q = 0; q < 8; q++) F[q] = FL[q] + (Sn[q] - L.u[q]) * msn;
+	} else if (sm >= 0.0f) {
+		const float d = ssn - msn;
+		for (int q = 0; q < 8; q++) F[q] = FL[q] + S2n[q] * ssn - Sn[q] * d - L.u[q] * msn;
+	} else if (ssp > 0.0f) {
+		const float d = ssp - msp;
+		for (int q = 0; q < 8; q++) F[q] = FR[q] + S2p[q] * ssp - Sp[q] * d - R.u[q] * msp;
+	} else {
+		for (int q = 0; q < 8; q++) F[q] = FR[q] + (Sp[q] - R.u[q]) * msp;
+	}
+	F[B0] = 0;
+	s_out = smax(fabsf(msn), fabsf(msp));
+	return ERR_OK;
+}
+
 /* source/mhd/hlld_athena.hpp:54-559 (ignores the background field, :77) */
 template <class T> __device__ __forceinline__ int flux_hlld(const double Ld[8],
 	const double Rd[8], const T* /*bg*/, const Phys& ph, T F[8], T& s_out)
 {
 	using A = Arith<T>;
+	if constexpr (std::is_same_v<T, float>) return flux_hlld32(Ld, Rd, ph, F, s_out);
 	const T gamma = (T)ph.gamma, mu0 = (T)ph.mu0, inv_mu0 = (T)ph.inv_mu0;
 	const T zero[3] = {0, 0, 0};
 	if (Ld[RHO] <= 0) return ERR_DENSITY;

@@ -203,6 +203,10 @@ class Grid:
         self._check(self._L.pamhd_adjoint_step(self._h, solver_id(solver), dt, state_ptr,
                                                lambda_ptr, stream))
 
+    def set_slabs(self, slabs):
+        """z-slab pipelining of the flux pass and the update (1 = off)."""
+        self._check(self._L.pamhd_set_slabs(self._h, int(slabs)))
+
     def set_poisson_smoother(self, fused):
         self._check(self._L.pamhd_set_poisson_smoother(self._h, 1 if fused else 0))
 

@@ -132,6 +132,30 @@ def test_launch_variants_bit_exact(gpu, variant):
     np.testing.assert_array_equal(out, ref)
 
 
+@pytest.mark.parametrize("slabs", [2, 3, 4, 8])
+def test_slab_pipeline_bit_exact(gpu, slabs):
+    """Flux pass and update pipelined over z-slabs on two streams give the
+    oracle's bits (FP64) and the unpipelined mixed-precision bits."""
+    from pamhd_amd import solver as S
+    cells = (20, 18, 16)
+    state, length, gamma, mu0 = _vortex(cells, seed=13)
+    og = O.make_grid(cells, length, gamma, mu0)
+    ref = state.copy()
+    st, steps_o, t_o = O.run_periodic(og, O.HLLD, 0.3, 0.0, 1e9, ref, max_steps=5)
+    assert st == 0
+    outs = {}
+    for prec in ("fp64", "mixed"):
+        for sl in (1, slabs):
+            with S.Grid(cells, length, gamma, mu0, precision=prec) as g:
+                g.upload(state)
+                g.set_slabs(sl)
+                g.run("hlld-athena", 0.3, 0.0, 1e9, max_steps=5)
+                outs[prec, sl] = g.download()
+    np.testing.assert_array_equal(outs["fp64", slabs], ref)
+    np.testing.assert_array_equal(outs["fp64", 1], ref)
+    np.testing.assert_array_equal(outs["mixed", slabs], outs["mixed", 1])
+
+
 def test_autotune_keeps_results(gpu):
     from pamhd_amd import solver as S
     cells = (32, 24, 16)
