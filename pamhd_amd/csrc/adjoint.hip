@@ -22,8 +22,8 @@ U^{n+1}, same layout as the state) to lambda^n:
   lflux_kernel   lambda_P of each face from the update (conserved and vb_flux)
                  and the transposed 12-term CT sums over G
   jac_kernel     lambda_L, lambda_R = J^T (dt lambda_P): the HLL Jacobian
-                 by forward-mode dual numbers over the 8 inputs of one side
-                 (two passes), in the same operation order as the forward
+                 by forward-mode dual numbers, four seeded inputs per pass
+                 (four passes), in the same operation order as the forward
   cell_kernel    lambda rho, m, E of each cell, lambda_vb (face-state and
                  vb_flux parts)
   face_kernel    lambda_face = G + (lambda_vb(c) + lambda_vb(c - e_d)) / 2
@@ -40,6 +40,7 @@ namespace pamhd_dev {
 namespace {
 
 constexpr int ABX = 64, ABY = 4;
+constexpr int JN = 4;   /* seeded directions per dual-number pass of the flux Jacobian */
 
 /* value and derivatives along N seeded input directions */
 template <int N> struct Dual {
@@ -408,20 +409,27 @@ __global__ __launch_bounds__(ABX * ABY) void adj_jac_kernel(AdjGrid g, const dou
 			lF[M0 + q] = g.dt * LP[(d * 8 + M0 + (d + q) % 3) * n + c];
 			lF[B0 + q] = g.dt * LP[(d * 8 + B0 + (d + q) % 3) * n + c];
 		}
+#pragma unroll
 		for (int side = 0; side < 2; side++) {
-			Dual<8> L[8], R[8], F[8];
-			for (int q = 0; q < 8; q++) {
-				L[q] = cst<8>(Ld[q]);
-				R[q] = cst<8>(Rd[q]);
-			}
-			Dual<8>* X = side ? R : L;
-			for (int q = 0; q < 8; q++) X[q].d[q] = 1;
+			/* J^T lambda_F for the 8 inputs of this side, JN inputs per pass */
 			double out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-			if (hll_dual<8>(L, R, g.ph, F)) {
-				for (int a = 0; a < 8; a++) {
-					double t = 0;
-					for (int q = 0; q < 8; q++) t += lF[q] * F[q].d[a];
-					out[a] = t;
+#pragma unroll
+			for (int a0 = 0; a0 < 8; a0 += JN) {
+				Dual<JN> L[8], R[8], F[8];
+				for (int q = 0; q < 8; q++) {
+					L[q] = cst<JN>(Ld[q]);
+					R[q] = cst<JN>(Rd[q]);
+				}
+				for (int q = 0; q < JN; q++) {
+					if (side) R[a0 + q].d[q] = 1;
+					else L[a0 + q].d[q] = 1;
+				}
+				if (hll_dual<JN>(L, R, g.ph, F)) {
+					for (int a = 0; a < JN; a++) {
+						double t = 0;
+						for (int q = 0; q < 8; q++) t += lF[q] * F[q].d[a];
+						out[a0 + a] = t;
+					}
 				}
 			}
 			/* back to the global frame: rho, m (3), E, B (3) */

@@ -116,3 +116,45 @@ def test_checkpoint_interval_does_not_change_gradient(gpu):
             grads.append(x.grad.detach().cpu().numpy())
     np.testing.assert_array_equal(grads[0], grads[1])
     np.testing.assert_array_equal(grads[0], grads[2])
+
+
+@pytest.mark.parametrize("cells", [(8, 8, 1), (6, 5, 4)])
+def test_adjoint_step_matches_autograd_of_restated_step(gpu, cells):
+    """One reverse step against torch.autograd of the PyTorch restatement of
+    the same step (tests/torch_step.py) on the CPU, component by component,
+    for a random lambda; the restatement's forward equals the device step to
+    rounding."""
+    import torch
+    import torch_step
+    from pamhd_amd import configs
+    from pamhd_amd.solver import Grid
+    state, length, gamma, mu0 = configs.orszag_tang_vortex(cells, noise=0.05, seed=3)
+    rng = np.random.default_rng(1)
+    state[1:4] *= 1 + 0.1 * rng.normal(size=state[1:4].shape)
+    state[3] += 0.01 * rng.normal(size=state[3].shape)
+    state[7] += 0.01 * rng.normal(size=state[7].shape)
+    state[5:8] += 0.005 * rng.normal(size=state[5:8].shape)    # divergence-carrying
+    n = state.shape[1]
+    nx, ny, nz = cells
+    dt = 0.2 * min(length) / 3
+    lam1 = rng.normal(size=(8, n))
+    with Grid(cells, length, gamma, mu0) as g:
+        g.upload(state)
+        g.step_fixed("hll-athena", dt)
+        fwd = g.download()
+        U = torch.from_numpy(state).to(gpu).contiguous()
+        lam = torch.from_numpy(lam1).to(gpu).contiguous()
+        g.adjoint_step("hll-athena", dt, U.data_ptr(), lam.data_ptr(),
+                       torch.cuda.current_stream().cuda_stream)
+        torch.cuda.synchronize()
+        lam0 = lam.cpu().numpy()
+    x = torch.from_numpy(state.reshape(8, nz, ny, nx).copy()).requires_grad_(True)
+    y = torch_step.step(x, dt, length, gamma, mu0)
+    np.testing.assert_allclose(y.detach().numpy().reshape(8, n), fwd, rtol=1e-11, atol=1e-13)
+    (y * torch.from_numpy(lam1.reshape(8, nz, ny, nx))).sum().backward()
+    ref = x.grad.numpy().reshape(8, n)
+    for q in range(8):
+        err = np.abs(lam0[q] - ref[q]).max() / max(np.abs(ref[q]).max(), 1e-300)
+        print("component", q, "max rel err", err, flush=True)
+    scale = np.abs(ref).max(1, keepdims=True) + 1e-300
+    assert np.all(np.abs(lam0 - ref) <= 1e-9 * scale)

@@ -28,7 +28,8 @@ def _test_box(cells=(12, 10, 14), direction="+x", seed=1):
     sb.cells = tuple(cells)
     sb.start = tuple(-c * h / 2 for c in cells)
     sb.direction = direction
-    sb.inner_radius = 2.2 * h
+    # planet cells must not touch the boundary layer's copy sources
+    sb.inner_radius = min(2.2, min(cells) / 2 - 2.5) * h
     sb.dipoles = [((3e18, -1e18, -9e18), (0.3 * h, -0.2 * h, 0.1 * h))]
     sb.bg_min_distance = 0.7 * h
     sb.bg_value = (1e-9, 0.0, -2e-9)
@@ -58,7 +59,7 @@ def test_box_setup_matches_oracle(gpu):
     np.testing.assert_array_equal(s, ob.init())
 
 
-@pytest.mark.parametrize("name,sid", SOLVERS)
+@pytest.mark.parametrize("name,sid", SOLVERS[:3])
 def test_box_run_bit_exact(gpu, name, sid):
     """Final-init step and 8 main-loop steps with boundaries."""
     from pamhd_amd.swbox import BoxGrid
@@ -73,6 +74,23 @@ def test_box_run_bit_exact(gpu, name, sid):
     print(name, "t", t_g, "max |diff| per component", np.abs(out - ref).max(1), flush=True)
     assert steps_g == 8 and t_g == t_o
     np.testing.assert_array_equal(out, ref)
+
+
+def test_box_roe_reports_the_oracles_error(gpu):
+    """Roe's intermediate state of this cold, fast wind is unphysical at the
+    first solved step (the reference throws); the device path stops with the
+    same error class as the oracle."""
+    from pamhd_amd import _lib
+    from pamhd_amd.swbox import BoxGrid
+    sb = _test_box(seed=2)
+    ob = O.Box(sb)
+    ref = ob.init()
+    st, steps_o, t_o = ob.run(O.ROE, 0.3, 0.0, 1e30, ref, max_steps=8)
+    assert st != 0
+    with BoxGrid(sb) as g:
+        with pytest.raises(_lib.PamhdError) as e:
+            g.run("roe-athena", 0.3, 0.0, 1e30, max_steps=8)
+    assert e.value.code == st
 
 
 @pytest.mark.parametrize("direction", ["-x", "+y", "-z"])
