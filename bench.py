@@ -280,6 +280,61 @@ def adjoint_run(n=512, steps=200, every=20, directions=32, seed=5):
             "amplitude_fd_rel_err": amp_errs}
 
 
+def precision_study(size=512, steps=200, clean_every=10, solver="hlld-athena", cfl=0.3):
+    """The precision workload: the vortex at size^3 with a seeded Gaussian
+    (sigma 1e-3) divergence-carrying face-field perturbation, `steps` HLLD
+    steps in FP64 and in mixed precision, divergence cleaning (multigrid to
+    a 1e-10 relative residual) before the first step and every `clean_every`
+    steps; relative L2 error per variable of the mixed run against the FP64
+    run, and the time of each run with and without the cleanings."""
+    import torch
+    from pamhd_amd import configs
+    from pamhd_amd.solver import Grid
+    cells = (size,) * 3
+    n = size ** 3
+    log("  building %d^3 state" % size)
+    state, length, gamma, mu0 = configs.orszag_tang_vortex(cells, noise=0.01, seed=2024,
+                                                           divergence_sigma=1e-3)
+    out, finals = {}, {}
+    stream = torch.cuda.current_stream().cuda_stream
+    for prec in ("fp64", "mixed"):
+        with Grid(cells, length, gamma, mu0, precision=prec) as g:
+            g.upload(state)
+            g.apply_boundaries(0.0, stream=stream)
+            g.timestep(solver, 0.0, cfl, sync=True, stream=stream)
+            tuning = g.autotune(solver, stream=stream)
+            torch.cuda.synchronize()
+            clean_ms, cycles, div = 0.0, [], []
+            t0 = time.perf_counter()
+            for k in range(steps):
+                if k % clean_every == 0:
+                    c0 = time.perf_counter()
+                    before, rel, cyc, st = g.clean_divergence(tol=1e-10, max_cycles=50, stream=stream)
+                    clean_ms += (time.perf_counter() - c0) * 1e3
+                    cycles.append(cyc)
+                    div.append(before)
+                g.step(solver, 1e300, cfl, sync=False, stream=stream)
+            torch.cuda.synchronize()
+            total_ms = (time.perf_counter() - t0) * 1e3
+            g.check()
+            finals[prec] = g.download()
+            div_end = g.div_face2volume()
+        out[prec] = {"total_ms": round(total_ms, 2), "clean_ms": round(clean_ms, 2),
+                     "step_ms": round((total_ms - clean_ms) / steps, 4),
+                     "cell_updates_per_s": n * steps / ((total_ms - clean_ms) / 1e3),
+                     "v_cycles": cycles, "div_before_cleanings": div, "div_end": div_end,
+                     "tuning": tuning.split("; chosen ")[-1]}
+        log("  %s: %.1f ms total, %.1f ms cleaning, %.3f ms/step" %
+            (prec, total_ms, clean_ms, out[prec]["step_ms"]))
+    del state
+    err = rel_l2(finals["mixed"], finals["fp64"])
+    return {"cells": list(cells), "steps": steps, "clean_every": clean_every, "solver": solver,
+            "cfl": cfl, "fp64": out["fp64"], "mixed": out["mixed"],
+            "mixed_speedup_steps": out["fp64"]["step_ms"] / out["mixed"]["step_ms"],
+            "mixed_speedup_total": out["fp64"]["total_ms"] / out["mixed"]["total_ms"],
+            "mixed_rel_l2_vs_fp64": err}
+
+
 def roofline(res, n, steps):
     """Step roofline: algorithmic 128 B per cell-update over the device time
     of one step (HIP events around the timed steps on the stream the kernels
@@ -315,6 +370,9 @@ def main():
     ap.add_argument("--ensemble-boxes", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=500,
                     help="0 skips the ensemble workload")
+    ap.add_argument("--precision-study", action="store_true",
+                    help="only run the 512^3 mixed-vs-FP64 workload with divergence "
+                         "cleaning every 10 steps and print its JSON")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (scales the "
                          "FETCH_SIZE / WRITE_SIZE counters of a --pmc run)")
@@ -336,6 +394,11 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         log("copied 3 x 1 GiB")
+        return
+    if args.precision_study:
+        log("precision workload: 512^3, 200 HLLD steps, cleaning every 10")
+        print(json.dumps({"metric": "precision workload", "precision_study": precision_study()}),
+              flush=True)
         return
     cells = (args.size,) * 3
     n = args.size ** 3
