@@ -175,8 +175,10 @@ __device__ __forceinline__ void load_face_state_d(const StateArrays& S, const Gr
 	U[B2] = B[(D + 2) % 3];
 }
 
-template <int SOLVER, class T, class PT, int D>
-__global__ __launch_bounds__(BX * BY) void flux_pass_kernel(GridDev g, StateArrays S,
+/* MINB > 1 asks the compiler for MINB blocks per CU (at most 128 VGPRs per
+   lane for 256-thread blocks with MINB = 4), an autotuned occupancy variant */
+template <int SOLVER, class T, class PT, int D, int MINB>
+__global__ __launch_bounds__(BX * BY, MINB) void flux_pass_kernel(GridDev g, StateArrays S,
 	const double* __restrict__ bg, PT* __restrict__ P, const double* __restrict__ dtp,
 	StepStatus* st, unsigned gx, unsigned gy, int remap, int k0)
 {
@@ -797,6 +799,7 @@ struct pamhd_handle {
 		int update_kz = 0;
 		int update_remap = 0;
 		int slabs = 1;   /* > 1: flux pass and update pipelined over z-slabs */
+		int flux_occ = 0; /* 1: flux kernels compiled for 4 blocks per CU */
 	} tune;
 	/* second stream and events of the slab pipeline, created on first use */
 	hipStream_t aux = nullptr;
@@ -888,9 +891,15 @@ void launch_flux_dirs(pamhd_handle* h, hipStream_t st, StepStatus* stat, int k0,
 	const unsigned gx = (h->g.nx + BX - 1) / BX, gy = (h->g.ny + BY - 1) / BY;
 	const dim3 grid(gx * gy * (unsigned)nk), block(BX, BY, 1);
 	const int r = h->tune.flux_remap;
-	flux_pass_kernel<SOLVER, T, PT, 0><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
-	flux_pass_kernel<SOLVER, T, PT, 1><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
-	flux_pass_kernel<SOLVER, T, PT, 2><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
+	if (h->tune.flux_occ) {
+		flux_pass_kernel<SOLVER, T, PT, 0, 4><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
+		flux_pass_kernel<SOLVER, T, PT, 1, 4><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
+		flux_pass_kernel<SOLVER, T, PT, 2, 4><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
+		return;
+	}
+	flux_pass_kernel<SOLVER, T, PT, 0, 1><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
+	flux_pass_kernel<SOLVER, T, PT, 1, 1><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
+	flux_pass_kernel<SOLVER, T, PT, 2, 1><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
 }
 
 /* faces owned by the cells of planes [k0, k0 + nk), all three directions */
@@ -1471,17 +1480,21 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	};
 	std::string report = "flux_pass:";
 	float best_flux = 1e30f;
-	int best_remap = 0;
+	int best_remap = 0, best_occ = 0;
+	for (int occ = 0; occ < 2; occ++)
 	for (int remap = 0; remap < 2; remap++) {
 		h->tune.flux_remap = remap;
+		h->tune.flux_occ = occ;
 		const float ms = time_it(flux);
-		report += " remap" + std::to_string(remap) + "=" + std::to_string(ms) + "ms";
+		report += " remap" + std::to_string(remap) + (occ ? "/occ4" : "") + "=" + std::to_string(ms) + "ms";
 		if (ms < best_flux) {
 			best_flux = ms;
 			best_remap = remap;
+			best_occ = occ;
 		}
 	}
 	h->tune.flux_remap = best_remap;
+	h->tune.flux_occ = best_occ;
 	flux();
 	report += "; update:";
 	const int kzs[] = {0, 4, 8, 16, 32};
@@ -1533,6 +1546,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	HIP_TRY(h, hipStreamSynchronize(st));
 	h->timing = timing;
 	report += "; chosen flux_remap=" + std::to_string(h->tune.flux_remap) +
+		" flux_occ4=" + std::to_string(h->tune.flux_occ) +
 		" update=" + (h->tune.update_kz ? "zmarch" + std::to_string(h->tune.update_kz) : std::string("cell")) +
 		" update_remap=" + std::to_string(h->tune.update_remap) +
 		" slabs=" + std::to_string(h->tune.slabs);
@@ -1544,7 +1558,8 @@ int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_
 {
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (update_kz < 0) return fail(h, PAMHD_E_ARG, "update_kz must be >= 0");
-	h->tune.flux_remap = flux_remap != 0;
+	h->tune.flux_remap = flux_remap & 1;
+	h->tune.flux_occ = (flux_remap >> 1) & 1;
 	h->tune.update_kz = update_kz;
 	h->tune.update_remap = update_remap != 0;
 	return PAMHD_OK;

@@ -112,10 +112,12 @@ def test_grid_steps_face_varying_background(gpu, name, sid):
     np.testing.assert_array_equal(out, ref)
 
 
-@pytest.mark.parametrize("variant", [(0, 0, 0), (1, 0, 1), (0, 4, 0), (1, 8, 1), (0, 32, 1)])
+@pytest.mark.parametrize("variant", [(0, 0, 0), (1, 0, 1), (0, 4, 0), (1, 8, 1), (0, 32, 1),
+                                     (2, 0, 0), (3, 0, 1)])
 def test_launch_variants_bit_exact(gpu, variant):
-    """Every launch variant the autotuner may pick (block order, per-cell or
-    z-marching update with its planes per block) gives the oracle's bits;
+    """Every launch variant the autotuner may pick (block order, occupancy
+    build of the flux kernels, per-cell or z-marching update with its planes
+    per block) gives the oracle's bits;
     uneven sizes exercise partial tiles and chunk ends."""
     from pamhd_amd import solver as S
     cells = (70, 9, 13)
