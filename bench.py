@@ -158,7 +158,9 @@ def divergence_cleaning(cells, cycles_max=50):
 def ensemble_run(nbox, steps, solver="hll-athena", cfl=0.3, seed=2024):
     """Config of the ensemble workload: `nbox` solar-wind boxes with seeded
     edges in {32, 48, 64, 96, 128}, HLL, `steps` main-loop steps after the
-    final-init step.  Times the ensemble (one launch per stage for all boxes,
+    final-init step, without the IMF (the boxes stop within 50-200 steps
+    with it; see configs.solar_wind_ensemble).  Times the ensemble (one
+    launch per stage for all boxes,
     steps replayed from a captured HIP graph, autotuned tile shape) against a
     loop over the boxes with one handle each (plain launches), on the same
     stream; results of the two are compared box by box."""
@@ -166,7 +168,7 @@ def ensemble_run(nbox, steps, solver="hll-athena", cfl=0.3, seed=2024):
     import torch
     from pamhd_amd import configs
     from pamhd_amd.swbox import BoxGrid, Ensemble
-    boxes = configs.solar_wind_ensemble(nbox=nbox, seed=seed)
+    boxes = configs.solar_wind_ensemble(nbox=nbox, seed=seed, imf=False)
     stream = torch.cuda.current_stream().cuda_stream
     t0 = time.perf_counter()
     with Ensemble(boxes) as e:
@@ -210,7 +212,7 @@ def ensemble_run(nbox, steps, solver="hll-athena", cfl=0.3, seed=2024):
             "ensemble_cell_updates_per_s": normal * steps / ens_s,
             "per_box_loop_cell_updates_per_s": normal * steps / loop_s,
             "speedup_vs_per_box_loop": loop_s / ens_s,
-            "tuning": tune.split("; chosen ")[-1], "graph": True,
+            "tuning": tune.split("; chosen ")[-1], "graph": True, "imf": False,
             "results_identical": bool(same), "min_dt": float(dts.min())}
 
 
@@ -221,8 +223,8 @@ def adjoint_run(n=512, steps=200, every=20, directions=32, seed=5):
     magnetic energy of a fixed 64 x 64 window at the end.  Times the forward
     rollout and the backward (hand-written adjoint, checkpoints every
     `every` steps) and checks the gradient against central differences
-    (h = 1e-6) along `directions` random directions and for the 4
-    amplitudes."""
+    (h = 1e-6) along `directions` smooth random directions (random-Fourier
+    fields of every state component) and for the 4 amplitudes."""
     import numpy as np
     import torch
     from pamhd_amd import configs
@@ -254,12 +256,12 @@ def adjoint_run(n=512, steps=200, every=20, directions=32, seed=5):
         loss(rollout(x, g, "hll-athena", dts, every)).backward()
         lam0 = x.grad.detach()
         rng = np.random.default_rng(17)
-        scale = U0.abs().mean(1, keepdim=True).clamp_min(1e-3)
+        scale = U0.abs().mean(1).clamp_min(1e-3).cpu().numpy()
         h = 1e-6
         errs = []
         with torch.no_grad():
             for _ in range(directions):
-                d = torch.as_tensor(rng.normal(size=U0.shape), device=dev) * scale
+                d = torch.as_tensor(configs.smooth_direction(n, scale, rng), device=dev)
                 lp = loss(rollout(U0 + h * d, g, "hll-athena", dts, every)).item()
                 lm = loss(rollout(U0 - h * d, g, "hll-athena", dts, every)).item()
                 fd, ad = (lp - lm) / (2 * h), float((lam0 * d).sum())

@@ -51,8 +51,13 @@ struct GridDev {
 	Phys ph;
 };
 
+/* the flux pass spreads its per-block maxima over NSLOT addresses per
+   dimension (slot = block id mod NSLOT), so 10^5 blocks do not all update
+   one word; the dt kernel reduces the slots */
+constexpr int NSLOT = 64;
+
 struct StepStatus {
-	unsigned long long smax_bits[3];
+	unsigned long long smax_bits[3][NSLOT];
 	unsigned int zero[3];
 	int err;
 	long long err_face;
@@ -135,7 +140,7 @@ __device__ __forceinline__ void reduce_speed(StepStatus* st, int d, double s, un
 			z += mz[w];
 		}
 		const unsigned long long bits = m > 0 ? (unsigned long long)__double_as_longlong(m) : 0ull;
-		atomicMax(&st->smax_bits[d], bits);
+		atomicMax(&st->smax_bits[d][blockIdx.x % NSLOT], bits);
 		if (z) atomicAdd(&st->zero[d], z);
 	}
 }
@@ -583,10 +588,23 @@ __global__ __launch_bounds__(BX * BY) void update_cell_kernel(GridDev g,
 
 /* lagged CFL step of minimize_timestep + set_minmax_substepping_period with a
    single substep (source/mhd/solve.hpp:1961-2006, substepping.hpp:303-338) */
-__global__ void dt_kernel(const StepStatus* __restrict__ prev, StepStatus* __restrict__ next,
-	double* __restrict__ dt_out, double cfl, double max_dt, double dx, double dy, double dz,
-	int fixed)
+/* one wave: lane s reduces slot s of each dimension (max of the ordered bit
+   patterns, i.e. of the non-negative speeds) and clears it for the next pass */
+__global__ __launch_bounds__(NSLOT) void dt_kernel(const StepStatus* __restrict__ prev,
+	StepStatus* __restrict__ next, double* __restrict__ dt_out, double cfl, double max_dt,
+	double dx, double dy, double dz, int fixed)
 {
+	const int lane = threadIdx.x;
+	unsigned long long m[3];
+	for (int d = 0; d < 3; d++) {
+		m[d] = prev->smax_bits[d][lane];
+		for (int off = NSLOT / 2; off > 0; off >>= 1) {
+			const unsigned long long o = __shfl_xor(m[d], off, NSLOT);
+			m[d] = m[d] < o ? o : m[d];
+		}
+		next->smax_bits[d][lane] = 0;
+	}
+	if (lane != 0) return;
 	const double len[3] = {dx, dy, dz};
 	double cfl_dt = DBL_MAX;
 	for (int d = 0; d < 3 && !fixed; d++) {
@@ -594,13 +612,10 @@ __global__ void dt_kernel(const StepStatus* __restrict__ prev, StepStatus* __res
 			cfl_dt = 0;
 			break;
 		}
-		cfl_dt = smin(cfl_dt, cfl * len[d] / __longlong_as_double((long long)prev->smax_bits[d]));
+		cfl_dt = smin(cfl_dt, cfl * len[d] / __longlong_as_double((long long)m[d]));
 	}
 	*dt_out = (fixed || max_dt <= cfl_dt) ? max_dt : cfl_dt;
-	for (int d = 0; d < 3; d++) {
-		next->smax_bits[d] = 0;
-		next->zero[d] = 0;
-	}
+	for (int d = 0; d < 3; d++) next->zero[d] = 0;
 	next->err = 0;
 	next->err_face = -1;
 }
@@ -1034,7 +1049,7 @@ int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
 	StepStatus* next = h->status + (1 - h->last);
 	{
 		TimedLaunch t(h, st, K_DT);
-		dt_kernel<<<1, 1, 0, st>>>(prev, next, h->dt, cfl, max_dt, h->g.dx, h->g.dy, h->g.dz, fixed_dt);
+		dt_kernel<<<1, NSLOT, 0, st>>>(prev, next, h->dt, cfl, max_dt, h->g.dx, h->g.dy, h->g.dz, fixed_dt);
 	}
 	HIP_TRY(h, hipGetLastError());
 	const int e = h->precision == PAMHD_MIXED
@@ -1104,7 +1119,7 @@ int pamhd_create(const pamhd_grid_desc* grid, const pamhd_physics* physics, int 
 	StepStatus init[2];
 	for (auto& s : init) {
 		for (int d = 0; d < 3; d++) {
-			s.smax_bits[d] = 0;
+			for (int q = 0; q < NSLOT; q++) s.smax_bits[d][q] = 0;
 			s.zero[d] = 1;
 		}
 		s.err = 0;
@@ -1277,8 +1292,10 @@ int pamhd_max_signal_speeds(pamhd_handle* h, double smax_out[3])
 	StepStatus st;
 	HIP_TRY(h, hipMemcpy(&st, h->status + h->last, sizeof(StepStatus), hipMemcpyDeviceToHost));
 	for (int d = 0; d < 3; d++) {
+		unsigned long long m = 0;
+		for (int q = 0; q < NSLOT; q++) m = m < st.smax_bits[d][q] ? st.smax_bits[d][q] : m;
 		double v;
-		memcpy(&v, &st.smax_bits[d], sizeof(v));
+		memcpy(&v, &m, sizeof(v));
 		smax_out[d] = v;
 	}
 	return PAMHD_OK;
@@ -1474,7 +1491,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 		return best / 3;
 	};
 	auto flux = [&]() {
-		dt_kernel<<<1, 1, 0, st>>>(prev, next, h->dt, 0.0, 0.0, h->g.dx, h->g.dy, h->g.dz, 0);
+		dt_kernel<<<1, NSLOT, 0, st>>>(prev, next, h->dt, 0.0, 0.0, h->g.dx, h->g.dy, h->g.dz, 0);
 		if (h->precision == PAMHD_MIXED) launch_flux_pass<float, float>(h, solver, st, next);
 		else launch_flux_pass<double, double>(h, solver, st, next);
 	};
@@ -1528,7 +1545,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 			if (S > 1 && S > h->g.nz / 2) continue;
 			h->tune.slabs = S;
 			const float ms = time_it([&]() {
-				dt_kernel<<<1, 1, 0, st>>>(prev, next, h->dt, 0.0, 0.0, h->g.dx, h->g.dy, h->g.dz, 0);
+				dt_kernel<<<1, NSLOT, 0, st>>>(prev, next, h->dt, 0.0, 0.0, h->g.dx, h->g.dy, h->g.dz, 0);
 				if (h->precision == PAMHD_MIXED) launch_step_body<float, float>(h, solver, st, next, 3);
 				else launch_step_body<double, double>(h, solver, st, next, 3);
 			});

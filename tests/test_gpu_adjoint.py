@@ -1,6 +1,7 @@
 """The hand-written adjoint of the periodic MHD step (HLL, frozen time-step
 schedule) behind torch.autograd, against FP64 central finite differences of
-the FP64 forward, on a small 2-D version of the adjoint workload.
+the FP64 forward, on a small 2-D version of the adjoint workload, and
+against torch.autograd of a PyTorch restatement of one step.
 
 The forward of the rollout is the library's FP64 step, so it equals an
 ordinary run with the same time steps bit for bit.  Gradients are checked
@@ -53,7 +54,10 @@ def test_rollout_forward_is_the_fp64_step(gpu):
 
 
 def test_adjoint_matches_finite_differences(gpu):
+    """Central differences along smooth random directions of all 8 state
+    components (random-Fourier fields, |k| <= 8)."""
     import torch
+    from pamhd_amd import configs
     from pamhd_amd.autograd import rollout
     w, amps, U0, g, dts = _setup(gpu)
     with g:
@@ -63,11 +67,11 @@ def test_adjoint_matches_finite_differences(gpu):
         lam0 = x.grad.detach()
         assert torch.isfinite(lam0).all()
         rng = np.random.default_rng(17)
-        scale = U0.abs().mean(1, keepdim=True).clamp_min(1e-3)
+        scale = U0.abs().mean(1).clamp_min(1e-3).cpu().numpy()
         h = 1e-6
         errs = []
         for q in range(8):
-            d = torch.as_tensor(rng.normal(size=U0.shape), device=gpu) * scale
+            d = torch.as_tensor(configs.smooth_direction(N, scale, rng), device=gpu)
             with torch.no_grad():
                 lp = _loss(rollout(U0 + h * d, g, "hll-athena", dts, every=EVERY)).item()
                 lm = _loss(rollout(U0 - h * d, g, "hll-athena", dts, every=EVERY)).item()

@@ -164,7 +164,7 @@ def test_config_box_runs_and_keeps_inflow(gpu):
     inflow wall keeps the solar-wind state and the interior divergence stays
     at roundoff."""
     from pamhd_amd.swbox import BoxGrid
-    sb = configs.solar_wind_ensemble(nbox=1, seed=7, edges=(32,))[0]
+    sb = configs.solar_wind_ensemble(nbox=1, seed=7, edges=(32,), imf=False)[0]
     with BoxGrid(sb) as g:
         steps, t = g.run("hll-athena", 0.3, 0.0, 1e30, max_steps=40)
         s = g.download()

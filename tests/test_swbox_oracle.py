@@ -187,6 +187,7 @@ def test_dipole_background_closed_form():
     sb = _box(edge=8)
     m = np.array([1e16, -2e16, -5e16])
     sb.dipoles = [(tuple(m), (0.0, 0.0, 0.0))]
+    sb.bg_value = (0.0, 0.0, 0.0)
     h = sb.cell_length[0]
     sb.bg_min_distance = 1.2 * h
     b = O.Box(sb)
