@@ -235,10 +235,11 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen,
 	void* stream);
 
 /* Sets the launch variants directly: flux_remap bit 0 = XCD-aware block
-   order of the flux pass, bit 1 = flux kernels built for 4 blocks per CU;
-   update_remap = XCD-aware order of the update pass; update_kz = 0 for the
-   per-cell update or the planes per block of the z-marching update.  All
-   give identical results. */
+   order of the flux pass, bit 1 = planes fastest in the block order, bit 2 =
+   flux kernels built for 4 blocks per CU; update_remap bits 0 and 1 the same
+   orders for the update pass (bit 0 only for the z-marching update);
+   update_kz = 0 for the per-cell update or the planes per block of the
+   z-marching update.  All give identical results. */
 int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_remap);
 
 /* Pipelines the flux pass and the per-cell update over `slabs` z-slabs on

@@ -83,6 +83,30 @@ __device__ __forceinline__ unsigned logical_block(unsigned id, unsigned nblocks)
 	return (id % NXCD) * (nblocks / NXCD) + id / NXCD;
 }
 
+/*
+Tile (bx, by) and plane k of logical block lb of a launch over gx * gy
+column tiles and gridDim / (gx * gy) planes from k0.  Bit 1 of `order`
+walks the planes fastest (consecutive blocks take the same column tile in
+consecutive planes, so the neighbouring planes a block reads were just
+fetched into the same XCD's L2 when bit 0, the XCD-aware order, is set),
+otherwise x tiles are fastest.
+*/
+__device__ __forceinline__ void tile_of_block(unsigned lb, unsigned gx, unsigned gy, unsigned nblocks,
+	int order, int k0, unsigned& bx, unsigned& by, unsigned& k)
+{
+	if (order & 2) {
+		const unsigned nk = nblocks / (gx * gy);
+		const unsigned r = lb / nk;
+		k = k0 + lb % nk;
+		bx = r % gx;
+		by = r / gx;
+	} else {
+		bx = lb % gx;
+		by = (lb / gx) % gy;
+		k = k0 + lb / (gx * gy);
+	}
+}
+
 /* cell state rotated into the frame of a d-face (get_rotated_vector,
    source/mhd/common.hpp:612-648; vol B of update_vol_B, solve.hpp:1661) */
 __device__ __forceinline__ void load_face_state(const double* __restrict__ s, long long n,
@@ -187,8 +211,9 @@ __global__ __launch_bounds__(BX * BY, MINB) void flux_pass_kernel(GridDev g, Sta
 	const double* __restrict__ bg, PT* __restrict__ P, const double* __restrict__ dtp,
 	StepStatus* st, unsigned gx, unsigned gy, int remap, int k0)
 {
-	const unsigned lb = remap ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
-	const unsigned bx = lb % gx, by = (lb / gx) % gy, k = k0 + lb / (gx * gy);
+	const unsigned lb = (remap & 1) ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+	unsigned bx, by, k;
+	tile_of_block(lb, gx, gy, gridDim.x, remap, k0, bx, by, k);
 	const int i = bx * BX + threadIdx.x;
 	const int j = by * BY + threadIdx.y;
 	const int n = (int)g.n;
@@ -517,8 +542,9 @@ __global__ __launch_bounds__(BX * BY) void update_cell_kernel(GridDev g,
 	const double* __restrict__ s, double* __restrict__ out,
 	const PT* __restrict__ P, int refits, unsigned gx, unsigned gy, int remap, int k0)
 {
-	const unsigned lb = remap ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
-	const unsigned bx = lb % gx, by = (lb / gx) % gy, k = k0 + lb / (gx * gy);
+	const unsigned lb = (remap & 1) ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+	unsigned bx, by, k;
+	tile_of_block(lb, gx, gy, gridDim.x, remap, k0, bx, by, k);
 	const int i = bx * BX + threadIdx.x;
 	const int j = by * BY + threadIdx.y;
 	if (i >= g.nx || j >= g.ny) return;
@@ -1499,7 +1525,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	float best_flux = 1e30f;
 	int best_remap = 0, best_occ = 0;
 	for (int occ = 0; occ < 2; occ++)
-	for (int remap = 0; remap < 2; remap++) {
+	for (int remap = 0; remap < 4; remap++) {
 		h->tune.flux_remap = remap;
 		h->tune.flux_occ = occ;
 		const float ms = time_it(flux);
@@ -1519,7 +1545,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	pamhd_handle::Tune best = h->tune;
 	for (int kz : kzs) {
 		if (kz > 0 && kz > h->g.nz) continue;
-		for (int remap = 0; remap < 2; remap++) {
+		for (int remap = 0; remap < (kz ? 2 : 4); remap++) {
 			h->tune.update_kz = kz;
 			h->tune.update_remap = remap;
 			const float ms = time_it([&]() {
@@ -1575,10 +1601,10 @@ int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_
 {
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (update_kz < 0) return fail(h, PAMHD_E_ARG, "update_kz must be >= 0");
-	h->tune.flux_remap = flux_remap & 1;
-	h->tune.flux_occ = (flux_remap >> 1) & 1;
+	h->tune.flux_remap = flux_remap & 3;
+	h->tune.flux_occ = (flux_remap >> 2) & 1;
 	h->tune.update_kz = update_kz;
-	h->tune.update_remap = update_remap != 0;
+	h->tune.update_remap = update_kz ? (update_remap & 1) : (update_remap & 3);
 	return PAMHD_OK;
 }
 

@@ -76,13 +76,30 @@ def test_box_run_bit_exact(gpu, name, sid):
     np.testing.assert_array_equal(out, ref)
 
 
+def test_box_roe_run_bit_exact(gpu):
+    """Roe with the wind's field in the background only: 8 steps, bit for bit."""
+    from pamhd_amd.swbox import BoxGrid
+    sb = _test_box(seed=2)
+    ob = O.Box(sb)
+    ref = ob.init()
+    st, steps_o, t_o = ob.run(O.ROE, 0.3, 0.0, 1e30, ref, max_steps=8)
+    assert st == 0 and steps_o == 8
+    with BoxGrid(sb) as g:
+        steps_g, t_g = g.run("roe-athena", 0.3, 0.0, 1e30, max_steps=8)
+        out = g.download()
+    assert steps_g == 8 and t_g == t_o
+    np.testing.assert_array_equal(out, ref)
+
+
 def test_box_roe_reports_the_oracles_error(gpu):
-    """Roe's intermediate state of this cold, fast wind is unphysical at the
-    first solved step (the reference throws); the device path stops with the
-    same error class as the oracle."""
+    """With a field of a few nT in the wind's own state, Roe's intermediate
+    state of this cold, fast wind is unphysical at the first solved step (the
+    reference throws); the device path stops with the same error class as
+    the oracle."""
     from pamhd_amd import _lib
     from pamhd_amd.swbox import BoxGrid
     sb = _test_box(seed=2)
+    sb.magnetic_field = (0.0, -5e-9, 4e-9)
     ob = O.Box(sb)
     ref = ob.init()
     st, steps_o, t_o = ob.run(O.ROE, 0.3, 0.0, 1e30, ref, max_steps=8)
