@@ -15,9 +15,9 @@ Data layout in HBM (structure of arrays, FP64, component-major):
 
 One step = dt_kernel (1 thread: lagged CFL step from the previous pass) ->
 flux_pass_kernel (one thread per face, all faces of one dimension in a
-z-slab of blocks) -> update_zmarch_kernel (column tiles marching in z:
-constrained-transport face update, conserved update, constant-pressure
-energy refits).  In mixed precision the flux array holds FP32 values.
+z-slab of blocks) -> update_cell_kernel or update_column_kernel (one thread
+per cell, or column tiles marching in z: constrained-transport face update,
+conserved update, constant-pressure energy refits).  In mixed precision the flux array holds FP32 values.
 */
 #include <hip/hip_runtime.h>
 
@@ -312,98 +312,88 @@ struct Nb {
 	}
 };
 
+/*
+The 12 edge terms of one face change, summed in a fixed order; the
+arguments are the terms in that order (see dBx_at, dBy_at, dBz_at).
+*/
+__device__ __forceinline__ double dBx_sum(double a0, double a1, double a2, double a3, double a4,
+	double a5, double a6, double a7, double a8, double a9, double a10, double a11)
+{
+	double r = 0;
+	r += a0; r -= a1; r -= a2; r += a3; r += a4; r += a5;
+	r -= a6; r -= a7; r -= a8; r -= a9; r += a10; r += a11;
+	return r;
+}
+
+__device__ __forceinline__ double dBy_sum(double a0, double a1, double a2, double a3, double a4,
+	double a5, double a6, double a7, double a8, double a9, double a10, double a11)
+{
+	double r = 0;
+	r += a0; r -= a1; r -= a2; r += a3; r += a4; r += a5;
+	r -= a6; r -= a7; r -= a8; r += a9; r -= a10; r += a11;
+	return r;
+}
+
+/* dBz_sum has the signs of dBx_sum */
+__device__ __forceinline__ double dBz_sum(double a0, double a1, double a2, double a3, double a4,
+	double a5, double a6, double a7, double a8, double a9, double a10, double a11)
+{
+	return dBx_sum(a0, a1, a2, a3, a4, a5, a6, a7, a8, a9, a10, a11);
+}
+
 /* dB of the x-face owned by (i, j, k); im = i-1, jm/jp = j-+1, km/kp = k-+1 */
 template <class PT> __device__ __forceinline__ double dBx_at(const EdgeTerms<PT>& E, const Nb& L,
 	int i, int im, int j, int jm, int jp, int k, int km, int kp)
 {
-	double r = 0;
-	r += E.ezx(L(i, jp, k));
-	r -= E.ezx(L(i, jm, k));
-	r -= E.eyx(L(i, j, kp));
-	r += E.eyx(L(i, j, km));
-	r += E.ezy(L(i, jp, k));
-	r += E.ezy(L(im, jp, k));
-	r -= E.ezy(L(i, j, k));
-	r -= E.ezy(L(im, j, k));
-	r -= E.eyz(L(i, j, kp));
-	r -= E.eyz(L(im, j, kp));
-	r += E.eyz(L(i, j, k));
-	r += E.eyz(L(im, j, k));
-	return r;
+	return dBx_sum(
+		E.ezx(L(i, jp, k)), E.ezx(L(i, jm, k)), E.eyx(L(i, j, kp)), E.eyx(L(i, j, km)),
+		E.ezy(L(i, jp, k)), E.ezy(L(im, jp, k)), E.ezy(L(i, j, k)), E.ezy(L(im, j, k)),
+		E.eyz(L(i, j, kp)), E.eyz(L(im, j, kp)), E.eyz(L(i, j, k)), E.eyz(L(im, j, k)));
 }
 
 template <class PT> __device__ __forceinline__ double dBy_at(const EdgeTerms<PT>& E, const Nb& L,
 	int i, int im, int ip, int j, int jm, int k, int km, int kp)
 {
-	double r = 0;
-	r += E.exy(L(i, j, kp));
-	r -= E.exy(L(i, j, km));
-	r -= E.ezy(L(ip, j, k));
-	r += E.ezy(L(im, j, k));
-	r += E.exz(L(i, j, kp));
-	r += E.exz(L(i, jm, kp));
-	r -= E.exz(L(i, j, k));
-	r -= E.exz(L(i, jm, k));
-	r -= E.ezx(L(ip, j, k));
-	r += E.ezx(L(i, j, k));
-	r -= E.ezx(L(ip, jm, k));
-	r += E.ezx(L(i, jm, k));
-	return r;
+	return dBy_sum(
+		E.exy(L(i, j, kp)), E.exy(L(i, j, km)), E.ezy(L(ip, j, k)), E.ezy(L(im, j, k)),
+		E.exz(L(i, j, kp)), E.exz(L(i, jm, kp)), E.exz(L(i, j, k)), E.exz(L(i, jm, k)),
+		E.ezx(L(ip, j, k)), E.ezx(L(i, j, k)), E.ezx(L(ip, jm, k)), E.ezx(L(i, jm, k)));
 }
 
 template <class PT> __device__ __forceinline__ double dBz_at(const EdgeTerms<PT>& E, const Nb& L,
 	int i, int im, int ip, int j, int jm, int jp, int k, int km)
 {
-	double r = 0;
-	r += E.eyz(L(ip, j, k));
-	r -= E.eyz(L(im, j, k));
-	r -= E.exz(L(i, jp, k));
-	r += E.exz(L(i, jm, k));
-	r += E.eyx(L(ip, j, k));
-	r += E.eyx(L(ip, j, km));
-	r -= E.eyx(L(i, j, k));
-	r -= E.eyx(L(i, j, km));
-	r -= E.exy(L(i, jp, k));
-	r -= E.exy(L(i, jp, km));
-	r += E.exy(L(i, j, k));
-	r += E.exy(L(i, j, km));
-	return r;
+	return dBz_sum(
+		E.eyz(L(ip, j, k)), E.eyz(L(im, j, k)), E.exz(L(i, jp, k)), E.exz(L(i, jm, k)),
+		E.eyx(L(ip, j, k)), E.eyx(L(ip, j, km)), E.eyx(L(i, j, k)), E.eyx(L(i, j, km)),
+		E.exy(L(i, jp, k)), E.exy(L(i, jp, km)), E.exy(L(i, j, k)), E.exy(L(i, j, km)));
 }
 
 /*
-Update pass: update_mhd_state + update_vol_B of source/mhd/solve.hpp:1503-1683,
-fused into one z-marching pass.  A block owns a UBX x UBY column tile and a
-chunk of kz planes; thread (i, j) walks k.  At plane k it
-  * computes the new face fields of its three faces (the z-face first, which
-    also completes the cell below),
-  * gets the new +x / +y faces from its neighbours through LDS (threads on
-    the tile's +x / +y edge recompute them),
-  * applies the x and y flux differences to its cell and carries the
-    partial state; the z difference and the energy refits follow at plane
-    k + 1, once the cell's +z face field is known.
-Every flux value is loaded once per plane by its owner and reused through
-L1 by the neighbours; the order of every sum is that of the reference
-(x, then y, then z per conserved variable).
+Update pass, column-marching variant: update_mhd_state + update_vol_B of
+source/mhd/solve.hpp:1503-1683 with thread (i, j) of a UBX x UBY column
+tile walking the planes [k0, k0 + kz) of its column.  Of the edge terms the
+six face changes around a cell use, those of plane k + 1 are loaded once,
+at plane k, and carried in registers through the next two planes; the
++z flux row of plane k + 1 becomes the next cell's own -z row, and the new
++z face field the next cell's -z face.  Per plane only the 12 terms of the
+edges parallel to z are loaded fresh (their x and y neighbours are loaded
+by the adjacent threads at the same time).  Every sum has the order of
+update_cell_kernel, so both kernels give identical results.
 */
 template <class PT>
-__global__ __launch_bounds__(UBX * UBY) void update_zmarch_kernel(GridDev g,
+__global__ __launch_bounds__(UBX * UBY) void update_column_kernel(GridDev g,
 	const double* __restrict__ s, double* __restrict__ out,
 	const PT* __restrict__ P, int refits, int kz, unsigned gx, unsigned gy, int remap)
 {
-	__shared__ double sbx[2][UBY][UBX];
-	__shared__ double sby[2][UBY][UBX];
 	const unsigned lb = remap ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
 	const unsigned bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
-	const int tx = threadIdx.x, ty = threadIdx.y;
-	const int i = bx * UBX + tx;
-	const int j = by * UBY + ty;
-	const bool active = i < g.nx && j < g.ny;
-	const int ic = active ? i : 0, jc = active ? j : 0;
-	const int im = wrap_dec(ic, g.nx), ip = wrap_inc(ic, g.nx), ipp = wrap_inc(ip, g.nx);
-	const int jm = wrap_dec(jc, g.ny), jp = wrap_inc(jc, g.ny), jpp = wrap_inc(jp, g.ny);
-	const bool own_xp = tx + 1 < UBX && i + 1 < g.nx;   /* +x neighbour in this block */
-	const bool own_yp = ty + 1 < UBY && j + 1 < g.ny;
+	const int i = bx * UBX + threadIdx.x;
+	const int j = by * UBY + threadIdx.y;
+	if (i >= g.nx || j >= g.ny) return;
 	const long long n = g.n;
+	const int im = wrap_dec(i, g.nx), ip = wrap_inc(i, g.nx);
+	const int jm = wrap_dec(j, g.ny), jp = wrap_inc(j, g.ny);
 	const Nb L{g.nx, g.ny};
 	const EdgeTerms<PT> E{P, n, g.dx, g.dy, g.dz};
 	const double axy = g.dx * g.dy, axz = g.dx * g.dz, ayz = g.dy * g.dz;
@@ -411,131 +401,122 @@ __global__ __launch_bounds__(UBX * UBY) void update_zmarch_kernel(GridDev g,
 	const PT* P0 = P;
 	const PT* P1 = P + 8 * n;
 	const PT* P2 = P + 16 * n;
-	const int k0 = bz * kz;
+	const int k0 = (int)bz * kz;
 	const int k1 = min(k0 + kz, g.nz);
+	if (k0 >= k1) return;
 
-	/* carried from plane k - 1 */
-	double c_rho = 0, c_m[3] = {0, 0, 0}, c_E = 0, c_vb[3] = {0, 0, 0};
-	double c_bx = 0, c_bxp = 0, c_by = 0, c_byp = 0, c_bz = 0;
-	double lo_z[8];
-	for (int q = 0; q < 8; q++) lo_z[q] = 0;
+	/* terms of planes k - 1 (suffix m) and k (suffix c) for k = k0; p / m
+	   before the plane suffix = the cell at +1 / -1 in x (eyx, eyz) or y
+	   (exy, exz) */
+	const int kb = wrap_dec(k0, g.nz);
+	double eyx_m = E.eyx(L(i, j, kb)), eyx_pm = E.eyx(L(ip, j, kb));
+	double exy_m = E.exy(L(i, j, kb)), exy_pm = E.exy(L(i, jp, kb));
+	double eyx_c = E.eyx(L(i, j, k0)), eyx_pc = E.eyx(L(ip, j, k0));
+	double exy_c = E.exy(L(i, j, k0)), exy_pc = E.exy(L(i, jp, k0));
+	double eyz_c = E.eyz(L(i, j, k0)), eyz_mc = E.eyz(L(im, j, k0)), eyz_pc = E.eyz(L(ip, j, k0));
+	double exz_c = E.exz(L(i, j, k0)), exz_mc = E.exz(L(i, jm, k0)), exz_pc = E.exz(L(i, jp, k0));
+	const long long cb = L(i, j, k0);
+	double s7 = s[7 * n + cb];
+	/* new -z face of the first cell (dBz_at at k0) */
+	double bz_new = s7 + dBz_sum(eyz_pc, eyz_mc, exz_pc, exz_mc, eyx_pc, eyx_pm, eyx_c, eyx_m,
+		exy_pc, exy_pm, exy_c, exy_m) / axy;
+	double lo2[8];
+#pragma unroll
+	for (int q = 0; q < 8; q++) lo2[q] = (double)P2[(long long)q * n + cb];
 
-	for (int k = k0; k <= k1; k++) {
-		const int kk = k < g.nz ? k : k - g.nz;
-		const int km = wrap_dec(kk, g.nz), kp = wrap_inc(kk, g.nz);
-		const long long c = L(ic, jc, kk);
-		double bz_new = 0;
-		double hi_z[8];
-		if (active) {
-			bz_new = s[7 * n + c] + dBz_at(E, L, ic, im, ip, jc, jm, jp, kk, km) / axy;
-			for (int q = 0; q < 8; q++) hi_z[q] = (double)P2[(long long)q * n + c];
-		}
-		if (active && k > k0) {
-			/* finish the cell below: z flux difference, new cell field, refits */
-			const double ilz = 1 / len[2];
-			c_rho += ((0.0 + lo_z[RHO]) - (0.0 + hi_z[RHO])) / len[2];
-			for (int q = 0; q < 3; q++) {
-				c_m[q] = c_m[q] + ((0.0 + lo_z[M0 + q]) - (0.0 + hi_z[M0 + q])) * ilz;
-			}
-			c_E += ((0.0 + lo_z[NRJ]) - (0.0 + hi_z[NRJ])) / len[2];
-			for (int q = 0; q < 3; q++) {
-				c_vb[q] = c_vb[q] + ((0.0 + lo_z[B0 + q]) - (0.0 + hi_z[B0 + q])) * ilz;
-			}
-			const double vb_new[3] = {0.5 * (c_bx + c_bxp), 0.5 * (c_by + c_byp), 0.5 * (c_bz + bz_new)};
-			double Ecur = c_E;
-			for (int r = 0; r < refits; r++) {
-				refit_energy(c_rho, c_m, Ecur, r == 0 ? c_vb : vb_new, vb_new, g.ph);
-			}
-			const long long cb = L(ic, jc, wrap_dec(kk, g.nz));
-			out[cb] = c_rho;
-			out[n + cb] = c_m[0];
-			out[2 * n + cb] = c_m[1];
-			out[3 * n + cb] = c_m[2];
-			out[4 * n + cb] = Ecur;
-		}
-		if (k == k1) break;
+	for (int k = k0; k < k1; k++) {
+		const int kp = wrap_inc(k, g.nz);
+		const long long c = L(i, j, k), cxp = L(ip, j, k), cyp = L(i, jp, k), czp = L(i, j, kp);
+		/* edges parallel to z in this plane */
+		const double
+			ezx_c = E.ezx(c), ezx_jm = E.ezx(L(i, jm, k)), ezx_jp = E.ezx(cyp),
+			ezx_p = E.ezx(cxp), ezx_pjm = E.ezx(L(ip, jm, k)), ezx_pjp = E.ezx(L(ip, jp, k)),
+			ezy_c = E.ezy(c), ezy_m = E.ezy(L(im, j, k)), ezy_p = E.ezy(cxp),
+			ezy_jp = E.ezy(cyp), ezy_mjp = E.ezy(L(im, jp, k)), ezy_pjp = E.ezy(L(ip, jp, k));
+		/* plane k + 1 */
+		const double
+			eyx_n = E.eyx(czp), eyx_pn = E.eyx(L(ip, j, kp)),
+			exy_n = E.exy(czp), exy_pn = E.exy(L(i, jp, kp)),
+			eyz_n = E.eyz(czp), eyz_mn = E.eyz(L(im, j, kp)), eyz_pn = E.eyz(L(ip, j, kp)),
+			exz_n = E.exz(czp), exz_mn = E.exz(L(i, jm, kp)), exz_pn = E.exz(L(i, jp, kp));
 
-		/* new x- and y-face fields of this plane */
-		double bx_new = 0, by_new = 0;
-		if (active) {
-			bx_new = s[5 * n + c] + dBx_at(E, L, ic, im, jc, jm, jp, kk, km, kp) / ayz;
-			by_new = s[6 * n + c] + dBy_at(E, L, ic, im, ip, jc, jm, kk, km, kp) / axz;
-			out[5 * n + c] = bx_new;
-			out[6 * n + c] = by_new;
-			out[7 * n + c] = bz_new;
-		}
-		const int par = k & 1;
-		sbx[par][ty][tx] = bx_new;
-		sby[par][ty][tx] = by_new;
-		__syncthreads();
-		if (active) {
-			double bxp_new, byp_new;
-			const long long cxp = L(ip, jc, kk), cyp = L(ic, jp, kk);
-			if (own_xp) {
-				bxp_new = sbx[par][ty][tx + 1];
-			} else {
-				bxp_new = s[5 * n + cxp] + dBx_at(E, L, ip, ic, jc, jm, jp, kk, km, kp) / ayz;
-			}
-			if (own_yp) {
-				byp_new = sby[par][ty + 1][tx];
-			} else {
-				byp_new = s[6 * n + cyp] + dBy_at(E, L, ic, im, ip, jp, jc, kk, km, kp) / axz;
-			}
+		const double s7n = s[7 * n + czp];
+		const double bx_new = s[5 * n + c] + dBx_sum(ezx_jp, ezx_jm, eyx_n, eyx_m, ezy_jp, ezy_mjp,
+			ezy_c, ezy_m, eyz_n, eyz_mn, eyz_c, eyz_mc) / ayz;
+		const double by_new = s[6 * n + c] + dBy_sum(exy_n, exy_m, ezy_p, ezy_m, exz_n, exz_mn,
+			exz_c, exz_mc, ezx_p, ezx_c, ezx_pjm, ezx_jm) / axz;
+		const double bxp_new = s[5 * n + cxp] + dBx_sum(ezx_pjp, ezx_pjm, eyx_pn, eyx_pm, ezy_pjp,
+			ezy_jp, ezy_p, ezy_c, eyz_pn, eyz_n, eyz_pc, eyz_c) / ayz;
+		const double byp_new = s[6 * n + cyp] + dBy_sum(exy_pn, exy_pm, ezy_pjp, ezy_mjp, exz_pn,
+			exz_n, exz_pc, exz_c, ezx_pjp, ezx_jp, ezx_p, ezx_c) / axz;
+		const double bzp_new = s7n + dBz_sum(eyz_pn, eyz_mn, exz_pn, exz_mn, eyx_pn, eyx_pc, eyx_n,
+			eyx_c, exy_pn, exy_pc, exy_n, exy_c) / axy;
+		out[5 * n + c] = bx_new;
+		out[6 * n + c] = by_new;
+		out[7 * n + c] = bz_new;
 
-			/* x and y flux differences of this plane's cell */
-			double rho = s[c];
-			double m[3] = {s[n + c], s[2 * n + c], s[3 * n + c]};
-			double En = s[4 * n + c];
-			const long long czp = L(ic, jc, kp);
-			double vb[3] = {
-				0.5 * (s[5 * n + c] + s[5 * n + cxp]),
-				0.5 * (s[6 * n + c] + s[6 * n + cyp]),
-				0.5 * (s[7 * n + c] + s[7 * n + czp])};
-			const PT* Pd[2] = {P0, P1};
-			const long long chi[2] = {cxp, cyp};
-			for (int d = 0; d < 2; d++) {
-				rho += ((0.0 + (double)Pd[d][c]) - (0.0 + (double)Pd[d][chi[d]])) / len[d];
-			}
-			for (int d = 0; d < 2; d++) {
-				const double il = 1 / len[d];
-				for (int q = 0; q < 3; q++) {
-					const long long o = (long long)(M0 + q) * n;
-					m[q] = m[q] + ((0.0 + (double)Pd[d][o + c]) - (0.0 + (double)Pd[d][o + chi[d]])) * il;
-				}
-			}
-			for (int d = 0; d < 2; d++) {
-				const long long o = (long long)NRJ * n;
-				En += ((0.0 + (double)Pd[d][o + c]) - (0.0 + (double)Pd[d][o + chi[d]])) / len[d];
-			}
-			for (int d = 0; d < 2; d++) {
-				const double il = 1 / len[d];
-				for (int q = 0; q < 3; q++) {
-					const long long o = (long long)(B0 + q) * n;
-					vb[q] = vb[q] + ((0.0 + (double)Pd[d][o + c]) - (0.0 + (double)Pd[d][o + chi[d]])) * il;
-				}
-			}
-			c_rho = rho;
-			for (int q = 0; q < 3; q++) {
-				c_m[q] = m[q];
-				c_vb[q] = vb[q];
-			}
-			c_E = En;
-			c_bx = bx_new;
-			c_bxp = bxp_new;
-			c_by = by_new;
-			c_byp = byp_new;
-			c_bz = bz_new;
-			for (int q = 0; q < 8; q++) lo_z[q] = hi_z[q];
+		double hi2[8];
+#pragma unroll
+		for (int q = 0; q < 8; q++) hi2[q] = (double)P2[(long long)q * n + czp];
+		auto lo = [&](int d, int q) -> double {
+			return d == 0 ? (double)P0[(long long)q * n + c]
+				: d == 1 ? (double)P1[(long long)q * n + c] : lo2[q];
+		};
+		auto hi = [&](int d, int q) -> double {
+			return d == 0 ? (double)P0[(long long)q * n + cxp]
+				: d == 1 ? (double)P1[(long long)q * n + cyp] : hi2[q];
+		};
+		double rho = s[c];
+		double m[3] = {s[n + c], s[2 * n + c], s[3 * n + c]};
+		double En = s[4 * n + c];
+		double vb[3] = {
+			0.5 * (s[5 * n + c] + s[5 * n + cxp]),
+			0.5 * (s[6 * n + c] + s[6 * n + cyp]),
+			0.5 * (s7 + s7n)};
+#pragma unroll
+		for (int d = 0; d < 3; d++) rho += ((0.0 + lo(d, RHO)) - (0.0 + hi(d, RHO))) / len[d];
+#pragma unroll
+		for (int d = 0; d < 3; d++) {
+			const double il = 1 / len[d];
+#pragma unroll
+			for (int q = 0; q < 3; q++) m[q] = m[q] + ((0.0 + lo(d, M0 + q)) - (0.0 + hi(d, M0 + q))) * il;
 		}
+#pragma unroll
+		for (int d = 0; d < 3; d++) En += ((0.0 + lo(d, NRJ)) - (0.0 + hi(d, NRJ))) / len[d];
+#pragma unroll
+		for (int d = 0; d < 3; d++) {
+			const double il = 1 / len[d];
+#pragma unroll
+			for (int q = 0; q < 3; q++) vb[q] = vb[q] + ((0.0 + lo(d, B0 + q)) - (0.0 + hi(d, B0 + q))) * il;
+		}
+		const double vb_new[3] = {0.5 * (bx_new + bxp_new), 0.5 * (by_new + byp_new),
+			0.5 * (bz_new + bzp_new)};
+		for (int r = 0; r < refits; r++) {
+			refit_energy(rho, m, En, r == 0 ? vb : vb_new, vb_new, g.ph);
+		}
+		out[c] = rho;
+		out[n + c] = m[0];
+		out[2 * n + c] = m[1];
+		out[3 * n + c] = m[2];
+		out[4 * n + c] = En;
+
+		/* plane k + 1 becomes the current plane */
+		eyx_m = eyx_c; eyx_pm = eyx_pc; exy_m = exy_c; exy_pm = exy_pc;
+		eyx_c = eyx_n; eyx_pc = eyx_pn; exy_c = exy_n; exy_pc = exy_pn;
+		eyz_c = eyz_n; eyz_mc = eyz_mn; eyz_pc = eyz_pn;
+		exz_c = exz_n; exz_mc = exz_mn; exz_pc = exz_pn;
+		bz_new = bzp_new;
+		s7 = s7n;
+#pragma unroll
+		for (int q = 0; q < 8; q++) lo2[q] = hi2[q];
 	}
-	(void)ipp;
-	(void)jpp;
 }
 
 /*
 Per-cell update variant (one thread per cell, no marching): the same sums in
-the same order as update_zmarch_kernel, recomputing the positive-side faces'
-dB instead of exchanging them.  Kept as an autotuning candidate.
+the same order as update_column_kernel, recomputing the positive-side faces'
+dB instead of carrying them between planes; also the kernel of the z-slab
+pipeline.
 */
 template <class PT>
 __global__ __launch_bounds__(BX * BY) void update_cell_kernel(GridDev g,
@@ -977,7 +958,7 @@ void launch_update(pamhd_handle* h, hipStream_t st, int refits, int k0 = 0, int 
 	}
 	const unsigned gx = (h->g.nx + UBX - 1) / UBX, gy = (h->g.ny + UBY - 1) / UBY;
 	const unsigned gz = (h->g.nz + kz - 1) / kz;
-	update_zmarch_kernel<PT><<<dim3(gx * gy * gz), dim3(UBX, UBY, 1), 0, st>>>(
+	update_column_kernel<PT><<<dim3(gx * gy * gz), dim3(UBX, UBY, 1), 0, st>>>(
 		h->g, h->state[h->cur], h->state[1 - h->cur], P, refits, kz, gx, gy, h->tune.update_remap);
 }
 
@@ -1540,7 +1521,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	h->tune.flux_occ = best_occ;
 	flux();
 	report += "; update:";
-	const int kzs[] = {0, 4, 8, 16, 32};
+	const int kzs[] = {0, 8, 16, 32, 64};
 	float best_upd = 1e30f;
 	pamhd_handle::Tune best = h->tune;
 	for (int kz : kzs) {
@@ -1552,7 +1533,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 				if (h->precision == PAMHD_MIXED) launch_update<float>(h, st, 3);
 				else launch_update<double>(h, st, 3);
 			});
-			report += " " + std::string(kz ? "zmarch" + std::to_string(kz) : "cell") +
+			report += " " + std::string(kz ? "column" + std::to_string(kz) : "cell") +
 				"/remap" + std::to_string(remap) + "=" + std::to_string(ms) + "ms";
 			if (ms < best_upd) {
 				best_upd = ms;
@@ -1590,7 +1571,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	h->timing = timing;
 	report += "; chosen flux_remap=" + std::to_string(h->tune.flux_remap) +
 		" flux_occ4=" + std::to_string(h->tune.flux_occ) +
-		" update=" + (h->tune.update_kz ? "zmarch" + std::to_string(h->tune.update_kz) : std::string("cell")) +
+		" update=" + (h->tune.update_kz ? "column" + std::to_string(h->tune.update_kz) : std::string("cell")) +
 		" update_remap=" + std::to_string(h->tune.update_remap) +
 		" slabs=" + std::to_string(h->tune.slabs);
 	if (buf && buflen) snprintf(buf, buflen, "%s", report.c_str());
