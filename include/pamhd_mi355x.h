@@ -236,7 +236,10 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen,
 
 /* Sets the launch variants directly: flux_remap bit 0 = XCD-aware block
    order of the flux pass, bit 1 = planes fastest in the block order, bit 2 =
-   flux kernels built for 4 blocks per CU; update_remap bits 0 and 1 the same
+   flux kernels built for 4 blocks per CU, bit 3 = HLLD's per-cell part
+   computed once per cell and shared through LDS, bit 4 = that part carried
+   along marches of 16 cells for the y- and z-faces (bit 5: 4 cells) (bits
+   3-5 for HLLD only; bit 1 is then ignored); update_remap bits 0 and 1 the same
    orders for the update pass (bit 0 only for the z-marching update);
    update_kz = 0 for the per-cell update or the planes per block of the
    z-marching update.  All give identical results. */

@@ -266,6 +266,252 @@ __global__ __launch_bounds__(BX * BY, MINB) void flux_pass_kernel(GridDev g, Sta
 	reduce_speed(st, D, sp, zero);
 }
 
+/*
+HLLD flux pass with the per-cell part shared through LDS.  A block's
+BX x BY threads own a tile of cells: BX along x by BY along y in plane k
+for D = 0 and 1, BX along x by BY along z in row j for D = 2.  Each thread
+computes hlld_side of its cell in the D-frame once (thermal pressure,
+1 / rho, fast speed, check stage) and stages it in LDS; the cells just
+outside the tile's -D edge are done by the first wave in a second pass.
+The thread's face (between the cell at -e_D and its own) then takes the
+left side from LDS, reloads the left state (a neighbour loaded it, so an
+L1/L2 hit) and runs HLLD's per-face part.  Every value equals
+flux_pass_kernel's: the same functions of the same inputs, the same first
+error per face (first_side_error).  k_end bounds the planes of a z-slab.
+*/
+template <class T, class PT, int D, int MINB>
+__global__ __launch_bounds__(BX * BY, MINB) void flux_hlld_shared_kernel(GridDev g, StateArrays S,
+	PT* __restrict__ P, const double* __restrict__ dtp, StepStatus* st, unsigned gx, unsigned gy,
+	int remap, int k0, int k_end)
+{
+	constexpr int C = D == 0 ? BX + 1 : BX;       /* LDS row length */
+	constexpr int NS = D == 0 ? BY * (BX + 1) : (BY + 1) * BX;
+	__shared__ T lp[NS], lir[NS], lcf[NS];
+	__shared__ int lsc[NS];                        /* stage | code << 4 */
+	const unsigned lb = (remap & 1) ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+	const int tx = threadIdx.x, ty = threadIdx.y;
+	int x0, y0, z0;                                /* tile origin */
+	if (D < 2) {
+		unsigned bx, by, kk;
+		tile_of_block(lb, gx, gy, gridDim.x, remap & 1, k0, bx, by, kk);
+		x0 = bx * BX;
+		y0 = by * BY;
+		z0 = (int)kk;
+	} else {
+		const unsigned bx = lb % gx, r = lb / gx;
+		x0 = bx * BX;
+		y0 = (int)(r % (unsigned)g.ny);
+		z0 = k0 + (int)(r / (unsigned)g.ny) * BY;
+	}
+	const int i = x0 + tx;
+	const int j = D < 2 ? y0 + ty : y0;
+	const int k = D < 2 ? z0 : z0 + ty;
+	const bool active = i < g.nx && j < g.ny && k < k_end;
+	const int own = D == 0 ? ty * C + tx + 1 : (ty + 1) * C + tx;
+	const int left = ty * C + tx;
+
+	double U[8];
+	Side<T> sd;
+	T cf = 0;
+	int code = 0, stage = 0;
+	if (active) {
+		load_face_state_d<D>(S, g, i, j, k, U);
+		stage = hlld_side<T>(U, g.ph, sd, cf, code);
+		lp[own] = sd.p;
+		lir[own] = sd.ir;
+		lcf[own] = cf;
+		lsc[own] = stage | code << 4;
+	}
+	/* the cells on the -D side of the tile */
+	if (ty == 0) {
+		int hi, hj, hk, slot;
+		bool h_active;
+		if (D == 0) {
+			hi = wrap_dec(x0, g.nx);
+			hj = y0 + tx;
+			hk = z0;
+			slot = tx * C;
+			h_active = tx < BY && hj < g.ny;
+		} else if (D == 1) {
+			hi = x0 + tx;
+			hj = wrap_dec(y0, g.ny);
+			hk = z0;
+			slot = tx;
+			h_active = hi < g.nx;
+		} else {
+			hi = x0 + tx;
+			hj = y0;
+			hk = wrap_dec(z0, g.nz);
+			slot = tx;
+			h_active = hi < g.nx && z0 < k_end;
+		}
+		if (h_active) {
+			double H[8];
+			Side<T> hs;
+			T hcf = 0;
+			int hcode;
+			load_face_state_d<D>(S, g, hi, hj, hk, H);
+			const int hst = hlld_side<T>(H, g.ph, hs, hcf, hcode);
+			lp[slot] = hs.p;
+			lir[slot] = hs.ir;
+			lcf[slot] = hcf;
+			lsc[slot] = hst | hcode << 4;
+		}
+	}
+	__syncthreads();
+
+	double sp = 0;
+	unsigned zero = 0;
+	if (active) {
+		const int c = i + g.nx * (j + g.ny * k);
+		const int sc = lsc[left];
+		const int stage_l = sc & 15, code_l = sc >> 4;
+		T F[8], sT = 0;
+		int e;
+		if (stage_l | stage) {
+			e = first_side_error(stage_l, code_l, stage, code);
+		} else {
+			double Ld[8];
+			load_face_state_d<D>(S, g, D == 0 ? wrap_dec(i, g.nx) : i, D == 1 ? wrap_dec(j, g.ny) : j,
+				D == 2 ? wrap_dec(k, g.nz) : k, Ld);
+			Side<T> Ls;
+#pragma unroll
+			for (int q = 0; q < 8; q++) Ls.u[q] = (T)Ld[q];
+			Ls.p = lp[left];
+			Ls.ir = lir[left];
+			Ls.v[0] = Ls.u[M0] * Ls.ir;
+			Ls.v[1] = Ls.u[M1] * Ls.ir;
+			Ls.v[2] = Ls.u[M2] * Ls.ir;
+			Ls.p64 = 0;
+			Ls.rho_pos = true;
+			const T cfl = lcf[left];
+			if constexpr (std::is_same_v<T, float>) {
+				e = flux_hlld32_face(Ls, sd, cfl, cf, g.ph, F, sT);
+			} else {
+				e = flux_hlld_face<T>(Ls, sd, cfl, cf, Ld, U, g.ph, F, sT);
+			}
+		}
+		T Fg[8];
+		if (e) {
+			record_error(st, e, (long long)D * g.n + c);
+			for (int q = 0; q < 8; q++) Fg[q] = 0;
+			sT = 0;
+		} else {
+			const T dt = (T)*dtp;
+			Fg[RHO] = dt * F[RHO];
+			Fg[NRJ] = dt * F[NRJ];
+			Fg[M0 + D] = dt * F[M0];
+			Fg[M0 + (D + 1) % 3] = dt * F[M1];
+			Fg[M0 + (D + 2) % 3] = dt * F[M2];
+			Fg[B0 + D] = dt * F[B0];
+			Fg[B0 + (D + 1) % 3] = dt * F[B1];
+			Fg[B0 + (D + 2) % 3] = dt * F[B2];
+		}
+		PT* Pd = P + (size_t)D * 8 * g.n;
+#pragma unroll
+		for (int q = 0; q < 8; q++) (Pd + (size_t)q * g.n)[c] = (PT)Fg[q];
+		sp = (double)sT;
+		zero = sp <= 0 ? 1u : 0u;
+	}
+	reduce_speed(st, D, sp, zero);
+}
+
+/*
+HLLD flux pass for the y- (D = 1) and z-faces (D = 2) with the per-cell part
+carried along a march: thread (i, t) of a BX x BY block walks km cells of
+its line along D (t is k for D = 1, j for D = 2), computing hlld_side once
+per cell; the side of the cell just walked becomes the left side of the
+next face, so a line of km faces needs km + 1 per-cell parts instead of
+2 km.  No barrier: each thread's march is independent.  Every value equals
+flux_pass_kernel's (the same functions of the same inputs, the same first
+error per face); the block's max signal speed is reduced once at the end.
+Lines along D are cut into chunks of km cells; [k0, k_end) bounds the
+planes of a z-slab.
+*/
+template <class T, class PT, int D, int MINB>
+__global__ __launch_bounds__(BX * BY, MINB) void flux_hlld_march_kernel(GridDev g, StateArrays S,
+	PT* __restrict__ P, const double* __restrict__ dtp, StepStatus* st, unsigned gx, int km,
+	int remap, int k0, int k_end)
+{
+	static_assert(D == 1 || D == 2, "march along y or z");
+	const unsigned lb = (remap & 1) ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+	/* blocks: x tiles fastest, then BY-line groups, then chunks along D */
+	const int nt = D == 1 ? k_end - k0 : g.ny;           /* lines across */
+	const unsigned gt = (unsigned)((nt + BY - 1) / BY);
+	const unsigned bx = lb % gx, r = lb / gx;
+	const int t = (int)(r % gt) * BY + threadIdx.y;
+	const int chunk = (int)(r / gt);
+	const int i = bx * BX + threadIdx.x;
+	const int n_d = D == 1 ? g.ny : k_end;                /* march end */
+	const int a0 = (D == 1 ? 0 : k0) + chunk * km;        /* first face */
+	const int a1 = min(a0 + km, n_d);
+	double sp = 0;
+	unsigned zero = 0;
+	if (i < g.nx && t < nt && a0 < a1) {
+		const int jj = D == 1 ? 0 : t;                     /* fixed coordinates */
+		const int kk = D == 1 ? k0 + t : 0;
+		const int nd = D == 1 ? g.ny : g.nz;
+		const T dt = (T)*dtp;
+		/* left side of the first face */
+		Side<T> Ls;
+		T cfl = 0;
+		int code_l;
+		int stage_l;
+		{
+			double U[8];
+			const int am = wrap_dec(a0, nd);
+			load_face_state_d<D>(S, g, i, D == 1 ? am : jj, D == 1 ? kk : am, U);
+			stage_l = hlld_side<T>(U, g.ph, Ls, cfl, code_l);
+		}
+		for (int a = a0; a < a1; a++) {
+			const int j = D == 1 ? a : jj;
+			const int k = D == 1 ? kk : a;
+			double U[8];
+			load_face_state_d<D>(S, g, i, j, k, U);
+			Side<T> Rs;
+			T cfr = 0;
+			int code_r;
+			const int stage_r = hlld_side<T>(U, g.ph, Rs, cfr, code_r);
+			T F[8], sT = 0;
+			int e;
+			if (stage_l | stage_r) {
+				e = first_side_error(stage_l, code_l, stage_r, code_r);
+			} else if constexpr (std::is_same_v<T, float>) {
+				e = flux_hlld32_face(Ls, Rs, cfl, cfr, g.ph, F, sT);
+			} else {
+				e = flux_hlld_face<T>(Ls, Rs, cfl, cfr, Ls.u, Rs.u, g.ph, F, sT);
+			}
+			const int c = i + g.nx * (j + g.ny * k);
+			T Fg[8];
+			if (e) {
+				record_error(st, e, (long long)D * g.n + c);
+				for (int q = 0; q < 8; q++) Fg[q] = 0;
+				sT = 0;
+			} else {
+				Fg[RHO] = dt * F[RHO];
+				Fg[NRJ] = dt * F[NRJ];
+				Fg[M0 + D] = dt * F[M0];
+				Fg[M0 + (D + 1) % 3] = dt * F[M1];
+				Fg[M0 + (D + 2) % 3] = dt * F[M2];
+				Fg[B0 + D] = dt * F[B0];
+				Fg[B0 + (D + 1) % 3] = dt * F[B1];
+				Fg[B0 + (D + 2) % 3] = dt * F[B2];
+			}
+			PT* Pd = P + (size_t)D * 8 * g.n;
+#pragma unroll
+			for (int q = 0; q < 8; q++) (Pd + (size_t)q * g.n)[c] = (PT)Fg[q];
+			const double s_face = (double)sT;
+			sp = smax(sp, s_face);
+			zero += s_face <= 0 ? 1u : 0u;
+			Ls = Rs;
+			cfl = cfr;
+			stage_l = stage_r;
+			code_l = code_r;
+		}
+	}
+	reduce_speed(st, D, sp, zero);
+}
+
 /* constant-thermal-pressure energy refit of update_vol_B,
    source/mhd/solve.hpp:1646-1672 */
 __device__ __forceinline__ void refit_energy(double rho, const double m[3], double& E,
@@ -822,6 +1068,10 @@ struct pamhd_handle {
 		int update_remap = 0;
 		int slabs = 1;   /* > 1: flux pass and update pipelined over z-slabs */
 		int flux_occ = 0; /* 1: flux kernels compiled for 4 blocks per CU */
+		/* HLLD's per-cell part: 0 per face, 1 shared through LDS, 2 carried
+		   along marches of flux_km cells (y- and z-faces) */
+		int flux_shared = 0;
+		int flux_km = 16;
 	} tune;
 	/* second stream and events of the slab pipeline, created on first use */
 	hipStream_t aux = nullptr;
@@ -913,6 +1163,36 @@ void launch_flux_dirs(pamhd_handle* h, hipStream_t st, StepStatus* stat, int k0,
 	const unsigned gx = (h->g.nx + BX - 1) / BX, gy = (h->g.ny + BY - 1) / BY;
 	const dim3 grid(gx * gy * (unsigned)nk), block(BX, BY, 1);
 	const int r = h->tune.flux_remap;
+	if (SOLVER == HLLD_ATHENA && h->tune.flux_shared == 2) {
+		const int km = h->tune.flux_km;
+		const unsigned g1 = gx * (unsigned)((nk + BY - 1) / BY) * (unsigned)((h->g.ny + km - 1) / km);
+		const unsigned g2 = gx * (unsigned)((h->g.ny + BY - 1) / BY) * (unsigned)((nk + km - 1) / km);
+		const int ke = k0 + nk;
+		if (h->tune.flux_occ) {
+			flux_pass_kernel<SOLVER, T, PT, 0, 4><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
+			flux_hlld_march_kernel<T, PT, 1, 4><<<g1, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
+			flux_hlld_march_kernel<T, PT, 2, 4><<<g2, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
+		} else {
+			flux_pass_kernel<SOLVER, T, PT, 0, 1><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
+			flux_hlld_march_kernel<T, PT, 1, 1><<<g1, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
+			flux_hlld_march_kernel<T, PT, 2, 1><<<g2, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
+		}
+		return;
+	}
+	if (SOLVER == HLLD_ATHENA && h->tune.flux_shared == 1) {
+		const dim3 grid2(gx * (unsigned)h->g.ny * (unsigned)((nk + BY - 1) / BY));
+		const int ke = k0 + nk;
+		if (h->tune.flux_occ) {
+			flux_hlld_shared_kernel<T, PT, 0, 4><<<grid, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, gy, r, k0, ke);
+			flux_hlld_shared_kernel<T, PT, 1, 4><<<grid, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, gy, r, k0, ke);
+			flux_hlld_shared_kernel<T, PT, 2, 4><<<grid2, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, gy, r, k0, ke);
+		} else {
+			flux_hlld_shared_kernel<T, PT, 0, 1><<<grid, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, gy, r, k0, ke);
+			flux_hlld_shared_kernel<T, PT, 1, 1><<<grid, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, gy, r, k0, ke);
+			flux_hlld_shared_kernel<T, PT, 2, 1><<<grid2, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, gy, r, k0, ke);
+		}
+		return;
+	}
 	if (h->tune.flux_occ) {
 		flux_pass_kernel<SOLVER, T, PT, 0, 4><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
 		flux_pass_kernel<SOLVER, T, PT, 1, 4><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
@@ -1504,21 +1784,32 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	};
 	std::string report = "flux_pass:";
 	float best_flux = 1e30f;
-	int best_remap = 0, best_occ = 0;
+	int best_remap = 0, best_occ = 0, best_shared = 0, best_km = 16;
+	/* (shared mode, cells per march) */
+	const int modes[][2] = {{0, 16}, {1, 16}, {2, 8}, {2, 16}, {2, 32}};
+	for (const auto& md : modes)
 	for (int occ = 0; occ < 2; occ++)
-	for (int remap = 0; remap < 4; remap++) {
+	for (int remap = 0; remap < (md[0] ? 2 : 4); remap++) {
+		if (md[0] && solver != HLLD_ATHENA) continue;
 		h->tune.flux_remap = remap;
 		h->tune.flux_occ = occ;
+		h->tune.flux_shared = md[0];
+		h->tune.flux_km = md[1];
 		const float ms = time_it(flux);
-		report += " remap" + std::to_string(remap) + (occ ? "/occ4" : "") + "=" + std::to_string(ms) + "ms";
+		report += std::string(md[0] == 1 ? " shared/" : md[0] == 2 ? " march" + std::to_string(md[1]) + "/" : " ") +
+			"remap" + std::to_string(remap) + (occ ? "/occ4" : "") + "=" + std::to_string(ms) + "ms";
 		if (ms < best_flux) {
 			best_flux = ms;
 			best_remap = remap;
 			best_occ = occ;
+			best_shared = md[0];
+			best_km = md[1];
 		}
 	}
 	h->tune.flux_remap = best_remap;
 	h->tune.flux_occ = best_occ;
+	h->tune.flux_shared = best_shared;
+	h->tune.flux_km = best_km;
 	flux();
 	report += "; update:";
 	const int kzs[] = {0, 8, 16, 32, 64};
@@ -1571,6 +1862,8 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	h->timing = timing;
 	report += "; chosen flux_remap=" + std::to_string(h->tune.flux_remap) +
 		" flux_occ4=" + std::to_string(h->tune.flux_occ) +
+		" flux_shared=" + std::to_string(h->tune.flux_shared) +
+		(h->tune.flux_shared == 2 ? "/km" + std::to_string(h->tune.flux_km) : std::string()) +
 		" update=" + (h->tune.update_kz ? "column" + std::to_string(h->tune.update_kz) : std::string("cell")) +
 		" update_remap=" + std::to_string(h->tune.update_remap) +
 		" slabs=" + std::to_string(h->tune.slabs);
@@ -1584,6 +1877,8 @@ int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_
 	if (update_kz < 0) return fail(h, PAMHD_E_ARG, "update_kz must be >= 0");
 	h->tune.flux_remap = flux_remap & 3;
 	h->tune.flux_occ = (flux_remap >> 2) & 1;
+	h->tune.flux_shared = (flux_remap & 16) ? 2 : (flux_remap >> 3) & 1;
+	h->tune.flux_km = (flux_remap & 32) ? 4 : 16;
 	h->tune.update_kz = update_kz;
 	h->tune.update_remap = update_kz ? (update_remap & 1) : (update_remap & 3);
 	return PAMHD_OK;

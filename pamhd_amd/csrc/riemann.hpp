@@ -382,6 +382,48 @@ template <class T> __device__ __forceinline__ void hlld_star(const Side<T>& s, T
 }
 
 /*
+HLLD's per-cell part: the state checks, thermal pressure, 1 / rho and fast
+speed of one side in the face frame.  HLLD ignores the background field, so
+the part is the same for both faces of a cell along one direction and a
+kernel may compute it once per cell.  Returns the stage of the first failed
+check (0 none, 1 density, 2 pressure, 3 fast speed) with its error code.
+flux_hlld checks the density of both sides, then the pressures, then the
+fast speeds, so the error of a face is that of the side whose check fails
+first, the left side on a tie (first_side_error).
+*/
+template <class T> __device__ __forceinline__ int hlld_side(const double U[8], const Phys& ph,
+	Side<T>& s, T& cf, int& code)
+{
+	code = ERR_OK;
+	if (U[RHO] <= 0) {
+		code = ERR_DENSITY;
+		return 1;
+	}
+	make_side<T>(U, ph, s);
+	if (s.p64 <= 0) {
+		code = ERR_PRESSURE;
+		return 2;
+	}
+	const T zero[3] = {0, 0, 0};
+	int e;
+	if constexpr (std::is_same_v<T, float>) {
+		e = fast_speed32(s, U, zero, (float)ph.gamma, (float)ph.inv_mu0, cf);
+	} else {
+		e = fast_speed<T>(s, U, zero, (T)ph.gamma, (T)ph.mu0, cf);
+	}
+	if (e) {
+		code = e;
+		return 3;
+	}
+	return 0;
+}
+
+__device__ __forceinline__ int first_side_error(int stage_l, int code_l, int stage_r, int code_r)
+{
+	return (stage_l && (!stage_r || stage_l <= stage_r)) ? code_l : code_r;
+}
+
+/*
 HLLD of source/mhd/hlld_athena.hpp:54-559 for the mixed path: the same
 states, branches and degenerate-case tests in FP32, arranged so that each
 denominator is inverted once (1/rho, 1/(S_M* - S_M), 1/sqrt(rho*), the star
@@ -390,24 +432,11 @@ fast_speed32: 12 reciprocals and 6 square roots per face instead of the
 FP64 order's 24 divisions and 14 square roots.  Pressures and the
 positivity checks are FP64 (make_side).
 */
-__device__ __forceinline__ int flux_hlld32(const double Ld[8], const double Rd[8], const Phys& ph,
-	float F[8], float& s_out)
+__device__ __forceinline__ int flux_hlld32_face(const Side<float>& L, const Side<float>& R,
+	float cfL, float cfR, const Phys& ph, float F[8], float& s_out)
 {
 	using A = Arith<float>;
-	if (Ld[RHO] <= 0) return ERR_DENSITY;
-	if (Rd[RHO] <= 0) return ERR_DENSITY;
-	Side<float> L, R;
-	make_side<float>(Ld, ph, L);
-	make_side<float>(Rd, ph, R);
-	if (L.p64 <= 0) return ERR_PRESSURE;
-	if (R.p64 <= 0) return ERR_PRESSURE;
-	const float gamma = (float)ph.gamma, inv = (float)ph.inv_mu0, inv_sqrt = (float)ph.inv_sqrt_mu0;
-	const float zero[3] = {0, 0, 0};
-	float cfL, cfR;
-	int e = fast_speed32(L, Ld, zero, gamma, inv, cfL);
-	if (e) return e;
-	e = fast_speed32(R, Rd, zero, gamma, inv, cfR);
-	if (e) return e;
+	const float inv = (float)ph.inv_mu0, inv_sqrt = (float)ph.inv_sqrt_mu0;
 	const float ms = smax(cfL, cfR);
 	const float *vL = L.v, *vR = R.v;
 	const float
@@ -542,32 +571,31 @@ __device__ __forceinline__ int flux_hlld32(const double Ld[8], const double Rd[8
 	return ERR_OK;
 }
 
-/* source/mhd/hlld_athena.hpp:54-559 (ignores the background field, :77) */
-template <class T> __device__ __forceinline__ int flux_hlld(const double Ld[8],
-	const double Rd[8], const T* /*bg*/, const Phys& ph, T F[8], T& s_out)
+__device__ __forceinline__ int flux_hlld32(const double Ld[8], const double Rd[8], const Phys& ph,
+	float F[8], float& s_out)
+{
+	Side<float> L, R;
+	float cfL = 0, cfR = 0;
+	int cL, cR;
+	const int sL = hlld_side<float>(Ld, ph, L, cfL, cL);
+	const int sR = hlld_side<float>(Rd, ph, R, cfR, cR);
+	if (sL | sR) return first_side_error(sL, cL, sR, cR);
+	return flux_hlld32_face(L, R, cfL, cfR, ph, F, s_out);
+}
+
+/* HLLD's per-face part in FP64, after both sides passed hlld_side */
+template <class T> __device__ __forceinline__ int flux_hlld_face(const Side<T>& L, const Side<T>& R,
+	T cfL, T cfR, const double Ld[8], const double Rd[8], const Phys& ph, T F[8], T& s_out)
 {
 	using A = Arith<T>;
-	if constexpr (std::is_same_v<T, float>) return flux_hlld32(Ld, Rd, ph, F, s_out);
-	const T gamma = (T)ph.gamma, mu0 = (T)ph.mu0, inv_mu0 = (T)ph.inv_mu0;
+	const T mu0 = (T)ph.mu0, inv_mu0 = (T)ph.inv_mu0;
 	const T zero[3] = {0, 0, 0};
-	if (Ld[RHO] <= 0) return ERR_DENSITY;
-	if (Rd[RHO] <= 0) return ERR_DENSITY;
-	Side<T> L, R;
-	make_side<T>(Ld, ph, L);
-	make_side<T>(Rd, ph, R);
-	if (L.p64 <= 0) return ERR_PRESSURE;
-	if (R.p64 <= 0) return ERR_PRESSURE;
-
 	const T
 		inv = inv_mu0,
 		inv_sqrt = (T)ph.inv_sqrt_mu0,
 		Bx = (L.u[B0] + R.u[B0]) / T(2),
 		Bx2 = Bx * Bx;
-	T cfL, cfR;
-	int e = fast_speed<T>(L, Ld, zero, gamma, mu0, cfL);
-	if (e) return e;
-	e = fast_speed<T>(R, Rd, zero, gamma, mu0, cfR);
-	if (e) return e;
+	int e;
 	const T ms = smax(cfL, cfR);
 	const T* vL = L.v;
 	const T* vR = R.v;
@@ -675,6 +703,23 @@ template <class T> __device__ __forceinline__ int flux_hlld(const double Ld[8],
 	F[B0] = 0;
 	s_out = smax(fabs(msn), fabs(msp));
 	return ERR_OK;
+}
+
+/* source/mhd/hlld_athena.hpp:54-559 (ignores the background field, :77) */
+template <class T> __device__ __forceinline__ int flux_hlld(const double Ld[8],
+	const double Rd[8], const T* /*bg*/, const Phys& ph, T F[8], T& s_out)
+{
+	if constexpr (std::is_same_v<T, float>) {
+		return flux_hlld32(Ld, Rd, ph, F, s_out);
+	} else {
+		Side<T> L, R;
+		T cfL = 0, cfR = 0;
+		int cL, cR;
+		const int sL = hlld_side<T>(Ld, ph, L, cfL, cL);
+		const int sR = hlld_side<T>(Rd, ph, R, cfR, cR);
+		if (sL | sR) return first_side_error(sL, cL, sR, cR);
+		return flux_hlld_face<T>(L, R, cfL, cfR, Ld, Rd, ph, F, s_out);
+	}
 }
 
 /*

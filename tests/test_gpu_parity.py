@@ -134,6 +134,58 @@ def test_launch_variants_bit_exact(gpu, variant):
     np.testing.assert_array_equal(out, ref)
 
 
+@pytest.mark.parametrize("variant", [(8, 0, 0), (9, 0, 1), (12, 16, 1), (13, 8, 0),
+                                     (16, 0, 0), (49, 16, 1), (52, 0, 0), (21, 8, 0)])
+def test_hlld_shared_sides_bit_exact(gpu, variant):
+    """HLLD with the per-cell part (pressure, 1/rho, fast speed) computed once
+    per cell, shared through LDS or carried along marches of 16 or 4 cells
+    (chunk ends inside the 9- and 13-cell lines): the oracle's bits in FP64, the per-face
+    kernel's bits in mixed precision and through the z-slab pipeline; the
+    70x9x13 grid leaves partial tiles in x, y and z."""
+    from pamhd_amd import solver as S
+    cells = (70, 9, 13)
+    state, length, gamma, mu0 = _vortex(cells, seed=17)
+    og = O.make_grid(cells, length, gamma, mu0)
+    ref = state.copy()
+    st, steps_o, t_o = O.run_periodic(og, O.HLLD, 0.3, 0.0, 1e9, ref, max_steps=4)
+    assert st == 0
+    outs = {}
+    for prec in ("fp64", "mixed"):
+        for v, slabs in ((None, 1), (variant, 1), (variant, 3)):
+            with S.Grid(cells, length, gamma, mu0, precision=prec) as g:
+                g.upload(state)
+                if v is not None:
+                    g.set_launch(*v)
+                g.set_slabs(slabs)
+                g.run("hlld-athena", 0.3, 0.0, 1e9, max_steps=4)
+                outs[prec, v, slabs] = g.download()
+    for slabs in (1, 3):
+        np.testing.assert_array_equal(outs["fp64", variant, slabs], ref)
+        np.testing.assert_array_equal(outs["mixed", variant, slabs], outs["mixed", None, 1])
+
+
+@pytest.mark.parametrize("prec", ["fp64", "mixed"])
+def test_hlld_shared_sides_report_errors(gpu, prec):
+    """A cell with negative thermal pressure stops the HLLD step with the
+    pressure error whether the per-cell part is shared or not."""
+    from pamhd_amd import _lib
+    from pamhd_amd import solver as S
+    cells = (12, 10, 9)
+    state, length, gamma, mu0 = _vortex(cells)
+    c = 5 + 12 * (4 + 10 * 3)
+    state[4, c] = 0.01 * state[4, c]
+    codes = []
+    for flux_bits in (0, 8, 16):
+        with S.Grid(cells, length, gamma, mu0, precision=prec) as g:
+            g.upload(state)
+            g.set_launch(flux_bits, 0, 0)
+            with pytest.raises(_lib.PamhdError) as info:
+                g.timestep("hlld-athena", 0.0, 0.3)
+            codes.append(info.value.code)
+    print("codes", codes, flush=True)
+    assert codes[0] == codes[1] == codes[2] == 2
+
+
 @pytest.mark.parametrize("slabs", [2, 3, 4, 8])
 def test_slab_pipeline_bit_exact(gpu, slabs):
     """Flux pass and update pipelined over z-slabs on two streams give the
