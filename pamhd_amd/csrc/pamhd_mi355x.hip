@@ -27,6 +27,7 @@ conserved update, constant-pressure energy refits).  In mixed precision the flux
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "pamhd_mi355x.h"
@@ -64,7 +65,7 @@ struct StepStatus {
 };
 
 constexpr int BX = 64, BY = 4;      /* flux pass and cell kernels */
-constexpr int UBX = 64, UBY = 4;    /* z-marching update */
+/* column update tiles (TX x TY threads): 64 x 4, 128 x 2, 32 x 8 */
 constexpr int NXCD = 8;             /* MI355X: 8 XCDs, blocks dispatched round-robin */
 
 __device__ __forceinline__ int wrap_dec(int i, int n) { return i == 0 ? n - 1 : i - 1; }
@@ -617,7 +618,7 @@ template <class PT> __device__ __forceinline__ double dBz_at(const EdgeTerms<PT>
 
 /*
 Update pass, column-marching variant: update_mhd_state + update_vol_B of
-source/mhd/solve.hpp:1503-1683 with thread (i, j) of a UBX x UBY column
+source/mhd/solve.hpp:1503-1683 with thread (i, j) of a TX x TY column
 tile walking the planes [k0, k0 + kz) of its column.  Of the edge terms the
 six face changes around a cell use, those of plane k + 1 are loaded once,
 at plane k, and carried in registers through the next two planes; the
@@ -627,15 +628,15 @@ edges parallel to z are loaded fresh (their x and y neighbours are loaded
 by the adjacent threads at the same time).  Every sum has the order of
 update_cell_kernel, so both kernels give identical results.
 */
-template <class PT>
-__global__ __launch_bounds__(UBX * UBY) void update_column_kernel(GridDev g,
+template <class PT, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void update_column_kernel(GridDev g,
 	const double* __restrict__ s, double* __restrict__ out,
 	const PT* __restrict__ P, int refits, int kz, unsigned gx, unsigned gy, int remap)
 {
 	const unsigned lb = remap ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
 	const unsigned bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
-	const int i = bx * UBX + threadIdx.x;
-	const int j = by * UBY + threadIdx.y;
+	const int i = bx * TX + threadIdx.x;
+	const int j = by * TY + threadIdx.y;
 	if (i >= g.nx || j >= g.ny) return;
 	const long long n = g.n;
 	const int im = wrap_dec(i, g.nx), ip = wrap_inc(i, g.nx);
@@ -1072,6 +1073,7 @@ struct pamhd_handle {
 		   along marches of flux_km cells (y- and z-faces) */
 		int flux_shared = 0;
 		int flux_km = 16;
+		int update_tile = 0;   /* column update tile: 64x4, 128x2, 32x8 */
 	} tune;
 	/* second stream and events of the slab pipeline, created on first use */
 	hipStream_t aux = nullptr;
@@ -1236,10 +1238,18 @@ void launch_update(pamhd_handle* h, hipStream_t st, int refits, int k0 = 0, int 
 			h->g, h->state[h->cur], h->state[1 - h->cur], P, refits, gx, gy, h->tune.update_remap, k0);
 		return;
 	}
-	const unsigned gx = (h->g.nx + UBX - 1) / UBX, gy = (h->g.ny + UBY - 1) / UBY;
 	const unsigned gz = (h->g.nz + kz - 1) / kz;
-	update_column_kernel<PT><<<dim3(gx * gy * gz), dim3(UBX, UBY, 1), 0, st>>>(
-		h->g, h->state[h->cur], h->state[1 - h->cur], P, refits, kz, gx, gy, h->tune.update_remap);
+	auto column = [&](auto tx, auto ty) {
+		constexpr int TX = decltype(tx)::value, TY = decltype(ty)::value;
+		const unsigned gx = (h->g.nx + TX - 1) / TX, gy = (h->g.ny + TY - 1) / TY;
+		update_column_kernel<PT, TX, TY><<<dim3(gx * gy * gz), dim3(TX, TY, 1), 0, st>>>(
+			h->g, h->state[h->cur], h->state[1 - h->cur], P, refits, kz, gx, gy, h->tune.update_remap);
+	};
+	switch (h->tune.update_tile) {
+	case 1: column(std::integral_constant<int, 128>(), std::integral_constant<int, 2>()); break;
+	case 2: column(std::integral_constant<int, 32>(), std::integral_constant<int, 8>()); break;
+	default: column(std::integral_constant<int, 64>(), std::integral_constant<int, 4>()); break;
+	}
 }
 
 int report_status(pamhd_handle* h, const StepStatus& st)
@@ -1815,16 +1825,19 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	const int kzs[] = {0, 8, 16, 32, 64};
 	float best_upd = 1e30f;
 	pamhd_handle::Tune best = h->tune;
-	for (int kz : kzs) {
+	for (int kz : kzs)
+	for (int tile = 0; tile < (kz ? 3 : 1); tile++) {
 		if (kz > 0 && kz > h->g.nz) continue;
 		for (int remap = 0; remap < (kz ? 2 : 4); remap++) {
 			h->tune.update_kz = kz;
 			h->tune.update_remap = remap;
+			h->tune.update_tile = tile;
 			const float ms = time_it([&]() {
 				if (h->precision == PAMHD_MIXED) launch_update<float>(h, st, 3);
 				else launch_update<double>(h, st, 3);
 			});
-			report += " " + std::string(kz ? "column" + std::to_string(kz) : "cell") +
+			static const char* tiles[] = {"64x4", "128x2", "32x8"};
+			report += " " + std::string(kz ? "column" + std::to_string(kz) + "/" + tiles[tile] : "cell") +
 				"/remap" + std::to_string(remap) + "=" + std::to_string(ms) + "ms";
 			if (ms < best_upd) {
 				best_upd = ms;
@@ -1864,7 +1877,8 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 		" flux_occ4=" + std::to_string(h->tune.flux_occ) +
 		" flux_shared=" + std::to_string(h->tune.flux_shared) +
 		(h->tune.flux_shared == 2 ? "/km" + std::to_string(h->tune.flux_km) : std::string()) +
-		" update=" + (h->tune.update_kz ? "column" + std::to_string(h->tune.update_kz) : std::string("cell")) +
+		" update=" + (h->tune.update_kz ? "column" + std::to_string(h->tune.update_kz) + "/tile" +
+			std::to_string(h->tune.update_tile) : std::string("cell")) +
 		" update_remap=" + std::to_string(h->tune.update_remap) +
 		" slabs=" + std::to_string(h->tune.slabs);
 	if (buf && buflen) snprintf(buf, buflen, "%s", report.c_str());
@@ -1881,6 +1895,7 @@ int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_
 	h->tune.flux_km = (flux_remap & 32) ? 4 : 16;
 	h->tune.update_kz = update_kz;
 	h->tune.update_remap = update_kz ? (update_remap & 1) : (update_remap & 3);
+	h->tune.update_tile = update_kz ? (update_remap >> 2) % 3 : 0;
 	return PAMHD_OK;
 }
 
