@@ -288,6 +288,7 @@ __global__ __launch_bounds__(BX * BY, MINB) void flux_hlld_shared_kernel(GridDev
 	constexpr int C = D == 0 ? BX + 1 : BX;       /* LDS row length */
 	constexpr int NS = D == 0 ? BY * (BX + 1) : (BY + 1) * BX;
 	__shared__ T lp[NS], lir[NS], lcf[NS];
+	__shared__ double lpt[NS];                     /* FP64 total pressure (FP32 sides) */
 	__shared__ int lsc[NS];                        /* stage | code << 4 */
 	const unsigned lb = (remap & 1) ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
 	const int tx = threadIdx.x, ty = threadIdx.y;
@@ -321,6 +322,7 @@ __global__ __launch_bounds__(BX * BY, MINB) void flux_hlld_shared_kernel(GridDev
 		lp[own] = sd.p;
 		lir[own] = sd.ir;
 		lcf[own] = cf;
+		lpt[own] = sd.pt64;
 		lsc[own] = stage | code << 4;
 	}
 	/* the cells on the -D side of the tile */
@@ -356,6 +358,7 @@ __global__ __launch_bounds__(BX * BY, MINB) void flux_hlld_shared_kernel(GridDev
 			lp[slot] = hs.p;
 			lir[slot] = hs.ir;
 			lcf[slot] = hcf;
+			lpt[slot] = hs.pt64;
 			lsc[slot] = hst | hcode << 4;
 		}
 	}
@@ -384,6 +387,7 @@ __global__ __launch_bounds__(BX * BY, MINB) void flux_hlld_shared_kernel(GridDev
 			Ls.v[1] = Ls.u[M1] * Ls.ir;
 			Ls.v[2] = Ls.u[M2] * Ls.ir;
 			Ls.p64 = 0;
+			Ls.pt64 = lpt[left];
 			Ls.rho_pos = true;
 			const T cfl = lcf[left];
 			if constexpr (std::is_same_v<T, float>) {

@@ -131,6 +131,7 @@ template <class T> struct Side {
 	T v[3];
 	T ir;        /* 1 / rho (0 when rho <= 0) */
 	double p64;
+	double pt64; /* FP64 total pressure p + B^2 / 2 mu0 (FP32 sides only) */
 	bool rho_pos;
 };
 
@@ -141,6 +142,11 @@ template <class T> __device__ __forceinline__ void make_side(const double U[8], 
 	s.rho_pos = U[RHO] > 0;
 	s.p64 = s.rho_pos ? pressure64(U, ph.gm1, ph.half_inv_mu0) : 0.0;
 	s.p = (T)s.p64;
+	if constexpr (std::is_same_v<T, float>) {
+		s.pt64 = s.p64 + ph.half_inv_mu0 * (U[B0] * U[B0] + U[B1] * U[B1] + U[B2] * U[B2]);
+	} else {
+		s.pt64 = 0;
+	}
 	if (s.rho_pos) {
 		const T inv = Arith<T>::rcp(s.u[RHO]);
 		s.ir = inv;
@@ -476,7 +482,11 @@ __device__ __forceinline__ int flux_hlld32_face(const Side<float>& L, const Side
 	const float Bx = (L.u[B0] + R.u[B0]) * 0.5f, Bx2 = Bx * Bx, Bx2i = Bx2 * inv;
 	const float sfdn = msn - vL[0], sfdp = msp - vR[0];
 	const float rsn = L.u[RHO] * sfdn, rsp = R.u[RHO] * sfdp;
-	const float sm = (sfdp * R.u[M0] - sfdn * L.u[M0] - ptR + ptL) * A::rcp(rsp - rsn);
+	/* the total-pressure difference in FP64: where the normal momenta vanish
+	   it alone sets the sign of S_M, and HLLD's flux jumps across S_M = 0
+	   when the two sides' normal fields differ (cell-centred B) */
+	const float dpt = (float)(L.pt64 - R.pt64);
+	const float sm = (sfdp * R.u[M0] - sfdn * L.u[M0] + dpt) * A::rcp(rsp - rsn);
 	const float smdn = msn - sm, smdp = msp - sm;
 	const float ismdn = A::rcp(smdn), ismdp = A::rcp(smdp);
 	const float dsn = rsn * ismdn, dsp = rsp * ismdp;
