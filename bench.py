@@ -364,7 +364,13 @@ def precision_study(size=512, steps=160, clean_every=10, report_every=20, solver
             "mixed_rel_l2_vs_fp64_by_step": errors}
 
 
-def roofline(res, n, steps):
+# HBM traffic of one step (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 summed over
+# the step's launches) measured with rocprofv3 --pmc for the launch variants
+# the autotuner picked: profiles/step_kernels_256cubed_hlld_counters.md
+STEP_TRAFFIC_BYTES = {("fp64", 256, "hlld-athena"): 13.84e9, ("mixed", 256, "hlld-athena"): 10.31e9}
+
+
+def roofline(res, n, steps, key=None):
     """Step roofline: algorithmic 128 B per cell-update over the device time
     of one step (HIP events around the timed steps on the stream the kernels
     run on; with z-slab pipelining the flux pass and update overlap).  The
@@ -376,7 +382,8 @@ def roofline(res, n, steps):
     achieved = BYTES_PER_CELL_UPDATE * n / (step_ms / 1e3) / 1e9
     return {"bound": "hbm", "kernel": "step (flux_pass + update)",
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": STEP_TRAFFIC_BYTES.get(key),
+            "traffic_unit": "bytes per step (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
             "step_ms": round(step_ms, 4),
             "tuning": res["tuning"].split("; chosen ")[-1],
             "kernel_ms_serialised": {"flux_pass": round(flux_ms / flux_n, 4),
@@ -489,13 +496,13 @@ def main():
                                % (args.size, args.solver, args.cfl),
                    "cells": list(cells), "solver": args.solver, "cfl": args.cfl,
                    "gamma": gamma, "precision": "fp64"},
-        "roofline": roofline(f64, n, args.steps),
+        "roofline": roofline(f64, n, args.steps, ("fp64", args.size, args.solver)),
         "cpu_baseline": base,
         "mixed": {"value": mix["value"], "ms_per_step": mix["ms"] / args.steps,
                   "dtype": "f32 flux/wave speeds, f64 state",
                   "speedup_vs_fp64": mix["value"] / f64["value"],
                   "rel_l2_vs_fp64": err,
-                  "roofline": roofline(mix, n, args.steps)},
+                  "roofline": roofline(mix, n, args.steps, ("mixed", args.size, args.solver))},
         "divergence_cleaning": clean,
         "ensemble": ens,
         "adjoint": adj,

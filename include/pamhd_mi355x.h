@@ -205,8 +205,10 @@ int pamhd_clean_divergence(pamhd_handle* h, double tol, int max_cycles,
 	double* div_before, double* rel_residual, int* cycles, void* stream);
 
 /* Smoother of the Poisson solve: 1 = LDS-tiled fused red-black sweeps where
-   every dimension of a level is even and >= 4 (default), 0 = one launch per
-   colour.  Both give identical results.  (No counterpart in the reference.) */
+   every dimension of a level is even and >= 4 (half the bytes of the two
+   colour passes, but latency bound: slower on MI355X at 256^3), 0 = one
+   launch per colour (default).  Both give identical results.  (No
+   counterpart in the reference.) */
 int pamhd_set_poisson_smoother(pamhd_handle* h, int fused);
 
 /* Largest face signal speed per dimension of the last flux pass (synchronous). */

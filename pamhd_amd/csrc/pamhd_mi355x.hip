@@ -1091,7 +1091,7 @@ struct pamhd_handle {
 	BoxSet* box = nullptr;    /* solar-wind box (non-periodic) handles only */
 	double* adj = nullptr;    /* adjoint scratch, allocated on first use */
 	PoissonMG* mg = nullptr;  /* multigrid hierarchy, allocated on first cleaning */
-	int mg_fused = 1;
+	int mg_fused = 0;   /* per-colour passes (faster; see pamhd_set_poisson_smoother) */
 	int cur = 0;    /* state buffer holding the current state */
 	int last = 0;   /* status entry of the last flux pass */
 	std::string err;

@@ -217,7 +217,12 @@ constexpr int TX = 32, TY = 8;
 constexpr int WX = TX + 4, WY = TY + 4;     /* window with a 2-cell ring */
 constexpr int NO = 4, NM = 3;               /* O planes k-1..k+2, M planes k-1..k+1 */
 
-__device__ __forceinline__ int wrapm(int v, int n) { return (v + n) % n; }
+/* v >= -n wrapped into [0, n); the modulo only for windows wider than the
+   level (levels narrower than a tile) */
+__device__ __forceinline__ int wrapm(int v, int n)
+{
+	return v < 0 ? v + n : v < n ? v : v < 2 * n ? v - n : v % n;
+}
 
 __device__ __forceinline__ double rv_window(const LevelArgs& L, const double (*w)[WY][WX],
 	int p, int pm, int pp, int x, int y, double fc)
@@ -234,33 +239,71 @@ __device__ __forceinline__ int pslot(int q, int n) { return ((q % n) + n) % n; }
 __global__ __launch_bounds__(TX * TY) void rb_fused_kernel(LevelArgs L, double* __restrict__ phi,
 	const double* __restrict__ f, const double* __restrict__ phi_in, int kchunk)
 {
+	constexpr int NT = TX * TY;
+	constexpr int FX = TX + 2, FY = TY + 2;           /* right-hand side over the red window */
+	constexpr int NWO = (WX * WY + NT - 1) / NT;       /* window values per thread */
+	constexpr int NWF = (FX * FY + NT - 1) / NT;
 	__shared__ double O[NO][WY][WX];
 	__shared__ double M[NM][WY][WX];
+	__shared__ double Fw[2][FY][FX];                   /* f of planes q, slot q & 1 */
 	const int i0 = blockIdx.x * TX, j0 = blockIdx.y * TY;
 	const int k0 = blockIdx.z * kchunk;
 	const int k1 = k0 + kchunk < L.n2 ? k0 + kchunk : L.n2;
 	const int tid = threadIdx.x + TX * threadIdx.y;
 	const int nz = L.n2;
+	double po[NWO], pf[NWF];
 
-	auto load_old = [&](int q) {
-		const int kk = wrapm(q, nz), s = pslot(q, NO);
-		for (int e = tid; e < WX * WY; e += TX * TY) {
-			const int x = e % WX, y = e / WX;
-			const int gi = wrapm(i0 + x - 2, L.n0), gj = wrapm(j0 + y - 2, L.n1);
-			O[s][y][x] = phi_in[gi + L.n0 * (gj + L.n1 * kk)];
+	/* global loads of a plane's window into registers, issued a plane ahead
+	   of their use so that their latency overlaps the sweeps of the plane
+	   before, and their stores into LDS after the next barrier */
+	auto fetch_old = [&](int q) {
+		const int kk = wrapm(q, nz);
+#pragma unroll
+		for (int r = 0; r < NWO; r++) {
+			const int e = tid + r * NT;
+			if (e < WX * WY) {
+				const int x = e % WX, y = e / WX;
+				const int gi = wrapm(i0 + x - 2, L.n0), gj = wrapm(j0 + y - 2, L.n1);
+				po[r] = phi_in[gi + L.n0 * (gj + L.n1 * kk)];
+			}
+		}
+	};
+	auto store_old = [&](int q) {
+		const int s = pslot(q, NO);
+#pragma unroll
+		for (int r = 0; r < NWO; r++) {
+			const int e = tid + r * NT;
+			if (e < WX * WY) O[s][e / WX][e % WX] = po[r];
+		}
+	};
+	auto fetch_f = [&](int q) {
+		const int kk = wrapm(q, nz);
+#pragma unroll
+		for (int r = 0; r < NWF; r++) {
+			const int e = tid + r * NT;
+			if (e < FX * FY) {
+				const int x = e % FX + 1, y = e / FX + 1;
+				const int gi = wrapm(i0 + x - 2, L.n0), gj = wrapm(j0 + y - 2, L.n1);
+				pf[r] = f[gi + L.n0 * (gj + L.n1 * kk)];
+			}
+		}
+	};
+	auto store_f = [&](int q) {
+#pragma unroll
+		for (int r = 0; r < NWF; r++) {
+			const int e = tid + r * NT;
+			if (e < FX * FY) Fw[q & 1][e / FX][e % FX] = pf[r];
 		}
 	};
 	auto red = [&](int q) {
 		const int kk = wrapm(q, nz);
 		const int p = pslot(q, NO), pm = pslot(q - 1, NO), pp = pslot(q + 1, NO);
 		const int s = pslot(q, NM);
-		for (int e = tid; e < (TX + 2) * (TY + 2); e += TX * TY) {
-			const int x = e % (TX + 2) + 1, y = e / (TX + 2) + 1;
+		for (int e = tid; e < FX * FY; e += NT) {
+			const int x = e % FX + 1, y = e / FX + 1;
 			const int gi = wrapm(i0 + x - 2, L.n0), gj = wrapm(j0 + y - 2, L.n1);
 			double v = O[p][y][x];
-			if (((gi + gj + kk) & 1) == 0) {
-				v = rv_window(L, O, p, pm, pp, x, y, f[gi + L.n0 * (gj + L.n1 * kk)]);
-			}
+			if (((gi + gj + kk) & 1) == 0) v = rv_window(L, O, p, pm, pp, x, y, Fw[q & 1][y - 1][x - 1]);
 			M[s][y][x] = v;
 		}
 	};
@@ -271,21 +314,33 @@ __global__ __launch_bounds__(TX * TY) void rb_fused_kernel(LevelArgs L, double* 
 		const int p = pslot(q, NM), pm = pslot(q - 1, NM), pp = pslot(q + 1, NM);
 		const int c = gi + L.n0 * (gj + L.n1 * q);
 		double v = M[p][y][x];
-		if (((gi + gj + q) & 1) == 1) v = rv_window(L, M, p, pm, pp, x, y, f[c]);
+		if (((gi + gj + q) & 1) == 1) v = rv_window(L, M, p, pm, pp, x, y, Fw[q & 1][y - 1][x - 1]);
 		phi[c] = v;
 	};
 
-	load_old(k0 - 2);
-	load_old(k0 - 1);
-	load_old(k0);
-	load_old(k0 + 1);
+	for (int q = k0 - 2; q <= k0 + 1; q++) {
+		fetch_old(q);
+		store_old(q);
+	}
+	fetch_f(k0 - 1);
+	store_f(k0 - 1);
+	fetch_f(k0);
+	store_f(k0);
 	__syncthreads();
 	red(k0 - 1);
 	red(k0);
+	fetch_old(k0 + 2);
+	fetch_f(k0 + 1);
 	for (int k = k0; k < k1; k++) {
 		__syncthreads();
-		/* O[k-2] is no longer needed: M[k-1] and M[k] exist */
-		load_old(k + 2);
+		/* O[k-2] is no longer needed: M[k-1] and M[k] exist; f[k+1] goes
+		   into the slot of f[k-1] */
+		store_old(k + 2);
+		store_f(k + 1);
+		if (k + 1 < k1) {
+			fetch_old(k + 3);
+			fetch_f(k + 2);
+		}
 		__syncthreads();
 		/* into the slot of M[k-2], no longer needed after black(k-1) */
 		red(k + 1);
@@ -431,9 +486,10 @@ void mg_smooth(PoissonMG& mg, hipStream_t st, int level, int sweeps)
 		return;
 	}
 	if (rb && mg.fused_smoother && fused_ok(L)) {
-		/* enough blocks for 256 CUs: split the march into z-chunks */
+		/* about 1024 blocks (4 per CU, all resident at once): split the
+		   march into z-chunks */
 		const int tiles = ((L.n[0] + TX - 1) / TX) * ((L.n[1] + TY - 1) / TY);
-		int chunks = (2048 + tiles - 1) / tiles;
+		int chunks = (1024 + tiles - 1) / tiles;
 		if (chunks > L.n[2] / 4) chunks = L.n[2] / 4;
 		if (chunks < 1) chunks = 1;
 		const int kc = (L.n[2] + chunks - 1) / chunks;

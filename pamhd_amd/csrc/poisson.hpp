@@ -40,7 +40,7 @@ struct PoissonMG {
 	size_t partial_len = 0;
 	double* result = nullptr;       /* device: 1 double */
 	double* host_result = nullptr;  /* pinned */
-	int fused_smoother = 1;         /* LDS-tiled red-black sweeps where possible */
+	int fused_smoother = 0;         /* 1: LDS-tiled fused red-black sweeps where possible */
 };
 
 /* allocates the level hierarchy for a dims[0] x dims[1] x dims[2] grid */
