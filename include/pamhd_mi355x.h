@@ -211,6 +211,22 @@ int pamhd_clean_divergence(pamhd_handle* h, double tol, int max_cycles,
    counterpart in the reference.) */
 int pamhd_set_poisson_smoother(pamhd_handle* h, int fused);
 
+/* Per-cell 2^N substepping of timestep(), source/mhd/solve.hpp:1837-1924
+   with source/substepping.hpp:54-399 (options "substep-min" and
+   "substep-max" of source/simulation_options.hpp, integer form).  With
+   substep_max > 0 pamhd_timestep / pamhd_step give every cell a period of
+   2^N substeps from its own CFL limit (at most one level above any face or
+   edge neighbour), solve each face at the pace of the smallest period
+   around its owner and return the step length (largest period x substep
+   length); substep_max = 0 is the global time step.  FP64 periodic grids;
+   set it before the zero-dt final init (the per-face speeds it needs are
+   measured from then on). */
+int pamhd_set_substepping(pamhd_handle* h, int substep_min, int substep_max);
+
+/* Largest period (substeps) of the last step: timestep()'s second return
+   value (1 without substepping). */
+int pamhd_last_max_substep(pamhd_handle* h, int* max_substep);
+
 /* Largest face signal speed per dimension of the last flux pass (synchronous). */
 int pamhd_max_signal_speeds(pamhd_handle* h, double smax[3]);
 

@@ -1232,6 +1232,420 @@ int or_run_periodic(const or_grid* g, int solver, double cfl, double t_start,
 }
 
 /* ---------------------------------------------------------------------- */
+/* per-cell 2^N substepping on a uniform periodic grid                    */
+/* ---------------------------------------------------------------------- */
+
+/*
+The periodic step with the reference's per-cell substepping,
+timestep(), source/mhd/solve.hpp:1837-1924, on a grid of normal cells:
+
+  set_minmax_substepping_period (options), substepping.hpp:194-261
+  minimize_timestep, solve.hpp:1927-2014
+  set_minmax_substepping_period (dt), substepping.hpp:264-399
+  restrict_substepping_period, substepping.hpp:99-191
+  update_substeps (2^N periods), substepping.hpp:54-96
+  for substep s = 1 .. max period:
+      get_fluxes, solve.hpp:240-569: the +d faces of cell c are solved when
+          s is a multiple of the smallest period among c and its face and
+          edge neighbours (m), with weight dt_sub * m into the flux slots of
+          both cells and into the edge terms of the face dB
+      update_mhd_state, solve.hpp:1503-1603: cells whose period divides s
+          apply and clear their slots and face dB, the others keep them
+      update_vol_B(s), solve.hpp:1618-1683: constant-pressure refit of the
+          updated cells from their own face copies
+  sync_magnetic_field, common_functions.hpp:184-269 (two visits per face,
+      the cell with the smaller index first)
+  update_vol_B(0) of all cells.
+
+Working layout: S 14 x n (rho, m, E, vol B, the cell's copies of its -x, +x,
+-y, +y, -z, +z faces), slots 48 x n ((2d + side) * 8 + q), face dB 6 x n.
+The state handed in and out is the face-once layout of or_step_periodic
+(copies are equal at step boundaries).  maxv (3 x n) holds the signal speed
+of each face (Max_v of both cells, which the reference sets together),
+negative before the first solution.
+
+Deviation: floor(log2(x)) of substepping.hpp:385 is computed exactly
+(ilogb), and a non-finite ratio (zero smallest dt) gives 0, the value the
+reference's int conversion produces on x86.
+*/
+
+#define SC(q, c) S[(long)(q) * n + (c)]
+#define SLOT(dir, q, c) slot[((long)(dir) * 8 + (q)) * n + (c)]
+
+static void sub_face_state(const double* S, long n, int d, long c, double U[8])
+{
+	const double m[3] = {SC(1, c), SC(2, c), SC(3, c)};
+	const double B[3] = {SC(5, c), SC(6, c), SC(7, c)};
+	U[RHO] = SC(0, c);
+	U[NRJ] = SC(4, c);
+	rot_to_face(d, m, U + M0);
+	rot_to_face(d, B, U + B0);
+}
+
+int or_substep_periods(const or_grid* g, double time_step_factor, double max_dt,
+	int substep_min, int substep_max, const double* maxv, int* period, int* minsub,
+	double* sub_dt_out, int* max_period_out)
+{
+	const dims_t D = {g->nx, g->ny, g->nz, (long)g->nx * g->ny * g->nz};
+	const long n = D.n;
+	const double len[3] = {g->dx, g->dy, g->dz};
+	double* ts = (double*)malloc(sizeof(double) * n);
+	int* lomin = (int*)malloc(sizeof(int) * n);
+	int* smaxv = (int*)malloc(sizeof(int) * n);
+	if (!ts || !lomin || !smaxv) { free(ts); free(lomin); free(smaxv); return OR_ERR_SOLVER; }
+
+	/* options: the smallest minimum substep is shifted to 0 */
+	int lo = substep_min;
+	for (long c = 0; c < n; c++) { lomin[c] = substep_min; smaxv[c] = substep_max; }
+	if (lo > 0) for (long c = 0; c < n; c++) lomin[c] -= lo;
+
+	/* minimize_timestep: each cell's limit over its six faces */
+	for (long k = 0; k < D.nz; k++)
+	for (long j = 0; j < D.ny; j++)
+	for (long i = 0; i < D.nx; i++) {
+		const long c = cid(&D, i, j, k);
+		double t = DBL_MAX;
+		for (int d = 0; d < 3; d++) {
+			const long fm = c, fp = cid(&D, i + (d == 0), j + (d == 1), k + (d == 2));
+			const double v[2] = {maxv[(long)d * n + fm], maxv[(long)d * n + fp]};
+			for (int sd = 0; sd < 2; sd++) {
+				if (v[sd] <= 0) t = 0;
+				else t = smin(time_step_factor * len[d] / v[sd], t);
+			}
+		}
+		ts[c] = t;
+	}
+
+	double smallest = DBL_MAX;
+	for (long c = 0; c < n; c++) smallest = smin(ts[c], smallest);
+	double sub_dt;
+	if (max_dt <= smallest) {
+		for (long c = 0; c < n; c++) lomin[c] = smaxv[c] = 0;
+		sub_dt = max_dt;
+	} else {
+		double ret = DBL_MAX;
+		for (long c = 0; c < n; c++) ret = smin(ret, ts[c] / (double)(1 << lomin[c]));
+		for (long c = 0; c < n; c++) {
+			const double dt = ts[c];
+			if (dt > 0) {
+				const double x = smin(dt, max_dt) / ret;
+				int step = (isfinite(x) && x > 0) ? ilogb(x) : 0;
+				if (step < 0) step = 0;
+				if (smaxv[c] > step) smaxv[c] = step;
+				if (lomin[c] > smaxv[c]) {
+					free(ts); free(lomin); free(smaxv);
+					return OR_ERR_SOLVER;
+				}
+			} else {
+				lomin[c] = smaxv[c] = 0;
+			}
+		}
+		sub_dt = ret;
+	}
+
+	/* restrict_substepping_period: at most one level above any face or
+	   edge neighbour (the fixed point is independent of the sweep order) */
+	int* e = period;
+	for (long c = 0; c < n; c++) e[c] = smaxv[c];
+	for (;;) {
+		long modified = 0;
+		for (long k = 0; k < D.nz; k++)
+		for (long j = 0; j < D.ny; j++)
+		for (long i = 0; i < D.nx; i++) {
+			const long c = cid(&D, i, j, k);
+			for (int dk = -1; dk <= 1; dk++)
+			for (int dj = -1; dj <= 1; dj++)
+			for (int di = -1; di <= 1; di++) {
+				const int nz_ = (di != 0) + (dj != 0) + (dk != 0);
+				if (nz_ == 0 || nz_ == 3) continue;   /* face and edge neighbours */
+				const long nb = cid(&D, i + di, j + dj, k + dk);
+				if (e[c] > e[nb] + 1) { e[c] = e[nb] + 1; modified++; }
+			}
+		}
+		if (!modified) break;
+	}
+	/* update_substeps: 2^N periods */
+	int maxp = -1;
+	for (long c = 0; c < n; c++) { e[c] = 1 << e[c]; if (e[c] > maxp) maxp = e[c]; }
+	/* smallest period among each cell and its face and edge neighbours */
+	for (long k = 0; k < D.nz; k++)
+	for (long j = 0; j < D.ny; j++)
+	for (long i = 0; i < D.nx; i++) {
+		const long c = cid(&D, i, j, k);
+		int m = e[c];
+		for (int dk = -1; dk <= 1; dk++)
+		for (int dj = -1; dj <= 1; dj++)
+		for (int di = -1; di <= 1; di++) {
+			const int nz_ = (di != 0) + (dj != 0) + (dk != 0);
+			if (nz_ == 0 || nz_ == 3) continue;
+			const int pn = e[cid(&D, i + di, j + dj, k + dk)];
+			if (pn < m) m = pn;
+		}
+		minsub[c] = m;
+	}
+	free(ts); free(lomin); free(smaxv);
+	*sub_dt_out = sub_dt;
+	*max_period_out = maxp;
+	return OR_OK;
+}
+
+int or_step_substep(const or_grid* g, int solver, double time_step_factor, double max_dt,
+	int substep_min, int substep_max, double* state, double* maxv, double* dt_out,
+	int* max_period_out, long* bad_face)
+{
+	const dims_t D = {g->nx, g->ny, g->nz, (long)g->nx * g->ny * g->nz};
+	const long n = D.n;
+	const double len[3] = {g->dx, g->dy, g->dz};
+	const double area[3] = {g->dy * g->dz, g->dx * g->dz, g->dx * g->dy};
+	int* period = (int*)malloc(sizeof(int) * n);
+	int* minsub = (int*)malloc(sizeof(int) * n);
+	double* S = (double*)malloc(sizeof(double) * 14 * n);
+	double* slot = (double*)calloc((size_t)48 * n, sizeof(double));
+	double* fdb = (double*)calloc((size_t)6 * n, sizeof(double));
+	double* P = (double*)malloc(sizeof(double) * 24 * n);
+	int err = OR_OK;
+	if (!period || !minsub || !S || !slot || !fdb || !P) { err = OR_ERR_SOLVER; goto done; }
+
+	double sub_dt;
+	int maxp;
+	err = or_substep_periods(g, time_step_factor, max_dt, substep_min, substep_max, maxv,
+		period, minsub, &sub_dt, &maxp);
+	if (err) goto done;
+
+	/* expand: vol B = face average, both copies of each face equal */
+	for (long k = 0; k < D.nz; k++)
+	for (long j = 0; j < D.ny; j++)
+	for (long i = 0; i < D.nx; i++) {
+		const long c = cid(&D, i, j, k);
+		for (int q = 0; q < 5; q++) SC(q, c) = state[q * n + c];
+		double B[3];
+		vol_b(&D, state, i, j, k, B);
+		for (int d = 0; d < 3; d++) {
+			SC(5 + d, c) = B[d];
+			SC(8 + 2 * d, c) = state[(5 + d) * n + c];
+			SC(9 + 2 * d, c) = state[(5 + d) * n + cid(&D, i + (d == 0), j + (d == 1), k + (d == 2))];
+		}
+	}
+
+	double total_dt = 0;
+	const double dx = g->dx, dy = g->dy, dz = g->dz;
+	for (int sub = 1; sub <= maxp; sub++) {
+		total_dt += sub_dt;
+		/* fluxes of the faces due at this substep; P = 0 for the others */
+		long bad = -1;
+		for (int d = 0; d < 3 && !err; d++)
+		for (long k = 0; k < D.nz; k++)
+		for (long j = 0; j < D.ny; j++)
+		for (long i = 0; i < D.nx; i++) {
+			const long c = cid(&D, i, j, k);
+			const long o = cid(&D, i - (d == 0), j - (d == 1), k - (d == 2));  /* owner: -side cell */
+			const int m = minsub[o];
+			if (sub % m != 0) {
+				for (int q = 0; q < 8; q++) P[(d * 8 + q) * n + c] = 0;
+				continue;
+			}
+			double L[8], R[8], bgr[3] = {0, 0, 0}, F[8], sp = 0;
+			sub_face_state(S, n, d, o, L);
+			sub_face_state(S, n, d, c, R);
+			if (g->bg) {
+				const double b[3] = {g->bg[(3 * d + 0) * n + o], g->bg[(3 * d + 1) * n + o],
+					g->bg[(3 * d + 2) * n + o]};
+				rot_to_face(d, b, bgr);
+			}
+			const int e = or_flux(solver, L, R, bgr, g->gamma, g->mu0, F, &sp);
+			if (e) {
+				if (!err || (long)d * n + c < bad) { err = e; bad = (long)d * n + c; }
+				continue;
+			}
+			maxv[(long)d * n + c] = sp;
+			double Fg[8];
+			Fg[RHO] = F[RHO];
+			Fg[NRJ] = F[NRJ];
+			rot_to_global(d, F + M0, Fg + M0);
+			rot_to_global(d, F + B0, Fg + B0);
+			const double min_dt = sub_dt * m;
+			for (int q = 0; q < 8; q++) P[(d * 8 + q) * n + c] = min_dt * Fg[q];
+		}
+		if (err) { if (bad_face) *bad_face = bad; goto done; }
+
+#define PF(d, q, i, j, k) P[((d) * 8 + (q)) * n + cid(&D, (i), (j), (k))]
+#define EZX(i, j, k) (+(PF(0, 6, i, j, k) * dz / 4))
+#define EYX(i, j, k) (-(PF(0, 7, i, j, k) * dy / 4))
+#define EXY(i, j, k) (+(PF(1, 7, i, j, k) * dx / 4))
+#define EZY(i, j, k) (-(PF(1, 5, i, j, k) * dz / 4))
+#define EXZ(i, j, k) (-(PF(2, 6, i, j, k) * dx / 4))
+#define EYZ(i, j, k) (+(PF(2, 5, i, j, k) * dy / 4))
+		for (long k = 0; k < D.nz; k++)
+		for (long j = 0; j < D.ny; j++)
+		for (long i = 0; i < D.nx; i++) {
+			const long c = cid(&D, i, j, k);
+			/* face dB of the cell's six copies: the 12 edge terms of each
+			   face in the order of or_step_periodic */
+			double db[6];
+			for (int side = 0; side < 2; side++) {
+				const long a = i + side;    /* x-face at a, y-face at j + side, z-face at k + side */
+				double r = 0;
+				r += EZX(a, j + 1, k); r -= EZX(a, j - 1, k); r -= EYX(a, j, k + 1); r += EYX(a, j, k - 1);
+				r += EZY(a, j + 1, k); r += EZY(a - 1, j + 1, k); r -= EZY(a, j, k); r -= EZY(a - 1, j, k);
+				r -= EYZ(a, j, k + 1); r -= EYZ(a - 1, j, k + 1); r += EYZ(a, j, k); r += EYZ(a - 1, j, k);
+				db[0 + side] = r;
+				const long b = j + side;
+				r = 0;
+				r += EXY(i, b, k + 1); r -= EXY(i, b, k - 1); r -= EZY(i + 1, b, k); r += EZY(i - 1, b, k);
+				r += EXZ(i, b, k + 1); r += EXZ(i, b - 1, k + 1); r -= EXZ(i, b, k); r -= EXZ(i, b - 1, k);
+				r -= EZX(i + 1, b, k); r += EZX(i, b, k); r -= EZX(i + 1, b - 1, k); r += EZX(i, b - 1, k);
+				db[2 + side] = r;
+				const long z = k + side;
+				r = 0;
+				r += EYZ(i + 1, j, z); r -= EYZ(i - 1, j, z); r -= EXZ(i, j + 1, z); r += EXZ(i, j - 1, z);
+				r += EYX(i + 1, j, z); r += EYX(i + 1, j, z - 1); r -= EYX(i, j, z); r -= EYX(i, j, z - 1);
+				r -= EXY(i, j + 1, z); r -= EXY(i, j + 1, z - 1); r += EXY(i, j, z); r += EXY(i, j, z - 1);
+				db[4 + side] = r;
+			}
+			for (int f = 0; f < 6; f++) fdb[(long)f * n + c] += db[f];
+			/* flux slots: -d slot from the cell's own d-face, +d from the next */
+			for (int d = 0; d < 3; d++) {
+				const long cp = cid(&D, i + (d == 0), j + (d == 1), k + (d == 2));
+				for (int q = 0; q < 8; q++) {
+					SLOT(2 * d, q, c) += P[(d * 8 + q) * n + c];
+					SLOT(2 * d + 1, q, c) += P[(d * 8 + q) * n + cp];
+				}
+			}
+			if (sub % period[c] != 0) continue;
+
+			/* update_mhd_state of a cell due at this substep */
+			double rho = SC(0, c), m[3] = {SC(1, c), SC(2, c), SC(3, c)}, E = SC(4, c);
+			double vb[3] = {SC(5, c), SC(6, c), SC(7, c)};
+			for (int d = 0; d < 3; d++) rho += (SLOT(2 * d, RHO, c) - SLOT(2 * d + 1, RHO, c)) / len[d];
+			for (int d = 0; d < 3; d++) {
+				const double il = 1 / len[d];
+				for (int q = 0; q < 3; q++)
+					m[q] = m[q] + (SLOT(2 * d, M0 + q, c) - SLOT(2 * d + 1, M0 + q, c)) * il;
+			}
+			for (int d = 0; d < 3; d++) E += (SLOT(2 * d, NRJ, c) - SLOT(2 * d + 1, NRJ, c)) / len[d];
+			for (int d = 0; d < 3; d++) {
+				const double il = 1 / len[d];
+				for (int q = 0; q < 3; q++)
+					vb[q] = vb[q] + (SLOT(2 * d, B0 + q, c) - SLOT(2 * d + 1, B0 + q, c)) * il;
+			}
+			for (int d = 0; d < 3; d++) {
+				SC(8 + 2 * d, c) += fdb[(long)(2 * d) * n + c] / area[d];
+				SC(9 + 2 * d, c) += fdb[(long)(2 * d + 1) * n + c] / area[d];
+			}
+			for (int dir = 0; dir < 6; dir++) {
+				for (int q = 0; q < 8; q++) SLOT(dir, q, c) = 0;
+				fdb[(long)dir * n + c] = 0;
+			}
+			/* update_vol_B(s): constant-pressure refit from the own copies */
+			SC(0, c) = rho;
+			for (int q = 0; q < 3; q++) SC(1 + q, c) = m[q];
+			if (rho > 0) {
+				const double vb_new[3] = {0.5 * (SC(8, c) + SC(9, c)), 0.5 * (SC(10, c) + SC(11, c)),
+					0.5 * (SC(12, c) + SC(13, c))};
+				refit_energy(rho, m, &E, vb, vb_new, g->gamma, g->mu0);
+				for (int q = 0; q < 3; q++) SC(5 + q, c) = vb_new[q];
+			} else {
+				for (int q = 0; q < 3; q++) SC(5 + q, c) = vb[q];
+			}
+			SC(4, c) = E;
+		}
+#undef PF
+#undef EZX
+#undef EYX
+#undef EXY
+#undef EZY
+#undef EXZ
+#undef EYZ
+	}
+
+	/* sync_magnetic_field: per face, the cell with the smaller index visits
+	   first; each visit moves both copies by half their difference */
+	for (int d = 0; d < 3; d++)
+	for (long k = 0; k < D.nz; k++)
+	for (long j = 0; j < D.ny; j++)
+	for (long i = 0; i < D.nx; i++) {
+		const long b = cid(&D, i, j, k);                                   /* -d copy */
+		const long a = cid(&D, i - (d == 0), j - (d == 1), k - (d == 2));   /* +d copy */
+		double* pa = &SC(9 + 2 * d, a);
+		double* pb = &SC(8 + 2 * d, b);
+		if (a < b) {
+			double diff = *pa - *pb; *pa -= diff / 2; *pb += diff / 2;
+			diff = *pb - *pa; *pb -= diff / 2; *pa += diff / 2;
+		} else {
+			double diff = *pb - *pa; *pb -= diff / 2; *pa += diff / 2;
+			diff = *pa - *pb; *pa -= diff / 2; *pb += diff / 2;
+		}
+	}
+	/* update_vol_B(0) of all cells, then back to the face-once layout */
+	for (long c = 0; c < n; c++) {
+		double rho = SC(0, c), m[3] = {SC(1, c), SC(2, c), SC(3, c)}, E = SC(4, c);
+		const double vb[3] = {SC(5, c), SC(6, c), SC(7, c)};
+		if (rho > 0) {
+			const double vb_new[3] = {0.5 * (SC(8, c) + SC(9, c)), 0.5 * (SC(10, c) + SC(11, c)),
+				0.5 * (SC(12, c) + SC(13, c))};
+			refit_energy(rho, m, &E, vb, vb_new, g->gamma, g->mu0);
+		}
+		for (int q = 0; q < 4; q++) state[q * n + c] = q == 0 ? rho : m[q - 1];
+		state[4 * n + c] = E;
+		for (int d = 0; d < 3; d++) state[(5 + d) * n + c] = SC(8 + 2 * d, c);
+	}
+	*dt_out = total_dt;
+	*max_period_out = maxp;
+done:
+	free(period); free(minsub); free(S); free(slot); free(fdb); free(P);
+	return err;
+}
+#undef SC
+#undef SLOT
+
+/* apply_boundaries of a periodic grid: one refit with the face average */
+void or_refit_periodic(const or_grid* g, double* state)
+{
+	const dims_t D = {g->nx, g->ny, g->nz, (long)g->nx * g->ny * g->nz};
+	const long n = D.n;
+	for (long k = 0; k < D.nz; k++)
+	for (long j = 0; j < D.ny; j++)
+	for (long i = 0; i < D.nx; i++) {
+		const long c = cid(&D, i, j, k);
+		double vb[3];
+		vol_b(&D, state, i, j, k, vb);
+		const double m[3] = {state[n + c], state[2 * n + c], state[3 * n + c]};
+		refit_energy(state[c], m, &state[4 * n + c], vb, vb, g->gamma, g->mu0);
+	}
+}
+
+int or_run_periodic_substep(const or_grid* g, int solver, double cfl, int substep_min,
+	int substep_max, double t_start, double t_end, long max_steps, double* state,
+	long* steps_taken, double* t_final, int* last_max_period)
+{
+	const long n = (long)g->nx * g->ny * g->nz;
+	double* maxv = (double*)malloc(sizeof(double) * 3 * n);
+	if (!maxv) return OR_ERR_SOLVER;
+	for (long f = 0; f < 3 * n; f++) maxv[f] = -1;
+	or_refit_periodic(g, state);
+	/* final init: the speeds are unknown, so dt = 0 and one substep */
+	double dt;
+	int mp = 0;
+	int e = or_step_substep(g, solver, cfl, 0.0, substep_min, substep_max, state, maxv, &dt, &mp, NULL);
+	if (e) { free(maxv); return e; }
+	double t = t_start;
+	long steps = 0;
+	while (t < t_end && steps < max_steps) {
+		e = or_step_substep(g, solver, cfl, t_end - t, substep_min, substep_max, state, maxv,
+			&dt, &mp, NULL);
+		if (e) { free(maxv); return e; }
+		or_refit_periodic(g, state);
+		t += dt;
+		steps++;
+	}
+	free(maxv);
+	if (steps_taken) *steps_taken = steps;
+	if (t_final) *t_final = t;
+	if (last_max_period) *last_max_period = mp;
+	return OR_OK;
+}
+
+/* ---------------------------------------------------------------------- */
 /* 1-D plain method of tests/mhd/reference.cpp                            */
 /* ---------------------------------------------------------------------- */
 

@@ -131,6 +131,30 @@ int or_run_periodic(const or_grid* g, int solver, double cfl, double t_start,
 	double t_end, long max_steps, double* state, long* steps_taken,
 	double* t_final);
 
+/*
+Per-cell 2^N substepping (timestep() of source/mhd/solve.hpp:1837-1924 with
+source/substepping.hpp).  or_substep_periods computes each cell's period
+(2^N), the smallest period among each cell and its face and edge neighbours
+(minsub), the substep length and the largest period from the per-face
+signal speeds maxv (3 * n, face on each cell's negative side; <= 0 = not
+yet known).  or_step_substep advances `state` (face-once layout) by one
+step of *dt_out = max period * substep length, updating maxv; it makes the
+reference's in-step refits (one per update of a cell, one after the face
+sync), not apply_boundaries' (or_refit_periodic).  or_run_periodic_substep
+is or_run_periodic with substepping.  substep_min and substep_max are the
+options' integer values ("substep-min", "substep-max").
+*/
+int or_substep_periods(const or_grid* g, double time_step_factor, double max_dt,
+	int substep_min, int substep_max, const double* maxv, int* period, int* minsub,
+	double* sub_dt, int* max_period);
+int or_step_substep(const or_grid* g, int solver, double time_step_factor, double max_dt,
+	int substep_min, int substep_max, double* state, double* maxv, double* dt_out,
+	int* max_period_out, long* bad_face);
+void or_refit_periodic(const or_grid* g, double* state);
+int or_run_periodic_substep(const or_grid* g, int solver, double cfl, int substep_min,
+	int substep_max, double t_start, double t_end, long max_steps, double* state,
+	long* steps_taken, double* t_final, int* last_max_period);
+
 /* ---- 1-D plain method of tests/mhd/reference.cpp ---- */
 typedef struct {
 	size_t ncells;           /* including the two end cells */

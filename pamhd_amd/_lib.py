@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(PKG, "lib", "libpamhd_mi355x.so")
 RUSANOV, RUSANOV_STAGGERED, HLL_ATHENA, HLLD_ATHENA, ROE_ATHENA, HYBRID = range(6)
 FP64, MIXED = 0, 1
 PAMHD_E_POISSON = 9
+PAMHD_E_ARG = 16
+PAMHD_E_UNSUPPORTED = 17
 
 EXPORTS = (
     "pamhd_abi_version", "pamhd_create", "pamhd_destroy", "pamhd_upload_state",
@@ -28,7 +30,7 @@ EXPORTS = (
     "pamhd_ensemble_check", "pamhd_ensemble_download", "pamhd_ensemble_cells",
     "pamhd_ensemble_set_tile", "pamhd_ensemble_autotune", "pamhd_ensemble_last_error",
     "pamhd_step_fixed", "pamhd_set_state_device", "pamhd_get_state_device",
-    "pamhd_adjoint_step", "pamhd_set_slabs",
+    "pamhd_adjoint_step", "pamhd_set_slabs", "pamhd_set_substepping", "pamhd_last_max_substep",
 )
 
 
@@ -111,6 +113,8 @@ def load():
                                         _vp, _vp, _vp, _vp]
     L.pamhd_autotune.argtypes = [H, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, _vp]
     L.pamhd_set_launch.argtypes = [H, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.pamhd_set_substepping.argtypes = [H, ctypes.c_int, ctypes.c_int]
+    L.pamhd_last_max_substep.argtypes = [H, ctypes.POINTER(ctypes.c_int)]
     L.pamhd_set_slabs.argtypes = [H, ctypes.c_int]
     L.pamhd_enable_timing.argtypes = [H, ctypes.c_int]
     L.pamhd_kernel_time.argtypes = [H, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_long)]

@@ -61,6 +61,33 @@ def orszag_tang_vortex(cells, noise=0.01, seed=12345, divergence_sigma=0.0):
     return state.reshape(8, n), (dx, dy, dz), gamma, mu0
 
 
+def hot_blob(cells, contrast=16.0, radius=0.2, seed=1, noise=0.01):
+    """Periodic unit box at rest with a uniform field, a hot sphere of
+    `contrast` times the background pressure (sound speed sqrt(contrast)
+    times larger) at the centre and seeded +-`noise` density noise: the
+    per-cell time steps differ by ~sqrt(contrast), so per-cell substepping
+    gives the outer cells longer periods.  Returns (state (8, n), cell
+    length, gamma, mu0)."""
+    nx, ny, nz = cells
+    dx, dy, dz = 1.0 / nx, 1.0 / ny, 1.0 / nz
+    gamma, mu0 = 5.0 / 3.0, 1.0
+    rng = np.random.default_rng(seed)
+    x = (np.arange(nx) + 0.5) * dx - 0.5
+    y = (np.arange(ny) + 0.5) * dy - 0.5
+    z = (np.arange(nz) + 0.5) * dz - 0.5
+    r = np.sqrt(z[:, None, None] ** 2 + y[None, :, None] ** 2 + x[None, None, :] ** 2)
+    rho = 1.0 + noise * (2.0 * rng.random((nz, ny, nx)) - 1.0)
+    p = np.where(r < radius, contrast, 1.0) * 0.1
+    B = (0.05, 0.03, 0.02)
+    n = nx * ny * nz
+    state = np.zeros((8, nz, ny, nx))
+    state[0] = rho
+    state[4] = p / (gamma - 1) + 0.5 / mu0 * (B[0] ** 2 + B[1] ** 2 + B[2] ** 2)
+    for d in range(3):
+        state[5 + d] = B[d]
+    return state.reshape(8, n), (dx, dy, dz), gamma, mu0
+
+
 # ---------------------------------------------------------------- solar wind
 
 K_BOLTZMANN = 1.380649e-23          # source/simulation_options.hpp:63

@@ -1050,6 +1050,345 @@ const char* error_name(int e)
 	}
 }
 
+/*
+Per-cell 2^N substepping of timestep(), source/mhd/solve.hpp:1837-1924 with
+source/substepping.hpp, on a periodic grid of normal cells (FP64).  Working
+layout: S 14 x n (rho, m, E, vol B, the cell's copies of its -x, +x, -y, +y,
+-z, +z faces), slots 48 x n ((2d + side) * 8 + q), face dB 6 x n, per-face
+signal speeds 3 x n, per-cell time step, level exponent, period and the
+smallest period of the cell's face and edge neighbourhood.  The state the
+handle keeps is the face-once layout; a substepped step expands it, runs
+the substeps, syncs the face copies and writes it back.  A face copy's
+change sums the same 12 edge terms in the same order as update_cell_kernel;
+slots and face dB accumulate substep by substep.
+*/
+struct SubstepDev {
+	double* S = nullptr;
+	double* slot = nullptr;
+	double* fdb = nullptr;
+	double* maxv = nullptr;
+	double* ts = nullptr;
+	int* e[2] = {nullptr, nullptr};
+	int* period = nullptr;
+	int* minsub = nullptr;
+	unsigned long long* scal = nullptr;   /* [0] smallest time step (bits), [1] largest period */
+	double* sub_dt = nullptr;
+	StepStatus* status = nullptr;
+	unsigned long long* host = nullptr;   /* pinned: [0] largest period, [1] substep length (bits) */
+	StepStatus* host_status = nullptr;
+	int smin = 0, smax = 0;
+	int last_max = 1;
+};
+
+__device__ __forceinline__ int cidx(const GridDev& g, int i, int j, int k)
+{
+	return i + g.nx * (j + g.ny * k);
+}
+
+/* the 18 face and edge neighbours of (i, j, k) */
+template <class F> __device__ __forceinline__ void for_face_edge_neighbours(const GridDev& g,
+	int i, int j, int k, F&& f)
+{
+	for (int dk = -1; dk <= 1; dk++)
+	for (int dj = -1; dj <= 1; dj++)
+	for (int di = -1; di <= 1; di++) {
+		const int nz_ = (di != 0) + (dj != 0) + (dk != 0);
+		if (nz_ == 0 || nz_ == 3) continue;
+		const int a = di < 0 ? wrap_dec(i, g.nx) : di > 0 ? wrap_inc(i, g.nx) : i;
+		const int b = dj < 0 ? wrap_dec(j, g.ny) : dj > 0 ? wrap_inc(j, g.ny) : j;
+		const int c = dk < 0 ? wrap_dec(k, g.nz) : dk > 0 ? wrap_inc(k, g.nz) : k;
+		f(cidx(g, a, b, c));
+	}
+}
+
+__global__ void ss_clear_kernel(SubstepDev d)
+{
+	if (threadIdx.x == 0) {
+		d.scal[0] = (unsigned long long)__double_as_longlong(DBL_MAX);
+		d.scal[1] = 0;
+		d.status->err = 0;
+		d.status->err_face = -1;
+	}
+}
+
+/* minimize_timestep, source/mhd/solve.hpp:1927-2014: a cell's limit over its
+   six faces, and the smallest over the grid */
+__global__ __launch_bounds__(BX * BY) void ss_ts_kernel(GridDev g, SubstepDev d, double cfl)
+{
+	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	double t = DBL_MAX;
+	if (i < g.nx && j < g.ny) {
+		const int c = cidx(g, i, j, k);
+		const double len[3] = {g.dx, g.dy, g.dz};
+		const int nb[3] = {cidx(g, wrap_inc(i, g.nx), j, k), cidx(g, i, wrap_inc(j, g.ny), k),
+			cidx(g, i, j, wrap_inc(k, g.nz))};
+		for (int q = 0; q < 3; q++) {
+			const double v[2] = {d.maxv[(long long)q * g.n + c], d.maxv[(long long)q * g.n + nb[q]]};
+			for (int sd = 0; sd < 2; sd++) t = v[sd] <= 0 ? 0.0 : smin(cfl * len[q] / v[sd], t);
+		}
+		d.ts[c] = t;
+	}
+	for (int off = 32; off > 0; off >>= 1) t = smin(t, __shfl_xor(t, off, 64));
+	if ((threadIdx.x + BX * threadIdx.y) % 64 == 0) {
+		atomicMin(&d.scal[0], (unsigned long long)__double_as_longlong(t));
+	}
+}
+
+/* set_minmax_substepping_period, source/substepping.hpp:264-399, with
+   uniform options (the smallest minimum is shifted to 0): the substep
+   length and each cell's largest level.  floor(log2 x) is exact (ilogb);
+   a non-finite ratio gives level 0. */
+__global__ __launch_bounds__(BX * BY) void ss_level_kernel(GridDev g, SubstepDev d, double max_dt)
+{
+	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	const double smallest = __longlong_as_double((long long)d.scal[0]);
+	const bool all_zero = max_dt <= smallest;
+	if (i == 0 && j == 0 && k == 0) *d.sub_dt = all_zero ? max_dt : smallest;
+	if (i >= g.nx || j >= g.ny) return;
+	const int c = cidx(g, i, j, k);
+	int lev = 0;
+	const double t = d.ts[c];
+	if (!all_zero && t > 0) {
+		const double x = smin(t, max_dt) / smallest;
+		int step = (__builtin_isfinite(x) && x > 0) ? ilogb(x) : 0;
+		if (step < 0) step = 0;
+		lev = d.smax > step ? step : d.smax;
+	}
+	d.e[0][c] = lev;
+}
+
+/* one Jacobi pass of restrict_substepping_period, substepping.hpp:99-191 */
+__global__ __launch_bounds__(BX * BY) void ss_restrict_kernel(GridDev g, const int* __restrict__ in,
+	int* __restrict__ out)
+{
+	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	if (i >= g.nx || j >= g.ny) return;
+	const int c = cidx(g, i, j, k);
+	int v = in[c];
+	for_face_edge_neighbours(g, i, j, k, [&](int nb) { if (v > in[nb] + 1) v = in[nb] + 1; });
+	out[c] = v;
+}
+
+/* update_substeps (2^N periods), substepping.hpp:54-96, and the smallest
+   period of each cell's face and edge neighbourhood (get_fluxes,
+   source/mhd/solve.hpp:252-289) */
+__global__ __launch_bounds__(BX * BY) void ss_period_kernel(GridDev g, SubstepDev d, const int* __restrict__ e)
+{
+	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	unsigned long long p = 0;
+	if (i < g.nx && j < g.ny) {
+		const int c = cidx(g, i, j, k);
+		int lo = e[c];
+		for_face_edge_neighbours(g, i, j, k, [&](int nb) { if (e[nb] < lo) lo = e[nb]; });
+		d.period[c] = 1 << e[c];
+		d.minsub[c] = 1 << lo;
+		p = 1ull << e[c];
+	}
+	for (int off = 32; off > 0; off >>= 1) {
+		const unsigned long long o = __shfl_xor(p, off, 64);
+		p = o > p ? o : p;
+	}
+	if ((threadIdx.x + BX * threadIdx.y) % 64 == 0) atomicMax(&d.scal[1], p);
+}
+
+/* the working layout from the face-once state */
+__global__ __launch_bounds__(BX * BY) void ss_expand_kernel(GridDev g, const double* __restrict__ s, SubstepDev d)
+{
+	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	const int c = cidx(g, i, j, k);
+	const int nb[3] = {cidx(g, wrap_inc(i, g.nx), j, k), cidx(g, i, wrap_inc(j, g.ny), k),
+		cidx(g, i, j, wrap_inc(k, g.nz))};
+	for (int q = 0; q < 5; q++) d.S[q * n + c] = s[q * n + c];
+	for (int q = 0; q < 3; q++) {
+		d.S[(5 + q) * n + c] = 0.5 * (s[(5 + q) * n + c] + s[(5 + q) * n + nb[q]]);
+		d.S[(8 + 2 * q) * n + c] = s[(5 + q) * n + c];
+		d.S[(9 + 2 * q) * n + c] = s[(5 + q) * n + nb[q]];
+	}
+}
+
+/* faces of direction D due at substep `sub`: solved from the cells' stored
+   volume field and weighted by the substep length times the owner's
+   smallest neighbourhood period; the others get 0 */
+template <int SOLVER, int D>
+__global__ __launch_bounds__(BX * BY) void ss_flux_kernel(GridDev g, SubstepDev d,
+	const double* __restrict__ bg, double* __restrict__ P, int sub)
+{
+	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	const int c = cidx(g, i, j, k);
+	const int o = cidx(g, D == 0 ? wrap_dec(i, g.nx) : i, D == 1 ? wrap_dec(j, g.ny) : j,
+		D == 2 ? wrap_dec(k, g.nz) : k);
+	double* Pd = P + (long long)D * 8 * n;
+	const int m = d.minsub[o];
+	if (sub % m != 0) {
+		for (int q = 0; q < 8; q++) Pd[q * n + c] = 0;
+		return;
+	}
+	double L[8], R[8];
+	const int cc[2] = {o, c};
+	double* U[2] = {L, R};
+	for (int sd = 0; sd < 2; sd++) {
+		const int x = cc[sd];
+		const double mm[3] = {d.S[1 * n + x], d.S[2 * n + x], d.S[3 * n + x]};
+		const double B[3] = {d.S[5 * n + x], d.S[6 * n + x], d.S[7 * n + x]};
+		U[sd][RHO] = d.S[x];
+		U[sd][NRJ] = d.S[4 * n + x];
+		U[sd][M0] = mm[D]; U[sd][M1] = mm[(D + 1) % 3]; U[sd][M2] = mm[(D + 2) % 3];
+		U[sd][B0] = B[D]; U[sd][B1] = B[(D + 1) % 3]; U[sd][B2] = B[(D + 2) % 3];
+	}
+	double b[3] = {0, 0, 0};
+	if (bg) {
+		const double* bgd = bg + (size_t)(3 * D) * n;
+		b[0] = bgd[(size_t)D * n + o];
+		b[1] = bgd[(size_t)((D + 1) % 3) * n + o];
+		b[2] = bgd[(size_t)((D + 2) % 3) * n + o];
+	}
+	double F[8], sp = 0;
+	const int e = riemann<SOLVER, double>(L, R, b, g.ph, F, sp);
+	if (e) {
+		record_error(d.status, e, (long long)D * n + c);
+		for (int q = 0; q < 8; q++) Pd[q * n + c] = 0;
+		return;
+	}
+	d.maxv[(long long)D * n + c] = sp;
+	double Fg[8];
+	Fg[RHO] = F[RHO];
+	Fg[NRJ] = F[NRJ];
+	Fg[M0 + D] = F[M0]; Fg[M0 + (D + 1) % 3] = F[M1]; Fg[M0 + (D + 2) % 3] = F[M2];
+	Fg[B0 + D] = F[B0]; Fg[B0 + (D + 1) % 3] = F[B1]; Fg[B0 + (D + 2) % 3] = F[B2];
+	const double min_dt = *d.sub_dt * m;
+	for (int q = 0; q < 8; q++) Pd[q * n + c] = min_dt * Fg[q];
+}
+
+/* slots and face dB gather this substep's contributions; cells whose period
+   divides `sub` apply them (update_mhd_state, source/mhd/solve.hpp:1503-1603)
+   and refit from their own face copies (update_vol_B, :1618-1683) */
+__global__ __launch_bounds__(BX * BY) void ss_update_kernel(GridDev g, SubstepDev d,
+	const double* __restrict__ P, int sub)
+{
+	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	const int c = cidx(g, i, j, k);
+	const int im = wrap_dec(i, g.nx), ip = wrap_inc(i, g.nx);
+	const int jm = wrap_dec(j, g.ny), jp = wrap_inc(j, g.ny);
+	const int km = wrap_dec(k, g.nz), kp = wrap_inc(k, g.nz);
+	const Nb L{g.nx, g.ny};
+	const EdgeTerms<double> E{P, n, g.dx, g.dy, g.dz};
+	const double db[6] = {
+		dBx_at(E, L, i, im, j, jm, jp, k, km, kp),
+		dBx_at(E, L, ip, i, j, jm, jp, k, km, kp),
+		dBy_at(E, L, i, im, ip, j, jm, k, km, kp),
+		dBy_at(E, L, i, im, ip, jp, j, k, km, kp),
+		dBz_at(E, L, i, im, ip, j, jm, jp, k, km),
+		dBz_at(E, L, i, im, ip, j, jm, jp, kp, k)};
+	double fd[6];
+	for (int f = 0; f < 6; f++) fd[f] = d.fdb[f * n + c] + db[f];
+	const int cp[3] = {cidx(g, ip, j, k), cidx(g, i, jp, k), cidx(g, i, j, kp)};
+	double sl[6][8];
+	for (int q3 = 0; q3 < 3; q3++)
+	for (int q = 0; q < 8; q++) {
+		sl[2 * q3][q] = d.slot[((2 * q3) * 8 + q) * n + c] + P[(q3 * 8 + q) * n + c];
+		sl[2 * q3 + 1][q] = d.slot[((2 * q3 + 1) * 8 + q) * n + c] + P[(q3 * 8 + q) * n + cp[q3]];
+	}
+	if (sub % d.period[c] != 0) {
+		for (int f = 0; f < 6; f++) d.fdb[f * n + c] = fd[f];
+		for (int dir = 0; dir < 6; dir++)
+		for (int q = 0; q < 8; q++) d.slot[(dir * 8 + q) * n + c] = sl[dir][q];
+		return;
+	}
+	const double len[3] = {g.dx, g.dy, g.dz};
+	const double area[3] = {g.dy * g.dz, g.dx * g.dz, g.dx * g.dy};
+	double rho = d.S[c], m[3] = {d.S[n + c], d.S[2 * n + c], d.S[3 * n + c]}, En = d.S[4 * n + c];
+	double vb[3] = {d.S[5 * n + c], d.S[6 * n + c], d.S[7 * n + c]};
+	for (int q3 = 0; q3 < 3; q3++) rho += (sl[2 * q3][RHO] - sl[2 * q3 + 1][RHO]) / len[q3];
+	for (int q3 = 0; q3 < 3; q3++) {
+		const double il = 1 / len[q3];
+		for (int q = 0; q < 3; q++) m[q] = m[q] + (sl[2 * q3][M0 + q] - sl[2 * q3 + 1][M0 + q]) * il;
+	}
+	for (int q3 = 0; q3 < 3; q3++) En += (sl[2 * q3][NRJ] - sl[2 * q3 + 1][NRJ]) / len[q3];
+	for (int q3 = 0; q3 < 3; q3++) {
+		const double il = 1 / len[q3];
+		for (int q = 0; q < 3; q++) vb[q] = vb[q] + (sl[2 * q3][B0 + q] - sl[2 * q3 + 1][B0 + q]) * il;
+	}
+	double fc[6];
+	for (int q3 = 0; q3 < 3; q3++) {
+		fc[2 * q3] = d.S[(8 + 2 * q3) * n + c] + fd[2 * q3] / area[q3];
+		fc[2 * q3 + 1] = d.S[(9 + 2 * q3) * n + c] + fd[2 * q3 + 1] / area[q3];
+	}
+	for (int f = 0; f < 6; f++) {
+		d.S[(8 + f) * n + c] = fc[f];
+		d.fdb[f * n + c] = 0;
+	}
+	for (int dir = 0; dir < 6; dir++)
+	for (int q = 0; q < 8; q++) d.slot[(dir * 8 + q) * n + c] = 0;
+	d.S[c] = rho;
+	for (int q = 0; q < 3; q++) d.S[(1 + q) * n + c] = m[q];
+	if (rho > 0) {
+		const double vb_new[3] = {0.5 * (fc[0] + fc[1]), 0.5 * (fc[2] + fc[3]), 0.5 * (fc[4] + fc[5])};
+		refit_energy(rho, m, En, vb, vb_new, g.ph);
+		for (int q = 0; q < 3; q++) d.S[(5 + q) * n + c] = vb_new[q];
+	} else {
+		for (int q = 0; q < 3; q++) d.S[(5 + q) * n + c] = vb[q];
+	}
+	d.S[4 * n + c] = En;
+}
+
+/* sync_magnetic_field, common_functions.hpp:184-269: the two visits to a
+   face, the cell with the smaller index first */
+template <int D>
+__global__ __launch_bounds__(BX * BY) void ss_sync_kernel(GridDev g, SubstepDev d)
+{
+	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	const int b = cidx(g, i, j, k);
+	const int a = cidx(g, D == 0 ? wrap_dec(i, g.nx) : i, D == 1 ? wrap_dec(j, g.ny) : j,
+		D == 2 ? wrap_dec(k, g.nz) : k);
+	double pa = d.S[(9 + 2 * D) * n + a], pb = d.S[(8 + 2 * D) * n + b];
+	if (a < b) {
+		double diff = pa - pb; pa -= diff / 2; pb += diff / 2;
+		diff = pb - pa; pb -= diff / 2; pa += diff / 2;
+	} else {
+		double diff = pb - pa; pb -= diff / 2; pa += diff / 2;
+		diff = pa - pb; pa -= diff / 2; pb += diff / 2;
+	}
+	d.S[(9 + 2 * D) * n + a] = pa;
+	d.S[(8 + 2 * D) * n + b] = pb;
+}
+
+/* update_vol_B(0) of all cells and the face-once state */
+__global__ __launch_bounds__(BX * BY) void ss_final_kernel(GridDev g, SubstepDev d, double* __restrict__ s)
+{
+	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	const int c = cidx(g, i, j, k);
+	const double rho = d.S[c], m[3] = {d.S[n + c], d.S[2 * n + c], d.S[3 * n + c]};
+	double En = d.S[4 * n + c];
+	const double vb[3] = {d.S[5 * n + c], d.S[6 * n + c], d.S[7 * n + c]};
+	if (rho > 0) {
+		const double vb_new[3] = {
+			0.5 * (d.S[8 * n + c] + d.S[9 * n + c]),
+			0.5 * (d.S[10 * n + c] + d.S[11 * n + c]),
+			0.5 * (d.S[12 * n + c] + d.S[13 * n + c])};
+		refit_energy(rho, m, En, vb, vb_new, g.ph);
+	}
+	s[c] = rho;
+	for (int q = 0; q < 3; q++) s[(1 + q) * n + c] = m[q];
+	s[4 * n + c] = En;
+	for (int q = 0; q < 3; q++) s[(5 + q) * n + c] = d.S[(8 + 2 * q) * n + c];
+}
+
+__global__ __launch_bounds__(BX * BY) void ss_fill_kernel(double* __restrict__ a, long long n, double v)
+{
+	const long long t = (long long)blockIdx.x * blockDim.x * blockDim.y + threadIdx.x + blockDim.x * threadIdx.y;
+	if (t < n) a[t] = v;
+}
+
 } // namespace
 
 struct pamhd_handle {
@@ -1091,6 +1430,7 @@ struct pamhd_handle {
 	BoxSet* box = nullptr;    /* solar-wind box (non-periodic) handles only */
 	double* adj = nullptr;    /* adjoint scratch, allocated on first use */
 	PoissonMG* mg = nullptr;  /* multigrid hierarchy, allocated on first cleaning */
+	SubstepDev* sub = nullptr; /* per-cell substepping, allocated when enabled */
 	int mg_fused = 0;   /* per-colour passes (faster; see pamhd_set_poisson_smoother) */
 	int cur = 0;    /* state buffer holding the current state */
 	int last = 0;   /* status entry of the last flux pass */
@@ -1324,6 +1664,125 @@ int launch_step_body(pamhd_handle* h, int solver, hipStream_t st, StepStatus* ne
 	return PAMHD_OK;
 }
 
+void substep_free(SubstepDev* d)
+{
+	if (!d) return;
+	hipFree(d->S);
+	hipFree(d->slot);
+	hipFree(d->fdb);
+	hipFree(d->maxv);
+	hipFree(d->ts);
+	hipFree(d->e[0]);
+	hipFree(d->e[1]);
+	hipFree(d->period);
+	hipFree(d->minsub);
+	hipFree(d->scal);
+	hipFree(d->sub_dt);
+	hipFree(d->status);
+	hipHostFree(d->host);
+	hipHostFree(d->host_status);
+	delete d;
+}
+
+int substep_alloc(pamhd_handle* h)
+{
+	const size_t n = (size_t)h->g.n;
+	SubstepDev* d = new SubstepDev();
+	hipError_t e = hipSuccess;
+	auto al = [&](auto** p, size_t bytes) {
+		if (e == hipSuccess) e = hipMalloc((void**)p, bytes);
+	};
+	al(&d->S, 14 * n * sizeof(double));
+	al(&d->slot, 48 * n * sizeof(double));
+	al(&d->fdb, 6 * n * sizeof(double));
+	al(&d->maxv, 3 * n * sizeof(double));
+	al(&d->ts, n * sizeof(double));
+	al(&d->e[0], n * sizeof(int));
+	al(&d->e[1], n * sizeof(int));
+	al(&d->period, n * sizeof(int));
+	al(&d->minsub, n * sizeof(int));
+	al(&d->scal, 2 * sizeof(unsigned long long));
+	al(&d->sub_dt, sizeof(double));
+	al(&d->status, sizeof(StepStatus));
+	if (e == hipSuccess) e = hipHostMalloc((void**)&d->host, 2 * sizeof(unsigned long long));
+	if (e == hipSuccess) e = hipHostMalloc((void**)&d->host_status, sizeof(StepStatus));
+	if (e != hipSuccess) {
+		substep_free(d);
+		return hip_fail(h, e, "substepping buffers");
+	}
+	/* slots and face dB start at zero and are zero again after every step;
+	   speeds are unknown until the next flux pass */
+	const unsigned blocks = (unsigned)((48 * n + 255) / 256);
+	ss_fill_kernel<<<blocks, dim3(BX, BY, 1)>>>(d->slot, (long long)(48 * n), 0.0);
+	ss_fill_kernel<<<(unsigned)((6 * n + 255) / 256), dim3(BX, BY, 1)>>>(d->fdb, (long long)(6 * n), 0.0);
+	ss_fill_kernel<<<(unsigned)((3 * n + 255) / 256), dim3(BX, BY, 1)>>>(d->maxv, (long long)(3 * n), -1.0);
+	HIP_TRY(h, hipGetLastError());
+	HIP_TRY(h, hipDeviceSynchronize());
+	h->sub = d;
+	return PAMHD_OK;
+}
+
+template <int SOLVER>
+void ss_launch_flux(pamhd_handle* h, double* P, int sub, hipStream_t st)
+{
+	const dim3 grid = cell_grid(h->g), block(BX, BY, 1);
+	ss_flux_kernel<SOLVER, 0><<<grid, block, 0, st>>>(h->g, *h->sub, h->bg, P, sub);
+	ss_flux_kernel<SOLVER, 1><<<grid, block, 0, st>>>(h->g, *h->sub, h->bg, P, sub);
+	ss_flux_kernel<SOLVER, 2><<<grid, block, 0, st>>>(h->g, *h->sub, h->bg, P, sub);
+}
+
+/* timestep() with per-cell substepping (and apply_boundaries' refit when
+   refits == 3), in place on the current state */
+int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
+	double* dt_taken, hipStream_t st)
+{
+	SubstepDev& d = *h->sub;
+	const dim3 grid = cell_grid(h->g), block(BX, BY, 1);
+	double* P = h->flux;   /* 24 x n FP64 (substepping runs in FP64) */
+	ss_clear_kernel<<<1, 64, 0, st>>>(d);
+	ss_ts_kernel<<<grid, block, 0, st>>>(h->g, d, cfl);
+	ss_level_kernel<<<grid, block, 0, st>>>(h->g, d, max_dt);
+	int cur = 0;
+	for (int r = 0; r < d.smax; r++) {
+		ss_restrict_kernel<<<grid, block, 0, st>>>(h->g, d.e[cur], d.e[1 - cur]);
+		cur = 1 - cur;
+	}
+	ss_period_kernel<<<grid, block, 0, st>>>(h->g, d, d.e[cur]);
+	HIP_TRY(h, hipGetLastError());
+	HIP_TRY(h, hipMemcpyAsync(d.host, d.scal + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+	HIP_TRY(h, hipMemcpyAsync(d.host + 1, d.sub_dt, sizeof(double), hipMemcpyDeviceToHost, st));
+	HIP_TRY(h, hipStreamSynchronize(st));
+	const int maxp = (int)d.host[0];
+	double sub_dt;
+	memcpy(&sub_dt, d.host + 1, sizeof(double));
+	if (maxp < 1) return fail(h, PAMHD_E_DEVICE, "substep periods not computed");
+
+	ss_expand_kernel<<<grid, block, 0, st>>>(h->g, h->state[h->cur], d);
+	double total = 0;
+	for (int sub = 1; sub <= maxp; sub++) {
+		total += sub_dt;
+		switch (solver) {
+		case RUSANOV: ss_launch_flux<RUSANOV>(h, P, sub, st); break;
+		case HLL_ATHENA: ss_launch_flux<HLL_ATHENA>(h, P, sub, st); break;
+		case HLLD_ATHENA: ss_launch_flux<HLLD_ATHENA>(h, P, sub, st); break;
+		case ROE_ATHENA: ss_launch_flux<ROE_ATHENA>(h, P, sub, st); break;
+		default: return fail(h, PAMHD_E_SOLVER, "invalid solver " + std::to_string(solver));
+		}
+		ss_update_kernel<<<grid, block, 0, st>>>(h->g, d, P, sub);
+	}
+	ss_sync_kernel<0><<<grid, block, 0, st>>>(h->g, d);
+	ss_sync_kernel<1><<<grid, block, 0, st>>>(h->g, d);
+	ss_sync_kernel<2><<<grid, block, 0, st>>>(h->g, d);
+	ss_final_kernel<<<grid, block, 0, st>>>(h->g, d, h->state[h->cur]);
+	if (refits == 3) refit_kernel<<<grid, block, 0, st>>>(h->g, h->state[h->cur]);
+	HIP_TRY(h, hipGetLastError());
+	HIP_TRY(h, hipMemcpyAsync(d.host_status, d.status, sizeof(StepStatus), hipMemcpyDeviceToHost, st));
+	HIP_TRY(h, hipStreamSynchronize(st));
+	d.last_max = maxp;
+	if (dt_taken) *dt_taken = total;
+	return report_status(h, *d.host_status);
+}
+
 int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
 	double* dt_taken, void* stream_, int fixed_dt = 0)
 {
@@ -1345,6 +1804,9 @@ int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
 			if (e) return fail(h, e, err);
 		}
 		return PAMHD_OK;
+	}
+	if (h->sub && h->sub->smax > 0 && !fixed_dt) {
+		return substep_step(h, solver, max_dt, cfl, refits, dt_taken, st);
 	}
 	StepStatus* prev = h->status + h->last;
 	StepStatus* next = h->status + (1 - h->last);
@@ -1485,6 +1947,7 @@ int pamhd_destroy(pamhd_handle* h)
 		delete h->mg;
 	}
 	boxset_destroy(h->box);
+	substep_free(h->sub);
 	hipFree(h->adj);
 	for (auto e : h->ev) hipEventDestroy(e);
 	if (h->aux) hipStreamDestroy(h->aux);
@@ -1727,6 +2190,33 @@ int pamhd_set_poisson_smoother(pamhd_handle* h, int fused)
 {
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	h->mg_fused = fused != 0;
+	return PAMHD_OK;
+}
+
+int pamhd_set_substepping(pamhd_handle* h, int substep_min, int substep_max)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "substepping needs a periodic grid");
+	if (substep_max < 0 || substep_max > 20) return fail(h, PAMHD_E_ARG, "substep_max must be in [0, 20]");
+	if (substep_min < 0) return fail(h, PAMHD_E_ARG, "substep_min must be >= 0");
+	if (substep_max > 0 && h->precision != PAMHD_FP64) {
+		return fail(h, PAMHD_E_UNSUPPORTED, "substepping runs in FP64 only");
+	}
+	if (substep_max > 0 && !h->sub) {
+		const int e = substep_alloc(h);
+		if (e) return e;
+	}
+	if (h->sub) {
+		h->sub->smin = substep_min;
+		h->sub->smax = substep_max;
+	}
+	return PAMHD_OK;
+}
+
+int pamhd_last_max_substep(pamhd_handle* h, int* max_substep)
+{
+	if (!h || !max_substep) return fail(h, PAMHD_E_ARG, "null argument");
+	*max_substep = (h->sub && h->sub->smax > 0) ? h->sub->last_max : 1;
 	return PAMHD_OK;
 }
 

@@ -152,6 +152,17 @@ class Grid:
         self._check(self._L.pamhd_autotune(self._h, solver_id(solver), buf, 4096, stream))
         return buf.value.decode()
 
+    def set_substepping(self, substep_min=0, substep_max=0):
+        """Per-cell 2^N substepping ("substep-min", "substep-max"); 0 = global
+        time step.  Set before the final-init timestep."""
+        self._check(self._L.pamhd_set_substepping(self._h, substep_min, substep_max))
+
+    def last_max_substep(self):
+        """Largest period (in substeps) of the last step."""
+        v = ctypes.c_int(0)
+        self._check(self._L.pamhd_last_max_substep(self._h, ctypes.byref(v)))
+        return v.value
+
     def set_launch(self, flux_remap=0, update_kz=0, update_remap=0):
         self._check(self._L.pamhd_set_launch(self._h, flux_remap, update_kz, update_remap))
 

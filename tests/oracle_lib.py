@@ -93,6 +93,20 @@ def lib():
     L.or_clean_divergence.argtypes = [ctypes.POINTER(OrGrid), _dp, ctypes.c_double,
                                       ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int)]
     L.or_clean_divergence.restype = ctypes.c_int
+    _ip = ctypes.POINTER(ctypes.c_int)
+    L.or_substep_periods.argtypes = [ctypes.POINTER(OrGrid), ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int, ctypes.c_int, _dp, _ip, _ip, _dp, _ip]
+    L.or_substep_periods.restype = ctypes.c_int
+    L.or_step_substep.argtypes = [ctypes.POINTER(OrGrid), ctypes.c_int, ctypes.c_double,
+                                  ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
+                                  _ip, _lp]
+    L.or_step_substep.restype = ctypes.c_int
+    L.or_refit_periodic.argtypes = [ctypes.POINTER(OrGrid), _dp]
+    L.or_refit_periodic.restype = None
+    L.or_run_periodic_substep.argtypes = [ctypes.POINTER(OrGrid), ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                          ctypes.c_double, ctypes.c_long, _dp, _lp, _dp, _ip]
+    L.or_run_periodic_substep.restype = ctypes.c_int
     L.or_set_threads.argtypes = [ctypes.c_int]
     L.or_set_threads.restype = ctypes.c_int
     _lib = L
@@ -177,6 +191,53 @@ def run_periodic(grid, solver, cfl, t_start, t_end, state, max_steps=1 << 40):
                                max_steps, _ptr(state), ctypes.byref(steps),
                                ctypes.byref(tf))
     return st, steps.value, tf.value
+
+
+def substep_periods(grid, cfl, max_dt, substep_min, substep_max, maxv):
+    """(status, period (n,), minsub (n,), substep length, largest period)."""
+    n = grid.nx * grid.ny * grid.nz
+    maxv = np.ascontiguousarray(maxv, dtype=np.float64)
+    period = np.zeros(n, dtype=np.int32)
+    minsub = np.zeros(n, dtype=np.int32)
+    sub_dt = ctypes.c_double(0)
+    mp = ctypes.c_int(0)
+    ip = ctypes.POINTER(ctypes.c_int)
+    st = lib().or_substep_periods(ctypes.byref(grid), cfl, max_dt, substep_min, substep_max,
+                                  _ptr(maxv), period.ctypes.data_as(ip), minsub.ctypes.data_as(ip),
+                                  ctypes.byref(sub_dt), ctypes.byref(mp))
+    return st, period, minsub, sub_dt.value, mp.value
+
+
+def step_substep(grid, solver, cfl, max_dt, substep_min, substep_max, state, maxv):
+    """One substepped step in place on state (8, n) and maxv (3, n); returns
+    (status, dt, largest period)."""
+    assert state.flags.c_contiguous and state.dtype == np.float64
+    assert maxv.flags.c_contiguous and maxv.dtype == np.float64
+    dt = ctypes.c_double(0)
+    mp = ctypes.c_int(0)
+    bad = ctypes.c_long(-1)
+    st = lib().or_step_substep(ctypes.byref(grid), solver, cfl, max_dt, substep_min, substep_max,
+                               _ptr(state), _ptr(maxv), ctypes.byref(dt), ctypes.byref(mp),
+                               ctypes.byref(bad))
+    return st, dt.value, mp.value
+
+
+def refit_periodic(grid, state):
+    lib().or_refit_periodic(ctypes.byref(grid), _ptr(state))
+
+
+def run_periodic_substep(grid, solver, cfl, substep_min, substep_max, t_start, t_end, state,
+                         max_steps=1 << 40):
+    """or_run_periodic with per-cell substepping (in place); returns
+    (status, steps, end time, largest period of the last step)."""
+    assert state.flags.c_contiguous and state.dtype == np.float64
+    steps = ctypes.c_long(0)
+    tf = ctypes.c_double(0)
+    mp = ctypes.c_int(0)
+    st = lib().or_run_periodic_substep(ctypes.byref(grid), solver, cfl, substep_min, substep_max,
+                                       t_start, t_end, max_steps, _ptr(state), ctypes.byref(steps),
+                                       ctypes.byref(tf), ctypes.byref(mp))
+    return st, steps.value, tf.value, mp.value
 
 
 def plain1d_run(solver, state, cell_size, gamma, mu0, t_end, cfl=0.5):
