@@ -77,6 +77,7 @@ enum pamhd_status {
 	PAMHD_E_ROE_PRESSURE = 7,  /* Roe intermediate-state pressure */
 	PAMHD_E_SOLVER = 8,        /* unknown solver */
 	PAMHD_E_POISSON = 9,       /* divergence cleaning: tolerance not reached */
+	PAMHD_E_SUBSTEP = 10,      /* substep-min above a cell's allowed level, or a level above 30 */
 	PAMHD_E_ARG = 16,          /* invalid argument */
 	PAMHD_E_UNSUPPORTED = 17,  /* valid request this build does not implement */
 	PAMHD_E_DEVICE = 18,       /* HIP runtime failure (allocation, launch) */
@@ -220,8 +221,18 @@ int pamhd_set_poisson_smoother(pamhd_handle* h, int fused);
    around its owner and return the step length (largest period x substep
    length); substep_max = 0 is the global time step.  FP64 periodic grids;
    set it before the zero-dt final init (the per-face speeds it needs are
-   measured from then on). */
+   measured from then on).  The reference's defaults are substep_min 0 and
+   substep_max 999 (simulation_options.hpp:55-56); a computed level above
+   30, or a cell whose substep_min exceeds its level (the reference aborts,
+   substepping.hpp:376-384), fails the step with PAMHD_E_SUBSTEP. */
 int pamhd_set_substepping(pamhd_handle* h, int substep_min, int substep_max);
+
+/* The same with each cell's own options (n ints each, x fastest), as the
+   reference evaluates string-valued "substep-min" / "substep-max" at the
+   cell centres and the current time before every step
+   (set_minmax_substepping_period(time, ...), substepping.hpp:197-255): the
+   caller sets them before each step.  Copies the arrays (synchronous). */
+int pamhd_set_substepping_cells(pamhd_handle* h, const int* substep_min, const int* substep_max);
 
 /* Largest period (substeps) of the last step: timestep()'s second return
    value (1 without substepping). */

@@ -14,6 +14,7 @@ v[i] * s (multiplication commutes exactly).
 #include "pamhd_oracle.h"
 
 #include <float.h>
+#include <limits.h>
 #include <math.h>
 #ifdef _OPENMP
 #include <omp.h>
@@ -1283,7 +1284,7 @@ static void sub_face_state(const double* S, long n, int d, long c, double U[8])
 }
 
 int or_substep_periods(const or_grid* g, double time_step_factor, double max_dt,
-	int substep_min, int substep_max, const double* maxv, int* period, int* minsub,
+	const int* substep_min, const int* substep_max, const double* maxv, int* period, int* minsub,
 	double* sub_dt_out, int* max_period_out)
 {
 	const dims_t D = {g->nx, g->ny, g->nz, (long)g->nx * g->ny * g->nz};
@@ -1294,9 +1295,13 @@ int or_substep_periods(const or_grid* g, double time_step_factor, double max_dt,
 	int* smaxv = (int*)malloc(sizeof(int) * n);
 	if (!ts || !lomin || !smaxv) { free(ts); free(lomin); free(smaxv); return OR_ERR_SOLVER; }
 
-	/* options: the smallest minimum substep is shifted to 0 */
-	int lo = substep_min;
-	for (long c = 0; c < n; c++) { lomin[c] = substep_min; smaxv[c] = substep_max; }
+	/* options of each cell; the smallest minimum is shifted to 0 */
+	int lo = INT_MAX;
+	for (long c = 0; c < n; c++) {
+		lomin[c] = substep_min[c];
+		smaxv[c] = substep_max[c];
+		if (lomin[c] < lo) lo = lomin[c];
+	}
 	if (lo > 0) for (long c = 0; c < n; c++) lomin[c] -= lo;
 
 	/* minimize_timestep: each cell's limit over its six faces */
@@ -1332,9 +1337,11 @@ int or_substep_periods(const or_grid* g, double time_step_factor, double max_dt,
 				int step = (isfinite(x) && x > 0) ? ilogb(x) : 0;
 				if (step < 0) step = 0;
 				if (smaxv[c] > step) smaxv[c] = step;
-				if (lomin[c] > smaxv[c]) {
+				/* the reference aborts here; a level above 30 would
+				   overflow its int period 1 << level */
+				if (lomin[c] > smaxv[c] || smaxv[c] > 30) {
 					free(ts); free(lomin); free(smaxv);
-					return OR_ERR_SOLVER;
+					return OR_ERR_SUBSTEP;
 				}
 			} else {
 				lomin[c] = smaxv[c] = 0;
@@ -1390,7 +1397,7 @@ int or_substep_periods(const or_grid* g, double time_step_factor, double max_dt,
 }
 
 int or_step_substep(const or_grid* g, int solver, double time_step_factor, double max_dt,
-	int substep_min, int substep_max, double* state, double* maxv, double* dt_out,
+	const int* substep_min, const int* substep_max, double* state, double* maxv, double* dt_out,
 	int* max_period_out, long* bad_face)
 {
 	const dims_t D = {g->nx, g->ny, g->nz, (long)g->nx * g->ny * g->nz};
@@ -1614,31 +1621,36 @@ void or_refit_periodic(const or_grid* g, double* state)
 	}
 }
 
-int or_run_periodic_substep(const or_grid* g, int solver, double cfl, int substep_min,
-	int substep_max, double t_start, double t_end, long max_steps, double* state,
+int or_run_periodic_substep(const or_grid* g, int solver, double cfl, int substep_min_all,
+	int substep_max_all, double t_start, double t_end, long max_steps, double* state,
 	long* steps_taken, double* t_final, int* last_max_period)
 {
 	const long n = (long)g->nx * g->ny * g->nz;
 	double* maxv = (double*)malloc(sizeof(double) * 3 * n);
-	if (!maxv) return OR_ERR_SOLVER;
+	int* substep_min = (int*)malloc(sizeof(int) * n);
+	int* substep_max = (int*)malloc(sizeof(int) * n);
+	if (!maxv || !substep_min || !substep_max) { free(maxv); free(substep_min); free(substep_max); return OR_ERR_SOLVER; }
+	for (long c = 0; c < n; c++) { substep_min[c] = substep_min_all; substep_max[c] = substep_max_all; }
 	for (long f = 0; f < 3 * n; f++) maxv[f] = -1;
 	or_refit_periodic(g, state);
 	/* final init: the speeds are unknown, so dt = 0 and one substep */
 	double dt;
 	int mp = 0;
 	int e = or_step_substep(g, solver, cfl, 0.0, substep_min, substep_max, state, maxv, &dt, &mp, NULL);
-	if (e) { free(maxv); return e; }
 	double t = t_start;
 	long steps = 0;
-	while (t < t_end && steps < max_steps) {
+	while (!e && t < t_end && steps < max_steps) {
 		e = or_step_substep(g, solver, cfl, t_end - t, substep_min, substep_max, state, maxv,
 			&dt, &mp, NULL);
-		if (e) { free(maxv); return e; }
+		if (e) break;
 		or_refit_periodic(g, state);
 		t += dt;
 		steps++;
 	}
 	free(maxv);
+	free(substep_min);
+	free(substep_max);
+	if (e) return e;
 	if (steps_taken) *steps_taken = steps;
 	if (t_final) *t_final = t;
 	if (last_max_period) *last_max_period = mp;

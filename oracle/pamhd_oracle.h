@@ -44,7 +44,8 @@ enum {
 	OR_ERR_FIELD = 5,        /* non-finite total magnetic field */
 	OR_ERR_ROE_DENSITY = 6,  /* Roe intermediate state density */
 	OR_ERR_ROE_PRESSURE = 7, /* Roe intermediate state pressure */
-	OR_ERR_SOLVER = 8        /* unknown solver */
+	OR_ERR_SOLVER = 8,       /* unknown solver */
+	OR_ERR_SUBSTEP = 10      /* a cell's substep-min above its allowed level, or a level above 30 */
 };
 
 /* ---- physics helpers, source/mhd/common.hpp ---- */
@@ -141,15 +142,16 @@ yet known).  or_step_substep advances `state` (face-once layout) by one
 step of *dt_out = max period * substep length, updating maxv; it makes the
 reference's in-step refits (one per update of a cell, one after the face
 sync), not apply_boundaries' (or_refit_periodic).  or_run_periodic_substep
-is or_run_periodic with substepping.  substep_min and substep_max are the
-options' integer values ("substep-min", "substep-max").
+is or_run_periodic with substepping.  substep_min and substep_max hold each
+cell's values of the options "substep-min" and "substep-max" (n ints; the
+run loop takes one value for all cells).
 */
 int or_substep_periods(const or_grid* g, double time_step_factor, double max_dt,
-	int substep_min, int substep_max, const double* maxv, int* period, int* minsub,
-	double* sub_dt, int* max_period);
+	const int* substep_min, const int* substep_max, const double* maxv, int* period,
+	int* minsub, double* sub_dt, int* max_period);
 int or_step_substep(const or_grid* g, int solver, double time_step_factor, double max_dt,
-	int substep_min, int substep_max, double* state, double* maxv, double* dt_out,
-	int* max_period_out, long* bad_face);
+	const int* substep_min, const int* substep_max, double* state, double* maxv,
+	double* dt_out, int* max_period_out, long* bad_face);
 void or_refit_periodic(const or_grid* g, double* state);
 int or_run_periodic_substep(const or_grid* g, int solver, double cfl, int substep_min,
 	int substep_max, double t_start, double t_end, long max_steps, double* state,

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(PKG, "lib", "libpamhd_mi355x.so")
 RUSANOV, RUSANOV_STAGGERED, HLL_ATHENA, HLLD_ATHENA, ROE_ATHENA, HYBRID = range(6)
 FP64, MIXED = 0, 1
 PAMHD_E_POISSON = 9
+PAMHD_E_SUBSTEP = 10
 PAMHD_E_ARG = 16
 PAMHD_E_UNSUPPORTED = 17
 
@@ -30,7 +31,8 @@ EXPORTS = (
     "pamhd_ensemble_check", "pamhd_ensemble_download", "pamhd_ensemble_cells",
     "pamhd_ensemble_set_tile", "pamhd_ensemble_autotune", "pamhd_ensemble_last_error",
     "pamhd_step_fixed", "pamhd_set_state_device", "pamhd_get_state_device",
-    "pamhd_adjoint_step", "pamhd_set_slabs", "pamhd_set_substepping", "pamhd_last_max_substep",
+    "pamhd_adjoint_step", "pamhd_set_slabs", "pamhd_set_substepping", "pamhd_set_substepping_cells",
+    "pamhd_last_max_substep",
 )
 
 
@@ -114,6 +116,7 @@ def load():
     L.pamhd_autotune.argtypes = [H, ctypes.c_int, ctypes.c_char_p, ctypes.c_size_t, _vp]
     L.pamhd_set_launch.argtypes = [H, ctypes.c_int, ctypes.c_int, ctypes.c_int]
     L.pamhd_set_substepping.argtypes = [H, ctypes.c_int, ctypes.c_int]
+    L.pamhd_set_substepping_cells.argtypes = [H, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.pamhd_last_max_substep.argtypes = [H, ctypes.POINTER(ctypes.c_int)]
     L.pamhd_set_slabs.argtypes = [H, ctypes.c_int]
     L.pamhd_enable_timing.argtypes = [H, ctypes.c_int]

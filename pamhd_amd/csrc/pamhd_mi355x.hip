@@ -1046,6 +1046,7 @@ const char* error_name(int e)
 	case PAMHD_E_ROE_DENSITY: return "non-physical density in Roe intermediate state";
 	case PAMHD_E_ROE_PRESSURE: return "non-physical pressure in Roe intermediate state";
 	case PAMHD_E_SOLVER: return "invalid solver";
+	case PAMHD_E_SUBSTEP: return "substep-min above the largest allowed substep level, or a level above 30";
 	default: return "error";
 	}
 }
@@ -1071,12 +1072,20 @@ struct SubstepDev {
 	int* e[2] = {nullptr, nullptr};
 	int* period = nullptr;
 	int* minsub = nullptr;
-	unsigned long long* scal = nullptr;   /* [0] smallest time step (bits), [1] largest period */
+	/* per-cell options: [0, n) substep-min shifted so that its smallest is 0,
+	   [n, 2n) substep-max; null when every cell has substep-min 0 and
+	   substep-max smax */
+	int* opt = nullptr;
+	int* opt_dev = nullptr;
+	/* [0] smallest time step, [1] largest period, [2] substep length
+	   (smallest time step / 2^substep-min), [3] largest level; the
+	   non-negative doubles are kept as their bits */
+	unsigned long long* scal = nullptr;
 	double* sub_dt = nullptr;
 	StepStatus* status = nullptr;
-	unsigned long long* host = nullptr;   /* pinned: [0] largest period, [1] substep length (bits) */
+	unsigned long long* host = nullptr;   /* pinned: [0] largest period, [1] substep length (bits), [2] largest level */
 	StepStatus* host_status = nullptr;
-	int smin = 0, smax = 0;
+	int smax = 0;   /* uniform substep-max, or the largest per-cell one */
 	int last_max = 1;
 };
 
@@ -1106,17 +1115,20 @@ __global__ void ss_clear_kernel(SubstepDev d)
 	if (threadIdx.x == 0) {
 		d.scal[0] = (unsigned long long)__double_as_longlong(DBL_MAX);
 		d.scal[1] = 0;
+		d.scal[2] = (unsigned long long)__double_as_longlong(DBL_MAX);
+		d.scal[3] = 0;
 		d.status->err = 0;
 		d.status->err_face = -1;
 	}
 }
 
 /* minimize_timestep, source/mhd/solve.hpp:1927-2014: a cell's limit over its
-   six faces, and the smallest over the grid */
+   six faces, the smallest over the grid and the smallest of time step /
+   2^substep-min (set_minmax_substepping_period, substepping.hpp:333-342) */
 __global__ __launch_bounds__(BX * BY) void ss_ts_kernel(GridDev g, SubstepDev d, double cfl)
 {
 	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
-	double t = DBL_MAX;
+	double t = DBL_MAX, r = DBL_MAX;
 	if (i < g.nx && j < g.ny) {
 		const int c = cidx(g, i, j, k);
 		const double len[3] = {g.dx, g.dy, g.dz};
@@ -1127,34 +1139,52 @@ __global__ __launch_bounds__(BX * BY) void ss_ts_kernel(GridDev g, SubstepDev d,
 			for (int sd = 0; sd < 2; sd++) t = v[sd] <= 0 ? 0.0 : smin(cfl * len[q] / v[sd], t);
 		}
 		d.ts[c] = t;
+		r = d.opt ? t / (double)(1 << d.opt[c]) : t;
 	}
-	for (int off = 32; off > 0; off >>= 1) t = smin(t, __shfl_xor(t, off, 64));
+	for (int off = 32; off > 0; off >>= 1) {
+		t = smin(t, __shfl_xor(t, off, 64));
+		r = smin(r, __shfl_xor(r, off, 64));
+	}
 	if ((threadIdx.x + BX * threadIdx.y) % 64 == 0) {
 		atomicMin(&d.scal[0], (unsigned long long)__double_as_longlong(t));
+		atomicMin(&d.scal[2], (unsigned long long)__double_as_longlong(r));
 	}
 }
 
-/* set_minmax_substepping_period, source/substepping.hpp:264-399, with
-   uniform options (the smallest minimum is shifted to 0): the substep
-   length and each cell's largest level.  floor(log2 x) is exact (ilogb);
-   a non-finite ratio gives level 0. */
+/* set_minmax_substepping_period, source/substepping.hpp:264-399: the
+   substep length and each cell's largest level.  floor(log2 x) is exact
+   (ilogb); a non-finite ratio gives level 0.  A cell whose substep-min
+   exceeds its level (the reference aborts) or whose level exceeds 30 (its
+   int period would overflow) reports PAMHD_E_SUBSTEP. */
 __global__ __launch_bounds__(BX * BY) void ss_level_kernel(GridDev g, SubstepDev d, double max_dt)
 {
 	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
 	const double smallest = __longlong_as_double((long long)d.scal[0]);
+	const double ret = __longlong_as_double((long long)d.scal[2]);
 	const bool all_zero = max_dt <= smallest;
-	if (i == 0 && j == 0 && k == 0) *d.sub_dt = all_zero ? max_dt : smallest;
-	if (i >= g.nx || j >= g.ny) return;
-	const int c = cidx(g, i, j, k);
+	if (i == 0 && j == 0 && k == 0) *d.sub_dt = all_zero ? max_dt : ret;
 	int lev = 0;
-	const double t = d.ts[c];
-	if (!all_zero && t > 0) {
-		const double x = smin(t, max_dt) / smallest;
-		int step = (__builtin_isfinite(x) && x > 0) ? ilogb(x) : 0;
-		if (step < 0) step = 0;
-		lev = d.smax > step ? step : d.smax;
+	if (i < g.nx && j < g.ny) {
+		const int c = cidx(g, i, j, k);
+		const double t = d.ts[c];
+		if (!all_zero && t > 0) {
+			const double x = smin(t, max_dt) / ret;
+			int step = (__builtin_isfinite(x) && x > 0) ? ilogb(x) : 0;
+			if (step < 0) step = 0;
+			const int cap = d.opt ? d.opt[g.n + c] : d.smax;
+			lev = cap > step ? step : cap;
+			if ((d.opt && d.opt[c] > lev) || lev > 30) {
+				record_error(d.status, PAMHD_E_SUBSTEP, c);
+				lev = 0;
+			}
+		}
+		d.e[0][c] = lev;
 	}
-	d.e[0][c] = lev;
+	for (int off = 32; off > 0; off >>= 1) {
+		const int o = __shfl_xor(lev, off, 64);
+		lev = o > lev ? o : lev;
+	}
+	if ((threadIdx.x + BX * threadIdx.y) % 64 == 0) atomicMax(&d.scal[3], (unsigned long long)lev);
 }
 
 /* one Jacobi pass of restrict_substepping_period, substepping.hpp:99-191 */
@@ -1601,6 +1631,13 @@ int report_status(pamhd_handle* h, const StepStatus& st)
 	if (st.err == 0) return PAMHD_OK;
 	const long long n = h->g.n;
 	const long long face = st.err_face;
+	if (st.err == PAMHD_E_SUBSTEP) {
+		const int i = (int)(face % h->g.nx), j = (int)((face / h->g.nx) % h->g.ny),
+			k = (int)(face / ((long long)h->g.nx * h->g.ny));
+		char buf[256];
+		snprintf(buf, sizeof(buf), "cell (%d,%d,%d): %s", i, j, k, error_name(st.err));
+		return fail(h, st.err, buf);
+	}
 	const int d = (int)(face / n);
 	const long long c = face - (long long)d * n;
 	const int i = (int)(c % h->g.nx), j = (int)((c / h->g.nx) % h->g.ny), k = (int)(c / ((long long)h->g.nx * h->g.ny));
@@ -1676,6 +1713,7 @@ void substep_free(SubstepDev* d)
 	hipFree(d->e[1]);
 	hipFree(d->period);
 	hipFree(d->minsub);
+	hipFree(d->opt_dev);
 	hipFree(d->scal);
 	hipFree(d->sub_dt);
 	hipFree(d->status);
@@ -1701,10 +1739,11 @@ int substep_alloc(pamhd_handle* h)
 	al(&d->e[1], n * sizeof(int));
 	al(&d->period, n * sizeof(int));
 	al(&d->minsub, n * sizeof(int));
-	al(&d->scal, 2 * sizeof(unsigned long long));
+	al(&d->opt_dev, 2 * n * sizeof(int));
+	al(&d->scal, 4 * sizeof(unsigned long long));
 	al(&d->sub_dt, sizeof(double));
 	al(&d->status, sizeof(StepStatus));
-	if (e == hipSuccess) e = hipHostMalloc((void**)&d->host, 2 * sizeof(unsigned long long));
+	if (e == hipSuccess) e = hipHostMalloc((void**)&d->host, 3 * sizeof(unsigned long long));
 	if (e == hipSuccess) e = hipHostMalloc((void**)&d->host_status, sizeof(StepStatus));
 	if (e != hipSuccess) {
 		substep_free(d);
@@ -1742,8 +1781,17 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 	ss_clear_kernel<<<1, 64, 0, st>>>(d);
 	ss_ts_kernel<<<grid, block, 0, st>>>(h->g, d, cfl);
 	ss_level_kernel<<<grid, block, 0, st>>>(h->g, d, max_dt);
+	HIP_TRY(h, hipGetLastError());
+	HIP_TRY(h, hipMemcpyAsync(d.host + 2, d.scal + 3, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+	HIP_TRY(h, hipMemcpyAsync(d.host_status, d.status, sizeof(StepStatus), hipMemcpyDeviceToHost, st));
+	HIP_TRY(h, hipStreamSynchronize(st));
+	if (d.host_status->err) return report_status(h, *d.host_status);
+	/* Jacobi passes of the restriction: a cell's restricted level is the
+	   smallest over cells m of level(m) + distance, reached within as many
+	   passes as the largest level */
+	const int passes = (int)d.host[2];
 	int cur = 0;
-	for (int r = 0; r < d.smax; r++) {
+	for (int r = 0; r < passes; r++) {
 		ss_restrict_kernel<<<grid, block, 0, st>>>(h->g, d.e[cur], d.e[1 - cur]);
 		cur = 1 - cur;
 	}
@@ -2197,7 +2245,7 @@ int pamhd_set_substepping(pamhd_handle* h, int substep_min, int substep_max)
 {
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "substepping needs a periodic grid");
-	if (substep_max < 0 || substep_max > 20) return fail(h, PAMHD_E_ARG, "substep_max must be in [0, 20]");
+	if (substep_max < 0) return fail(h, PAMHD_E_ARG, "substep_max must be >= 0");
 	if (substep_min < 0) return fail(h, PAMHD_E_ARG, "substep_min must be >= 0");
 	if (substep_max > 0 && h->precision != PAMHD_FP64) {
 		return fail(h, PAMHD_E_UNSUPPORTED, "substepping runs in FP64 only");
@@ -2207,9 +2255,47 @@ int pamhd_set_substepping(pamhd_handle* h, int substep_min, int substep_max)
 		if (e) return e;
 	}
 	if (h->sub) {
-		h->sub->smin = substep_min;
+		/* the same minimum in every cell is shifted to 0 */
+		h->sub->opt = nullptr;
 		h->sub->smax = substep_max;
 	}
+	return PAMHD_OK;
+}
+
+int pamhd_set_substepping_cells(pamhd_handle* h, const int* substep_min, const int* substep_max)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (!substep_min || !substep_max) return fail(h, PAMHD_E_ARG, "null argument");
+	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "substepping needs a periodic grid");
+	const long long n = h->g.n;
+	int lo = INT_MAX, hi = 0;
+	for (long long c = 0; c < n; c++) {
+		if (substep_min[c] < 0 || substep_max[c] < 0) {
+			return fail(h, PAMHD_E_ARG, "substep options must be >= 0 (cell " + std::to_string(c) + ")");
+		}
+		lo = substep_min[c] < lo ? substep_min[c] : lo;
+		hi = substep_max[c] > hi ? substep_max[c] : hi;
+	}
+	if (hi > 0 && h->precision != PAMHD_FP64) {
+		return fail(h, PAMHD_E_UNSUPPORTED, "substepping runs in FP64 only");
+	}
+	if (hi == 0) {
+		/* every period is 1: the global step */
+		if (h->sub) { h->sub->opt = nullptr; h->sub->smax = 0; }
+		return PAMHD_OK;
+	}
+	if (!h->sub) {
+		const int e = substep_alloc(h);
+		if (e) return e;
+	}
+	std::vector<int> opt((size_t)(2 * n));
+	for (long long c = 0; c < n; c++) {
+		opt[(size_t)c] = substep_min[c] - lo;
+		opt[(size_t)(n + c)] = substep_max[c];
+	}
+	HIP_TRY(h, hipMemcpy(h->sub->opt_dev, opt.data(), opt.size() * sizeof(int), hipMemcpyHostToDevice));
+	h->sub->opt = h->sub->opt_dev;
+	h->sub->smax = hi;
 	return PAMHD_OK;
 }
 

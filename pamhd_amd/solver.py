@@ -153,9 +153,19 @@ class Grid:
         return buf.value.decode()
 
     def set_substepping(self, substep_min=0, substep_max=0):
-        """Per-cell 2^N substepping ("substep-min", "substep-max"); 0 = global
+        """Per-cell 2^N substepping ("substep-min", "substep-max"): ints for
+        every cell, or arrays of each cell's values (x fastest, set again
+        before each step when they depend on time); substep_max 0 = global
         time step.  Set before the final-init timestep."""
-        self._check(self._L.pamhd_set_substepping(self._h, substep_min, substep_max))
+        if np.ndim(substep_min) == 0 and np.ndim(substep_max) == 0:
+            self._check(self._L.pamhd_set_substepping(self._h, int(substep_min), int(substep_max)))
+            return
+        n = self.n
+        lo = np.ascontiguousarray(np.broadcast_to(substep_min, (n,)), dtype=np.int32)
+        hi = np.ascontiguousarray(np.broadcast_to(substep_max, (n,)), dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        self._check(self._L.pamhd_set_substepping_cells(self._h, lo.ctypes.data_as(ip),
+                                                        hi.ctypes.data_as(ip)))
 
     def last_max_substep(self):
         """Largest period (in substeps) of the last step."""

@@ -16,6 +16,7 @@ ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
 REF_SO = os.path.join(ORACLE_DIR, "_ref", "libref_flux.so")
 
 RUSANOV, RUSANOV_STAGGERED, HLL, HLLD, ROE, HYBRID = range(6)
+SUBSTEP_ERROR = 10   # OR_ERR_SUBSTEP
 SOLVERS = {"rusanov": RUSANOV, "hll_athena": HLL, "hlld_athena": HLLD,
            "roe_athena": ROE}
 
@@ -95,11 +96,10 @@ def lib():
     L.or_clean_divergence.restype = ctypes.c_int
     _ip = ctypes.POINTER(ctypes.c_int)
     L.or_substep_periods.argtypes = [ctypes.POINTER(OrGrid), ctypes.c_double, ctypes.c_double,
-                                     ctypes.c_int, ctypes.c_int, _dp, _ip, _ip, _dp, _ip]
+                                     _ip, _ip, _dp, _ip, _ip, _dp, _ip]
     L.or_substep_periods.restype = ctypes.c_int
     L.or_step_substep.argtypes = [ctypes.POINTER(OrGrid), ctypes.c_int, ctypes.c_double,
-                                  ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp, _dp,
-                                  _ip, _lp]
+                                  ctypes.c_double, _ip, _ip, _dp, _dp, _dp, _ip, _lp]
     L.or_step_substep.restype = ctypes.c_int
     L.or_refit_periodic.argtypes = [ctypes.POINTER(OrGrid), _dp]
     L.or_refit_periodic.restype = None
@@ -193,16 +193,27 @@ def run_periodic(grid, solver, cfl, t_start, t_end, state, max_steps=1 << 40):
     return st, steps.value, tf.value
 
 
-def substep_periods(grid, cfl, max_dt, substep_min, substep_max, maxv):
-    """(status, period (n,), minsub (n,), substep length, largest period)."""
+def _substep_options(grid, substep_min, substep_max):
+    """each cell's substep-min and substep-max (scalars apply to every cell)"""
     n = grid.nx * grid.ny * grid.nz
+    lo = np.ascontiguousarray(np.broadcast_to(substep_min, (n,)), dtype=np.int32)
+    hi = np.ascontiguousarray(np.broadcast_to(substep_max, (n,)), dtype=np.int32)
+    ip = ctypes.POINTER(ctypes.c_int)
+    return lo, hi, lo.ctypes.data_as(ip), hi.ctypes.data_as(ip)
+
+
+def substep_periods(grid, cfl, max_dt, substep_min, substep_max, maxv):
+    """(status, period (n,), minsub (n,), substep length, largest period);
+    substep_min / substep_max are ints or per-cell arrays."""
+    n = grid.nx * grid.ny * grid.nz
+    lo, hi, lo_p, hi_p = _substep_options(grid, substep_min, substep_max)
     maxv = np.ascontiguousarray(maxv, dtype=np.float64)
     period = np.zeros(n, dtype=np.int32)
     minsub = np.zeros(n, dtype=np.int32)
     sub_dt = ctypes.c_double(0)
     mp = ctypes.c_int(0)
     ip = ctypes.POINTER(ctypes.c_int)
-    st = lib().or_substep_periods(ctypes.byref(grid), cfl, max_dt, substep_min, substep_max,
+    st = lib().or_substep_periods(ctypes.byref(grid), cfl, max_dt, lo_p, hi_p,
                                   _ptr(maxv), period.ctypes.data_as(ip), minsub.ctypes.data_as(ip),
                                   ctypes.byref(sub_dt), ctypes.byref(mp))
     return st, period, minsub, sub_dt.value, mp.value
@@ -210,13 +221,15 @@ def substep_periods(grid, cfl, max_dt, substep_min, substep_max, maxv):
 
 def step_substep(grid, solver, cfl, max_dt, substep_min, substep_max, state, maxv):
     """One substepped step in place on state (8, n) and maxv (3, n); returns
-    (status, dt, largest period)."""
+    (status, dt, largest period).  substep_min / substep_max are ints or
+    per-cell arrays."""
     assert state.flags.c_contiguous and state.dtype == np.float64
     assert maxv.flags.c_contiguous and maxv.dtype == np.float64
+    lo, hi, lo_p, hi_p = _substep_options(grid, substep_min, substep_max)
     dt = ctypes.c_double(0)
     mp = ctypes.c_int(0)
     bad = ctypes.c_long(-1)
-    st = lib().or_step_substep(ctypes.byref(grid), solver, cfl, max_dt, substep_min, substep_max,
+    st = lib().or_step_substep(ctypes.byref(grid), solver, cfl, max_dt, lo_p, hi_p,
                                _ptr(state), _ptr(maxv), ctypes.byref(dt), ctypes.byref(mp),
                                ctypes.byref(bad))
     return st, dt.value, mp.value

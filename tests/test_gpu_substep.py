@@ -57,3 +57,91 @@ def test_substepping_needs_fp64(gpu):
         with pytest.raises(_lib.PamhdError) as e:
             g.set_substepping(0, 2)
         assert e.value.code == _lib.PAMHD_E_UNSUPPORTED
+
+
+def _cell_coordinates(cells):
+    nx, ny, nz = cells
+    x = (np.arange(nx) + 0.5) / nx - 0.5
+    y = (np.arange(ny) + 0.5) / ny - 0.5
+    z = (np.arange(nz) + 0.5) / nz - 0.5
+    zz, yy, xx = np.meshgrid(z, y, x, indexing="ij")
+    return xx.reshape(-1), np.sqrt(xx * xx + yy * yy + zz * zz).reshape(-1)
+
+
+@pytest.mark.parametrize("name,sid", [("hll-athena", O.HLL), ("hlld-athena", O.HLLD)])
+def test_per_cell_time_dependent_options_bit_exact(gpu, name, sid):
+    """string-valued substep-min / substep-max: each cell's options, set
+    again before every step from the cell centre and the time, as the main
+    loop of tests/mhd/test.cpp evaluates them"""
+    from pamhd_amd.solver import Grid
+    cells = (16, 12, 10)
+    state, length, gamma, mu0 = configs.hot_blob(cells, contrast=30.0, radius=0.25, seed=3)
+    x, r = _cell_coordinates(cells)
+
+    def options(t, step):
+        lo = np.where(r > 0.4, 3, 2).astype(np.int32)
+        hi = np.where(x < -0.5 + 0.25 * step, 1, 3).astype(np.int32)
+        return lo, hi
+
+    g_o = O.make_grid(cells, length, gamma, mu0)
+    ref = state.copy()
+    maxv = -np.ones((3, ref.shape[1]))
+    lo, hi = options(0.0, 0)
+    st, dt0, _ = O.step_substep(g_o, sid, 0.3, 0.0, lo, hi, ref, maxv)
+    assert st == 0
+    with Grid(cells, length, gamma, mu0) as g:
+        g.upload(state)
+        g.set_substepping(lo, hi)
+        assert g.timestep(name, 0.0, 0.3) == dt0 == 0.0
+        t_o = t_g = 0.0
+        periods = []
+        for step in range(1, 6):
+            lo, hi = options(t_o, step)
+            st, dt_o, mp_o = O.step_substep(g_o, sid, 0.3, 1e9 - t_o, lo, hi, ref, maxv)
+            assert st == 0
+            O.refit_periodic(g_o, ref)
+            g.set_substepping(lo, hi)
+            dt_g = g.step(name, 1e9 - t_g, 0.3)
+            assert dt_g == dt_o and g.last_max_substep() == mp_o
+            t_o += dt_o
+            t_g += dt_g
+            periods.append(mp_o)
+        out = g.download()
+    print(name, "largest periods", periods, flush=True)
+    assert max(periods) > 1
+    np.testing.assert_array_equal(out, ref)
+
+
+def test_reference_default_options_bit_exact(gpu):
+    """substep-min 0, substep-max 999 (the options' defaults): levels are
+    bounded by the time-step ratios alone"""
+    from pamhd_amd.solver import Grid
+    cells = (16, 12, 10)
+    state, length, gamma, mu0 = configs.hot_blob(cells, contrast=30.0, radius=0.25, seed=3)
+    g_o = O.make_grid(cells, length, gamma, mu0)
+    ref = state.copy()
+    st, steps_o, t_o, mp_o = O.run_periodic_substep(g_o, O.HLL, 0.3, 0, 999, 0.0, 1e9, ref, max_steps=4)
+    assert st == 0 and mp_o > 1
+    with Grid(cells, length, gamma, mu0) as g:
+        g.upload(state)
+        g.set_substepping(0, 999)
+        steps_g, t_g = g.run("hll-athena", 0.3, 0.0, 1e9, max_steps=4)
+        assert g.last_max_substep() == mp_o and t_g == t_o
+        out = g.download()
+    np.testing.assert_array_equal(out, ref)
+
+
+def test_substep_min_above_the_allowed_level_fails_the_step(gpu):
+    from pamhd_amd import _lib
+    from pamhd_amd.solver import Grid
+    cells = (8, 8, 8)
+    state, length, gamma, mu0 = configs.hot_blob(cells, contrast=30.0, radius=0.2, seed=2)
+    _, r = _cell_coordinates(cells)
+    lo = np.where(r < 0.15, 2, 0).astype(np.int32)
+    with Grid(cells, length, gamma, mu0) as g:
+        g.upload(state)
+        g.set_substepping(lo, 1)
+        g.timestep("hll-athena", 0.0, 0.3)
+        with pytest.raises(_lib.PamhdError) as e:
+            g.step("hll-athena", 1e9, 0.3)
+        assert e.value.code == _lib.PAMHD_E_SUBSTEP

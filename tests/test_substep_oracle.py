@@ -145,3 +145,62 @@ def test_translation_invariance_and_time_accounting():
     assert mp > 1
     back = np.roll(b.reshape(8, cells[2], cells[1], cells[0]), -shift, 3).reshape(8, -1)
     np.testing.assert_allclose(back, a, rtol=1e-13, atol=1e-15)
+
+
+def _cell_radius(cells):
+    nx, ny, nz = cells
+    x = (np.arange(nx) + 0.5) / nx - 0.5
+    y = (np.arange(ny) + 0.5) / ny - 0.5
+    z = (np.arange(nz) + 0.5) / nz - 0.5
+    zz, yy, xx = np.meshgrid(z, y, x, indexing="ij")
+    return np.sqrt(xx * xx + yy * yy + zz * zz).reshape(-1), xx.reshape(-1)
+
+
+def test_per_cell_options_match_a_brute_force_restriction():
+    """Each cell's own substep-min / substep-max (the string-valued options):
+    the minimum shifted so that its smallest is 0, the substep length as the
+    smallest time step / 2^substep-min, each cell's level capped by its own
+    substep-max."""
+    cells = (8, 8, 8)
+    state, length, gamma, mu0 = configs.hot_blob(cells, contrast=30.0, radius=0.2, seed=2)
+    g = O.make_grid(cells, length, gamma, mu0)
+    maxv = _maxv_after_steps(g, state, O.HLL)
+    r, x = _cell_radius(cells)
+    lo = np.where(r > 0.35, 3, 2).astype(np.int32)     # shifted to 0 / 1
+    hi = np.where(x < 0, 4, 2).astype(np.int32)
+    cfl, max_dt = 0.3, 1e9
+    st, period, minsub, sub_dt, mp = O.substep_periods(g, cfl, max_dt, lo, hi, maxv)
+    assert st == 0
+    nx, ny, nz = cells
+    f = maxv.reshape(3, nz, ny, nx)
+    ts = np.full((nz, ny, nx), np.finfo(float).max)
+    for d in range(3):
+        for v in (f[d], np.roll(f[d], -1, 2 - d)):
+            ts = np.minimum(ts, np.where(v > 0, cfl * length[d] / np.where(v > 0, v, 1), 0.0))
+    ts = ts.reshape(-1)
+    lmin = lo - lo.min()
+    ret = (ts / 2.0 ** lmin).min()
+    assert sub_dt == ret and ret < ts.min()
+    _, ex = np.frexp(np.minimum(ts, max_dt) / ret)
+    step = np.minimum(hi, np.maximum(0, ex - 1))
+    assert np.all(lmin <= step)
+    nbrs = _neighbourhood(cells)
+    e = step.copy()
+    for _ in range(int(step.max()) + 1):   # min-plus closure by relaxation
+        for nb in nbrs:
+            e = np.minimum(e, e[nb] + 1)
+    np.testing.assert_array_equal(period, 2 ** e)
+    assert mp == period.max() and mp >= 4
+
+
+def test_substep_min_above_the_allowed_level_is_an_error():
+    cells = (8, 8, 8)
+    state, length, gamma, mu0 = configs.hot_blob(cells, contrast=30.0, radius=0.2, seed=2)
+    g = O.make_grid(cells, length, gamma, mu0)
+    maxv = _maxv_after_steps(g, state, O.HLL)
+    r, _ = _cell_radius(cells)
+    lo = np.where(r < 0.15, 2, 0).astype(np.int32)
+    st, *_ = O.substep_periods(g, 0.3, 1e9, lo, 1, maxv)
+    assert st == O.SUBSTEP_ERROR
+    st, *_ = O.substep_periods(g, 0.3, 1e9, lo, 999, maxv)
+    assert st == 0
