@@ -419,3 +419,33 @@ class Box:
         st = _box_lib().or_box_run(ctypes.byref(self.b), solver, cfl, t_start, t_end, max_steps,
                                    _ptr(state), ctypes.byref(steps), ctypes.byref(tf))
         return st, steps.value, tf.value
+
+
+SOLVER_IDS = {"rusanov": RUSANOV, "hll-athena": HLL, "hlld-athena": HLLD, "roe-athena": ROE}
+
+
+def run_config(cfg, state, bg=None, max_steps=1 << 40):
+    """The program loop of tests/mhd/test.cpp for a loaded configuration
+    (pamhd_amd.config.Config) in the oracle, in place on state: the final
+    init, then substepped steps with the options evaluated at each step's
+    start time, each followed by apply_boundaries' refit.  Returns
+    (status, steps, end time, the largest period of every step)."""
+    g = make_grid(cfg.cells, cfg.cell_length, cfg.adiabatic_index, cfg.vacuum_permeability, bg)
+    sid = SOLVER_IDS[cfg.solver]
+    maxv = -np.ones((3, state.shape[1]))
+    t = cfg.time_start
+    t_end = cfg.time_start + cfg.time_length
+    lo, hi = cfg.substep_options(t)
+    st, dt, mp = step_substep(g, sid, cfg.time_step_factor, 0.0, lo, hi, state, maxv)
+    steps = 0
+    periods = []
+    while st == 0 and t < t_end and steps < max_steps:
+        lo, hi = cfg.substep_options(t)
+        st, dt, mp = step_substep(g, sid, cfg.time_step_factor, t_end - t, lo, hi, state, maxv)
+        if st:
+            break
+        refit_periodic(g, state)
+        t += dt
+        steps += 1
+        periods.append(mp)
+    return st, steps, t, periods

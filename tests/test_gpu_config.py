@@ -1,0 +1,74 @@
+"""The reference's JSON configurations run on the GPU through the loader
+(pamhd_amd.config.run: test.cpp's program loop over the C ABI) against the
+same configurations in the oracle (oracle_lib.run_config), bit for bit, and
+the golden ones against the reference's .dc results."""
+import os
+
+import numpy as np
+import pytest
+
+import golden_cases as G
+import oracle_lib as O
+from pamhd_amd import config, dc
+
+pytestmark = pytest.mark.gpu
+
+CFG_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config")
+CONFIGS = sorted(f[:-5] for f in os.listdir(CFG_DIR) if not f.startswith("two_pop"))
+# whole runs, but at most this many steps of the 3-D Kelvin-Helmholtz case
+MAX_STEPS = {"kelvin_helmholtz_3d_roe_athena": 60}
+
+
+@pytest.mark.parametrize("name", CONFIGS)
+def test_config_runs_bit_exact(gpu, name):
+    cfg = config.load(os.path.join(CFG_DIR, name + ".json"))
+    limit = MAX_STEPS.get(name, 1 << 40)
+    state, bg = cfg.initial_state()
+    st, steps_o, t_o, periods = O.run_config(cfg, state, bg, max_steps=limit)
+    assert st == 0
+    res = config.run(cfg, max_steps=limit)
+    print(name, cfg.cells, cfg.solver, "steps", res["steps"], "largest period", max(periods),
+          flush=True)
+    assert res["steps"] == steps_o and res["time"] == t_o
+    assert res["max_substep"] == periods[-1]
+    np.testing.assert_array_equal(res["state"], state)
+    if name in G.GOLDENS:
+        meta, ids, mhd = G.read_dc(os.path.join(G.DC_DIR, name + ".dc"))
+        assert res["steps"] == meta["sim_step"]
+        gold = G.dc_to_grid(meta, ids, mhd)
+        out = res["state"]
+        mine = np.concatenate([out[:5], G.volume_field(cfg.cells, out)])
+        for k, (viol, rel) in G.mhd_diff(mine, gold).items():
+            assert viol <= 1e-15, (name, k, viol, rel)
+
+
+def test_time_dependent_substep_options(gpu):
+    """substep-max as an expression of t, set again before every step"""
+    import json
+    obj = json.load(open(os.path.join(CFG_DIR, "substep_x1.json")))
+    obj["substep-min"] = 0
+    obj["substep-max"] = "t < 0.3 ? (x < 0.5 ? 0 : 1) : 2"
+    obj["time-length"] = 0.6
+    cfg = config.load(obj)
+    assert cfg.substeps_depend_on_time()
+    state, bg = cfg.initial_state()
+    st, steps_o, t_o, periods = O.run_config(cfg, state, bg)
+    assert st == 0 and 2 in periods and max(periods) > 2
+    res = config.run(cfg)
+    assert res["steps"] == steps_o and res["time"] == t_o
+    np.testing.assert_array_equal(res["state"], state)
+
+
+def test_saves_follow_save_mhd_n(gpu, tmp_path):
+    """save-mhd-n 0: the state after the final init (step 0) and at the end,
+    as .dc files that read back to the run's state"""
+    cfg = config.load(os.path.join(CFG_DIR, "alfven_2d_x_1.json"))
+    res = config.run(cfg, output_directory=str(tmp_path))
+    names = sorted(os.path.basename(p) for p in res["saves"])
+    assert names == ["mhd_000000000.dc", "mhd_%09d.dc" % res["steps"]]
+    meta, ids, rows = dc.read(res["saves"][-1])
+    assert meta["sim_step"] == res["steps"] and meta["sim_time"] == res["time"]
+    mhd_rows, _ = dc.cells_of_grid_state(res["state"], cfg.cells)
+    np.testing.assert_array_equal(rows["mhd     "], mhd_rows)
+    fast = config.run(cfg)
+    np.testing.assert_array_equal(fast["state"], res["state"])
