@@ -634,11 +634,9 @@ def run(cfg, precision="fp64", output_directory=None, max_steps=1 << 62, grid=No
             saves.append(path)
 
         per_step_options = precision == "fp64" and cfg.substeps_depend_on_time()
-        if output_directory is None and not per_step_options:
-            # nothing to do between steps: the library's own loop
-            steps, t = g.run(cfg.solver, cfg.time_step_factor, t, t_end, max_steps=max_steps)
-            return dict(steps=steps, time=t, state=g.download(), saves=saves,
-                        max_substep=g.last_max_substep())
+        # the state already holds apply_boundaries' refit of the
+        # initialisation, so the loop starts at the final init (pamhd_run
+        # would refit once more)
         g.timestep(cfg.solver, 0.0, cfg.time_step_factor)
         step = 0
         if cfg.save_n >= 0:

@@ -45,15 +45,15 @@ def test_config_runs_bit_exact(gpu, name):
 def test_time_dependent_substep_options(gpu):
     """substep-max as an expression of t, set again before every step"""
     import json
-    obj = json.load(open(os.path.join(CFG_DIR, "substep_x1.json")))
-    obj["substep-min"] = 0
-    obj["substep-max"] = "t < 0.3 ? (x < 0.5 ? 0 : 1) : 2"
-    obj["time-length"] = 0.6
+    obj = json.load(open(os.path.join(CFG_DIR, "blast_wave_2d_hll_athena.json")))
+    obj["substep-max"] = "t < 0.2 ? (x < 0 ? 1 : 0) : 3"
+    obj["time-length"] = 0.5
     cfg = config.load(obj)
     assert cfg.substeps_depend_on_time()
     state, bg = cfg.initial_state()
     st, steps_o, t_o, periods = O.run_config(cfg, state, bg)
-    assert st == 0 and 2 in periods and max(periods) > 2
+    # periods of at most 2 while substep-max is 1, then up to 4
+    assert st == 0 and periods[0] == 2 and max(periods) == 4
     res = config.run(cfg)
     assert res["steps"] == steps_o and res["time"] == t_o
     np.testing.assert_array_equal(res["state"], state)
