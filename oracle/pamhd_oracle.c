@@ -2636,3 +2636,709 @@ int or_box_run(const or_box* b, int solver, double cfl, double t_start, double t
 	if (t_final) *t_final = t;
 	return e;
 }
+
+/* ======================================================================
+   Uniform grids with value and copy boundaries (declarations and the
+   layout in pamhd_oracle.h)
+   ====================================================================== */
+
+typedef struct { long id; int fn; int en[3]; } bnb_t;
+
+#define BS(q, c) S[(long)(q) * n + (c)]
+#define BF(f, c) S[(long)(8 + (f)) * n + (c)]
+#define FI(f, c) finfo[(long)(f) * n + (c)]
+
+static inline long bgrid_n(const or_bgrid* g) { return (long)g->nx * g->ny * g->nz; }
+
+/* face index (-x, +x, -y, +y, -z, +z) of a direction -3..-1, +1..+3 */
+static inline int fidx(int dir) { return 2 * ((dir < 0 ? -dir : dir) - 1) + (dir > 0); }
+
+/* neighbours of cell (i, j, k): offsets -1..1 with z slowest and x fastest,
+   periodic dimensions wrapped, closed ones cut; fn = +-(d + 1) for a face
+   neighbour, en = (edge dimension, offset sign in the lower other
+   dimension, in the higher one) for an edge neighbour (en[0] = -1 else) */
+static int bnb_list(const or_bgrid* g, long i, long j, long k, bnb_t* out)
+{
+	const long N[3] = {g->nx, g->ny, g->nz}, idx[3] = {i, j, k};
+	int m = 0;
+	for (int oz = -1; oz <= 1; oz++)
+	for (int oy = -1; oy <= 1; oy++)
+	for (int ox = -1; ox <= 1; ox++) {
+		const int o[3] = {ox, oy, oz};
+		const int nzc = (ox != 0) + (oy != 0) + (oz != 0);
+		if (nzc == 0) continue;
+		long p[3];
+		int ok = 1;
+		for (int d = 0; d < 3 && ok; d++) {
+			long v = idx[d] + o[d];
+			if (v < 0 || v >= N[d]) {
+				if (!g->periodic[d]) ok = 0;
+				else v = (v + N[d]) % N[d];
+			}
+			p[d] = v;
+		}
+		if (!ok) continue;
+		bnb_t* b = out + m++;
+		b->id = p[0] + N[0] * (p[1] + N[1] * p[2]);
+		b->fn = 0;
+		b->en[0] = -1;
+		b->en[1] = b->en[2] = 0;
+		if (nzc == 1) {
+			for (int d = 0; d < 3; d++) if (o[d]) b->fn = o[d] * (d + 1);
+		} else if (nzc == 2) {
+			const int e = !ox ? 0 : (!oy ? 1 : 2);
+			const int d1 = e == 0 ? 1 : 0, d2 = e == 2 ? 1 : 2;
+			b->en[0] = e;
+			b->en[1] = o[d1];
+			b->en[2] = o[d2];
+		}
+	}
+	return m;
+}
+
+static inline void bcell_ijk(const or_bgrid* g, long c, long* i, long* j, long* k)
+{
+	*i = c % g->nx;
+	*j = (c / g->nx) % g->ny;
+	*k = c / ((long)g->nx * g->ny);
+}
+
+int or_bnd_classify(const or_bgrid* g, const unsigned char* in_value,
+	const unsigned char* in_copy, signed char* vtype, signed char* ctype, signed char* finfo)
+{
+	const long n = bgrid_n(g);
+	bnb_t nb[26];
+	long i, j, k;
+	for (int v = 0; v < 4; v++) {
+		signed char* t = vtype + (long)v * n;
+		const unsigned char* iv = in_value + (long)v * n;
+		const unsigned char* ic = in_copy + (long)v * n;
+		for (long c = 0; c < n; c++) t[c] = iv[c] ? 3 : (ic[c] ? 4 : 1);
+		/* boundary cells without a normal neighbour (the cell itself aside)
+		   are not solved */
+		for (long c = 0; c < n; c++) {
+			if (t[c] == 1) continue;
+			bcell_ijk(g, c, &i, &j, &k);
+			const int m = bnb_list(g, i, j, k, nb);
+			int have = 0;
+			for (int q = 0; q < m && !have; q++) {
+				if (nb[q].id == c) continue;
+				if (t[nb[q].id] == 1) have = 1;
+			}
+			if (!have) t[c] = 2;
+		}
+		/* copy cells need a normal neighbour to copy from */
+		for (long c = 0; c < n; c++) {
+			if (t[c] != 4) continue;
+			bcell_ijk(g, c, &i, &j, &k);
+			const int m = bnb_list(g, i, j, k, nb);
+			int sources = 0;
+			for (int q = 0; q < m; q++) if (t[nb[q].id] == 1) sources++;
+			if (!sources) t[c] = 2;
+		}
+	}
+	/* set_solver_info */
+	for (long c = 0; c < n; c++) {
+		ctype[c] = 1;
+		for (int v = 0; v < 4; v++) {
+			const int t = vtype[(long)v * n + c];
+			if (t == 3 || t == 4) ctype[c] = 0;
+		}
+		const int ds = vtype[c] == 2;
+		for (int v = 1; v < 4; v++) {
+			if ((vtype[(long)v * n + c] == 2) != ds) return OR_ERR_SOLVER;
+		}
+		if (ds) ctype[c] = -1;
+	}
+	/* classify_faces: faces sharing an edge with a normal cell are normal */
+	for (long c = 0; c < n; c++) {
+		for (int f = 0; f < 6; f++) FI(f, c) = -99;
+		if (ctype[c] == 1) {
+			for (int f = 0; f < 6; f++) FI(f, c) = 1;
+			continue;
+		}
+		bcell_ijk(g, c, &i, &j, &k);
+		const int m = bnb_list(g, i, j, k, nb);
+		for (int q = 0; q < m; q++) {
+			if (ctype[nb[q].id] < 1) continue;
+			const int fn = nb[q].fn;
+			if (fn != 0) {
+				for (int dir = -3; dir <= 3; dir++) {
+					if (dir == 0 || dir == -fn) continue;
+					FI(fidx(dir), c) = 1;
+				}
+			}
+			if (nb[q].en[0] >= 0) {
+				const int e = nb[q].en[0];
+				const int dir1 = 1 + (e == 0 ? 1 : 0), dir2 = 1 + (e == 2 ? 1 : 2);
+				FI(fidx(nb[q].en[1] < 0 ? -dir1 : +dir1), c) = 1;
+				FI(fidx(nb[q].en[2] < 0 ? -dir2 : +dir2), c) = 1;
+			}
+		}
+	}
+	/* the other faces of a cell with a normal face are boundary faces */
+	for (long c = 0; c < n; c++) {
+		int normal = 0;
+		for (int f = 0; f < 6; f++) if (FI(f, c) == 1) normal = 1;
+		if (!normal) continue;
+		for (int f = 0; f < 6; f++) if (FI(f, c) < 0) FI(f, c) = 0;
+	}
+	/* faces next to normal faces of neighbours are boundary faces */
+	for (long c = 0; c < n; c++) {
+		bcell_ijk(g, c, &i, &j, &k);
+		const int m = bnb_list(g, i, j, k, nb);
+		for (int q = 0; q < m; q++) {
+			const long b = nb[q].id;
+			const int fn = nb[q].fn;
+			if (fn != 0) {
+				for (int dir = -3; dir <= 3; dir++) {
+					if (dir == 0) continue;
+					if (FI(fidx(dir), b) < 1) continue;
+					if (dir == fn && FI(fidx(dir), c) < 0) {
+						FI(fidx(dir), c) = 0;
+						continue;
+					}
+					if (dir == -fn && FI(fidx(fn), c) < 1) return OR_ERR_SOLVER;
+					if (FI(fidx(dir), b) == 1 && FI(fidx(dir), c) < 0) FI(fidx(dir), c) = 0;
+				}
+			}
+			if (nb[q].en[0] >= 0) {
+				const int e = nb[q].en[0];
+				const int dir1 = 1 + (e == 0 ? 1 : 0), dir2 = 1 + (e == 2 ? 1 : 2);
+				const int s1 = nb[q].en[1] < 0 ? -1 : 1, s2 = nb[q].en[2] < 0 ? -1 : 1;
+				if (FI(fidx(-s1 * dir1), b) == 1 || FI(fidx(-s2 * dir2), b) == 1) {
+					if (FI(fidx(s1 * dir1), c) < 0) FI(fidx(s1 * dir1), c) = 0;
+					if (FI(fidx(s2 * dir2), c) < 0) FI(fidx(s2 * dir2), c) = 0;
+				}
+			}
+		}
+	}
+	for (long c = 0; c < n; c++)
+		for (int f = 0; f < 6; f++) if (FI(f, c) < -1) FI(f, c) = -1;
+	return OR_OK;
+}
+
+static inline void bvel(const double* S, long n, long c, double v[3])
+{
+	/* get_velocity, source/mhd/common.hpp:121-130 */
+	if (BS(0, c) > 0) {
+		const double inv = 1 / BS(0, c);
+		for (int q = 0; q < 3; q++) v[q] = BS(1 + q, c) * inv;
+	} else {
+		v[0] = v[1] = v[2] = 0;
+	}
+}
+
+static inline int bpressure(const or_bgrid* g, const double* S, long n, long c, double* p)
+{
+	const double U[8] = {BS(0, c), BS(1, c), BS(2, c), BS(3, c), BS(4, c), BS(5, c), BS(6, c), BS(7, c)};
+	return or_get_pressure(U, g->gamma, g->mu0, p);
+}
+
+/* update_vol_B with constant thermal pressure for one cell (cells with
+   rho <= 0 are skipped) */
+static void bnd_refit(const or_bgrid* g, double* S, long n, long c)
+{
+	if (BS(0, c) <= 0) return;
+	double p, v[3], B[3];
+	bpressure(g, S, n, c, &p);
+	for (int d = 0; d < 3; d++) {
+		BS(5 + d, c) = 0.5 * (BF(2 * d, c) + BF(2 * d + 1, c));
+		B[d] = BS(5 + d, c);
+	}
+	const double inv = 1 / BS(0, c);
+	for (int q = 0; q < 3; q++) v[q] = BS(1 + q, c) * inv;
+	BS(4, c) = or_total_energy(BS(0, c), v, p, B, g->gamma, g->mu0);
+}
+
+int or_bnd_apply(const or_bgrid* g, const unsigned char* val_mask, const double* n_val,
+	const double* v_val, const double* p_val, const double* b_val, double* S)
+{
+	const long n = bgrid_n(g);
+	const signed char* ctype = g->ctype;
+	const signed char* finfo = g->finfo;
+	bnb_t nb[26];
+	long i, j, k;
+	/* number density */
+	for (long c = 0; c < n; c++) if (val_mask[c]) BS(0, c) = g->proton_mass * n_val[c];
+	for (long c = 0; c < n; c++) {
+		if (g->vtype[c] != 4) continue;
+		bcell_ijk(g, c, &i, &j, &k);
+		const int m = bnb_list(g, i, j, k, nb);
+		double fs = 0, es = 0;
+		size_t nf = 0, ne = 0;
+		for (int q = 0; q < m; q++) {
+			const long b = nb[q].id;
+			if (ctype[b] < 1) continue;
+			if (nb[q].fn != 0) { fs += BS(0, b); nf++; }
+			if (nb[q].en[0] >= 0) { es += BS(0, b); ne++; }
+		}
+		if (nf > 0) BS(0, c) = fs / nf;
+		else if (ne > 0) BS(0, c) = es / ne;
+	}
+	/* velocity */
+	for (long c = 0; c < n; c++) {
+		if (!val_mask[n + c]) continue;
+		for (int q = 0; q < 3; q++) BS(1 + q, c) = v_val[(long)q * n + c] * BS(0, c);
+	}
+	for (long c = 0; c < n; c++) {
+		if (g->vtype[n + c] != 4) continue;
+		bcell_ijk(g, c, &i, &j, &k);
+		const int m = bnb_list(g, i, j, k, nb);
+		double fs[3] = {0, 0, 0}, es[3] = {0, 0, 0}, v[3];
+		size_t nf = 0, ne = 0;
+		for (int q = 0; q < m; q++) {
+			const long b = nb[q].id;
+			if (ctype[b] < 1) continue;
+			if (nb[q].fn != 0) {
+				bvel(S, n, b, v);
+				for (int r = 0; r < 3; r++) fs[r] = fs[r] + v[r];
+				nf++;
+			}
+			if (nb[q].en[0] >= 0) {
+				bvel(S, n, b, v);
+				for (int r = 0; r < 3; r++) es[r] = es[r] + v[r];
+				ne++;
+			}
+		}
+		/* mul(mul(Mas, sum), 1 / count) with an integer quotient: the
+		   factor is 1 for one source and 0 for more (the reference's
+		   integer division) */
+		if (nf > 0) {
+			const double f = (double)(1 / nf);
+			for (int r = 0; r < 3; r++) BS(1 + r, c) = fs[r] * BS(0, c) * f;
+		} else if (ne > 0) {
+			const double f = (double)(1 / ne);
+			for (int r = 0; r < 3; r++) BS(1 + r, c) = es[r] * BS(0, c) * f;
+		}
+	}
+	/* pressure */
+	for (long c = 0; c < n; c++) {
+		if (!val_mask[2 * n + c]) continue;
+		if (BS(0, c) <= 0) return OR_ERR_DENSITY;
+		double v[3];
+		bvel(S, n, c, v);
+		const double B[3] = {BS(5, c), BS(6, c), BS(7, c)};
+		BS(4, c) = or_total_energy(BS(0, c), v, p_val[c], B, g->gamma, g->mu0);
+	}
+	for (long c = 0; c < n; c++) {
+		if (g->vtype[2 * n + c] != 4) continue;
+		bcell_ijk(g, c, &i, &j, &k);
+		const int m = bnb_list(g, i, j, k, nb);
+		double fs = 0, es = 0, p;
+		size_t nf = 0, ne = 0;
+		for (int q = 0; q < m; q++) {
+			const long b = nb[q].id;
+			if (ctype[b] < 1) continue;
+			if (nb[q].fn != 0) {
+				if (bpressure(g, S, n, b, &p)) return OR_ERR_DENSITY;
+				fs += p;
+				nf++;
+			}
+			if (nb[q].en[0] >= 0) {
+				if (bpressure(g, S, n, b, &p)) return OR_ERR_DENSITY;
+				es += p;
+				ne++;
+			}
+		}
+		if (nf == 0 && ne == 0) continue;
+		if (BS(0, c) <= 0) return OR_ERR_DENSITY;
+		double v[3];
+		bvel(S, n, c, v);
+		const double B[3] = {BS(5, c), BS(6, c), BS(7, c)};
+		BS(4, c) = or_total_energy(BS(0, c), v, nf > 0 ? fs / nf : es / ne, B, g->gamma, g->mu0);
+	}
+	/* face field: value boundaries on the boundary faces */
+	for (long c = 0; c < n; c++) {
+		if (!val_mask[3 * n + c]) continue;
+		for (int f = 0; f < 6; f++) if (FI(f, c) == 0) BF(f, c) = b_val[(long)f * n + c];
+	}
+	/* face field: copy boundaries (uniform grids: neighbours of equal size) */
+	for (long c = 0; c < n; c++) {
+		if (g->vtype[3 * n + c] != 4) continue;
+		int cnt[6] = {0, 0, 0, 0, 0, 0};
+		for (int d = 0; d < 3; d++) {
+			for (int s = 0; s < 2; s++) if (FI(2 * d + s, c) < 1) BF(2 * d + s, c) = 0;
+			const int lo = FI(2 * d, c), hi = FI(2 * d + 1, c);
+			if ((lo == 0 && hi == 0) || (lo == -1 && hi == -1)) return OR_ERR_SOLVER;
+			if (lo == 0 && hi == 1) { BF(2 * d, c) += BF(2 * d + 1, c); cnt[2 * d]++; }
+			if (lo == 1 && hi == 0) { BF(2 * d + 1, c) += BF(2 * d, c); cnt[2 * d + 1]++; }
+		}
+		bcell_ijk(g, c, &i, &j, &k);
+		const int m = bnb_list(g, i, j, k, nb);
+		for (int q = 0; q < m; q++) {
+			const long b = nb[q].id;
+			const int fn = nb[q].fn;
+			if (fn == 0 && nb[q].en[0] < 0) continue;
+			if (fn != 0) {
+				for (int dir = -3; dir <= 3; dir++) {
+					if (dir == 0) continue;
+					const int f = fidx(dir);
+					if (FI(f, c) == 0 && FI(f, b) == 1) { BF(f, c) += BF(f, b); cnt[f]++; }
+				}
+			}
+			if (nb[q].en[0] >= 0) {
+				const int e = nb[q].en[0];
+				const int d1 = e == 0 ? 1 : 0, d2 = e == 2 ? 1 : 2;
+				const int dirs[2] = {nb[q].en[1] * (d1 + 1), nb[q].en[2] * (d2 + 1)};
+				for (int r = 0; r < 2; r++) {
+					const int f = fidx(dirs[r]), fo = fidx(-dirs[r]);
+					if (FI(f, c) == 0 && FI(fo, b) == 1) { BF(f, c) += BF(fo, b); cnt[f]++; }
+				}
+			}
+		}
+		for (int f = 0; f < 6; f++) {
+			if (FI(f, c) != 0) continue;
+			if (cnt[f] == 0) return OR_ERR_SOLVER;
+			BF(f, c) /= cnt[f];
+		}
+	}
+	/* update_vol_B(0) at constant pressure */
+	for (long c = 0; c < n; c++) if (ctype[c] >= 0) bnd_refit(g, S, n, c);
+	return OR_OK;
+}
+
+void or_bnd_sync(const or_bgrid* g, double* S)
+{
+	const long n = bgrid_n(g);
+	const signed char* ctype = g->ctype;
+	const long N[3] = {g->nx, g->ny, g->nz};
+	for (int d = 0; d < 3; d++)
+	for (long c = 0; c < n; c++) {
+		long idx[3];
+		bcell_ijk(g, c, idx, idx + 1, idx + 2);
+		if (ctype[c] < 0) continue;
+		long up = idx[d] + 1;
+		if (up >= N[d]) {
+			if (!g->periodic[d]) continue;
+			up = 0;
+		}
+		idx[d] = up;
+		const long b = idx[0] + N[0] * (idx[1] + N[1] * idx[2]);
+		if (ctype[b] < 0) continue;
+		double* pc = &BF(2 * d + 1, c);   /* c's + copy */
+		double* pb = &BF(2 * d, b);       /* b's - copy */
+		if (c <= b) {
+			/* the visit from c comes first (a one-cell dimension: the
+			   cell's -d visit, copies -d then +d) */
+			if (c == b) {
+				double diff = *pb - *pc; *pb -= diff / 2; *pc += diff / 2;
+				diff = *pc - *pb; *pc -= diff / 2; *pb += diff / 2;
+			} else {
+				double diff = *pc - *pb; *pc -= diff / 2; *pb += diff / 2;
+				diff = *pb - *pc; *pb -= diff / 2; *pc += diff / 2;
+			}
+		} else {
+			double diff = *pb - *pc; *pb -= diff / 2; *pc += diff / 2;
+			diff = *pc - *pb; *pc -= diff / 2; *pb += diff / 2;
+		}
+	}
+}
+
+/* the cell across the face on c's side dir (-3..+3), or -1 */
+static long bnd_face_nb(const or_bgrid* g, long c, int dir)
+{
+	long idx[3];
+	bcell_ijk(g, c, idx, idx + 1, idx + 2);
+	const long N[3] = {g->nx, g->ny, g->nz};
+	const int d = (dir < 0 ? -dir : dir) - 1;
+	long v = idx[d] + (dir < 0 ? -1 : 1);
+	if (v < 0 || v >= N[d]) {
+		if (!g->periodic[d]) return -1;
+		v = (v + N[d]) % N[d];
+	}
+	idx[d] = v;
+	return idx[0] + N[0] * (idx[1] + N[1] * idx[2]);
+}
+
+/* each cell's period (2^level) and the smallest period around it, the
+   substep length and the largest period: or_substep_periods over the cells
+   of type >= 0 and the faces between them */
+static int bnd_periods(const or_bgrid* g, double cfl, double max_dt, const int* substep_min,
+	const int* substep_max, const double* maxv, int* period, int* minsub, double* sub_dt_out,
+	int* max_period_out, long* bad_cell)
+{
+	const long n = bgrid_n(g);
+	const signed char* ctype = g->ctype;
+	const double len[3] = {g->dx, g->dy, g->dz};
+	double* ts = (double*)malloc(sizeof(double) * n);
+	int* lomin = (int*)malloc(sizeof(int) * n);
+	int* smaxv = (int*)malloc(sizeof(int) * n);
+	bnb_t nb[26];
+	long i, j, k;
+	int err = OR_OK;
+	if (!ts || !lomin || !smaxv) { err = OR_ERR_SOLVER; goto done; }
+	int lo = INT_MAX;
+	for (long c = 0; c < n; c++) {
+		lomin[c] = substep_min[c];
+		smaxv[c] = substep_max[c];
+		if (lomin[c] < lo) lo = lomin[c];
+	}
+	if (lo > 0) for (long c = 0; c < n; c++) lomin[c] -= lo;
+	/* minimize_timestep over the faces between cells of type >= 0 */
+	for (long c = 0; c < n; c++) {
+		double t = DBL_MAX;
+		if (ctype[c] >= 0) {
+			for (int dir = -3; dir <= 3; dir++) {
+				if (dir == 0) continue;
+				const long b = bnd_face_nb(g, c, dir);
+				if (b < 0 || ctype[b] < 0) continue;
+				const int d = (dir < 0 ? -dir : dir) - 1;
+				const double v = maxv[(long)d * n + (dir < 0 ? c : b)];
+				if (v <= 0) t = 0;
+				else t = smin(cfl * len[d] / v, t);
+			}
+		}
+		ts[c] = t;
+	}
+	double smallest = DBL_MAX;
+	for (long c = 0; c < n; c++) if (ctype[c] >= 0) smallest = smin(ts[c], smallest);
+	double sub_dt;
+	if (max_dt <= smallest) {
+		for (long c = 0; c < n; c++) lomin[c] = smaxv[c] = 0;
+		sub_dt = max_dt;
+	} else {
+		double ret = DBL_MAX;
+		for (long c = 0; c < n; c++) {
+			if (ctype[c] < 0) continue;
+			ret = smin(ret, ts[c] / (double)(1 << lomin[c]));
+		}
+		for (long c = 0; c < n; c++) {
+			if (ctype[c] < 0) continue;
+			const double dt = ts[c];
+			if (dt > 0) {
+				const double x = smin(dt, max_dt) / ret;
+				int step = (isfinite(x) && x > 0) ? ilogb(x) : 0;
+				if (step < 0) step = 0;
+				if (smaxv[c] > step) smaxv[c] = step;
+				if (lomin[c] > smaxv[c] || smaxv[c] > 30) {
+					if (bad_cell) *bad_cell = c;
+					err = OR_ERR_SUBSTEP;
+					goto done;
+				}
+			} else {
+				lomin[c] = smaxv[c] = 0;
+			}
+		}
+		sub_dt = ret;
+	}
+	/* restrict_substepping_period over face and edge neighbours of type >= 0 */
+	for (long c = 0; c < n; c++) period[c] = ctype[c] >= 0 ? smaxv[c] : 0;
+	for (;;) {
+		long modified = 0;
+		for (long c = 0; c < n; c++) {
+			if (ctype[c] < 0) continue;
+			bcell_ijk(g, c, &i, &j, &k);
+			const int m = bnb_list(g, i, j, k, nb);
+			for (int q = 0; q < m; q++) {
+				const long b = nb[q].id;
+				if (ctype[b] < 0) continue;
+				if (nb[q].fn == 0 && nb[q].en[0] < 0) continue;
+				if (period[c] > period[b] + 1) { period[c] = period[b] + 1; modified++; }
+			}
+		}
+		if (!modified) break;
+	}
+	int maxp = -1;
+	for (long c = 0; c < n; c++) {
+		period[c] = 1 << period[c];
+		if (ctype[c] >= 0 && period[c] > maxp) maxp = period[c];
+	}
+	for (long c = 0; c < n; c++) {
+		int m_ = period[c];
+		if (ctype[c] >= 0) {
+			bcell_ijk(g, c, &i, &j, &k);
+			const int m = bnb_list(g, i, j, k, nb);
+			for (int q = 0; q < m; q++) {
+				const long b = nb[q].id;
+				if (ctype[b] < 0) continue;
+				if (nb[q].fn == 0 && nb[q].en[0] < 0) continue;
+				if (period[b] < m_) m_ = period[b];
+			}
+		}
+		minsub[c] = m_;
+	}
+	*sub_dt_out = sub_dt;
+	*max_period_out = maxp;
+done:
+	free(ts); free(lomin); free(smaxv);
+	return err;
+}
+
+/* P of the d-face on the negative side of cell (i, j, k): periodic
+   dimensions wrap, 0 beyond a closed end */
+static inline double bnd_pf(const or_bgrid* g, const double* P, long n, int d, int q,
+	long i, long j, long k)
+{
+	long idx[3] = {i, j, k};
+	const long N[3] = {g->nx, g->ny, g->nz};
+	for (int a = 0; a < 3; a++) {
+		if (idx[a] < 0 || idx[a] >= N[a]) {
+			if (!g->periodic[a]) return 0.0;
+			idx[a] = (idx[a] + N[a]) % N[a];
+		}
+	}
+	return P[((long)d * 8 + q) * n + idx[0] + N[0] * (idx[1] + N[1] * idx[2])];
+}
+
+int or_bnd_step(const or_bgrid* g, int solver, double time_step_factor, double max_dt,
+	const int* substep_min, const int* substep_max, double* S, double* maxv,
+	double* dt_out, int* max_period_out, long* bad_face)
+{
+	const long n = bgrid_n(g);
+	const signed char* ctype = g->ctype;
+	const double len[3] = {g->dx, g->dy, g->dz};
+	const double area[3] = {g->dy * g->dz, g->dx * g->dz, g->dx * g->dy};
+	const double dx = g->dx, dy = g->dy, dz = g->dz;
+	int* period = (int*)malloc(sizeof(int) * n);
+	int* minsub = (int*)malloc(sizeof(int) * n);
+	double* slot = (double*)calloc((size_t)48 * n, sizeof(double));
+	double* fdb = (double*)calloc((size_t)6 * n, sizeof(double));
+	double* P = (double*)malloc(sizeof(double) * 24 * n);
+	int err = OR_OK;
+	if (!period || !minsub || !slot || !fdb || !P) { err = OR_ERR_SOLVER; goto done; }
+	double sub_dt;
+	int maxp;
+	long bad_cell = -1;
+	err = bnd_periods(g, time_step_factor, max_dt, substep_min, substep_max, maxv, period,
+		minsub, &sub_dt, &maxp, &bad_cell);
+	if (err) { if (bad_face) *bad_face = bad_cell; goto done; }
+
+	double total_dt = 0;
+	for (int sub = 1; sub <= maxp; sub++) {
+		total_dt += sub_dt;
+		/* fluxes of the faces due at this substep between cells of type
+		   >= 0, owned by the cell on the negative side; P = 0 elsewhere */
+		long bad = -1;
+		for (int d = 0; d < 3 && !err; d++)
+		for (long c = 0; c < n; c++) {
+			for (int q = 0; q < 8; q++) P[(d * 8 + q) * n + c] = 0;
+			const long o = bnd_face_nb(g, c, -(d + 1));
+			if (o < 0 || ctype[o] < 0 || ctype[c] < 0) continue;
+			const int m = minsub[o];
+			if (sub % m != 0) continue;
+			double L[8], R[8], bgr[3] = {0, 0, 0}, F[8], sp = 0;
+			{
+				const double ml[3] = {BS(1, o), BS(2, o), BS(3, o)}, Bl[3] = {BS(5, o), BS(6, o), BS(7, o)};
+				const double mr[3] = {BS(1, c), BS(2, c), BS(3, c)}, Br[3] = {BS(5, c), BS(6, c), BS(7, c)};
+				L[RHO] = BS(0, o); L[NRJ] = BS(4, o);
+				R[RHO] = BS(0, c); R[NRJ] = BS(4, c);
+				rot_to_face(d, ml, L + M0); rot_to_face(d, Bl, L + B0);
+				rot_to_face(d, mr, R + M0); rot_to_face(d, Br, R + B0);
+			}
+			if (g->bg) {
+				const double b[3] = {g->bg[(3 * d + 0) * n + o], g->bg[(3 * d + 1) * n + o],
+					g->bg[(3 * d + 2) * n + o]};
+				rot_to_face(d, b, bgr);
+			}
+			const int e = or_flux(solver, L, R, bgr, g->gamma, g->mu0, F, &sp);
+			if (e) {
+				if (!err || (long)d * n + c < bad) { err = e; bad = (long)d * n + c; }
+				continue;
+			}
+			maxv[(long)d * n + c] = sp;
+			double Fg[8];
+			Fg[RHO] = F[RHO];
+			Fg[NRJ] = F[NRJ];
+			rot_to_global(d, F + M0, Fg + M0);
+			rot_to_global(d, F + B0, Fg + B0);
+			const double min_dt = sub_dt * m;
+			for (int q = 0; q < 8; q++) P[(d * 8 + q) * n + c] = min_dt * Fg[q];
+		}
+		if (err) { if (bad_face) *bad_face = bad; goto done; }
+
+		/* flux of the d-face on the negative side of the cell at offset
+		   (a, b, e) from (i, j, k); 0 where that cell does not exist */
+#define BPF(d, q, a, b, e) bnd_pf(g, P, n, (d), (q), i + (a), j + (b), k + (e))
+#define EZX(a, b, e) (+(BPF(0, 6, a, b, e) * dz / 4))
+#define EYX(a, b, e) (-(BPF(0, 7, a, b, e) * dy / 4))
+#define EXY(a, b, e) (+(BPF(1, 7, a, b, e) * dx / 4))
+#define EZY(a, b, e) (-(BPF(1, 5, a, b, e) * dz / 4))
+#define EXZ(a, b, e) (-(BPF(2, 6, a, b, e) * dx / 4))
+#define EYZ(a, b, e) (+(BPF(2, 5, a, b, e) * dy / 4))
+		for (long c = 0; c < n; c++) {
+			long i, j, k;
+			bcell_ijk(g, c, &i, &j, &k);
+			double db[6];
+			for (int side = 0; side < 2; side++) {
+				const long a = side;
+				double r = 0;
+				r += EZX(a, 1, 0); r -= EZX(a, -1, 0); r -= EYX(a, 0, 1); r += EYX(a, 0, -1);
+				r += EZY(a, 1, 0); r += EZY(a - 1, 1, 0); r -= EZY(a, 0, 0); r -= EZY(a - 1, 0, 0);
+				r -= EYZ(a, 0, 1); r -= EYZ(a - 1, 0, 1); r += EYZ(a, 0, 0); r += EYZ(a - 1, 0, 0);
+				db[0 + side] = r;
+				const long b = side;
+				r = 0;
+				r += EXY(0, b, 1); r -= EXY(0, b, -1); r -= EZY(1, b, 0); r += EZY(-1, b, 0);
+				r += EXZ(0, b, 1); r += EXZ(0, b - 1, 1); r -= EXZ(0, b, 0); r -= EXZ(0, b - 1, 0);
+				r -= EZX(1, b, 0); r += EZX(0, b, 0); r -= EZX(1, b - 1, 0); r += EZX(0, b - 1, 0);
+				db[2 + side] = r;
+				const long z = side;
+				r = 0;
+				r += EYZ(1, 0, z); r -= EYZ(-1, 0, z); r -= EXZ(0, 1, z); r += EXZ(0, -1, z);
+				r += EYX(1, 0, z); r += EYX(1, 0, z - 1); r -= EYX(0, 0, z); r -= EYX(0, 0, z - 1);
+				r -= EXY(0, 1, z); r -= EXY(0, 1, z - 1); r += EXY(0, 0, z); r += EXY(0, 0, z - 1);
+				db[4 + side] = r;
+			}
+			for (int f = 0; f < 6; f++) fdb[(long)f * n + c] += db[f];
+			for (int d = 0; d < 3; d++) {
+				const long cp = bnd_face_nb(g, c, d + 1);
+				for (int q = 0; q < 8; q++) {
+					slot[((long)(2 * d) * 8 + q) * n + c] += P[(d * 8 + q) * n + c];
+					if (cp >= 0) slot[((long)(2 * d + 1) * 8 + q) * n + c] += P[(d * 8 + q) * n + cp];
+				}
+			}
+			const int due = sub % period[c] == 0;
+			if (ctype[c] > 0 && !due) continue;   /* normal cells accumulate until due */
+			if (ctype[c] > 0) {
+				/* update_mhd_state */
+				double rho = BS(0, c), m[3] = {BS(1, c), BS(2, c), BS(3, c)}, E = BS(4, c);
+				double vb[3] = {BS(5, c), BS(6, c), BS(7, c)};
+#define SL(dir, q) slot[((long)(dir) * 8 + (q)) * n + c]
+				for (int d = 0; d < 3; d++) rho += (SL(2 * d, RHO) - SL(2 * d + 1, RHO)) / len[d];
+				for (int d = 0; d < 3; d++) {
+					const double il = 1 / len[d];
+					for (int q = 0; q < 3; q++) m[q] = m[q] + (SL(2 * d, M0 + q) - SL(2 * d + 1, M0 + q)) * il;
+				}
+				for (int d = 0; d < 3; d++) E += (SL(2 * d, NRJ) - SL(2 * d + 1, NRJ)) / len[d];
+				for (int d = 0; d < 3; d++) {
+					const double il = 1 / len[d];
+					for (int q = 0; q < 3; q++) vb[q] = vb[q] + (SL(2 * d, B0 + q) - SL(2 * d + 1, B0 + q)) * il;
+				}
+#undef SL
+				for (int d = 0; d < 3; d++) {
+					BF(2 * d, c) += fdb[(long)(2 * d) * n + c] / area[d];
+					BF(2 * d + 1, c) += fdb[(long)(2 * d + 1) * n + c] / area[d];
+				}
+				BS(0, c) = rho;
+				for (int q = 0; q < 3; q++) BS(1 + q, c) = m[q];
+				BS(4, c) = E;
+				for (int q = 0; q < 3; q++) BS(5 + q, c) = vb[q];
+			}
+			for (int dir = 0; dir < 6; dir++) {
+				for (int q = 0; q < 8; q++) slot[((long)dir * 8 + q) * n + c] = 0;
+				fdb[(long)dir * n + c] = 0;
+			}
+			/* update_vol_B(substep) of cells of type >= 0 due now */
+			if (ctype[c] >= 0 && due) bnd_refit(g, S, n, c);
+		}
+#undef BPF
+#undef EZX
+#undef EYX
+#undef EXY
+#undef EZY
+#undef EXZ
+#undef EYZ
+	}
+	or_bnd_sync(g, S);
+	for (long c = 0; c < n; c++) if (ctype[c] >= 0) bnd_refit(g, S, n, c);
+	*dt_out = total_dt;
+	*max_period_out = maxp;
+done:
+	free(period); free(minsub); free(slot); free(fdb); free(P);
+	return err;
+}
+
+#undef BS
+#undef BF
+#undef FI

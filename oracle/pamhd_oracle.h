@@ -244,6 +244,62 @@ void or_box_sync(const or_box* b, double* state);
 int or_box_run(const or_box* b, int solver, double cfl, double t_start, double t_end,
 	long max_steps, double* state, long* steps_taken, double* t_final);
 
+/* ---- uniform grids with value and copy boundaries ----
+
+A grid of nx*ny*nz cells, periodic or closed in each dimension, whose state
+keeps each cell's own copies of its six faces (14 doubles per cell,
+component-major: rho, m (3), E, Vol_B (3), face B of the -x, +x, -y, +y,
+-z, +z faces).  Neighbours are those of a neighbourhood of length 1 in the
+order z slowest, x fastest (offsets -1..1); periodic dimensions wrap (a
+one-cell periodic dimension makes a cell its own neighbour), closed ones
+end.
+
+or_bnd_classify restates Boundaries::classify (source/boundaries/
+boundaries.hpp:100-226) for each of the four variables (number density,
+velocity, pressure, magnetic field), set_solver_info (source/mhd/
+boundaries.hpp:67-245) and classify_faces (:1039-1217).  Inputs per
+variable v: in_value[v*n + c] / in_copy[v*n + c] = 1 where cell c overlaps
+a geometry of one of v's value / copy boundaries.  Outputs: vtype[v*n + c]
+(1 normal, 2 not solved, 3 value boundary, 4 copy boundary with at least
+one normal neighbour), ctype[c] (1 / 0 / -1) and finfo[f*n + c] for the six
+faces f = -x, +x, -y, +y, -z, +z (1 / 0 / -1).  Returns OR_ERR_SOLVER when
+the variables' not-solved cells differ (the reference throws).
+
+or_bnd_apply restates apply_boundaries (source/mhd/boundaries.hpp:
+1261-1297): the value and copy boundaries of number density, velocity and
+pressure (apply_fluid_boundaries, :738-1026), of the face field
+(apply_magnetic_field_boundaries_staggered, :250-640, uniform grids), and
+the constant-pressure refit of every cell of type >= 0.  Value boundaries
+come resolved per cell (the last boundary covering a cell wins):
+val_mask[v*n + c], n_val (n), v_val (3n), p_val (n), b_val (6n, the field
+component normal to each face at its centre).  or_bnd_sync is
+sync_magnetic_field (source/common_functions.hpp:184-269).
+
+or_bnd_step is timestep() (source/mhd/solve.hpp:1837-1924) with per-cell
+substepping over the cell types: cells of type >= 0 get periods, faces
+between two of them fluxes, normal cells the update; every cell of type
+>= 0 refits at its period; the face copies are synced at the end. */
+
+typedef struct or_bgrid {
+	int nx, ny, nz;
+	double dx, dy, dz;
+	double gamma, mu0, proton_mass;
+	int periodic[3];
+	const signed char* ctype;   /* n */
+	const signed char* finfo;   /* 6 n */
+	const signed char* vtype;   /* 4 n */
+	const double* bg;           /* 9 n or NULL: Bg_B(cell)(+d), as or_grid */
+} or_bgrid;
+
+int or_bnd_classify(const or_bgrid* g, const unsigned char* in_value,
+	const unsigned char* in_copy, signed char* vtype, signed char* ctype, signed char* finfo);
+int or_bnd_apply(const or_bgrid* g, const unsigned char* val_mask, const double* n_val,
+	const double* v_val, const double* p_val, const double* b_val, double* state);
+void or_bnd_sync(const or_bgrid* g, double* state);
+int or_bnd_step(const or_bgrid* g, int solver, double time_step_factor, double max_dt,
+	const int* substep_min, const int* substep_max, double* state, double* maxv,
+	double* dt_out, int* max_period_out, long* bad_face);
+
 /* OpenMP threads of the grid step (n > 0 sets it); returns the count in use */
 int or_set_threads(int n);
 

@@ -132,10 +132,33 @@ class _Variable:
     def __init__(self, obj, ncomp, name):
         if not isinstance(obj, dict) or "default" not in obj:
             raise ConfigError("%s: object with a default key required" % name)
-        for key in ("value-boundaries", "copy-boundaries"):
-            if obj.get(key):
-                raise ConfigError("%s: %s are not supported (periodic grids only)" % (name, key))
         self.default = _Value(obj["default"], ncomp, name + " default")
+        # value_boundaries.hpp / value_boundary.hpp: time series per geometry
+        self.value_boundaries = []
+        for i, b in enumerate(obj.get("value-boundaries", [])):
+            what = "%s value-boundaries[%d]" % (name, i)
+            gid = b.get("geometry-id") if isinstance(b, dict) else None
+            if not isinstance(gid, int) or isinstance(gid, bool) or gid < 0:
+                raise ConfigError(what + ": geometry-id required")
+            stamps = b.get("time-stamps")
+            vals = b.get("values")
+            if not isinstance(stamps, list) or not stamps:
+                raise ConfigError(what + ": time-stamps array required")
+            if isinstance(vals, dict):
+                raise ConfigError(what + ": point-cloud data is not supported")
+            if not isinstance(vals, list) or len(vals) != len(stamps):
+                raise ConfigError(what + ": one value per time stamp required")
+            stamps = [float(t) for t in stamps]
+            if stamps != sorted(stamps):
+                raise ConfigError(what + ": time stamps must not decrease")
+            self.value_boundaries.append((gid, stamps, [_Value(v, ncomp, what) for v in vals]))
+        # copy_boundaries.hpp: geometry ids
+        self.copy_boundaries = []
+        for i, b in enumerate(obj.get("copy-boundaries", [])):
+            gid = b.get("geometry-id") if isinstance(b, dict) else None
+            if not isinstance(gid, int) or isinstance(gid, bool) or gid < 0:
+                raise ConfigError("%s copy-boundaries[%d]: geometry-id required" % (name, i))
+            self.copy_boundaries.append(gid)
         self.regions = []
         for i, r in enumerate(obj.get("initial-conditions", [])):
             gid = r.get("geometry-id") if isinstance(r, dict) else None
@@ -256,9 +279,6 @@ class Config:
         self.start = tuple(float(np.asarray(v)) for v in vec3("start", cenv))
         if g.get("max-ref-lvl", 0) not in (0,):
             raise ConfigError("adaptive mesh refinement (max-ref-lvl > 0) is not supported")
-        for d in range(3):
-            if not self.periodic[d] and self.cells[d] > 1:
-                raise ConfigError("dimension %d is not periodic: boundaries are not supported" % d)
         # cell length, tests/mhd/test.cpp:339-346
         self.cell_length = tuple(self.volume[d] / self.cells[d] for d in range(3))
 
@@ -435,8 +455,151 @@ class Config:
 
     def initial_state(self, t=None):
         """(state (8, n) face-once, background (9, n) or None) after the
-        initialisation sequence of tests/mhd/test.cpp:420-450"""
+        initialisation sequence of tests/mhd/test.cpp:420-450 (periodic
+        configurations; bounded ones: bounded_initial_state)"""
+        if self.bounded:
+            raise ConfigError("a configuration with boundaries or closed dimensions: "
+                              "use bounded_initial_state")
         t = self.time_start if t is None else t
+        rho, mom, E, volB, F = self._initialise(t, None)
+        gamma, mu0 = self.adiabatic_index, self.vacuum_permeability
+        # apply_boundaries: update_vol_B with constant pressure (cells with
+        # positive density); the volume field is unchanged here
+        pos = rho > 0
+        if pos.any():
+            rr, mm, BB = rho[pos], mom[:, pos], volB[:, pos]
+            kin = 0.5 / rr * (mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2])
+            mag = 0.5 / mu0 * (BB[0] * BB[0] + BB[1] * BB[1] + BB[2] * BB[2])
+            p_old = (E[pos] - kin - mag) * (gamma - 1)
+            v = mm * (1 / rr)
+            E[pos] = _energy(rr, v, p_old, BB, gamma, mu0)
+        # sync_magnetic_field: the two copies of every face, the visit of
+        # the cell with the smaller index first
+        self._sync(F)
+        state = np.empty((8, self.n))
+        state[0] = rho
+        state[1:4] = mom
+        state[4] = E
+        state[5:8] = F[0]
+        return state, self._background_faces()
+
+    def bounded_initial_state(self, t=None):
+        """(state (14, n), background (9, n) or None, (vtype, ctype, finfo))
+        of a configuration with boundaries: the cell and face types
+        (set_solver_info, classify_faces), then initialize_magnetic_field,
+        update_vol_B of the cells of type >= 0 and initialize_fluid
+        (tests/mhd/test.cpp:412-444).  The state holds each cell's copies of
+        its six faces; apply_boundaries and sync_magnetic_field come next
+        (the run applies them on the device)."""
+        t = self.time_start if t is None else t
+        types = self.classify()
+        rho, mom, E, volB, F = self._initialise(t, types[1])
+        S = np.empty((14, self.n))
+        S[0] = rho
+        S[1:4] = mom
+        S[4] = E
+        S[5:8] = volB
+        for d in range(3):
+            S[8 + 2 * d] = F[0, d]
+            S[9 + 2 * d] = F[1, d]
+        return S, self._background_faces(), types
+
+    @property
+    def bounded(self):
+        """boundaries or a closed dimension: the cell-typed step"""
+        return not all(self.periodic) or any(
+            v.value_boundaries or v.copy_boundaries for v in self._variables())
+
+    def _variables(self):
+        return (self.number_density, self.velocity, self.pressure, self.magnetic_field)
+
+    def boundary_masks(self):
+        """(in_value, in_copy), (4, n) each: cells overlapping a value / copy
+        boundary geometry of each variable (test.cpp:394-399)"""
+        iv = np.zeros((4, self.n), dtype=np.uint8)
+        ic = np.zeros((4, self.n), dtype=np.uint8)
+        for v, var in enumerate(self._variables()):
+            for gid, _, _ in var.value_boundaries:
+                iv[v, self.overlapping(gid)] = 1
+            for gid in var.copy_boundaries:
+                ic[v, self.overlapping(gid)] = 1
+        return iv, ic
+
+    def classify(self):
+        """(vtype (4, n), ctype (n,), finfo (6, n)), see pamhd_amd.boundary"""
+        from . import boundary
+        iv, ic = self.boundary_masks()
+        try:
+            vtype, ctype, finfo = boundary.classify(self.cells, self.periodic, iv, ic)
+        except boundary.ClassificationError as e:
+            raise ConfigError(str(e)) from None
+        # apply_magnetic_field_boundaries_staggered throws for a copy cell
+        # whose two faces of one dimension are both boundary or both not
+        # solved (source/mhd/boundaries.hpp:333-337)
+        cp = vtype[3] == 4
+        for d in range(3):
+            lo, hi = finfo[2 * d], finfo[2 * d + 1]
+            bad = cp & (((lo == 0) & (hi == 0)) | ((lo == -1) & (hi == -1)))
+            if bad.any():
+                c = int(np.nonzero(bad)[0][0])
+                raise ConfigError("magnetic field copy cell %d has no normal face in dimension %d "
+                                  "(the reference's boundary code rejects it)" % (c, d))
+        return vtype, ctype, finfo
+
+    def boundary_values(self, t):
+        """The value boundaries at time t, resolved per cell (a later
+        boundary overrides an earlier one on the cells they share):
+        (mask (4, n) uint8, number density (n), velocity (3, n), pressure
+        (n), face field (6, n)).  Fluid values at the cell centres, the
+        field's normal component at the centres of the faces -x ... +z
+        (apply_fluid_boundaries, source/mhd/boundaries.hpp:738-1026, and
+        apply_magnetic_field_boundaries_staggered, :270-305, whose latitude
+        is asin(z / x))."""
+        from .boundary import nearest_stamp
+        n = self.n
+        mask = np.zeros((4, n), dtype=np.uint8)
+        out = [np.zeros(n), np.zeros((3, n)), np.zeros(n), np.zeros((6, n))]
+        _, _, ctr = self.cell_bounds()
+        for v, var in enumerate(self._variables()):
+            for gid, stamps, vals in var.value_boundaries:
+                cells = self.overlapping(gid)
+                if not len(cells):
+                    continue
+                val = vals[nearest_stamp(stamps, t)]
+                c = ctr[:, cells]
+                if v < 3:
+                    env = self._env(t, c[0], c[1], c[2], val.needs(), False)
+                    got = val((len(cells),), **env)
+                    if v == 1:
+                        out[1][:, cells] = got
+                    else:
+                        out[v][cells] = got[0]
+                else:
+                    for f in range(6):
+                        d, side = f // 2, (1 if f % 2 else -1)
+                        pos = c.copy()
+                        pos[d] = pos[d] + side * self.cell_length[d] / 2
+                        env = self._env_bfield(t, pos, val.needs())
+                        out[3][f, cells] = val((len(cells),), **env)[d]
+                mask[v, cells] = 1
+        return (mask, *out)
+
+    @staticmethod
+    def _env_bfield(t, pos, needs):
+        env = {"t": t, "x": pos[0], "y": pos[1], "z": pos[2]}
+        if needs & {"radius", "lat", "lon"}:
+            x, y, z = pos
+            env["radius"] = np.sqrt(x * x + y * y + z * z)
+            with np.errstate(all="ignore"):
+                env["lat"] = _asin(z / x)
+            env["lon"] = _atan2(y, x).astype(np.float64)
+        return env
+
+    def _initialise(self, t, ctype):
+        """face copies, then the fluid: (rho, m, E, Vol_B, F) with F of
+        shape (2, 3, n) = [minus / plus copy][dimension][cell]; Vol_B is
+        the average of each cell's own copies where its type is >= 0 (all
+        cells when ctype is None) and 0 elsewhere"""
         n = self.n
         gamma, mu0, mp = self.adiabatic_index, self.vacuum_permeability, self.proton_mass
         if gamma <= 1:
@@ -449,6 +612,8 @@ class Config:
                 F[:, :, cells] = self._face_copies(val, t, cells)
         # update_vol_B(constant pressure = false)
         volB = 0.5 * (F[0] + F[1])
+        if ctype is not None:
+            volB[:, ctype < 0] = 0.0
         # initialize_fluid: defaults at the emulated centres (lat guarded at r = 0)
         _, _, ctr = self.emulated_bounds()
         x, y, z = ctr
@@ -491,25 +656,7 @@ class Config:
                     v = mom[:, cells][:, good] * (1 / rg)
                     e[good] = _energy(rg, v, pv[good], volB[:, cells][:, good], gamma, mu0)
                 E[cells] = e
-        # apply_boundaries: update_vol_B with constant pressure (cells with
-        # positive density); the volume field is unchanged here
-        pos = rho > 0
-        if pos.any():
-            rr, mm, BB = rho[pos], mom[:, pos], volB[:, pos]
-            kin = 0.5 / rr * (mm[0] * mm[0] + mm[1] * mm[1] + mm[2] * mm[2])
-            mag = 0.5 / mu0 * (BB[0] * BB[0] + BB[1] * BB[1] + BB[2] * BB[2])
-            p_old = (E[pos] - kin - mag) * (gamma - 1)
-            v = mm * (1 / rr)
-            E[pos] = _energy(rr, v, p_old, BB, gamma, mu0)
-        # sync_magnetic_field: the two copies of every face, the visit of
-        # the cell with the smaller index first
-        self._sync(F)
-        state = np.empty((8, n))
-        state[0] = rho
-        state[1:4] = mom
-        state[4] = E
-        state[5:8] = F[0]
-        return state, self._background_faces()
+        return rho, mom, E, volB, F
 
     def _sync(self, F):
         """sync_magnetic_field on the copies F[0] (minus faces) and F[1]

@@ -35,6 +35,18 @@ class OrGrid(ctypes.Structure):
                 ("mu0", ctypes.c_double), ("bg", _dp)]
 
 
+_bp = ctypes.POINTER(ctypes.c_byte)
+_ubp = ctypes.POINTER(ctypes.c_ubyte)
+
+
+class OrBgrid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int), ("ny", ctypes.c_int), ("nz", ctypes.c_int),
+                ("dx", ctypes.c_double), ("dy", ctypes.c_double), ("dz", ctypes.c_double),
+                ("gamma", ctypes.c_double), ("mu0", ctypes.c_double),
+                ("proton_mass", ctypes.c_double), ("periodic", ctypes.c_int * 3),
+                ("ctype", _bp), ("finfo", _bp), ("vtype", _bp), ("bg", _dp)]
+
+
 class OrPlain1d(ctypes.Structure):
     _fields_ = [("ncells", ctypes.c_size_t), ("cell_size", ctypes.c_double),
                 ("gamma", ctypes.c_double), ("mu0", ctypes.c_double),
@@ -107,6 +119,16 @@ def lib():
                                           ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_long, _dp, _lp, _dp, _ip]
     L.or_run_periodic_substep.restype = ctypes.c_int
+    B = ctypes.POINTER(OrBgrid)
+    L.or_bnd_classify.argtypes = [B, _ubp, _ubp, _bp, _bp, _bp]
+    L.or_bnd_classify.restype = ctypes.c_int
+    L.or_bnd_apply.argtypes = [B, _ubp, _dp, _dp, _dp, _dp, _dp]
+    L.or_bnd_apply.restype = ctypes.c_int
+    L.or_bnd_sync.argtypes = [B, _dp]
+    L.or_bnd_sync.restype = None
+    L.or_bnd_step.argtypes = [B, ctypes.c_int, ctypes.c_double, ctypes.c_double, _ip, _ip,
+                              _dp, _dp, _dp, _ip, _lp]
+    L.or_bnd_step.restype = ctypes.c_int
     L.or_set_threads.argtypes = [ctypes.c_int]
     L.or_set_threads.restype = ctypes.c_int
     _lib = L
@@ -449,3 +471,100 @@ def run_config(cfg, state, bg=None, max_steps=1 << 40):
         steps += 1
         periods.append(mp)
     return st, steps, t, periods
+
+
+# --- uniform grids with value and copy boundaries --------------------------
+
+class BoundedGrid:
+    """An or_bgrid with its cell and face types.  in_value / in_copy are
+    (4, n) masks of the cells overlapping a value / copy boundary geometry
+    of number density, velocity, pressure and magnetic field."""
+
+    def __init__(self, cells, lengths, gamma, mu0, proton_mass, periodic, in_value, in_copy,
+                 bg=None):
+        n = cells[0] * cells[1] * cells[2]
+        self.n = n
+        self.vtype = np.zeros((4, n), dtype=np.int8)
+        self.ctype = np.zeros(n, dtype=np.int8)
+        self.finfo = np.zeros((6, n), dtype=np.int8)
+        self.bg = None if bg is None else np.ascontiguousarray(bg, dtype=np.float64)
+        self.g = OrBgrid(cells[0], cells[1], cells[2], lengths[0], lengths[1], lengths[2],
+                         gamma, mu0, proton_mass, (ctypes.c_int * 3)(*[int(bool(p)) for p in periodic]),
+                         self.ctype.ctypes.data_as(_bp), self.finfo.ctypes.data_as(_bp),
+                         self.vtype.ctypes.data_as(_bp),
+                         None if self.bg is None else _ptr(self.bg))
+        iv = np.ascontiguousarray(in_value, dtype=np.uint8)
+        ic = np.ascontiguousarray(in_copy, dtype=np.uint8)
+        self.status = lib().or_bnd_classify(ctypes.byref(self.g), iv.ctypes.data_as(_ubp),
+                                            ic.ctypes.data_as(_ubp),
+                                            self.vtype.ctypes.data_as(_bp),
+                                            self.ctype.ctypes.data_as(_bp),
+                                            self.finfo.ctypes.data_as(_bp))
+
+    def apply(self, state, val_mask, n_val, v_val, p_val, b_val):
+        assert state.flags.c_contiguous and state.shape == (14, self.n)
+        m = np.ascontiguousarray(val_mask, dtype=np.uint8)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (n_val, v_val, p_val, b_val)]
+        return lib().or_bnd_apply(ctypes.byref(self.g), m.ctypes.data_as(_ubp),
+                                  *[_ptr(a) for a in arrs], _ptr(state))
+
+    def sync(self, state):
+        lib().or_bnd_sync(ctypes.byref(self.g), _ptr(state))
+
+    def step(self, solver, cfl, max_dt, substep_min, substep_max, state, maxv):
+        """one substepped timestep in place; returns (status, dt, largest period)"""
+        assert state.flags.c_contiguous and state.shape == (14, self.n)
+        n = self.n
+        lo = np.ascontiguousarray(np.broadcast_to(substep_min, (n,)), dtype=np.int32)
+        hi = np.ascontiguousarray(np.broadcast_to(substep_max, (n,)), dtype=np.int32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        dt = ctypes.c_double(0)
+        mp = ctypes.c_int(0)
+        bad = ctypes.c_long(-1)
+        st = lib().or_bnd_step(ctypes.byref(self.g), solver, cfl, max_dt, lo.ctypes.data_as(ip),
+                               hi.ctypes.data_as(ip), _ptr(state), _ptr(maxv), ctypes.byref(dt),
+                               ctypes.byref(mp), ctypes.byref(bad))
+        return st, dt.value, mp.value
+
+
+def run_bounded_config(cfg, max_steps=1 << 40, check_types=True):
+    """run_config for a configuration with boundaries: the cell and face
+    types from the oracle's classification (checked against the product's
+    when check_types), apply_boundaries + sync at the start, the final
+    init, then steps each followed by apply_boundaries at the new time and
+    sync_magnetic_field.  Returns (status, steps, end time, periods, state)."""
+    S, bg, types = cfg.bounded_initial_state()
+    iv, ic = cfg.boundary_masks()
+    G = BoundedGrid(cfg.cells, cfg.cell_length, cfg.adiabatic_index, cfg.vacuum_permeability,
+                    cfg.proton_mass, cfg.periodic, iv, ic, bg)
+    if G.status:
+        return G.status, 0, cfg.time_start, [], S
+    if check_types:
+        np.testing.assert_array_equal(G.vtype, types[0])
+        np.testing.assert_array_equal(G.ctype, types[1])
+        np.testing.assert_array_equal(G.finfo, types[2])
+    sid = SOLVER_IDS[cfg.solver]
+    maxv = -np.ones((3, S.shape[1]))
+    t = cfg.time_start
+    t_end = cfg.time_start + cfg.time_length
+    st = G.apply(S, *cfg.boundary_values(t))
+    if st:
+        return st, 0, t, [], S
+    G.sync(S)
+    lo, hi = cfg.substep_options(t)
+    st, dt, mp = G.step(sid, cfg.time_step_factor, 0.0, lo, hi, S, maxv)
+    steps = 0
+    periods = []
+    while st == 0 and t < t_end and steps < max_steps:
+        lo, hi = cfg.substep_options(t)
+        st, dt, mp = G.step(sid, cfg.time_step_factor, t_end - t, lo, hi, S, maxv)
+        if st:
+            break
+        t += dt
+        st = G.apply(S, *cfg.boundary_values(t))
+        if st:
+            break
+        G.sync(S)
+        steps += 1
+        periods.append(mp)
+    return st, steps, t, periods, S

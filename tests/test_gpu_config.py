@@ -14,7 +14,9 @@ from pamhd_amd import config, dc
 pytestmark = pytest.mark.gpu
 
 CFG_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config")
-CONFIGS = sorted(f[:-5] for f in os.listdir(CFG_DIR) if not f.startswith("two_pop"))
+CONFIGS = sorted(f[:-5] for f in os.listdir(CFG_DIR)
+                 if not f.startswith(("two_pop", "bow_shock"))
+                 and not config.load(os.path.join(CFG_DIR, f)).bounded)
 # whole runs, but at most this many steps of the 3-D Kelvin-Helmholtz case
 MAX_STEPS = {"kelvin_helmholtz_3d_roe_athena": 60}
 
@@ -45,15 +47,15 @@ def test_config_runs_bit_exact(gpu, name):
 def test_time_dependent_substep_options(gpu):
     """substep-max as an expression of t, set again before every step"""
     import json
-    obj = json.load(open(os.path.join(CFG_DIR, "blast_wave_2d_hll_athena.json")))
-    obj["substep-max"] = "t < 0.2 ? (x < 0 ? 1 : 0) : 3"
-    obj["time-length"] = 0.5
+    obj = json.load(open(os.path.join(CFG_DIR, "blast_wave_3d_rusanov.json")))
+    obj["substep-max"] = "t < 0.01 ? 0 : 999"
+    obj["time-length"] = 0.1
     cfg = config.load(obj)
     assert cfg.substeps_depend_on_time()
     state, bg = cfg.initial_state()
     st, steps_o, t_o, periods = O.run_config(cfg, state, bg)
-    # periods of at most 2 while substep-max is 1, then up to 4
-    assert st == 0 and periods[0] == 2 and max(periods) == 4
+    # the global step while substep-max is 0, then periods up to 4
+    assert st == 0 and periods[0] == 1 and max(periods) == 4
     res = config.run(cfg)
     assert res["steps"] == steps_o and res["time"] == t_o
     np.testing.assert_array_equal(res["state"], state)
