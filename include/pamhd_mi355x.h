@@ -234,6 +234,42 @@ int pamhd_set_substepping(pamhd_handle* h, int substep_min, int substep_max);
    caller sets them before each step.  Copies the arrays (synchronous). */
 int pamhd_set_substepping_cells(pamhd_handle* h, const int* substep_min, const int* substep_max);
 
+/*
+Uniform grids with value and copy boundaries and closed dimensions
+(tests/mhd/test.cpp with source/boundaries/ and source/mhd/boundaries.hpp).
+The caller classifies the cells (Boundaries::classify, boundaries.hpp:
+100-226; set_solver_info, mhd/boundaries.hpp:67-245; classify_faces,
+:1039-1217) and passes, for n cells in x-fastest order: var_type (4 n:
+number density, velocity, pressure, magnetic field; 1 normal, 2 not
+solved, 3 value cell, 4 copy cell), cell_type (n: 1, 0, -1) and face_type
+(6 n, faces -x, +x, -y, +y, -z, +z of each cell: 1, 0, -1); grid->periodic
+gives the periodic dimensions, the others end.  FP64; the state holds 14
+doubles per cell (rho, m (3), E, Vol_B (3), the cell's own copies of its
+six faces' B), read and written by pamhd_upload_state /
+pamhd_download_state.  pamhd_timestep is timestep() with per-cell
+substepping over the cell types (pamhd_set_substepping* give the options;
+(0, 0) by default: the global step); pamhd_apply_boundaries and the fused
+pamhd_step apply the boundaries (apply_boundaries, mhd/boundaries.hpp:
+1261-1297) and sync_magnetic_field.  Divergence cleaning, the adjoint,
+autotuning and device-pointer state access are not available on such
+grids (PAMHD_E_UNSUPPORTED).
+*/
+int pamhd_create_bounded(const pamhd_grid_desc* grid, const pamhd_physics* physics,
+	const signed char* var_type, const signed char* cell_type, const signed char* face_type,
+	pamhd_handle** out);
+
+/* The value boundaries for the next pamhd_apply_boundaries / pamhd_step,
+   resolved per cell (the last boundary covering a cell wins; the caller
+   evaluates time-dependent values at the time the boundaries are applied,
+   Value_Boundary::get_data, source/boundaries/value_boundary.hpp): mask
+   (4 n, 1 where a variable has a value boundary), number density (n, the
+   value before the proton mass factor), velocity (3 n), pressure (n) and
+   face field (6 n, the component normal to each face at its centre, used
+   on boundary faces).  Synchronous copies. */
+int pamhd_set_boundary_values(pamhd_handle* h, const unsigned char* mask,
+	const double* number_density, const double* velocity, const double* pressure,
+	const double* face_field);
+
 /* Largest period (substeps) of the last step: timestep()'s second return
    value (1 without substepping). */
 int pamhd_last_max_substep(pamhd_handle* h, int* max_substep);

@@ -710,13 +710,33 @@ class Config:
         return any(isinstance(v, Expression) and "t" in v.variables()
                    for v in (self.substep_min, self.substep_max))
 
+    def boundaries_depend_on_time(self):
+        """a value boundary with more than one time stamp or an expression
+        of t"""
+        for var in self._variables():
+            for _, stamps, vals in var.value_boundaries:
+                if len(stamps) > 1 or any("t" in v.needs() for v in vals):
+                    return True
+        return False
+
     def make_grid(self, precision="fp64"):
         """a solver.Grid holding the initial state (with the background
-        field and the substep options at time-start)"""
+        field and the substep options at time-start); a grid with
+        boundaries has them applied and its faces synced
+        (test.cpp:446-457)"""
         from .solver import Grid
-        state, bg = self.initial_state()
-        g = Grid(self.cells, self.cell_length, self.adiabatic_index, self.vacuum_permeability,
-                 self.proton_mass, precision=precision, start=self.start)
+        if self.bounded:
+            if precision != "fp64":
+                raise ConfigError("grids with boundaries run in FP64")
+            state, bg, types = self.bounded_initial_state()
+            g = Grid(self.cells, self.cell_length, self.adiabatic_index,
+                     self.vacuum_permeability, self.proton_mass, start=self.start,
+                     periodic=self.periodic, types=types)
+        else:
+            state, bg = self.initial_state()
+            g = Grid(self.cells, self.cell_length, self.adiabatic_index,
+                     self.vacuum_permeability, self.proton_mass, precision=precision,
+                     start=self.start)
         try:
             g.upload(state)
             if bg is not None:
@@ -724,6 +744,9 @@ class Config:
             if precision == "fp64":
                 lo, hi = self.substep_options(self.time_start)
                 g.set_substepping(lo, hi)
+            if self.bounded:
+                g.set_boundary_values(*self.boundary_values(self.time_start))
+                g.apply_boundaries(self.time_start)
         except Exception:
             g.close()
             raise
@@ -781,6 +804,7 @@ def run(cfg, precision="fp64", output_directory=None, max_steps=1 << 62, grid=No
             saves.append(path)
 
         per_step_options = precision == "fp64" and cfg.substeps_depend_on_time()
+        timed_boundaries = cfg.bounded and cfg.boundaries_depend_on_time()
         # the state already holds apply_boundaries' refit of the
         # initialisation, so the loop starts at the final init (pamhd_run
         # would refit once more)
@@ -793,8 +817,15 @@ def run(cfg, precision="fp64", output_directory=None, max_steps=1 << 62, grid=No
             step += 1
             if per_step_options:
                 g.set_substepping(*cfg.substep_options(t))
-            dt = g.step(cfg.solver, t_end - t, cfg.time_step_factor)
-            t += dt
+            if timed_boundaries:
+                # the boundaries take their values at the step's end time
+                dt = g.timestep(cfg.solver, t_end - t, cfg.time_step_factor)
+                t += dt
+                g.set_boundary_values(*cfg.boundary_values(t))
+                g.apply_boundaries(t)
+            else:
+                dt = g.step(cfg.solver, t_end - t, cfg.time_step_factor)
+                t += dt
             if (cfg.save_n >= 0 and t >= t_end) or (cfg.save_n > 0 and t >= next_save):
                 if cfg.save_n > 0 and next_save <= t:
                     next_save += cfg.save_n * math.ceil(max(cfg.save_n, t - next_save) / cfg.save_n)

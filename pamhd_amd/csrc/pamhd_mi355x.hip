@@ -563,6 +563,30 @@ struct Nb {
 	}
 };
 
+/* the same over a grid with closed dimensions: a coordinate of -1 (beyond
+   a closed end) names no cell, and its flux terms are those of a zero
+   flux */
+struct EdgeTermsB {
+	const double* P;
+	long long n;
+	double dx, dy, dz;
+	__device__ __forceinline__ double at(int q, long long c) const { return c < 0 ? 0.0 : P[q * n + c]; }
+	__device__ __forceinline__ double ezx(long long c) const { return +(at(6, c) * dz / 4); }
+	__device__ __forceinline__ double eyx(long long c) const { return -(at(7, c) * dy / 4); }
+	__device__ __forceinline__ double exy(long long c) const { return +(at(15, c) * dx / 4); }
+	__device__ __forceinline__ double ezy(long long c) const { return -(at(13, c) * dz / 4); }
+	__device__ __forceinline__ double exz(long long c) const { return -(at(22, c) * dx / 4); }
+	__device__ __forceinline__ double eyz(long long c) const { return +(at(21, c) * dy / 4); }
+};
+
+struct NbB {
+	int nx, ny;
+	__device__ __forceinline__ long long operator()(int i, int j, int k) const
+	{
+		return (i < 0 || j < 0 || k < 0) ? -1 : i + (long long)nx * (j + (long long)ny * k);
+	}
+};
+
 /*
 The 12 edge terms of one face change, summed in a fixed order; the
 arguments are the terms in that order (see dBx_at, dBy_at, dBz_at).
@@ -593,7 +617,7 @@ __device__ __forceinline__ double dBz_sum(double a0, double a1, double a2, doubl
 }
 
 /* dB of the x-face owned by (i, j, k); im = i-1, jm/jp = j-+1, km/kp = k-+1 */
-template <class PT> __device__ __forceinline__ double dBx_at(const EdgeTerms<PT>& E, const Nb& L,
+template <class ET, class NB> __device__ __forceinline__ double dBx_at(const ET& E, const NB& L,
 	int i, int im, int j, int jm, int jp, int k, int km, int kp)
 {
 	return dBx_sum(
@@ -602,7 +626,7 @@ template <class PT> __device__ __forceinline__ double dBx_at(const EdgeTerms<PT>
 		E.eyz(L(i, j, kp)), E.eyz(L(im, j, kp)), E.eyz(L(i, j, k)), E.eyz(L(im, j, k)));
 }
 
-template <class PT> __device__ __forceinline__ double dBy_at(const EdgeTerms<PT>& E, const Nb& L,
+template <class ET, class NB> __device__ __forceinline__ double dBy_at(const ET& E, const NB& L,
 	int i, int im, int ip, int j, int jm, int k, int km, int kp)
 {
 	return dBy_sum(
@@ -611,7 +635,7 @@ template <class PT> __device__ __forceinline__ double dBy_at(const EdgeTerms<PT>
 		E.ezx(L(ip, j, k)), E.ezx(L(i, j, k)), E.ezx(L(ip, jm, k)), E.ezx(L(i, jm, k)));
 }
 
-template <class PT> __device__ __forceinline__ double dBz_at(const EdgeTerms<PT>& E, const Nb& L,
+template <class ET, class NB> __device__ __forceinline__ double dBz_at(const ET& E, const NB& L,
 	int i, int im, int ip, int j, int jm, int jp, int k, int km)
 {
 	return dBz_sum(
@@ -1087,26 +1111,57 @@ struct SubstepDev {
 	StepStatus* host_status = nullptr;
 	int smax = 0;   /* uniform substep-max, or the largest per-cell one */
 	int last_max = 1;
+	/* grids with boundaries: S is the state itself; cell types (1 normal,
+	   0 boundary, -1 not solved), face types of the six faces, the
+	   variables' boundary kinds (1 normal, 2 not solved, 3 value, 4 copy),
+	   periodicity, and the value boundaries' values (mask 4 n; number
+	   density n, velocity 3 n, pressure n, face field 6 n) */
+	bool bounded = false;
+	const signed char* ct = nullptr;
+	const signed char* fi = nullptr;
+	const signed char* vt = nullptr;
+	int per[3] = {1, 1, 1};
+	unsigned char* bmask = nullptr;
+	double* bval = nullptr;
+	signed char* types = nullptr;   /* allocation behind ct, fi, vt */
 };
+
+/* index of a neighbour coordinate: periodic dimensions wrap (within one
+   length), closed ones give -1 beyond their ends */
+__device__ __forceinline__ int ss_coord(int v, int len, int per)
+{
+	if (v < 0) return per ? v + len : -1;
+	if (v >= len) return per ? v - len : -1;
+	return v;
+}
+
+__device__ __forceinline__ bool ss_active(const SubstepDev& d, long long c)
+{
+	return !d.ct || d.ct[c] >= 0;
+}
 
 __device__ __forceinline__ int cidx(const GridDev& g, int i, int j, int k)
 {
 	return i + g.nx * (j + g.ny * k);
 }
 
-/* the 18 face and edge neighbours of (i, j, k) */
+/* the face and edge neighbours of (i, j, k) that exist and are of type
+   >= 0, in the order of the neighbourhood */
 template <class F> __device__ __forceinline__ void for_face_edge_neighbours(const GridDev& g,
-	int i, int j, int k, F&& f)
+	const SubstepDev& d, int i, int j, int k, F&& f)
 {
 	for (int dk = -1; dk <= 1; dk++)
 	for (int dj = -1; dj <= 1; dj++)
 	for (int di = -1; di <= 1; di++) {
 		const int nz_ = (di != 0) + (dj != 0) + (dk != 0);
 		if (nz_ == 0 || nz_ == 3) continue;
-		const int a = di < 0 ? wrap_dec(i, g.nx) : di > 0 ? wrap_inc(i, g.nx) : i;
-		const int b = dj < 0 ? wrap_dec(j, g.ny) : dj > 0 ? wrap_inc(j, g.ny) : j;
-		const int c = dk < 0 ? wrap_dec(k, g.nz) : dk > 0 ? wrap_inc(k, g.nz) : k;
-		f(cidx(g, a, b, c));
+		const int a = ss_coord(i + di, g.nx, d.per[0]);
+		const int b = ss_coord(j + dj, g.ny, d.per[1]);
+		const int c = ss_coord(k + dk, g.nz, d.per[2]);
+		if (a < 0 || b < 0 || c < 0) continue;
+		const int id = cidx(g, a, b, c);
+		if (!ss_active(d, id)) continue;
+		f(id);
 	}
 }
 
@@ -1131,15 +1186,30 @@ __global__ __launch_bounds__(BX * BY) void ss_ts_kernel(GridDev g, SubstepDev d,
 	double t = DBL_MAX, r = DBL_MAX;
 	if (i < g.nx && j < g.ny) {
 		const int c = cidx(g, i, j, k);
-		const double len[3] = {g.dx, g.dy, g.dz};
-		const int nb[3] = {cidx(g, wrap_inc(i, g.nx), j, k), cidx(g, i, wrap_inc(j, g.ny), k),
-			cidx(g, i, j, wrap_inc(k, g.nz))};
-		for (int q = 0; q < 3; q++) {
-			const double v[2] = {d.maxv[(long long)q * g.n + c], d.maxv[(long long)q * g.n + nb[q]]};
-			for (int sd = 0; sd < 2; sd++) t = v[sd] <= 0 ? 0.0 : smin(cfl * len[q] / v[sd], t);
+		if (ss_active(d, c)) {
+			const double len[3] = {g.dx, g.dy, g.dz};
+			const int id[3] = {i, j, k};
+			const int N[3] = {g.nx, g.ny, g.nz};
+			for (int q = 0; q < 3; q++) {
+				/* the faces on the cell's two sides, where the neighbour
+				   exists and is of type >= 0 */
+				int lo[3] = {i, j, k}, hi[3] = {i, j, k};
+				lo[q] = ss_coord(id[q] - 1, N[q], d.per[q]);
+				hi[q] = ss_coord(id[q] + 1, N[q], d.per[q]);
+				const int bl = lo[q] < 0 ? -1 : cidx(g, lo[0], lo[1], lo[2]);
+				const int bh = hi[q] < 0 ? -1 : cidx(g, hi[0], hi[1], hi[2]);
+				if (bl >= 0 && ss_active(d, bl)) {
+					const double v = d.maxv[(long long)q * g.n + c];
+					t = v <= 0 ? 0.0 : smin(cfl * len[q] / v, t);
+				}
+				if (bh >= 0 && ss_active(d, bh)) {
+					const double v = d.maxv[(long long)q * g.n + bh];
+					t = v <= 0 ? 0.0 : smin(cfl * len[q] / v, t);
+				}
+			}
+			r = d.opt ? t / (double)(1 << d.opt[c]) : t;
 		}
 		d.ts[c] = t;
-		r = d.opt ? t / (double)(1 << d.opt[c]) : t;
 	}
 	for (int off = 32; off > 0; off >>= 1) {
 		t = smin(t, __shfl_xor(t, off, 64));
@@ -1167,7 +1237,7 @@ __global__ __launch_bounds__(BX * BY) void ss_level_kernel(GridDev g, SubstepDev
 	if (i < g.nx && j < g.ny) {
 		const int c = cidx(g, i, j, k);
 		const double t = d.ts[c];
-		if (!all_zero && t > 0) {
+		if (ss_active(d, c) && !all_zero && t > 0) {
 			const double x = smin(t, max_dt) / ret;
 			int step = (__builtin_isfinite(x) && x > 0) ? ilogb(x) : 0;
 			if (step < 0) step = 0;
@@ -1188,14 +1258,16 @@ __global__ __launch_bounds__(BX * BY) void ss_level_kernel(GridDev g, SubstepDev
 }
 
 /* one Jacobi pass of restrict_substepping_period, substepping.hpp:99-191 */
-__global__ __launch_bounds__(BX * BY) void ss_restrict_kernel(GridDev g, const int* __restrict__ in,
-	int* __restrict__ out)
+__global__ __launch_bounds__(BX * BY) void ss_restrict_kernel(GridDev g, SubstepDev d,
+	const int* __restrict__ in, int* __restrict__ out)
 {
 	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
 	if (i >= g.nx || j >= g.ny) return;
 	const int c = cidx(g, i, j, k);
 	int v = in[c];
-	for_face_edge_neighbours(g, i, j, k, [&](int nb) { if (v > in[nb] + 1) v = in[nb] + 1; });
+	if (ss_active(d, c)) {
+		for_face_edge_neighbours(g, d, i, j, k, [&](int nb) { if (v > in[nb] + 1) v = in[nb] + 1; });
+	}
 	out[c] = v;
 }
 
@@ -1209,10 +1281,11 @@ __global__ __launch_bounds__(BX * BY) void ss_period_kernel(GridDev g, SubstepDe
 	if (i < g.nx && j < g.ny) {
 		const int c = cidx(g, i, j, k);
 		int lo = e[c];
-		for_face_edge_neighbours(g, i, j, k, [&](int nb) { if (e[nb] < lo) lo = e[nb]; });
+		const bool act = ss_active(d, c);
+		if (act) for_face_edge_neighbours(g, d, i, j, k, [&](int nb) { if (e[nb] < lo) lo = e[nb]; });
 		d.period[c] = 1 << e[c];
 		d.minsub[c] = 1 << lo;
-		p = 1ull << e[c];
+		p = act ? 1ull << e[c] : 0ull;
 	}
 	for (int off = 32; off > 0; off >>= 1) {
 		const unsigned long long o = __shfl_xor(p, off, 64);
@@ -1249,9 +1322,17 @@ __global__ __launch_bounds__(BX * BY) void ss_flux_kernel(GridDev g, SubstepDev 
 	if (i >= g.nx || j >= g.ny) return;
 	const long long n = g.n;
 	const int c = cidx(g, i, j, k);
-	const int o = cidx(g, D == 0 ? wrap_dec(i, g.nx) : i, D == 1 ? wrap_dec(j, g.ny) : j,
-		D == 2 ? wrap_dec(k, g.nz) : k);
+	const int oi = D == 0 ? ss_coord(i - 1, g.nx, d.per[0]) : i;
+	const int oj = D == 1 ? ss_coord(j - 1, g.ny, d.per[1]) : j;
+	const int ok = D == 2 ? ss_coord(k - 1, g.nz, d.per[2]) : k;
 	double* Pd = P + (long long)D * 8 * n;
+	/* a face between two cells of type >= 0, solved by its negative-side
+	   owner at the pace of the owner's smallest neighbourhood period */
+	const int o = (oi < 0 || oj < 0 || ok < 0) ? -1 : cidx(g, oi, oj, ok);
+	if (o < 0 || !ss_active(d, o) || !ss_active(d, c)) {
+		for (int q = 0; q < 8; q++) Pd[q * n + c] = 0;
+		return;
+	}
 	const int m = d.minsub[o];
 	if (sub % m != 0) {
 		for (int q = 0; q < 8; q++) Pd[q * n + c] = 0;
@@ -1303,11 +1384,11 @@ __global__ __launch_bounds__(BX * BY) void ss_update_kernel(GridDev g, SubstepDe
 	if (i >= g.nx || j >= g.ny) return;
 	const long long n = g.n;
 	const int c = cidx(g, i, j, k);
-	const int im = wrap_dec(i, g.nx), ip = wrap_inc(i, g.nx);
-	const int jm = wrap_dec(j, g.ny), jp = wrap_inc(j, g.ny);
-	const int km = wrap_dec(k, g.nz), kp = wrap_inc(k, g.nz);
-	const Nb L{g.nx, g.ny};
-	const EdgeTerms<double> E{P, n, g.dx, g.dy, g.dz};
+	const int im = ss_coord(i - 1, g.nx, d.per[0]), ip = ss_coord(i + 1, g.nx, d.per[0]);
+	const int jm = ss_coord(j - 1, g.ny, d.per[1]), jp = ss_coord(j + 1, g.ny, d.per[1]);
+	const int km = ss_coord(k - 1, g.nz, d.per[2]), kp = ss_coord(k + 1, g.nz, d.per[2]);
+	const NbB L{g.nx, g.ny};
+	const EdgeTermsB E{P, n, g.dx, g.dy, g.dz};
 	const double db[6] = {
 		dBx_at(E, L, i, im, j, jm, jp, k, km, kp),
 		dBx_at(E, L, ip, i, j, jm, jp, k, km, kp),
@@ -1317,17 +1398,43 @@ __global__ __launch_bounds__(BX * BY) void ss_update_kernel(GridDev g, SubstepDe
 		dBz_at(E, L, i, im, ip, j, jm, jp, kp, k)};
 	double fd[6];
 	for (int f = 0; f < 6; f++) fd[f] = d.fdb[f * n + c] + db[f];
-	const int cp[3] = {cidx(g, ip, j, k), cidx(g, i, jp, k), cidx(g, i, j, kp)};
+	const long long cp[3] = {L(ip, j, k), L(i, jp, k), L(i, j, kp)};
 	double sl[6][8];
 	for (int q3 = 0; q3 < 3; q3++)
 	for (int q = 0; q < 8; q++) {
 		sl[2 * q3][q] = d.slot[((2 * q3) * 8 + q) * n + c] + P[(q3 * 8 + q) * n + c];
-		sl[2 * q3 + 1][q] = d.slot[((2 * q3 + 1) * 8 + q) * n + c] + P[(q3 * 8 + q) * n + cp[q3]];
+		sl[2 * q3 + 1][q] = d.slot[((2 * q3 + 1) * 8 + q) * n + c];
+		if (cp[q3] >= 0) sl[2 * q3 + 1][q] += P[(q3 * 8 + q) * n + cp[q3]];
 	}
-	if (sub % d.period[c] != 0) {
+	const int ct = d.ct ? d.ct[c] : 1;
+	const bool due = sub % d.period[c] == 0;
+	if (ct > 0 && !due) {
+		/* normal cells accumulate until their period ends */
 		for (int f = 0; f < 6; f++) d.fdb[f * n + c] = fd[f];
 		for (int dir = 0; dir < 6; dir++)
 		for (int q = 0; q < 8; q++) d.slot[(dir * 8 + q) * n + c] = sl[dir][q];
+		return;
+	}
+	for (int f = 0; f < 6; f++) d.fdb[f * n + c] = 0;
+	for (int dir = 0; dir < 6; dir++)
+	for (int q = 0; q < 8; q++) d.slot[(dir * 8 + q) * n + c] = 0;
+	if (ct <= 0) {
+		/* boundary cells: update_vol_B at their period (constant
+		   pressure, own face copies); cells not solved: nothing */
+		if (ct == 0 && due) {
+			const double rho = d.S[c], m[3] = {d.S[n + c], d.S[2 * n + c], d.S[3 * n + c]};
+			if (rho > 0) {
+				double En = d.S[4 * n + c];
+				const double vb[3] = {d.S[5 * n + c], d.S[6 * n + c], d.S[7 * n + c]};
+				const double vb_new[3] = {
+					0.5 * (d.S[8 * n + c] + d.S[9 * n + c]),
+					0.5 * (d.S[10 * n + c] + d.S[11 * n + c]),
+					0.5 * (d.S[12 * n + c] + d.S[13 * n + c])};
+				refit_energy(rho, m, En, vb, vb_new, g.ph);
+				for (int q = 0; q < 3; q++) d.S[(5 + q) * n + c] = vb_new[q];
+				d.S[4 * n + c] = En;
+			}
+		}
 		return;
 	}
 	const double len[3] = {g.dx, g.dy, g.dz};
@@ -1349,12 +1456,7 @@ __global__ __launch_bounds__(BX * BY) void ss_update_kernel(GridDev g, SubstepDe
 		fc[2 * q3] = d.S[(8 + 2 * q3) * n + c] + fd[2 * q3] / area[q3];
 		fc[2 * q3 + 1] = d.S[(9 + 2 * q3) * n + c] + fd[2 * q3 + 1] / area[q3];
 	}
-	for (int f = 0; f < 6; f++) {
-		d.S[(8 + f) * n + c] = fc[f];
-		d.fdb[f * n + c] = 0;
-	}
-	for (int dir = 0; dir < 6; dir++)
-	for (int q = 0; q < 8; q++) d.slot[(dir * 8 + q) * n + c] = 0;
+	for (int f = 0; f < 6; f++) d.S[(8 + f) * n + c] = fc[f];
 	d.S[c] = rho;
 	for (int q = 0; q < 3; q++) d.S[(1 + q) * n + c] = m[q];
 	if (rho > 0) {
@@ -1376,8 +1478,14 @@ __global__ __launch_bounds__(BX * BY) void ss_sync_kernel(GridDev g, SubstepDev 
 	if (i >= g.nx || j >= g.ny) return;
 	const long long n = g.n;
 	const int b = cidx(g, i, j, k);
-	const int a = cidx(g, D == 0 ? wrap_dec(i, g.nx) : i, D == 1 ? wrap_dec(j, g.ny) : j,
-		D == 2 ? wrap_dec(k, g.nz) : k);
+	const int ai = D == 0 ? ss_coord(i - 1, g.nx, d.per[0]) : i;
+	const int aj = D == 1 ? ss_coord(j - 1, g.ny, d.per[1]) : j;
+	const int ak = D == 2 ? ss_coord(k - 1, g.nz, d.per[2]) : k;
+	if (ai < 0 || aj < 0 || ak < 0) return;
+	const int a = cidx(g, ai, aj, ak);
+	if (!ss_active(d, a) || !ss_active(d, b)) return;
+	/* a's + copy and b's - copy; the visit from the smaller index first,
+	   from the -D side when a cell is its own neighbour */
 	double pa = d.S[(9 + 2 * D) * n + a], pb = d.S[(8 + 2 * D) * n + b];
 	if (a < b) {
 		double diff = pa - pb; pa -= diff / 2; pb += diff / 2;
@@ -1400,13 +1508,21 @@ __global__ __launch_bounds__(BX * BY) void ss_final_kernel(GridDev g, SubstepDev
 	const double rho = d.S[c], m[3] = {d.S[n + c], d.S[2 * n + c], d.S[3 * n + c]};
 	double En = d.S[4 * n + c];
 	const double vb[3] = {d.S[5 * n + c], d.S[6 * n + c], d.S[7 * n + c]};
-	if (rho > 0) {
-		const double vb_new[3] = {
-			0.5 * (d.S[8 * n + c] + d.S[9 * n + c]),
-			0.5 * (d.S[10 * n + c] + d.S[11 * n + c]),
-			0.5 * (d.S[12 * n + c] + d.S[13 * n + c])};
-		refit_energy(rho, m, En, vb, vb_new, g.ph);
+	const double vb_new[3] = {
+		0.5 * (d.S[8 * n + c] + d.S[9 * n + c]),
+		0.5 * (d.S[10 * n + c] + d.S[11 * n + c]),
+		0.5 * (d.S[12 * n + c] + d.S[13 * n + c])};
+	if (d.bounded) {
+		/* the state stays in the working layout: update_vol_B(0) of the
+		   cells of type >= 0 in place */
+		if (ss_active(d, c) && rho > 0) {
+			refit_energy(rho, m, En, vb, vb_new, g.ph);
+			for (int q = 0; q < 3; q++) d.S[(5 + q) * n + c] = vb_new[q];
+			d.S[4 * n + c] = En;
+		}
+		return;
 	}
+	if (rho > 0) refit_energy(rho, m, En, vb, vb_new, g.ph);
 	s[c] = rho;
 	for (int q = 0; q < 3; q++) s[(1 + q) * n + c] = m[q];
 	s[4 * n + c] = En;
@@ -1701,6 +1817,210 @@ int launch_step_body(pamhd_handle* h, int solver, hipStream_t st, StepStatus* ne
 	return PAMHD_OK;
 }
 
+/*
+apply_boundaries on a grid with boundaries (source/mhd/boundaries.hpp:
+1261-1297): value and copy boundaries of number density, velocity and
+pressure (apply_fluid_boundaries, :738-1026), of the face field
+(apply_magnetic_field_boundaries_staggered, :250-640, uniform grids), then
+update_vol_B(0) at constant pressure of every cell of type >= 0.  One
+launch per stage; a stage writes only boundary cells and reads only its
+own cell and normal cells, which no stage writes.  Neighbours are visited
+in the order of the neighbourhood (offsets -1..1, z slowest).
+*/
+struct BndNb { int id; int fn; int en0, en1, en2; };
+
+template <class F> __device__ __forceinline__ void for_neighbours(const GridDev& g, const SubstepDev& d,
+	int i, int j, int k, F&& f)
+{
+	for (int oz = -1; oz <= 1; oz++)
+	for (int oy = -1; oy <= 1; oy++)
+	for (int ox = -1; ox <= 1; ox++) {
+		const int nzc = (ox != 0) + (oy != 0) + (oz != 0);
+		if (nzc == 0) continue;
+		const int a = ss_coord(i + ox, g.nx, d.per[0]);
+		const int b = ss_coord(j + oy, g.ny, d.per[1]);
+		const int c = ss_coord(k + oz, g.nz, d.per[2]);
+		if (a < 0 || b < 0 || c < 0) continue;
+		BndNb nb{cidx(g, a, b, c), 0, -1, 0, 0};
+		if (nzc == 1) {
+			nb.fn = ox ? ox : (oy ? 2 * oy : 3 * oz);
+		} else if (nzc == 2) {
+			/* edge: its dimension, then the offsets in the lower and the
+			   higher other dimension */
+			if (!ox) { nb.en0 = 0; nb.en1 = oy; nb.en2 = oz; }
+			else if (!oy) { nb.en0 = 1; nb.en1 = ox; nb.en2 = oz; }
+			else { nb.en0 = 2; nb.en1 = ox; nb.en2 = oy; }
+		}
+		f(nb);
+	}
+}
+
+__device__ __forceinline__ int face_of(int dir) { return 2 * ((dir < 0 ? -dir : dir) - 1) + (dir > 0); }
+
+__device__ __forceinline__ void bnd_velocity(const SubstepDev& d, long long n, int c, double v[3])
+{
+	if (d.S[c] > 0) {
+		const double inv = 1 / d.S[c];
+		for (int q = 0; q < 3; q++) v[q] = d.S[(1 + q) * n + c] * inv;
+	} else {
+		v[0] = v[1] = v[2] = 0;
+	}
+}
+
+__device__ __forceinline__ double bnd_pressure(const SubstepDev& d, long long n, int c, const Phys& ph)
+{
+	double U[8];
+	for (int q = 0; q < 8; q++) U[q] = d.S[q * n + c];
+	return pressure64(U, ph.gm1, ph.half_inv_mu0);
+}
+
+/* STAGE 0 number density, 1 velocity, 2 pressure */
+template <int STAGE>
+__global__ __launch_bounds__(BX * BY) void bnd_fluid_kernel(GridDev g, SubstepDev d, double proton_mass)
+{
+	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	const int c = cidx(g, i, j, k);
+	if (d.bmask[STAGE * n + c]) {
+		if (STAGE == 0) {
+			d.S[c] = proton_mass * d.bval[c];
+		} else if (STAGE == 1) {
+			for (int q = 0; q < 3; q++) d.S[(1 + q) * n + c] = d.bval[(1 + q) * n + c] * d.S[c];
+		} else {
+			if (d.S[c] <= 0) { record_error(d.status, PAMHD_E_DENSITY, 2 * n + c); return; }
+			double v[3];
+			bnd_velocity(d, n, c, v);
+			const double B[3] = {d.S[5 * n + c], d.S[6 * n + c], d.S[7 * n + c]};
+			d.S[4 * n + c] = total_energy64(d.S[c], v, d.bval[4 * n + c], B, g.ph.gm1, g.ph.half_inv_mu0);
+		}
+	}
+	if (d.vt[STAGE * n + c] != 4) return;
+	/* copy: the average over normal face neighbours, else over normal
+	   edge neighbours */
+	double fs[3] = {0, 0, 0}, es[3] = {0, 0, 0};
+	unsigned long long nf = 0, ne = 0;
+	bool bad = false;
+	for_neighbours(g, d, i, j, k, [&](const BndNb& b) {
+		if (d.ct[b.id] < 1) return;
+		double v[3] = {0, 0, 0};
+		if (b.fn != 0 || b.en0 >= 0) {
+			if (STAGE == 0) v[0] = d.S[b.id];
+			else if (STAGE == 1) bnd_velocity(d, n, b.id, v);
+			else {
+				if (d.S[b.id] <= 0) bad = true;
+				else v[0] = bnd_pressure(d, n, b.id, g.ph);
+			}
+		}
+		if (b.fn != 0) {
+			for (int q = 0; q < 3; q++) fs[q] = fs[q] + v[q];
+			nf++;
+		}
+		if (b.en0 >= 0) {
+			for (int q = 0; q < 3; q++) es[q] = es[q] + v[q];
+			ne++;
+		}
+	});
+	if (bad) { record_error(d.status, PAMHD_E_DENSITY, 2 * n + c); return; }
+	if (nf == 0 && ne == 0) return;
+	const double* src = nf > 0 ? fs : es;
+	const unsigned long long cnt = nf > 0 ? nf : ne;
+	if (STAGE == 0) {
+		d.S[c] = src[0] / cnt;
+	} else if (STAGE == 1) {
+		/* mul(mul(Mas, sum), 1 / count): an integer quotient, 1 for one
+		   source and 0 for more */
+		const double f = (double)(1ull / cnt);
+		for (int q = 0; q < 3; q++) d.S[(1 + q) * n + c] = src[q] * d.S[c] * f;
+	} else {
+		if (d.S[c] <= 0) { record_error(d.status, PAMHD_E_DENSITY, 2 * n + c); return; }
+		double v[3];
+		bnd_velocity(d, n, c, v);
+		const double B[3] = {d.S[5 * n + c], d.S[6 * n + c], d.S[7 * n + c]};
+		d.S[4 * n + c] = total_energy64(d.S[c], v, src[0] / cnt, B, g.ph.gm1, g.ph.half_inv_mu0);
+	}
+}
+
+__global__ __launch_bounds__(BX * BY) void bnd_field_value_kernel(GridDev g, SubstepDev d)
+{
+	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	const int c = cidx(g, i, j, k);
+	if (!d.bmask[3 * n + c]) return;
+	for (int f = 0; f < 6; f++) {
+		if (d.fi[f * n + c] == 0) d.S[(8 + f) * n + c] = d.bval[(5 + f) * n + c];
+	}
+}
+
+__global__ __launch_bounds__(BX * BY) void bnd_field_copy_kernel(GridDev g, SubstepDev d)
+{
+	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	const int c = cidx(g, i, j, k);
+	if (d.vt[3 * n + c] != 4) return;
+	double F[6];
+	int cnt[6] = {0, 0, 0, 0, 0, 0};
+	signed char fi[6];
+	for (int f = 0; f < 6; f++) {
+		fi[f] = d.fi[f * n + c];
+		F[f] = d.S[(8 + f) * n + c];
+	}
+	for (int q3 = 0; q3 < 3; q3++) {
+		for (int s = 0; s < 2; s++) if (fi[2 * q3 + s] < 1) F[2 * q3 + s] = 0;
+		if (fi[2 * q3] == 0 && fi[2 * q3 + 1] == 1) { F[2 * q3] += F[2 * q3 + 1]; cnt[2 * q3]++; }
+		if (fi[2 * q3] == 1 && fi[2 * q3 + 1] == 0) { F[2 * q3 + 1] += F[2 * q3]; cnt[2 * q3 + 1]++; }
+	}
+	for_neighbours(g, d, i, j, k, [&](const BndNb& b) {
+		if (b.fn == 0 && b.en0 < 0) return;
+		if (b.fn != 0) {
+			for (int dir = -3; dir <= 3; dir++) {
+				if (dir == 0) continue;
+				const int f = face_of(dir);
+				if (fi[f] == 0 && d.fi[f * n + b.id] == 1) { F[f] += d.S[(8 + f) * n + b.id]; cnt[f]++; }
+			}
+		}
+		if (b.en0 >= 0) {
+			const int d1 = b.en0 == 0 ? 1 : 0, d2 = b.en0 == 2 ? 1 : 2;
+			const int dirs[2] = {b.en1 * (d1 + 1), b.en2 * (d2 + 1)};
+			for (int r = 0; r < 2; r++) {
+				const int f = face_of(dirs[r]), fo = face_of(-dirs[r]);
+				if (fi[f] == 0 && d.fi[fo * n + b.id] == 1) { F[f] += d.S[(8 + fo) * n + b.id]; cnt[f]++; }
+			}
+		}
+	});
+	for (int f = 0; f < 6; f++) {
+		if (fi[f] == 0) {
+			if (cnt[f] == 0) { record_error(d.status, PAMHD_E_SOLVER, (long long)f * n + c); return; }
+			F[f] /= cnt[f];
+		}
+		d.S[(8 + f) * n + c] = F[f];
+	}
+}
+
+/* update_vol_B(0) with constant pressure of the cells of type >= 0 */
+__global__ __launch_bounds__(BX * BY) void bnd_refit_kernel(GridDev g, SubstepDev d)
+{
+	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	const int c = cidx(g, i, j, k);
+	if (!ss_active(d, c)) return;
+	const double rho = d.S[c];
+	if (rho <= 0) return;
+	const double m[3] = {d.S[n + c], d.S[2 * n + c], d.S[3 * n + c]};
+	double En = d.S[4 * n + c];
+	const double vb[3] = {d.S[5 * n + c], d.S[6 * n + c], d.S[7 * n + c]};
+	const double vb_new[3] = {
+		0.5 * (d.S[8 * n + c] + d.S[9 * n + c]),
+		0.5 * (d.S[10 * n + c] + d.S[11 * n + c]),
+		0.5 * (d.S[12 * n + c] + d.S[13 * n + c])};
+	refit_energy(rho, m, En, vb, vb_new, g.ph);
+	for (int q = 0; q < 3; q++) d.S[(5 + q) * n + c] = vb_new[q];
+	d.S[4 * n + c] = En;
+}
+
 void substep_free(SubstepDev* d)
 {
 	if (!d) return;
@@ -1714,6 +2034,9 @@ void substep_free(SubstepDev* d)
 	hipFree(d->period);
 	hipFree(d->minsub);
 	hipFree(d->opt_dev);
+	hipFree(d->types);
+	hipFree(d->bmask);
+	hipFree(d->bval);
 	hipFree(d->scal);
 	hipFree(d->sub_dt);
 	hipFree(d->status);
@@ -1770,6 +2093,24 @@ void ss_launch_flux(pamhd_handle* h, double* P, int sub, hipStream_t st)
 	ss_flux_kernel<SOLVER, 2><<<grid, block, 0, st>>>(h->g, *h->sub, h->bg, P, sub);
 }
 
+/* apply_boundaries and the main loop's sync_magnetic_field on a grid with
+   boundaries */
+void launch_boundaries(pamhd_handle* h, hipStream_t st)
+{
+	SubstepDev& d = *h->sub;
+	const dim3 grid = cell_grid(h->g), block(BX, BY, 1);
+	const double mp = h->phys.proton_mass;
+	bnd_fluid_kernel<0><<<grid, block, 0, st>>>(h->g, d, mp);
+	bnd_fluid_kernel<1><<<grid, block, 0, st>>>(h->g, d, mp);
+	bnd_fluid_kernel<2><<<grid, block, 0, st>>>(h->g, d, mp);
+	bnd_field_value_kernel<<<grid, block, 0, st>>>(h->g, d);
+	bnd_field_copy_kernel<<<grid, block, 0, st>>>(h->g, d);
+	bnd_refit_kernel<<<grid, block, 0, st>>>(h->g, d);
+	ss_sync_kernel<0><<<grid, block, 0, st>>>(h->g, d);
+	ss_sync_kernel<1><<<grid, block, 0, st>>>(h->g, d);
+	ss_sync_kernel<2><<<grid, block, 0, st>>>(h->g, d);
+}
+
 /* timestep() with per-cell substepping (and apply_boundaries' refit when
    refits == 3), in place on the current state */
 int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
@@ -1792,7 +2133,7 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 	const int passes = (int)d.host[2];
 	int cur = 0;
 	for (int r = 0; r < passes; r++) {
-		ss_restrict_kernel<<<grid, block, 0, st>>>(h->g, d.e[cur], d.e[1 - cur]);
+		ss_restrict_kernel<<<grid, block, 0, st>>>(h->g, d, d.e[cur], d.e[1 - cur]);
 		cur = 1 - cur;
 	}
 	ss_period_kernel<<<grid, block, 0, st>>>(h->g, d, d.e[cur]);
@@ -1805,7 +2146,7 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 	memcpy(&sub_dt, d.host + 1, sizeof(double));
 	if (maxp < 1) return fail(h, PAMHD_E_DEVICE, "substep periods not computed");
 
-	ss_expand_kernel<<<grid, block, 0, st>>>(h->g, h->state[h->cur], d);
+	if (!d.bounded) ss_expand_kernel<<<grid, block, 0, st>>>(h->g, h->state[h->cur], d);
 	double total = 0;
 	for (int sub = 1; sub <= maxp; sub++) {
 		total += sub_dt;
@@ -1822,7 +2163,10 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 	ss_sync_kernel<1><<<grid, block, 0, st>>>(h->g, d);
 	ss_sync_kernel<2><<<grid, block, 0, st>>>(h->g, d);
 	ss_final_kernel<<<grid, block, 0, st>>>(h->g, d, h->state[h->cur]);
-	if (refits == 3) refit_kernel<<<grid, block, 0, st>>>(h->g, h->state[h->cur]);
+	if (refits == 3) {
+		if (d.bounded) launch_boundaries(h, st);
+		else refit_kernel<<<grid, block, 0, st>>>(h->g, h->state[h->cur]);
+	}
 	HIP_TRY(h, hipGetLastError());
 	HIP_TRY(h, hipMemcpyAsync(d.host_status, d.status, sizeof(StepStatus), hipMemcpyDeviceToHost, st));
 	HIP_TRY(h, hipStreamSynchronize(st));
@@ -1852,6 +2196,10 @@ int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
 			if (e) return fail(h, e, err);
 		}
 		return PAMHD_OK;
+	}
+	if (h->sub && h->sub->bounded) {
+		if (fixed_dt) return fail(h, PAMHD_E_UNSUPPORTED, "fixed steps on a grid with boundaries");
+		return substep_step(h, solver, max_dt, cfl, refits, dt_taken, st);
 	}
 	if (h->sub && h->sub->smax > 0 && !fixed_dt) {
 		return substep_step(h, solver, max_dt, cfl, refits, dt_taken, st);
@@ -1963,6 +2311,65 @@ int pamhd_create_solar_wind_box(const pamhd_grid_desc* grid, const pamhd_physics
 	return PAMHD_OK;
 }
 
+int pamhd_create_bounded(const pamhd_grid_desc* grid, const pamhd_physics* physics,
+	const signed char* var_type, const signed char* cell_type, const signed char* face_type,
+	pamhd_handle** out)
+{
+	if (!grid || !physics || !var_type || !cell_type || !face_type || !out) {
+		return fail(nullptr, PAMHD_E_ARG, "null argument");
+	}
+	*out = nullptr;
+	pamhd_grid_desc periodic = *grid;
+	for (int d = 0; d < 3; d++) periodic.periodic[d] = 1;
+	pamhd_handle* h = nullptr;
+	int e = pamhd_create(&periodic, physics, PAMHD_FP64, &h);
+	if (e) return e;
+	h->desc = *grid;
+	e = substep_alloc(h);
+	if (e) { pamhd_destroy(h); return e; }
+	SubstepDev& d = *h->sub;
+	const long long n = h->g.n;
+	hipError_t he = hipMalloc(&d.types, (size_t)11 * n);
+	if (he == hipSuccess) he = hipMalloc(&d.bmask, (size_t)4 * n);
+	if (he == hipSuccess) he = hipMalloc(&d.bval, sizeof(double) * 11 * n);
+	if (he == hipSuccess) he = hipMemcpy(d.types, cell_type, (size_t)n, hipMemcpyHostToDevice);
+	if (he == hipSuccess) he = hipMemcpy(d.types + n, face_type, (size_t)6 * n, hipMemcpyHostToDevice);
+	if (he == hipSuccess) he = hipMemcpy(d.types + 7 * n, var_type, (size_t)4 * n, hipMemcpyHostToDevice);
+	if (he == hipSuccess) he = hipMemset(d.bmask, 0, (size_t)4 * n);
+	if (he == hipSuccess) he = hipMemset(d.S, 0, sizeof(double) * 14 * n);
+	if (he != hipSuccess) {
+		const int r = fail(nullptr, PAMHD_E_DEVICE, std::string("boundary arrays: ") + hipGetErrorString(he));
+		pamhd_destroy(h);
+		return r;
+	}
+	d.bounded = true;
+	d.ct = d.types;
+	d.fi = d.types + n;
+	d.vt = d.types + 7 * n;
+	for (int q = 0; q < 3; q++) d.per[q] = grid->periodic[q] ? 1 : 0;
+	d.smax = 0;
+	d.opt = nullptr;
+	*out = h;
+	return PAMHD_OK;
+}
+
+int pamhd_set_boundary_values(pamhd_handle* h, const unsigned char* mask, const double* number_density,
+	const double* velocity, const double* pressure, const double* face_field)
+{
+	if (!h || !mask || !number_density || !velocity || !pressure || !face_field) {
+		return fail(h, PAMHD_E_ARG, "null argument");
+	}
+	if (!h->sub || !h->sub->bounded) return fail(h, PAMHD_E_ARG, "not a grid with boundaries");
+	SubstepDev& d = *h->sub;
+	const size_t n = (size_t)h->g.n;
+	HIP_TRY(h, hipMemcpy(d.bmask, mask, 4 * n, hipMemcpyHostToDevice));
+	HIP_TRY(h, hipMemcpy(d.bval, number_density, sizeof(double) * n, hipMemcpyHostToDevice));
+	HIP_TRY(h, hipMemcpy(d.bval + n, velocity, sizeof(double) * 3 * n, hipMemcpyHostToDevice));
+	HIP_TRY(h, hipMemcpy(d.bval + 4 * n, pressure, sizeof(double) * n, hipMemcpyHostToDevice));
+	HIP_TRY(h, hipMemcpy(d.bval + 5 * n, face_field, sizeof(double) * 6 * n, hipMemcpyHostToDevice));
+	return PAMHD_OK;
+}
+
 int pamhd_box_cell_types(pamhd_handle* h, signed char* cell_type, unsigned char* rule)
 {
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
@@ -2012,6 +2419,10 @@ int pamhd_upload_state(pamhd_handle* h, const double* state)
 		HIP_TRY(h, boxset_upload(h->box, 0, state));
 		return PAMHD_OK;
 	}
+	if (h->sub && h->sub->bounded) {
+		HIP_TRY(h, hipMemcpy(h->sub->S, state, sizeof(double) * 14 * h->g.n, hipMemcpyHostToDevice));
+		return PAMHD_OK;
+	}
 	HIP_TRY(h, hipMemcpy(h->state[h->cur], state, sizeof(double) * 8 * h->g.n, hipMemcpyHostToDevice));
 	return PAMHD_OK;
 }
@@ -2023,6 +2434,11 @@ int pamhd_download_state(pamhd_handle* h, double* state)
 		HIP_TRY(h, boxset_download(h->box, 0, state));
 		return PAMHD_OK;
 	}
+	if (h->sub && h->sub->bounded) {
+		HIP_TRY(h, hipDeviceSynchronize());
+		HIP_TRY(h, hipMemcpy(state, h->sub->S, sizeof(double) * 14 * h->g.n, hipMemcpyDeviceToHost));
+		return PAMHD_OK;
+	}
 	HIP_TRY(h, hipMemcpy(state, h->state[h->cur], sizeof(double) * 8 * h->g.n, hipMemcpyDeviceToHost));
 	return PAMHD_OK;
 }
@@ -2030,7 +2446,7 @@ int pamhd_download_state(pamhd_handle* h, double* state)
 int pamhd_state_components(pamhd_handle* h, int* components)
 {
 	if (!h || !components) return fail(h, PAMHD_E_ARG, "null argument");
-	*components = h->box ? 14 : 8;
+	*components = (h->box || (h->sub && h->sub->bounded)) ? 14 : 8;
 	return PAMHD_OK;
 }
 
@@ -2038,6 +2454,10 @@ int pamhd_state_device_ptr(pamhd_handle* h, double** ptr)
 {
 	if (!h || !ptr) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "solar-wind box state is not a flat array");
+	if (h->sub && h->sub->bounded) {
+		*ptr = h->sub->S;
+		return PAMHD_OK;
+	}
 	*ptr = h->state[h->cur];
 	return PAMHD_OK;
 }
@@ -2075,6 +2495,17 @@ int pamhd_apply_boundaries(pamhd_handle* h, double /*simulation_time*/, void* st
 		if (boxset_launch_boundaries(h->box, (hipStream_t)stream)) return fail(h, PAMHD_E_DEVICE, "boundary launch");
 		return PAMHD_OK;
 	}
+	if (h->sub && h->sub->bounded) {
+		/* value and copy boundaries, refit, sync; synchronous so that a
+		   boundary error is reported here */
+		const hipStream_t st = (hipStream_t)stream;
+		ss_clear_kernel<<<1, 64, 0, st>>>(*h->sub);
+		launch_boundaries(h, st);
+		HIP_TRY(h, hipGetLastError());
+		HIP_TRY(h, hipMemcpyAsync(h->sub->host_status, h->sub->status, sizeof(StepStatus), hipMemcpyDeviceToHost, st));
+		HIP_TRY(h, hipStreamSynchronize(st));
+		return report_status(h, *h->sub->host_status);
+	}
 	{
 		TimedLaunch t(h, (hipStream_t)stream, K_REFIT);
 		refit_kernel<<<cell_grid(h->g), dim3(BX, BY, 1), 0, (hipStream_t)stream>>>(h->g, h->state[h->cur]);
@@ -2092,12 +2523,17 @@ int pamhd_check(pamhd_handle* h)
 		return e ? fail(h, e, err) : PAMHD_OK;
 	}
 	HIP_TRY(h, hipDeviceSynchronize());
+	if (h->sub && h->sub->bounded) {
+		HIP_TRY(h, hipMemcpy(h->sub->host_status, h->sub->status, sizeof(StepStatus), hipMemcpyDeviceToHost));
+		return report_status(h, *h->sub->host_status);
+	}
 	HIP_TRY(h, hipMemcpy(h->host_status, h->status + h->last, sizeof(StepStatus), hipMemcpyDeviceToHost));
 	return report_status(h, *h->host_status);
 }
 
 int pamhd_max_signal_speeds(pamhd_handle* h, double smax_out[3])
 {
+	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h || !smax_out) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "per-box speeds: use the ensemble interface");
 	HIP_TRY(h, hipDeviceSynchronize());
@@ -2115,6 +2551,7 @@ int pamhd_max_signal_speeds(pamhd_handle* h, double smax_out[3])
 
 int pamhd_div_face2volume(pamhd_handle* h, double* avg)
 {
+	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h || !avg) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h->box) {
 		HIP_TRY(h, boxset_div_face2volume(h->box, 0, avg));
@@ -2147,6 +2584,7 @@ int pamhd_div_face2volume(pamhd_handle* h, double* avg)
 int pamhd_clean_divergence(pamhd_handle* h, double tol, int max_cycles, double* div_before,
 	double* rel_residual, int* cycles, void* stream_)
 {
+	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "divergence cleaning needs a periodic grid");
 	if (!(tol >= 0) || max_cycles < 0) return fail(h, PAMHD_E_ARG, "tol and max_cycles must be >= 0");
@@ -2202,6 +2640,7 @@ int pamhd_step_fixed(pamhd_handle* h, int solver, double dt, void* stream)
 
 int pamhd_set_state_device(pamhd_handle* h, const double* state, void* stream)
 {
+	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h || !state) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "solar-wind box state is not a flat array");
 	HIP_TRY(h, hipMemcpyAsync(h->state[h->cur], state, sizeof(double) * 8 * h->g.n,
@@ -2211,6 +2650,7 @@ int pamhd_set_state_device(pamhd_handle* h, const double* state, void* stream)
 
 int pamhd_get_state_device(pamhd_handle* h, double* state, void* stream)
 {
+	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h || !state) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "solar-wind box state is not a flat array");
 	HIP_TRY(h, hipMemcpyAsync(state, h->state[h->cur], sizeof(double) * 8 * h->g.n,
@@ -2221,6 +2661,7 @@ int pamhd_get_state_device(pamhd_handle* h, double* state, void* stream)
 int pamhd_adjoint_step(pamhd_handle* h, int solver, double dt, const double* state, double* lambda,
 	void* stream)
 {
+	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h || !state || !lambda) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint needs a periodic grid");
 	if (solver != HLL_ATHENA) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint is implemented for hll-athena");
@@ -2342,6 +2783,7 @@ int pamhd_face_flux_batch(int solver, int precision, size_t n, const double* L, 
 
 int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* stream_)
 {
+	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "solar-wind box: use the ensemble autotuner");
 	if (solver != RUSANOV && solver != HLL_ATHENA && solver != HLLD_ATHENA && solver != ROE_ATHENA) {

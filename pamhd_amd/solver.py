@@ -44,7 +44,10 @@ class Grid:
 
     def __init__(self, cells, cell_length, adiabatic_index=5.0 / 3.0,
                  vacuum_permeability=1.0, proton_mass=1.0, precision="fp64",
-                 start=(0.0, 0.0, 0.0)):
+                 start=(0.0, 0.0, 0.0), periodic=(True, True, True), types=None):
+        """types = (var_type (4, n), cell_type (n,), face_type (6, n)) makes a
+        grid with boundaries (pamhd_create_bounded): FP64, 14 doubles per
+        cell, closed where periodic is False"""
         L = _lib.load()
         self.cells = tuple(int(c) for c in cells)
         self.n = self.cells[0] * self.cells[1] * self.cells[2]
@@ -52,14 +55,27 @@ class Grid:
         self.gamma = float(adiabatic_index)
         self.mu0 = float(vacuum_permeability)
         self.precision = {"fp64": _lib.FP64, "mixed": _lib.MIXED}[precision]
+        self.components = 8 if types is None else 14
         desc = _lib.GridDesc((ctypes.c_int32 * 3)(*self.cells),
                              (ctypes.c_double * 3)(*start),
                              (ctypes.c_double * 3)(*self.cell_length),
-                             (ctypes.c_int32 * 3)(1, 1, 1))
+                             (ctypes.c_int32 * 3)(*[1 if p else 0 for p in periodic]))
         phys = _lib.Physics(self.gamma, self.mu0, float(proton_mass))
         h = ctypes.c_void_p()
-        _lib.check(L.pamhd_create(ctypes.byref(desc), ctypes.byref(phys),
-                                  self.precision, ctypes.byref(h)))
+        if types is None:
+            _lib.check(L.pamhd_create(ctypes.byref(desc), ctypes.byref(phys),
+                                      self.precision, ctypes.byref(h)))
+        else:
+            if self.precision != _lib.FP64:
+                raise ValueError("grids with boundaries run in FP64")
+            bp = ctypes.POINTER(ctypes.c_byte)
+            arrs = [np.ascontiguousarray(a, dtype=np.int8).reshape(-1) for a in types]
+            for a, k in zip(arrs, (4, 1, 6)):
+                if a.size != k * self.n:
+                    raise ValueError("type arrays must hold 4 n, n and 6 n entries")
+            _lib.check(L.pamhd_create_bounded(ctypes.byref(desc), ctypes.byref(phys),
+                                              *[a.ctypes.data_as(bp) for a in arrs],
+                                              ctypes.byref(h)))
         self._h = h
         self._L = L
         self.time = 0.0
@@ -90,14 +106,24 @@ class Grid:
     # --- state ----------------------------------------------------------
     def upload(self, state):
         state = np.ascontiguousarray(state, dtype=np.float64)
-        if state.shape != (8, self.n):
-            raise ValueError("state must have shape (8, %d)" % self.n)
+        if state.shape != (self.components, self.n):
+            raise ValueError("state must have shape (%d, %d)" % (self.components, self.n))
         self._check(self._L.pamhd_upload_state(self._h, state.ctypes.data))
 
     def download(self):
-        out = np.empty((8, self.n))
+        out = np.empty((self.components, self.n))
         self._check(self._L.pamhd_download_state(self._h, out.ctypes.data))
         return out
+
+    def set_boundary_values(self, mask, number_density, velocity, pressure, face_field):
+        """value boundaries resolved per cell (pamhd_set_boundary_values)"""
+        m = np.ascontiguousarray(mask, dtype=np.uint8)
+        arrs = [np.ascontiguousarray(a, dtype=np.float64)
+                for a in (number_density, velocity, pressure, face_field)]
+        dp = ctypes.POINTER(ctypes.c_double)
+        self._check(self._L.pamhd_set_boundary_values(
+            self._h, m.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte)),
+            *[a.ctypes.data_as(dp) for a in arrs]))
 
     def device_ptr(self):
         p = ctypes.c_void_p()

@@ -1,7 +1,8 @@
 """The reference's JSON configurations run on the GPU through the loader
 (pamhd_amd.config.run: test.cpp's program loop over the C ABI) against the
-same configurations in the oracle (oracle_lib.run_config), bit for bit, and
-the golden ones against the reference's .dc results."""
+same configurations in the oracle (oracle_lib.run_config, or
+run_bounded_config for grids with boundaries or closed dimensions), bit for
+bit, and the golden ones against the reference's .dc results."""
 import os
 
 import numpy as np
@@ -15,18 +16,23 @@ pytestmark = pytest.mark.gpu
 
 CFG_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "config")
 CONFIGS = sorted(f[:-5] for f in os.listdir(CFG_DIR)
-                 if not f.startswith(("two_pop", "bow_shock"))
-                 and not config.load(os.path.join(CFG_DIR, f)).bounded)
-# whole runs, but at most this many steps of the 3-D Kelvin-Helmholtz case
+                 if not f.startswith(("two_pop", "bow_shock")))
+# whole runs of the periodic configurations, but at most 60 steps of the
+# 3-D Kelvin-Helmholtz case and 300 of those with boundaries
 MAX_STEPS = {"kelvin_helmholtz_3d_roe_athena": 60}
+BOUNDED_STEPS = 300
 
 
 @pytest.mark.parametrize("name", CONFIGS)
 def test_config_runs_bit_exact(gpu, name):
     cfg = config.load(os.path.join(CFG_DIR, name + ".json"))
-    limit = MAX_STEPS.get(name, 1 << 40)
-    state, bg = cfg.initial_state()
-    st, steps_o, t_o, periods = O.run_config(cfg, state, bg, max_steps=limit)
+    if cfg.bounded:
+        limit = BOUNDED_STEPS
+        st, steps_o, t_o, periods, state = O.run_bounded_config(cfg, max_steps=limit)
+    else:
+        limit = MAX_STEPS.get(name, 1 << 40)
+        state, bg = cfg.initial_state()
+        st, steps_o, t_o, periods = O.run_config(cfg, state, bg, max_steps=limit)
     assert st == 0
     res = config.run(cfg, max_steps=limit)
     print(name, cfg.cells, cfg.solver, "steps", res["steps"], "largest period", max(periods),
