@@ -231,7 +231,10 @@ int pamhd_set_substepping(pamhd_handle* h, int substep_min, int substep_max);
    reference evaluates string-valued "substep-min" / "substep-max" at the
    cell centres and the current time before every step
    (set_minmax_substepping_period(time, ...), substepping.hpp:197-255): the
-   caller sets them before each step.  Copies the arrays (synchronous). */
+   caller sets them before each step.  Copies the arrays (synchronous).
+   From the first call on, steps run on the substepping path whatever the
+   options (all periods 1 give the global step), so that the per-face
+   speeds behind the lagged time step carry over when the options change. */
 int pamhd_set_substepping_cells(pamhd_handle* h, const int* substep_min, const int* substep_max);
 
 /*

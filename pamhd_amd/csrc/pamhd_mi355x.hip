@@ -2201,7 +2201,9 @@ int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
 		if (fixed_dt) return fail(h, PAMHD_E_UNSUPPORTED, "fixed steps on a grid with boundaries");
 		return substep_step(h, solver, max_dt, cfl, refits, dt_taken, st);
 	}
-	if (h->sub && h->sub->smax > 0 && !fixed_dt) {
+	/* once substepping is set up, its own per-face speeds carry the lagged
+	   time step from step to step (periods of 1 give the global step) */
+	if (h->sub && !fixed_dt) {
 		return substep_step(h, solver, max_dt, cfl, refits, dt_taken, st);
 	}
 	StepStatus* prev = h->status + h->last;
@@ -2717,13 +2719,9 @@ int pamhd_set_substepping_cells(pamhd_handle* h, const int* substep_min, const i
 		lo = substep_min[c] < lo ? substep_min[c] : lo;
 		hi = substep_max[c] > hi ? substep_max[c] : hi;
 	}
-	if (hi > 0 && h->precision != PAMHD_FP64) {
-		return fail(h, PAMHD_E_UNSUPPORTED, "substepping runs in FP64 only");
-	}
-	if (hi == 0) {
-		/* every period is 1: the global step */
-		if (h->sub) { h->sub->opt = nullptr; h->sub->smax = 0; }
-		return PAMHD_OK;
+	if (h->precision != PAMHD_FP64) {
+		if (hi > 0) return fail(h, PAMHD_E_UNSUPPORTED, "substepping runs in FP64 only");
+		return PAMHD_OK;   /* every period is 1: the global step */
 	}
 	if (!h->sub) {
 		const int e = substep_alloc(h);
@@ -2743,7 +2741,7 @@ int pamhd_set_substepping_cells(pamhd_handle* h, const int* substep_min, const i
 int pamhd_last_max_substep(pamhd_handle* h, int* max_substep)
 {
 	if (!h || !max_substep) return fail(h, PAMHD_E_ARG, "null argument");
-	*max_substep = (h->sub && h->sub->smax > 0) ? h->sub->last_max : 1;
+	*max_substep = h->sub ? h->sub->last_max : 1;
 	return PAMHD_OK;
 }
 
