@@ -23,10 +23,13 @@ Keys (defaults as the reference's):
   number-density, velocity, pressure, magnetic-field, each a "default"
   (number, array or expression) and optional "initial-conditions"
   regions {"geometry-id", "value"};
-* source/background_magnetic_field.hpp:90-125: background-magnetic-field
-  "value" (a constant field).
-Anything else that changes the step (value or copy boundaries, AMR,
-dipole background fields, point-cloud data) raises ConfigError.
+* source/background_magnetic_field.hpp:90-325: background-magnetic-field
+  "value", "min-distance", "line-dipoles" and "dipoles";
+* value-boundaries (time series) and copy-boundaries of each variable
+  (source/boundaries/value_boundaries.hpp, copy_boundaries.hpp), with the
+  grid's closed dimensions (pamhd_amd.boundary).
+Anything else that changes the step (AMR, point-cloud data, two fluid
+populations) raises ConfigError.
 
 The initial state follows test.cpp:420-450: initialize_magnetic_field
 (source/mhd/initialize.hpp:316-597: 7-point Gauss-Legendre averages of the
@@ -234,18 +237,114 @@ class Config:
         self.velocity = _Variable(obj.get("velocity"), 3, "velocity")
         self.pressure = _Variable(obj.get("pressure"), 1, "pressure")
         self.magnetic_field = _Variable(obj.get("magnetic-field"), 3, "magnetic-field")
+        self._background(obj)
+
+    def _background(self, obj):
+        """background_magnetic_field.hpp:90-325: a constant, line dipoles
+        and point dipoles, with min-distance (default: machine epsilon)"""
         self.background = None
-        if "background-magnetic-field" in obj:
-            bg = obj["background-magnetic-field"]
-            for key in bg:
-                if key not in ("value", "comment") and not key.startswith("comment"):
-                    raise ConfigError("background-magnetic-field %s is not supported "
-                                      "(constant value only)" % key)
-            if "value" in bg:
-                v = bg["value"]
-                if not isinstance(v, list) or len(v) != 3:
-                    raise ConfigError("Background magnetic field value must have 3 components")
-                self.background = tuple(float(c) for c in v)
+        self.bg_min_distance = 2.220446049250313e-16
+        self.bg_line_dipoles = []
+        self.bg_dipoles = []
+        if "background-magnetic-field" not in obj:
+            return
+        bg = obj["background-magnetic-field"]
+        const = [0.0, 0.0, 0.0]
+        if "value" in bg:
+            v = bg["value"]
+            if not isinstance(v, list) or len(v) != 3:
+                raise ConfigError("Background magnetic field value must have 3 components")
+            const = [float(c) for c in v]
+        if "min-distance" in bg:
+            self.bg_min_distance = float(bg["min-distance"])
+        for i, ld in enumerate(bg.get("line-dipoles", [])):
+            try:
+                dim = int(ld["line-dimension"])
+                fdir = int(ld["field-direction-through-line"])
+                mag = float(ld["magnitude"])
+                pos = [float(x) for x in ld["position"]]
+            except (KeyError, TypeError, ValueError):
+                raise ConfigError("line-dipoles[%d]: line-dimension, field-direction-through-line, "
+                                  "magnitude and position required" % i) from None
+            if not 1 <= dim <= 3 or fdir == 0 or abs(fdir) > 3 or len(pos) != 3:
+                raise ConfigError("line-dipoles[%d]: invalid dimension, direction or position" % i)
+            # the two coordinates across the line
+            p2 = [pos[a] for a in range(3) if a != dim - 1]
+            self.bg_line_dipoles.append((dim, fdir, mag, p2))
+        for i, dp in enumerate(bg.get("dipoles", [])):
+            try:
+                m = [float(x) for x in dp["moment"]]
+                pos = [float(x) for x in dp["position"]]
+            except (KeyError, TypeError, ValueError):
+                raise ConfigError("dipoles[%d]: moment and position required" % i) from None
+            if len(m) != 3 or len(pos) != 3:
+                raise ConfigError("dipoles[%d]: moment and position need 3 components" % i)
+            self.bg_dipoles.append((m, pos))
+        self.background = tuple(const)
+
+    def background_field(self, x, y, z):
+        """get_background_field (background_magnetic_field.hpp:331-494) at
+        points (x, y, z): a (3, ...) array"""
+        mu0 = self.vacuum_permeability
+        shape = np.broadcast(x, y, z).shape
+        P = [np.broadcast_to(np.asarray(a, dtype=np.float64), shape) for a in (x, y, z)]
+        ret = [np.full(shape, c) for c in self.background]
+        mind = self.bg_min_distance
+        with np.errstate(all="ignore"):
+            for dim, fdir, mag, p2 in self.bg_line_dipoles:
+                dp = [None, None, None]
+                others = [a for a in range(3) if a != dim - 1]
+                dp[dim - 1] = P[dim - 1]
+                dp[others[0]] = p2[0]
+                dp[others[1]] = p2[1]
+                r = [P[a] + (-dp[a]) for a in range(3)]
+                r1 = np.sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2])
+                near = r1 < mind
+                fnear = mu0 * mag / (2 * math.pi * math.pow(mind, 3))
+                base = mag * mu0 / (2 * math.pi * r1 * r1 * r1 * r1)
+                val = [base, base, base]
+                ad = abs(fdir)
+                if ad == dim:
+                    for a in range(3):
+                        val[a] = val[a] * (-r1 / 2) if a == dim - 1 else np.zeros(shape)
+                    far = val
+                else:
+                    c1, c2 = others
+                    v1, v2 = (c2, c1) if ad - 1 == c1 else (c1, c2)
+                    val[dim - 1] = np.zeros(shape)
+                    val[v1] = val[v1] * (r[c1] * r[c2])
+                    val[v2] = val[v2] * (r[c2] * r[c2] - r[c1] * r[c1])
+                    far = val
+                if fdir < 0:
+                    far = [v * -1 for v in far]
+                # inside min-distance: the line-dimension (or the direction
+                # through the line) component gets the clamped magnitude
+                for a in range(3):
+                    add_far = far[a]
+                    if ad == dim:
+                        add_near = fnear if a == dim - 1 else None
+                        sign = 1
+                    else:
+                        add_near = fnear if a == ad - 1 else None
+                        sign = 1 if fdir > 0 else -1
+                    if add_near is None:
+                        ret[a] = np.where(near, ret[a], ret[a] + add_far)
+                    else:
+                        nearv = ret[a] + add_near if sign > 0 else ret[a] - add_near
+                        ret[a] = np.where(near, nearv, ret[a] + add_far)
+            for m, pos in self.bg_dipoles:
+                r = [P[a] + (-pos[a]) for a in range(3)]
+                r1 = np.sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2])
+                near = r1 < mind
+                fnear = mu0 / (2 * math.pi * math.pow(mind, 3))
+                inv = 1 / r1
+                u = [r[a] * inv for a in range(3)]
+                dot = m[0] * u[0] + m[1] * u[1] + m[2] * u[2]
+                f = mu0 / (4 * math.pi) / (r1 * r1 * r1)
+                for a in range(3):
+                    far = (u[a] * dot * 3 + (-m[a])) * f
+                    ret[a] = np.where(near, ret[a] + m[a] * fnear, ret[a] + far)
+        return np.stack(ret)
 
     # --- grid-options ------------------------------------------------------
     def _grid(self, obj):
@@ -420,37 +519,43 @@ class Config:
         return out
 
     def _background_faces(self):
-        """Bg_B of the positive faces, (9, n): [3 * dim + component]"""
+        """Bg_B of the positive faces, (9, n): [3 * dim + component], the
+        field averaged over each face as initialize_magnetic_field does
+        (initialize.hpp:499-522)"""
         if self.background is None:
             return None
         lo, hi, _ = self.emulated_bounds()
         n = self.n
         out = np.empty((9, n))
-        const = self.background
         for dim in range(3):
             o = [a for a in range(3) if a != dim]
+            pos = hi[dim]
             s0, e0, s1, e1 = lo[o[0]], hi[o[0]], lo[o[1]], hi[o[1]]
             d0, d1 = e0 - s0, e1 - s1
-            for comp in range(3):
-                c = const[comp]
-                f = lambda *a, c=c: np.full(np.broadcast(*a).shape, c)
-                res = np.empty(n)
-                cases = [(d0 == 0) & (d1 == 0), (d0 == 0) & (d1 != 0),
-                         (d0 != 0) & (d1 == 0), (d0 != 0) & (d1 != 0)]
-                for ci, sel in enumerate(cases):
-                    if not sel.any():
-                        continue
-                    S0, E0, S1, E1 = s0[sel], e0[sel], s1[sel], e1[sel]
-                    if ci == 0:
-                        res[sel] = c
-                    elif ci == 1:
-                        res[sel] = _gl7(lambda v: f(v), S1, E1) / (E1 - S1)
-                    elif ci == 2:
-                        res[sel] = _gl7(lambda u: f(u), S0, E0) / (E0 - S0)
-                    else:
-                        res[sel] = _gl7(lambda v: _gl7(lambda u: f(u, v), S0, E0), S1, E1) \
-                            / (E0 - S0) / (E1 - S1)
-                out[3 * dim + comp] = res
+            res = np.empty((3, n))
+            cases = [(d0 == 0) & (d1 == 0), (d0 == 0) & (d1 != 0),
+                     (d0 != 0) & (d1 == 0), (d0 != 0) & (d1 != 0)]
+            for ci, sel in enumerate(cases):
+                if not sel.any():
+                    continue
+                S0, E0, S1, E1, Pd = s0[sel], e0[sel], s1[sel], e1[sel], pos[sel]
+
+                def f(u, v, Pd=Pd):
+                    c = [None, None, None]
+                    c[dim] = Pd
+                    c[o[0]] = u
+                    c[o[1]] = v
+                    return self.background_field(*c)
+                if ci == 0:
+                    res[:, sel] = f(S0, S1)
+                elif ci == 1:
+                    res[:, sel] = _gl7(lambda v: f(S0, v), S1, E1) / (E1 - S1)
+                elif ci == 2:
+                    res[:, sel] = _gl7(lambda u: f(u, S1), S0, E0) / (E0 - S0)
+                else:
+                    res[:, sel] = _gl7(lambda v: _gl7(lambda u: f(u, v), S0, E0), S1, E1) \
+                        / (E0 - S0) / (E1 - S1)
+            out[3 * dim:3 * dim + 3] = res
         return out
 
     def initial_state(self, t=None):

@@ -247,7 +247,8 @@ def test_substep_configs_run_in_the_oracle(name):
     (lambda o: o["pressure"].update({"value-boundaries": [{"geometry-id": 0, "value": 1}]}),
      "time-stamps"),
     (lambda o: o.update({"substep-max": -1}), "negative"),
-    (lambda o: o.update({"background-magnetic-field": {"dipoles": []}}), "background"),
+    (lambda o: o.update({"background-magnetic-field": {"dipoles": [{"moment": [1, 2]}]}}),
+     "dipoles"),
 ])
 def test_unsupported_configurations_are_rejected(edit, match):
     obj = json.load(open(cfg_path("alfven_1d_px")))
@@ -273,3 +274,86 @@ def test_all_fixture_configs_initialise():
             state, bg = cfg.initial_state()
             assert state.shape == (8, cfg.n)
         assert np.all(np.isfinite(state)) and np.all(state[0] > 0) and np.all(state[4] > 0), f
+
+
+def _bg_config(bg):
+    obj = json.load(open(cfg_path("alfven_2d_x_1")))
+    obj["background-magnetic-field"] = bg
+    return config.load(obj)
+
+
+def test_point_dipole_background():
+    """B = mu0 / (4 pi r^3) (3 (m . r^) r^ - m) outside min-distance,
+    mu0 m / (2 pi d^3) inside it, plus the constant"""
+    m = (1e15, -2e15, 3e15)
+    cfg = _bg_config({"value": [1e-9, 0, 0], "min-distance": 0.5,
+                      "dipoles": [{"moment": list(m), "position": [0.1, 0.2, -0.3]}]})
+    rng = np.random.default_rng(3)
+    p = rng.uniform(-3, 3, (3, 200))
+    B = cfg.background_field(*p)
+    r = p - np.array([[0.1], [0.2], [-0.3]])
+    r1 = np.sqrt((r * r).sum(0))
+    u = r / r1
+    mv = np.array(m)[:, None]
+    far = cfg.vacuum_permeability / (4 * np.pi * r1 ** 3) * (3 * (mv * u).sum(0) * u - mv)
+    near = cfg.vacuum_permeability * mv / (2 * np.pi * 0.5 ** 3) * np.ones_like(r1)
+    want = np.where(r1 < 0.5, near, far) + np.array([[1e-9], [0], [0]])
+    np.testing.assert_allclose(B, want, rtol=1e-12, atol=1e-30)
+
+
+@pytest.mark.parametrize("dim,fdir", [(1, 2), (1, -3), (2, 1), (2, 3), (3, -1), (3, 2),
+                                      (1, 1), (2, -2), (3, 3)])
+def test_line_dipole_background(dim, fdir):
+    """the line dipole of background_magnetic_field.hpp:338-458: with
+    k = mu0 m / (2 pi r^4) and (a, b) the coordinates across the line
+    (r = position - dipole position), a field through the line in a
+    direction across it has the components k a b and k (b^2 - a^2) (in the
+    reference's assignment of the two to the two directions), one along the
+    line -k r / 2; it does not depend on the coordinate along the line,
+    scales as 1 / r^2 (1 / r^3 along the line), changes sign with the
+    direction (outside min-distance) and is the clamped value inside it.  (It is the
+    reference's expression, not a potential field.)"""
+    mu0 = 1.2566370614359173e-06
+    obj = json.load(open(cfg_path("alfven_2d_x_1")))
+    obj["vacuum-permeability"] = mu0
+    pos = np.array([0.3, -0.2, 0.1])
+
+    def cfg_for(f):
+        obj["background-magnetic-field"] = {"min-distance": 0.1, "line-dipoles": [
+            {"line-dimension": dim, "field-direction-through-line": f, "magnitude": 1e12,
+             "position": list(pos)}]}
+        return config.load(obj)
+    cfg = cfg_for(fdir)
+    rng = np.random.default_rng(dim * 10 + fdir + 5)
+    p = rng.uniform(1.0, 2.0, (3, 50)) * rng.choice([-1, 1], (3, 50))
+    B = cfg.background_field(*p)
+    others = [a for a in range(3) if a != dim - 1]
+    r = p - pos[:, None]
+    r[dim - 1] = 0
+    r1 = np.sqrt((r * r).sum(0))
+    k = mu0 * 1e12 / (2 * np.pi * r1 ** 4)
+    want = np.zeros((3, 50))
+    c1, c2 = others
+    if abs(fdir) == dim:
+        want[dim - 1] = -k * r1 / 2
+    else:
+        v1, v2 = (c2, c1) if abs(fdir) - 1 == c1 else (c1, c2)
+        want[v1] = k * r[c1] * r[c2]
+        want[v2] = k * (r[c2] ** 2 - r[c1] ** 2)
+    want *= np.sign(fdir)
+    np.testing.assert_allclose(B, want, rtol=1e-12, atol=1e-40)
+    q = p.copy()
+    q[dim - 1] += 7.5
+    np.testing.assert_array_equal(cfg.background_field(*q), B)
+    for a in others:
+        q[a] = pos[a] + 2 * (p[a] - pos[a])
+    np.testing.assert_allclose(cfg.background_field(*q), B / (8 if abs(fdir) == dim else 4),
+                               rtol=1e-12, atol=1e-40)
+    np.testing.assert_allclose(cfg_for(-fdir).background_field(*p), -B, rtol=0, atol=0)
+    inside = cfg.background_field(pos[0] + 0.01, pos[1] + 0.02, pos[2] - 0.03)
+    clamp = np.zeros(3)
+    # along the line the clamped value keeps its sign (the reference adds it
+    # whatever the direction's sign)
+    sign = 1 if abs(fdir) == dim else np.sign(fdir)
+    clamp[abs(fdir) - 1] = sign * mu0 * 1e12 / (2 * np.pi * 0.1 ** 3)
+    np.testing.assert_allclose(inside, clamp, rtol=1e-14)
