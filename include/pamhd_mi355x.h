@@ -309,8 +309,8 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen,
    along marches of 16 cells for the y- and z-faces (bit 5: 4 cells) (bits
    3-5 for HLLD only; bit 1 is then ignored); update_remap bits 0 and 1 the same
    orders for the update pass (bit 0 only for the z-marching update) and,
-   for the z-marching update, bits 2-3 its tile (0: 64 x 4, 1: 128 x 2,
-   2: 32 x 8 threads); update_kz = 0 for the per-cell update or the planes
+   for the z-marching update, bits 2-4 its tile (0: 64 x 4, 1: 128 x 2,
+   2: 32 x 8, 3: 128 x 4, 4: 64 x 8 threads); update_kz = 0 for the per-cell update or the planes
    per block of the z-marching update.  All give identical results. */
 int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_remap);
 

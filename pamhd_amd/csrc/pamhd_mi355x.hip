@@ -1562,7 +1562,7 @@ struct pamhd_handle {
 		   along marches of flux_km cells (y- and z-faces) */
 		int flux_shared = 0;
 		int flux_km = 16;
-		int update_tile = 0;   /* column update tile: 64x4, 128x2, 32x8 */
+		int update_tile = 0;   /* column update tile: 64x4, 128x2, 32x8, 128x4, 64x8 */
 	} tune;
 	/* second stream and events of the slab pipeline, created on first use */
 	hipStream_t aux = nullptr;
@@ -1738,6 +1738,8 @@ void launch_update(pamhd_handle* h, hipStream_t st, int refits, int k0 = 0, int 
 	switch (h->tune.update_tile) {
 	case 1: column(std::integral_constant<int, 128>(), std::integral_constant<int, 2>()); break;
 	case 2: column(std::integral_constant<int, 32>(), std::integral_constant<int, 8>()); break;
+	case 3: column(std::integral_constant<int, 128>(), std::integral_constant<int, 4>()); break;
+	case 4: column(std::integral_constant<int, 64>(), std::integral_constant<int, 8>()); break;
 	default: column(std::integral_constant<int, 64>(), std::integral_constant<int, 4>()); break;
 	}
 }
@@ -2846,7 +2848,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	float best_upd = 1e30f;
 	pamhd_handle::Tune best = h->tune;
 	for (int kz : kzs)
-	for (int tile = 0; tile < (kz ? 3 : 1); tile++) {
+	for (int tile = 0; tile < (kz ? 5 : 1); tile++) {
 		if (kz > 0 && kz > h->g.nz) continue;
 		for (int remap = 0; remap < (kz ? 2 : 4); remap++) {
 			h->tune.update_kz = kz;
@@ -2856,7 +2858,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 				if (h->precision == PAMHD_MIXED) launch_update<float>(h, st, 3);
 				else launch_update<double>(h, st, 3);
 			});
-			static const char* tiles[] = {"64x4", "128x2", "32x8"};
+			static const char* tiles[] = {"64x4", "128x2", "32x8", "128x4", "64x8"};
 			report += " " + std::string(kz ? "column" + std::to_string(kz) + "/" + tiles[tile] : "cell") +
 				"/remap" + std::to_string(remap) + "=" + std::to_string(ms) + "ms";
 			if (ms < best_upd) {
@@ -2915,7 +2917,7 @@ int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_
 	h->tune.flux_km = (flux_remap & 32) ? 4 : 16;
 	h->tune.update_kz = update_kz;
 	h->tune.update_remap = update_kz ? (update_remap & 1) : (update_remap & 3);
-	h->tune.update_tile = update_kz ? (update_remap >> 2) % 3 : 0;
+	h->tune.update_tile = update_kz ? (update_remap >> 2) % 5 : 0;
 	return PAMHD_OK;
 }
 

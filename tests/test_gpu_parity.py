@@ -114,7 +114,7 @@ def test_grid_steps_face_varying_background(gpu, name, sid):
 
 @pytest.mark.parametrize("variant", [(0, 0, 0), (1, 0, 1), (0, 4, 0), (1, 8, 1), (0, 32, 1),
                                      (2, 0, 2), (3, 0, 3), (4, 0, 0), (7, 0, 2),
-                                     (0, 8, 4), (1, 4, 9), (0, 16, 8)])
+                                     (0, 8, 4), (1, 4, 9), (0, 16, 8), (0, 8, 12), (1, 16, 17)])
 def test_launch_variants_bit_exact(gpu, variant):
     """Every launch variant the autotuner may pick (block order, occupancy
     build of the flux kernels, per-cell or z-marching update with its planes
