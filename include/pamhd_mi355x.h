@@ -205,6 +205,13 @@ step then keeps the face divergence at roundoff).
 int pamhd_clean_divergence(pamhd_handle* h, double tol, int max_cycles,
 	double* div_before, double* rel_residual, int* cycles, void* stream);
 
+/* How pamhd_adjoint_step forms the transposed Jacobian of the HLL flux:
+   1 (default) by hand-written reverse accumulation, 0 by forward dual
+   numbers (16 seeded directions per face; kept as the check of the
+   former).  Both give the exact derivative up to rounding.  (No
+   counterpart in the reference.) */
+int pamhd_set_adjoint_jacobian(pamhd_handle* h, int reverse);
+
 /* Smoother of the Poisson solve: 1 = LDS-tiled fused red-black sweeps where
    every dimension of a level is even and >= 4 (half the bytes of the two
    colour passes, but latency bound: slower on MI355X at 256^3), 0 = one

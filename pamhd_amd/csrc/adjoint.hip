@@ -201,6 +201,171 @@ template <int N> __device__ bool hll_dual(const Dual<N> L[8], const Dual<N> R[8]
 	return true;
 }
 
+/*
+J^T g of the same HLL flux by reverse accumulation: the forward pass
+recomputes the intermediate values, the reverse pass carries the adjoint
+g of the face-frame flux back to both sides' inputs (gL, gR).  It returns
+what hll_dual returns (false: no gradient), and the same derivatives up to
+rounding, for about three flux evaluations instead of the 16 seeded
+directions of the dual-number form.
+*/
+__device__ bool hll_adjoint(const double L[8], const double R[8], const Phys& ph,
+	const double gF[8], double gL[8], double gR[8])
+{
+	for (int q = 0; q < 8; q++) gL[q] = gR[q] = 0;
+	if (L[RHO] <= 0 || R[RHO] <= 0) return false;
+	const double gamma = ph.gamma, mu0 = ph.mu0, inv_mu0 = ph.inv_mu0;
+	const double gm1 = ph.gm1, hinv = ph.half_inv_mu0;
+	const double* U[2] = {L, R};
+	double* gU[2] = {gL, gR};
+	double p[2], v[2][3], cf[2], mag[2], sound[2], alfven[2] = {0, 0}, s2[2] = {0, 0};
+	double a2[2] = {0, 0}, ratio[2] = {0, 0}, rts[2] = {1, 1}, bsq[2], msq[2];
+	bool hasB[2];
+	for (int s = 0; s < 2; s++) {
+		const double* u = U[s];
+		msq[s] = u[M0] * u[M0] + u[M1] * u[M1] + u[M2] * u[M2];
+		bsq[s] = u[B0] * u[B0] + u[B1] * u[B1] + u[B2] * u[B2];
+		p[s] = (u[NRJ] - 0.5 / u[RHO] * msq[s] - hinv * bsq[s]) * gm1;
+		if (!(p[s] > 0) || !__builtin_isnormal(p[s])) return false;
+		const double inv = 1 / u[RHO];
+		for (int i = 0; i < 3; i++) v[s][i] = u[M0 + i] * inv;
+		mag[s] = __builtin_sqrt(bsq[s]);
+		sound[s] = __builtin_sqrt(gamma * p[s] / u[RHO]);
+		hasB[s] = mag[s] > 0;
+		if (!hasB[s]) {
+			cf[s] = sound[s];
+		} else {
+			alfven[s] = mag[s] / __builtin_sqrt(mu0 * u[RHO]);
+			s2[s] = sound[s] * sound[s];
+			a2[s] = alfven[s] * alfven[s];
+			ratio[s] = u[B0] / mag[s];
+			const double ts = s2[s] * s2[s] + a2[s] * a2[s]
+				+ 2 * s2[s] * a2[s] * (1 - 2 * (ratio[s] * ratio[s]));
+			if (!(ts > 0)) return false;
+			rts[s] = __builtin_sqrt(ts);
+			cf[s] = __builtin_sqrt(0.5 * (s2[s] + a2[s] + rts[s]));
+		}
+	}
+	/* wave speeds with the tie rules of std::max / std::min */
+	const int ims = (cf[0] < cf[1]) ? 1 : 0;
+	const double ms = cf[ims];
+	const bool lo = v[0][0] <= v[1][0];
+	const int sn = lo ? 0 : 1, sp = lo ? 1 : 0;
+	const double msn = v[sn][0] - ms, msp = v[sp][0] + ms;
+	const bool take_n = !(0.0 < msn), take_p = !(msp < 0.0);
+	const double bm = take_n ? msn : 0.0, bp = take_p ? msp : 0.0;
+	const double den0 = bp - bm;
+	if (!__builtin_isnormal(den0) || den0 < 0) return true;   /* zero flux */
+	/* physical fluxes, then the HLL combination */
+	double FF[2][8], pB1[2], vB[2];
+	for (int s = 0; s < 2; s++) {
+		const double* u = U[s];
+		const double* B = u + B0;
+		pB1[s] = 0.5 * inv_mu0 * bsq[s];
+		vB[s] = v[s][0] * B[0] + v[s][1] * B[1] + v[s][2] * B[2];
+		double* f = FF[s];
+		f[RHO] = u[M0];
+		for (int i = 0; i < 3; i++) f[M0 + i] = u[M0 + i] * v[s][0] - B[i] * B[0] * inv_mu0;
+		f[M0] += p[s] + pB1[s];
+		f[NRJ] = v[s][0] * (u[NRJ] + p[s] + pB1[s]) - B[0] * vB[s] * inv_mu0;
+		for (int i = 0; i < 3; i++) f[B0 + i] = B[i] * v[s][0] - v[s][i] * B[0];
+		f[B0] = 0;
+	}
+	const double den = bp - bm, sum = bp + bm, w = sum / den / 2;
+	double gFL[8], gFR[8], gw = 0;
+	for (int q = 0; q < 8; q++) {
+		if (q == B0) { gFL[q] = gFR[q] = 0; continue; }
+		const double fl = FF[0][q] - L[q] * bm, fr = FF[1][q] - R[q] * bp;
+		gFL[q] = gF[q] * (0.5 + w);
+		gFR[q] = gF[q] * (0.5 - w);
+		gw += gF[q] * (fl - fr);
+	}
+	const double gsum = gw / (2 * den), gden = -gw * sum / (2 * den * den);
+	double gbp = gsum + gden, gbm = gsum - gden;
+	for (int q = 0; q < 8; q++) {
+		if (q == B0) continue;
+		gL[q] -= gFL[q] * bm;
+		gbm -= gFL[q] * L[q];
+		gR[q] -= gFR[q] * bp;
+		gbp -= gFR[q] * R[q];
+	}
+	double gp[2] = {0, 0}, gv[2][3] = {{0, 0, 0}, {0, 0, 0}}, gcf[2] = {0, 0};
+	for (int s = 0; s < 2; s++) {
+		const double* gf = s ? gFR : gFL;
+		const double* u = U[s];
+		const double* B = u + B0;
+		double* gu = gU[s];
+		double gpB1 = 0;
+		gu[M0] += gf[RHO];
+		for (int i = 0; i < 3; i++) {
+			gu[M0 + i] += gf[M0 + i] * v[s][0];
+			gv[s][0] += gf[M0 + i] * u[M0 + i];
+			gu[B0 + i] -= gf[M0 + i] * B[0] * inv_mu0;
+			gu[B0] -= gf[M0 + i] * B[i] * inv_mu0;
+		}
+		gp[s] += gf[M0];
+		gpB1 += gf[M0];
+		gv[s][0] += gf[NRJ] * (u[NRJ] + p[s] + pB1[s]);
+		gu[NRJ] += gf[NRJ] * v[s][0];
+		gp[s] += gf[NRJ] * v[s][0];
+		gpB1 += gf[NRJ] * v[s][0];
+		gu[B0] -= gf[NRJ] * vB[s] * inv_mu0;
+		const double gvB = -gf[NRJ] * B[0] * inv_mu0;
+		for (int i = 0; i < 3; i++) {
+			gv[s][i] += gvB * B[i];
+			gu[B0 + i] += gvB * v[s][i];
+		}
+		for (int i = 1; i < 3; i++) {
+			gu[B0 + i] += gf[B0 + i] * v[s][0];
+			gv[s][0] += gf[B0 + i] * B[i];
+			gv[s][i] -= gf[B0 + i] * B[0];
+			gu[B0] -= gf[B0 + i] * v[s][i];
+		}
+		for (int i = 0; i < 3; i++) gu[B0 + i] += gpB1 * inv_mu0 * B[i];
+	}
+	double gms = 0;
+	if (take_n) { gv[sn][0] += gbm; gms -= gbm; }
+	if (take_p) { gv[sp][0] += gbp; gms += gbp; }
+	gcf[ims] += gms;
+	for (int s = 0; s < 2; s++) {
+		const double* u = U[s];
+		double* gu = gU[s];
+		const double rho = u[RHO];
+		double gsound = 0, gmag = 0;
+		if (!hasB[s]) {
+			gsound += gcf[s];
+		} else {
+			const double gQ = gcf[s] / (2 * cf[s]);
+			double gs2 = 0.5 * gQ, ga2 = 0.5 * gQ;
+			const double gts = 0.5 * gQ / (2 * rts[s]);
+			const double r2 = ratio[s] * ratio[s];
+			gs2 += gts * (2 * s2[s] + 2 * a2[s] * (1 - 2 * r2));
+			ga2 += gts * (2 * a2[s] + 2 * s2[s] * (1 - 2 * r2));
+			const double gratio = gts * 2 * s2[s] * a2[s] * (-4 * ratio[s]);
+			gsound += 2 * sound[s] * gs2;
+			const double galf = 2 * alfven[s] * ga2;
+			gu[B0] += gratio / mag[s];
+			gmag -= gratio * u[B0] / (mag[s] * mag[s]);
+			gmag += galf / __builtin_sqrt(mu0 * rho);
+			gu[RHO] -= 0.5 * alfven[s] / rho * galf;
+			for (int i = 0; i < 3; i++) gu[B0 + i] += gmag * u[B0 + i] / mag[s];
+		}
+		gp[s] += gsound * gamma / (2 * rho * sound[s]);
+		gu[RHO] -= gsound * sound[s] / (2 * rho);
+		for (int i = 0; i < 3; i++) {
+			gu[M0 + i] += gv[s][i] / rho;
+			gu[RHO] -= gv[s][i] * u[M0 + i] / (rho * rho);
+		}
+		gu[NRJ] += gp[s] * gm1;
+		for (int i = 0; i < 3; i++) {
+			gu[M0 + i] -= gp[s] * gm1 * u[M0 + i] / rho;
+			gu[B0 + i] -= gp[s] * gm1 * 2 * hinv * u[B0 + i];
+		}
+		gu[RHO] += gp[s] * gm1 * 0.5 * msq[s] / (rho * rho);
+	}
+	return true;
+}
+
 struct AdjGrid {
 	int nx, ny, nz, n;
 	double len[3];
@@ -444,6 +609,37 @@ __global__ __launch_bounds__(ABX * ABY) void adj_jac_kernel(AdjGrid g, const dou
 	}
 }
 
+/* the same W by hll_adjoint */
+__global__ __launch_bounds__(ABX * ABY) void adj_jacr_kernel(AdjGrid g, const double* __restrict__ s,
+	const double* __restrict__ LP, double* __restrict__ W)
+{
+	int i, j, k;
+	if (!thread_cell(g, i, j, k)) return;
+	const int n = g.n;
+	const int c = cell_at(g, i, j, k);
+	for (int d = 0; d < 3; d++) {
+		double Ld[8], Rd[8], lF[8], out[2][8];
+		face_state(g, s, d, i - (d == 0), j - (d == 1), k - (d == 2), Ld);
+		face_state(g, s, d, i, j, k, Rd);
+		lF[RHO] = g.dt * LP[(d * 8 + RHO) * n + c];
+		lF[NRJ] = g.dt * LP[(d * 8 + NRJ) * n + c];
+		for (int q = 0; q < 3; q++) {
+			lF[M0 + q] = g.dt * LP[(d * 8 + M0 + (d + q) % 3) * n + c];
+			lF[B0 + q] = g.dt * LP[(d * 8 + B0 + (d + q) % 3) * n + c];
+		}
+		hll_adjoint(Ld, Rd, g.ph, lF, out[0], out[1]);
+		for (int side = 0; side < 2; side++) {
+			double* w = W + (size_t)(d * 16 + side * 8) * n;
+			w[RHO * n + c] = out[side][RHO];
+			w[NRJ * n + c] = out[side][NRJ];
+			for (int q = 0; q < 3; q++) {
+				w[(M0 + (d + q) % 3) * n + c] = out[side][M0 + q];
+				w[(B0 + (d + q) % 3) * n + c] = out[side][B0 + q];
+			}
+		}
+	}
+}
+
 /* lambda^n of rho, m, E (in place) and lambda_vb of each cell */
 __global__ __launch_bounds__(ABX * ABY) void adj_cell_kernel(AdjGrid g, double* __restrict__ lam,
 	const double* __restrict__ mu, const double* __restrict__ W, double* __restrict__ V)
@@ -484,7 +680,7 @@ __global__ __launch_bounds__(ABX * ABY) void adj_face_kernel(AdjGrid g, double* 
 size_t adjoint_scratch_doubles(long long n) { return (size_t)108 * n; }
 
 void adjoint_launch(int nx, int ny, int nz, const double len[3], double dt, const Phys& ph,
-	const double* state, double* lam, double* scratch, hipStream_t st)
+	const double* state, double* lam, double* scratch, hipStream_t st, int reverse)
 {
 	AdjGrid g;
 	g.nx = nx;
@@ -506,7 +702,8 @@ void adjoint_launch(int nx, int ny, int nz, const double len[3], double dt, cons
 	adj_mu_kernel<<<grid, block, 0, st>>>(g, state, P, lam, mu);
 	adj_lface_kernel<<<grid, block, 0, st>>>(g, lam, mu, G);
 	adj_lflux_kernel<<<grid, block, 0, st>>>(g, lam, mu, G, LP);
-	adj_jac_kernel<<<grid, block, 0, st>>>(g, state, LP, W);
+	if (reverse) adj_jacr_kernel<<<grid, block, 0, st>>>(g, state, LP, W);
+	else adj_jac_kernel<<<grid, block, 0, st>>>(g, state, LP, W);
 	adj_cell_kernel<<<grid, block, 0, st>>>(g, lam, mu, W, V);
 	adj_face_kernel<<<grid, block, 0, st>>>(g, lam, G, V);
 }

@@ -38,7 +38,7 @@ conserved update, constant-pressure energy refits).  In mixed precision the flux
 namespace pamhd_dev {
 size_t adjoint_scratch_doubles(long long n);
 void adjoint_launch(int nx, int ny, int nz, const double len[3], double dt, const Phys& ph,
-	const double* state, double* lam, double* scratch, hipStream_t st);
+	const double* state, double* lam, double* scratch, hipStream_t st, int reverse);
 }
 
 using namespace pamhd_dev;
@@ -1578,6 +1578,7 @@ struct pamhd_handle {
 	PoissonMG* mg = nullptr;  /* multigrid hierarchy, allocated on first cleaning */
 	SubstepDev* sub = nullptr; /* per-cell substepping, allocated when enabled */
 	int mg_fused = 0;   /* per-colour passes (faster; see pamhd_set_poisson_smoother) */
+	int adj_reverse = 1;   /* flux Jacobian of the adjoint: 1 reverse accumulation, 0 dual numbers */
 	int cur = 0;    /* state buffer holding the current state */
 	int last = 0;   /* status entry of the last flux pass */
 	std::string err;
@@ -2674,8 +2675,15 @@ int pamhd_adjoint_step(pamhd_handle* h, int solver, double dt, const double* sta
 	if (!h->adj) HIP_TRY(h, hipMalloc(&h->adj, sizeof(double) * adjoint_scratch_doubles(h->g.n)));
 	const double len[3] = {h->g.dx, h->g.dy, h->g.dz};
 	adjoint_launch(h->g.nx, h->g.ny, h->g.nz, len, dt, h->g.ph, state, lambda, h->adj,
-		(hipStream_t)stream);
+		(hipStream_t)stream, h->adj_reverse);
 	HIP_TRY(h, hipGetLastError());
+	return PAMHD_OK;
+}
+
+int pamhd_set_adjoint_jacobian(pamhd_handle* h, int reverse)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	h->adj_reverse = reverse != 0;
 	return PAMHD_OK;
 }
 

@@ -254,6 +254,11 @@ class Grid:
         """z-slab pipelining of the flux pass and the update (1 = off)."""
         self._check(self._L.pamhd_set_slabs(self._h, int(slabs)))
 
+    def set_adjoint_jacobian(self, reverse=True):
+        """the adjoint's flux Jacobian: reverse accumulation (default) or
+        forward dual numbers"""
+        self._check(self._L.pamhd_set_adjoint_jacobian(self._h, 1 if reverse else 0))
+
     def set_poisson_smoother(self, fused):
         self._check(self._L.pamhd_set_poisson_smoother(self._h, 1 if fused else 0))
 

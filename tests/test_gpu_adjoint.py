@@ -122,11 +122,13 @@ def test_checkpoint_interval_does_not_change_gradient(gpu):
     np.testing.assert_array_equal(grads[0], grads[2])
 
 
+@pytest.mark.parametrize("reverse", [True, False])
 @pytest.mark.parametrize("cells", [(8, 8, 1), (6, 5, 4)])
-def test_adjoint_step_matches_autograd_of_restated_step(gpu, cells):
+def test_adjoint_step_matches_autograd_of_restated_step(gpu, cells, reverse):
     """One reverse step against torch.autograd of the PyTorch restatement of
     the same step (tests/torch_step.py) on the CPU, component by component,
-    for a random lambda; the restatement's forward equals the device step to
+    for a random lambda, with the flux Jacobian by reverse accumulation and
+    by dual numbers; the restatement's forward equals the device step to
     rounding."""
     import torch
     import torch_step
@@ -143,6 +145,7 @@ def test_adjoint_step_matches_autograd_of_restated_step(gpu, cells):
     dt = 0.2 * min(length) / 3
     lam1 = rng.normal(size=(8, n))
     with Grid(cells, length, gamma, mu0) as g:
+        g.set_adjoint_jacobian(reverse)
         g.upload(state)
         g.step_fixed("hll-athena", dt)
         fwd = g.download()
@@ -162,3 +165,35 @@ def test_adjoint_step_matches_autograd_of_restated_step(gpu, cells):
         print("component", q, "max rel err", err, flush=True)
     scale = np.abs(ref).max(1, keepdims=True) + 1e-300
     assert np.all(np.abs(lam0 - ref) <= 1e-9 * scale)
+
+
+def test_reverse_and_dual_jacobians_agree(gpu):
+    """the two forms of the flux Jacobian on a hot blob with a field-free
+    region (both branches of the fast speed) and random lambda: the same
+    reverse step to rounding"""
+    import torch
+    from pamhd_amd import configs
+    from pamhd_amd.solver import Grid
+    cells = (16, 12, 10)
+    state, length, gamma, mu0 = configs.hot_blob(cells, contrast=30.0, radius=0.25, seed=3)
+    state[5:8, : state.shape[1] // 3] = 0.0      # no field in a third of the cells
+    n = state.shape[1]
+    kin = 0.5 * (state[1:4] ** 2).sum(0) / state[0]
+    vb = 0.5 * (state[5:8] + np.roll(state[5:8], -1, 1))
+    state[4] = np.maximum(state[4], kin + 0.5 / mu0 * (vb ** 2).sum(0) + 0.1)
+    rng = np.random.default_rng(7)
+    lam1 = rng.normal(size=(8, n))
+    outs = []
+    for reverse in (True, False):
+        with Grid(cells, length, gamma, mu0) as g:
+            g.set_adjoint_jacobian(reverse)
+            U = torch.from_numpy(state).to(gpu).contiguous()
+            lam = torch.from_numpy(lam1).to(gpu).contiguous()
+            g.adjoint_step("hll-athena", 0.01, U.data_ptr(), lam.data_ptr(),
+                           torch.cuda.current_stream().cuda_stream)
+            torch.cuda.synchronize()
+            outs.append(lam.cpu().numpy())
+    scale = np.abs(outs[1]).max(1, keepdims=True) + 1e-300
+    err = np.abs(outs[0] - outs[1]) / scale
+    print("max relative difference per component", err.max(1), flush=True)
+    assert np.all(err <= 1e-12)
