@@ -450,7 +450,8 @@ E_e = esign * P(pd, pq) * len_l / 4 for the six edge fields
      3 EZY = -P(1,5) dz/4   4 EXZ = -P(2,6) dx/4   5 EYZ = +P(2,5) dy/4
 */
 struct CtTerm { signed char t, e, sign, o0, o1, o2; };
-__constant__ CtTerm CT_TERMS[36] = {
+/* compile-time tables: the loops over them unroll into straight-line code */
+constexpr CtTerm CT_TERMS[36] = {
 	{0, 0, +1, 0, 1, 0}, {0, 0, -1, 0, -1, 0}, {0, 1, -1, 0, 0, 1}, {0, 1, +1, 0, 0, -1},
 	{0, 3, +1, 0, 1, 0}, {0, 3, +1, -1, 1, 0}, {0, 3, -1, 0, 0, 0}, {0, 3, -1, -1, 0, 0},
 	{0, 5, -1, 0, 0, 1}, {0, 5, -1, -1, 0, 1}, {0, 5, +1, 0, 0, 0}, {0, 5, +1, -1, 0, 0},
@@ -461,7 +462,7 @@ __constant__ CtTerm CT_TERMS[36] = {
 	{2, 1, +1, 1, 0, 0}, {2, 1, +1, 1, 0, -1}, {2, 1, -1, 0, 0, 0}, {2, 1, -1, 0, 0, -1},
 	{2, 2, -1, 0, 1, 0}, {2, 2, -1, 0, 1, -1}, {2, 2, +1, 0, 0, 0}, {2, 2, +1, 0, 0, -1}};
 /* per edge field: flux dim, component, sign, length dim */
-__constant__ signed char CT_EMF[6][4] = {
+constexpr signed char CT_EMF[6][4] = {
 	{0, 6, +1, 2}, {0, 7, -1, 1}, {1, 7, +1, 0}, {1, 5, -1, 2}, {2, 6, -1, 0}, {2, 5, +1, 1}};
 
 __device__ __forceinline__ double emf(const AdjGrid& g, const double* __restrict__ P, int e, int c)
@@ -474,6 +475,7 @@ __device__ __forceinline__ double emf(const AdjGrid& g, const double* __restrict
 __device__ double face_dB(const AdjGrid& g, const double* __restrict__ P, int t, int i, int j, int k)
 {
 	double r = 0;
+#pragma unroll
 	for (int q = 12 * t; q < 12 * t + 12; q++) {
 		const CtTerm& T = CT_TERMS[q];
 		r += T.sign * emf(g, P, T.e, nb(g, i, j, k, T.o0, T.o1, T.o2));
@@ -494,6 +496,7 @@ __global__ __launch_bounds__(ABX * ABY) void adj_mu_kernel(AdjGrid g, const doub
 	const int n = g.n;
 	const int c = cell_at(g, i, j, k);
 	const double lE = lam[4 * n + c];
+#pragma unroll
 	for (int d = 0; d < 3; d++) {
 		const int cp = nb(g, i, j, k, d == 0, d == 1, d == 2);
 		const double f0 = s[(5 + d) * n + c] + face_dB(g, P, d, i, j, k) / area(g, d);
@@ -531,6 +534,7 @@ __global__ __launch_bounds__(ABX * ABY) void adj_lflux_kernel(AdjGrid g, const d
 	if (!thread_cell(g, i, j, k)) return;
 	const int n = g.n;
 	const int c = cell_at(g, i, j, k);
+#pragma unroll
 	for (int d = 0; d < 3; d++) {
 		const int cm = nb(g, i, j, k, -(d == 0), -(d == 1), -(d == 2));
 		const double il = 1 / g.len[d];
@@ -540,9 +544,11 @@ __global__ __launch_bounds__(ABX * ABY) void adj_lflux_kernel(AdjGrid g, const d
 		lp[NRJ] = (lam[4 * n + c] - lam[4 * n + cm]) * il;
 		for (int q = 0; q < 3; q++) lp[B0 + q] = (mu[(3 + q) * n + c] - mu[(3 + q) * n + cm]) * il;
 		/* transposed edge sums: E_e(c) enters face t of cell c - offset */
+#pragma unroll
 		for (int e = 0; e < 6; e++) {
 			if (CT_EMF[e][0] != d) continue;
 			double lE = 0;
+#pragma unroll
 			for (int q = 0; q < 36; q++) {
 				const CtTerm& T = CT_TERMS[q];
 				if (T.e != e) continue;
