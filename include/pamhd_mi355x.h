@@ -268,6 +268,12 @@ int pamhd_create_bounded(const pamhd_grid_desc* grid, const pamhd_physics* physi
 	const signed char* var_type, const signed char* cell_type, const signed char* face_type,
 	pamhd_handle** out);
 
+/* What a handle steps: PAMHD_GRID_PERIODIC (pamhd_create),
+   PAMHD_GRID_BOUNDED (pamhd_create_bounded) or PAMHD_GRID_SOLAR_WIND_BOX
+   (pamhd_create_solar_wind_box). */
+enum { PAMHD_GRID_PERIODIC = 0, PAMHD_GRID_BOUNDED = 1, PAMHD_GRID_SOLAR_WIND_BOX = 2 };
+int pamhd_grid_kind(pamhd_handle* h, int* kind);
+
 /* The value boundaries for the next pamhd_apply_boundaries / pamhd_step,
    resolved per cell (the last boundary covering a cell wins; the caller
    evaluates time-dependent values at the time the boundaries are applied,

@@ -17,6 +17,8 @@ PAMHD_E_POISSON = 9
 PAMHD_E_SUBSTEP = 10
 PAMHD_E_ARG = 16
 PAMHD_E_UNSUPPORTED = 17
+# pamhd_grid_kind
+GRID_KINDS = ("periodic", "bounded", "solar wind box")
 
 EXPORTS = (
     "pamhd_abi_version", "pamhd_create", "pamhd_destroy", "pamhd_upload_state",
@@ -27,6 +29,7 @@ EXPORTS = (
     "pamhd_face_flux_batch", "pamhd_autotune", "pamhd_set_launch", "pamhd_enable_timing", "pamhd_kernel_time",
     "pamhd_last_error", "pamhd_create_solar_wind_box", "pamhd_state_components",
     "pamhd_create_bounded", "pamhd_set_boundary_values", "pamhd_set_adjoint_jacobian",
+    "pamhd_grid_kind",
     "pamhd_box_cell_types", "pamhd_box_background", "pamhd_ensemble_create",
     "pamhd_ensemble_destroy", "pamhd_ensemble_timestep", "pamhd_ensemble_run",
     "pamhd_ensemble_check", "pamhd_ensemble_download", "pamhd_ensemble_cells",
@@ -100,6 +103,7 @@ def load():
     L.pamhd_set_boundary_values.argtypes = [H, ctypes.POINTER(ctypes.c_ubyte), _dpp, _dpp,
                                             _dpp, _dpp]
     L.pamhd_set_adjoint_jacobian.argtypes = [H, ctypes.c_int]
+    L.pamhd_grid_kind.argtypes = [H, ctypes.POINTER(ctypes.c_int)]
     L.pamhd_destroy.argtypes = [H]
     L.pamhd_upload_state.argtypes = [H, _vp]
     L.pamhd_download_state.argtypes = [H, _vp]

@@ -20,12 +20,14 @@ extern "C" int pamhd_run(pamhd_handle* h, int solver, double time_step_factor,
 	double* t_final, void* stream)
 {
 	if (!h) return PAMHD_E_ARG;
-	int components = 8;
-	int e = pamhd_state_components(h, &components);
+	int kind = PAMHD_GRID_PERIODIC;
+	int e = pamhd_grid_kind(h, &kind);
 	if (e) return e;
 	/* a solar-wind box is created with its boundaries applied
-	   (tests/mhd/solar_wind_box.cpp:448-461) */
-	if (components == 8) {
+	   (tests/mhd/solar_wind_box.cpp:448-461); the others get them here
+	   (a grid with boundaries: the values last given to
+	   pamhd_set_boundary_values) */
+	if (kind != PAMHD_GRID_SOLAR_WIND_BOX) {
 		e = pamhd_apply_boundaries(h, t_start, stream);
 		if (e) return e;
 	}

@@ -2455,6 +2455,13 @@ int pamhd_state_components(pamhd_handle* h, int* components)
 	return PAMHD_OK;
 }
 
+int pamhd_grid_kind(pamhd_handle* h, int* kind)
+{
+	if (!h || !kind) return fail(h, PAMHD_E_ARG, "null argument");
+	*kind = h->box ? PAMHD_GRID_SOLAR_WIND_BOX : ((h->sub && h->sub->bounded) ? PAMHD_GRID_BOUNDED : PAMHD_GRID_PERIODIC);
+	return PAMHD_OK;
+}
+
 int pamhd_state_device_ptr(pamhd_handle* h, double** ptr)
 {
 	if (!h || !ptr) return fail(h, PAMHD_E_ARG, "null argument");

@@ -103,6 +103,13 @@ class Grid:
     def last_error(self):
         return _lib.last_error(self._h)
 
+    @property
+    def kind(self):
+        """"periodic" or "bounded" (pamhd_grid_kind)"""
+        k = ctypes.c_int(-1)
+        self._check(self._L.pamhd_grid_kind(self._h, ctypes.byref(k)))
+        return _lib.GRID_KINDS[k.value]
+
     # --- state ----------------------------------------------------------
     def upload(self, state):
         state = np.ascontiguousarray(state, dtype=np.float64)

@@ -114,6 +114,13 @@ class BoxGrid:
     def _check(self, code):
         return _lib.check(code, self._h)
 
+    @property
+    def kind(self):
+        """"solar wind box" (pamhd_grid_kind)"""
+        k = ctypes.c_int(-1)
+        self._check(self._L.pamhd_grid_kind(self._h, ctypes.byref(k)))
+        return _lib.GRID_KINDS[k.value]
+
     def download(self):
         out = np.empty((14, self.n))
         self._check(self._L.pamhd_download_state(self._h, out.ctypes.data))

@@ -80,3 +80,30 @@ def test_saves_follow_save_mhd_n(gpu, tmp_path):
     np.testing.assert_array_equal(rows["mhd     "], mhd_rows)
     fast = config.run(cfg)
     np.testing.assert_array_equal(fast["state"], res["state"])
+
+
+@pytest.mark.parametrize("name", ["shock_tube_1d_px_hll_athena", "rarefaction_1d_px_hll_athena",
+                                  "reconnection_2d_y1_hll_athena"])
+def test_bounded_grid_through_pamhd_run(gpu, name):
+    """pamhd_run on a grid with constant boundaries: apply_boundaries at
+    time-start, the final init and the loop, all inside the library, give
+    the oracle's bits"""
+    from pamhd_amd.solver import Grid
+    cfg = config.load(os.path.join(CFG_DIR, name + ".json"))
+    assert cfg.bounded and not cfg.boundaries_depend_on_time()
+    assert not cfg.substeps_depend_on_time()
+    st, steps_o, t_o, periods, want = O.run_bounded_config(cfg, max_steps=40)
+    assert st == 0
+    state, bg, types = cfg.bounded_initial_state()
+    with Grid(cfg.cells, cfg.cell_length, cfg.adiabatic_index, cfg.vacuum_permeability,
+              cfg.proton_mass, start=cfg.start, periodic=cfg.periodic, types=types) as g:
+        assert g.kind == "bounded"
+        g.upload(state)
+        if bg is not None:
+            g.set_background_faces(bg)
+        g.set_substepping(*cfg.substep_options(cfg.time_start))
+        g.set_boundary_values(*cfg.boundary_values(cfg.time_start))
+        steps, t = g.run(cfg.solver, cfg.time_step_factor, cfg.time_start,
+                         cfg.time_start + cfg.time_length, max_steps=40)
+        assert steps == steps_o and t == t_o
+        np.testing.assert_array_equal(g.download(), want)

@@ -79,6 +79,7 @@ def test_grid_steps_fp64_bit_exact(gpu, name, sid):
     st, steps_o, t_o = O.run_periodic(og, sid, 0.5, 0.0, 1e9, ref, max_steps=6)
     assert st == 0
     with S.Grid(cells, length, gamma, mu0) as g:
+        assert g.kind == "periodic"
         g.upload(state)
         steps_g, t_g = g.run(name, 0.5, 0.0, 1e9, max_steps=6)
         out = g.download()

@@ -49,6 +49,7 @@ def test_box_setup_matches_oracle(gpu):
     sb = _test_box()
     ob = O.Box(sb)
     with BoxGrid(sb) as g:
+        assert g.kind == "solar wind box"
         ct, ru = g.cell_types()
         bg = g.background()
         s = g.download()
