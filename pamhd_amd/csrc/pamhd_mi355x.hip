@@ -517,6 +517,153 @@ __global__ __launch_bounds__(BX * BY, MINB) void flux_hlld_march_kernel(GridDev 
 	reduce_speed(st, D, sp, zero);
 }
 
+/*
+Moves one 32-bit word of a wave's per-cell values one lane up: lane l
+receives lane l - 1's r, lane 0 receives b.
+*/
+__device__ __forceinline__ unsigned lane_up_u32(unsigned r, unsigned b)
+{
+	const unsigned up = (unsigned)__shfl_up((int)r, 1u, 64);
+	return threadIdx.x == 0 ? b : up;
+}
+
+__device__ __forceinline__ double lane_up(double r, double b)
+{
+	const unsigned long long rv = (unsigned long long)__double_as_longlong(r);
+	const unsigned long long bv = (unsigned long long)__double_as_longlong(b);
+	const unsigned lo = lane_up_u32((unsigned)rv, (unsigned)bv);
+	const unsigned hi = lane_up_u32((unsigned)(rv >> 32), (unsigned)(bv >> 32));
+	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+__device__ __forceinline__ float lane_up(float r, float b)
+{
+	return __uint_as_float(lane_up_u32(__float_as_uint(r), __float_as_uint(b)));
+}
+
+__device__ __forceinline__ int lane_up(int r, int b)
+{
+	return (int)lane_up_u32((unsigned)r, (unsigned)b);
+}
+
+/* a per-cell HLLD part with its fast speed and check stage, moved as above */
+template <class T> struct SideState {
+	Side<T> s;
+	T cf;
+	int stage, code;
+};
+
+template <class T> __device__ __forceinline__ SideState<T> lane_up(const SideState<T>& r,
+	const SideState<T>& b)
+{
+	SideState<T> o;
+#pragma unroll
+	for (int q = 0; q < 8; q++) o.s.u[q] = lane_up(r.s.u[q], b.s.u[q]);
+	o.s.p = lane_up(r.s.p, b.s.p);
+#pragma unroll
+	for (int q = 0; q < 3; q++) o.s.v[q] = lane_up(r.s.v[q], b.s.v[q]);
+	o.s.ir = lane_up(r.s.ir, b.s.ir);
+	o.s.p64 = lane_up(r.s.p64, b.s.p64);
+	o.s.pt64 = lane_up(r.s.pt64, b.s.pt64);
+	o.s.rho_pos = lane_up((int)r.s.rho_pos, (int)b.s.rho_pos) != 0;
+	o.cf = lane_up(r.cf, b.cf);
+	o.stage = lane_up(r.stage, b.stage);
+	o.code = lane_up(r.code, b.code);
+	return o;
+}
+
+/*
+HLLD flux pass for the x-faces with the per-cell part computed once per
+cell: a wave is the 64 lanes of one x tile (lane l at i = i0 + l), and each
+thread walks km cells of its line along y (plane k0 + t, the y-march's
+blocks).  At each step every lane computes hlld_side of its own cell; the
+left side of lane l's face is lane l - 1's, moved one lane up, and lane 0's
+is that of the cell at i0 - 1 (periodic), computed for all km rows of the
+march before it starts, one row per lane, and kept in LDS (the wave's own
+slots: written and read by the same wave, in order, no block barrier).
+Lanes past the grid's end compute the side of the last cell (in bounds, not
+used) so that every lane takes part in the moves.  A line of 64 faces needs
+64 per-cell parts plus 1 / km for the tile's left edge instead of 128.
+Every value equals flux_pass_kernel's (the same functions of the same
+inputs, the same first error per face).
+*/
+template <class T, class PT, int MINB>
+__global__ __launch_bounds__(BX * BY, MINB) void flux_hlld_xmarch_kernel(GridDev g, StateArrays S,
+	PT* __restrict__ P, const double* __restrict__ dtp, StepStatus* st, unsigned gx, int km,
+	int remap, int k0, int k_end)
+{
+	static_assert(BX == 64, "a wave is one x tile");
+	const unsigned lb = (remap & 1) ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+	const int nt = k_end - k0;
+	const unsigned gt = (unsigned)((nt + BY - 1) / BY);
+	const unsigned bx = lb % gx, r = lb / gx;
+	const int t = (int)(r % gt) * BY + threadIdx.y;
+	const int chunk = (int)(r / gt);
+	const int i0 = (int)bx * BX;
+	const int i = i0 + threadIdx.x;
+	const int a0 = chunk * km;
+	const int a1 = min(a0 + km, g.ny);
+	double sp = 0;
+	unsigned zero = 0;
+	/* t, a0 and a1 are the same in every lane of the wave */
+	if (t < nt && a0 < a1) {
+		const int kk = k0 + t;
+		const int ic = min(i, g.nx - 1);
+		const T dt = (T)*dtp;
+		__shared__ SideState<T> edge[BY][BX];
+		{
+			SideState<T> e{};
+			double U[8];
+			load_face_state_d<0>(S, g, wrap_dec(i0, g.nx), a0 + min((int)threadIdx.x, a1 - a0 - 1),
+				kk, U);
+			e.stage = hlld_side<T>(U, g.ph, e.s, e.cf, e.code);
+			edge[threadIdx.y][threadIdx.x] = e;
+		}
+		__builtin_amdgcn_wave_barrier();
+		for (int a = a0; a < a1; a++) {
+			double U[8];
+			load_face_state_d<0>(S, g, ic, a, kk, U);
+			SideState<T> R;
+			R.cf = 0;
+			R.stage = hlld_side<T>(U, g.ph, R.s, R.cf, R.code);
+			const SideState<T> L = lane_up(R, edge[threadIdx.y][a - a0]);
+			if (i < g.nx) {
+				T F[8], sT = 0;
+				int e;
+				if (L.stage | R.stage) {
+					e = first_side_error(L.stage, L.code, R.stage, R.code);
+				} else if constexpr (std::is_same_v<T, float>) {
+					e = flux_hlld32_face(L.s, R.s, L.cf, R.cf, g.ph, F, sT);
+				} else {
+					e = flux_hlld_face<T>(L.s, R.s, L.cf, R.cf, L.s.u, R.s.u, g.ph, F, sT);
+				}
+				const int c = i + g.nx * (a + g.ny * kk);
+				T Fg[8];
+				if (e) {
+					record_error(st, e, (long long)c);
+					for (int q = 0; q < 8; q++) Fg[q] = 0;
+					sT = 0;
+				} else {
+					Fg[RHO] = dt * F[RHO];
+					Fg[NRJ] = dt * F[NRJ];
+					Fg[M0] = dt * F[M0];
+					Fg[M0 + 1] = dt * F[M1];
+					Fg[M0 + 2] = dt * F[M2];
+					Fg[B0] = dt * F[B0];
+					Fg[B0 + 1] = dt * F[B1];
+					Fg[B0 + 2] = dt * F[B2];
+				}
+#pragma unroll
+				for (int q = 0; q < 8; q++) (P + (size_t)q * g.n)[c] = (PT)Fg[q];
+				const double s_face = (double)sT;
+				sp = smax(sp, s_face);
+				zero += s_face <= 0 ? 1u : 0u;
+			}
+		}
+	}
+	reduce_speed(st, 0, sp, zero);
+}
+
 /* constant-thermal-pressure energy refit of update_vol_B,
    source/mhd/solve.hpp:1646-1672 */
 __device__ __forceinline__ void refit_energy(double rho, const double m[3], double& E,
@@ -1656,17 +1803,22 @@ void launch_flux_dirs(pamhd_handle* h, hipStream_t st, StepStatus* stat, int k0,
 	const unsigned gx = (h->g.nx + BX - 1) / BX, gy = (h->g.ny + BY - 1) / BY;
 	const dim3 grid(gx * gy * (unsigned)nk), block(BX, BY, 1);
 	const int r = h->tune.flux_remap;
-	if (SOLVER == HLLD_ATHENA && h->tune.flux_shared == 2) {
+	if (SOLVER == HLLD_ATHENA && h->tune.flux_shared >= 2) {
 		const int km = h->tune.flux_km;
 		const unsigned g1 = gx * (unsigned)((nk + BY - 1) / BY) * (unsigned)((h->g.ny + km - 1) / km);
 		const unsigned g2 = gx * (unsigned)((h->g.ny + BY - 1) / BY) * (unsigned)((nk + km - 1) / km);
 		const int ke = k0 + nk;
+		/* flux_shared 3: the x-faces marched too (km <= 64: one row per lane
+		   for the tile's left edge) */
+		const bool xm = h->tune.flux_shared == 3 && km <= 64;
 		if (h->tune.flux_occ) {
-			flux_pass_kernel<SOLVER, T, PT, 0, 4><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
+			if (xm) flux_hlld_xmarch_kernel<T, PT, 4><<<g1, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
+			else flux_pass_kernel<SOLVER, T, PT, 0, 4><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
 			flux_hlld_march_kernel<T, PT, 1, 4><<<g1, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
 			flux_hlld_march_kernel<T, PT, 2, 4><<<g2, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
 		} else {
-			flux_pass_kernel<SOLVER, T, PT, 0, 1><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
+			if (xm) flux_hlld_xmarch_kernel<T, PT, 1><<<g1, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
+			else flux_pass_kernel<SOLVER, T, PT, 0, 1><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
 			flux_hlld_march_kernel<T, PT, 1, 1><<<g1, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
 			flux_hlld_march_kernel<T, PT, 2, 1><<<g2, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
 		}
@@ -2833,7 +2985,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	float best_flux = 1e30f;
 	int best_remap = 0, best_occ = 0, best_shared = 0, best_km = 16;
 	/* (shared mode, cells per march) */
-	const int modes[][2] = {{0, 16}, {1, 16}, {2, 8}, {2, 16}, {2, 32}};
+	const int modes[][2] = {{0, 16}, {1, 16}, {2, 8}, {2, 16}, {2, 32}, {3, 8}, {3, 16}, {3, 32}};
 	for (const auto& md : modes)
 	for (int occ = 0; occ < 2; occ++)
 	for (int remap = 0; remap < (md[0] ? 2 : 4); remap++) {
@@ -2843,7 +2995,8 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 		h->tune.flux_shared = md[0];
 		h->tune.flux_km = md[1];
 		const float ms = time_it(flux);
-		report += std::string(md[0] == 1 ? " shared/" : md[0] == 2 ? " march" + std::to_string(md[1]) + "/" : " ") +
+		report += std::string(md[0] == 1 ? " shared/" : md[0] == 2 ? " march" + std::to_string(md[1]) + "/"
+			: md[0] == 3 ? " xmarch" + std::to_string(md[1]) + "/" : " ") +
 			"remap" + std::to_string(remap) + (occ ? "/occ4" : "") + "=" + std::to_string(ms) + "ms";
 		if (ms < best_flux) {
 			best_flux = ms;
@@ -2913,7 +3066,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	report += "; chosen flux_remap=" + std::to_string(h->tune.flux_remap) +
 		" flux_occ4=" + std::to_string(h->tune.flux_occ) +
 		" flux_shared=" + std::to_string(h->tune.flux_shared) +
-		(h->tune.flux_shared == 2 ? "/km" + std::to_string(h->tune.flux_km) : std::string()) +
+		(h->tune.flux_shared >= 2 ? "/km" + std::to_string(h->tune.flux_km) : std::string()) +
 		" update=" + (h->tune.update_kz ? "column" + std::to_string(h->tune.update_kz) + "/tile" +
 			std::to_string(h->tune.update_tile) : std::string("cell")) +
 		" update_remap=" + std::to_string(h->tune.update_remap) +
@@ -2928,7 +3081,7 @@ int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_
 	if (update_kz < 0) return fail(h, PAMHD_E_ARG, "update_kz must be >= 0");
 	h->tune.flux_remap = flux_remap & 3;
 	h->tune.flux_occ = (flux_remap >> 2) & 1;
-	h->tune.flux_shared = (flux_remap & 16) ? 2 : (flux_remap >> 3) & 1;
+	h->tune.flux_shared = (flux_remap & 64) ? 3 : (flux_remap & 16) ? 2 : (flux_remap >> 3) & 1;
 	h->tune.flux_km = (flux_remap & 32) ? 4 : 16;
 	h->tune.update_kz = update_kz;
 	h->tune.update_remap = update_kz ? (update_remap & 1) : (update_remap & 3);

@@ -137,11 +137,14 @@ def test_launch_variants_bit_exact(gpu, variant):
 
 
 @pytest.mark.parametrize("variant", [(8, 0, 0), (9, 0, 1), (12, 16, 1), (13, 8, 0),
-                                     (16, 0, 0), (49, 16, 1), (52, 0, 0), (21, 8, 0)])
+                                     (16, 0, 0), (49, 16, 1), (52, 0, 0), (21, 8, 0),
+                                     (64, 0, 0), (65, 8, 1), (96, 16, 0), (68, 0, 1)])
 def test_hlld_shared_sides_bit_exact(gpu, variant):
     """HLLD with the per-cell part (pressure, 1/rho, fast speed) computed once
     per cell, shared through LDS or carried along marches of 16 or 4 cells
-    (chunk ends inside the 9- and 13-cell lines): the oracle's bits in FP64, the per-face
+    (chunk ends inside the 9- and 13-cell lines), the x-faces too with each
+    lane's part moved to the next lane (the 64-lane tile's left edge and the
+    partial second tile): the oracle's bits in FP64, the per-face
     kernel's bits in mixed precision and through the z-slab pipeline; the
     70x9x13 grid leaves partial tiles in x, y and z."""
     from pamhd_amd import solver as S
@@ -167,17 +170,20 @@ def test_hlld_shared_sides_bit_exact(gpu, variant):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
-def test_hlld_shared_sides_report_errors(gpu, prec):
+@pytest.mark.parametrize("i", [5, 11])
+def test_hlld_shared_sides_report_errors(gpu, prec, i):
     """A cell with negative thermal pressure stops the HLLD step with the
-    pressure error whether the per-cell part is shared or not."""
+    pressure error whether the per-cell part is shared or not (at i = 11 it
+    is the left side of the periodic x-face at i = 0, the lane-0 edge of the
+    x-march)."""
     from pamhd_amd import _lib
     from pamhd_amd import solver as S
     cells = (12, 10, 9)
     state, length, gamma, mu0 = _vortex(cells)
-    c = 5 + 12 * (4 + 10 * 3)
+    c = i + 12 * (4 + 10 * 3)
     state[4, c] = 0.01 * state[4, c]
     codes = []
-    for flux_bits in (0, 8, 16):
+    for flux_bits in (0, 8, 16, 64):
         with S.Grid(cells, length, gamma, mu0, precision=prec) as g:
             g.upload(state)
             g.set_launch(flux_bits, 0, 0)
@@ -185,7 +191,7 @@ def test_hlld_shared_sides_report_errors(gpu, prec):
                 g.timestep("hlld-athena", 0.0, 0.3)
             codes.append(info.value.code)
     print("codes", codes, flush=True)
-    assert codes[0] == codes[1] == codes[2] == 2
+    assert codes == [2, 2, 2, 2]
 
 
 @pytest.mark.parametrize("slabs", [2, 3, 4, 8])
