@@ -367,7 +367,7 @@ def precision_study(size=512, steps=160, clean_every=10, report_every=20, solver
 # HBM traffic of one step (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 summed over
 # the step's launches) measured with rocprofv3 --pmc for the launch variants
 # the autotuner picked: profiles/step_kernels_256cubed_hlld_counters.md
-STEP_TRAFFIC_BYTES = {("fp64", 256, "hlld-athena"): 13.84e9, ("mixed", 256, "hlld-athena"): 10.31e9}
+STEP_TRAFFIC_BYTES = {("fp64", 256, "hlld-athena"): 14.34e9, ("mixed", 256, "hlld-athena"): 10.13e9}
 
 
 def roofline(res, n, steps, key=None):
