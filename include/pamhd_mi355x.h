@@ -309,8 +309,11 @@ Autotuner: times the launch variants of the flux and update passes that give
 bit-identical results (block order, per-cell vs z-marching update and its
 planes per block) on the handle's current state with the given solver, keeps
 the fastest for later steps and writes a report to buf (may be NULL).  The
-state and the lagged time step are not changed.  (No counterpart in the
-reference.)
+state and the lagged time step are not changed.  Until it (or
+pamhd_set_launch) is called a handle uses the variants it picks for the
+256^3 HLLD step: XCD-aware flux block order, HLLD's per-cell part carried
+along 16-cell marches in all three directions, the z-marching update with 8
+planes per 128 x 2 tile.  (No counterpart in the reference.)
 */
 int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen,
 	void* stream);

@@ -1699,17 +1699,20 @@ struct pamhd_handle {
 	double* host_dt = nullptr;    /* pinned */
 	StepStatus* host_status = nullptr;
 	/* launch choices, set by pamhd_autotune (defaults: per-cell update) */
+	/* until pamhd_autotune or pamhd_set_launch: the variants the autotuner
+	   picks for the 256^3 HLLD step (every variant gives the same bits) */
 	struct Tune {
-		int flux_remap = 0;
-		int update_kz = 0;
+		int flux_remap = 1;
+		int update_kz = 8;
 		int update_remap = 0;
 		int slabs = 1;   /* > 1: flux pass and update pipelined over z-slabs */
 		int flux_occ = 0; /* 1: flux kernels compiled for 4 blocks per CU */
 		/* HLLD's per-cell part: 0 per face, 1 shared through LDS, 2 carried
-		   along marches of flux_km cells (y- and z-faces) */
-		int flux_shared = 0;
+		   along marches of flux_km cells (y- and z-faces), 3 as 2 and the
+		   x-faces marched along y with the part moved across lanes */
+		int flux_shared = 3;
 		int flux_km = 16;
-		int update_tile = 0;   /* column update tile: 64x4, 128x2, 32x8, 128x4, 64x8 */
+		int update_tile = 1;   /* column update tile: 64x4, 128x2, 32x8, 128x4, 64x8 */
 	} tune;
 	/* second stream and events of the slab pipeline, created on first use */
 	hipStream_t aux = nullptr;

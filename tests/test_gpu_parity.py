@@ -156,17 +156,16 @@ def test_hlld_shared_sides_bit_exact(gpu, variant):
     assert st == 0
     outs = {}
     for prec in ("fp64", "mixed"):
-        for v, slabs in ((None, 1), (variant, 1), (variant, 3)):
+        for v, slabs in (((0, 0, 0), 1), (variant, 1), (variant, 3)):
             with S.Grid(cells, length, gamma, mu0, precision=prec) as g:
                 g.upload(state)
-                if v is not None:
-                    g.set_launch(*v)
+                g.set_launch(*v)
                 g.set_slabs(slabs)
                 g.run("hlld-athena", 0.3, 0.0, 1e9, max_steps=4)
                 outs[prec, v, slabs] = g.download()
     for slabs in (1, 3):
         np.testing.assert_array_equal(outs["fp64", variant, slabs], ref)
-        np.testing.assert_array_equal(outs["mixed", variant, slabs], outs["mixed", None, 1])
+        np.testing.assert_array_equal(outs["mixed", variant, slabs], outs["mixed", (0, 0, 0), 1])
 
 
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
@@ -210,6 +209,8 @@ def test_slab_pipeline_bit_exact(gpu, slabs):
         for sl in (1, slabs):
             with S.Grid(cells, length, gamma, mu0, precision=prec) as g:
                 g.upload(state)
+                # the per-cell update: the pipeline's kernel
+                g.set_launch(0, 0, 0)
                 g.set_slabs(sl)
                 g.run("hlld-athena", 0.3, 0.0, 1e9, max_steps=5)
                 outs[prec, sl] = g.download()
