@@ -305,6 +305,25 @@ int pamhd_face_flux_batch(int solver, int precision, size_t n, const double* L,
 	void* stream);
 
 /*
+The 1-D plain method of tests/mhd/reference.cpp: solve_mhd
+(reference.cpp:284-440; fluxes of the faces between neighbouring cells,
+the upwind electric field on each cell's +x edge, reference.cpp:393-404)
+with apply_fluxes (mhd/common.hpp:707-743) and the main loop
+(reference.cpp:830-877: lagged dt = cfl * min(cell_size / max signal
+speed), the first step of length 0, until t >= t_end), on the GPU.  The two
+end cells receive no fluid flux and the last cell no field change, as in
+the reference.  state: host, 8 * ncells FP64 component-major {rho, m (3),
+E, B (3)} (cell-centred field), read and overwritten.  At most max_steps
+steps (the reference has no such limit).  Errors: the first failing face's
+solver error (the state is then that before the failing step) or the first
+failing cell's density / pressure check (the state after the step).
+(Replaces the reference program's loop; no handle.)
+*/
+int pamhd_plain1d_run(size_t ncells, double cell_size, double adiabatic_index,
+	double vacuum_permeability, double cfl, int solver, double t_end, long max_steps,
+	double* state, long* steps_taken, void* stream);
+
+/*
 Autotuner: times the launch variants of the flux and update passes that give
 bit-identical results (block order, per-cell vs z-marching update and its
 planes per block) on the handle's current state with the given solver, keeps

@@ -27,8 +27,8 @@ HIP_FLAGS = [
 def build_library(verbose=False):
     os.makedirs(LIBDIR, exist_ok=True)
     srcs = [os.path.join(CSRC, f) for f in
-            ("pamhd_mi355x.hip", "poisson.hip", "swbox.hip", "adjoint.hip", "ensemble.cpp",
-              "driver.cpp")]
+            ("pamhd_mi355x.hip", "poisson.hip", "swbox.hip", "adjoint.hip", "plain1d.hip",
+             "ensemble.cpp", "driver.cpp")]
     srcs = [s for s in srcs if os.path.exists(s)]
     tmp = LIB + ".tmp"
     cmd = [HIPCC] + HIP_FLAGS + ["-I", os.path.join(ROOT, "include"), "-I", CSRC,

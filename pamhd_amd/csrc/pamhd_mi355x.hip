@@ -33,6 +33,7 @@ conserved update, constant-pressure energy refits).  In mixed precision the flux
 #include "pamhd_mi355x.h"
 #include "poisson.hpp"
 #include "riemann.hpp"
+#include "plain1d.hpp"
 #include "swbox.hpp"
 
 namespace pamhd_dev {
@@ -2914,6 +2915,26 @@ int pamhd_last_max_substep(pamhd_handle* h, int* max_substep)
 {
 	if (!h || !max_substep) return fail(h, PAMHD_E_ARG, "null argument");
 	*max_substep = h->sub ? h->sub->last_max : 1;
+	return PAMHD_OK;
+}
+
+int pamhd_plain1d_run(size_t ncells, double cell_size, double adiabatic_index,
+	double vacuum_permeability, double cfl, int solver, double t_end, long max_steps,
+	double* state, long* steps_taken, void* stream)
+{
+	if (!state) return fail(nullptr, PAMHD_E_ARG, "null state");
+	if (!(cell_size > 0) || !(adiabatic_index > 1) || !(vacuum_permeability > 0) || !(cfl > 0) ||
+		max_steps < 0)
+		return fail(nullptr, PAMHD_E_ARG, "invalid cell size, physics constants, cfl or step limit");
+	std::string err;
+	const int e = pamhd_dev::plain1d_run((long long)ncells, cell_size, adiabatic_index,
+		vacuum_permeability, cfl, solver, t_end, max_steps, state, steps_taken,
+		(hipStream_t)stream, err);
+	if (e) {
+		set_global_error(e == PAMHD_E_ARG || e == PAMHD_E_SOLVER || e == PAMHD_E_DEVICE ? err :
+			std::string(error_name(e)) + " (" + err + ")");
+		return e;
+	}
 	return PAMHD_OK;
 }
 

@@ -301,3 +301,21 @@ def face_flux_batch(solver, L, R, bg=None, adiabatic_index=5.0 / 3.0,
                                          vacuum_permeability, F.data_ptr(),
                                          s.data_ptr(), st.data_ptr(), stream))
     return F, s, st
+
+
+def plain1d_run(solver, state, cell_size, adiabatic_index, vacuum_permeability, t_end,
+                cfl=0.5, max_steps=1 << 40, stream=None):
+    """The 1-D plain method of tests/mhd/reference.cpp on the GPU
+    (pamhd_plain1d_run): state is a float64 array (8, ncells) {rho, m, E, B}
+    with cell-centred B, advanced in place from t = 0 to t_end.  Returns the
+    number of steps."""
+    Lib = _lib.load()
+    if not (isinstance(state, np.ndarray) and state.dtype == np.float64
+            and state.flags.c_contiguous and state.ndim == 2 and state.shape[0] == 8):
+        raise ValueError("state must be a C-contiguous float64 array of shape (8, ncells)")
+    steps = ctypes.c_long(0)
+    _lib.check(Lib.pamhd_plain1d_run(state.shape[1], cell_size, adiabatic_index,
+                                     vacuum_permeability, cfl, solver_id(solver), t_end,
+                                     max_steps, state.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                     ctypes.byref(steps), stream))
+    return steps.value

@@ -88,3 +88,20 @@ def test_product_path_does_not_use_the_oracle():
             if f.endswith((".py", ".hip", ".cpp", ".hpp", ".h")):
                 text = open(os.path.join(dirpath, f)).read()
                 assert "oracle" not in text.replace("oracle's", ""), f
+
+
+def test_plain1d_argument_checks_without_device():
+    """pamhd_plain1d_run rejects a null state, fewer than two cells, bad
+    constants and unknown solvers before allocating anything"""
+    from pamhd_amd import _lib
+    L = _lib.load()
+    s = (ctypes.c_double * 16)()
+    n = ctypes.c_long(-1)
+    assert L.pamhd_plain1d_run(2, 0.5, 2.0, 1.0, 0.5, 2, 0.1, 10, None, None, None) == 16
+    assert L.pamhd_plain1d_run(1, 0.5, 2.0, 1.0, 0.5, 2, 0.1, 10, s, ctypes.byref(n), None) == 16
+    assert L.pamhd_plain1d_run(2, -0.5, 2.0, 1.0, 0.5, 2, 0.1, 10, s, None, None) == 16
+    assert L.pamhd_plain1d_run(2, 0.5, 1.0, 1.0, 0.5, 2, 0.1, 10, s, None, None) == 16
+    assert L.pamhd_plain1d_run(2, 0.5, 2.0, 1.0, 0.5, 1, 0.1, 10, s, None, None) == 8
+    assert L.pamhd_plain1d_run(2, 0.5, 2.0, 1.0, 0.5, 9, 0.1, 10, s, None, None) == 8
+    assert n.value == -1
+    assert _lib.last_error()
