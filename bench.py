@@ -403,6 +403,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="0 skips the adjoint workload")
+    ap.add_argument("--other-solver-steps", type=int, default=20,
+                    help="timed FP64 steps of Rusanov, HLL and Roe on the same workload (0: skip)")
     ap.add_argument("--ensemble-boxes", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=500,
                     help="0 skips the ensemble workload")
@@ -448,6 +450,20 @@ def main():
                                 args.cfl, args.steps, args.warmup)
         log("  %s: %.3f ms/step, %.3e cell-updates/s" % (prec, results[prec]["ms"] / args.steps,
                                                           results[prec]["value"]))
+    # the other solvers of the same workload (FP64, autotuned), as a record
+    # beside the headline solver
+    others = {}
+    for other in ("rusanov", "hll-athena", "roe-athena"):
+        if args.other_solver_steps <= 0 or other == args.solver:
+            continue
+        log("gpu run: fp64, %s, 2 warmup + %d timed steps" % (other, args.other_solver_steps))
+        r = run_gpu("fp64", state, cells, length, gamma, mu0, other, args.cfl,
+                    args.other_solver_steps, 2)
+        r.pop("final")
+        others[other] = {"ms_per_step": r["ms"] / args.other_solver_steps,
+                         "cell_updates_per_s": r["value"], "tuning": r["tuning"].split("chosen ")[-1]}
+        log("  %s: %.3f ms/step, %.3e cell-updates/s" % (other, others[other]["ms_per_step"],
+                                                          r["value"]))
     del state
     err = rel_l2(results["mixed"].pop("final"), results["fp64"].pop("final"))
     log("  mixed vs fp64 relative L2 after %d steps: %s" % (args.warmup + args.steps, err))
@@ -503,6 +519,7 @@ def main():
                   "speedup_vs_fp64": mix["value"] / f64["value"],
                   "rel_l2_vs_fp64": err,
                   "roofline": roofline(mix, n, args.steps, ("mixed", args.size, args.solver))},
+        "other_solvers_fp64": others,
         "divergence_cleaning": clean,
         "ensemble": ens,
         "adjoint": adj,
