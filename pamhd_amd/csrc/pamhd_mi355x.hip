@@ -665,6 +665,122 @@ __global__ __launch_bounds__(BX * BY, MINB) void flux_hlld_xmarch_kernel(GridDev
 	reduce_speed(st, 0, sp, zero);
 }
 
+/* any per-cell record moved one lane up, word by word (lane_up_u32) */
+template <class C> __device__ __forceinline__ C lane_up_words(const C& r, const C& b)
+{
+	static_assert(sizeof(C) % 4 == 0, "whole 32-bit words");
+	constexpr int W = sizeof(C) / 4;
+	unsigned rw[W], bw[W], ow[W];
+	__builtin_memcpy(rw, &r, sizeof(C));
+	__builtin_memcpy(bw, &b, sizeof(C));
+#pragma unroll
+	for (int q = 0; q < W; q++) ow[q] = lane_up_u32(rw[q], bw[q]);
+	C o;
+	__builtin_memcpy(&o, ow, sizeof(C));
+	return o;
+}
+
+/* dt * F of one face rotated back to the global frame and stored (zero
+   flux and an error record when e != 0); returns the face's signal speed */
+template <class T, class PT, int D>
+__device__ __forceinline__ double store_face_flux(const GridDev& g, PT* __restrict__ P,
+	StepStatus* st, int c, int e, const T F[8], T sT, T dt)
+{
+	T Fg[8];
+	if (e) {
+		record_error(st, e, (long long)D * g.n + c);
+		for (int q = 0; q < 8; q++) Fg[q] = 0;
+		sT = 0;
+	} else {
+		Fg[RHO] = dt * F[RHO];
+		Fg[NRJ] = dt * F[NRJ];
+		Fg[M0 + D] = dt * F[M0];
+		Fg[M0 + (D + 1) % 3] = dt * F[M1];
+		Fg[M0 + (D + 2) % 3] = dt * F[M2];
+		Fg[B0 + D] = dt * F[B0];
+		Fg[B0 + (D + 1) % 3] = dt * F[B1];
+		Fg[B0 + (D + 2) % 3] = dt * F[B2];
+	}
+	PT* Pd = P + (size_t)D * 8 * g.n;
+#pragma unroll
+	for (int q = 0; q < 8; q++) (Pd + (size_t)q * g.n)[c] = (PT)Fg[q];
+	return (double)sT;
+}
+
+/*
+HLL and Rusanov on grids without a background field, with the per-cell
+part (flux_cell: checks, pressure, fast speed, physical flux) computed once
+per cell: y- and z-faces (D = 1, 2) along the marches of
+flux_hlld_march_kernel, x-faces (D = 0) along y with each lane's part moved
+to the next lane as in flux_hlld_xmarch_kernel (the tile's left-edge parts
+in LDS; marches of at most 32 rows).  Every value equals flux_pass_kernel's.
+*/
+template <int SOLVER, class T, class PT, int D, int MINB>
+__global__ __launch_bounds__(BX * BY, MINB) void flux_cell_march_kernel(GridDev g, StateArrays S,
+	PT* __restrict__ P, const double* __restrict__ dtp, StepStatus* st, unsigned gx, int km,
+	int remap, int k0, int k_end)
+{
+	static_assert(BX == 64, "a wave is one x tile");
+	const unsigned lb = (remap & 1) ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+	/* D = 0, 1: lines across are the planes of the slab, marches along y;
+	   D = 2: lines across are the rows, marches along z */
+	const int nt = D == 2 ? g.ny : k_end - k0;
+	const unsigned gt = (unsigned)((nt + BY - 1) / BY);
+	const unsigned bx = lb % gx, r = lb / gx;
+	const int t = (int)(r % gt) * BY + threadIdx.y;
+	const int chunk = (int)(r / gt);
+	const int i0 = (int)bx * BX;
+	const int i = i0 + threadIdx.x;
+	const int n_d = D == 2 ? k_end : g.ny;
+	const int a0 = (D == 2 ? k0 : 0) + chunk * km;
+	const int a1 = min(a0 + km, n_d);
+	double sp = 0;
+	unsigned zero = 0;
+	/* D = 0: t, a0 and a1 are the same in every lane of the wave */
+	if (t < nt && a0 < a1 && (D == 0 || i < g.nx)) {
+		const int jj = D == 2 ? t : 0;
+		const int kk = D == 2 ? 0 : k0 + t;
+		const int ic = min(i, g.nx - 1);
+		const T dt = (T)*dtp;
+		/* left-edge parts of up to 32 rows per wave */
+		__shared__ FluxCell<T> edge[D == 0 ? BY : 1][D == 0 ? 32 : 1];
+		FluxCell<T> Lc;
+		if constexpr (D == 0) {
+			FluxCell<T> e;
+			double U[8];
+			load_face_state_d<0>(S, g, wrap_dec(i0, g.nx), a0 + min((int)threadIdx.x, a1 - a0 - 1),
+				kk, U);
+			flux_cell<SOLVER, T>(U, g.ph, e);
+			if (threadIdx.x < 32) edge[threadIdx.y][threadIdx.x] = e;
+			__builtin_amdgcn_wave_barrier();
+		} else {
+			double U[8];
+			const int am = wrap_dec(a0, D == 1 ? g.ny : g.nz);
+			load_face_state_d<D>(S, g, i, D == 1 ? am : jj, D == 1 ? kk : am, U);
+			flux_cell<SOLVER, T>(U, g.ph, Lc);
+		}
+		for (int a = a0; a < a1; a++) {
+			const int j = D == 2 ? jj : a;
+			const int k = D == 2 ? a : kk;
+			double U[8];
+			load_face_state_d<D>(S, g, D == 0 ? ic : i, j, k, U);
+			FluxCell<T> Rc;
+			flux_cell<SOLVER, T>(U, g.ph, Rc);
+			if constexpr (D == 0) Lc = lane_up_words(Rc, edge[threadIdx.y][a - a0]);
+			if (D != 0 || i < g.nx) {
+				T F[8], sT = 0;
+				const int e = flux_face<SOLVER, T>(Lc, Rc, F, sT);
+				const int c = i + g.nx * (j + g.ny * k);
+				const double s_face = store_face_flux<T, PT, D>(g, P, st, c, e, F, sT, dt);
+				sp = smax(sp, s_face);
+				zero += s_face <= 0 ? 1u : 0u;
+			}
+			if constexpr (D != 0) Lc = Rc;
+		}
+	}
+	reduce_speed(st, D, sp, zero);
+}
+
 /* constant-thermal-pressure energy refit of update_vol_B,
    source/mhd/solve.hpp:1646-1672 */
 __device__ __forceinline__ void refit_energy(double rho, const double m[3], double& E,
@@ -1826,6 +1942,20 @@ void launch_flux_dirs(pamhd_handle* h, hipStream_t st, StepStatus* stat, int k0,
 			flux_hlld_march_kernel<T, PT, 1, 1><<<g1, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
 			flux_hlld_march_kernel<T, PT, 2, 1><<<g2, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
 		}
+		return;
+	}
+	if ((SOLVER == HLL_ATHENA || SOLVER == RUSANOV) && h->tune.flux_shared >= 2 && !h->bg) {
+		const int km = h->tune.flux_km;
+		const unsigned gyz = gx * (unsigned)((nk + BY - 1) / BY) * (unsigned)((h->g.ny + km - 1) / km);
+		const unsigned gz = gx * (unsigned)((h->g.ny + BY - 1) / BY) * (unsigned)((nk + km - 1) / km);
+		const int ke = k0 + nk;
+		constexpr int SV = SOLVER == HLL_ATHENA ? HLL_ATHENA : RUSANOV;
+		if (h->tune.flux_shared == 3 && km <= 32)
+			flux_cell_march_kernel<SV, T, PT, 0, 1><<<gyz, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
+		else
+			flux_pass_kernel<SOLVER, T, PT, 0, 1><<<grid, block, 0, st>>>(h->g, S, h->bg, P, h->dt, stat, gx, gy, r, k0);
+		flux_cell_march_kernel<SV, T, PT, 1, 1><<<gyz, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
+		flux_cell_march_kernel<SV, T, PT, 2, 1><<<gz, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
 		return;
 	}
 	if (SOLVER == HLLD_ATHENA && h->tune.flux_shared == 1) {
@@ -3013,7 +3143,11 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	for (const auto& md : modes)
 	for (int occ = 0; occ < 2; occ++)
 	for (int remap = 0; remap < (md[0] ? 2 : 4); remap++) {
-		if (md[0] && solver != HLLD_ATHENA) continue;
+		/* the per-cell part: HLLD, and HLL / Rusanov without a background
+		   field (marches only, one occupancy build) */
+		if (md[0] && solver != HLLD_ATHENA &&
+			(md[0] == 1 || occ || h->bg || (solver != HLL_ATHENA && solver != RUSANOV)))
+			continue;
 		h->tune.flux_remap = remap;
 		h->tune.flux_occ = occ;
 		h->tune.flux_shared = md[0];

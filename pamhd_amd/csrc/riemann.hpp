@@ -359,6 +359,143 @@ template <class T> __device__ __forceinline__ int flux_hll(const double Ld[8],
 	return ERR_OK;
 }
 
+/*
+HLL and Rusanov split into a per-cell part and a per-face part, for faces
+without a background field: fast speed, physical flux and the side checks
+depend on one cell only, so a kernel may compute them once per cell and
+pass them to both of the cell's faces along a direction.  The per-face
+parts return exactly what flux_hll / flux_rusanov return for the same two
+states (the same calls on the same inputs, the checks in the same order).
+*/
+template <class T> struct FluxCell {
+	Side<T> s;
+	T cf;        /* fast speed */
+	T aux;       /* HLL: B^2 / 2 mu0; Rusanov: |m_x / rho| */
+	T F[8];      /* physical flux */
+	int e_rho;   /* the solver's density check */
+	int e_fs;    /* fast_speed's error */
+	int e_pf;    /* physical_flux's error */
+};
+
+template <int SOLVER, class T> __device__ __forceinline__ void flux_cell(const double U[8],
+	const Phys& ph, FluxCell<T>& c)
+{
+	using A = Arith<T>;
+	const T zero[3] = {0, 0, 0};
+	if constexpr (SOLVER == RUSANOV) {
+		c.e_rho = (!__builtin_isnormal(U[RHO]) || U[RHO] < 0) ? ERR_DENSITY : ERR_OK;
+	} else {
+		c.e_rho = U[RHO] <= 0 ? ERR_DENSITY : ERR_OK;
+	}
+	make_side<T>(U, ph, c.s);
+	c.cf = 0;
+	c.e_fs = fast_speed<T>(c.s, U, zero, (T)ph.gamma, (T)ph.mu0, c.cf);
+	if constexpr (SOLVER == RUSANOV) {
+		c.aux = fabs(A::div(c.s.u[M0], c.s.u[RHO]));
+	} else {
+		c.aux = A::div(dot3(c.s.u + B0, c.s.u + B0), (T)ph.two_mu0);
+	}
+	for (int q = 0; q < 8; q++) c.F[q] = 0;
+	c.e_pf = physical_flux<T>(c.s, U, zero, (T)ph.inv_mu0, c.F);
+}
+
+/* flux_rusanov's per-face part */
+template <class T> __device__ __forceinline__ int flux_rusanov_face(const FluxCell<T>& L,
+	const FluxCell<T>& R, T F[8], T& s_out)
+{
+	using A = Arith<T>;
+	if (L.e_rho) return L.e_rho;
+	if (R.e_rho) return R.e_rho;
+	if (L.e_fs) return L.e_fs;
+	if (R.e_fs) return R.e_fs;
+	const T
+		max_fast = smax(L.cf, R.cf),
+		s = smax(L.aux + max_fast, R.aux + max_fast);
+	if (!A::isfinite(max_fast) || !A::isfinite(s)) return ERR_SPEED;
+	if (L.e_pf) return L.e_pf;
+	if (R.e_pf) return R.e_pf;
+	const T* FL = L.F;
+	const T* FR = R.F;
+	if (!A::isfinite(FL[RHO]) || !A::isfinite(FR[RHO])) return ERR_FLUX;
+	F[RHO] = (FL[RHO] + FR[RHO]) / T(2) - (R.s.u[RHO] - L.s.u[RHO]) * (s / T(2));
+	F[NRJ] = (FL[NRJ] + FR[NRJ]) / T(2) - (R.s.u[NRJ] - L.s.u[NRJ]) * (s / T(2));
+	const T hs = -s / T(2);
+	for (int i = 0; i < 3; i++) {
+		F[M0 + i] = (FL[M0 + i] + FR[M0 + i]) * T(0.5) + (R.s.u[M0 + i] - L.s.u[M0 + i]) * hs;
+		F[B0 + i] = (FL[B0 + i] + FR[B0 + i]) * T(0.5) + (R.s.u[B0 + i] - L.s.u[B0 + i]) * hs;
+	}
+	if (!A::isfinite(F[RHO])) return ERR_FLUX;
+	s_out = s;
+	return ERR_OK;
+}
+
+/* flux_hll's per-face part */
+template <class T> __device__ __forceinline__ int flux_hll_face(const FluxCell<T>& Lc,
+	const FluxCell<T>& Rc, T F[8], T& s_out)
+{
+	using A = Arith<T>;
+	const Side<T>& L = Lc.s;
+	const Side<T>& R = Rc.s;
+	if (Lc.e_rho || Rc.e_rho) return ERR_DENSITY;
+	if (Lc.e_fs) return Lc.e_fs;
+	if (Rc.e_fs) return Rc.e_fs;
+	const T
+		pmL = Lc.aux,
+		pmR = Rc.aux,
+		ms = smax(Lc.cf, Rc.cf),
+		msn = (L.v[0] <= R.v[0]) ? L.v[0] - ms : R.v[0] - ms,
+		msp = (L.v[0] <= R.v[0]) ? R.v[0] + ms : L.v[0] + ms,
+		bm = smin(msn, T(0)),
+		bp = smax(msp, T(0));
+	if (!__builtin_isnormal(L.p64) || L.p64 < 0) return ERR_PRESSURE;
+	if (!A::isfinite(pmL) || pmL < 0) return ERR_PRESSURE;
+	if (!A::isfinite(msn)) return ERR_SPEED;
+	if (!__builtin_isnormal(R.p64) || R.p64 < 0) return ERR_PRESSURE;
+	if (!A::isfinite(pmR) || pmR < 0) return ERR_PRESSURE;
+	if (!A::isfinite(msp)) return ERR_SPEED;
+
+	if (!A::isnormal(bp - bm) || bp - bm < 0) {
+		for (int q = 0; q < 8; q++) F[q] = 0;
+		s_out = 0;
+		return ERR_OK;
+	}
+	if (Lc.e_pf) return Lc.e_pf;
+	if (Rc.e_pf) return Rc.e_pf;
+	T FL[8], FR[8];
+	for (int q = 0; q < 8; q++) {
+		FL[q] = Lc.F[q];
+		FR[q] = Rc.F[q];
+	}
+	FL[RHO] -= L.u[RHO] * bm;
+	FL[NRJ] -= L.u[NRJ] * bm;
+	FR[RHO] -= R.u[RHO] * bp;
+	FR[NRJ] -= R.u[NRJ] * bp;
+	for (int i = 0; i < 3; i++) {
+		FL[M0 + i] = FL[M0 + i] + L.u[M0 + i] * (-bm);
+		FL[B0 + i] = FL[B0 + i] + L.u[B0 + i] * (-bm);
+		FR[M0 + i] = FR[M0 + i] + R.u[M0 + i] * (-bp);
+		FR[B0 + i] = FR[B0 + i] + R.u[B0 + i] * (-bp);
+	}
+	FR[B0] = FL[B0] = 0;
+	const T den = bp - bm, sum = bp + bm;
+	F[RHO] = (FL[RHO] + FR[RHO]) / T(2) + A::div((FL[RHO] - FR[RHO]) * sum, den) / T(2.0);
+	F[NRJ] = (FL[NRJ] + FR[NRJ]) / T(2) + A::div((FL[NRJ] - FR[NRJ]) * sum, den) / T(2.0);
+	const T w = A::div(sum, den) / T(2.0);
+	for (int i = 0; i < 3; i++) {
+		F[M0 + i] = (FL[M0 + i] + FR[M0 + i]) * T(0.5) + (FL[M0 + i] - FR[M0 + i]) * w;
+		F[B0 + i] = (FL[B0 + i] + FR[B0 + i]) * T(0.5) + (FL[B0 + i] - FR[B0 + i]) * w;
+	}
+	s_out = smax(fabs(bp), fabs(bm));
+	return ERR_OK;
+}
+
+template <int SOLVER, class T> __device__ __forceinline__ int flux_face(const FluxCell<T>& L,
+	const FluxCell<T>& R, T F[8], T& s_out)
+{
+	if constexpr (SOLVER == RUSANOV) return flux_rusanov_face<T>(L, R, F, s_out);
+	else return flux_hll_face<T>(L, R, F, s_out);
+}
+
 /* one side's star state, source/mhd/hlld_athena.hpp:316-401 (eqns 43-48) */
 template <class T> __device__ __forceinline__ void hlld_star(const Side<T>& s, T nm, T sfd,
 	T smd, T sm, T ds, T Bx, T Bx2, T inv, T ptst, T S[8], T& vdbs)

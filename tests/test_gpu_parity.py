@@ -168,6 +168,56 @@ def test_hlld_shared_sides_bit_exact(gpu, variant):
         np.testing.assert_array_equal(outs["mixed", variant, slabs], outs["mixed", (0, 0, 0), 1])
 
 
+@pytest.mark.parametrize("solver,sid", [("rusanov", O.RUSANOV), ("hll-athena", O.HLL)])
+@pytest.mark.parametrize("variant", [(0, 0, 0), (16, 0, 0), (64, 0, 0), (80, 8, 1), (96, 16, 0)])
+def test_hll_rusanov_cell_parts_bit_exact(gpu, solver, sid, variant):
+    """HLL and Rusanov with the per-cell part (checks, pressure, fast
+    speed, physical flux) computed once per cell along marches, the x-faces
+    too with each lane's part moved to the next lane: the oracle's bits in
+    FP64 (70x9x13: partial tiles, chunk ends), and the per-face kernel's
+    bits in mixed precision"""
+    from pamhd_amd import solver as S
+    cells = (70, 9, 13)
+    state, length, gamma, mu0 = _vortex(cells, seed=21)
+    og = O.make_grid(cells, length, gamma, mu0)
+    ref = state.copy()
+    st, steps_o, t_o = O.run_periodic(og, sid, 0.3, 0.0, 1e9, ref, max_steps=4)
+    assert st == 0
+    outs = {}
+    for prec in ("fp64", "mixed"):
+        for v in ((0, 0, 0), variant):
+            with S.Grid(cells, length, gamma, mu0, precision=prec) as g:
+                g.upload(state)
+                g.set_launch(*v)
+                g.run(solver, 0.3, 0.0, 1e9, max_steps=4)
+                outs[prec, v] = g.download()
+    np.testing.assert_array_equal(outs["fp64", variant], ref)
+    np.testing.assert_array_equal(outs["mixed", variant], outs["mixed", (0, 0, 0)])
+
+
+@pytest.mark.parametrize("solver", ["rusanov", "hll-athena"])
+@pytest.mark.parametrize("i", [5, 11])
+def test_hll_rusanov_cell_parts_report_errors(gpu, solver, i):
+    """a cell with negative thermal pressure stops the step with the same
+    error per face or with the per-cell parts (at i = 11 it is the lane-0
+    edge of the x-march)"""
+    from pamhd_amd import _lib
+    from pamhd_amd import solver as S
+    cells = (12, 10, 9)
+    state, length, gamma, mu0 = _vortex(cells)
+    c = i + 12 * (4 + 10 * 3)
+    state[4, c] = 0.01 * state[4, c]
+    codes = []
+    for flux_bits in (0, 16, 64):
+        with S.Grid(cells, length, gamma, mu0) as g:
+            g.upload(state)
+            g.set_launch(flux_bits, 0, 0)
+            with pytest.raises(_lib.PamhdError) as info:
+                g.timestep(solver, 0.0, 0.3)
+            codes.append(info.value.code)
+    assert codes[0] == codes[1] == codes[2] != 0
+
+
 @pytest.mark.parametrize("prec", ["fp64", "mixed"])
 @pytest.mark.parametrize("i", [5, 11])
 def test_hlld_shared_sides_report_errors(gpu, prec, i):
