@@ -343,7 +343,8 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen,
    computed once per cell and shared through LDS, bit 4 = that part carried
    along marches of 16 cells for the y- and z-faces (bit 5: 4 cells), bit 6
    = as bit 4 and the x-faces marched along y too, each lane's part moved to
-   the next lane (bits 3-6 for HLLD only; bit 1 is then ignored);
+   the next lane (bits 3-6 for HLLD, bits 4-6 also for HLL and Rusanov
+   on grids without a background field; bit 1 is then ignored);
    update_remap bits 0 and 1 the same
    orders for the update pass (bit 0 only for the z-marching update) and,
    for the z-marching update, bits 2-4 its tile (0: 64 x 4, 1: 128 x 2,
