@@ -332,7 +332,8 @@ state and the lagged time step are not changed.  Until it (or
 pamhd_set_launch) is called a handle uses the variants it picks for the
 256^3 HLLD step: XCD-aware flux block order, HLLD's per-cell part carried
 along 16-cell marches in all three directions, the z-marching update with 8
-planes per 128 x 2 tile.  (No counterpart in the reference.)
+planes per 128 x 2 tile (grids of fewer than 16 planes: per-face flux
+kernels and the per-cell update).  (No counterpart in the reference.)
 */
 int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen,
 	void* stream);

@@ -2551,6 +2551,15 @@ int pamhd_create(const pamhd_grid_desc* grid, const pamhd_physics* physics, int 
 	h->g = GridDev{grid->cells[0], grid->cells[1], grid->cells[2], n,
 		grid->cell_length[0], grid->cell_length[1], grid->cell_length[2],
 		make_phys(physics->adiabatic_index, physics->vacuum_permeability)};
+	/* thin grids: the x/y marches put one wave per block on each of fewer
+	   than BY planes and the column update walks fewer planes than its
+	   setup loads, so their defaults are per face and per cell */
+	if (grid->cells[2] < 4 * BY) {
+		h->tune.flux_remap = 0;
+		h->tune.flux_shared = 0;
+		h->tune.update_kz = 0;
+		h->tune.update_tile = 0;
+	}
 	auto cleanup = [&](int code, const char* what, hipError_t e) {
 		std::string m = std::string(what) + ": " + hipGetErrorString(e);
 		pamhd_destroy(h);
