@@ -70,8 +70,8 @@ int pamhd_ensemble_destroy(pamhd_ensemble* e)
 {
 	if (!e) return PAMHD_OK;
 	if (e->own) {
-		hipStreamSynchronize(e->own);
-		hipStreamDestroy(e->own);
+		(void)hipStreamSynchronize(e->own);
+		(void)hipStreamDestroy(e->own);
 	}
 	boxset_destroy(e->set);
 	delete e;
@@ -112,20 +112,20 @@ int pamhd_ensemble_run(pamhd_ensemble* e, int solver, double time_step_factor, l
 	int r2 = r1 ? r1 : boxset_launch_step(e->set, solver, 1e300, time_step_factor, true, cap, e->err);
 	he = hipStreamEndCapture(cap, &graph);
 	if (r2 || he != hipSuccess) {
-		if (graph) hipGraphDestroy(graph);
+		if (graph) (void)hipGraphDestroy(graph);
 		return efail(e, r2 ? r2 : PAMHD_E_DEVICE, r2 ? e->err : std::string("end capture: ") + hipGetErrorString(he));
 	}
 	he = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
 	if (he != hipSuccess) {
-		hipGraphDestroy(graph);
+		(void)hipGraphDestroy(graph);
 		return efail(e, PAMHD_E_DEVICE, std::string("instantiate: ") + hipGetErrorString(he));
 	}
 	const long reps = (steps - steps % 2) / 2;
 	for (long r = 0; r < reps && he == hipSuccess; r++) he = hipGraphLaunch(exec, cap);
 	/* the executable graph may be destroyed once its launches complete */
-	hipStreamSynchronize(cap);
-	hipGraphExecDestroy(exec);
-	hipGraphDestroy(graph);
+	(void)hipStreamSynchronize(cap);
+	(void)hipGraphExecDestroy(exec);
+	(void)hipGraphDestroy(graph);
 	if (he != hipSuccess) return efail(e, PAMHD_E_DEVICE, std::string("graph launch: ") + hipGetErrorString(he));
 	return PAMHD_OK;
 }
@@ -140,7 +140,7 @@ int pamhd_ensemble_check(pamhd_ensemble* e, double* dt_per_box, void* stream)
 int pamhd_ensemble_download(pamhd_ensemble* e, int box, double* state)
 {
 	if (!e || !state || box < 0 || box >= boxset_count(e->set)) return efail(e, PAMHD_E_ARG, "invalid box or null state");
-	hipStreamSynchronize(e->own);
+	(void)hipStreamSynchronize(e->own);
 	const hipError_t he = boxset_download(e->set, box, state);
 	return he == hipSuccess ? PAMHD_OK : efail(e, PAMHD_E_DEVICE, hipGetErrorString(he));
 }

@@ -109,31 +109,6 @@ __device__ __forceinline__ void tile_of_block(unsigned lb, unsigned gx, unsigned
 	}
 }
 
-/* cell state rotated into the frame of a d-face (get_rotated_vector,
-   source/mhd/common.hpp:612-648; vol B of update_vol_B, solve.hpp:1661) */
-__device__ __forceinline__ void load_face_state(const double* __restrict__ s, long long n,
-	const GridDev& g, int d, int i, int j, int k, double U[8])
-{
-	const long long c = i + (long long)g.nx * (j + (long long)g.ny * k);
-	const long long cx = wrap_inc(i, g.nx) + (long long)g.nx * (j + (long long)g.ny * k);
-	const long long cy = i + (long long)g.nx * (wrap_inc(j, g.ny) + (long long)g.ny * k);
-	const long long cz = i + (long long)g.nx * (j + (long long)g.ny * wrap_inc(k, g.nz));
-	const double m[3] = {s[n + c], s[2 * n + c], s[3 * n + c]};
-	const double B[3] = {
-		0.5 * (s[5 * n + c] + s[5 * n + cx]),
-		0.5 * (s[6 * n + c] + s[6 * n + cy]),
-		0.5 * (s[7 * n + c] + s[7 * n + cz])};
-	U[RHO] = s[c];
-	U[NRJ] = s[4 * n + c];
-	const int a = d, b = (d + 1) % 3, e = (d + 2) % 3;
-	U[M0] = m[a];
-	U[M1] = m[b];
-	U[M2] = m[e];
-	U[B0] = B[a];
-	U[B1] = B[b];
-	U[B2] = B[e];
-}
-
 __device__ __forceinline__ void record_error(StepStatus* st, int code, long long face)
 {
 	if (atomicCAS(&st->err, 0, code) == 0) {
@@ -1865,7 +1840,7 @@ struct TimedLaunch {
 	TimedLaunch(pamhd_handle* h_, hipStream_t st_, int k_) : h(h_), st(st_), k(k_)
 	{
 		if (h->timing && hipEventCreate(&a) == hipSuccess && hipEventCreate(&b) == hipSuccess) {
-			hipEventRecord(a, st);
+			(void)hipEventRecord(a, st);
 		} else {
 			a = b = nullptr;
 		}
@@ -1873,7 +1848,7 @@ struct TimedLaunch {
 	~TimedLaunch()
 	{
 		if (a && b) {
-			hipEventRecord(b, st);
+			(void)hipEventRecord(b, st);
 			h->timed.push_back({k, a, b});
 		}
 	}
@@ -1882,14 +1857,14 @@ struct TimedLaunch {
 void collect_times(pamhd_handle* h)
 {
 	for (auto& t : h->timed) {
-		hipEventSynchronize(t.stop);
+		(void)hipEventSynchronize(t.stop);
 		float ms = 0;
 		if (hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
 			h->kernel_ms[t.kernel] += ms;
 			h->kernel_launches[t.kernel]++;
 		}
-		hipEventDestroy(t.start);
-		hipEventDestroy(t.stop);
+		(void)hipEventDestroy(t.start);
+		(void)hipEventDestroy(t.stop);
 	}
 	h->timed.clear();
 }
@@ -2313,24 +2288,24 @@ __global__ __launch_bounds__(BX * BY) void bnd_refit_kernel(GridDev g, SubstepDe
 void substep_free(SubstepDev* d)
 {
 	if (!d) return;
-	hipFree(d->S);
-	hipFree(d->slot);
-	hipFree(d->fdb);
-	hipFree(d->maxv);
-	hipFree(d->ts);
-	hipFree(d->e[0]);
-	hipFree(d->e[1]);
-	hipFree(d->period);
-	hipFree(d->minsub);
-	hipFree(d->opt_dev);
-	hipFree(d->types);
-	hipFree(d->bmask);
-	hipFree(d->bval);
-	hipFree(d->scal);
-	hipFree(d->sub_dt);
-	hipFree(d->status);
-	hipHostFree(d->host);
-	hipHostFree(d->host_status);
+	(void)hipFree(d->S);
+	(void)hipFree(d->slot);
+	(void)hipFree(d->fdb);
+	(void)hipFree(d->maxv);
+	(void)hipFree(d->ts);
+	(void)hipFree(d->e[0]);
+	(void)hipFree(d->e[1]);
+	(void)hipFree(d->period);
+	(void)hipFree(d->minsub);
+	(void)hipFree(d->opt_dev);
+	(void)hipFree(d->types);
+	(void)hipFree(d->bmask);
+	(void)hipFree(d->bval);
+	(void)hipFree(d->scal);
+	(void)hipFree(d->sub_dt);
+	(void)hipFree(d->status);
+	(void)hipHostFree(d->host);
+	(void)hipHostFree(d->host_status);
 	delete d;
 }
 
@@ -2690,24 +2665,24 @@ int pamhd_destroy(pamhd_handle* h)
 {
 	if (!h) return PAMHD_OK;
 	collect_times(h);
-	hipFree(h->state[0]);
-	hipFree(h->state[1]);
-	hipFree(h->flux);
-	hipFree(h->bg);
-	hipFree(h->dt);
-	hipFree(h->status);
-	hipFree(h->scratch);
+	(void)hipFree(h->state[0]);
+	(void)hipFree(h->state[1]);
+	(void)hipFree(h->flux);
+	(void)hipFree(h->bg);
+	(void)hipFree(h->dt);
+	(void)hipFree(h->status);
+	(void)hipFree(h->scratch);
 	if (h->mg) {
 		mg_destroy(*h->mg);
 		delete h->mg;
 	}
 	boxset_destroy(h->box);
 	substep_free(h->sub);
-	hipFree(h->adj);
-	for (auto e : h->ev) hipEventDestroy(e);
-	if (h->aux) hipStreamDestroy(h->aux);
-	hipHostFree(h->host_dt);
-	hipHostFree(h->host_status);
+	(void)hipFree(h->adj);
+	for (auto e : h->ev) (void)hipEventDestroy(e);
+	if (h->aux) (void)hipStreamDestroy(h->aux);
+	(void)hipHostFree(h->host_dt);
+	(void)hipHostFree(h->host_status);
 	delete h;
 	return PAMHD_OK;
 }
@@ -2774,7 +2749,7 @@ int pamhd_set_background_faces(pamhd_handle* h, const double* bg)
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "solar-wind boxes compute their background field");
 	if (!bg) {
-		hipFree(h->bg);
+		(void)hipFree(h->bg);
 		h->bg = nullptr;
 		return PAMHD_OK;
 	}
@@ -2867,7 +2842,7 @@ int pamhd_div_face2volume(pamhd_handle* h, double* avg)
 	const dim3 grid = cell_grid(h->g);
 	const size_t nb = (size_t)grid.x * grid.y * grid.z;
 	if (h->scratch_len < nb) {
-		hipFree(h->scratch);
+		(void)hipFree(h->scratch);
 		h->scratch = nullptr;
 		HIP_TRY(h, hipMalloc(&h->scratch, sizeof(double) * nb));
 		h->scratch_len = nb;
@@ -3132,11 +3107,11 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	auto time_it = [&](auto&& launch) -> float {
 		launch();
 		float best = 0;
-		hipEventRecord(a, st);
+		(void)hipEventRecord(a, st);
 		for (int r = 0; r < 3; r++) launch();
-		hipEventRecord(b, st);
-		hipEventSynchronize(b);
-		hipEventElapsedTime(&best, a, b);
+		(void)hipEventRecord(b, st);
+		(void)hipEventSynchronize(b);
+		(void)hipEventElapsedTime(&best, a, b);
 		return best / 3;
 	};
 	auto flux = [&]() {
@@ -3225,8 +3200,8 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 		}
 		h->tune.slabs = best_slabs;
 	}
-	hipEventDestroy(a);
-	hipEventDestroy(b);
+	(void)hipEventDestroy(a);
+	(void)hipEventDestroy(b);
 	HIP_TRY(h, hipGetLastError());
 	HIP_TRY(h, hipStreamSynchronize(st));
 	h->timing = timing;

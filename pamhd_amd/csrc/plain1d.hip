@@ -156,10 +156,10 @@ int plain1d_run(long long n, double cell_size, double gamma, double mu0, double 
 	PlainStatus* st = nullptr;
 	PlainStatus* host = nullptr;
 	auto release = [&]() {
-		if (S) hipFree(S);
-		if (PF) hipFree(PF);
-		if (st) hipFree(st);
-		if (host) hipHostFree(host);
+		if (S) (void)hipFree(S);
+		if (PF) (void)hipFree(PF);
+		if (st) (void)hipFree(st);
+		if (host) (void)hipHostFree(host);
 	};
 	auto hip_err = [&](hipError_t e, const char* what) {
 		err = std::string(what) + ": " + hipGetErrorString(e);

@@ -448,14 +448,14 @@ hipError_t mg_create(PoissonMG& mg, const int dims[3], const double h[3])
 void mg_destroy(PoissonMG& mg)
 {
 	for (auto& L : mg.lv) {
-		hipFree(L.phi);
-		hipFree(L.f);
-		hipFree(L.r);
+		(void)hipFree(L.phi);
+		(void)hipFree(L.f);
+		(void)hipFree(L.r);
 	}
 	mg.lv.clear();
-	hipFree(mg.partial);
-	hipFree(mg.result);
-	hipHostFree(mg.host_result);
+	(void)hipFree(mg.partial);
+	(void)hipFree(mg.result);
+	(void)hipHostFree(mg.host_result);
 	mg.partial = mg.result = mg.host_result = nullptr;
 	mg.partial_len = 0;
 }

@@ -1205,7 +1205,7 @@ template <class T> __device__ __forceinline__ int flux_roe(const double Ld[8],
 	const double Rd[8], const T* /*bg*/, const Phys& ph, T F[8], T& s_out)
 {
 	using A = Arith<T>;
-	const T gamma = (T)ph.gamma, mu0 = (T)ph.mu0, inv_mu0 = (T)ph.inv_mu0;
+	const T gamma = (T)ph.gamma, mu0 = (T)ph.mu0;
 	const T zero[3] = {0, 0, 0};
 	if (Ld[RHO] <= 0) return ERR_DENSITY;
 	if (Rd[RHO] <= 0) return ERR_DENSITY;

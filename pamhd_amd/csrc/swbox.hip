@@ -684,12 +684,12 @@ hipError_t build_blocks(BoxSet* s)
 		for (long long q = 0; q < nb; q++) bb.push_back(b);
 	}
 	s->nblocks = (int)bb.size();
-	hipFree(s->blockbox);
+	(void)hipFree(s->blockbox);
 	s->blockbox = nullptr;
 	hipError_t e;
 	if ((e = hipMalloc(&s->blockbox, sizeof(int) * bb.size())) != hipSuccess) return e;
 	if ((e = hipMemcpy(s->blockbox, bb.data(), sizeof(int) * bb.size(), hipMemcpyHostToDevice)) != hipSuccess) return e;
-	hipFree(s->partial);
+	(void)hipFree(s->partial);
 	s->partial = nullptr;
 	if ((e = hipMalloc(&s->partial, sizeof(double) * bb.size())) != hipSuccess) return e;
 	return hipMemcpy(s->boxes, s->host.data(), sizeof(BoxDev) * s->nbox, hipMemcpyHostToDevice);
@@ -726,18 +726,18 @@ int boxset_set_tile(BoxSet* s, int tile)
 void boxset_destroy(BoxSet* s)
 {
 	if (!s) return;
-	hipFree(s->boxes);
-	hipFree(s->blockbox);
-	hipFree(s->S);
-	hipFree(s->P);
-	hipFree(s->bg);
-	hipFree(s->ctype);
-	hipFree(s->rule);
-	hipFree(s->status);
-	hipFree(s->dt);
-	hipFree(s->partial);
-	hipHostFree(s->host_status);
-	hipHostFree(s->host_dt);
+	(void)hipFree(s->boxes);
+	(void)hipFree(s->blockbox);
+	(void)hipFree(s->S);
+	(void)hipFree(s->P);
+	(void)hipFree(s->bg);
+	(void)hipFree(s->ctype);
+	(void)hipFree(s->rule);
+	(void)hipFree(s->status);
+	(void)hipFree(s->dt);
+	(void)hipFree(s->partial);
+	(void)hipHostFree(s->host_status);
+	(void)hipHostFree(s->host_dt);
 	delete s;
 }
 
@@ -1007,16 +1007,16 @@ int boxset_autotune(BoxSet* s, int solver, double cfl, int steps, hipStream_t st
 	hipError_t e;
 	const size_t nS = sizeof(double) * NQ * s->N, nSt = sizeof(BoxStatus) * 2 * s->nbox;
 	if ((e = hipMalloc(&saveS, nS)) != hipSuccess || (e = hipMalloc(&saveSt, nSt)) != hipSuccess) {
-		hipFree(saveS);
+		(void)hipFree(saveS);
 		err = std::string("autotune buffers: ") + hipGetErrorString(e);
 		return PAMHD_E_DEVICE;
 	}
 	hipEvent_t a, b;
-	hipEventCreate(&a);
-	hipEventCreate(&b);
+	(void)hipEventCreate(&a);
+	(void)hipEventCreate(&b);
 	const int last0 = s->last;
-	hipMemcpyAsync(saveS, s->S, nS, hipMemcpyDeviceToDevice, st);
-	hipMemcpyAsync(saveSt, s->status, nSt, hipMemcpyDeviceToDevice, st);
+	(void)hipMemcpyAsync(saveS, s->S, nS, hipMemcpyDeviceToDevice, st);
+	(void)hipMemcpyAsync(saveSt, s->status, nSt, hipMemcpyDeviceToDevice, st);
 	int best = s->tile;
 	float best_ms = 1e30f;
 	int rc = PAMHD_OK;
@@ -1027,29 +1027,29 @@ int boxset_autotune(BoxSet* s, int solver, double cfl, int steps, hipStream_t st
 			break;
 		}
 		rc = boxset_launch_step(s, solver, 1e300, cfl, true, st, err);
-		hipEventRecord(a, st);
+		(void)hipEventRecord(a, st);
 		for (int q = 0; q < steps && rc == PAMHD_OK; q++) rc = boxset_launch_step(s, solver, 1e300, cfl, true, st, err);
-		hipEventRecord(b, st);
-		hipEventSynchronize(b);
+		(void)hipEventRecord(b, st);
+		(void)hipEventSynchronize(b);
 		float ms = 0;
-		hipEventElapsedTime(&ms, a, b);
+		(void)hipEventElapsedTime(&ms, a, b);
 		ms /= steps;
 		report += " " + std::to_string(TILES[t].tx) + "x" + std::to_string(TILES[t].ty) + "=" + std::to_string(ms) + "ms";
 		if (ms < best_ms) {
 			best_ms = ms;
 			best = t;
 		}
-		hipMemcpyAsync(s->S, saveS, nS, hipMemcpyDeviceToDevice, st);
-		hipMemcpyAsync(s->status, saveSt, nSt, hipMemcpyDeviceToDevice, st);
+		(void)hipMemcpyAsync(s->S, saveS, nS, hipMemcpyDeviceToDevice, st);
+		(void)hipMemcpyAsync(s->status, saveSt, nSt, hipMemcpyDeviceToDevice, st);
 		s->last = last0;
 	}
-	hipStreamSynchronize(st);
+	(void)hipStreamSynchronize(st);
 	if (rc == PAMHD_OK) rc = boxset_set_tile(s, best);
 	report += "; chosen " + std::to_string(TILES[best].tx) + "x" + std::to_string(TILES[best].ty);
-	hipEventDestroy(a);
-	hipEventDestroy(b);
-	hipFree(saveS);
-	hipFree(saveSt);
+	(void)hipEventDestroy(a);
+	(void)hipEventDestroy(b);
+	(void)hipFree(saveS);
+	(void)hipFree(saveSt);
 	return rc;
 }
 
