@@ -368,6 +368,13 @@ def precision_study(size=512, steps=160, clean_every=10, report_every=20, solver
 # the step's launches) measured with rocprofv3 --pmc for the launch variants
 # the autotuner picked: profiles/step_kernels_256cubed_hlld_counters.md
 STEP_TRAFFIC_BYTES = {("fp64", 256, "hlld-athena"): 14.34e9, ("mixed", 256, "hlld-athena"): 10.13e9}
+# VALU wave instructions of one step's three flux launches (SQ_INSTS_VALU of
+# the same profiled runs, the launch variants the autotuner picked; the flux
+# passes are VALU-issue bound)
+FLUX_VALU_WAVE_INSTR = {("fp64", 256, "hlld-athena"): 7.47e8, ("mixed", 256, "hlld-athena"): 4.08e8}
+# one wave64 VALU instruction per clock per CU: 256 CUs x 2.4 GHz (the
+# 78.6 TFLOP/s FP64 vector peak is 128 FLOP, one 64-lane FMA, per clock per CU)
+VALU_PEAK_WAVE_INSTR_PER_S = 256 * 2.4e9
 
 
 def roofline(res, n, steps, key=None):
@@ -380,6 +387,13 @@ def roofline(res, n, steps, key=None):
     upd_ms, upd_n = kt["update"]
     step_ms = res["ms"] / steps
     achieved = BYTES_PER_CELL_UPDATE * n / (step_ms / 1e3) / 1e9
+    valu = None
+    if key in FLUX_VALU_WAVE_INSTR and flux_n:
+        rate = FLUX_VALU_WAVE_INSTR[key] / (flux_ms / flux_n / 1e3)
+        valu = {"bound": "valu", "kernel": "flux_pass (x, y, z faces)", "achieved": rate,
+                "peak": VALU_PEAK_WAVE_INSTR_PER_S, "unit": "wave instructions/s",
+                "frac": round(rate / VALU_PEAK_WAVE_INSTR_PER_S, 4),
+                "instructions": FLUX_VALU_WAVE_INSTR[key]}
     return {"bound": "hbm", "kernel": "step (flux_pass + update)",
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": STEP_TRAFFIC_BYTES.get(key),
@@ -388,7 +402,8 @@ def roofline(res, n, steps, key=None):
             "tuning": res["tuning"].split("; chosen ")[-1],
             "kernel_ms_serialised": {"flux_pass": round(flux_ms / flux_n, 4),
                                      "update": round(upd_ms / upd_n, 4),
-                                     "dt": round(kt["dt"][0] / max(kt["dt"][1], 1), 4)}}
+                                     "dt": round(kt["dt"][0] / max(kt["dt"][1], 1), 4)},
+            "flux_valu": valu}
 
 
 def main():
