@@ -116,6 +116,44 @@ def run_gpu(prec, state, cells, length, gamma, mu0, solver, cfl, steps, warmup):
             "tuning": tuning, "final": final}
 
 
+def substep_run(size=256, steps=10, solver="hlld-athena", cfl=0.3):
+    """Per-cell 2^N substepping with the reference's default options
+    (substep-min 0, substep-max 999) on a 'size'^3 periodic box at rest with
+    a hot sphere (30x the background pressure): FP64 ms per main-loop step,
+    the largest period and the cells' mean period."""
+    import torch
+    from pamhd_amd import configs
+    from pamhd_amd.solver import Grid
+    cells = (size,) * 3
+    state, length, gamma, mu0 = configs.hot_blob(cells, contrast=30.0, radius=0.25, seed=3)
+    n = size ** 3
+    with Grid(cells, length, gamma, mu0) as g:
+        g.upload(state)
+        g.set_substepping(0, 999)
+        stream = torch.cuda.current_stream().cuda_stream
+        g.apply_boundaries(0.0, stream=stream)
+        g.timestep(solver, 0.0, cfl, stream=stream)
+        for _ in range(2):
+            g.step(solver, 1e300, cfl, stream=stream)
+        torch.cuda.synchronize()
+        start = torch.cuda.Event(enable_timing=True)
+        stop = torch.cuda.Event(enable_timing=True)
+        start.record()
+        for _ in range(steps):
+            g.step(solver, 1e300, cfl, stream=stream)
+        stop.record()
+        torch.cuda.synchronize()
+        ms = start.elapsed_time(stop) / steps
+        g.check()
+        largest = g.last_max_substep()
+    del state
+    return {"cells": list(cells), "solver": solver, "cfl": cfl, "substep_min": 0,
+            "substep_max": 999, "steps": steps, "ms_per_step": round(ms, 4),
+            "cell_updates_per_s": n / (ms / 1e3), "largest_period": largest,
+            "workload": "periodic box at rest, uniform field, hot sphere of 30x the pressure "
+                        "(radius 0.25), +-1% density noise"}
+
+
 def rel_l2(a, b):
     """Relative L2 error per conserved variable of a against b; momentum and
     face field as vectors (their z components are roundoff-sized in this
@@ -420,6 +458,8 @@ def main():
                     help="0 skips the adjoint workload")
     ap.add_argument("--other-solver-steps", type=int, default=20,
                     help="timed FP64 steps of Rusanov, HLL and Roe on the same workload (0: skip)")
+    ap.add_argument("--substep-steps", type=int, default=10,
+                    help="timed steps of the per-cell substepping record (0: skip)")
     ap.add_argument("--ensemble-boxes", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=500,
                     help="0 skips the ensemble workload")
@@ -486,6 +526,12 @@ def main():
     clean = divergence_cleaning(cells)
     log("  %s" % clean)
 
+    sub = None
+    if args.substep_steps > 0:
+        log("per-cell substepping, %d^3 hot sphere, options (0, 999)" % args.size)
+        sub = substep_run(args.size, args.substep_steps, args.solver, args.cfl)
+        log("  %.3f ms/step, largest period %d" % (sub["ms_per_step"], sub["largest_period"]))
+
     ens = None
     if args.ensemble_steps > 0:
         log("ensemble: %d solar-wind boxes, %d steps" % (args.ensemble_boxes, args.ensemble_steps))
@@ -535,6 +581,7 @@ def main():
                   "rel_l2_vs_fp64": err,
                   "roofline": roofline(mix, n, args.steps, ("mixed", args.size, args.solver))},
         "other_solvers_fp64": others,
+        "substepping": sub,
         "divergence_cleaning": clean,
         "ensemble": ens,
         "adjoint": adj,

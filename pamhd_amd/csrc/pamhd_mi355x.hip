@@ -1404,16 +1404,42 @@ template <class F> __device__ __forceinline__ void for_face_edge_neighbours(cons
 	}
 }
 
+/* the step's four reductions (0: smallest cell time step, 1: largest
+   period, 2: smallest time step / 2^substep-min, 3: largest level) are
+   spread over NSLOT slots each, a wave's atomic going to the slot of its
+   block (one address per reduction serialises 2^18 atomics at 256^3), and
+   folded into slot 0 by ss_fold_kernel before they are read */
+__device__ __forceinline__ unsigned ss_slot()
+{
+	return (blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) % NSLOT;
+}
+
 __global__ void ss_clear_kernel(SubstepDev d)
 {
+	const unsigned long long big = (unsigned long long)__double_as_longlong(DBL_MAX);
+	for (int s = threadIdx.x; s < NSLOT; s += blockDim.x) {
+		d.scal[0 * NSLOT + s] = big;
+		d.scal[1 * NSLOT + s] = 0;
+		d.scal[2 * NSLOT + s] = big;
+		d.scal[3 * NSLOT + s] = 0;
+	}
 	if (threadIdx.x == 0) {
-		d.scal[0] = (unsigned long long)__double_as_longlong(DBL_MAX);
-		d.scal[1] = 0;
-		d.scal[2] = (unsigned long long)__double_as_longlong(DBL_MAX);
-		d.scal[3] = 0;
 		d.status->err = 0;
 		d.status->err_face = -1;
 	}
+}
+
+/* reduction q's NSLOT slots into slot 0 (one wave): min when is_min (the
+   slots hold non-negative doubles' bit patterns, which order as the values
+   do), max otherwise */
+__global__ __launch_bounds__(NSLOT) void ss_fold_kernel(SubstepDev d, int q, int is_min)
+{
+	unsigned long long v = d.scal[q * NSLOT + threadIdx.x];
+	for (int off = NSLOT / 2; off > 0; off >>= 1) {
+		const unsigned long long o = __shfl_xor(v, off, 64);
+		v = is_min ? (o < v ? o : v) : (o > v ? o : v);
+	}
+	if (threadIdx.x == 0) d.scal[q * NSLOT] = v;
 }
 
 /* minimize_timestep, source/mhd/solve.hpp:1927-2014: a cell's limit over its
@@ -1455,8 +1481,8 @@ __global__ __launch_bounds__(BX * BY) void ss_ts_kernel(GridDev g, SubstepDev d,
 		r = smin(r, __shfl_xor(r, off, 64));
 	}
 	if ((threadIdx.x + BX * threadIdx.y) % 64 == 0) {
-		atomicMin(&d.scal[0], (unsigned long long)__double_as_longlong(t));
-		atomicMin(&d.scal[2], (unsigned long long)__double_as_longlong(r));
+		atomicMin(&d.scal[0 * NSLOT + ss_slot()], (unsigned long long)__double_as_longlong(t));
+		atomicMin(&d.scal[2 * NSLOT + ss_slot()], (unsigned long long)__double_as_longlong(r));
 	}
 }
 
@@ -1469,7 +1495,7 @@ __global__ __launch_bounds__(BX * BY) void ss_level_kernel(GridDev g, SubstepDev
 {
 	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
 	const double smallest = __longlong_as_double((long long)d.scal[0]);
-	const double ret = __longlong_as_double((long long)d.scal[2]);
+	const double ret = __longlong_as_double((long long)d.scal[2 * NSLOT]);
 	const bool all_zero = max_dt <= smallest;
 	if (i == 0 && j == 0 && k == 0) *d.sub_dt = all_zero ? max_dt : ret;
 	int lev = 0;
@@ -1493,7 +1519,8 @@ __global__ __launch_bounds__(BX * BY) void ss_level_kernel(GridDev g, SubstepDev
 		const int o = __shfl_xor(lev, off, 64);
 		lev = o > lev ? o : lev;
 	}
-	if ((threadIdx.x + BX * threadIdx.y) % 64 == 0) atomicMax(&d.scal[3], (unsigned long long)lev);
+	if ((threadIdx.x + BX * threadIdx.y) % 64 == 0)
+		atomicMax(&d.scal[3 * NSLOT + ss_slot()], (unsigned long long)lev);
 }
 
 /* one Jacobi pass of restrict_substepping_period, substepping.hpp:99-191 */
@@ -1530,7 +1557,7 @@ __global__ __launch_bounds__(BX * BY) void ss_period_kernel(GridDev g, SubstepDe
 		const unsigned long long o = __shfl_xor(p, off, 64);
 		p = o > p ? o : p;
 	}
-	if ((threadIdx.x + BX * threadIdx.y) % 64 == 0) atomicMax(&d.scal[1], p);
+	if ((threadIdx.x + BX * threadIdx.y) % 64 == 0) atomicMax(&d.scal[1 * NSLOT + ss_slot()], p);
 }
 
 /* the working layout from the face-once state */
@@ -2327,7 +2354,7 @@ int substep_alloc(pamhd_handle* h)
 	al(&d->period, n * sizeof(int));
 	al(&d->minsub, n * sizeof(int));
 	al(&d->opt_dev, 2 * n * sizeof(int));
-	al(&d->scal, 4 * sizeof(unsigned long long));
+	al(&d->scal, 4 * NSLOT * sizeof(unsigned long long));
 	al(&d->sub_dt, sizeof(double));
 	al(&d->status, sizeof(StepStatus));
 	if (e == hipSuccess) e = hipHostMalloc((void**)&d->host, 3 * sizeof(unsigned long long));
@@ -2385,9 +2412,12 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 	double* P = h->flux;   /* 24 x n FP64 (substepping runs in FP64) */
 	ss_clear_kernel<<<1, 64, 0, st>>>(d);
 	ss_ts_kernel<<<grid, block, 0, st>>>(h->g, d, cfl);
+	ss_fold_kernel<<<1, NSLOT, 0, st>>>(d, 0, 1);
+	ss_fold_kernel<<<1, NSLOT, 0, st>>>(d, 2, 1);
 	ss_level_kernel<<<grid, block, 0, st>>>(h->g, d, max_dt);
+	ss_fold_kernel<<<1, NSLOT, 0, st>>>(d, 3, 0);
 	HIP_TRY(h, hipGetLastError());
-	HIP_TRY(h, hipMemcpyAsync(d.host + 2, d.scal + 3, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+	HIP_TRY(h, hipMemcpyAsync(d.host + 2, d.scal + 3 * NSLOT, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
 	HIP_TRY(h, hipMemcpyAsync(d.host_status, d.status, sizeof(StepStatus), hipMemcpyDeviceToHost, st));
 	HIP_TRY(h, hipStreamSynchronize(st));
 	if (d.host_status->err) return report_status(h, *d.host_status);
@@ -2401,8 +2431,9 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 		cur = 1 - cur;
 	}
 	ss_period_kernel<<<grid, block, 0, st>>>(h->g, d, d.e[cur]);
+	ss_fold_kernel<<<1, NSLOT, 0, st>>>(d, 1, 0);
 	HIP_TRY(h, hipGetLastError());
-	HIP_TRY(h, hipMemcpyAsync(d.host, d.scal + 1, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+	HIP_TRY(h, hipMemcpyAsync(d.host, d.scal + 1 * NSLOT, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
 	HIP_TRY(h, hipMemcpyAsync(d.host + 1, d.sub_dt, sizeof(double), hipMemcpyDeviceToHost, st));
 	HIP_TRY(h, hipStreamSynchronize(st));
 	const int maxp = (int)d.host[0];
