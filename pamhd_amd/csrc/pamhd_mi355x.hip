@@ -1662,18 +1662,23 @@ __global__ __launch_bounds__(BX * BY) void ss_update_kernel(GridDev g, SubstepDe
 		dBy_at(E, L, i, im, ip, jp, j, k, km, kp),
 		dBz_at(E, L, i, im, ip, j, jm, jp, k, km),
 		dBz_at(E, L, i, im, ip, j, jm, jp, kp, k)};
+	/* a cell of period 1 never accumulates, and every cell's sums are
+	   cleared at the end of its period (the step's last substep ends every
+	   period), so its sums are +0 here: they are neither read nor cleared */
+	const int period = d.period[c];
+	const bool held = period > 1;
 	double fd[6];
-	for (int f = 0; f < 6; f++) fd[f] = d.fdb[f * n + c] + db[f];
+	for (int f = 0; f < 6; f++) fd[f] = (held ? d.fdb[f * n + c] : 0.0) + db[f];
 	const long long cp[3] = {L(ip, j, k), L(i, jp, k), L(i, j, kp)};
 	double sl[6][8];
 	for (int q3 = 0; q3 < 3; q3++)
 	for (int q = 0; q < 8; q++) {
-		sl[2 * q3][q] = d.slot[((2 * q3) * 8 + q) * n + c] + P[(q3 * 8 + q) * n + c];
-		sl[2 * q3 + 1][q] = d.slot[((2 * q3 + 1) * 8 + q) * n + c];
+		sl[2 * q3][q] = (held ? d.slot[((2 * q3) * 8 + q) * n + c] : 0.0) + P[(q3 * 8 + q) * n + c];
+		sl[2 * q3 + 1][q] = held ? d.slot[((2 * q3 + 1) * 8 + q) * n + c] : 0.0;
 		if (cp[q3] >= 0) sl[2 * q3 + 1][q] += P[(q3 * 8 + q) * n + cp[q3]];
 	}
 	const int ct = d.ct ? d.ct[c] : 1;
-	const bool due = sub % d.period[c] == 0;
+	const bool due = sub % period == 0;
 	if (ct > 0 && !due) {
 		/* normal cells accumulate until their period ends */
 		for (int f = 0; f < 6; f++) d.fdb[f * n + c] = fd[f];
@@ -1681,9 +1686,11 @@ __global__ __launch_bounds__(BX * BY) void ss_update_kernel(GridDev g, SubstepDe
 		for (int q = 0; q < 8; q++) d.slot[(dir * 8 + q) * n + c] = sl[dir][q];
 		return;
 	}
-	for (int f = 0; f < 6; f++) d.fdb[f * n + c] = 0;
-	for (int dir = 0; dir < 6; dir++)
-	for (int q = 0; q < 8; q++) d.slot[(dir * 8 + q) * n + c] = 0;
+	if (held) {
+		for (int f = 0; f < 6; f++) d.fdb[f * n + c] = 0;
+		for (int dir = 0; dir < 6; dir++)
+		for (int q = 0; q < 8; q++) d.slot[(dir * 8 + q) * n + c] = 0;
+	}
 	if (ct <= 0) {
 		/* boundary cells: update_vol_B at their period (constant
 		   pressure, own face copies); cells not solved: nothing */
