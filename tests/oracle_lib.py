@@ -12,7 +12,9 @@ import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 ORACLE_DIR = os.path.join(ROOT, "oracle")
-ORACLE_SO = os.path.join(ORACLE_DIR, "liboracle.so")
+# PAMHD_ORACLE_SO points the tests at another build of the oracle (a
+# deliberately broken copy, to check that the pins catch it)
+ORACLE_SO = os.environ.get("PAMHD_ORACLE_SO", os.path.join(ORACLE_DIR, "liboracle.so"))
 REF_SO = os.path.join(ORACLE_DIR, "_ref", "libref_flux.so")
 
 RUSANOV, RUSANOV_STAGGERED, HLL, HLLD, ROE, HYBRID = range(6)
