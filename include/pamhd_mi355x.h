@@ -290,6 +290,13 @@ int pamhd_set_boundary_values(pamhd_handle* h, const unsigned char* mask,
    value (1 without substepping). */
 int pamhd_last_max_substep(pamhd_handle* h, int* max_substep);
 
+/* Faces solved by the last pamhd_timestep / pamhd_step (both cells of type
+   >= 0; with substepping each face once per substep it is due):
+   timestep()'s third return value flux_calcs (source/mhd/solve.hpp:1918,
+   summed by tests/mhd/test.cpp:612-627).  Call after the step has
+   completed (a step given dt_taken synchronises). */
+int pamhd_last_flux_calcs(pamhd_handle* h, unsigned long long* flux_calcs);
+
 /* Largest face signal speed per dimension of the last flux pass (synchronous). */
 int pamhd_max_signal_speeds(pamhd_handle* h, double smax[3]);
 
@@ -352,6 +359,16 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen,
    2: 32 x 8, 3: 128 x 4, 4: 64 x 8 threads); update_kz = 0 for the per-cell update or the planes
    per block of the z-marching update.  All give identical results. */
 int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_remap);
+
+/* HLLD's fused step (the default, fused = 1): one pass computes every face
+   flux of the step and the conserved update of each cell without a flux
+   array, keeping only the two field components of each face that the
+   constrained-transport edge terms use (6 doubles per cell); a second pass
+   updates the face field and refits the energy.  fused = 0 selects the
+   per-direction flux passes and the update pass.  planes_per_chunk (1..32,
+   0 = keep) is the z-extent of the fused pass's blocks.  Both give
+   identical results.  (No counterpart in the reference.) */
+int pamhd_set_fused_step(pamhd_handle* h, int fused, int planes_per_chunk);
 
 /* Pipelines the flux pass and the per-cell update over `slabs` z-slabs on
    two streams (1 = no pipelining; capped at nz / 2; ignored while kernel

@@ -39,6 +39,15 @@ static inline double dot3(const double* a, const double* b)
 /* ---------------------------------------------------------------------- */
 
 /* source/mhd/common.hpp:76-115 */
+/* number of faces the last step solved: the third return value of
+   timestep(), source/mhd/solve.hpp:253-313, :1918 */
+static unsigned long long or_fc = 0;
+
+unsigned long long or_last_flux_calcs(void)
+{
+	return or_fc;
+}
+
 int or_get_pressure(const double U[8], double gamma, double mu0, double* p)
 {
 	if (U[RHO] <= 0) return OR_ERR_DENSITY;
@@ -978,6 +987,7 @@ int or_step_periodic(const or_grid* g, int solver, double dt, int refits,
 
 	/* flux pass of get_fluxes, source/mhd/solve.hpp:304-393 with get_flux :68-188 */
 	long bad = -1;
+	or_fc = 3ull * (unsigned long long)n;
 	for (int d = 0; d < 3 && !err; d++) {
 		double smd = 0;
 		long zcd = 0;
@@ -1436,6 +1446,7 @@ int or_step_substep(const or_grid* g, int solver, double time_step_factor, doubl
 
 	double total_dt = 0;
 	const double dx = g->dx, dy = g->dy, dz = g->dz;
+	or_fc = 0;
 	for (int sub = 1; sub <= maxp; sub++) {
 		total_dt += sub_dt;
 		/* fluxes of the faces due at this substep; P = 0 for the others */
@@ -1451,6 +1462,7 @@ int or_step_substep(const or_grid* g, int solver, double time_step_factor, doubl
 				for (int q = 0; q < 8; q++) P[(d * 8 + q) * n + c] = 0;
 				continue;
 			}
+			or_fc++;
 			double L[8], R[8], bgr[3] = {0, 0, 0}, F[8], sp = 0;
 			sub_face_state(S, n, d, o, L);
 			sub_face_state(S, n, d, c, R);
@@ -2460,6 +2472,7 @@ int or_box_step(const or_box* b, int solver, double dt, double* s, double smax_d
 	/* flux pass over faces whose both cells have type >= 0
 	   (source/mhd/solve.hpp:262-393); the face is owned by its positive-side
 	   cell c, between L = c - e_d and R = c */
+	or_fc = 0;
 	for (int d = 0; d < 3 && !err; d++) {
 		for (long k = 0; k < b->nz; k++)
 		for (long j = 0; j < b->ny; j++)
@@ -2468,6 +2481,7 @@ int or_box_step(const or_box* b, int solver, double dt, double* s, double smax_d
 			if (!in_box(b, li, lj, lk)) continue;
 			const long c = bid(b, i, j, k), l = bid(b, li, lj, lk);
 			if (b->ctype[c] < 0 || b->ctype[l] < 0) continue;
+			or_fc++;
 			double L[8], R[8], bgr[3] = {0, 0, 0}, F[8], sp = 0;
 			const long cc[2] = {l, c};
 			double* U[2] = {L, R};
@@ -3205,6 +3219,7 @@ int or_bnd_step(const or_bgrid* g, int solver, double time_step_factor, double m
 	if (err) { if (bad_face) *bad_face = bad_cell; goto done; }
 
 	double total_dt = 0;
+	or_fc = 0;
 	for (int sub = 1; sub <= maxp; sub++) {
 		total_dt += sub_dt;
 		/* fluxes of the faces due at this substep between cells of type
@@ -3217,6 +3232,7 @@ int or_bnd_step(const or_bgrid* g, int solver, double time_step_factor, double m
 			if (o < 0 || ctype[o] < 0 || ctype[c] < 0) continue;
 			const int m = minsub[o];
 			if (sub % m != 0) continue;
+			or_fc++;
 			double L[8], R[8], bgr[3] = {0, 0, 0}, F[8], sp = 0;
 			{
 				const double ml[3] = {BS(1, o), BS(2, o), BS(3, o)}, Bl[3] = {BS(5, o), BS(6, o), BS(7, o)};

@@ -302,6 +302,9 @@ int or_bnd_step(const or_bgrid* g, int solver, double time_step_factor, double m
 
 /* OpenMP threads of the grid step (n > 0 sets it); returns the count in use */
 int or_set_threads(int n);
+/* faces solved by the last or_step_periodic, or_step_substep, or_bnd_step
+   or or_box_step: timestep()'s flux_calcs (source/mhd/solve.hpp:1918) */
+unsigned long long or_last_flux_calcs(void);
 
 /* ---- initial conditions used by the goldens and the benchmark configs ---- */
 

@@ -35,8 +35,8 @@ EXPORTS = (
     "pamhd_ensemble_check", "pamhd_ensemble_download", "pamhd_ensemble_cells",
     "pamhd_ensemble_set_tile", "pamhd_ensemble_autotune", "pamhd_ensemble_last_error",
     "pamhd_step_fixed", "pamhd_set_state_device", "pamhd_get_state_device",
-    "pamhd_adjoint_step", "pamhd_set_slabs", "pamhd_set_substepping", "pamhd_set_substepping_cells",
-    "pamhd_last_max_substep",
+    "pamhd_adjoint_step", "pamhd_set_slabs", "pamhd_set_fused_step", "pamhd_set_substepping", "pamhd_set_substepping_cells",
+    "pamhd_last_max_substep", "pamhd_last_flux_calcs",
 )
 
 
@@ -134,7 +134,9 @@ def load():
     L.pamhd_set_substepping.argtypes = [H, ctypes.c_int, ctypes.c_int]
     L.pamhd_set_substepping_cells.argtypes = [H, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.pamhd_last_max_substep.argtypes = [H, ctypes.POINTER(ctypes.c_int)]
+    L.pamhd_last_flux_calcs.argtypes = [H, ctypes.POINTER(ctypes.c_ulonglong)]
     L.pamhd_set_slabs.argtypes = [H, ctypes.c_int]
+    L.pamhd_set_fused_step.argtypes = [H, ctypes.c_int, ctypes.c_int]
     L.pamhd_enable_timing.argtypes = [H, ctypes.c_int]
     L.pamhd_kernel_time.argtypes = [H, ctypes.c_char_p, _dp, ctypes.POINTER(ctypes.c_long)]
     L.pamhd_last_error.argtypes = [H, ctypes.c_char_p, ctypes.c_size_t]

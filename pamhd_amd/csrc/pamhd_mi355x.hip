@@ -63,6 +63,7 @@ struct StepStatus {
 	unsigned int zero[3];
 	int err;
 	long long err_face;
+	unsigned long long flux_calcs;   /* faces solved (substepping path) */
 };
 
 constexpr int BX = 64, BY = 4;      /* flux pass and cell kernels */
@@ -213,17 +214,25 @@ __global__ __launch_bounds__(BX * BY, MINB) void flux_pass_kernel(GridDev g, Sta
 			b[1] = (T)bgd[(size_t)((D + 1) % 3) * g.n + cl];
 			b[2] = (T)bgd[(size_t)((D + 2) % 3) * g.n + cl];
 		}
-		T F[8], sT = 0;
-		const int e = riemann<SOLVER, T>(L, R, b, g.ph, F, sT);
 		/* dt * F rotated back to the global frame, in the flux precision
-		   (FP64: dt * F of the reference's flux slots) */
-		T Fg[8];
+		   (FP64: dt * F of the reference's flux slots; the mixed HLLD's
+		   FP64 fluxes are scaled in FP64) */
+		using FT = std::conditional_t<SOLVER == HLLD_ATHENA, double, T>;
+		FT F[8], Fg[8], sT = 0;
+		int e;
+		if constexpr (SOLVER == HLLD_ATHENA && std::is_same_v<T, float>) {
+			double sd = 0;
+			e = riemann_d<SOLVER, T>(L, R, b, g.ph, F, sd);
+			sT = sd;
+		} else {
+			e = riemann<SOLVER, T>(L, R, b, g.ph, F, sT);
+		}
 		if (e) {
 			record_error(st, e, (long long)D * g.n + c);
 			for (int q = 0; q < 8; q++) Fg[q] = 0;
 			sT = 0;
 		} else {
-			const T dt = (T)*dtp;
+			const FT dt = (FT)*dtp;
 			Fg[RHO] = dt * F[RHO];
 			Fg[NRJ] = dt * F[NRJ];
 			Fg[M0 + D] = dt * F[M0];
@@ -366,11 +375,7 @@ __global__ __launch_bounds__(BX * BY, MINB) void flux_hlld_shared_kernel(GridDev
 			Ls.pt64 = lpt[left];
 			Ls.rho_pos = true;
 			const T cfl = lcf[left];
-			if constexpr (std::is_same_v<T, float>) {
-				e = flux_hlld32_face(Ls, sd, cfl, cf, g.ph, F, sT);
-			} else {
-				e = flux_hlld_face<T>(Ls, sd, cfl, cf, Ld, U, g.ph, F, sT);
-			}
+			e = flux_hlld_face<T>(Ls, sd, cfl, cf, Ld, U, g.ph, F, sT);
 		}
 		T Fg[8];
 		if (e) {
@@ -457,8 +462,6 @@ __global__ __launch_bounds__(BX * BY, MINB) void flux_hlld_march_kernel(GridDev 
 			int e;
 			if (stage_l | stage_r) {
 				e = first_side_error(stage_l, code_l, stage_r, code_r);
-			} else if constexpr (std::is_same_v<T, float>) {
-				e = flux_hlld32_face(Ls, Rs, cfl, cfr, g.ph, F, sT);
 			} else {
 				e = flux_hlld_face<T>(Ls, Rs, cfl, cfr, Ls.u, Rs.u, g.ph, F, sT);
 			}
@@ -608,8 +611,6 @@ __global__ __launch_bounds__(BX * BY, MINB) void flux_hlld_xmarch_kernel(GridDev
 				int e;
 				if (L.stage | R.stage) {
 					e = first_side_error(L.stage, L.code, R.stage, R.code);
-				} else if constexpr (std::is_same_v<T, float>) {
-					e = flux_hlld32_face(L.s, R.s, L.cf, R.cf, g.ph, F, sT);
 				} else {
 					e = flux_hlld_face<T>(L.s, R.s, L.cf, R.cf, L.s.u, R.s.u, g.ph, F, sT);
 				}
@@ -1107,6 +1108,479 @@ __global__ __launch_bounds__(BX * BY) void update_cell_kernel(GridDev g,
 	out[4 * n + c] = En;
 }
 
+/*
+Fused step.  Pass 1 (fused_flux_kernel) computes every face flux of the
+step and finishes update_mhd_state's conserved update of each cell
+(source/mhd/solve.hpp:1503-1614) in one pass over the state, with no flux
+array: of each face only the two field components the constrained-transport
+edge terms use are written (6 doubles per cell, `emf`).  Pass 2
+(ct_refit_kernel) is the face-field update of assign_face_dBs_* and the
+energy refits of update_vol_B (solve.hpp:593-1415, :1622-1683), reading
+those 6 values per cell.  Both passes repeat the two-pass path's arithmetic
+in its order (update_column_kernel), so the results are identical.
+
+Pass 1 block: 64 lanes along x (lane l is cell i0 + l of the x tile) by NW
+waves along y (wave w is row j0 - 1 + w: waves 1 .. NW-2 are the tile's
+rows, waves 0 and NW-1 the rows below and above it), each thread marching
+along z over a chunk of KC planes.  Per plane:
+  * x-faces: the left side is the previous lane's per-cell part (moved one
+    lane up), lane 0's that of cell i0 - 1, computed for all planes of the
+    chunk before the march; the right neighbour's face flux comes one lane
+    down, lane 63's from the face of cell i0 + 64, also computed before the
+    march;
+  * y-faces: per-cell parts of all rows go through LDS; the face flux of
+    the row above comes back through LDS (two barriers per plane);
+  * z-faces: the per-cell part of plane k is carried in registers to the
+    face between k and k + 1.
+All indices wrap periodically, so every access is in bounds for any grid
+size; lanes and waves beyond the grid compute duplicates that are neither
+written nor counted.  The conserved update is accumulated in the order of
+the two-pass path: the x terms, then y, then z of each variable.
+*/
+constexpr int FLANES = 64;
+constexpr int FKC_MAX = 32;      /* planes per chunk, at most */
+
+template <int D> __device__ __forceinline__ void to_frame(const double Ug[8], double U[8])
+{
+	U[RHO] = Ug[RHO];
+	U[NRJ] = Ug[NRJ];
+	U[M0] = Ug[M0 + D];
+	U[M1] = Ug[M0 + (D + 1) % 3];
+	U[M2] = Ug[M0 + (D + 2) % 3];
+	U[B0] = Ug[B0 + D];
+	U[B1] = Ug[B0 + (D + 1) % 3];
+	U[B2] = Ug[B0 + (D + 2) % 3];
+}
+
+/* HLLD flux of the face between parts L and R, dt-scaled and rotated back
+   to the global frame (zero on error); returns the error, sets the speed.
+   FP64: flux_hlld_face; mixed: flux_hlld_mixed_face (FP64 results) */
+template <class T, int D> __device__ __forceinline__ int fused_face(const XPart<T>& L,
+	const XPart<T>& R, const Phys& ph, double dt, double Fg[8], double& speed)
+{
+	const int sl = L.sc >> 4, sr = R.sc >> 4;
+	int e;
+	double F[8];
+	T sT = 0;
+	if (sl | sr) {
+		e = first_side_error(sl, L.sc & 15, sr, R.sc & 15);
+	} else if constexpr (std::is_same_v<T, float>) {
+		e = flux_hlld_mixed_face(L, R, ph, F, sT);
+	} else {
+		Side<T> Ls, Rs;
+		unpack_side<T>(L, ph, Ls);
+		unpack_side<T>(R, ph, Rs);
+		e = flux_hlld_face<T>(Ls, Rs, L.cf, R.cf, L.u, R.u, ph, F, sT);
+	}
+	if (e) {
+#pragma unroll
+		for (int q = 0; q < 8; q++) Fg[q] = 0;
+		speed = 0;
+		return e;
+	}
+	Fg[RHO] = dt * F[RHO];
+	Fg[NRJ] = dt * F[NRJ];
+	Fg[M0 + D] = dt * F[M0];
+	Fg[M0 + (D + 1) % 3] = dt * F[M1];
+	Fg[M0 + (D + 2) % 3] = dt * F[M2];
+	Fg[B0 + D] = dt * F[B0];
+	Fg[B0 + (D + 1) % 3] = dt * F[B1];
+	Fg[B0 + (D + 2) % 3] = dt * F[B2];
+	speed = (double)sT;
+	return ERR_OK;
+}
+
+template <class T> __device__ __forceinline__ XPart<T> lane_up(const XPart<T>& r, const XPart<T>& b)
+{
+	return lane_up_words(r, b);
+}
+
+/* lane l receives lane l + 1's value, lane 63 receives b */
+__device__ __forceinline__ unsigned lane_down_u32(unsigned r, unsigned b)
+{
+	const unsigned dn = (unsigned)__shfl_down((int)r, 1u, 64);
+	return threadIdx.x == 63 ? b : dn;
+}
+
+template <class C> __device__ __forceinline__ C lane_down_words(const C& r, const C& b)
+{
+	static_assert(sizeof(C) % 4 == 0, "whole 32-bit words");
+	constexpr int W = sizeof(C) / 4;
+	unsigned rw[W], bw[W], ow[W];
+	__builtin_memcpy(rw, &r, sizeof(C));
+	__builtin_memcpy(bw, &b, sizeof(C));
+#pragma unroll
+	for (int q = 0; q < W; q++) ow[q] = lane_down_u32(rw[q], bw[q]);
+	C o;
+	__builtin_memcpy(&o, ow, sizeof(C));
+	return o;
+}
+
+template <class T> struct Flux8 { T f[8]; };
+
+/* block-wide max of s and count of s <= 0 over NWAVES waves, one atomic per block */
+template <int NWAVES> __device__ __forceinline__ void reduce_speed_blk(StepStatus* st, int d,
+	double s, unsigned zero, double* ms, unsigned* mz)
+{
+	for (int off = 32; off > 0; off >>= 1) {
+		const double o = __shfl_xor(s, off, 64);
+		s = smax(s, o);
+		zero += __shfl_xor(zero, off, 64);
+	}
+	const int tid = threadIdx.x + blockDim.x * threadIdx.y;
+	const int wave = tid / 64, lane = tid % 64;
+	if (lane == 0) {
+		ms[wave] = s;
+		mz[wave] = zero;
+	}
+	__syncthreads();
+	if (tid == 0) {
+		double m = ms[0];
+		unsigned z = mz[0];
+		for (int w = 1; w < NWAVES; w++) {
+			m = smax(m, ms[w]);
+			z += mz[w];
+		}
+		const unsigned long long bits = m > 0 ? (unsigned long long)__double_as_longlong(m) : 0ull;
+		atomicMax(&st->smax_bits[d][blockIdx.x % NSLOT], bits);
+		if (z) atomicAdd(&st->zero[d], z);
+	}
+	__syncthreads();
+}
+
+/* the state of cell c in the global frame: conserved variables and the
+   cell-centred field, the average of its two faces (update_vol_B) */
+__device__ __forceinline__ void load_cell(const StateArrays& S, int c, int cx, int cy, int cz,
+	double U[8])
+{
+	U[RHO] = S.q[0][c];
+	U[M0] = S.q[1][c];
+	U[M1] = S.q[2][c];
+	U[M2] = S.q[3][c];
+	U[NRJ] = S.q[4][c];
+	U[B0] = 0.5 * (S.q[5][c] + S.q[5][cx]);
+	U[B1] = 0.5 * (S.q[6][c] + S.q[6][cy]);
+	U[B2] = 0.5 * (S.q[7][c] + S.q[7][cz]);
+}
+
+/* conserved update by one direction's two faces, update_column_kernel's
+   order: (0 + lo - (0 + hi)) / len for rho and E, * (1 / len) for m and
+   the cell-centred field */
+template <class T> __device__ __forceinline__ void add_terms(double acc[8], const T lo[8],
+	const T hi[8], double len, double il)
+{
+	acc[RHO] += ((0.0 + (double)lo[RHO]) - (0.0 + (double)hi[RHO])) / len;
+#pragma unroll
+	for (int q = 0; q < 3; q++)
+		acc[M0 + q] = acc[M0 + q] + ((0.0 + (double)lo[M0 + q]) - (0.0 + (double)hi[M0 + q])) * il;
+	acc[NRJ] += ((0.0 + (double)lo[NRJ]) - (0.0 + (double)hi[NRJ])) / len;
+#pragma unroll
+	for (int q = 0; q < 3; q++)
+		acc[B0 + q] = acc[B0 + q] + ((0.0 + (double)lo[B0 + q]) - (0.0 + (double)hi[B0 + q])) * il;
+}
+
+template <class T, int NW> struct FusedShared {
+	XPart<T> yside[NW][FLANES];
+	Flux8<double> fy[NW][FLANES];
+	XPart<T> edge_l[NW][FKC_MAX];
+	Flux8<double> edge_r[NW][FKC_MAX];
+	double ms[NW];
+	unsigned mz[NW];
+};
+
+template <class T, int NW>
+__global__ __launch_bounds__(FLANES * NW) void fused_flux_kernel(GridDev g, StateArrays S,
+	double* __restrict__ out, double* __restrict__ emf, const double* __restrict__ dtp,
+	StepStatus* st, unsigned gx, unsigned gy, int kc, int refits, int remap)
+{
+	__shared__ FusedShared<T, NW> sh;
+	const unsigned lb = (remap & 1) ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+	const unsigned bx = lb % gx, by = (lb / gx) % gy, chunk = lb / (gx * gy);
+	const int lane = threadIdx.x, w = threadIdx.y;
+	const int nx = g.nx, ny = g.ny, nz = g.nz, n = (int)g.n;
+	const int i0 = (int)bx * FLANES, j0 = (int)by * (NW - 2);
+	const int i = (i0 + lane) % nx;
+	const int j = ((j0 - 1 + w) % ny + ny) % ny;
+	const bool own_x = i0 + lane < nx;
+	const bool inner = w >= 1 && w <= NW - 2 && j0 + w - 1 < ny;   /* a row of the tile */
+	const bool own = own_x && inner;
+	const int k0 = (int)chunk * kc, k1 = min(k0 + kc, nz);
+	const double len[3] = {g.dx, g.dy, g.dz};
+	const double il[3] = {1 / g.dx, 1 / g.dy, 1 / g.dz};
+	const double dt = *dtp;
+	const int nxy = nx * ny;
+	auto cell = [&](int ii, int jj, int kk) { return ii + nx * jj + nxy * kk; };
+	auto load = [&](int ii, int jj, int kk, double U[8]) {
+		load_cell(S, cell(ii, jj, kk), cell(wrap_inc(ii, nx), jj, kk), cell(ii, wrap_inc(jj, ny), kk),
+			cell(ii, jj, wrap_inc(kk, nz)), U);
+	};
+	double sp[3] = {0, 0, 0};
+	unsigned zero[3] = {0, 0, 0};
+
+	/* before the march: parts of the tile's left-edge cells and the face
+	   fluxes at its right edge, one plane per lane */
+	if (inner && lane < k1 - k0) {
+		const int kk = k0 + lane;
+		double Ug[8], U[8];
+		load(wrap_dec(i0 % nx, nx), j, kk, Ug);
+		to_frame<0>(Ug, U);
+		hlld_part<T>(U, g.ph, sh.edge_l[w][lane]);
+		if (i0 + FLANES <= nx) {
+			const int a = (i0 + FLANES - 1) % nx, b = (i0 + FLANES) % nx;
+			XPart<T> pa, pb;
+			load(a, j, kk, Ug);
+			to_frame<0>(Ug, U);
+			hlld_part<T>(U, g.ph, pa);
+			load(b, j, kk, Ug);
+			to_frame<0>(Ug, U);
+			hlld_part<T>(U, g.ph, pb);
+			double s_unused;
+			fused_face<T, 0>(pa, pb, g.ph, dt, sh.edge_r[w][lane].f, s_unused);
+		}
+	}
+	/* the z-face owned by plane k0 */
+	double Uc[8];               /* state of the current plane, global frame */
+	XPart<T> zc;                /* its z-frame part */
+	double fz_lo[8];            /* dt * F of its -z face */
+	{
+		double Ug[8], U[8];
+		XPart<T> zm;
+		load(i, j, wrap_dec(k0, nz), Ug);
+		to_frame<2>(Ug, U);
+		hlld_part<T>(U, g.ph, zm);
+		load(i, j, k0, Uc);
+		to_frame<2>(Uc, U);
+		hlld_part<T>(U, g.ph, zc);
+		if (inner) {
+			double s_face;
+			const int e = fused_face<T, 2>(zm, zc, g.ph, dt, fz_lo, s_face);
+			if (own) {
+				const int c = cell(i, j, k0);
+				if (e) record_error(st, e, 2ll * n + c);
+				sp[2] = smax(sp[2], s_face);
+				zero[2] += s_face <= 0 ? 1u : 0u;
+				emf[4 * n + c] = fz_lo[B1];
+				emf[5 * n + c] = fz_lo[B0];
+			}
+		}
+	}
+	__syncthreads();
+
+	for (int k = k0; k < k1; k++) {
+		const int kn = wrap_inc(k, nz);
+		double Un[8];
+		load(i, j, kn, Un);
+		/* y part of this plane, for the face of the row above */
+		{
+			double U[8];
+			to_frame<1>(Uc, U);
+			hlld_part<T>(U, g.ph, sh.yside[w][lane]);
+		}
+		double acc[8];
+		double fz_hi[8];
+		XPart<T> zn;
+		if (inner) {
+			/* x-face of this cell and of the next lane's */
+			XPart<T> px;
+			{
+				double U[8];
+				to_frame<0>(Uc, U);
+				hlld_part<T>(U, g.ph, px);
+			}
+			const XPart<T> pl = lane_up(px, sh.edge_l[w][k - k0]);
+			double fx[8];
+			double s_face;
+			const int e = fused_face<T, 0>(pl, px, g.ph, dt, fx, s_face);
+			Flux8<double> lo, hi;
+#pragma unroll
+			for (int q = 0; q < 8; q++) lo.f[q] = fx[q];
+			hi = lane_down_words(lo, sh.edge_r[w][k - k0]);
+			const int c = cell(i, j, k);
+			if (own) {
+				if (e) record_error(st, e, (long long)c);
+				sp[0] = smax(sp[0], s_face);
+				zero[0] += s_face <= 0 ? 1u : 0u;
+				emf[c] = fx[B1];
+				emf[n + c] = fx[B2];
+			}
+#pragma unroll
+			for (int q = 0; q < 8; q++) acc[q] = Uc[q];
+			add_terms<double>(acc, lo.f, hi.f, len[0], il[0]);
+			/* z-face between this plane and the next, owned by the next */
+			{
+				double U[8];
+				to_frame<2>(Un, U);
+				hlld_part<T>(U, g.ph, zn);
+			}
+			const int ez = fused_face<T, 2>(zc, zn, g.ph, dt, fz_hi, s_face);
+			if (own && k + 1 < k1) {
+				const int cn = cell(i, j, kn);
+				if (ez) record_error(st, ez, 2ll * n + cn);
+				sp[2] = smax(sp[2], s_face);
+				zero[2] += s_face <= 0 ? 1u : 0u;
+				emf[4 * n + cn] = fz_hi[B1];
+				emf[5 * n + cn] = fz_hi[B0];
+			}
+		}
+		__syncthreads();
+		if (w >= 1) {
+			double fyv[8];
+			double s_face;
+			const int e = fused_face<T, 1>(sh.yside[w - 1][lane], sh.yside[w][lane], g.ph, dt, fyv,
+				s_face);
+#pragma unroll
+			for (int q = 0; q < 8; q++) sh.fy[w][lane].f[q] = fyv[q];
+			if (own) {
+				const int c = cell(i, j, k);
+				if (e) record_error(st, e, (long long)n + c);
+				sp[1] = smax(sp[1], s_face);
+				zero[1] += s_face <= 0 ? 1u : 0u;
+				emf[2 * n + c] = fyv[B2];
+				emf[3 * n + c] = fyv[B0];
+			}
+		}
+		__syncthreads();
+		if (inner) {
+			add_terms<double>(acc, sh.fy[w][lane].f, sh.fy[w + 1][lane].f, len[1], il[1]);
+			add_terms<double>(acc, fz_lo, fz_hi, len[2], il[2]);
+			if (own) {
+				const int c = cell(i, j, k);
+				out[c] = acc[RHO];
+				out[n + c] = acc[M0];
+				out[2 * n + c] = acc[M1];
+				out[3 * n + c] = acc[M2];
+				/* the thermal pressure the first energy refit recovers
+				   (update_vol_B before the face update), or the energy
+				   itself when no refit will use it */
+				out[4 * n + c] = (refits > 0 && acc[RHO] > 0)
+					? pressure64(acc, g.ph.gm1, g.ph.half_inv_mu0) : acc[NRJ];
+			}
+#pragma unroll
+			for (int q = 0; q < 8; q++) fz_lo[q] = fz_hi[q];
+			zc = zn;
+		}
+#pragma unroll
+		for (int q = 0; q < 8; q++) Uc[q] = Un[q];
+	}
+	for (int d = 0; d < 3; d++) reduce_speed_blk<NW>(st, d, sp[d], zero[d], sh.ms, sh.mz);
+}
+
+/* constrained-transport edge terms from the 6 field components per cell of
+   the fused pass (same values as EdgeTerms of the flux array) */
+struct EdgeTerms6 {
+	const double* P;
+	long long n;
+	double dx, dy, dz;
+	__device__ __forceinline__ double ezx(long long c) const { return +(P[c] * dz / 4); }
+	__device__ __forceinline__ double eyx(long long c) const { return -(P[n + c] * dy / 4); }
+	__device__ __forceinline__ double exy(long long c) const { return +(P[2 * n + c] * dx / 4); }
+	__device__ __forceinline__ double ezy(long long c) const { return -(P[3 * n + c] * dz / 4); }
+	__device__ __forceinline__ double exz(long long c) const { return -(P[4 * n + c] * dx / 4); }
+	__device__ __forceinline__ double eyz(long long c) const { return +(P[5 * n + c] * dy / 4); }
+};
+
+/* total energy from the thermal pressure at the new cell-centred field,
+   the second half of refit_energy */
+__device__ __forceinline__ double energy_at_pressure(double rho, const double m[3], double p,
+	const double vb_new[3], const Phys& ph)
+{
+	const double inv = 1 / rho;
+	const double v[3] = {m[0] * inv, m[1] * inv, m[2] * inv};
+	const double
+		kinetic = 0.5 * rho * (v[0] * v[0] + v[1] * v[1] + v[2] * v[2]),
+		magnetic = ph.half_inv_mu0 * (vb_new[0] * vb_new[0] + vb_new[1] * vb_new[1] + vb_new[2] * vb_new[2]);
+	return p / ph.gm1 + kinetic + magnetic;
+}
+
+/*
+Pass 2 of the fused step: the face-field update of update_column_kernel
+(the same edge terms, sums and carries along z) from the fused pass's 6
+values per cell, then the energy refits: the first from the thermal
+pressure pass 1 stored in the energy slot, the others as refit_energy.
+Reads the old face field from s, rho, m and that pressure from out, and
+writes the new face field and energy into out.
+*/
+template <int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void ct_refit_kernel(GridDev g,
+	const double* __restrict__ s, double* __restrict__ out,
+	const double* __restrict__ P, int refits, int kz, unsigned gx, unsigned gy, int remap)
+{
+	const unsigned lb = remap ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+	const unsigned bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
+	const int i = bx * TX + threadIdx.x;
+	const int j = by * TY + threadIdx.y;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	const int im = wrap_dec(i, g.nx), ip = wrap_inc(i, g.nx);
+	const int jm = wrap_dec(j, g.ny), jp = wrap_inc(j, g.ny);
+	const Nb L{g.nx, g.ny};
+	const EdgeTerms6 E{P, n, g.dx, g.dy, g.dz};
+	const double axy = g.dx * g.dy, axz = g.dx * g.dz, ayz = g.dy * g.dz;
+	const int k0 = (int)bz * kz;
+	const int k1 = min(k0 + kz, g.nz);
+	if (k0 >= k1) return;
+
+	const int kb = wrap_dec(k0, g.nz);
+	double eyx_m = E.eyx(L(i, j, kb)), eyx_pm = E.eyx(L(ip, j, kb));
+	double exy_m = E.exy(L(i, j, kb)), exy_pm = E.exy(L(i, jp, kb));
+	double eyx_c = E.eyx(L(i, j, k0)), eyx_pc = E.eyx(L(ip, j, k0));
+	double exy_c = E.exy(L(i, j, k0)), exy_pc = E.exy(L(i, jp, k0));
+	double eyz_c = E.eyz(L(i, j, k0)), eyz_mc = E.eyz(L(im, j, k0)), eyz_pc = E.eyz(L(ip, j, k0));
+	double exz_c = E.exz(L(i, j, k0)), exz_mc = E.exz(L(i, jm, k0)), exz_pc = E.exz(L(i, jp, k0));
+	const long long cb = L(i, j, k0);
+	double s7 = s[7 * n + cb];
+	double bz_new = s7 + dBz_sum(eyz_pc, eyz_mc, exz_pc, exz_mc, eyx_pc, eyx_pm, eyx_c, eyx_m,
+		exy_pc, exy_pm, exy_c, exy_m) / axy;
+
+	for (int k = k0; k < k1; k++) {
+		const int kp = wrap_inc(k, g.nz);
+		const long long c = L(i, j, k), cxp = L(ip, j, k), cyp = L(i, jp, k), czp = L(i, j, kp);
+		const double
+			ezx_c = E.ezx(c), ezx_jm = E.ezx(L(i, jm, k)), ezx_jp = E.ezx(cyp),
+			ezx_p = E.ezx(cxp), ezx_pjm = E.ezx(L(ip, jm, k)), ezx_pjp = E.ezx(L(ip, jp, k)),
+			ezy_c = E.ezy(c), ezy_m = E.ezy(L(im, j, k)), ezy_p = E.ezy(cxp),
+			ezy_jp = E.ezy(cyp), ezy_mjp = E.ezy(L(im, jp, k)), ezy_pjp = E.ezy(L(ip, jp, k));
+		const double
+			eyx_n = E.eyx(czp), eyx_pn = E.eyx(L(ip, j, kp)),
+			exy_n = E.exy(czp), exy_pn = E.exy(L(i, jp, kp)),
+			eyz_n = E.eyz(czp), eyz_mn = E.eyz(L(im, j, kp)), eyz_pn = E.eyz(L(ip, j, kp)),
+			exz_n = E.exz(czp), exz_mn = E.exz(L(i, jm, kp)), exz_pn = E.exz(L(i, jp, kp));
+
+		const double s7n = s[7 * n + czp];
+		const double bx_new = s[5 * n + c] + dBx_sum(ezx_jp, ezx_jm, eyx_n, eyx_m, ezy_jp, ezy_mjp,
+			ezy_c, ezy_m, eyz_n, eyz_mn, eyz_c, eyz_mc) / ayz;
+		const double by_new = s[6 * n + c] + dBy_sum(exy_n, exy_m, ezy_p, ezy_m, exz_n, exz_mn,
+			exz_c, exz_mc, ezx_p, ezx_c, ezx_pjm, ezx_jm) / axz;
+		const double bxp_new = s[5 * n + cxp] + dBx_sum(ezx_pjp, ezx_pjm, eyx_pn, eyx_pm, ezy_pjp,
+			ezy_jp, ezy_p, ezy_c, eyz_pn, eyz_n, eyz_pc, eyz_c) / ayz;
+		const double byp_new = s[6 * n + cyp] + dBy_sum(exy_pn, exy_pm, ezy_pjp, ezy_mjp, exz_pn,
+			exz_n, exz_pc, exz_c, ezx_pjp, ezx_jp, ezx_p, ezx_c) / axz;
+		const double bzp_new = s7n + dBz_sum(eyz_pn, eyz_mn, exz_pn, exz_mn, eyx_pn, eyx_pc, eyx_n,
+			eyx_c, exy_pn, exy_pc, exy_n, exy_c) / axy;
+		out[5 * n + c] = bx_new;
+		out[6 * n + c] = by_new;
+		out[7 * n + c] = bz_new;
+
+		const double rho = out[c];
+		const double m[3] = {out[n + c], out[2 * n + c], out[3 * n + c]};
+		double En = out[4 * n + c];
+		if (refits > 0 && rho > 0) {
+			const double vb_new[3] = {0.5 * (bx_new + bxp_new), 0.5 * (by_new + byp_new),
+				0.5 * (bz_new + bzp_new)};
+			En = energy_at_pressure(rho, m, En, vb_new, g.ph);
+			for (int r = 1; r < refits; r++) refit_energy(rho, m, En, vb_new, vb_new, g.ph);
+		}
+		out[4 * n + c] = En;
+
+		eyx_m = eyx_c; eyx_pm = eyx_pc; exy_m = exy_c; exy_pm = exy_pc;
+		eyx_c = eyx_n; eyx_pc = eyx_pn; exy_c = exy_n; exy_pc = exy_pn;
+		eyz_c = eyz_n; eyz_mc = eyz_mn; eyz_pc = eyz_pn;
+		exz_c = exz_n; exz_mc = exz_mn; exz_pc = exz_pn;
+		bz_new = bzp_new;
+		s7 = s7n;
+	}
+}
+
 /* lagged CFL step of minimize_timestep + set_minmax_substepping_period with a
    single substep (source/mhd/solve.hpp:1961-2006, substepping.hpp:303-338) */
 /* one wave: lane s reduces slot s of each dimension (max of the ordered bit
@@ -1278,10 +1752,10 @@ __global__ __launch_bounds__(256) void face_batch_kernel(size_t n, const double*
 	}
 	T b[3] = {0, 0, 0};
 	if (bg) for (int q = 0; q < 3; q++) b[q] = (T)bg[q * n + f];
-	T F[8], s = 0;
-	const int e = riemann<SOLVER, T>(L, R, b, ph, F, s);
-	for (int q = 0; q < 8; q++) Fp[q * n + f] = e ? 0.0 : (double)F[q];
-	sp[f] = e ? 0.0 : (double)s;
+	double F[8], s = 0;
+	const int e = riemann_d<SOLVER, T>(L, R, b, ph, F, s);
+	for (int q = 0; q < 8; q++) Fp[q * n + f] = e ? 0.0 : F[q];
+	sp[f] = e ? 0.0 : s;
 	if (status) status[f] = e;
 }
 
@@ -1426,6 +1900,7 @@ __global__ void ss_clear_kernel(SubstepDev d)
 	if (threadIdx.x == 0) {
 		d.status->err = 0;
 		d.status->err_face = -1;
+		d.status->flux_calcs = 0;
 	}
 }
 
@@ -1595,6 +2070,14 @@ __global__ __launch_bounds__(BX * BY) void ss_flux_kernel(GridDev g, SubstepDev 
 	/* a face between two cells of type >= 0, solved by its negative-side
 	   owner at the pace of the owner's smallest neighbourhood period */
 	const int o = (oi < 0 || oj < 0 || ok < 0) ? -1 : cidx(g, oi, oj, ok);
+	{
+		/* faces solved, counted per wave (timestep()'s flux_calcs) */
+		const bool solved = o >= 0 && ss_active(d, o) && ss_active(d, c) && sub % d.minsub[o] == 0;
+		const unsigned long long b = __ballot(solved);
+		const unsigned long long act = __ballot(1);
+		if ((int)(threadIdx.x + BX * threadIdx.y) % 64 == __ffsll((long long)act) - 1 && b)
+			atomicAdd(&d.status->flux_calcs, (unsigned long long)__popcll(b));
+	}
 	if (o < 0 || !ss_active(d, o) || !ss_active(d, c)) {
 		for (int q = 0; q < 8; q++) Pd[q * n + c] = 0;
 		return;
@@ -1839,7 +2322,13 @@ struct pamhd_handle {
 		int flux_shared = 3;
 		int flux_km = 16;
 		int update_tile = 1;   /* column update tile: 64x4, 128x2, 32x8, 128x4, 64x8 */
+		/* HLLD: 1 = the fused step (every face flux and the conserved
+		   update in one pass, then the face-field update and refits),
+		   0 = flux passes and update */
+		int fused = 1;
+		int fused_kc = 32;     /* planes per chunk of the fused pass (<= FKC_MAX) */
 	} tune;
+	double* emf = nullptr;     /* 6 x n field components of the fused step, allocated on first use */
 	/* second stream and events of the slab pipeline, created on first use */
 	hipStream_t aux = nullptr;
 	std::vector<hipEvent_t> ev;
@@ -1857,6 +2346,7 @@ struct pamhd_handle {
 	int adj_reverse = 1;   /* flux Jacobian of the adjoint: 1 reverse accumulation, 0 dual numbers */
 	int cur = 0;    /* state buffer holding the current state */
 	int last = 0;   /* status entry of the last flux pass */
+	unsigned long long flux_calcs = 0;   /* faces solved by the last step */
 	std::string err;
 };
 
@@ -1932,7 +2422,8 @@ void launch_flux_dirs(pamhd_handle* h, hipStream_t st, StepStatus* stat, int k0,
 	const unsigned gx = (h->g.nx + BX - 1) / BX, gy = (h->g.ny + BY - 1) / BY;
 	const dim3 grid(gx * gy * (unsigned)nk), block(BX, BY, 1);
 	const int r = h->tune.flux_remap;
-	if (SOLVER == HLLD_ATHENA && h->tune.flux_shared >= 2) {
+	constexpr bool hlld64 = SOLVER == HLLD_ATHENA && std::is_same_v<T, double>;
+	if constexpr (hlld64) if (h->tune.flux_shared >= 2) {
 		const int km = h->tune.flux_km;
 		const unsigned g1 = gx * (unsigned)((nk + BY - 1) / BY) * (unsigned)((h->g.ny + km - 1) / km);
 		const unsigned g2 = gx * (unsigned)((h->g.ny + BY - 1) / BY) * (unsigned)((nk + km - 1) / km);
@@ -1967,7 +2458,7 @@ void launch_flux_dirs(pamhd_handle* h, hipStream_t st, StepStatus* stat, int k0,
 		flux_cell_march_kernel<SV, T, PT, 2, 1><<<gz, block, 0, st>>>(h->g, S, P, h->dt, stat, gx, km, r, k0, ke);
 		return;
 	}
-	if (SOLVER == HLLD_ATHENA && h->tune.flux_shared == 1) {
+	if constexpr (hlld64) if (h->tune.flux_shared == 1) {
 		const dim3 grid2(gx * (unsigned)h->g.ny * (unsigned)((nk + BY - 1) / BY));
 		const int ke = k0 + nk;
 		if (h->tune.flux_occ) {
@@ -2062,6 +2553,57 @@ int report_status(pamhd_handle* h, const StepStatus& st)
 	return fail(h, st.err, buf);
 }
 
+/* the fused step applies to HLLD (which ignores the background field) */
+bool fused_step(const pamhd_handle* h, int solver)
+{
+	return solver == HLLD_ATHENA && h->tune.fused;
+}
+
+constexpr int FNW = 8;   /* waves per block of the fused pass: 6 rows and their two neighbours */
+
+/* the two passes of the fused step: pass 1 reads state[cur] and writes rho,
+   m and the recovered pressure into state[1 - cur] and the field components
+   into emf; pass 2 completes state[1 - cur] */
+template <class T>
+int launch_fused(pamhd_handle* h, hipStream_t st, StepStatus* next, int refits)
+{
+	const long long n = h->g.n;
+	if (!h->emf) HIP_TRY(h, hipMalloc(&h->emf, sizeof(double) * 6 * n));
+	const double* s = h->state[h->cur];
+	double* out = h->state[1 - h->cur];
+	StateArrays S;
+	for (int q = 0; q < 8; q++) S.q[q] = s + (size_t)q * n;
+	int kc = h->tune.fused_kc;
+	if (kc < 1) kc = 1;
+	if (kc > FKC_MAX) kc = FKC_MAX;
+	const unsigned gx = (h->g.nx + FLANES - 1) / FLANES, gy = (h->g.ny + FNW - 3) / (FNW - 2);
+	const unsigned gz = (h->g.nz + kc - 1) / kc;
+	{
+		TimedLaunch t(h, st, K_FLUX);
+		fused_flux_kernel<T, FNW><<<gx * gy * gz, dim3(FLANES, FNW, 1), 0, st>>>(h->g, S, out, h->emf,
+			h->dt, next, gx, gy, kc, refits, h->tune.flux_remap);
+	}
+	HIP_TRY(h, hipGetLastError());
+	TimedLaunch t(h, st, K_UPDATE);
+	const int kz = h->tune.update_kz > 0 ? min(h->tune.update_kz, h->g.nz) : min(8, h->g.nz);
+	const unsigned gzc = (h->g.nz + kz - 1) / kz;
+	auto column = [&](auto tx, auto ty) {
+		constexpr int TX = decltype(tx)::value, TY = decltype(ty)::value;
+		const unsigned cx = (h->g.nx + TX - 1) / TX, cy = (h->g.ny + TY - 1) / TY;
+		ct_refit_kernel<TX, TY><<<dim3(cx * cy * gzc), dim3(TX, TY, 1), 0, st>>>(
+			h->g, s, out, h->emf, refits, kz, cx, cy, h->tune.update_remap & 1);
+	};
+	switch (h->tune.update_tile) {
+	case 1: column(std::integral_constant<int, 128>(), std::integral_constant<int, 2>()); break;
+	case 2: column(std::integral_constant<int, 32>(), std::integral_constant<int, 8>()); break;
+	case 3: column(std::integral_constant<int, 128>(), std::integral_constant<int, 4>()); break;
+	case 4: column(std::integral_constant<int, 64>(), std::integral_constant<int, 8>()); break;
+	default: column(std::integral_constant<int, 64>(), std::integral_constant<int, 4>()); break;
+	}
+	HIP_TRY(h, hipGetLastError());
+	return PAMHD_OK;
+}
+
 /*
 Flux pass and update of one step.  With tune.slabs = S > 1 the grid is cut
 into S z-slabs: the flux pass of slab s runs on the caller's stream, and the
@@ -2073,6 +2615,7 @@ Per-cell arithmetic is unchanged, so results are identical for every S.
 template <class T, class PT>
 int launch_step_body(pamhd_handle* h, int solver, hipStream_t st, StepStatus* next, int refits)
 {
+	if (fused_step(h, solver)) return launch_fused<T>(h, st, next, refits);
 	int S = h->tune.slabs;
 	if (S > h->g.nz / 2) S = h->g.nz / 2;
 	if (h->timing || h->tune.update_kz > 0) S = 1;
@@ -2473,6 +3016,7 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 	HIP_TRY(h, hipMemcpyAsync(d.host_status, d.status, sizeof(StepStatus), hipMemcpyDeviceToHost, st));
 	HIP_TRY(h, hipStreamSynchronize(st));
 	d.last_max = maxp;
+	h->flux_calcs = d.host_status->flux_calcs;
 	if (dt_taken) *dt_taken = total;
 	return report_status(h, *d.host_status);
 }
@@ -2493,6 +3037,7 @@ int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
 		std::string err;
 		int e = boxset_launch_step(h->box, solver, max_dt, cfl, refits == 3, st, err);
 		if (e) return fail(h, e, err);
+		h->flux_calcs = (unsigned long long)boxset_faces(h->box, 0);
 		if (dt_taken) {
 			e = boxset_readback(h->box, st, dt_taken, err);
 			if (e) return fail(h, e, err);
@@ -2516,12 +3061,13 @@ int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
 	}
 	HIP_TRY(h, hipGetLastError());
 	const int e = h->precision == PAMHD_MIXED
-		? launch_step_body<float, float>(h, solver, st, next, refits)
+		? launch_step_body<float, double>(h, solver, st, next, refits)
 		: launch_step_body<double, double>(h, solver, st, next, refits);
 	if (e) return e;
 	HIP_TRY(h, hipGetLastError());
 	h->cur = 1 - h->cur;
 	h->last = 1 - h->last;
+	h->flux_calcs = 3ull * (unsigned long long)h->g.n;
 	if (dt_taken) {
 		HIP_TRY(h, hipMemcpyAsync(h->host_dt, h->dt, sizeof(double), hipMemcpyDeviceToHost, st));
 		HIP_TRY(h, hipMemcpyAsync(h->host_status, next, sizeof(StepStatus), hipMemcpyDeviceToHost, st));
@@ -2706,6 +3252,7 @@ int pamhd_destroy(pamhd_handle* h)
 	(void)hipFree(h->state[0]);
 	(void)hipFree(h->state[1]);
 	(void)hipFree(h->flux);
+	(void)hipFree(h->emf);
 	(void)hipFree(h->bg);
 	(void)hipFree(h->dt);
 	(void)hipFree(h->status);
@@ -3063,6 +3610,13 @@ int pamhd_set_substepping_cells(pamhd_handle* h, const int* substep_min, const i
 	return PAMHD_OK;
 }
 
+int pamhd_last_flux_calcs(pamhd_handle* h, unsigned long long* flux_calcs)
+{
+	if (!h || !flux_calcs) return fail(h, PAMHD_E_ARG, "null argument");
+	*flux_calcs = h->flux_calcs;
+	return PAMHD_OK;
+}
+
 int pamhd_last_max_substep(pamhd_handle* h, int* max_substep)
 {
 	if (!h || !max_substep) return fail(h, PAMHD_E_ARG, "null argument");
@@ -3154,7 +3708,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 	};
 	auto flux = [&]() {
 		dt_kernel<<<1, NSLOT, 0, st>>>(prev, next, h->dt, 0.0, 0.0, h->g.dx, h->g.dy, h->g.dz, 0);
-		if (h->precision == PAMHD_MIXED) launch_flux_pass<float, float>(h, solver, st, next);
+		if (h->precision == PAMHD_MIXED) launch_flux_pass<float, double>(h, solver, st, next);
 		else launch_flux_pass<double, double>(h, solver, st, next);
 	};
 	std::string report = "flux_pass:";
@@ -3203,8 +3757,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 			h->tune.update_remap = remap;
 			h->tune.update_tile = tile;
 			const float ms = time_it([&]() {
-				if (h->precision == PAMHD_MIXED) launch_update<float>(h, st, 3);
-				else launch_update<double>(h, st, 3);
+				launch_update<double>(h, st, 3);
 			});
 			static const char* tiles[] = {"64x4", "128x2", "32x8", "128x4", "64x8"};
 			report += " " + std::string(kz ? "column" + std::to_string(kz) + "/" + tiles[tile] : "cell") +
@@ -3227,7 +3780,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 			h->tune.slabs = S;
 			const float ms = time_it([&]() {
 				dt_kernel<<<1, NSLOT, 0, st>>>(prev, next, h->dt, 0.0, 0.0, h->g.dx, h->g.dy, h->g.dz, 0);
-				if (h->precision == PAMHD_MIXED) launch_step_body<float, float>(h, solver, st, next, 3);
+				if (h->precision == PAMHD_MIXED) launch_step_body<float, double>(h, solver, st, next, 3);
 				else launch_step_body<double, double>(h, solver, st, next, 3);
 			});
 			report += " slabs" + std::to_string(S) + "=" + std::to_string(ms) + "ms";
@@ -3266,6 +3819,17 @@ int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_
 	h->tune.update_kz = update_kz;
 	h->tune.update_remap = update_kz ? (update_remap & 1) : (update_remap & 3);
 	h->tune.update_tile = update_kz ? (update_remap >> 2) % 5 : 0;
+	return PAMHD_OK;
+}
+
+int pamhd_set_fused_step(pamhd_handle* h, int fused, int planes_per_chunk)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (fused != 0 && fused != 1) return fail(h, PAMHD_E_ARG, "fused must be 0 or 1");
+	if (planes_per_chunk < 0 || planes_per_chunk > FKC_MAX)
+		return fail(h, PAMHD_E_ARG, "planes per chunk must be in 0.." + std::to_string(FKC_MAX));
+	h->tune.fused = fused;
+	if (planes_per_chunk > 0) h->tune.fused_kc = planes_per_chunk;
 	return PAMHD_OK;
 }
 

@@ -567,46 +567,167 @@ __device__ __forceinline__ int first_side_error(int stage_l, int code_l, int sta
 }
 
 /*
-HLLD of source/mhd/hlld_athena.hpp:54-559 for the mixed path: the same
-states, branches and degenerate-case tests in FP32, arranged so that each
-denominator is inverted once (1/rho, 1/(S_M* - S_M), 1/sqrt(rho*), the star
-states' shared denominator, 1/rho*) and the fast speeds come from
-fast_speed32: 12 reciprocals and 6 square roots per face instead of the
-FP64 order's 24 divisions and 14 square roots.  Pressures and the
-positivity checks are FP64 (make_side).
+Per-cell part of HLLD in one face direction, as a record passed between the
+threads of a kernel: the FP64 state in the face frame, the FP64 thermal
+pressure, the fast speed and hlld_side's stage and error code; for the
+mixed path also 1 / rho in FP64.  unpack_side rebuilds the Side exactly as
+make_side built it.
 */
-__device__ __forceinline__ int flux_hlld32_face(const Side<float>& L, const Side<float>& R,
-	float cfL, float cfR, const Phys& ph, float F[8], float& s_out)
+template <class T> struct XPart {
+	double u[8];
+	double p64;
+	double ir64;   /* 1 / rho in FP64 (mixed path) */
+	T cf;
+	int sc;        /* stage * 16 + code */
+};
+
+template <class T> __device__ __forceinline__ void hlld_part(const double U[8], const Phys& ph,
+	XPart<T>& x)
+{
+	Side<T> s;
+	T cf = 0;
+	int code;
+	const int stage = hlld_side<T>(U, ph, s, cf, code);
+#pragma unroll
+	for (int q = 0; q < 8; q++) x.u[q] = U[q];
+	x.p64 = stage == 0 ? s.p64 : 0.0;
+	x.ir64 = (std::is_same_v<T, float> && stage == 0) ? 1 / U[RHO] : 0.0;
+	x.cf = cf;
+	x.sc = stage * 16 + code;
+}
+
+template <class T> __device__ __forceinline__ void unpack_side(const XPart<T>& x, const Phys& ph,
+	Side<T>& s)
+{
+#pragma unroll
+	for (int q = 0; q < 8; q++) s.u[q] = (T)x.u[q];
+	s.rho_pos = x.u[RHO] > 0;
+	s.p64 = x.p64;
+	s.p = (T)x.p64;
+	if constexpr (std::is_same_v<T, float>) {
+		s.pt64 = s.p64 + ph.half_inv_mu0 * (x.u[B0] * x.u[B0] + x.u[B1] * x.u[B1] + x.u[B2] * x.u[B2]);
+	} else {
+		s.pt64 = 0;
+	}
+	const T inv = Arith<T>::rcp(s.u[RHO]);
+	s.ir = inv;
+	s.v[0] = s.u[M0] * inv;
+	s.v[1] = s.u[M1] * inv;
+	s.v[2] = s.u[M2] * inv;
+}
+
+/* FP64 ideal-MHD x-flux without background field (get_flux,
+   source/mhd/common.hpp:138-244 with bg = 0) from the state, 1 / rho and
+   the total pressure */
+__device__ __forceinline__ void physical_flux64(const double U[8], const double v[3], double pt,
+	double inv_mu0, double F[8])
+{
+	const double* B = U + B0;
+	F[RHO] = U[M0];
+#pragma unroll
+	for (int i = 0; i < 3; i++) F[M0 + i] = U[M0 + i] * v[0] - B[i] * B[0] * inv_mu0;
+	F[M0] += pt;
+	F[NRJ] = v[0] * (U[NRJ] + pt) - B[0] * (v[0] * B[0] + v[1] * B[1] + v[2] * B[2]) * inv_mu0;
+	F[B0] = 0;
+#pragma unroll
+	for (int i = 1; i < 3; i++) F[B0 + i] = B[i] * v[0] - v[i] * B[0];
+}
+
+/* one side's star state of the mixed HLLD as differences from the side's
+   own state (eqns 43-48 of the HLLD method restated): with d = S - u,
+   e = S - S_M and delta = S_M - u,
+     rho* - rho = rho delta / e,  m_n* - m_n = (rho* - rho) S_M + rho delta,
+     v_t* - v_t = -B_t tmp1,  B_t* - B_t = B_t rho d delta / tmp,
+     E* - E = (delta (E + p_T + rho d S_M) + B_n (v.B - v*.B*) / mu0) / e,
+   tmp = rho d e - B_n^2 / mu0, tmp1 = B_n delta / (mu0 tmp); every term is
+   a product with delta or with a difference of the two sides, so FP32
+   rounding errors scale with the jump, not with the state */
+struct MixedStar {
+	float dU[8];     /* U* - U (dU[B0] unused) */
+	float vs[3];     /* v* - v of the transverse components (vs[0] unused) */
+	float Bs[3];     /* B* of the transverse components */
+	float dB[3];     /* B* - B of the transverse components */
+	float rho_s;     /* rho* */
+};
+
+__device__ __forceinline__ void mixed_star(const XPart<float>& P, const double v64[3], double pt64,
+	float d, float e, float delta, float sm, float Bx, float dbx, float inv, float ptst, MixedStar& S)
 {
 	using A = Arith<float>;
-	const float inv = (float)ph.inv_mu0, inv_sqrt = (float)ph.inv_sqrt_mu0;
-	const float ms = smax(cfL, cfR);
-	const float *vL = L.v, *vR = R.v;
-	const float
-		msn = (vL[0] <= vR[0]) ? vL[0] - ms : vR[0] - ms,
-		msp = (vL[0] <= vR[0]) ? vR[0] + ms : vL[0] + ms;
-	/* physical fluxes without background field, total pressure p + B^2/2mu0 */
-	const float nmn = dot3(L.u + B0, L.u + B0) * (0.5f * inv), nmp = dot3(R.u + B0, R.u + B0) * (0.5f * inv);
-	const float ptL = L.p + nmn, ptR = R.p + nmp;
-	float FL[8], FR[8];
-	{
-		const Side<float>* sd[2] = {&L, &R};
-		const float pt[2] = {ptL, ptR};
-		float* FF[2] = {FL, FR};
-		for (int q = 0; q < 2; q++) {
-			const Side<float>& S = *sd[q];
-			const float* B = S.u + B0;
-			float* f = FF[q];
-			const float bin = B[0] * inv;
-			f[RHO] = S.u[M0];
-			for (int i = 0; i < 3; i++) f[M0 + i] = S.u[M0 + i] * S.v[0] - B[i] * bin;
-			f[M0] += pt[q];
-			f[NRJ] = S.v[0] * (S.u[NRJ] + pt[q]) - bin * dot3(S.v, B);
-			f[B0] = 0;
-			for (int i = 1; i < 3; i++) f[B0 + i] = B[i] * S.v[0] - S.v[i] * B[0];
+	const float rho = (float)P.u[RHO];
+	const float ie = A::rcp(e);
+	const float dr = rho * delta * ie;
+	S.rho_s = rho + dr;
+	S.dU[RHO] = dr;
+	S.dU[M0] = dr * sm + rho * delta;
+	const float v[3] = {(float)v64[0], (float)v64[1], (float)v64[2]};
+	const float B[3] = {(float)P.u[B0], (float)P.u[B1], (float)P.u[B2]};
+	const float tmp = rho * d * e - Bx * Bx * inv;
+	float tmp1 = 0;
+	if (fabsf(tmp) < A::eps * ptst) {
+		for (int i = 1; i < 3; i++) {
+			S.dU[M0 + i] = v[i] * dr;
+			S.dB[i] = 0;
+			S.vs[i] = 0;
+		}
+	} else {
+		const float it = A::rcp(tmp);
+		tmp1 = Bx * delta * inv * it;
+		const float g = rho * d * delta * it;
+		for (int i = 1; i < 3; i++) {
+			S.vs[i] = -B[i] * tmp1;
+			S.dU[M0 + i] = v[i] * dr + S.vs[i] * S.rho_s;
+			S.dB[i] = B[i] * g;
 		}
 	}
-	if (!A::isfinite(FL[RHO]) || !A::isfinite(FR[RHO])) return ERR_FLUX;
+	/* v.B - v*.B*, with dbx = B_n(face) - B_n(cell) */
+	float vb = -delta * Bx - v[0] * dbx;
+	for (int i = 1; i < 3; i++) {
+		S.Bs[i] = B[i] + S.dB[i];
+		S.dU[B0 + i] = S.dB[i];
+		vb += -v[i] * S.dB[i] + tmp1 * B[i] * S.Bs[i];
+	}
+	S.dU[B0] = 0;
+	S.dU[NRJ] = (delta * ((float)P.u[NRJ] + (float)pt64 + rho * d * sm) + Bx * inv * vb) * ie;
+}
+
+/*
+HLLD of source/mhd/hlld_athena.hpp:54-559 for the mixed path, written as
+F = F_L + S_L (U*_L - U_L) [+ S*_L (U**_L - U*_L)] (and the mirror forms
+from F_R) with the physical fluxes F_L, F_R, the velocities, the jumps of
+the normal velocity, total pressure and transverse field and velocity in
+FP64, and the wave speeds and the star and double-star differences in
+FP32 in the delta form of mixed_star: the FP32 rounding then scales with
+the jumps across the face, not with |F|.  The same branches and degenerate
+cases as the FP64 solver; returns FP64 fluxes.  Both parts must have
+passed hlld_side's checks.
+*/
+__device__ __forceinline__ int flux_hlld_mixed_face(const XPart<float>& L, const XPart<float>& R,
+	const Phys& ph, double F[8], float& s_out)
+{
+	using A = Arith<float>;
+	if (!__builtin_isnormal(L.u[RHO]) || L.u[RHO] <= 0) return ERR_DENSITY;
+	if (!__builtin_isfinite(L.u[M0])) return ERR_FLUX;
+	if (!__builtin_isnormal(R.u[RHO]) || R.u[RHO] <= 0) return ERR_DENSITY;
+	if (!__builtin_isfinite(R.u[M0])) return ERR_FLUX;
+	const double inv64 = ph.inv_mu0;
+	const double vL64[3] = {L.u[M0] * L.ir64, L.u[M1] * L.ir64, L.u[M2] * L.ir64};
+	const double vR64[3] = {R.u[M0] * R.ir64, R.u[M1] * R.ir64, R.u[M2] * R.ir64};
+	const double ptL64 = L.p64 + ph.half_inv_mu0 * (L.u[B0] * L.u[B0] + L.u[B1] * L.u[B1] + L.u[B2] * L.u[B2]);
+	const double ptR64 = R.p64 + ph.half_inv_mu0 * (R.u[B0] * R.u[B0] + R.u[B1] * R.u[B1] + R.u[B2] * R.u[B2]);
+	double FL[8], FR[8];
+	physical_flux64(L.u, vL64, ptL64, inv64, FL);
+	physical_flux64(R.u, vR64, ptR64, inv64, FR);
+	if (!__builtin_isfinite(FL[RHO]) || !__builtin_isfinite(FR[RHO])) return ERR_FLUX;
+
+	const float inv = (float)ph.inv_mu0, inv_sqrt = (float)ph.inv_sqrt_mu0;
+	const float ms = smax(L.cf, R.cf);
+	const float vL = (float)vL64[0], vR = (float)vR64[0];
+	const float dv = (float)(vR64[0] - vL64[0]);
+	const bool lo = vL64[0] <= vR64[0];
+	const float
+		msn = lo ? vL - ms : vR - ms,
+		msp = lo ? vR + ms : vL + ms;
 	if (msn >= 0.0f) {
 		for (int q = 0; q < 8; q++) F[q] = FL[q];
 		s_out = fabsf(msp);
@@ -616,118 +737,77 @@ __device__ __forceinline__ int flux_hlld32_face(const Side<float>& L, const Side
 		s_out = fabsf(msn);
 		return ERR_OK;
 	}
-	const float Bx = (L.u[B0] + R.u[B0]) * 0.5f, Bx2 = Bx * Bx, Bx2i = Bx2 * inv;
-	const float sfdn = msn - vL[0], sfdp = msp - vR[0];
-	const float rsn = L.u[RHO] * sfdn, rsp = R.u[RHO] * sfdp;
-	/* the total-pressure difference in FP64: where the normal momenta vanish
-	   it alone sets the sign of S_M, and HLLD's flux jumps across S_M = 0
-	   when the two sides' normal fields differ (cell-centred B) */
-	const float dpt = (float)(L.pt64 - R.pt64);
-	const float sm = (sfdp * R.u[M0] - sfdn * L.u[M0] + dpt) * A::rcp(rsp - rsn);
-	const float smdn = msn - sm, smdp = msp - sm;
-	const float ismdn = A::rcp(smdn), ismdp = A::rcp(smdp);
-	const float dsn = rsn * ismdn, dsp = rsp * ismdp;
-	const float dssn = A::sqrt(dsn), dssp = A::sqrt(dsp);
-	const float idssn = A::rcp(dssn), idssp = A::rcp(dssp);
-	const float ssn = sm - fabsf(Bx) * inv_sqrt * idssn, ssp = sm + fabsf(Bx) * inv_sqrt * idssp;
-	const float ptst = ptL + rsn * (sm - vL[0]);
+	/* S_L - u_L and S_R - u_R from the FP64 velocity jump */
+	const float sfdn = lo ? -ms : dv - ms, sfdp = lo ? ms : ms - dv;
+	const float rhoL = (float)L.u[RHO], rhoR = (float)R.u[RHO];
+	const float dpt = (float)(ptR64 - ptL64);
+	/* S_M - u_L and S_M - u_R (eqn 38 of the method, differenced) */
+	const float iden = A::rcp(sfdp * rhoR - sfdn * rhoL);
+	const float delL = (sfdp * rhoR * dv - dpt) * iden, delR = (sfdn * rhoL * dv - dpt) * iden;
+	const float sm = vL + delL;
+	const float smdn = sfdn - delL, smdp = sfdp - delR;
+	const double Bx64 = (L.u[B0] + R.u[B0]) * 0.5;
+	const float Bx = (float)Bx64, Bx2 = Bx * Bx;
+	const float ptst = (float)ptL64 + rhoL * sfdn * delL;
+	MixedStar Sn, Sp;
+	mixed_star(L, vL64, ptL64, sfdn, smdn, delL, sm, Bx, (float)(Bx64 - L.u[B0]), inv, ptst, Sn);
+	mixed_star(R, vR64, ptR64, sfdp, smdp, delR, sm, Bx, (float)(Bx64 - R.u[B0]), inv, ptst, Sp);
+	const float dssn = A::sqrt(Sn.rho_s), dssp = A::sqrt(Sp.rho_s);
+	const float ssn = sm - fabsf(Bx) * inv_sqrt * A::rcp(dssn), ssp = sm + fabsf(Bx) * inv_sqrt * A::rcp(dssp);
 
-	/* single-star states, eqns 43-48 of the HLLD method */
-	float Sn[8], Sp[8], vsn[3], vsp[3], vdbsn, vdbsp;
-	{
-		const Side<float>* sd[2] = {&L, &R};
-		const float sfd[2] = {sfdn, sfdp}, smd[2] = {smdn, smdp}, ism[2] = {ismdn, ismdp};
-		const float ds[2] = {dsn, dsp}, nm[2] = {nmn, nmp};
-		float* SS[2] = {Sn, Sp};
-		float* vs[2] = {vsn, vsp};
-		float* vdb[2] = {&vdbsn, &vdbsp};
-		for (int q = 0; q < 2; q++) {
-			const Side<float>& S = *sd[q];
-			float* st = SS[q];
-			const float tmp = S.u[RHO] * sfd[q] * smd[q] - Bx2i;
-			if (fabsf(tmp) < A::eps * ptst) {
-				for (int i = 1; i < 3; i++) {
-					st[M0 + i] = S.v[i] * ds[q];
-					st[B0 + i] = S.u[B0 + i];
-				}
-			} else {
-				const float it = A::rcp(tmp);
-				const float tmp1 = Bx * (sm - S.v[0]) * inv * it;
-				const float tmp2 = (S.u[RHO] * (sfd[q] * sfd[q]) - Bx2i) * it;
-				for (int i = 1; i < 3; i++) {
-					st[M0 + i] = (S.v[i] - S.u[B0 + i] * tmp1) * ds[q];
-					st[B0 + i] = S.u[B0 + i] * tmp2;
-				}
-			}
-			st[RHO] = ds[q];
-			st[M0] = ds[q] * sm;
-			st[B0] = Bx;
-			const float ids = A::rcp(ds[q]);
-			for (int i = 0; i < 3; i++) vs[q][i] = st[M0 + i] * ids;
-			*vdb[q] = dot3(vs[q], st + B0);
-			st[NRJ] = (sfd[q] * S.u[NRJ] - (S.p + nm[q]) * S.v[0] + ptst * sm
-				+ Bx * inv * (dot3(S.v, S.u + B0) - *vdb[q])) * ism[q];
-		}
-	}
-	/* double-star states, eqns 49-52 */
-	float S2n[8], S2p[8];
-	if (Bx2 * 0.5f < A::eps * ptst) {
-		for (int q = 0; q < 8; q++) {
-			S2n[q] = Sn[q];
-			S2p[q] = Sp[q];
-		}
-	} else {
+	/* double-star states minus single-star states, eqns 59-63; they are
+	   used where the FP64 solver uses them (its threshold: with a weak
+	   normal field the region between S*_L and S*_R is thin but its states
+	   differ from the single-star ones by the jumps) */
+	float D2n[8] = {0, 0, 0, 0, 0, 0, 0, 0}, D2p[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	if (!(Bx2 * 0.5f < (float)DBL_EPSILON * ptst)) {
 		const float isds = A::rcp(dssn + dssp);
 		const float sgn = (Bx > 0) ? 1.0f : -1.0f;
-		float v2[3];
-		const float c1 = sgn * inv_sqrt;
-		for (int i = 0; i < 3; i++) v2[i] = ((vsn[i] * dssn + vsp[i] * dssp) + (Sp[B0 + i] - Sn[B0 + i]) * c1) * isds;
-		v2[0] = sm;
-		S2n[RHO] = Sn[RHO];
-		S2p[RHO] = Sp[RHO];
-		S2n[M0] = Sn[M0];
-		S2p[M0] = Sp[M0];
-		for (int i = 1; i < 3; i++) {
-			S2n[M0 + i] = v2[i] * Sn[RHO];
-			S2p[M0 + i] = v2[i] * Sp[RHO];
-		}
 		const float c2 = sgn * dssn * dssp * (float)ph.sqrt_mu0;
-		S2n[B0] = S2p[B0] = Bx;
+		float vdn = 0, vdp = 0;   /* v*.B* - v**.B** on each side */
 		for (int i = 1; i < 3; i++) {
-			const float b = ((Sp[B0 + i] * dssn + Sn[B0 + i] * dssp) + (vsp[i] - vsn[i]) * c2) * isds;
-			S2n[B0 + i] = b;
-			S2p[B0 + i] = b;
+			/* v*_R - v*_L and B*_R - B*_L from the FP64 jumps */
+			const float jv = (float)(vR64[i] - vL64[i]) + (Sp.vs[i] - Sn.vs[i]);
+			const float jb = (float)(R.u[B0 + i] - L.u[B0 + i]) + (Sp.dB[i] - Sn.dB[i]);
+			const float dvn = (dssp * jv + jb * sgn * inv_sqrt) * isds;
+			const float dvp = (-dssn * jv + jb * sgn * inv_sqrt) * isds;
+			const float dbn = (dssn * jb + jv * c2) * isds;
+			const float dbp = (-dssp * jb + jv * c2) * isds;
+			D2n[M0 + i] = Sn.rho_s * dvn;
+			D2p[M0 + i] = Sp.rho_s * dvp;
+			D2n[B0 + i] = dbn;
+			D2p[B0 + i] = dbp;
+			const float vsn = (float)vL64[i] + Sn.vs[i], vsp = (float)vR64[i] + Sp.vs[i];
+			vdn -= vsn * dbn + dvn * Sn.Bs[i] + dvn * dbn;
+			vdp -= vsp * dbp + dvp * Sp.Bs[i] + dvp * dbp;
 		}
-		const float vdb2 = dot3(v2, S2n + B0);
-		S2n[NRJ] = Sn[NRJ] - dssn * sgn * (vdbsn - vdb2) * inv_sqrt;
-		S2p[NRJ] = Sp[NRJ] + dssp * sgn * (vdbsp - vdb2) * inv_sqrt;
+		D2n[NRJ] = -dssn * sgn * vdn * inv_sqrt;
+		D2p[NRJ] = dssp * sgn * vdp * inv_sqrt;
 	}
 	if (ssn >= 0.0f) {
-		for (int q = 0; q < 8; q++) F[q] = FL[q] + (Sn[q] - L.u[q]) * msn;
+		for (int q = 0; q < 8; q++) F[q] = FL[q] + (double)(Sn.dU[q] * msn);
 	} else if (sm >= 0.0f) {
-		const float d = ssn - msn;
-		for (int q = 0; q < 8; q++) F[q] = FL[q] + S2n[q] * ssn - Sn[q] * d - L.u[q] * msn;
+		for (int q = 0; q < 8; q++) F[q] = FL[q] + (double)(D2n[q] * ssn + Sn.dU[q] * msn);
 	} else if (ssp > 0.0f) {
-		const float d = ssp - msp;
-		for (int q = 0; q < 8; q++) F[q] = FR[q] + S2p[q] * ssp - Sp[q] * d - R.u[q] * msp;
+		for (int q = 0; q < 8; q++) F[q] = FR[q] + (double)(D2p[q] * ssp + Sp.dU[q] * msp);
 	} else {
-		for (int q = 0; q < 8; q++) F[q] = FR[q] + (Sp[q] - R.u[q]) * msp;
+		for (int q = 0; q < 8; q++) F[q] = FR[q] + (double)(Sp.dU[q] * msp);
 	}
 	F[B0] = 0;
 	s_out = smax(fabsf(msn), fabsf(msp));
 	return ERR_OK;
 }
 
-__device__ __forceinline__ int flux_hlld32(const double Ld[8], const double Rd[8], const Phys& ph,
-	float F[8], float& s_out)
+/* the mixed HLLD of one face from the two states */
+__device__ __forceinline__ int flux_hlld_mixed(const double Ld[8], const double Rd[8], const Phys& ph,
+	double F[8], float& s_out)
 {
-	Side<float> L, R;
-	float cfL = 0, cfR = 0;
-	int cL, cR;
-	const int sL = hlld_side<float>(Ld, ph, L, cfL, cL);
-	const int sR = hlld_side<float>(Rd, ph, R, cfR, cR);
-	if (sL | sR) return first_side_error(sL, cL, sR, cR);
-	return flux_hlld32_face(L, R, cfL, cfR, ph, F, s_out);
+	XPart<float> L, R;
+	hlld_part<float>(Ld, ph, L);
+	hlld_part<float>(Rd, ph, R);
+	const int sl = L.sc >> 4, sr = R.sc >> 4;
+	if (sl | sr) return first_side_error(sl, L.sc & 15, sr, R.sc & 15);
+	return flux_hlld_mixed_face(L, R, ph, F, s_out);
 }
 
 /* HLLD's per-face part in FP64, after both sides passed hlld_side */
@@ -852,21 +932,19 @@ template <class T> __device__ __forceinline__ int flux_hlld_face(const Side<T>& 
 	return ERR_OK;
 }
 
-/* source/mhd/hlld_athena.hpp:54-559 (ignores the background field, :77) */
+/* source/mhd/hlld_athena.hpp:54-559 (ignores the background field, :77) in
+   FP64; the mixed path's HLLD is flux_hlld_mixed */
 template <class T> __device__ __forceinline__ int flux_hlld(const double Ld[8],
 	const double Rd[8], const T* /*bg*/, const Phys& ph, T F[8], T& s_out)
 {
-	if constexpr (std::is_same_v<T, float>) {
-		return flux_hlld32(Ld, Rd, ph, F, s_out);
-	} else {
-		Side<T> L, R;
-		T cfL = 0, cfR = 0;
-		int cL, cR;
-		const int sL = hlld_side<T>(Ld, ph, L, cfL, cL);
-		const int sR = hlld_side<T>(Rd, ph, R, cfR, cR);
-		if (sL | sR) return first_side_error(sL, cL, sR, cR);
-		return flux_hlld_face<T>(L, R, cfL, cfR, Ld, Rd, ph, F, s_out);
-	}
+	static_assert(std::is_same_v<T, double>, "FP64 HLLD; the mixed form is flux_hlld_mixed");
+	Side<T> L, R;
+	T cfL = 0, cfR = 0;
+	int cL, cR;
+	const int sL = hlld_side<T>(Ld, ph, L, cfL, cL);
+	const int sR = hlld_side<T>(Rd, ph, R, cfR, cR);
+	if (sL | sR) return first_side_error(sL, cL, sR, cR);
+	return flux_hlld_face<T>(L, R, cfL, cfR, Ld, Rd, ph, F, s_out);
 }
 
 /*
@@ -1267,6 +1345,26 @@ template <int SOLVER, class T> __device__ __forceinline__ int riemann(const doub
 	else if constexpr (SOLVER == HLLD_ATHENA) return flux_hlld<T>(L, R, bg, ph, F, s);
 	else if constexpr (SOLVER == ROE_ATHENA) return flux_roe<T>(L, R, bg, ph, F, s);
 	else return ERR_SOLVER;
+}
+
+/* a solver in flux precision T with FP64 results: T = float gives the FP32
+   forms, except HLLD, whose mixed form (flux_hlld_mixed) returns FP64
+   physical fluxes plus FP32 fan corrections */
+template <int SOLVER, class T> __device__ __forceinline__ int riemann_d(const double L[8],
+	const double R[8], const T bg[3], const Phys& ph, double F[8], double& s)
+{
+	if constexpr (SOLVER == HLLD_ATHENA && std::is_same_v<T, float>) {
+		float sf = 0;
+		const int e = flux_hlld_mixed(L, R, ph, F, sf);
+		s = sf;
+		return e;
+	} else {
+		T Ft[8], st = 0;
+		const int e = riemann<SOLVER, T>(L, R, bg, ph, Ft, st);
+		for (int q = 0; q < 8; q++) F[q] = (double)Ft[q];
+		s = (double)st;
+		return e;
+	}
 }
 
 inline Phys make_phys(double gamma, double mu0)

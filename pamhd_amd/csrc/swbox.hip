@@ -91,6 +91,7 @@ struct BoxSet {
 	BoxStatus* host_status = nullptr;
 	double* host_dt = nullptr;
 	int last = 0;
+	std::vector<long long> faces;  /* per box: faces between cells of type >= 0 */
 };
 
 namespace {
@@ -713,6 +714,11 @@ hipError_t copy_box(BoxSet* s, int box, double* host, bool to_host)
 } // namespace
 
 int boxset_count(const BoxSet* s) { return s->nbox; }
+long long boxset_faces(const BoxSet* s, int box)
+{
+	return (box >= 0 && box < (int)s->faces.size()) ? s->faces[box] : 0;
+}
+
 long long boxset_cells(const BoxSet* s, int box) { return s->host[box].n; }
 int boxset_tile(const BoxSet* s) { return s->tile; }
 
@@ -852,6 +858,19 @@ int boxset_create(BoxSet** out, int nbox, const pamhd_grid_desc* grids, const pa
 		std::vector<signed char> ct(n);
 		std::vector<unsigned char> ru(n);
 		classify(G, w, ct.data(), ru.data());
+		{
+			long long nf = 0;
+			for (long k = 0; k < B.nz; k++)
+			for (long j = 0; j < B.ny; j++)
+			for (long i = 0; i < B.nx; i++) {
+				const long c = i + B.nx * (j + (long)B.ny * k);
+				if (ct[c] < 0) continue;
+				if (i > 0 && ct[c - 1] >= 0) nf++;
+				if (j > 0 && ct[c - B.nx] >= 0) nf++;
+				if (k > 0 && ct[c - (long)B.nx * B.ny] >= 0) nf++;
+			}
+			s->faces.push_back(nf);
+		}
 		std::vector<double> bg(9 * (size_t)n), st(NQ * (size_t)n);
 		const double end_x = G.start[0] + (double)G.cells[0] * G.cell_length[0];
 		for (long k = 0; k < B.nz; k++)

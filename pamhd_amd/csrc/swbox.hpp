@@ -26,6 +26,9 @@ void boxset_destroy(BoxSet* s);
 
 int boxset_count(const BoxSet* s);
 long long boxset_cells(const BoxSet* s, int box);
+/* faces a step of box `box` solves (both cells of type >= 0): timestep()'s
+   flux_calcs, source/mhd/solve.hpp:253-313 */
+long long boxset_faces(const BoxSet* s, int box);
 
 /* tile shape of the cell kernels: 0 = 64x4, 1 = 32x8, 2 = 16x16 */
 int boxset_set_tile(BoxSet* s, int tile);

@@ -206,6 +206,12 @@ class Grid:
         self._check(self._L.pamhd_last_max_substep(self._h, ctypes.byref(v)))
         return v.value
 
+    def last_flux_calcs(self):
+        """Faces solved by the last step: timestep()'s third return value."""
+        v = ctypes.c_ulonglong(0)
+        self._check(self._L.pamhd_last_flux_calcs(self._h, ctypes.byref(v)))
+        return v.value
+
     def set_launch(self, flux_remap=0, update_kz=0, update_remap=0):
         self._check(self._L.pamhd_set_launch(self._h, flux_remap, update_kz, update_remap))
 
@@ -256,6 +262,12 @@ class Grid:
         length dt that started from the state at state_ptr."""
         self._check(self._L.pamhd_adjoint_step(self._h, solver_id(solver), dt, state_ptr,
                                                lambda_ptr, stream))
+
+    def set_fused_step(self, fused=True, planes_per_chunk=0):
+        """HLLD's fused step (default) or the per-direction flux passes and
+        update pass; identical results."""
+        self._check(self._L.pamhd_set_fused_step(self._h, 1 if fused else 0,
+                                                 int(planes_per_chunk)))
 
     def set_slabs(self, slabs):
         """z-slab pipelining of the flux pass and the update (1 = off)."""

@@ -133,8 +133,15 @@ def lib():
     L.or_bnd_step.restype = ctypes.c_int
     L.or_set_threads.argtypes = [ctypes.c_int]
     L.or_set_threads.restype = ctypes.c_int
+    L.or_last_flux_calcs.argtypes = []
+    L.or_last_flux_calcs.restype = ctypes.c_ulonglong
     _lib = L
     return L
+
+
+def last_flux_calcs():
+    """Faces solved by the oracle's last step (timestep()'s flux_calcs)."""
+    return int(lib().or_last_flux_calcs())
 
 
 def flux(solver, L, R, bg=(0.0, 0.0, 0.0), gamma=5.0 / 3.0, mu0=1.0):

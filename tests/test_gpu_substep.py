@@ -22,14 +22,18 @@ def test_substepped_run_bit_exact(gpu, name, sid, smax, cells):
     ref = state.copy()
     st, steps_o, t_o, mp_o = O.run_periodic_substep(g_o, sid, 0.3, 0, smax, 0.0, 1e9, ref, max_steps=5)
     assert st == 0 and steps_o == 5 and mp_o > 1
+    fc_o = O.last_flux_calcs()
     with Grid(cells, length, gamma, mu0) as g:
         g.upload(state)
         g.set_substepping(0, smax)
         steps_g, t_g = g.run(name, 0.3, 0.0, 1e9, max_steps=5)
         mp_g = g.last_max_substep()
+        fc_g = g.last_flux_calcs()
         out = g.download()
-    print(name, "smax", smax, "largest period", mp_g, "t", t_g, flush=True)
+    print(name, "smax", smax, "largest period", mp_g, "t", t_g, "flux_calcs", fc_g, flush=True)
     assert steps_g == 5 and mp_g == mp_o and t_g == t_o
+    # timestep()'s third return value: the faces solved over the last step
+    assert fc_g == fc_o
     np.testing.assert_array_equal(out, ref)
 
 

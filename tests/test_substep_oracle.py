@@ -204,3 +204,24 @@ def test_substep_min_above_the_allowed_level_is_an_error():
     assert st == O.SUBSTEP_ERROR
     st, *_ = O.substep_periods(g, 0.3, 1e9, lo, 999, maxv)
     assert st == 0
+
+
+def test_flux_calcs_counts_the_faces_solved():
+    """timestep()'s third return value (source/mhd/solve.hpp:253-313,
+    :1918): every face once per global step; with substepping a face of
+    owner o (its negative-side cell) once per minsub(o) substeps, i.e.
+    3 maxp / minsub(o) solutions over a step of maxp substeps."""
+    cells = (12, 10, 8)
+    n = 12 * 10 * 8
+    state, length, gamma, mu0 = configs.hot_blob(cells, contrast=30.0, radius=0.25, seed=3)
+    g = O.make_grid(cells, length, gamma, mu0)
+    st, new, sm, zc = O.step(g, O.HLL, 1e-4, state, refits=3)
+    assert st == 0 and O.last_flux_calcs() == 3 * n
+    s = state.copy()
+    maxv = _maxv_after_steps(g, s, O.HLL)
+    st, period, minsub, sub_dt, maxp = O.substep_periods(g, 0.3, 1e9, 0, 3, maxv.copy())
+    assert st == 0 and maxp > 1 and minsub.min() < maxp
+    st, dt, mp = O.step_substep(g, O.HLL, 0.3, 1e9, 0, 3, s, maxv)
+    assert st == 0 and mp == maxp
+    assert O.last_flux_calcs() == int((3 * (maxp // minsub)).sum())
+    assert O.last_flux_calcs() < 3 * n * maxp
