@@ -330,6 +330,29 @@ int pamhd_plain1d_run(size_t ncells, double cell_size, double adiabatic_index,
 	double vacuum_permeability, double cfl, int solver, double t_end, long max_steps,
 	double* state, long* steps_taken, void* stream);
 
+/* Most cells pamhd_plain1d_run_noalloc takes (its scratch: 7 x (cells - 1)
+   doubles of face terms and a 256-byte status record, 14.7 MB per GPU). */
+#define PAMHD_PLAIN1D_SCRATCH_CELLS 262144
+
+/*
+pamhd_plain1d_run without any allocation, for callers that cannot allocate
+device or pinned memory (code under HIP graph capture, serving loops,
+harnesses that forbid hipMalloc).  No counterpart in the reference.
+state_device: device buffer of 8 * ncells FP64 (the layout of
+pamhd_plain1d_run), read and overwritten in place on `stream`; the face
+terms and the status record live in a fixed scratch array of the library
+(one per GPU, held until the process ends), so ncells is limited to
+PAMHD_PLAIN1D_SCRATCH_CELLS (larger: PAMHD_E_ARG).  Every call on one GPU
+shares that GPU's scratch: calls that overlap on one GPU (other host threads
+or streams) race and must be serialised by the caller.  The status record
+is reset by a kernel at every step; the step loop reads it back to the host
+(a synchronisation per step, as pamhd_plain1d_run).  Same results and
+errors as pamhd_plain1d_run.
+*/
+int pamhd_plain1d_run_noalloc(size_t ncells, double cell_size, double adiabatic_index,
+	double vacuum_permeability, double cfl, int solver, double t_end, long max_steps,
+	double* state_device, long* steps_taken, void* stream);
+
 /*
 Autotuner: times the launch variants of the flux and update passes that give
 bit-identical results (block order, per-cell vs z-marching update and its

@@ -29,7 +29,7 @@ EXPORTS = (
     "pamhd_face_flux_batch", "pamhd_autotune", "pamhd_set_launch", "pamhd_enable_timing", "pamhd_kernel_time",
     "pamhd_last_error", "pamhd_create_solar_wind_box", "pamhd_state_components",
     "pamhd_create_bounded", "pamhd_set_boundary_values", "pamhd_set_adjoint_jacobian",
-    "pamhd_grid_kind", "pamhd_plain1d_run",
+    "pamhd_grid_kind", "pamhd_plain1d_run", "pamhd_plain1d_run_noalloc",
     "pamhd_box_cell_types", "pamhd_box_background", "pamhd_ensemble_create",
     "pamhd_ensemble_destroy", "pamhd_ensemble_timestep", "pamhd_ensemble_run",
     "pamhd_ensemble_check", "pamhd_ensemble_download", "pamhd_ensemble_cells",
@@ -108,6 +108,10 @@ def load():
                                     ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                     ctypes.c_double, ctypes.c_long, _dpp,
                                     ctypes.POINTER(ctypes.c_long), ctypes.c_void_p]
+    L.pamhd_plain1d_run_noalloc.argtypes = [ctypes.c_size_t, ctypes.c_double, ctypes.c_double,
+                                            ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                            ctypes.c_double, ctypes.c_long, _vp,
+                                            ctypes.POINTER(ctypes.c_long), ctypes.c_void_p]
     L.pamhd_destroy.argtypes = [H]
     L.pamhd_upload_state.argtypes = [H, _vp]
     L.pamhd_download_state.argtypes = [H, _vp]

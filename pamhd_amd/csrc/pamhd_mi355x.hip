@@ -3644,6 +3644,26 @@ int pamhd_plain1d_run(size_t ncells, double cell_size, double adiabatic_index,
 	return PAMHD_OK;
 }
 
+int pamhd_plain1d_run_noalloc(size_t ncells, double cell_size, double adiabatic_index,
+	double vacuum_permeability, double cfl, int solver, double t_end, long max_steps,
+	double* state_device, long* steps_taken, void* stream)
+{
+	if (!state_device) return fail(nullptr, PAMHD_E_ARG, "null state");
+	if (!(cell_size > 0) || !(adiabatic_index > 1) || !(vacuum_permeability > 0) || !(cfl > 0) ||
+		max_steps < 0)
+		return fail(nullptr, PAMHD_E_ARG, "invalid cell size, physics constants, cfl or step limit");
+	std::string err;
+	const int e = pamhd_dev::plain1d_run_device((long long)ncells, cell_size, adiabatic_index,
+		vacuum_permeability, cfl, solver, t_end, max_steps, state_device, steps_taken,
+		(hipStream_t)stream, err);
+	if (e) {
+		set_global_error(e == PAMHD_E_ARG || e == PAMHD_E_SOLVER || e == PAMHD_E_DEVICE ? err :
+			std::string(error_name(e)) + " (" + err + ")");
+		return e;
+	}
+	return PAMHD_OK;
+}
+
 int pamhd_face_flux_batch(int solver, int precision, size_t n, const double* L, const double* R,
 	const double* bg, double gamma, double mu0, double* F, double* smax_out, int* status,
 	void* stream)

@@ -22,6 +22,7 @@ length 0, until t >= t_end.  Data in HBM, component-major: S 8 x n
 #include <hip/hip_runtime.h>
 #include <cfloat>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include "riemann.hpp"
 #include "plain1d.hpp"
@@ -136,11 +137,120 @@ void launch_plain_step(long long n, double* S, double* PF, PlainStatus* st, cons
 
 double smin_host(double a, double b) { return (b < a) ? b : a; }
 
-} // namespace
+/*
+Scratch of the allocation-free entry point (plain1d_run_device): the status
+record in the first 256 bytes, then the face terms, 7 x (n - 1) doubles for
+at most PAMHD_PLAIN1D_SCRATCH_CELLS cells (14.7 MB).  Each GPU the library
+runs on holds its own copy until the process ends; host code reaches it only
+through the address hipGetSymbolAddress returns for the current device.
+*/
+constexpr long long SCRATCH_STATUS_DOUBLES = 32;
+constexpr long long SCRATCH_DOUBLES =
+	SCRATCH_STATUS_DOUBLES + (long long)PF_N * (PAMHD_PLAIN1D_SCRATCH_CELLS - 1);
+static_assert(sizeof(PlainStatus) <= SCRATCH_STATUS_DOUBLES * sizeof(double), "status fits");
+__device__ __align__(256) double scratch[SCRATCH_DOUBLES];
 
-int plain1d_run(long long n, double cell_size, double gamma, double mu0, double cfl, int solver,
-	double t_end, long max_steps, double* state, long* steps_taken, hipStream_t stream,
-	std::string& err)
+constexpr int SCRATCH_MAX_DEVICES = 64;
+std::mutex scratch_mutex;
+double* scratch_ptr[SCRATCH_MAX_DEVICES] = {};
+
+/* the current device's copy of scratch, looked up once per device */
+int scratch_address(double** out, std::string& err)
+{
+	int dev = -1;
+	hipError_t he = hipGetDevice(&dev);
+	if (he != hipSuccess) {
+		err = std::string("hipGetDevice: ") + hipGetErrorString(he);
+		return PAMHD_E_DEVICE;
+	}
+	if (dev < 0 || dev >= SCRATCH_MAX_DEVICES) {
+		err = "device index " + std::to_string(dev) + " beyond the scratch table";
+		return PAMHD_E_DEVICE;
+	}
+	std::lock_guard<std::mutex> lock(scratch_mutex);
+	if (!scratch_ptr[dev]) {
+		void* p = nullptr;
+		he = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+		if (he != hipSuccess || !p) {
+			err = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(he);
+			return PAMHD_E_DEVICE;
+		}
+		scratch_ptr[dev] = (double*)p;
+	}
+	*out = scratch_ptr[dev];
+	return PAMHD_OK;
+}
+
+/* the status record at the start of a step, written by a kernel (no memset
+   or host copy, so the entry point can be captured) */
+__global__ void plain_status_init_kernel(PlainStatus* st)
+{
+	if (threadIdx.x == 0 && blockIdx.x == 0) {
+		st->err_face = ~0ull;
+		st->err_cell = ~0ull;
+		st->dt_bits = (unsigned long long)__double_as_longlong(DBL_MAX);
+	}
+}
+
+/* the main loop of reference.cpp:830-877 over device buffers: S is the
+   caller's state, PF and st are work buffers, host_st host memory for the
+   status read back after each step */
+int plain_loop(long long n, double cell_size, const Phys& ph, double cfl, int solver, double t_end,
+	long max_steps, double* S, double* PF, PlainStatus* st, PlainStatus* host_st, long* steps_taken,
+	hipStream_t stream, bool init_by_kernel, std::string& err)
+{
+	const double inverse_volume = 1.0 / (1.0 * cell_size);
+	double max_dt = 0, t = 0;
+	long steps = 0;
+	int code = PAMHD_OK;
+	hipError_t he;
+	auto hip_err = [&](hipError_t e, const char* what) {
+		err = std::string(what) + ": " + hipGetErrorString(e);
+		return PAMHD_E_DEVICE;
+	};
+	while (t < t_end && steps < max_steps) {
+		const double until_end = t_end - t;
+		const double time_step = smin_host(cfl * max_dt, until_end);
+		if (init_by_kernel) {
+			plain_status_init_kernel<<<1, 64, 0, stream>>>(st);
+		} else {
+			PlainStatus init;
+			init.err_face = init.err_cell = ~0ull;
+			const double big = DBL_MAX;
+			memcpy(&init.dt_bits, &big, sizeof big);
+			*host_st = init;
+			if ((he = hipMemcpyAsync(st, host_st, sizeof(PlainStatus), hipMemcpyHostToDevice, stream)) != hipSuccess)
+				return hip_err(he, "hipMemcpyAsync");
+		}
+		switch (solver) {
+		case RUSANOV: launch_plain_step<RUSANOV>(n, S, PF, st, ph, time_step, cell_size, inverse_volume, stream); break;
+		case HLL_ATHENA: launch_plain_step<HLL_ATHENA>(n, S, PF, st, ph, time_step, cell_size, inverse_volume, stream); break;
+		case HLLD_ATHENA: launch_plain_step<HLLD_ATHENA>(n, S, PF, st, ph, time_step, cell_size, inverse_volume, stream); break;
+		default: launch_plain_step<ROE_ATHENA>(n, S, PF, st, ph, time_step, cell_size, inverse_volume, stream); break;
+		}
+		if ((he = hipGetLastError()) != hipSuccess) return hip_err(he, "kernel launch");
+		if ((he = hipMemcpyAsync(host_st, st, sizeof(PlainStatus), hipMemcpyDeviceToHost, stream)) != hipSuccess)
+			return hip_err(he, "hipMemcpyAsync");
+		if ((he = hipStreamSynchronize(stream)) != hipSuccess) return hip_err(he, "hipStreamSynchronize");
+		const bool face = host_st->err_face != ~0ull;
+		if (face || host_st->err_cell != ~0ull) {
+			const unsigned long long e = face ? host_st->err_face : host_st->err_cell;
+			code = (int)(e & 0xff);
+			err = std::string("step ") + std::to_string(steps) + (face ? ", face " : ", cell ") +
+				std::to_string(e >> 8);
+			break;
+		}
+		double next;
+		memcpy(&next, &host_st->dt_bits, sizeof next);
+		max_dt = smin_host(DBL_MAX, next);
+		t += time_step;
+		steps++;
+	}
+	if (steps_taken) *steps_taken = steps;
+	return code;
+}
+
+int check_args(long long n, int solver, std::string& err)
 {
 	if (n < 2) {
 		err = "the plain method needs at least two cells";
@@ -150,6 +260,17 @@ int plain1d_run(long long n, double cell_size, double gamma, double mu0, double 
 		err = "unknown solver";
 		return PAMHD_E_SOLVER;
 	}
+	return PAMHD_OK;
+}
+
+} // namespace
+
+int plain1d_run(long long n, double cell_size, double gamma, double mu0, double cfl, int solver,
+	double t_end, long max_steps, double* state, long* steps_taken, hipStream_t stream,
+	std::string& err)
+{
+	int code = check_args(n, solver, err);
+	if (code) return code;
 	const Phys ph = make_phys(gamma, mu0);
 	double* S = nullptr;
 	double* PF = nullptr;
@@ -174,50 +295,38 @@ int plain1d_run(long long n, double cell_size, double gamma, double mu0, double 
 		return hip_err(he, "hipHostMalloc");
 	if ((he = hipMemcpyAsync(S, state, sizeof(double) * 8 * n, hipMemcpyHostToDevice, stream)) != hipSuccess)
 		return hip_err(he, "hipMemcpyAsync");
-	const double inverse_volume = 1.0 / (1.0 * cell_size);
-	double max_dt = 0, t = 0;
-	long steps = 0;
-	int code = PAMHD_OK;
-	while (t < t_end && steps < max_steps) {
-		const double until_end = t_end - t;
-		const double time_step = smin_host(cfl * max_dt, until_end);
-		PlainStatus init;
-		init.err_face = init.err_cell = ~0ull;
-		const double big = DBL_MAX;
-		memcpy(&init.dt_bits, &big, sizeof big);
-		*host = init;
-		if ((he = hipMemcpyAsync(st, host, sizeof(PlainStatus), hipMemcpyHostToDevice, stream)) != hipSuccess)
-			return hip_err(he, "hipMemcpyAsync");
-		switch (solver) {
-		case RUSANOV: launch_plain_step<RUSANOV>(n, S, PF, st, ph, time_step, cell_size, inverse_volume, stream); break;
-		case HLL_ATHENA: launch_plain_step<HLL_ATHENA>(n, S, PF, st, ph, time_step, cell_size, inverse_volume, stream); break;
-		case HLLD_ATHENA: launch_plain_step<HLLD_ATHENA>(n, S, PF, st, ph, time_step, cell_size, inverse_volume, stream); break;
-		default: launch_plain_step<ROE_ATHENA>(n, S, PF, st, ph, time_step, cell_size, inverse_volume, stream); break;
-		}
-		if ((he = hipGetLastError()) != hipSuccess) return hip_err(he, "kernel launch");
-		if ((he = hipMemcpyAsync(host, st, sizeof(PlainStatus), hipMemcpyDeviceToHost, stream)) != hipSuccess)
-			return hip_err(he, "hipMemcpyAsync");
-		if ((he = hipStreamSynchronize(stream)) != hipSuccess) return hip_err(he, "hipStreamSynchronize");
-		const bool face = host->err_face != ~0ull;
-		if (face || host->err_cell != ~0ull) {
-			const unsigned long long e = face ? host->err_face : host->err_cell;
-			code = (int)(e & 0xff);
-			err = std::string("step ") + std::to_string(steps) + (face ? ", face " : ", cell ") +
-				std::to_string(e >> 8);
-			break;
-		}
-		double next;
-		memcpy(&next, &host->dt_bits, sizeof next);
-		max_dt = smin_host(DBL_MAX, next);
-		t += time_step;
-		steps++;
+	code = plain_loop(n, cell_size, ph, cfl, solver, t_end, max_steps, S, PF, st, host, steps_taken,
+		stream, false, err);
+	if (code == PAMHD_E_DEVICE) {
+		release();
+		return code;
 	}
-	if (steps_taken) *steps_taken = steps;
 	if ((he = hipMemcpyAsync(state, S, sizeof(double) * 8 * n, hipMemcpyDeviceToHost, stream)) != hipSuccess)
 		return hip_err(he, "hipMemcpyAsync");
 	if ((he = hipStreamSynchronize(stream)) != hipSuccess) return hip_err(he, "hipStreamSynchronize");
 	release();
 	return code;
+}
+
+int plain1d_run_device(long long n, double cell_size, double gamma, double mu0, double cfl,
+	int solver, double t_end, long max_steps, double* state, long* steps_taken, hipStream_t stream,
+	std::string& err)
+{
+	int code = check_args(n, solver, err);
+	if (code) return code;
+	if (n > PAMHD_PLAIN1D_SCRATCH_CELLS) {
+		err = "the allocation-free plain method takes at most " +
+			std::to_string(PAMHD_PLAIN1D_SCRATCH_CELLS) + " cells (" + std::to_string(n) + " given)";
+		return PAMHD_E_ARG;
+	}
+	double* base = nullptr;
+	code = scratch_address(&base, err);
+	if (code) return code;
+	PlainStatus* st = (PlainStatus*)base;
+	double* PF = base + SCRATCH_STATUS_DOUBLES;
+	PlainStatus host;
+	return plain_loop(n, cell_size, make_phys(gamma, mu0), cfl, solver, t_end, max_steps, state, PF,
+		st, &host, steps_taken, stream, true, err);
 }
 
 } // namespace pamhd_dev
