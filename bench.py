@@ -41,40 +41,94 @@ def cpu_model():
     return platform.processor() or "unknown"
 
 
-def cpu_baseline(solver_name, cfl, seconds, threads):
-    """The FP64 oracle (a port of the reference's step) on a 64^3 sample of
-    the same workload, stepped until about `seconds` of wall time."""
+def host_threads():
+    """Threads for the CPU leg: the cores this process may run on, at most
+    16 (a one-GPU box's share of its host)."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cpu_baseline(solver_name, cfl, seconds, threads, state, cells, length, gamma, mu0,
+                 max_steps=1 << 30):
+    """The FP64 oracle (a port of the reference's step: same per-face
+    arithmetic, CT, refits and lagged CFL, OpenMP over cells) on the same
+    workload as the GPU, after its final-init step, stepped until about
+    `seconds` of wall time (at least one step, at most max_steps)."""
     sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
     import oracle_lib as O
-    from pamhd_amd import configs
-    cells = (64, 64, 64)
-    state, length, gamma, mu0 = configs.orszag_tang_vortex(cells, noise=0.01, seed=2024)
     sid = {"rusanov": O.RUSANOV, "hll-athena": O.HLL, "hlld-athena": O.HLLD,
            "roe-athena": O.ROE}[solver_name]
     used = O.lib().or_set_threads(threads)
     g = O.make_grid(cells, length, gamma, mu0)
-    st, steps, _ = O.run_periodic(g, sid, cfl, 0.0, 1e30, state, max_steps=0)
     # final init (not timed), then timed steps
     st, new, sm, zc = O.step(g, sid, 0.0, state, refits=2)
-    state = new
+    cur = new
     n = cells[0] * cells[1] * cells[2]
     done = 0
     t0 = time.perf_counter()
     while True:
         dt = O.cfl_dt(g, cfl, sm, zc)
-        st, new, sm, zc = O.step(g, sid, dt, state, refits=3)
+        st, new, sm, zc = O.step(g, sid, dt, cur, refits=3)
         if st != 0:
             raise RuntimeError("oracle step failed with %d" % st)
-        state = new
+        cur = new
         done += 1
         el = time.perf_counter() - t0
-        if el >= seconds:
+        if el >= seconds or done >= max_steps:
             break
-    return {"value": n * done / el, "unit": "cell-updates/s", "cores": used,
-            "kind": "port",
-            "sample": "%d steps of the %dx%dx%d periodic vortex, %s, FP64 oracle "
-                      "(oracle/pamhd_oracle.c, -O2, no FMA), %d thread(s), %s"
-                      % (done, *cells, solver_name, used, cpu_model())}
+    return {"value": n * done / el, "unit": "cell-updates/s", "cores": used, "kind": "port",
+            "sample": "%d step(s) of the %dx%dx%d periodic vortex after its final init, %s, "
+                      "FP64 oracle (oracle/pamhd_oracle.c, -O2, no FMA), %d thread(s), %s"
+                      % (done, *cells, solver_name, used, cpu_model()),
+            "steps": done, "seconds": round(el, 3)}
+
+
+def flux_calibration(faces=200000, reps=5, seed=7):
+    """Per-face time of the reference's own compiled Rusanov and HLL flux
+    functions (oracle/_ref/libref_flux.so, built from the reference's
+    headers) against the oracle's restatement of them, one thread, on the
+    same random faces (best of `reps` runs each); the ratio says how the
+    restated CPU baseline relates to the reference's arithmetic.  None when
+    the reference build is absent."""
+    import ctypes
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests"))
+    import numpy as np
+    import oracle_lib as O
+    import states
+    if not os.path.exists(O.REF_SO):
+        return None
+    R = ctypes.CDLL(O.REF_SO)
+    dp = ctypes.POINTER(ctypes.c_double)
+    f = R.ref_flux_batch
+    f.argtypes = [ctypes.c_int, ctypes.c_size_t, dp, dp, dp, ctypes.c_double, ctypes.c_double,
+                  dp, dp]
+    f.restype = ctypes.c_longlong
+    L, Rr, _ = states.random_faces(faces, seed, kind="generic")
+    L, Rr = np.ascontiguousarray(L), np.ascontiguousarray(Rr)
+    F, sp = np.zeros((8, faces)), np.zeros(faces)
+    p = lambda a: a.ctypes.data_as(dp)
+    out = {}
+    for name, which, sid in (("rusanov", 0, O.RUSANOV), ("hll-athena", 2, O.HLL)):
+        t_ref, t_or = [], []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            f(which, faces, p(L), p(Rr), None, 5.0 / 3.0, 1.0, p(F), p(sp))
+            t1 = time.perf_counter()
+            Fo, so, st = O.flux_batch(sid, L, Rr)
+            t2 = time.perf_counter()
+            t_ref.append(t1 - t0)
+            t_or.append(t2 - t1)
+        same = bool(np.array_equal(F, Fo) and np.array_equal(sp, so))
+        out[name] = {"reference_ns_per_face": round(min(t_ref) / faces * 1e9, 2),
+                     "oracle_ns_per_face": round(min(t_or) / faces * 1e9, 2),
+                     "reference_over_oracle": min(t_ref) / min(t_or), "identical": same}
+    out["faces"] = faces
+    out["note"] = ("reference flux headers compiled with clang++ -O2 -ffp-contract=off; "
+                   "HLLD and Roe headers need gensimcell / muparserx and are not built")
+    return out
 
 
 def run_gpu(prec, state, cells, length, gamma, mu0, solver, cfl, steps, warmup):
@@ -452,7 +506,8 @@ def main():
     ap.add_argument("--size", type=int, default=256)
     ap.add_argument("--solver", default="hlld-athena")
     ap.add_argument("--cfl", type=float, default=0.3)
-    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="wall time of the all-core CPU baseline (the 1-core figure is one step)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--adjoint-steps", type=int, default=200,
                     help="0 skips the adjoint workload")
@@ -549,9 +604,25 @@ def main():
 
     base = None
     if not args.no_cpu_baseline:
-        log("cpu baseline: oracle, 1 thread, ~%.0f s" % args.cpu_seconds)
-        base = cpu_baseline(args.solver, args.cfl, args.cpu_seconds, 1)
-        log("  %.3e cell-updates/s" % base["value"])
+        threads = host_threads()
+        log("cpu baseline: oracle on the %d^3 workload, %d threads (nproc %s), ~%.0f s"
+            % (args.size, threads, os.cpu_count(), args.cpu_seconds))
+        state, length, gamma, mu0 = configs.orszag_tang_vortex(cells, noise=0.01, seed=2024)
+        base = cpu_baseline(args.solver, args.cfl, args.cpu_seconds, threads, state, cells,
+                            length, gamma, mu0)
+        log("  %.3e cell-updates/s (%d steps)" % (base["value"], base["steps"]))
+        log("cpu baseline: 1 thread, one step")
+        one = cpu_baseline(args.solver, args.cfl, 0.0, 1, state, cells, length, gamma, mu0,
+                           max_steps=1)
+        del state
+        log("  %.3e cell-updates/s" % one["value"])
+        base["single_core"] = {"value": one["value"], "cores": 1, "steps": one["steps"],
+                               "seconds": one["seconds"]}
+        base["nproc"] = os.cpu_count()
+        base["cpu_model"] = cpu_model()
+        log("flux calibration: reference's compiled flux vs the oracle's, per face")
+        base["calibration"] = flux_calibration()
+        log("  %s" % base["calibration"])
 
     f64 = results["fp64"]
     mix = results["mixed"]

@@ -383,14 +383,18 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen,
    per block of the z-marching update.  All give identical results. */
 int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_remap);
 
-/* HLLD's fused step (the default, fused = 1): one pass computes every face
-   flux of the step and the conserved update of each cell without a flux
-   array, keeping only the two field components of each face that the
-   constrained-transport edge terms use (6 doubles per cell); a second pass
-   updates the face field and refits the energy.  fused = 0 selects the
-   per-direction flux passes and the update pass.  planes_per_chunk (1..32,
-   0 = keep) is the z-extent of the fused pass's blocks.  Both give
-   identical results.  (No counterpart in the reference.) */
+/* HLLD's step structure (no counterpart in the reference; all give
+   identical results): fused = 0 per-direction flux passes that store every
+   face flux, then an update pass; 1 one pass computing every face flux and
+   the conserved update of each cell without a flux array; 2 per-direction
+   marches, each adding its faces' flux differences to the new state in
+   place (no flux array); 1 and 2 keep only the two field components of
+   each face the constrained-transport edge terms use (6 doubles per cell)
+   and finish with a pass that updates the face field and refits the
+   energy.  -1 (the default) picks 2 in FP64 and 1 in mixed precision;
+   pamhd_autotune times them all.  planes_per_chunk (0 = keep) is the
+   z-extent of mode 1's blocks (1..32) or the cells per march of mode 2
+   (1..32). */
 int pamhd_set_fused_step(pamhd_handle* h, int fused, int planes_per_chunk);
 
 /* Pipelines the flux pass and the per-cell update over `slabs` z-slabs on

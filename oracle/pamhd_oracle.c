@@ -970,6 +970,20 @@ static void refit_energy(double rho, const double m[3], double* E,
 	*E = or_total_energy(rho, v, p, vb_new, gamma, mu0);
 }
 
+/* the periodic step's flux buffer, grown as needed and kept between calls
+   (one caller at a time) */
+static double* step_flux_buffer(size_t count)
+{
+	static double* buf = NULL;
+	static size_t cap = 0;
+	if (count > cap) {
+		free(buf);
+		buf = (double*)malloc(sizeof(double) * count);
+		cap = buf ? count : 0;
+	}
+	return buf;
+}
+
 int or_step_periodic(const or_grid* g, int solver, double dt, int refits,
 	const double* in, double* out, double smax_dim[3], long zero_dim[3],
 	long* bad_face)
@@ -978,8 +992,9 @@ int or_step_periodic(const or_grid* g, int solver, double dt, int refits,
 	const long n = D.n;
 	const double len[3] = {g->dx, g->dy, g->dz};
 	/* P[(d*8 + q)*n + c] = dt * F_q (global frame) of the d-face owned by
-	   cell c, i.e. the face on the cell's negative d side */
-	double* P = (double*)malloc(sizeof(double) * 24 * n);
+	   cell c, i.e. the face on the cell's negative d side; kept between
+	   calls (the reference keeps its flux slots in the cells) */
+	double* P = step_flux_buffer(24 * n);
 	if (!P) return OR_ERR_SOLVER;
 	int err = OR_OK;
 	double sm[3] = {0, 0, 0};
@@ -1034,10 +1049,7 @@ int or_step_periodic(const or_grid* g, int solver, double dt, int refits,
 	if (err && bad_face) *bad_face = bad;
 	if (smax_dim) for (int d = 0; d < 3; d++) smax_dim[d] = sm[d];
 	if (zero_dim) for (int d = 0; d < 3; d++) zero_dim[d] = zc[d];
-	if (err) {
-		free(P);
-		return err;
-	}
+	if (err) return err;
 
 	const double dx = g->dx, dy = g->dy, dz = g->dz;
 #define PF(d, q, i, j, k) P[((d) * 8 + (q)) * n + cid(&D, (i), (j), (k))]
@@ -1158,7 +1170,6 @@ int or_step_periodic(const or_grid* g, int solver, double dt, int refits,
 #undef EZY
 #undef EXZ
 #undef EYZ
-	free(P);
 	return OR_OK;
 }
 

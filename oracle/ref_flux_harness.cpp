@@ -102,3 +102,33 @@ extern "C" int ref_flux_hll(const double* L, const double* R, const double* bg,
 		return pamhd::mhd::athena::get_flux_hll<Mas_T, Mom_T, Nrj_T, Mag_T>(l, r, b, gamma, mu0);
 	}, L, R, bg, F, smax);
 }
+
+/* A batch of faces in the layout of or_face_flux_batch (component-major
+   L, R (8 * n), bg (3 * n) or NULL, F (8 * n), smax (n)), for timing the
+   reference's compiled flux against the restatement's on the same faces:
+   which = 0 Rusanov, 2 HLL (the solver numbering of
+   source/mhd/common.hpp:54-61).  Returns the number of faces whose solver
+   threw (their flux and speed are zero). */
+extern "C" long long ref_flux_batch(int which, size_t n, const double* L, const double* R,
+	const double* bg, double gamma, double mu0, double* F, double* smax)
+{
+	long long bad = 0;
+	for (size_t f = 0; f < n; f++) {
+		double l[8], r[8], b[3] = {0, 0, 0}, out[8] = {0, 0, 0, 0, 0, 0, 0, 0}, s = 0;
+		for (int q = 0; q < 8; q++) {
+			l[q] = L[q * n + f];
+			r[q] = R[q * n + f];
+		}
+		if (bg) for (int q = 0; q < 3; q++) b[q] = bg[q * n + f];
+		const int e = which == 0 ? ref_flux_rusanov(l, r, b, gamma, mu0, out, &s)
+			: ref_flux_hll(l, r, b, gamma, mu0, out, &s);
+		if (e) {
+			bad++;
+			for (int q = 0; q < 8; q++) out[q] = 0;
+			s = 0;
+		}
+		for (int q = 0; q < 8; q++) F[q * n + f] = out[q];
+		smax[f] = s;
+	}
+	return bad;
+}

@@ -1581,6 +1581,305 @@ __global__ __launch_bounds__(TX * TY) void ct_refit_kernel(GridDev g,
 	}
 }
 
+/*
+Accumulating step (HLLD): the three per-direction marches of the two-pass
+path, each finishing its direction's part of update_mhd_state's conserved
+update (source/mhd/solve.hpp:1503-1614) instead of storing face fluxes:
+
+  acc_x_kernel      x-faces, a wave per 64-cell x tile walking km rows along
+                    y (the left part of lane l is lane l - 1's, the face of
+                    lane l + 1 comes one lane down; the tile's left-edge parts
+                    and right-edge face fluxes are computed for all km rows
+                    before the march, one row per lane); writes
+                    out = U + x terms (8 values: rho, m, E, cell-centred B)
+  acc_march_kernel  y- (D = 1) and z-faces (D = 2), a thread walking km cells
+                    of a line along D with the part of the cell just walked
+                    carried to the next face (km + 1 faces for km cells);
+                    out += D terms in place; the z pass ends the update: it
+                    stores rho, m and the thermal pressure the first energy
+                    refit recovers (update_vol_B before the face update)
+
+Each pass stores the two field components of its faces that the
+constrained-transport edge terms use (`emf`, 6 values per cell in all), and
+ct_refit_kernel then updates the face field and refits the energy.  The
+conserved variables are accumulated in the two-pass update's order (x, then
+y, then z terms of each variable, each term ((0 + lo) - (0 + hi)) / len or
+* (1 / len)), so the results are identical to it.  FP64 carries make_side's
+record per cell; the mixed path the delta form's record (XPart).
+*/
+template <class T> struct APartOf;
+template <> struct APartOf<double> { using type = SideState<double>; };
+template <> struct APartOf<float> { using type = XPart<float>; };
+
+template <class T> __device__ __forceinline__ void apart_make(const double U[8], const Phys& ph,
+	typename APartOf<T>::type& p)
+{
+	if constexpr (std::is_same_v<T, double>) {
+		p.cf = 0;
+		p.stage = hlld_side<double>(U, ph, p.s, p.cf, p.code);
+	} else {
+		hlld_part<float>(U, ph, p);
+	}
+}
+
+__device__ __forceinline__ SideState<double> apart_up(const SideState<double>& r, const SideState<double>& b)
+{
+	return lane_up(r, b);
+}
+
+__device__ __forceinline__ XPart<float> apart_up(const XPart<float>& r, const XPart<float>& b)
+{
+	return lane_up_words(r, b);
+}
+
+/* dt * F of the face between parts L and R in the global frame, zero on
+   error; returns the error and sets the face's signal speed */
+template <class T, int D> __device__ __forceinline__ int apart_face(const typename APartOf<T>::type& L,
+	const typename APartOf<T>::type& R, const Phys& ph, double dt, double Fg[8], double& speed)
+{
+	double F[8];
+	int e;
+	if constexpr (std::is_same_v<T, double>) {
+		double s = 0;
+		if (L.stage | R.stage) e = first_side_error(L.stage, L.code, R.stage, R.code);
+		else e = flux_hlld_face<double>(L.s, R.s, L.cf, R.cf, L.s.u, R.s.u, ph, F, s);
+		speed = s;
+	} else {
+		float s = 0;
+		const int sl = L.sc >> 4, sr = R.sc >> 4;
+		if (sl | sr) e = first_side_error(sl, L.sc & 15, sr, R.sc & 15);
+		else e = flux_hlld_mixed_face(L, R, ph, F, s);
+		speed = s;
+	}
+	if (e) {
+#pragma unroll
+		for (int q = 0; q < 8; q++) Fg[q] = 0;
+		speed = 0;
+		return e;
+	}
+	Fg[RHO] = dt * F[RHO];
+	Fg[NRJ] = dt * F[NRJ];
+	Fg[M0 + D] = dt * F[M0];
+	Fg[M0 + (D + 1) % 3] = dt * F[M1];
+	Fg[M0 + (D + 2) % 3] = dt * F[M2];
+	Fg[B0 + D] = dt * F[B0];
+	Fg[B0 + (D + 1) % 3] = dt * F[B1];
+	Fg[B0 + (D + 2) % 3] = dt * F[B2];
+	return ERR_OK;
+}
+
+/* the two edge-term components of a D-face's flux: x (By, Bz), y (Bz, Bx),
+   z (By, Bx), at emf[2 D] and emf[2 D + 1] */
+template <int D> __device__ __forceinline__ void store_emf(double* __restrict__ emf, size_t n, int c,
+	const double Fg[8])
+{
+	if constexpr (D == 0) {
+		emf[c] = Fg[B1];
+		emf[n + c] = Fg[B2];
+	} else if constexpr (D == 1) {
+		emf[2 * n + c] = Fg[B2];
+		emf[3 * n + c] = Fg[B0];
+	} else {
+		emf[4 * n + c] = Fg[B1];
+		emf[5 * n + c] = Fg[B0];
+	}
+}
+
+constexpr int ACC_KMAX = 32;   /* cells per march of the accumulating x pass, at most */
+
+template <class T, int MINB>
+__global__ __launch_bounds__(BX * BY, MINB) void acc_x_kernel(GridDev g, StateArrays S,
+	double* __restrict__ out, double* __restrict__ emf, const double* __restrict__ dtp,
+	StepStatus* st, unsigned gx, int km, int remap)
+{
+	static_assert(BX == 64, "a wave is one x tile");
+	using P = typename APartOf<T>::type;
+	const unsigned lb = (remap & 1) ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+	const unsigned gt = (unsigned)((g.nz + BY - 1) / BY);
+	const unsigned bx = lb % gx, r = lb / gx;
+	const int kk = (int)(r % gt) * BY + threadIdx.y;
+	const int chunk = (int)(r / gt);
+	const int lane = threadIdx.x;
+	const int tid = lane + BX * threadIdx.y;
+	const int i0 = (int)bx * BX;
+	const int i = (i0 + lane) % g.nx;         /* lanes past the grid wrap */
+	const bool own_x = i0 + lane < g.nx;
+	const int a0 = chunk * km;
+	const int a1 = min(a0 + km, g.ny);
+	const size_t n = g.n;
+	double sp = 0;
+	unsigned zero = 0;
+	/* per wave: the tile's left-edge parts and right-edge face fluxes of
+	   the march's rows, and each lane's face flux (read by the lane below) */
+	__shared__ P edge_l[BY][ACC_KMAX];
+	__shared__ Flux8<double> edge_r[BY][ACC_KMAX];
+	__shared__ double lo_s[8][BX * BY];
+	/* kk, a0 and a1 are the same in every lane of the wave */
+	if (kk < g.nz && a0 < a1) {
+		const double dt = *dtp;
+		const double il = 1 / g.dx;
+		if (lane < a1 - a0) {
+			const int row = a0 + lane;
+			double U[8];
+			load_face_state_d<0>(S, g, wrap_dec(i0, g.nx), row, kk, U);
+			apart_make<T>(U, g.ph, edge_l[threadIdx.y][lane]);
+			if (i0 + BX <= g.nx) {
+				P pa, pb;
+				load_face_state_d<0>(S, g, i0 + BX - 1, row, kk, U);
+				apart_make<T>(U, g.ph, pa);
+				load_face_state_d<0>(S, g, (i0 + BX) % g.nx, row, kk, U);
+				apart_make<T>(U, g.ph, pb);
+				double s_unused;
+				apart_face<T, 0>(pa, pb, g.ph, dt, edge_r[threadIdx.y][lane].f, s_unused);
+			}
+		}
+		__builtin_amdgcn_wave_barrier();
+		for (int a = a0; a < a1; a++) {
+			double s_face;
+			int e;
+			{
+				double U[8];
+				load_face_state_d<0>(S, g, i, a, kk, U);
+				P R;
+				apart_make<T>(U, g.ph, R);
+				const P L = apart_up(R, edge_l[threadIdx.y][a - a0]);
+				double lo[8];
+				e = apart_face<T, 0>(L, R, g.ph, dt, lo, s_face);
+#pragma unroll
+				for (int q = 0; q < 8; q++) lo_s[q][tid] = lo[q];
+			}
+			/* the faces of this lane and of the lane above, from LDS (the
+			   same wave wrote them: LDS operations of a wave complete in
+			   order), and the cell's state again (a cache hit) rather than
+			   held in registers across the face */
+			__asm__ volatile("" ::: "memory");
+			__builtin_amdgcn_wave_barrier();
+			if (own_x) {
+				const int c = i + g.nx * (a + g.ny * kk);
+				double lo[8], hi[8], U[8];
+				const bool last = lane == BX - 1;
+				const int up = last ? tid : tid + 1;
+#pragma unroll
+				for (int q = 0; q < 8; q++) {
+					lo[q] = lo_s[q][tid];
+					hi[q] = last ? edge_r[threadIdx.y][a - a0].f[q] : lo_s[q][up];
+				}
+				if (e) record_error(st, e, (long long)c);
+				sp = smax(sp, s_face);
+				zero += s_face <= 0 ? 1u : 0u;
+				store_emf<0>(emf, n, c, lo);
+				load_face_state_d<0>(S, g, i, a, kk, U);
+				add_terms<double>(U, lo, hi, g.dx, il);
+#pragma unroll
+				for (int q = 0; q < 8; q++) out[q * n + c] = U[q];
+			}
+			__builtin_amdgcn_wave_barrier();
+			__asm__ volatile("" ::: "memory");
+		}
+	}
+	reduce_speed(st, 0, sp, zero);
+}
+
+template <class T, int D, int MINB>
+__global__ __launch_bounds__(BX * BY, MINB) void acc_march_kernel(GridDev g, StateArrays S,
+	double* __restrict__ out, double* __restrict__ emf, const double* __restrict__ dtp,
+	StepStatus* st, unsigned gx, int km, int remap, int refits)
+{
+	static_assert(D == 1 || D == 2, "march along y or z");
+	using P = typename APartOf<T>::type;
+	const unsigned lb = (remap & 1) ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
+	const int nt = D == 1 ? g.nz : g.ny;                 /* lines across */
+	const unsigned gt = (unsigned)((nt + BY - 1) / BY);
+	const unsigned bx = lb % gx, r = lb / gx;
+	const int t = (int)(r % gt) * BY + threadIdx.y;
+	const int chunk = (int)(r / gt);
+	const int i = bx * BX + threadIdx.x;
+	const int nd = D == 1 ? g.ny : g.nz;
+	const int a0 = chunk * km;
+	const int a1 = min(a0 + km, nd);
+	const size_t n = g.n;
+	double sp = 0;
+	unsigned zero = 0;
+	if (i < g.nx && t < nt && a0 < a1) {
+		const int jj = D == 1 ? 0 : t;
+		const int kk = D == 1 ? t : 0;
+		const double dt = *dtp;
+		const double len = D == 1 ? g.dy : g.dz;
+		const double il = 1 / len;
+		auto cell = [&](int a) { return D == 1 ? i + g.nx * (a + g.ny * kk) : i + g.nx * (jj + g.ny * a); };
+		/* the flux of the cell's -D face, kept in LDS across the next face */
+		__shared__ double flo_s[8][BX * BY];
+		const int tid = threadIdx.x + BX * threadIdx.y;
+		P pl, pc;
+		{
+			double U[8];
+			const int am = wrap_dec(a0, nd);
+			load_face_state_d<D>(S, g, i, D == 1 ? am : jj, D == 1 ? kk : am, U);
+			apart_make<T>(U, g.ph, pl);
+			load_face_state_d<D>(S, g, i, D == 1 ? a0 : jj, D == 1 ? kk : a0, U);
+			apart_make<T>(U, g.ph, pc);
+		}
+		double flo[8];
+		{
+			double s_face;
+			const int e = apart_face<T, D>(pl, pc, g.ph, dt, flo, s_face);
+			const int c = cell(a0);
+			if (e) record_error(st, e, (long long)D * g.n + c);
+			sp = smax(sp, s_face);
+			zero += s_face <= 0 ? 1u : 0u;
+			store_emf<D>(emf, n, c, flo);
+#pragma unroll
+			for (int q = 0; q < 8; q++) flo_s[q][tid] = flo[q];
+		}
+		for (int a = a0; a < a1; a++) {
+			const int an = wrap_inc(a, nd);
+			P pn;
+			{
+				double U[8];
+				load_face_state_d<D>(S, g, i, D == 1 ? an : jj, D == 1 ? kk : an, U);
+				apart_make<T>(U, g.ph, pn);
+			}
+			double fhi[8];
+			double s_face;
+			const int e = apart_face<T, D>(pc, pn, g.ph, dt, fhi, s_face);
+			if (a + 1 < a1) {
+				const int cn = cell(a + 1);
+				if (e) record_error(st, e, (long long)D * g.n + cn);
+				sp = smax(sp, s_face);
+				zero += s_face <= 0 ? 1u : 0u;
+				store_emf<D>(emf, n, cn, fhi);
+			}
+			const int c = cell(a);
+			double acc[8], lo[8];
+			/* a real LDS round trip (the compiler would otherwise keep the
+			   carried flux in registers across the face) */
+			__asm__ volatile("" ::: "memory");
+#pragma unroll
+			for (int q = 0; q < 8; q++) {
+				acc[q] = out[q * n + c];
+				lo[q] = flo_s[q][tid];
+			}
+			add_terms<double>(acc, lo, fhi, len, il);
+			if constexpr (D == 1) {
+#pragma unroll
+				for (int q = 0; q < 8; q++) out[q * n + c] = acc[q];
+			} else {
+#pragma unroll
+				for (int q = 0; q < 4; q++) out[q * n + c] = acc[q];
+				/* the thermal pressure the first energy refit recovers
+				   (update_vol_B before the face update), or the energy
+				   itself when no refit will use it */
+				out[4 * n + c] = (refits > 0 && acc[RHO] > 0)
+					? pressure64(acc, g.ph.gm1, g.ph.half_inv_mu0) : acc[NRJ];
+			}
+#pragma unroll
+			for (int q = 0; q < 8; q++) flo_s[q][tid] = fhi[q];
+			pc = pn;
+		}
+	}
+	reduce_speed(st, D, sp, zero);
+}
+
 /* lagged CFL step of minimize_timestep + set_minmax_substepping_period with a
    single substep (source/mhd/solve.hpp:1961-2006, substepping.hpp:303-338) */
 /* one wave: lane s reduces slot s of each dimension (max of the ordered bit
@@ -2325,8 +2624,12 @@ struct pamhd_handle {
 		/* HLLD: 1 = the fused step (every face flux and the conserved
 		   update in one pass, then the face-field update and refits),
 		   0 = flux passes and update */
-		int fused = 1;
-		int fused_kc = 32;     /* planes per chunk of the fused pass (<= FKC_MAX) */
+		/* HLLD step structure: 0 = flux passes and update, 1 = the fused
+		   single pass, 2 = the accumulating marches (each followed by the
+		   face-field update and refits), -1 = by precision (FP64 2, mixed 1) */
+		int fused = -1;
+		int fused_kc = 16;     /* planes per chunk of the fused pass (<= FKC_MAX) */
+		int acc_km = 32;       /* cells per march of the accumulating step */
 	} tune;
 	double* emf = nullptr;     /* 6 x n field components of the fused step, allocated on first use */
 	/* second stream and events of the slab pipeline, created on first use */
@@ -2554,9 +2857,42 @@ int report_status(pamhd_handle* h, const StepStatus& st)
 }
 
 /* the fused step applies to HLLD (which ignores the background field) */
+/* the step structure used for HLLD (which ignores the background field):
+   0 flux passes and update, 1 fused single pass, 2 accumulating marches */
+int hlld_mode(const pamhd_handle* h)
+{
+	if (h->tune.fused >= 0) return h->tune.fused;
+	return h->precision == PAMHD_MIXED ? 1 : 2;
+}
+
 bool fused_step(const pamhd_handle* h, int solver)
 {
-	return solver == HLLD_ATHENA && h->tune.fused;
+	return solver == HLLD_ATHENA && hlld_mode(h) != 0;
+}
+
+/* the face-field update and energy refits after the fused or accumulating
+   passes: reads the old face field from s, rho, m and the recovered
+   pressure from out, the field components from emf; completes out */
+int launch_ct_refit(pamhd_handle* h, hipStream_t st, const double* s, double* out, int refits)
+{
+	TimedLaunch t(h, st, K_UPDATE);
+	const int kz = h->tune.update_kz > 0 ? min(h->tune.update_kz, h->g.nz) : min(8, h->g.nz);
+	const unsigned gzc = (h->g.nz + kz - 1) / kz;
+	auto column = [&](auto tx, auto ty) {
+		constexpr int TX = decltype(tx)::value, TY = decltype(ty)::value;
+		const unsigned cx = (h->g.nx + TX - 1) / TX, cy = (h->g.ny + TY - 1) / TY;
+		ct_refit_kernel<TX, TY><<<dim3(cx * cy * gzc), dim3(TX, TY, 1), 0, st>>>(
+			h->g, s, out, h->emf, refits, kz, cx, cy, h->tune.update_remap & 1);
+	};
+	switch (h->tune.update_tile) {
+	case 1: column(std::integral_constant<int, 128>(), std::integral_constant<int, 2>()); break;
+	case 2: column(std::integral_constant<int, 32>(), std::integral_constant<int, 8>()); break;
+	case 3: column(std::integral_constant<int, 128>(), std::integral_constant<int, 4>()); break;
+	case 4: column(std::integral_constant<int, 64>(), std::integral_constant<int, 8>()); break;
+	default: column(std::integral_constant<int, 64>(), std::integral_constant<int, 4>()); break;
+	}
+	HIP_TRY(h, hipGetLastError());
+	return PAMHD_OK;
 }
 
 constexpr int FNW = 8;   /* waves per block of the fused pass: 6 rows and their two neighbours */
@@ -2584,24 +2920,36 @@ int launch_fused(pamhd_handle* h, hipStream_t st, StepStatus* next, int refits)
 			h->dt, next, gx, gy, kc, refits, h->tune.flux_remap);
 	}
 	HIP_TRY(h, hipGetLastError());
-	TimedLaunch t(h, st, K_UPDATE);
-	const int kz = h->tune.update_kz > 0 ? min(h->tune.update_kz, h->g.nz) : min(8, h->g.nz);
-	const unsigned gzc = (h->g.nz + kz - 1) / kz;
-	auto column = [&](auto tx, auto ty) {
-		constexpr int TX = decltype(tx)::value, TY = decltype(ty)::value;
-		const unsigned cx = (h->g.nx + TX - 1) / TX, cy = (h->g.ny + TY - 1) / TY;
-		ct_refit_kernel<TX, TY><<<dim3(cx * cy * gzc), dim3(TX, TY, 1), 0, st>>>(
-			h->g, s, out, h->emf, refits, kz, cx, cy, h->tune.update_remap & 1);
-	};
-	switch (h->tune.update_tile) {
-	case 1: column(std::integral_constant<int, 128>(), std::integral_constant<int, 2>()); break;
-	case 2: column(std::integral_constant<int, 32>(), std::integral_constant<int, 8>()); break;
-	case 3: column(std::integral_constant<int, 128>(), std::integral_constant<int, 4>()); break;
-	case 4: column(std::integral_constant<int, 64>(), std::integral_constant<int, 8>()); break;
-	default: column(std::integral_constant<int, 64>(), std::integral_constant<int, 4>()); break;
+	return launch_ct_refit(h, st, s, out, refits);
+}
+
+/* the accumulating step: x, y and z marches into state[1 - cur], then the
+   face-field update and refits */
+template <class T>
+int launch_accumulate(pamhd_handle* h, hipStream_t st, StepStatus* next, int refits)
+{
+	const long long n = h->g.n;
+	if (!h->emf) HIP_TRY(h, hipMalloc(&h->emf, sizeof(double) * 6 * n));
+	const double* s = h->state[h->cur];
+	double* out = h->state[1 - h->cur];
+	StateArrays S;
+	for (int q = 0; q < 8; q++) S.q[q] = s + (size_t)q * n;
+	const int km = std::min(std::max(1, h->tune.acc_km), ACC_KMAX);
+	const unsigned gx = (h->g.nx + BX - 1) / BX;
+	const unsigned tz = (h->g.nz + BY - 1) / BY, ty = (h->g.ny + BY - 1) / BY;
+	const unsigned cy = (h->g.ny + km - 1) / km, cz = (h->g.nz + km - 1) / km;
+	const int r = h->tune.flux_remap;
+	const dim3 block(BX, BY, 1);
+	{
+		TimedLaunch t(h, st, K_FLUX);
+		acc_x_kernel<T, 1><<<gx * tz * cy, block, 0, st>>>(h->g, S, out, h->emf, h->dt, next, gx, km, r);
+		acc_march_kernel<T, 1, 3><<<gx * tz * cy, block, 0, st>>>(h->g, S, out, h->emf, h->dt, next,
+			gx, km, r, refits);
+		acc_march_kernel<T, 2, 3><<<gx * ty * cz, block, 0, st>>>(h->g, S, out, h->emf, h->dt, next,
+			gx, km, r, refits);
 	}
 	HIP_TRY(h, hipGetLastError());
-	return PAMHD_OK;
+	return launch_ct_refit(h, st, s, out, refits);
 }
 
 /*
@@ -2615,7 +2963,10 @@ Per-cell arithmetic is unchanged, so results are identical for every S.
 template <class T, class PT>
 int launch_step_body(pamhd_handle* h, int solver, hipStream_t st, StepStatus* next, int refits)
 {
-	if (fused_step(h, solver)) return launch_fused<T>(h, st, next, refits);
+	if (fused_step(h, solver)) {
+		return hlld_mode(h) == 2 ? launch_accumulate<T>(h, st, next, refits)
+			: launch_fused<T>(h, st, next, refits);
+	}
 	int S = h->tune.slabs;
 	if (S > h->g.nz / 2) S = h->g.nz / 2;
 	if (h->timing || h->tune.update_kz > 0) S = 1;
@@ -3811,12 +4162,67 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 		}
 		h->tune.slabs = best_slabs;
 	}
+	/* HLLD: the step structure (flux passes and update, fused single pass,
+	   accumulating marches) and its chunk length, then the tile of the
+	   face-field update that follows the latter two */
+	if (solver == HLLD_ATHENA) {
+		auto step = [&]() {
+			dt_kernel<<<1, NSLOT, 0, st>>>(prev, next, h->dt, 0.0, 0.0, h->g.dx, h->g.dy, h->g.dz, 0);
+			if (h->precision == PAMHD_MIXED) launch_step_body<float, double>(h, solver, st, next, 3);
+			else launch_step_body<double, double>(h, solver, st, next, 3);
+		};
+		report += "; hlld step:";
+		pamhd_handle::Tune best_t = h->tune;
+		best_t.fused = 0;
+		h->tune.fused = 0;
+		float best_ms = time_it(step);
+		report += " passes+update=" + std::to_string(best_ms) + "ms";
+		const int chunks[] = {8, 16, 32};
+		for (int mode = 1; mode <= 2; mode++)
+		for (int c : chunks) {
+			h->tune = best_t;
+			h->tune.fused = mode;
+			if (mode == 1) h->tune.fused_kc = c;
+			else h->tune.acc_km = c;
+			h->tune.update_kz = 8;
+			h->tune.update_tile = 1;
+			const float ms = time_it(step);
+			report += std::string(mode == 1 ? " fused/kc" : " accumulate/km") + std::to_string(c) + "=" +
+				std::to_string(ms) + "ms";
+			if (ms < best_ms) {
+				best_ms = ms;
+				best_t = h->tune;
+			}
+		}
+		h->tune = best_t;
+		if (h->tune.fused) {
+			const int ct_kz[] = {8, 16, 32};
+			for (int kz : ct_kz)
+			for (int tile = 0; tile < 5; tile++) {
+				if (kz > h->g.nz) continue;
+				h->tune = best_t;
+				h->tune.update_kz = kz;
+				h->tune.update_tile = tile;
+				const float ms = time_it(step);
+				report += " ct" + std::to_string(kz) + "/tile" + std::to_string(tile) + "=" +
+					std::to_string(ms) + "ms";
+				if (ms < best_ms) {
+					best_ms = ms;
+					best_t = h->tune;
+				}
+			}
+			h->tune = best_t;
+		}
+	}
 	(void)hipEventDestroy(a);
 	(void)hipEventDestroy(b);
 	HIP_TRY(h, hipGetLastError());
 	HIP_TRY(h, hipStreamSynchronize(st));
 	h->timing = timing;
-	report += "; chosen flux_remap=" + std::to_string(h->tune.flux_remap) +
+	report += "; chosen fused=" + std::to_string(h->tune.fused) +
+		(h->tune.fused == 1 ? "/kc" + std::to_string(h->tune.fused_kc) :
+		 h->tune.fused == 2 ? "/km" + std::to_string(h->tune.acc_km) : std::string()) +
+		" flux_remap=" + std::to_string(h->tune.flux_remap) +
 		" flux_occ4=" + std::to_string(h->tune.flux_occ) +
 		" flux_shared=" + std::to_string(h->tune.flux_shared) +
 		(h->tune.flux_shared >= 2 ? "/km" + std::to_string(h->tune.flux_km) : std::string()) +
@@ -3845,11 +4251,15 @@ int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_
 int pamhd_set_fused_step(pamhd_handle* h, int fused, int planes_per_chunk)
 {
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
-	if (fused != 0 && fused != 1) return fail(h, PAMHD_E_ARG, "fused must be 0 or 1");
-	if (planes_per_chunk < 0 || planes_per_chunk > FKC_MAX)
-		return fail(h, PAMHD_E_ARG, "planes per chunk must be in 0.." + std::to_string(FKC_MAX));
+	if (fused < -1 || fused > 2) return fail(h, PAMHD_E_ARG, "fused must be -1, 0, 1 or 2");
+	const int most = fused == 2 ? ACC_KMAX : FKC_MAX;
+	if (planes_per_chunk < 0 || planes_per_chunk > most)
+		return fail(h, PAMHD_E_ARG, "planes per chunk must be in 0.." + std::to_string(most));
 	h->tune.fused = fused;
-	if (planes_per_chunk > 0) h->tune.fused_kc = planes_per_chunk;
+	if (planes_per_chunk > 0) {
+		if (fused == 2) h->tune.acc_km = planes_per_chunk;
+		else h->tune.fused_kc = planes_per_chunk;
+	}
 	return PAMHD_OK;
 }
 

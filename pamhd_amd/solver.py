@@ -263,11 +263,23 @@ class Grid:
         self._check(self._L.pamhd_adjoint_step(self._h, solver_id(solver), dt, state_ptr,
                                                lambda_ptr, stream))
 
-    def set_fused_step(self, fused=True, planes_per_chunk=0):
-        """HLLD's fused step (default) or the per-direction flux passes and
-        update pass; identical results."""
-        self._check(self._L.pamhd_set_fused_step(self._h, 1 if fused else 0,
-                                                 int(planes_per_chunk)))
+    STEP_MODES = {"auto": -1, "passes": 0, "fused": 1, "accumulate": 2}
+
+    def set_fused_step(self, mode="auto", planes_per_chunk=0):
+        """HLLD's step structure (identical results): "passes" (per-direction
+        flux passes and an update pass), "fused" (one pass, then the
+        face-field update), "accumulate" (per-direction marches that update
+        the state, then the face-field update) or "auto" (accumulate in
+        FP64, fused in mixed precision); True / False select fused /
+        passes.  planes_per_chunk: planes per fused block (1..32) or cells
+        per accumulating march (1..32); 0 keeps the current value."""
+        if mode is True or mode is False:
+            m = 1 if mode else 0
+        elif isinstance(mode, str):
+            m = self.STEP_MODES[mode]
+        else:
+            m = int(mode)
+        self._check(self._L.pamhd_set_fused_step(self._h, m, int(planes_per_chunk)))
 
     def set_slabs(self, slabs):
         """z-slab pipelining of the flux pass and the update (1 = off)."""

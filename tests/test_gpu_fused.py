@@ -1,14 +1,18 @@
-"""HLLD's fused step (every face flux and the conserved update in one pass,
-then the face-field update and energy refits) against the FP64 oracle and
-against the per-direction flux passes and update pass, through the C ABI.
+"""HLLD's step structures against the FP64 oracle and against each other,
+through the C ABI: the per-direction flux passes and update pass
+("passes"), the fused single pass ("fused": every face flux and the
+conserved update in one pass) and the accumulating marches ("accumulate":
+x, y and z marches each adding their faces' flux differences to the new
+state), the latter two followed by the face-field update and energy refits.
 
 The fused pass tiles x by 64 cells, y by 6 rows with a row on each side and
-z by chunks of planes, with every index wrapping periodically, so the grid
-shapes below cover tiles cut short in each dimension, grids smaller than one
-tile (lanes and waves wrapping several times), thin and 2-D grids, and
-chunks of 1 to 32 planes.  FP64 results must be bit-identical; the mixed
-path must equal the two-pass mixed path bit for bit too (the same FP32
-face arithmetic and FP64 accumulation in the same order).
+z by chunks of planes; the accumulating marches tile x by 64 lanes and walk
+km cells along y or z; every index wraps periodically, so the grid shapes
+below cover tiles cut short in each dimension, grids smaller than one tile
+(lanes and waves wrapping several times), thin and 2-D grids, and chunks of
+1 to 32 planes.  FP64 results must be bit-identical to the oracle; the
+mixed path must give the same bits in every structure (the same FP32 face
+arithmetic and FP64 accumulation in the same order).
 """
 import numpy as np
 import pytest
@@ -41,26 +45,29 @@ SHAPES = [(64, 12, 10), (70, 13, 33), (130, 7, 5), (16, 12, 10), (5, 3, 4),
 
 
 @pytest.mark.parametrize("cells", SHAPES)
-def test_fused_fp64_bit_exact_vs_oracle(gpu, cells):
-    print("cells", cells, flush=True)
+@pytest.mark.parametrize("mode", ["fused", "accumulate"])
+def test_fused_fp64_bit_exact_vs_oracle(gpu, cells, mode):
+    print("cells", cells, mode, flush=True)
     state, length, gamma, mu0 = _state(cells, 5)
     ref = state.copy()
     og = O.make_grid(cells, length, gamma, mu0)
     st, steps_o, t_o = O.run_periodic(og, O.HLLD, 0.3, 0.0, 1e9, ref, max_steps=4)
     assert st == 0
-    out, n, t, _, fc = _run(cells, 4, True)
+    out, n, t, _, fc = _run(cells, 4, mode)
     assert n == steps_o == 4 and t == t_o
     np.testing.assert_array_equal(out, ref)
     # every face once per step (timestep()'s flux_calcs)
     assert fc == O.last_flux_calcs() == 3 * cells[0] * cells[1] * cells[2]
 
 
-@pytest.mark.parametrize("kc", [1, 3, 7, 32])
-def test_fused_chunks_match_two_pass(gpu, kc):
+@pytest.mark.parametrize("mode,kc", [("fused", 1), ("fused", 3), ("fused", 7), ("fused", 32),
+                                     ("accumulate", 1), ("accumulate", 3), ("accumulate", 7),
+                                     ("accumulate", 32)])
+def test_fused_chunks_match_two_pass(gpu, mode, kc):
     cells = (66, 14, 37)
-    print("cells", cells, "planes per chunk", kc, flush=True)
-    a, na, ta, sa, _ = _run(cells, 5, True, kc)
-    b, nb, tb, sb, _ = _run(cells, 5, False)
+    print("cells", cells, mode, "planes per chunk", kc, flush=True)
+    a, na, ta, sa, _ = _run(cells, 5, mode, kc)
+    b, nb, tb, sb, _ = _run(cells, 5, "passes")
     assert na == nb and ta == tb
     np.testing.assert_array_equal(a, b)
     np.testing.assert_array_equal(sa, sb)
@@ -69,17 +76,17 @@ def test_fused_chunks_match_two_pass(gpu, kc):
 @pytest.mark.parametrize("cells", [(64, 12, 10), (70, 13, 33)])
 def test_fused_mixed_matches_two_pass_mixed(gpu, cells):
     print("cells", cells, flush=True)
-    a, na, ta, _, _ = _run(cells, 6, True, precision="mixed")
-    b, nb, tb, _, _ = _run(cells, 6, False, precision="mixed")
-    assert na == nb and ta == tb
-    np.testing.assert_array_equal(a, b)
+    b, nb, tb, _, _ = _run(cells, 6, "passes", precision="mixed")
+    for mode in ("fused", "accumulate"):
+        a, na, ta, _, _ = _run(cells, 6, mode, precision="mixed")
+        assert na == nb and ta == tb
+        np.testing.assert_array_equal(a, b)
 
 
 def test_fused_reports_the_failing_face(gpu):
-    """A cell of negative thermal pressure: the fused step fails with the
-    pressure error on a face of that cell, as the two-pass step does (which
-    of the cell's faces is reported first depends on the order the faces
-    are computed in)."""
+    """A cell of negative thermal pressure: every step structure fails with
+    the pressure error on a face of that cell (which of the cell's faces is
+    reported first depends on the order the faces are computed in)."""
     from pamhd_amd import _lib
     from pamhd_amd import solver as S
     cells = (64, 12, 10)
@@ -87,14 +94,14 @@ def test_fused_reports_the_failing_face(gpu):
     i, j, k = 5, 7, 4
     c = i + 64 * (j + 12 * k)
     state[4, c] = 0.0                 # E = 0: negative thermal pressure
-    for fused in (True, False):
+    for mode in ("fused", "accumulate", "passes"):
         with S.Grid(cells, length, gamma, mu0) as g:
-            g.set_fused_step(fused)
+            g.set_fused_step(mode)
             g.upload(state)
             with pytest.raises(_lib.PamhdError) as ei:
                 g.run("hlld-athena", 0.3, 0.0, 1e9, max_steps=2)
         msg = str(ei.value)
-        print("fused" if fused else "two-pass", msg, flush=True)
+        print(mode, msg, flush=True)
         assert ei.value.code == 2
         named = [(i, j, k), (i + 1, j, k), (i, j + 1, k), (i, j, k + 1)]
         assert any("(%d,%d,%d)" % p in msg for p in named)
