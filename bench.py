@@ -200,10 +200,15 @@ def substep_run(size=256, steps=10, solver="hlld-athena", cfl=0.3):
         ms = start.elapsed_time(stop) / steps
         g.check()
         largest = g.last_max_substep()
+        periods = g.last_substep_periods()
     del state
+    import numpy as np
+    vals, counts = np.unique(periods, return_counts=True)
     return {"cells": list(cells), "solver": solver, "cfl": cfl, "substep_min": 0,
             "substep_max": 999, "steps": steps, "ms_per_step": round(ms, 4),
             "cell_updates_per_s": n / (ms / 1e3), "largest_period": largest,
+            "mean_period": float(periods.mean()),
+            "period_share": {str(int(v)): float(c) / n for v, c in zip(vals, counts)},
             "workload": "periodic box at rest, uniform field, hot sphere of 30x the pressure "
                         "(radius 0.25), +-1% density noise"}
 
@@ -457,16 +462,31 @@ def precision_study(size=512, steps=160, clean_every=10, report_every=20, solver
 
 
 # HBM traffic of one step (FETCH_SIZE x 2.00 + WRITE_SIZE x 1.00 summed over
-# the step's launches) measured with rocprofv3 --pmc for the launch variants
-# the autotuner picked: profiles/step_kernels_256cubed_hlld_counters.md
-STEP_TRAFFIC_BYTES = {("fp64", 256, "hlld-athena"): 14.34e9, ("mixed", 256, "hlld-athena"): 10.13e9}
-# VALU wave instructions of one step's three flux launches (SQ_INSTS_VALU of
-# the same profiled runs, the launch variants the autotuner picked; the flux
-# passes are VALU-issue bound)
-FLUX_VALU_WAVE_INSTR = {("fp64", 256, "hlld-athena"): 7.47e8, ("mixed", 256, "hlld-athena"): 4.08e8}
+# the step's launches) measured with rocprofv3 --pmc for the step structure
+# and launch variants the autotuner picks (keyed by precision, size, solver
+# and HLLD step structure): profiles/step_kernels_256cubed_hlld_counters.md
+STEP_TRAFFIC_BYTES = {("fp64", 256, "hlld-athena", "accumulate"): 13.73e9,
+                      ("mixed", 256, "hlld-athena", "accumulate"): 14.55e9,
+                      ("fp64", 256, "hlld-athena", "passes"): 14.34e9}
+# VALU wave instructions of one step's flux launches (SQ_INSTS_VALU of the
+# same profiled runs; the flux launches are VALU-issue bound)
+FLUX_VALU_WAVE_INSTR = {("fp64", 256, "hlld-athena", "accumulate"): 8.57e8,
+                        ("mixed", 256, "hlld-athena", "accumulate"): 5.90e8,
+                        ("fp64", 256, "hlld-athena", "passes"): 7.47e8}
+STRUCTURES = {"0": "passes", "1": "fused", "2": "accumulate"}
 # one wave64 VALU instruction per clock per CU: 256 CUs x 2.4 GHz (the
 # 78.6 TFLOP/s FP64 vector peak is 128 FLOP, one 64-lane FMA, per clock per CU)
 VALU_PEAK_WAVE_INSTR_PER_S = 256 * 2.4e9
+
+
+def structure_of(tuning):
+    """the HLLD step structure the autotuner chose ("passes", "fused",
+    "accumulate"), from its report's "chosen fused=M..." entry"""
+    chosen = tuning.split("; chosen ")[-1]
+    for tok in chosen.split():
+        if tok.startswith("fused="):
+            return STRUCTURES.get(tok.split("=")[1].split("/")[0], "passes")
+    return "passes"
 
 
 def roofline(res, n, steps, key=None):
@@ -486,7 +506,10 @@ def roofline(res, n, steps, key=None):
                 "peak": VALU_PEAK_WAVE_INSTR_PER_S, "unit": "wave instructions/s",
                 "frac": round(rate / VALU_PEAK_WAVE_INSTR_PER_S, 4),
                 "instructions": FLUX_VALU_WAVE_INSTR[key]}
-    return {"bound": "hbm", "kernel": "step (flux_pass + update)",
+    kernel = {"passes": "step (flux passes + update)",
+              "fused": "step (fused flux/update pass + face-field update)",
+              "accumulate": "step (x, y, z accumulating marches + face-field update)"}
+    return {"bound": "hbm", "kernel": kernel.get(key[3] if key else "passes"),
             "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": STEP_TRAFFIC_BYTES.get(key),
             "traffic_unit": "bytes per step (rocprofv3 FETCH_SIZE/WRITE_SIZE, copy-calibrated)",
@@ -515,6 +538,8 @@ def main():
                     help="timed FP64 steps of Rusanov, HLL and Roe on the same workload (0: skip)")
     ap.add_argument("--substep-steps", type=int, default=10,
                     help="timed steps of the per-cell substepping record (0: skip)")
+    ap.add_argument("--no-cleaning", action="store_true",
+                    help="skip the divergence-cleaning record")
     ap.add_argument("--ensemble-boxes", type=int, default=64)
     ap.add_argument("--ensemble-steps", type=int, default=500,
                     help="0 skips the ensemble workload")
@@ -577,9 +602,11 @@ def main():
     del state
     err = rel_l2(results["mixed"].pop("final"), results["fp64"].pop("final"))
     log("  mixed vs fp64 relative L2 after %d steps: %s" % (args.warmup + args.steps, err))
-    log("divergence cleaning, %d^3" % args.size)
-    clean = divergence_cleaning(cells)
-    log("  %s" % clean)
+    clean = None
+    if not args.no_cleaning:
+        log("divergence cleaning, %d^3" % args.size)
+        clean = divergence_cleaning(cells)
+        log("  %s" % clean)
 
     sub = None
     if args.substep_steps > 0:
@@ -644,13 +671,15 @@ def main():
                                % (args.size, args.solver, args.cfl),
                    "cells": list(cells), "solver": args.solver, "cfl": args.cfl,
                    "gamma": gamma, "precision": "fp64"},
-        "roofline": roofline(f64, n, args.steps, ("fp64", args.size, args.solver)),
+        "roofline": roofline(f64, n, args.steps, ("fp64", args.size, args.solver,
+                                                  structure_of(f64["tuning"]))),
         "cpu_baseline": base,
         "mixed": {"value": mix["value"], "ms_per_step": mix["ms"] / args.steps,
                   "dtype": "f32 flux/wave speeds, f64 state",
                   "speedup_vs_fp64": mix["value"] / f64["value"],
                   "rel_l2_vs_fp64": err,
-                  "roofline": roofline(mix, n, args.steps, ("mixed", args.size, args.solver))},
+                  "roofline": roofline(mix, n, args.steps, ("mixed", args.size, args.solver,
+                                                            structure_of(mix["tuning"])))},
         "other_solvers_fp64": others,
         "substepping": sub,
         "divergence_cleaning": clean,

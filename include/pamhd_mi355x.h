@@ -290,6 +290,12 @@ int pamhd_set_boundary_values(pamhd_handle* h, const unsigned char* mask,
    value (1 without substepping). */
 int pamhd_last_max_substep(pamhd_handle* h, int* max_substep);
 
+/* Each cell's substep period (2^level) of the last substepped step, n ints
+   on the host (the reference's per-cell Substep variable,
+   source/substepping.hpp:54-96); 1 everywhere without substepping.
+   Synchronous. */
+int pamhd_last_substep_periods(pamhd_handle* h, int* periods);
+
 /* Faces solved by the last pamhd_timestep / pamhd_step (both cells of type
    >= 0; with substepping each face once per substep it is due):
    timestep()'s third return value flux_calcs (source/mhd/solve.hpp:1918,
@@ -391,10 +397,9 @@ int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_
    place (no flux array); 1 and 2 keep only the two field components of
    each face the constrained-transport edge terms use (6 doubles per cell)
    and finish with a pass that updates the face field and refits the
-   energy.  -1 (the default) picks 2 in FP64 and 1 in mixed precision;
-   pamhd_autotune times them all.  planes_per_chunk (0 = keep) is the
-   z-extent of mode 1's blocks (1..32) or the cells per march of mode 2
-   (1..32). */
+   energy.  -1 (the default) picks 2; pamhd_autotune times them all.
+   planes_per_chunk (0 = keep) is the z-extent of mode 1's blocks (1..32)
+   or the cells per march of mode 2 (1..32). */
 int pamhd_set_fused_step(pamhd_handle* h, int fused, int planes_per_chunk);
 
 /* Pipelines the flux pass and the per-cell update over `slabs` z-slabs on

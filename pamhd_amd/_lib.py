@@ -36,7 +36,7 @@ EXPORTS = (
     "pamhd_ensemble_set_tile", "pamhd_ensemble_autotune", "pamhd_ensemble_last_error",
     "pamhd_step_fixed", "pamhd_set_state_device", "pamhd_get_state_device",
     "pamhd_adjoint_step", "pamhd_set_slabs", "pamhd_set_fused_step", "pamhd_set_substepping", "pamhd_set_substepping_cells",
-    "pamhd_last_max_substep", "pamhd_last_flux_calcs",
+    "pamhd_last_max_substep", "pamhd_last_flux_calcs", "pamhd_last_substep_periods",
 )
 
 
@@ -138,6 +138,7 @@ def load():
     L.pamhd_set_substepping.argtypes = [H, ctypes.c_int, ctypes.c_int]
     L.pamhd_set_substepping_cells.argtypes = [H, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int)]
     L.pamhd_last_max_substep.argtypes = [H, ctypes.POINTER(ctypes.c_int)]
+    L.pamhd_last_substep_periods.argtypes = [H, _vp]
     L.pamhd_last_flux_calcs.argtypes = [H, ctypes.POINTER(ctypes.c_ulonglong)]
     L.pamhd_set_slabs.argtypes = [H, ctypes.c_int]
     L.pamhd_set_fused_step.argtypes = [H, ctypes.c_int, ctypes.c_int]

@@ -1161,7 +1161,7 @@ template <class T, int D> __device__ __forceinline__ int fused_face(const XPart<
 	const int sl = L.sc >> 4, sr = R.sc >> 4;
 	int e;
 	double F[8];
-	T sT = 0;
+	double sT = 0;
 	if (sl | sr) {
 		e = first_side_error(sl, L.sc & 15, sr, R.sc & 15);
 	} else if constexpr (std::is_same_v<T, float>) {
@@ -1645,7 +1645,7 @@ template <class T, int D> __device__ __forceinline__ int apart_face(const typena
 		else e = flux_hlld_face<double>(L.s, R.s, L.cf, R.cf, L.s.u, R.s.u, ph, F, s);
 		speed = s;
 	} else {
-		float s = 0;
+		double s = 0;
 		const int sl = L.sc >> 4, sr = R.sc >> 4;
 		if (sl | sr) e = first_side_error(sl, L.sc & 15, sr, R.sc & 15);
 		else e = flux_hlld_mixed_face(L, R, ph, F, s);
@@ -2177,11 +2177,12 @@ template <class F> __device__ __forceinline__ void for_face_edge_neighbours(cons
 	}
 }
 
-/* the step's four reductions (0: smallest cell time step, 1: largest
-   period, 2: smallest time step / 2^substep-min, 3: largest level) are
-   spread over NSLOT slots each, a wave's atomic going to the slot of its
-   block (one address per reduction serialises 2^18 atomics at 256^3), and
-   folded into slot 0 by ss_fold_kernel before they are read */
+/* the step's five reductions (0: smallest cell time step, 1: largest
+   period, 2: smallest time step / 2^substep-min, 3: largest level, 4: faces
+   solved) are spread over NSLOT slots each, a wave's atomic going to the
+   slot of its block (one address per reduction serialises 2^18 atomics at
+   256^3), and folded by ss_fold_kernel / ss_count_kernel before they are
+   read */
 __device__ __forceinline__ unsigned ss_slot()
 {
 	return (blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) % NSLOT;
@@ -2196,11 +2197,21 @@ __global__ void ss_clear_kernel(SubstepDev d)
 		d.scal[2 * NSLOT + s] = big;
 		d.scal[3 * NSLOT + s] = 0;
 	}
+	for (int s = threadIdx.x; s < NSLOT; s += blockDim.x) d.scal[4 * NSLOT + s] = 0;
 	if (threadIdx.x == 0) {
 		d.status->err = 0;
 		d.status->err_face = -1;
 		d.status->flux_calcs = 0;
 	}
+}
+
+/* the faces solved over the step: the fifth reduction's slots summed into
+   the status record */
+__global__ __launch_bounds__(NSLOT) void ss_count_kernel(SubstepDev d)
+{
+	unsigned long long v = d.scal[4 * NSLOT + threadIdx.x];
+	for (int off = NSLOT / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+	if (threadIdx.x == 0) d.status->flux_calcs = v;
 }
 
 /* reduction q's NSLOT slots into slot 0 (one wave): min when is_min (the
@@ -2370,12 +2381,13 @@ __global__ __launch_bounds__(BX * BY) void ss_flux_kernel(GridDev g, SubstepDev 
 	   owner at the pace of the owner's smallest neighbourhood period */
 	const int o = (oi < 0 || oj < 0 || ok < 0) ? -1 : cidx(g, oi, oj, ok);
 	{
-		/* faces solved, counted per wave (timestep()'s flux_calcs) */
+		/* faces solved (timestep()'s flux_calcs), counted per wave into the
+		   block's slot of the fifth reduction */
 		const bool solved = o >= 0 && ss_active(d, o) && ss_active(d, c) && sub % d.minsub[o] == 0;
 		const unsigned long long b = __ballot(solved);
 		const unsigned long long act = __ballot(1);
 		if ((int)(threadIdx.x + BX * threadIdx.y) % 64 == __ffsll((long long)act) - 1 && b)
-			atomicAdd(&d.status->flux_calcs, (unsigned long long)__popcll(b));
+			atomicAdd(&d.scal[4 * NSLOT + ss_slot()], (unsigned long long)__popcll(b));
 	}
 	if (o < 0 || !ss_active(d, o) || !ss_active(d, c)) {
 		for (int q = 0; q < 8; q++) Pd[q * n + c] = 0;
@@ -2626,7 +2638,7 @@ struct pamhd_handle {
 		   0 = flux passes and update */
 		/* HLLD step structure: 0 = flux passes and update, 1 = the fused
 		   single pass, 2 = the accumulating marches (each followed by the
-		   face-field update and refits), -1 = by precision (FP64 2, mixed 1) */
+		   face-field update and refits), -1 = the default (2) */
 		int fused = -1;
 		int fused_kc = 16;     /* planes per chunk of the fused pass (<= FKC_MAX) */
 		int acc_km = 32;       /* cells per march of the accumulating step */
@@ -2861,8 +2873,7 @@ int report_status(pamhd_handle* h, const StepStatus& st)
    0 flux passes and update, 1 fused single pass, 2 accumulating marches */
 int hlld_mode(const pamhd_handle* h)
 {
-	if (h->tune.fused >= 0) return h->tune.fused;
-	return h->precision == PAMHD_MIXED ? 1 : 2;
+	return h->tune.fused >= 0 ? h->tune.fused : 2;
 }
 
 bool fused_step(const pamhd_handle* h, int solver)
@@ -3255,7 +3266,7 @@ int substep_alloc(pamhd_handle* h)
 	al(&d->period, n * sizeof(int));
 	al(&d->minsub, n * sizeof(int));
 	al(&d->opt_dev, 2 * n * sizeof(int));
-	al(&d->scal, 4 * NSLOT * sizeof(unsigned long long));
+	al(&d->scal, 5 * NSLOT * sizeof(unsigned long long));
 	al(&d->sub_dt, sizeof(double));
 	al(&d->status, sizeof(StepStatus));
 	if (e == hipSuccess) e = hipHostMalloc((void**)&d->host, 3 * sizeof(unsigned long long));
@@ -3359,6 +3370,7 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 	ss_sync_kernel<1><<<grid, block, 0, st>>>(h->g, d);
 	ss_sync_kernel<2><<<grid, block, 0, st>>>(h->g, d);
 	ss_final_kernel<<<grid, block, 0, st>>>(h->g, d, h->state[h->cur]);
+	ss_count_kernel<<<1, NSLOT, 0, st>>>(d);
 	if (refits == 3) {
 		if (d.bounded) launch_boundaries(h, st);
 		else refit_kernel<<<grid, block, 0, st>>>(h->g, h->state[h->cur]);
@@ -3972,6 +3984,18 @@ int pamhd_last_max_substep(pamhd_handle* h, int* max_substep)
 {
 	if (!h || !max_substep) return fail(h, PAMHD_E_ARG, "null argument");
 	*max_substep = h->sub ? h->sub->last_max : 1;
+	return PAMHD_OK;
+}
+
+int pamhd_last_substep_periods(pamhd_handle* h, int* periods)
+{
+	if (!h || !periods) return fail(h, PAMHD_E_ARG, "null argument");
+	if (!h->sub || !h->sub->period) {
+		for (long long c = 0; c < h->g.n; c++) periods[c] = 1;
+		return PAMHD_OK;
+	}
+	HIP_TRY(h, hipDeviceSynchronize());
+	HIP_TRY(h, hipMemcpy(periods, h->sub->period, sizeof(int) * (size_t)h->g.n, hipMemcpyDeviceToHost));
 	return PAMHD_OK;
 }
 

@@ -577,22 +577,92 @@ template <class T> struct XPart {
 	double u[8];
 	double p64;
 	double ir64;   /* 1 / rho in FP64 (mixed path) */
-	T cf;
+	double cf;     /* fast speed (the mixed path's to ~1e-14 of the FP64 one) */
 	int sc;        /* stage * 16 + code */
 };
+
+/* sqrt(x) within ~1e-14 relative: FP32 reciprocal square root refined by
+   one Newton step in FP64 (the error squares: (1e-7)^2), for x in FP32's
+   normal range; IEEE FP64 square root outside it */
+__device__ __forceinline__ double sqrt_refined(double x)
+{
+	if (!(x > 1e-30 && x < 1e30)) return __builtin_sqrt(x);
+	const double y0 = (double)__builtin_amdgcn_rsqf((float)x);
+	const double y1 = y0 * (1.5 - (0.5 * x) * (y0 * y0));
+	return x * y1;
+}
+
+/*
+Fast magnetosonic speed of the mixed path in FP64 without an FP64 division
+or square root sequence: c_s^2 = gamma p / rho, v_A^2 = |B|^2 / (mu0 rho),
+v_An^2 = B_n^2 / (mu0 rho) with the cell's FP64 1 / rho, and c_f^2 =
+(c_s^2 + v_A^2 + sqrt((c_s^2 + v_A^2)^2 - 4 c_s^2 v_An^2)) / 2, the
+quantity of get_fast_magnetosonic_speed (source/mhd/common.hpp:415-491)
+with its checks; square roots by sqrt_refined.  Accurate wave speeds keep
+the mixed path's CFL time step equal to the FP64 one to ~1e-14 and its
+flux error proportional to the jumps.
+*/
+__device__ __forceinline__ int fast_speed_mixed(const double U[8], double p64, double ir64,
+	const Phys& ph, double& cf)
+{
+	if (!__builtin_isfinite(U[B0])) return ERR_FIELD;
+	if (!__builtin_isnormal(U[RHO]) || U[RHO] < 0) return ERR_DENSITY;
+	if (!__builtin_isnormal(p64) || p64 < 0) return ERR_PRESSURE;
+	const double im = ir64 * ph.inv_mu0;
+	const double cs2 = ph.gamma * p64 * ir64;
+	const double van2 = U[B0] * U[B0] * im;
+	const double va2 = (U[B0] * U[B0] + U[B1] * U[B1] + U[B2] * U[B2]) * im;
+	if (!(va2 > 0)) {
+		cf = sqrt_refined(cs2);
+		return ERR_OK;
+	}
+	const double sum = cs2 + va2;
+	if (!__builtin_isnormal(sum) || sum < 0) return ERR_SPEED;
+	const double disc = sum * sum - 4.0 * cs2 * van2;
+	if (!__builtin_isnormal(disc) || disc < 0) return ERR_SPEED;
+	cf = sqrt_refined(0.5 * (sum + sqrt_refined(disc)));
+	return ERR_OK;
+}
 
 template <class T> __device__ __forceinline__ void hlld_part(const double U[8], const Phys& ph,
 	XPart<T>& x)
 {
-	Side<T> s;
-	T cf = 0;
-	int code;
-	const int stage = hlld_side<T>(U, ph, s, cf, code);
 #pragma unroll
 	for (int q = 0; q < 8; q++) x.u[q] = U[q];
-	x.p64 = stage == 0 ? s.p64 : 0.0;
-	x.ir64 = (std::is_same_v<T, float> && stage == 0) ? 1 / U[RHO] : 0.0;
-	x.cf = cf;
+	int stage, code = ERR_OK;
+	if constexpr (std::is_same_v<T, float>) {
+		/* hlld_side's checks in its order: density, pressure, fast speed */
+		x.p64 = 0.0;
+		x.ir64 = 0.0;
+		x.cf = 0.0;
+		if (U[RHO] <= 0) {
+			stage = 1;
+			code = ERR_DENSITY;
+		} else {
+			const double p = pressure64(U, ph.gm1, ph.half_inv_mu0);
+			if (p <= 0) {
+				stage = 2;
+				code = ERR_PRESSURE;
+			} else {
+				const double ir = 1 / U[RHO];
+				double cf = 0;
+				code = fast_speed_mixed(U, p, ir, ph, cf);
+				stage = code ? 3 : 0;
+				if (!stage) {
+					x.p64 = p;
+					x.ir64 = ir;
+					x.cf = cf;
+				}
+			}
+		}
+	} else {
+		Side<T> s;
+		T cf = 0;
+		stage = hlld_side<T>(U, ph, s, cf, code);
+		x.p64 = stage == 0 ? s.p64 : 0.0;
+		x.ir64 = 0.0;
+		x.cf = cf;
+	}
 	x.sc = stage * 16 + code;
 }
 
@@ -703,7 +773,7 @@ cases as the FP64 solver; returns FP64 fluxes.  Both parts must have
 passed hlld_side's checks.
 */
 __device__ __forceinline__ int flux_hlld_mixed_face(const XPart<float>& L, const XPart<float>& R,
-	const Phys& ph, double F[8], float& s_out)
+	const Phys& ph, double F[8], double& s_out)
 {
 	using A = Arith<float>;
 	if (!__builtin_isnormal(L.u[RHO]) || L.u[RHO] <= 0) return ERR_DENSITY;
@@ -720,31 +790,35 @@ __device__ __forceinline__ int flux_hlld_mixed_face(const XPart<float>& L, const
 	physical_flux64(R.u, vR64, ptR64, inv64, FR);
 	if (!__builtin_isfinite(FL[RHO]) || !__builtin_isfinite(FR[RHO])) return ERR_FLUX;
 
-	const float inv = (float)ph.inv_mu0, inv_sqrt = (float)ph.inv_sqrt_mu0;
-	const float ms = smax(L.cf, R.cf);
-	const float vL = (float)vL64[0], vR = (float)vR64[0];
-	const float dv = (float)(vR64[0] - vL64[0]);
+	/* outer wave speeds in FP64 (accurate fast speeds): the CFL speed and
+	   the supersonic tests of the FP64 solver */
+	const double ms = L.cf > R.cf ? L.cf : R.cf;
+	const double dv64 = vR64[0] - vL64[0];
 	const bool lo = vL64[0] <= vR64[0];
-	const float
-		msn = lo ? vL - ms : vR - ms,
-		msp = lo ? vR + ms : vL + ms;
-	if (msn >= 0.0f) {
+	const double
+		msn = lo ? vL64[0] - ms : vR64[0] - ms,
+		msp = lo ? vR64[0] + ms : vL64[0] + ms;
+	if (msn >= 0.0) {
 		for (int q = 0; q < 8; q++) F[q] = FL[q];
-		s_out = fabsf(msp);
+		s_out = fabs(msp);
 		return ERR_OK;
-	} else if (msp <= 0.0f) {
+	} else if (msp <= 0.0) {
 		for (int q = 0; q < 8; q++) F[q] = FR[q];
-		s_out = fabsf(msn);
+		s_out = fabs(msn);
 		return ERR_OK;
 	}
-	/* S_L - u_L and S_R - u_R from the FP64 velocity jump */
-	const float sfdn = lo ? -ms : dv - ms, sfdp = lo ? ms : ms - dv;
+	const float inv = (float)ph.inv_mu0, inv_sqrt = (float)ph.inv_sqrt_mu0;
+	const float vL = (float)vL64[0];
+	const float dv = (float)dv64;
+	/* S_L - u_L and S_R - u_R */
+	const float sfdn = (float)(lo ? -ms : dv64 - ms), sfdp = (float)(lo ? ms : ms - dv64);
 	const float rhoL = (float)L.u[RHO], rhoR = (float)R.u[RHO];
 	const float dpt = (float)(ptR64 - ptL64);
 	/* S_M - u_L and S_M - u_R (eqn 38 of the method, differenced) */
 	const float iden = A::rcp(sfdp * rhoR - sfdn * rhoL);
 	const float delL = (sfdp * rhoR * dv - dpt) * iden, delR = (sfdn * rhoL * dv - dpt) * iden;
 	const float sm = vL + delL;
+	const double sm64 = vL64[0] + (double)delL;
 	const float smdn = sfdn - delL, smdp = sfdp - delR;
 	const double Bx64 = (L.u[B0] + R.u[B0]) * 0.5;
 	const float Bx = (float)Bx64, Bx2 = Bx * Bx;
@@ -753,7 +827,9 @@ __device__ __forceinline__ int flux_hlld_mixed_face(const XPart<float>& L, const
 	mixed_star(L, vL64, ptL64, sfdn, smdn, delL, sm, Bx, (float)(Bx64 - L.u[B0]), inv, ptst, Sn);
 	mixed_star(R, vR64, ptR64, sfdp, smdp, delR, sm, Bx, (float)(Bx64 - R.u[B0]), inv, ptst, Sp);
 	const float dssn = A::sqrt(Sn.rho_s), dssp = A::sqrt(Sp.rho_s);
-	const float ssn = sm - fabsf(Bx) * inv_sqrt * A::rcp(dssn), ssp = sm + fabsf(Bx) * inv_sqrt * A::rcp(dssp);
+	const double
+		ssn = sm64 - (double)(fabsf(Bx) * inv_sqrt * A::rcp(dssn)),
+		ssp = sm64 + (double)(fabsf(Bx) * inv_sqrt * A::rcp(dssp));
 
 	/* double-star states minus single-star states, eqns 59-63; they are
 	   used where the FP64 solver uses them (its threshold: with a weak
@@ -784,23 +860,25 @@ __device__ __forceinline__ int flux_hlld_mixed_face(const XPart<float>& L, const
 		D2n[NRJ] = -dssn * sgn * vdn * inv_sqrt;
 		D2p[NRJ] = dssp * sgn * vdp * inv_sqrt;
 	}
-	if (ssn >= 0.0f) {
-		for (int q = 0; q < 8; q++) F[q] = FL[q] + (double)(Sn.dU[q] * msn);
-	} else if (sm >= 0.0f) {
-		for (int q = 0; q < 8; q++) F[q] = FL[q] + (double)(D2n[q] * ssn + Sn.dU[q] * msn);
-	} else if (ssp > 0.0f) {
-		for (int q = 0; q < 8; q++) F[q] = FR[q] + (double)(D2p[q] * ssp + Sp.dU[q] * msp);
+	/* F = F_L + S_L (U*_L - U_L) [+ S*_L (U**_L - U*_L)] or the mirror form,
+	   the FP32 differences scaled by FP64 wave speeds */
+	if (ssn >= 0.0) {
+		for (int q = 0; q < 8; q++) F[q] = FL[q] + (double)Sn.dU[q] * msn;
+	} else if (sm64 >= 0.0) {
+		for (int q = 0; q < 8; q++) F[q] = FL[q] + ((double)D2n[q] * ssn + (double)Sn.dU[q] * msn);
+	} else if (ssp > 0.0) {
+		for (int q = 0; q < 8; q++) F[q] = FR[q] + ((double)D2p[q] * ssp + (double)Sp.dU[q] * msp);
 	} else {
-		for (int q = 0; q < 8; q++) F[q] = FR[q] + (double)(Sp.dU[q] * msp);
+		for (int q = 0; q < 8; q++) F[q] = FR[q] + (double)Sp.dU[q] * msp;
 	}
 	F[B0] = 0;
-	s_out = smax(fabsf(msn), fabsf(msp));
+	s_out = fabs(msn) > fabs(msp) ? fabs(msn) : fabs(msp);
 	return ERR_OK;
 }
 
 /* the mixed HLLD of one face from the two states */
 __device__ __forceinline__ int flux_hlld_mixed(const double Ld[8], const double Rd[8], const Phys& ph,
-	double F[8], float& s_out)
+	double F[8], double& s_out)
 {
 	XPart<float> L, R;
 	hlld_part<float>(Ld, ph, L);
@@ -864,8 +942,12 @@ template <class T> __device__ __forceinline__ int flux_hlld_face(const Side<T>& 
 		dssn = A::sqrt(dsn),
 		dsp = A::div(R.u[RHO] * sfdp, smdp),
 		dssp = A::sqrt(dsp),
-		ssn = sm - A::div(fabs(Bx), A::sqrt(dsn * mu0)),
-		ssp = sm + A::div(fabs(Bx), A::sqrt(dsp * mu0)),
+		/* sqrt(rho* mu0): with mu0 = 1 the product is exact, so it is the
+		   square root just taken (a uniform branch) */
+		sqmn = ph.mu0 == 1.0 ? dssn : A::sqrt(dsn * mu0),
+		sqmp = ph.mu0 == 1.0 ? dssp : A::sqrt(dsp * mu0),
+		ssn = sm - A::div(fabs(Bx), sqmn),
+		ssp = sm + A::div(fabs(Bx), sqmp),
 		ptst = L.p + nmn + L.u[RHO] * sfdn * (sm - vL[0]);
 
 	T Sn[8], Sp[8], vdbsn, vdbsp;
@@ -1354,10 +1436,7 @@ template <int SOLVER, class T> __device__ __forceinline__ int riemann_d(const do
 	const double R[8], const T bg[3], const Phys& ph, double F[8], double& s)
 {
 	if constexpr (SOLVER == HLLD_ATHENA && std::is_same_v<T, float>) {
-		float sf = 0;
-		const int e = flux_hlld_mixed(L, R, ph, F, sf);
-		s = sf;
-		return e;
+		return flux_hlld_mixed(L, R, ph, F, s);
 	} else {
 		T Ft[8], st = 0;
 		const int e = riemann<SOLVER, T>(L, R, bg, ph, Ft, st);

@@ -206,6 +206,13 @@ class Grid:
         self._check(self._L.pamhd_last_max_substep(self._h, ctypes.byref(v)))
         return v.value
 
+    def last_substep_periods(self):
+        """Each cell's substep period in the last substepped step (int32 (n,))."""
+        n = int(np.prod(self.cells))
+        out = np.empty(n, dtype=np.int32)
+        self._check(self._L.pamhd_last_substep_periods(self._h, out.ctypes.data))
+        return out
+
     def last_flux_calcs(self):
         """Faces solved by the last step: timestep()'s third return value."""
         v = ctypes.c_ulonglong(0)
@@ -269,9 +276,8 @@ class Grid:
         """HLLD's step structure (identical results): "passes" (per-direction
         flux passes and an update pass), "fused" (one pass, then the
         face-field update), "accumulate" (per-direction marches that update
-        the state, then the face-field update) or "auto" (accumulate in
-        FP64, fused in mixed precision); True / False select fused /
-        passes.  planes_per_chunk: planes per fused block (1..32) or cells
+        the state, then the face-field update) or "auto" (accumulate);
+        True / False select fused / passes.  planes_per_chunk: planes per fused block (1..32) or cells
         per accumulating march (1..32); 0 keeps the current value."""
         if mode is True or mode is False:
             m = 1 if mode else 0

@@ -37,6 +37,33 @@ def test_substepped_run_bit_exact(gpu, name, sid, smax, cells):
     np.testing.assert_array_equal(out, ref)
 
 
+def test_substepped_run_many_blocks_per_reduction_slot(gpu):
+    """The step's reductions (smallest time step, largest level and period,
+    faces solved) go to 64 slots by block index; a grid of 320 blocks puts
+    several blocks into each slot (the reference's default options 0, 999)."""
+    from pamhd_amd.solver import Grid
+    cells = (64, 32, 40)
+    state, length, gamma, mu0 = configs.hot_blob(cells, contrast=30.0, radius=0.25, seed=6)
+    g_o = O.make_grid(cells, length, gamma, mu0)
+    ref = state.copy()
+    st, steps_o, t_o, mp_o = O.run_periodic_substep(g_o, O.HLLD, 0.3, 0, 999, 0.0, 1e9, ref,
+                                                    max_steps=4)
+    assert st == 0 and steps_o == 4 and mp_o > 1
+    fc_o = O.last_flux_calcs()
+    with Grid(cells, length, gamma, mu0) as g:
+        g.upload(state)
+        g.set_substepping(0, 999)
+        steps_g, t_g = g.run("hlld-athena", 0.3, 0.0, 1e9, max_steps=4)
+        mp_g = g.last_max_substep()
+        fc_g = g.last_flux_calcs()
+        periods = g.last_substep_periods()
+        out = g.download()
+    print("largest period", mp_g, "periods", np.unique(periods, return_counts=True), flush=True)
+    assert steps_g == 4 and mp_g == mp_o and t_g == t_o and fc_g == fc_o
+    assert periods.max() == mp_g and periods.min() >= 1
+    np.testing.assert_array_equal(out, ref)
+
+
 def test_substep_max_zero_is_the_global_step(gpu):
     from pamhd_amd.solver import Grid
     cells = (12, 10, 8)
