@@ -465,13 +465,13 @@ def precision_study(size=512, steps=160, clean_every=10, report_every=20, solver
 # the step's launches) measured with rocprofv3 --pmc for the step structure
 # and launch variants the autotuner picks (keyed by precision, size, solver
 # and HLLD step structure): profiles/step_kernels_256cubed_hlld_counters.md
-STEP_TRAFFIC_BYTES = {("fp64", 256, "hlld-athena", "accumulate"): 13.73e9,
-                      ("mixed", 256, "hlld-athena", "accumulate"): 14.55e9,
+STEP_TRAFFIC_BYTES = {("fp64", 256, "hlld-athena", "accumulate"): 13.64e9,
+                      ("mixed", 256, "hlld-athena", "accumulate"): 14.94e9,
                       ("fp64", 256, "hlld-athena", "passes"): 14.34e9}
 # VALU wave instructions of one step's flux launches (SQ_INSTS_VALU of the
 # same profiled runs; the flux launches are VALU-issue bound)
-FLUX_VALU_WAVE_INSTR = {("fp64", 256, "hlld-athena", "accumulate"): 8.57e8,
-                        ("mixed", 256, "hlld-athena", "accumulate"): 5.90e8,
+FLUX_VALU_WAVE_INSTR = {("fp64", 256, "hlld-athena", "accumulate"): 8.44e8,
+                        ("mixed", 256, "hlld-athena", "accumulate"): 5.95e8,
                         ("fp64", 256, "hlld-athena", "passes"): 7.47e8}
 STRUCTURES = {"0": "passes", "1": "fused", "2": "accumulate"}
 # one wave64 VALU instruction per clock per CU: 256 CUs x 2.4 GHz (the

@@ -2091,13 +2091,16 @@ const char* error_name(int e)
 Per-cell 2^N substepping of timestep(), source/mhd/solve.hpp:1837-1924 with
 source/substepping.hpp, on a periodic grid of normal cells (FP64).  Working
 layout: S 14 x n (rho, m, E, vol B, the cell's copies of its -x, +x, -y, +y,
--z, +z faces), slots 48 x n ((2d + side) * 8 + q), face dB 6 x n, per-face
-signal speeds 3 x n, per-cell time step, level exponent, period and the
-smallest period of the cell's face and edge neighbourhood.  The state the
-handle keeps is the face-once layout; a substepped step expands it, runs
-the substeps, syncs the face copies and writes it back.  A face copy's
-change sums the same 12 edge terms in the same order as update_cell_kernel;
-slots and face dB accumulate substep by substep.
+-z, +z faces), per-face flux sums 48 x n ((d * 8 + q): the sum serving the
+slower of the face's two cells, or both; ((3 + d) * 8 + q): the faster
+cell's where the periods differ), face dB sums 9 x n (the same two sums
+per face, and the +d faces at a closed end), per-face signal
+speeds 3 x n, per-cell time step, level exponent, period and the smallest
+period of the cell's face and edge neighbourhood.  The state the handle
+keeps is the face-once layout; a substepped step expands it, runs the
+substeps, syncs the face copies and writes it back.  A face copy's change
+sums the same 12 edge terms in the same order as update_cell_kernel; the
+sums a cell applies equal its flux slots and face dB of the reference.
 */
 struct SubstepDev {
 	double* S = nullptr;
@@ -2179,25 +2182,27 @@ template <class F> __device__ __forceinline__ void for_face_edge_neighbours(cons
 
 /* the step's five reductions (0: smallest cell time step, 1: largest
    period, 2: smallest time step / 2^substep-min, 3: largest level, 4: faces
-   solved) are spread over NSLOT slots each, a wave's atomic going to the
-   slot of its block (one address per reduction serialises 2^18 atomics at
-   256^3), and folded by ss_fold_kernel / ss_count_kernel before they are
-   read */
+   solved) are spread over SS_SLOTS slots each, a wave's atomic going to the
+   slot of its block (a few addresses serialise the 2^18 waves' atomics of
+   a 256^3 grid), and folded by ss_fold_kernel / ss_count_kernel before
+   they are read */
+constexpr int SS_SLOTS = 4096;
+constexpr int SS_FOLD = 256;
 __device__ __forceinline__ unsigned ss_slot()
 {
-	return (blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) % NSLOT;
+	return (blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) % SS_SLOTS;
 }
 
-__global__ void ss_clear_kernel(SubstepDev d)
+__global__ __launch_bounds__(SS_FOLD) void ss_clear_kernel(SubstepDev d)
 {
 	const unsigned long long big = (unsigned long long)__double_as_longlong(DBL_MAX);
-	for (int s = threadIdx.x; s < NSLOT; s += blockDim.x) {
-		d.scal[0 * NSLOT + s] = big;
-		d.scal[1 * NSLOT + s] = 0;
-		d.scal[2 * NSLOT + s] = big;
-		d.scal[3 * NSLOT + s] = 0;
+	for (int s = threadIdx.x; s < SS_SLOTS; s += blockDim.x) {
+		d.scal[0 * SS_SLOTS + s] = big;
+		d.scal[1 * SS_SLOTS + s] = 0;
+		d.scal[2 * SS_SLOTS + s] = big;
+		d.scal[3 * SS_SLOTS + s] = 0;
 	}
-	for (int s = threadIdx.x; s < NSLOT; s += blockDim.x) d.scal[4 * NSLOT + s] = 0;
+	for (int s = threadIdx.x; s < SS_SLOTS; s += blockDim.x) d.scal[4 * SS_SLOTS + s] = 0;
 	if (threadIdx.x == 0) {
 		d.status->err = 0;
 		d.status->err_face = -1;
@@ -2206,25 +2211,43 @@ __global__ void ss_clear_kernel(SubstepDev d)
 }
 
 /* the faces solved over the step: the fifth reduction's slots summed into
-   the status record */
-__global__ __launch_bounds__(NSLOT) void ss_count_kernel(SubstepDev d)
+   the status record (one block of SS_FOLD threads) */
+__global__ __launch_bounds__(SS_FOLD) void ss_count_kernel(SubstepDev d)
 {
-	unsigned long long v = d.scal[4 * NSLOT + threadIdx.x];
-	for (int off = NSLOT / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
-	if (threadIdx.x == 0) d.status->flux_calcs = v;
+	__shared__ unsigned long long w[SS_FOLD / 64];
+	unsigned long long v = 0;
+	for (int s = threadIdx.x; s < SS_SLOTS; s += SS_FOLD) v += d.scal[4 * SS_SLOTS + s];
+	for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+	if (threadIdx.x % 64 == 0) w[threadIdx.x / 64] = v;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int k = 1; k < SS_FOLD / 64; k++) v += w[k];
+		d.status->flux_calcs = v;
+	}
 }
 
-/* reduction q's NSLOT slots into slot 0 (one wave): min when is_min (the
-   slots hold non-negative doubles' bit patterns, which order as the values
-   do), max otherwise */
-__global__ __launch_bounds__(NSLOT) void ss_fold_kernel(SubstepDev d, int q, int is_min)
+/* reduction q's SS_SLOTS slots into slot 0 (one block of SS_FOLD threads):
+   min when is_min (the slots hold non-negative doubles' bit patterns, which
+   order as the values do), max otherwise */
+__global__ __launch_bounds__(SS_FOLD) void ss_fold_kernel(SubstepDev d, int q, int is_min)
 {
-	unsigned long long v = d.scal[q * NSLOT + threadIdx.x];
-	for (int off = NSLOT / 2; off > 0; off >>= 1) {
+	__shared__ unsigned long long w[SS_FOLD / 64];
+	const unsigned long long* a = d.scal + (size_t)q * SS_SLOTS;
+	unsigned long long v = a[threadIdx.x];
+	for (int s = threadIdx.x + SS_FOLD; s < SS_SLOTS; s += SS_FOLD) {
+		const unsigned long long o = a[s];
+		v = is_min ? (o < v ? o : v) : (o > v ? o : v);
+	}
+	for (int off = 32; off > 0; off >>= 1) {
 		const unsigned long long o = __shfl_xor(v, off, 64);
 		v = is_min ? (o < v ? o : v) : (o > v ? o : v);
 	}
-	if (threadIdx.x == 0) d.scal[q * NSLOT] = v;
+	if (threadIdx.x % 64 == 0) w[threadIdx.x / 64] = v;
+	__syncthreads();
+	if (threadIdx.x == 0) {
+		for (int k = 1; k < SS_FOLD / 64; k++) v = is_min ? (w[k] < v ? w[k] : v) : (w[k] > v ? w[k] : v);
+		d.scal[(size_t)q * SS_SLOTS] = v;
+	}
 }
 
 /* minimize_timestep, source/mhd/solve.hpp:1927-2014: a cell's limit over its
@@ -2266,8 +2289,8 @@ __global__ __launch_bounds__(BX * BY) void ss_ts_kernel(GridDev g, SubstepDev d,
 		r = smin(r, __shfl_xor(r, off, 64));
 	}
 	if ((threadIdx.x + BX * threadIdx.y) % 64 == 0) {
-		atomicMin(&d.scal[0 * NSLOT + ss_slot()], (unsigned long long)__double_as_longlong(t));
-		atomicMin(&d.scal[2 * NSLOT + ss_slot()], (unsigned long long)__double_as_longlong(r));
+		atomicMin(&d.scal[0 * SS_SLOTS + ss_slot()], (unsigned long long)__double_as_longlong(t));
+		atomicMin(&d.scal[2 * SS_SLOTS + ss_slot()], (unsigned long long)__double_as_longlong(r));
 	}
 }
 
@@ -2280,7 +2303,7 @@ __global__ __launch_bounds__(BX * BY) void ss_level_kernel(GridDev g, SubstepDev
 {
 	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
 	const double smallest = __longlong_as_double((long long)d.scal[0]);
-	const double ret = __longlong_as_double((long long)d.scal[2 * NSLOT]);
+	const double ret = __longlong_as_double((long long)d.scal[2 * SS_SLOTS]);
 	const bool all_zero = max_dt <= smallest;
 	if (i == 0 && j == 0 && k == 0) *d.sub_dt = all_zero ? max_dt : ret;
 	int lev = 0;
@@ -2305,7 +2328,7 @@ __global__ __launch_bounds__(BX * BY) void ss_level_kernel(GridDev g, SubstepDev
 		lev = o > lev ? o : lev;
 	}
 	if ((threadIdx.x + BX * threadIdx.y) % 64 == 0)
-		atomicMax(&d.scal[3 * NSLOT + ss_slot()], (unsigned long long)lev);
+		atomicMax(&d.scal[3 * SS_SLOTS + ss_slot()], (unsigned long long)lev);
 }
 
 /* one Jacobi pass of restrict_substepping_period, substepping.hpp:99-191 */
@@ -2342,7 +2365,7 @@ __global__ __launch_bounds__(BX * BY) void ss_period_kernel(GridDev g, SubstepDe
 		const unsigned long long o = __shfl_xor(p, off, 64);
 		p = o > p ? o : p;
 	}
-	if ((threadIdx.x + BX * threadIdx.y) % 64 == 0) atomicMax(&d.scal[1 * NSLOT + ss_slot()], p);
+	if ((threadIdx.x + BX * threadIdx.y) % 64 == 0) atomicMax(&d.scal[1 * SS_SLOTS + ss_slot()], p);
 }
 
 /* the working layout from the face-once state */
@@ -2362,12 +2385,25 @@ __global__ __launch_bounds__(BX * BY) void ss_expand_kernel(GridDev g, const dou
 	}
 }
 
-/* faces of direction D due at substep `sub`: solved from the cells' stored
-   volume field and weighted by the substep length times the owner's
-   smallest neighbourhood period; the others get 0 */
+/*
+Faces of direction D due at substep `sub` (source/mhd/solve.hpp:252-393 with
+substepping): a face between two cells of type >= 0 is solved by its
+negative-side owner o at the pace of o's smallest neighbourhood period m,
+from the cells' stored volume field, and weighted by the substep length
+times m.  Its contribution goes into per-face accumulators instead of the
+two cells' flux slots: both cells receive the same contributions, so one
+sum serves both when their periods are equal; where they differ, the
+faster cell (period pmin) gets a second sum.  A sum restarts (0 + c) at
+the first solve after the cell(s) it serves applied their slots: at
+substep s with (s - m) % p == 0, m dividing every period around o.  The
+slots of the reference hold exactly these sums (0 + c1 + c2 + ..., never
+-0, so the zeros the reference's unsolved substeps would add change no
+bit).  The face's two field components the edge terms use go to emf (0
+when the face is not solved at this substep).
+*/
 template <int SOLVER, int D>
 __global__ __launch_bounds__(BX * BY) void ss_flux_kernel(GridDev g, SubstepDev d,
-	const double* __restrict__ bg, double* __restrict__ P, int sub)
+	const double* __restrict__ bg, double* __restrict__ emf, int sub)
 {
 	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
 	if (i >= g.nx || j >= g.ny) return;
@@ -2376,26 +2412,34 @@ __global__ __launch_bounds__(BX * BY) void ss_flux_kernel(GridDev g, SubstepDev 
 	const int oi = D == 0 ? ss_coord(i - 1, g.nx, d.per[0]) : i;
 	const int oj = D == 1 ? ss_coord(j - 1, g.ny, d.per[1]) : j;
 	const int ok = D == 2 ? ss_coord(k - 1, g.nz, d.per[2]) : k;
-	double* Pd = P + (long long)D * 8 * n;
-	/* a face between two cells of type >= 0, solved by its negative-side
-	   owner at the pace of the owner's smallest neighbourhood period */
+	double* A = d.slot + (long long)D * 8 * n;          /* sum of the slower cell */
+	double* A2 = d.slot + (long long)(3 + D) * 8 * n;   /* sum of the faster cell */
+	double* Em = emf + (long long)(2 * D) * n;
 	const int o = (oi < 0 || oj < 0 || ok < 0) ? -1 : cidx(g, oi, oj, ok);
+	const bool act = o >= 0 && ss_active(d, o) && ss_active(d, c);
 	{
 		/* faces solved (timestep()'s flux_calcs), counted per wave into the
 		   block's slot of the fifth reduction */
-		const bool solved = o >= 0 && ss_active(d, o) && ss_active(d, c) && sub % d.minsub[o] == 0;
+		const bool solved = act && (sub & (d.minsub[o] - 1)) == 0;
 		const unsigned long long b = __ballot(solved);
-		const unsigned long long act = __ballot(1);
-		if ((int)(threadIdx.x + BX * threadIdx.y) % 64 == __ffsll((long long)act) - 1 && b)
-			atomicAdd(&d.scal[4 * NSLOT + ss_slot()], (unsigned long long)__popcll(b));
+		const unsigned long long a = __ballot(1);
+		if ((int)(threadIdx.x + BX * threadIdx.y) % 64 == __ffsll((long long)a) - 1 && b)
+			atomicAdd(&d.scal[4 * SS_SLOTS + ss_slot()], (unsigned long long)__popcll(b));
 	}
-	if (o < 0 || !ss_active(d, o) || !ss_active(d, c)) {
-		for (int q = 0; q < 8; q++) Pd[q * n + c] = 0;
+	if (!act) {
+		/* no face, or a cell not solved: nothing is ever added */
+		for (int q = 0; q < 8; q++) {
+			A[q * n + c] = 0;
+			A2[q * n + c] = 0;
+		}
+		Em[c] = 0;
+		Em[n + c] = 0;
 		return;
 	}
 	const int m = d.minsub[o];
-	if (sub % m != 0) {
-		for (int q = 0; q < 8; q++) Pd[q * n + c] = 0;
+	if (sub & (m - 1)) {
+		Em[c] = 0;
+		Em[n + c] = 0;
 		return;
 	}
 	double L[8], R[8];
@@ -2419,26 +2463,60 @@ __global__ __launch_bounds__(BX * BY) void ss_flux_kernel(GridDev g, SubstepDev 
 	}
 	double F[8], sp = 0;
 	const int e = riemann<SOLVER, double>(L, R, b, g.ph, F, sp);
-	if (e) {
-		record_error(d.status, e, (long long)D * n + c);
-		for (int q = 0; q < 8; q++) Pd[q * n + c] = 0;
-		return;
-	}
-	d.maxv[(long long)D * n + c] = sp;
 	double Fg[8];
-	Fg[RHO] = F[RHO];
-	Fg[NRJ] = F[NRJ];
-	Fg[M0 + D] = F[M0]; Fg[M0 + (D + 1) % 3] = F[M1]; Fg[M0 + (D + 2) % 3] = F[M2];
-	Fg[B0 + D] = F[B0]; Fg[B0 + (D + 1) % 3] = F[B1]; Fg[B0 + (D + 2) % 3] = F[B2];
+	if (e) {
+		/* the step fails; the face contributes nothing */
+		record_error(d.status, e, (long long)D * n + c);
+		for (int q = 0; q < 8; q++) Fg[q] = 0;
+	} else {
+		d.maxv[(long long)D * n + c] = sp;
+		Fg[RHO] = F[RHO];
+		Fg[NRJ] = F[NRJ];
+		Fg[M0 + D] = F[M0]; Fg[M0 + (D + 1) % 3] = F[M1]; Fg[M0 + (D + 2) % 3] = F[M2];
+		Fg[B0 + D] = F[B0]; Fg[B0 + (D + 1) % 3] = F[B1]; Fg[B0 + (D + 2) % 3] = F[B2];
+	}
 	const double min_dt = *d.sub_dt * m;
-	for (int q = 0; q < 8; q++) Pd[q * n + c] = min_dt * Fg[q];
+	double cq[8];
+	for (int q = 0; q < 8; q++) cq[q] = min_dt * Fg[q];
+	const int po = d.period[o], pc = d.period[c];
+	const int pmax = po > pc ? po : pc, pmin = po > pc ? pc : po;
+	const bool fresh = ((sub - m) & (pmax - 1)) == 0;
+	for (int q = 0; q < 8; q++) A[q * n + c] = (fresh ? 0.0 : A[q * n + c]) + cq[q];
+	if (pmin < pmax) {
+		const bool fresh2 = ((sub - m) & (pmin - 1)) == 0;
+		for (int q = 0; q < 8; q++) A2[q * n + c] = (fresh2 ? 0.0 : A2[q * n + c]) + cq[q];
+	}
+	/* the edge-term components: x-face (By, Bz), y (Bz, Bx), z (By, Bx) */
+	Em[c] = cq[D == 1 ? B0 + 2 : B0 + 1];
+	Em[n + c] = cq[D == 0 ? B0 + 2 : B0];
 }
 
-/* slots and face dB gather this substep's contributions; cells whose period
-   divides `sub` apply them (update_mhd_state, source/mhd/solve.hpp:1503-1603)
-   and refit from their own face copies (update_vol_B, :1618-1683) */
-__global__ __launch_bounds__(BX * BY) void ss_update_kernel(GridDev g, SubstepDev d,
-	const double* __restrict__ P, int sub)
+/* constrained-transport edge terms from the substep's 6 field components
+   per cell (emf), 0 beyond a closed end */
+struct EdgeTerms6B {
+	const double* P;
+	long long n;
+	double dx, dy, dz;
+	__device__ __forceinline__ double at(int q, long long c) const { return c < 0 ? 0.0 : P[q * n + c]; }
+	__device__ __forceinline__ double ezx(long long c) const { return +(at(0, c) * dz / 4); }
+	__device__ __forceinline__ double eyx(long long c) const { return -(at(1, c) * dy / 4); }
+	__device__ __forceinline__ double exy(long long c) const { return +(at(2, c) * dx / 4); }
+	__device__ __forceinline__ double ezy(long long c) const { return -(at(3, c) * dz / 4); }
+	__device__ __forceinline__ double exz(long long c) const { return -(at(4, c) * dx / 4); }
+	__device__ __forceinline__ double eyz(long long c) const { return +(at(5, c) * dy / 4); }
+};
+
+/*
+Face dB sums of substep `sub`: each face's dB (the same 12 edge terms, in
+the same order, for both of its copies) is summed per face like the
+fluxes: one sum where the two cells' periods are equal, a second for the
+faster cell where they differ, each restarting at the first substep after
+the cells it serves applied (x % p is x & (p - 1): periods are powers of
+two).  A cell computes the sums of the faces it owns (its -x, -y, -z
+faces) and of its +d faces at a closed end (kept in its end slots).
+*/
+__global__ __launch_bounds__(BX * BY) void ss_dbsum_kernel(GridDev g, SubstepDev d,
+	const double* __restrict__ emf, int sub)
 {
 	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
 	if (i >= g.nx || j >= g.ny) return;
@@ -2448,47 +2526,42 @@ __global__ __launch_bounds__(BX * BY) void ss_update_kernel(GridDev g, SubstepDe
 	const int jm = ss_coord(j - 1, g.ny, d.per[1]), jp = ss_coord(j + 1, g.ny, d.per[1]);
 	const int km = ss_coord(k - 1, g.nz, d.per[2]), kp = ss_coord(k + 1, g.nz, d.per[2]);
 	const NbB L{g.nx, g.ny};
-	const EdgeTermsB E{P, n, g.dx, g.dy, g.dz};
-	const double db[6] = {
-		dBx_at(E, L, i, im, j, jm, jp, k, km, kp),
-		dBx_at(E, L, ip, i, j, jm, jp, k, km, kp),
-		dBy_at(E, L, i, im, ip, j, jm, k, km, kp),
-		dBy_at(E, L, i, im, ip, jp, j, k, km, kp),
-		dBz_at(E, L, i, im, ip, j, jm, jp, k, km),
-		dBz_at(E, L, i, im, ip, j, jm, jp, kp, k)};
-	/* a cell of period 1 never accumulates, and every cell's sums are
-	   cleared at the end of its period (the step's last substep ends every
-	   period), so its sums are +0 here: they are neither read nor cleared */
-	const int period = d.period[c];
-	const bool held = period > 1;
-	double fd[6];
-	for (int f = 0; f < 6; f++) fd[f] = (held ? d.fdb[f * n + c] : 0.0) + db[f];
-	const long long cp[3] = {L(ip, j, k), L(i, jp, k), L(i, j, kp)};
-	double sl[6][8];
-	for (int q3 = 0; q3 < 3; q3++)
-	for (int q = 0; q < 8; q++) {
-		sl[2 * q3][q] = (held ? d.slot[((2 * q3) * 8 + q) * n + c] : 0.0) + P[(q3 * 8 + q) * n + c];
-		sl[2 * q3 + 1][q] = held ? d.slot[((2 * q3 + 1) * 8 + q) * n + c] : 0.0;
-		if (cp[q3] >= 0) sl[2 * q3 + 1][q] += P[(q3 * 8 + q) * n + cp[q3]];
-	}
+	const EdgeTerms6B E{emf, n, g.dx, g.dy, g.dz};
+	const int pc = d.period[c];
+	double* fdb = d.fdb;
+	auto accumulate = [&](long long slow_i, long long fast_i, long long other, double db) {
+		const int po = other >= 0 ? d.period[other] : pc;
+		const int pmax = po > pc ? po : pc, pmin = po > pc ? pc : po;
+		fdb[slow_i] = ((((sub - 1) & (pmax - 1)) == 0) ? 0.0 : fdb[slow_i]) + db;
+		if (pmin < pmax) fdb[fast_i] = ((((sub - 1) & (pmin - 1)) == 0) ? 0.0 : fdb[fast_i]) + db;
+	};
+	accumulate(c, 3 * n + c, L(im, j, k), dBx_at(E, L, i, im, j, jm, jp, k, km, kp));
+	accumulate(n + c, 4 * n + c, L(i, jm, k), dBy_at(E, L, i, im, ip, j, jm, k, km, kp));
+	accumulate(2 * n + c, 5 * n + c, L(i, j, km), dBz_at(E, L, i, im, ip, j, jm, jp, k, km));
+	if (ip < 0) accumulate(6 * n + c, 6 * n + c, -1, dBx_at(E, L, ip, i, j, jm, jp, k, km, kp));
+	if (jp < 0) accumulate(7 * n + c, 7 * n + c, -1, dBy_at(E, L, i, im, ip, jp, j, k, km, kp));
+	if (kp < 0) accumulate(8 * n + c, 8 * n + c, -1, dBz_at(E, L, i, im, ip, j, jm, jp, kp, k));
+}
+
+/*
+The update of cells due at substep `sub` (update_mhd_state,
+source/mhd/solve.hpp:1503-1603, and update_vol_B, :1618-1683, from their
+own face copies): a due cell applies the flux and face dB sums of its six
+faces that serve it (the faster cell's where the two periods differ).
+*/
+__global__ __launch_bounds__(BX * BY) void ss_update_kernel(GridDev g, SubstepDev d, int sub)
+{
+	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	if (i >= g.nx || j >= g.ny) return;
+	const long long n = g.n;
+	const int c = cidx(g, i, j, k);
+	const int pc = d.period[c];
+	if (sub & (pc - 1)) return;
 	const int ct = d.ct ? d.ct[c] : 1;
-	const bool due = sub % period == 0;
-	if (ct > 0 && !due) {
-		/* normal cells accumulate until their period ends */
-		for (int f = 0; f < 6; f++) d.fdb[f * n + c] = fd[f];
-		for (int dir = 0; dir < 6; dir++)
-		for (int q = 0; q < 8; q++) d.slot[(dir * 8 + q) * n + c] = sl[dir][q];
-		return;
-	}
-	if (held) {
-		for (int f = 0; f < 6; f++) d.fdb[f * n + c] = 0;
-		for (int dir = 0; dir < 6; dir++)
-		for (int q = 0; q < 8; q++) d.slot[(dir * 8 + q) * n + c] = 0;
-	}
 	if (ct <= 0) {
 		/* boundary cells: update_vol_B at their period (constant
 		   pressure, own face copies); cells not solved: nothing */
-		if (ct == 0 && due) {
+		if (ct == 0) {
 			const double rho = d.S[c], m[3] = {d.S[n + c], d.S[2 * n + c], d.S[3 * n + c]};
 			if (rho > 0) {
 				double En = d.S[4 * n + c];
@@ -2503,6 +2576,27 @@ __global__ __launch_bounds__(BX * BY) void ss_update_kernel(GridDev g, SubstepDe
 			}
 		}
 		return;
+	}
+	const int im = ss_coord(i - 1, g.nx, d.per[0]), ip = ss_coord(i + 1, g.nx, d.per[0]);
+	const int jm = ss_coord(j - 1, g.ny, d.per[1]), jp = ss_coord(j + 1, g.ny, d.per[1]);
+	const int km = ss_coord(k - 1, g.nz, d.per[2]), kp = ss_coord(k + 1, g.nz, d.per[2]);
+	const NbB L{g.nx, g.ny};
+	/* the neighbour across each face (-1: none) */
+	const long long nb[6] = {L(im, j, k), L(ip, j, k), L(i, jm, k), L(i, jp, k), L(i, j, km), L(i, j, kp)};
+	double sl[6][8], fd[6];
+	for (int q3 = 0; q3 < 3; q3++)
+	for (int side = 0; side < 2; side++) {
+		const int f = 2 * q3 + side;
+		const long long other = nb[f];
+		const int po = other >= 0 ? d.period[other] : pc;
+		const bool fast = pc < po;    /* this cell is the faster of the two */
+		/* the face's owner: c for -d, the +d neighbour for +d (none: the
+		   cell's end slot, and a flux sum of +0, the value of a slot
+		   nothing was added to) */
+		const long long own = side == 0 ? c : other;
+		fd[f] = own >= 0 ? d.fdb[(long long)((fast ? 3 : 0) + q3) * n + own] : d.fdb[(long long)(6 + q3) * n + c];
+		const double* A = d.slot + (long long)((fast ? 3 : 0) + q3) * 8 * n;
+		for (int q = 0; q < 8; q++) sl[f][q] = own >= 0 ? A[q * n + own] : 0.0;
 	}
 	const double len[3] = {g.dx, g.dy, g.dz};
 	const double area[3] = {g.dy * g.dz, g.dx * g.dz, g.dx * g.dy};
@@ -3258,7 +3352,7 @@ int substep_alloc(pamhd_handle* h)
 	};
 	al(&d->S, 14 * n * sizeof(double));
 	al(&d->slot, 48 * n * sizeof(double));
-	al(&d->fdb, 6 * n * sizeof(double));
+	al(&d->fdb, 9 * n * sizeof(double));
 	al(&d->maxv, 3 * n * sizeof(double));
 	al(&d->ts, n * sizeof(double));
 	al(&d->e[0], n * sizeof(int));
@@ -3266,7 +3360,7 @@ int substep_alloc(pamhd_handle* h)
 	al(&d->period, n * sizeof(int));
 	al(&d->minsub, n * sizeof(int));
 	al(&d->opt_dev, 2 * n * sizeof(int));
-	al(&d->scal, 5 * NSLOT * sizeof(unsigned long long));
+	al(&d->scal, 5 * SS_SLOTS * sizeof(unsigned long long));
 	al(&d->sub_dt, sizeof(double));
 	al(&d->status, sizeof(StepStatus));
 	if (e == hipSuccess) e = hipHostMalloc((void**)&d->host, 3 * sizeof(unsigned long long));
@@ -3279,7 +3373,7 @@ int substep_alloc(pamhd_handle* h)
 	   speeds are unknown until the next flux pass */
 	const unsigned blocks = (unsigned)((48 * n + 255) / 256);
 	ss_fill_kernel<<<blocks, dim3(BX, BY, 1)>>>(d->slot, (long long)(48 * n), 0.0);
-	ss_fill_kernel<<<(unsigned)((6 * n + 255) / 256), dim3(BX, BY, 1)>>>(d->fdb, (long long)(6 * n), 0.0);
+	ss_fill_kernel<<<(unsigned)((9 * n + 255) / 256), dim3(BX, BY, 1)>>>(d->fdb, (long long)(9 * n), 0.0);
 	ss_fill_kernel<<<(unsigned)((3 * n + 255) / 256), dim3(BX, BY, 1)>>>(d->maxv, (long long)(3 * n), -1.0);
 	HIP_TRY(h, hipGetLastError());
 	HIP_TRY(h, hipDeviceSynchronize());
@@ -3321,15 +3415,15 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 {
 	SubstepDev& d = *h->sub;
 	const dim3 grid = cell_grid(h->g), block(BX, BY, 1);
-	double* P = h->flux;   /* 24 x n FP64 (substepping runs in FP64) */
-	ss_clear_kernel<<<1, 64, 0, st>>>(d);
+	double* P = h->flux;   /* the substep's field components per face, 6 x n (of 24 x n) */
+	ss_clear_kernel<<<1, SS_FOLD, 0, st>>>(d);
 	ss_ts_kernel<<<grid, block, 0, st>>>(h->g, d, cfl);
-	ss_fold_kernel<<<1, NSLOT, 0, st>>>(d, 0, 1);
-	ss_fold_kernel<<<1, NSLOT, 0, st>>>(d, 2, 1);
+	ss_fold_kernel<<<1, SS_FOLD, 0, st>>>(d, 0, 1);
+	ss_fold_kernel<<<1, SS_FOLD, 0, st>>>(d, 2, 1);
 	ss_level_kernel<<<grid, block, 0, st>>>(h->g, d, max_dt);
-	ss_fold_kernel<<<1, NSLOT, 0, st>>>(d, 3, 0);
+	ss_fold_kernel<<<1, SS_FOLD, 0, st>>>(d, 3, 0);
 	HIP_TRY(h, hipGetLastError());
-	HIP_TRY(h, hipMemcpyAsync(d.host + 2, d.scal + 3 * NSLOT, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+	HIP_TRY(h, hipMemcpyAsync(d.host + 2, d.scal + 3 * SS_SLOTS, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
 	HIP_TRY(h, hipMemcpyAsync(d.host_status, d.status, sizeof(StepStatus), hipMemcpyDeviceToHost, st));
 	HIP_TRY(h, hipStreamSynchronize(st));
 	if (d.host_status->err) return report_status(h, *d.host_status);
@@ -3343,9 +3437,9 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 		cur = 1 - cur;
 	}
 	ss_period_kernel<<<grid, block, 0, st>>>(h->g, d, d.e[cur]);
-	ss_fold_kernel<<<1, NSLOT, 0, st>>>(d, 1, 0);
+	ss_fold_kernel<<<1, SS_FOLD, 0, st>>>(d, 1, 0);
 	HIP_TRY(h, hipGetLastError());
-	HIP_TRY(h, hipMemcpyAsync(d.host, d.scal + 1 * NSLOT, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+	HIP_TRY(h, hipMemcpyAsync(d.host, d.scal + 1 * SS_SLOTS, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
 	HIP_TRY(h, hipMemcpyAsync(d.host + 1, d.sub_dt, sizeof(double), hipMemcpyDeviceToHost, st));
 	HIP_TRY(h, hipStreamSynchronize(st));
 	const int maxp = (int)d.host[0];
@@ -3364,13 +3458,14 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 		case ROE_ATHENA: ss_launch_flux<ROE_ATHENA>(h, P, sub, st); break;
 		default: return fail(h, PAMHD_E_SOLVER, "invalid solver " + std::to_string(solver));
 		}
-		ss_update_kernel<<<grid, block, 0, st>>>(h->g, d, P, sub);
+		ss_dbsum_kernel<<<grid, block, 0, st>>>(h->g, d, P, sub);
+		ss_update_kernel<<<grid, block, 0, st>>>(h->g, d, sub);
 	}
 	ss_sync_kernel<0><<<grid, block, 0, st>>>(h->g, d);
 	ss_sync_kernel<1><<<grid, block, 0, st>>>(h->g, d);
 	ss_sync_kernel<2><<<grid, block, 0, st>>>(h->g, d);
 	ss_final_kernel<<<grid, block, 0, st>>>(h->g, d, h->state[h->cur]);
-	ss_count_kernel<<<1, NSLOT, 0, st>>>(d);
+	ss_count_kernel<<<1, SS_FOLD, 0, st>>>(d);
 	if (refits == 3) {
 		if (d.bounded) launch_boundaries(h, st);
 		else refit_kernel<<<grid, block, 0, st>>>(h->g, h->state[h->cur]);
@@ -3729,7 +3824,7 @@ int pamhd_apply_boundaries(pamhd_handle* h, double /*simulation_time*/, void* st
 		/* value and copy boundaries, refit, sync; synchronous so that a
 		   boundary error is reported here */
 		const hipStream_t st = (hipStream_t)stream;
-		ss_clear_kernel<<<1, 64, 0, st>>>(*h->sub);
+		ss_clear_kernel<<<1, SS_FOLD, 0, st>>>(*h->sub);
 		launch_boundaries(h, st);
 		HIP_TRY(h, hipGetLastError());
 		HIP_TRY(h, hipMemcpyAsync(h->sub->host_status, h->sub->status, sizeof(StepStatus), hipMemcpyDeviceToHost, st));
