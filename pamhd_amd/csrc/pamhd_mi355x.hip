@@ -1969,24 +1969,54 @@ __global__ __launch_bounds__(BX * BY) void div_kernel(GridDev g, const double* _
 
 /* right-hand side of the cleaning Poisson problem: the face divergence of
    each cell, and the squared roundoff scale sum_d 2|B_d|/len_d of its faces */
+/* right-hand side of the cleaning's Poisson problem, the face divergence of
+   each cell, into the multigrid level's layout; a block takes CLEAN_KZ
+   planes of a BX x BY tile and writes the block sums of |div| and of the
+   squared divergence scale (2 sum_d |B_d| / h_d)^2 of its cells as a pair */
+constexpr int CLEAN_KZ = 8;
+
 __global__ __launch_bounds__(BX * BY) void clean_rhs_kernel(GridDev g, const double* __restrict__ s,
-	double* __restrict__ rhs, double* __restrict__ scale2)
+	double* __restrict__ rhs, double* __restrict__ pairs, int packed)
 {
+	__shared__ double sh[2][BX * BY];
 	const int i = blockIdx.x * BX + threadIdx.x;
 	const int j = blockIdx.y * BY + threadIdx.y;
-	const int k = blockIdx.z;
-	if (i >= g.nx || j >= g.ny) return;
 	const long long n = g.n;
-	const long long c = i + (long long)g.nx * (j + (long long)g.ny * k);
-	const long long cx = wrap_inc(i, g.nx) + (long long)g.nx * (j + (long long)g.ny * k);
-	const long long cy = i + (long long)g.nx * (wrap_inc(j, g.ny) + (long long)g.ny * k);
-	const long long cz = i + (long long)g.nx * (j + (long long)g.ny * wrap_inc(k, g.nz));
-	rhs[c] = (s[5 * n + cx] - s[5 * n + c]) / g.dx
-		+ (s[6 * n + cy] - s[6 * n + c]) / g.dy
-		+ (s[7 * n + cz] - s[7 * n + c]) / g.dz;
-	const double t = 2 * fabs(s[5 * n + c]) / g.dx + 2 * fabs(s[6 * n + c]) / g.dy
-		+ 2 * fabs(s[7 * n + c]) / g.dz;
-	scale2[c] = t * t;
+	double sa = 0, ss = 0;
+	if (i < g.nx && j < g.ny) {
+		for (int m = 0; m < CLEAN_KZ; m++) {
+			const int k = blockIdx.z * CLEAN_KZ + m;
+			if (k >= g.nz) break;
+			const long long c = i + (long long)g.nx * (j + (long long)g.ny * k);
+			const long long cx = wrap_inc(i, g.nx) + (long long)g.nx * (j + (long long)g.ny * k);
+			const long long cy = i + (long long)g.nx * (wrap_inc(j, g.ny) + (long long)g.ny * k);
+			const long long cz = i + (long long)g.nx * (j + (long long)g.ny * wrap_inc(k, g.nz));
+			const double d = (s[5 * n + cx] - s[5 * n + c]) / g.dx
+				+ (s[6 * n + cy] - s[6 * n + c]) / g.dy
+				+ (s[7 * n + cz] - s[7 * n + c]) / g.dz;
+			rhs[mg_index(packed, g.nx, g.ny, (int)(n / 2), i, j, k)] = d;
+			const double t = 2 * fabs(s[5 * n + c]) / g.dx + 2 * fabs(s[6 * n + c]) / g.dy
+				+ 2 * fabs(s[7 * n + c]) / g.dz;
+			sa += fabs(d);
+			ss += t * t;
+		}
+	}
+	const int tid = threadIdx.x + BX * threadIdx.y;
+	sh[0][tid] = sa;
+	sh[1][tid] = ss;
+	__syncthreads();
+	for (int off = BX * BY / 2; off > 0; off >>= 1) {
+		if (tid < off) {
+			sh[0][tid] += sh[0][tid + off];
+			sh[1][tid] += sh[1][tid + off];
+		}
+		__syncthreads();
+	}
+	if (tid == 0) {
+		const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+		pairs[2 * b] = sh[0][0];
+		pairs[2 * b + 1] = sh[1][0];
+	}
 }
 
 /* projection B_face -= grad phi (a dimension of length 1 has no gradient)
@@ -1994,7 +2024,7 @@ __global__ __launch_bounds__(BX * BY) void clean_rhs_kernel(GridDev g, const dou
    the new one; the refit needs the new field of the cell's positive faces,
    which is recomputed here from phi */
 __global__ __launch_bounds__(BX * BY) void project_kernel(GridDev g, const double* __restrict__ s,
-	double* __restrict__ o, const double* __restrict__ phi)
+	double* __restrict__ o, const double* __restrict__ phi_m, int packed)
 {
 	const int i = blockIdx.x * BX + threadIdx.x;
 	const int j = blockIdx.y * BY + threadIdx.y;
@@ -2002,24 +2032,26 @@ __global__ __launch_bounds__(BX * BY) void project_kernel(GridDev g, const doubl
 	if (i >= g.nx || j >= g.ny) return;
 	const long long n = g.n;
 	auto id = [&](int a, int b, int e) { return a + (long long)g.nx * (b + (long long)g.ny * e); };
+	/* phi in the multigrid level's layout */
+	auto phi_at = [&](int a, int b, int e) { return phi_m[mg_index(packed, g.nx, g.ny, (int)(n / 2), a, b, e)]; };
 	const long long c = id(i, j, k);
 	const int ip = wrap_inc(i, g.nx), jp = wrap_inc(j, g.ny), kp = wrap_inc(k, g.nz);
 	const int im = wrap_dec(i, g.nx), jm = wrap_dec(j, g.ny), km = wrap_dec(k, g.nz);
-	const double pc = phi[c];
+	const double pc = phi_at(i, j, k);
 	double b[3] = {s[5 * n + c], s[6 * n + c], s[7 * n + c]};
 	double bp[3] = {s[5 * n + id(ip, j, k)], s[6 * n + id(i, jp, k)], s[7 * n + id(i, j, kp)]};
 	const double vb_old[3] = {0.5 * (b[0] + bp[0]), 0.5 * (b[1] + bp[1]), 0.5 * (b[2] + bp[2])};
 	if (g.nx > 1) {
-		b[0] -= (pc - phi[id(im, j, k)]) / g.dx;
-		bp[0] -= (phi[id(ip, j, k)] - pc) / g.dx;
+		b[0] -= (pc - phi_at(im, j, k)) / g.dx;
+		bp[0] -= (phi_at(ip, j, k) - pc) / g.dx;
 	}
 	if (g.ny > 1) {
-		b[1] -= (pc - phi[id(i, jm, k)]) / g.dy;
-		bp[1] -= (phi[id(i, jp, k)] - pc) / g.dy;
+		b[1] -= (pc - phi_at(i, jm, k)) / g.dy;
+		bp[1] -= (phi_at(i, jp, k) - pc) / g.dy;
 	}
 	if (g.nz > 1) {
-		b[2] -= (pc - phi[id(i, j, km)]) / g.dz;
-		bp[2] -= (phi[id(i, j, kp)] - pc) / g.dz;
+		b[2] -= (pc - phi_at(i, j, km)) / g.dz;
+		bp[2] -= (phi_at(i, j, kp) - pc) / g.dz;
 	}
 	const double vb_new[3] = {0.5 * (b[0] + bp[0]), 0.5 * (b[1] + bp[1]), 0.5 * (b[2] + bp[2])};
 	const double rho = s[c];
@@ -3930,18 +3962,22 @@ int pamhd_clean_divergence(pamhd_handle* h, double tol, int max_cycles, double* 
 	mg.fused_smoother = h->mg_fused;
 	MgLevel& F = mg.lv[0];
 	const int n = (int)h->g.n;
-	clean_rhs_kernel<<<cell_grid(h->g), dim3(BX, BY, 1), 0, st>>>(h->g, h->state[h->cur], F.f, F.r);
+	const dim3 rgrid((h->g.nx + BX - 1) / BX, (h->g.ny + BY - 1) / BY, (h->g.nz + CLEAN_KZ - 1) / CLEAN_KZ);
+	const int nb = (int)(rgrid.x * rgrid.y * rgrid.z);
+	if ((size_t)2 * nb > mg.norm_part_len) return fail(h, PAMHD_E_DEVICE, "cleaning: partial-sum buffer too small");
+	clean_rhs_kernel<<<rgrid, dim3(BX, BY, 1), 0, st>>>(h->g, h->state[h->cur], F.f, mg.norm_part,
+		F.packed);
 	HIP_TRY(h, hipGetLastError());
-	double fl, dsum;
-	HIP_TRY(h, mg_sum(mg, st, F.r, n, 0, 0.0, &fl));
-	HIP_TRY(h, mg_sum(mg, st, F.f, n, 2, 0.0, &dsum));
+	double sums[2];
+	HIP_TRY(h, mg_sum_pairs(mg, st, mg.norm_part, nb, sums));
+	const double dsum = sums[0], fl = sums[1];
 	const double atol = 8 * DBL_EPSILON * std::sqrt(fl);
 	double rel = 0;
 	int it = 0;
 	bool ok = true;
 	HIP_TRY(h, mg_solve(mg, st, tol, atol, max_cycles, &rel, &it, &ok));
 	project_kernel<<<cell_grid(h->g), dim3(BX, BY, 1), 0, st>>>(h->g, h->state[h->cur],
-		h->state[1 - h->cur], F.phi);
+		h->state[1 - h->cur], F.phi, F.packed);
 	HIP_TRY(h, hipGetLastError());
 	HIP_TRY(h, hipStreamSynchronize(st));
 	h->cur = 1 - h->cur;

@@ -12,7 +12,7 @@ restriction averages the children, prolongation is trilinear in the
 coarsened dimensions; V(2,2) cycles with 60 sweeps on the coarsest level.
 Every kernel computes each value in a fixed operation order, so results do
 not depend on the launch shape; the residual norms of the stopping test are
-deterministic two-stage reductions.
+deterministic one-pass reductions (block moments merged pairwise).
 */
 #ifndef PAMHD_POISSON_HPP
 #define PAMHD_POISSON_HPP
@@ -23,8 +23,20 @@ deterministic two-stage reductions.
 
 namespace pamhd_dev {
 
+/* index of cell (i, j, k) (in range) of a level: x fastest, or, on a level
+   smoothed red-black with an even x size (packed), the two colours
+   ((i + j + k) & 1) as separate halves, each x fastest over the colour's
+   cells, so that a colour pass reads and writes whole cache lines (the 7
+   neighbours of a cell are of the other colour) */
+__host__ __device__ inline int mg_index(int packed, int n0, int n1, int half_n, int i, int j, int k)
+{
+	if (!packed) return i + n0 * (j + n1 * k);
+	return ((i + j + k) & 1) * half_n + (i >> 1) + (n0 >> 1) * (j + n1 * k);
+}
+
 struct MgLevel {
 	int n[3];
+	int packed = 0;  /* colour-split layout (mg_index) */
 	double h[3];
 	double hh[3];    /* h * h */
 	double diag;     /* sum over active dimensions of 2 / (h * h) */
@@ -37,10 +49,13 @@ struct MgLevel {
 struct PoissonMG {
 	std::vector<MgLevel> lv;
 	double* partial = nullptr;      /* per-block partial sums */
+	double* norm_part = nullptr;    /* per-block residual moments of the stopping test */
+	size_t norm_part_len = 0;       /* doubles */
 	size_t partial_len = 0;
-	double* result = nullptr;       /* device: 1 double */
-	double* host_result = nullptr;  /* pinned */
+	double* result = nullptr;       /* device: 2 doubles */
+	double* host_result = nullptr;  /* pinned: 2 doubles */
 	int fused_smoother = 0;         /* 1: LDS-tiled fused red-black sweeps where possible */
+	int tail = -1;                  /* first of the bottom levels cycled in one block, or -1 */
 };
 
 /* allocates the level hierarchy for a dims[0] x dims[1] x dims[2] grid */
@@ -61,6 +76,10 @@ hipError_t mg_solve(PoissonMG& mg, hipStream_t st, double tol, double atol, int 
 hipError_t mg_sum(PoissonMG& mg, hipStream_t st, const double* a, int n, int mode,
 	double shift, double* out);
 
+/* out[0], out[1] = the sums of the even and odd entries of the nb pairs
+   (device memory, e.g. per-block partial sums in mg.norm_part); synchronises
+   the stream */
+hipError_t mg_sum_pairs(PoissonMG& mg, hipStream_t st, const double* pairs, int nb, double out[2]);
 /* launches one V-cycle on the hierarchy (exposed for timing) */
 void mg_vcycle(PoissonMG& mg, hipStream_t st, int level);
 
