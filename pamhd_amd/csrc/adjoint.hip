@@ -16,17 +16,27 @@ The forward step restated as a map U^n -> U^{n+1} (dt given):
                  later refits with vb_old = vb_new are the identity, and the
                  kinetic terms cancel exactly)
 The reverse step takes lambda^{n+1} (the loss's gradient with respect to
-U^{n+1}, same layout as the state) to lambda^n:
-  mu_kernel      mu_new = lambda_E vb_new / mu0, mu_flux = -lambda_E vb_flux / mu0
-  lface_kernel   G = lambda_face + (mu_new(c) + mu_new(c - e_d)) / 2
-  lflux_kernel   lambda_P of each face from the update (conserved and vb_flux)
-                 and the transposed 12-term CT sums over G
-  jac_kernel     lambda_L, lambda_R = J^T (dt lambda_P): the HLL Jacobian
-                 by forward-mode dual numbers, four seeded inputs per pass
-                 (four passes), in the same operation order as the forward
-  cell_kernel    lambda rho, m, E of each cell, lambda_vb (face-state and
-                 vb_flux parts)
-  face_kernel    lambda_face = G + (lambda_vb(c) + lambda_vb(c - e_d)) / 2
+U^{n+1}, same layout as the state) to lambda^n in four kernels, with
+20 n doubles of scratch in 2-D (about 22 n in 3-D):
+  pe_kernel      the forward step's six magnetic-flux (edge field) parts of
+                 the three faces of each cell: Pe (6 n); a thread per face
+  g_kernel       mu_new = lambda_E vb_new / mu0 at the cell and at its lower
+                 neighbours, G = lambda_face + (mu_new(c) + mu_new(c - e_d)) / 2
+                 (3 n), and mu_flux = -lambda_E vb_flux / mu0 (3 n); a thread
+                 per cell and direction
+  tile_kernel    per tile of 256 cells: lambda_P of each face owned by a cell
+                 from the update (conserved and vb_flux) and the transposed
+                 12-term CT sums over G, then lambda_L, lambda_R = J^T (dt
+                 lambda_P) (the HLL Jacobian by reverse accumulation, or by
+                 forward-mode dual numbers); lambda_R is added to the cell's
+                 accumulator, lambda_L goes to the left cell through LDS when
+                 that cell is in the tile and to a per-tile boundary buffer
+                 otherwise; the accumulators (lambda rho, m, E and lambda_vb,
+                 8 n) are written out (over Pe, which is dead by then)
+  final_kernel   adds the boundary buffer's terms, writes lambda rho, m, E and
+                 lambda_face = G + (lambda_vb(c) + lambda_vb(c - e_d)) / 2
+Each flux Jacobian is evaluated once per face; every sum has a fixed order,
+so a reverse step is reproducible bit for bit.
 */
 #include <hip/hip_runtime.h>
 
@@ -219,7 +229,8 @@ __device__ bool hll_adjoint(const double L[8], const double R[8], const Phys& ph
 	const double* U[2] = {L, R};
 	double* gU[2] = {gL, gR};
 	double p[2], v[2][3], cf[2], mag[2], sound[2], alfven[2] = {0, 0}, s2[2] = {0, 0};
-	double a2[2] = {0, 0}, ratio[2] = {0, 0}, rts[2] = {1, 1}, bsq[2], msq[2];
+	double a2[2] = {0, 0}, ratio[2] = {0, 0}, rts[2] = {1, 1}, bsq[2], msq[2], irho[2];
+	double sqmr[2] = {1, 1};
 	bool hasB[2];
 	for (int s = 0; s < 2; s++) {
 		const double* u = U[s];
@@ -228,6 +239,7 @@ __device__ bool hll_adjoint(const double L[8], const double R[8], const Phys& ph
 		p[s] = (u[NRJ] - 0.5 / u[RHO] * msq[s] - hinv * bsq[s]) * gm1;
 		if (!(p[s] > 0) || !__builtin_isnormal(p[s])) return false;
 		const double inv = 1 / u[RHO];
+		irho[s] = inv;
 		for (int i = 0; i < 3; i++) v[s][i] = u[M0 + i] * inv;
 		mag[s] = __builtin_sqrt(bsq[s]);
 		sound[s] = __builtin_sqrt(gamma * p[s] / u[RHO]);
@@ -235,7 +247,8 @@ __device__ bool hll_adjoint(const double L[8], const double R[8], const Phys& ph
 		if (!hasB[s]) {
 			cf[s] = sound[s];
 		} else {
-			alfven[s] = mag[s] / __builtin_sqrt(mu0 * u[RHO]);
+			sqmr[s] = __builtin_sqrt(mu0 * u[RHO]);
+			alfven[s] = mag[s] / sqmr[s];
 			s2[s] = sound[s] * sound[s];
 			a2[s] = alfven[s] * alfven[s];
 			ratio[s] = u[B0] / mag[s];
@@ -280,7 +293,11 @@ __device__ bool hll_adjoint(const double L[8], const double R[8], const Phys& ph
 		gFR[q] = gF[q] * (0.5 - w);
 		gw += gF[q] * (fl - fr);
 	}
-	const double gsum = gw / (2 * den), gden = -gw * sum / (2 * den * den);
+	/* the reverse sweep multiplies by reciprocals computed once (1 / rho is
+	   the forward's own); each product differs from the quotient by at most
+	   an ulp */
+	const double i2den = 0.5 / den;
+	const double gsum = gw * i2den, gden = -gw * sum * i2den / den;
 	double gbp = gsum + gden, gbm = gsum - gden;
 	for (int q = 0; q < 8; q++) {
 		if (q == B0) continue;
@@ -330,38 +347,42 @@ __device__ bool hll_adjoint(const double L[8], const double R[8], const Phys& ph
 	for (int s = 0; s < 2; s++) {
 		const double* u = U[s];
 		double* gu = gU[s];
-		const double rho = u[RHO];
+		const double ir = irho[s];
 		double gsound = 0, gmag = 0;
 		if (!hasB[s]) {
 			gsound += gcf[s];
 		} else {
-			const double gQ = gcf[s] / (2 * cf[s]);
+			const double gQ = gcf[s] * 0.5 / cf[s];
 			double gs2 = 0.5 * gQ, ga2 = 0.5 * gQ;
-			const double gts = 0.5 * gQ / (2 * rts[s]);
+			const double gts = 0.25 * gQ / rts[s];
 			const double r2 = ratio[s] * ratio[s];
 			gs2 += gts * (2 * s2[s] + 2 * a2[s] * (1 - 2 * r2));
 			ga2 += gts * (2 * a2[s] + 2 * s2[s] * (1 - 2 * r2));
 			const double gratio = gts * 2 * s2[s] * a2[s] * (-4 * ratio[s]);
 			gsound += 2 * sound[s] * gs2;
 			const double galf = 2 * alfven[s] * ga2;
-			gu[B0] += gratio / mag[s];
-			gmag -= gratio * u[B0] / (mag[s] * mag[s]);
-			gmag += galf / __builtin_sqrt(mu0 * rho);
-			gu[RHO] -= 0.5 * alfven[s] / rho * galf;
-			for (int i = 0; i < 3; i++) gu[B0 + i] += gmag * u[B0 + i] / mag[s];
+			const double imag = 1 / mag[s];
+			gu[B0] += gratio * imag;
+			gmag -= gratio * u[B0] * imag * imag;
+			gmag += galf / sqmr[s];
+			gu[RHO] -= 0.5 * alfven[s] * ir * galf;
+			const double gmi = gmag * imag;
+			for (int i = 0; i < 3; i++) gu[B0 + i] += gmi * u[B0 + i];
 		}
-		gp[s] += gsound * gamma / (2 * rho * sound[s]);
-		gu[RHO] -= gsound * sound[s] / (2 * rho);
+		const double gsr = gsound / sound[s] * 0.5 * ir;
+		gp[s] += gsr * gamma;
+		gu[RHO] -= gsound * sound[s] * 0.5 * ir;
 		for (int i = 0; i < 3; i++) {
-			gu[M0 + i] += gv[s][i] / rho;
-			gu[RHO] -= gv[s][i] * u[M0 + i] / (rho * rho);
+			gu[M0 + i] += gv[s][i] * ir;
+			gu[RHO] -= gv[s][i] * v[s][i] * ir;
 		}
-		gu[NRJ] += gp[s] * gm1;
+		const double gpg = gp[s] * gm1;
+		gu[NRJ] += gpg;
 		for (int i = 0; i < 3; i++) {
-			gu[M0 + i] -= gp[s] * gm1 * u[M0 + i] / rho;
-			gu[B0 + i] -= gp[s] * gm1 * 2 * hinv * u[B0 + i];
+			gu[M0 + i] -= gpg * v[s][i];
+			gu[B0 + i] -= gpg * 2 * hinv * u[B0 + i];
 		}
-		gu[RHO] += gp[s] * gm1 * 0.5 * msq[s] / (rho * rho);
+		gu[RHO] += gpg * 0.5 * msq[s] * ir * ir;
 	}
 	return true;
 }
@@ -373,7 +394,43 @@ struct AdjGrid {
 	Phys ph;
 };
 
-__device__ __forceinline__ int wrapi(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+/* tiles of tile_kernel: tx x ty x tz cells (256), ntx x nty x ntz of them;
+   a tile keeps boundary slots for the faces of its lower x, y and z layers
+   whose left cell lies in another tile (none in a dimension one tile spans) */
+struct AdjTiles {
+	int tx, ty, tz;
+	int ntx, nty, ntz;
+	int sx, sy, sz, slots;
+};
+
+AdjTiles tiles_for(int nx, int ny, int nz)
+{
+	AdjTiles T;
+	T.tx = 64;
+	T.ty = nz > 1 ? 2 : 4;
+	T.tz = nz > 1 ? 2 : 1;
+	T.ntx = (nx + T.tx - 1) / T.tx;
+	T.nty = (ny + T.ty - 1) / T.ty;
+	T.ntz = (nz + T.tz - 1) / T.tz;
+	T.sx = T.ntx > 1 ? T.ty * T.tz : 0;
+	T.sy = T.nty > 1 ? T.tx * T.tz : 0;
+	T.sz = T.ntz > 1 ? T.tx * T.ty : 0;
+	T.slots = T.sx + T.sy + T.sz;
+	return T;
+}
+
+constexpr int TILE_CELLS = 256;
+
+/* i in [-2n, 2n) wrapped into [0, n): new_face offsets a neighbour's
+   coordinates by one and face_dB by one more, and n may be 1 */
+__device__ __forceinline__ int wrapi(int i, int n)
+{
+	if (i < 0) i += n;
+	if (i < 0) i += n;
+	if (i >= n) i -= n;
+	if (i >= n) i -= n;
+	return i;
+}
 
 __device__ __forceinline__ int cell_at(const AdjGrid& g, int i, int j, int k)
 {
@@ -413,33 +470,41 @@ __device__ __forceinline__ void face_state(const AdjGrid& g, const double* __res
 	}
 }
 
-/* forward fluxes P (global frame, times dt) of every face */
-__global__ __launch_bounds__(ABX * ABY) void adj_flux_kernel(AdjGrid g, const double* __restrict__ s,
-	double* __restrict__ P)
+/* slot in Pe of the d-flux (times dt) of global field component 5 + b
+   (b != d): the face-frame component q = (b - d + 3) % 3 is 1 or 2 */
+__device__ __forceinline__ int pe_slot(int d, int b)
 {
-	int i, j, k;
-	if (!thread_cell(g, i, j, k)) return;
+	return 2 * d + (b - d + 3) % 3 - 1;
+}
+
+template <int D>
+__device__ __forceinline__ void pe_face(const AdjGrid& g, const double* __restrict__ s,
+	double* __restrict__ Pe, int i, int j, int k)
+{
 	const int c = cell_at(g, i, j, k);
-	for (int d = 0; d < 3; d++) {
-		double Ld[8], Rd[8];
-		face_state(g, s, d, i - (d == 0), j - (d == 1), k - (d == 2), Ld);
-		face_state(g, s, d, i, j, k, Rd);
-		Dual<1> L[8], R[8], F[8];
-		for (int q = 0; q < 8; q++) {
-			L[q] = cst<1>(Ld[q]);
-			R[q] = cst<1>(Rd[q]);
-		}
-		double Fg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-		if (hll_dual<1>(L, R, g.ph, F)) {
-			Fg[RHO] = F[RHO].v;
-			Fg[NRJ] = F[NRJ].v;
-			for (int q = 0; q < 3; q++) {
-				Fg[M0 + (d + q) % 3] = F[M0 + q].v;
-				Fg[B0 + (d + q) % 3] = F[B0 + q].v;
-			}
-		}
-		for (int q = 0; q < 8; q++) P[(d * 8 + q) * g.n + c] = g.dt * Fg[q];
-	}
+	const double bg[3] = {0, 0, 0};
+	double Ld[8], Rd[8], F[8], sp;
+	face_state(g, s, D, i - (D == 0), j - (D == 1), k - (D == 2), Ld);
+	face_state(g, s, D, i, j, k, Rd);
+	const bool ok = flux_hll<double>(Ld, Rd, bg, g.ph, F, sp) == ERR_OK;
+	for (int q = 1; q < 3; q++) Pe[(2 * D + q - 1) * g.n + c] = g.dt * (ok ? F[B0 + q] : 0.0);
+}
+
+/* the forward step's magnetic fluxes (global frame, times dt) of every face:
+   the normal component's flux is zero, the other two go to Pe; a thread
+   per face (blockIdx.z = k + nz d) */
+__global__ __launch_bounds__(ABX * ABY) void adj_pe_kernel(AdjGrid g, const double* __restrict__ s,
+	double* __restrict__ Pe)
+{
+	const int i = blockIdx.x * ABX + threadIdx.x;
+	const int j = blockIdx.y * ABY + threadIdx.y;
+	const int k = blockIdx.z % g.nz, d = blockIdx.z / g.nz;
+	if (i >= g.nx || j >= g.ny) return;
+	/* d is uniform in a block: one branch per block, and inside it the
+	   rotation and the tables resolve at compile time */
+	if (d == 0) pe_face<0>(g, s, Pe, i, j, k);
+	else if (d == 1) pe_face<1>(g, s, Pe, i, j, k);
+	else pe_face<2>(g, s, Pe, i, j, k);
 }
 
 /*
@@ -465,20 +530,20 @@ constexpr CtTerm CT_TERMS[36] = {
 constexpr signed char CT_EMF[6][4] = {
 	{0, 6, +1, 2}, {0, 7, -1, 1}, {1, 7, +1, 0}, {1, 5, -1, 2}, {2, 6, -1, 0}, {2, 5, +1, 1}};
 
-__device__ __forceinline__ double emf(const AdjGrid& g, const double* __restrict__ P, int e, int c)
+__device__ __forceinline__ double emf(const AdjGrid& g, const double* __restrict__ Pe, int e, int c)
 {
 	const int pd = CT_EMF[e][0], pq = CT_EMF[e][1], sg = CT_EMF[e][2], ld = CT_EMF[e][3];
-	return sg * (P[(pd * 8 + pq) * g.n + c] * g.len[ld] / 4);
+	return sg * (Pe[pe_slot(pd, pq - 5) * g.n + c] * g.len[ld] / 4);
 }
 
 /* dB of the t-face owned by (i, j, k) */
-__device__ double face_dB(const AdjGrid& g, const double* __restrict__ P, int t, int i, int j, int k)
+__device__ double face_dB(const AdjGrid& g, const double* __restrict__ Pe, int t, int i, int j, int k)
 {
 	double r = 0;
 #pragma unroll
 	for (int q = 12 * t; q < 12 * t + 12; q++) {
 		const CtTerm& T = CT_TERMS[q];
-		r += T.sign * emf(g, P, T.e, nb(g, i, j, k, T.o0, T.o1, T.o2));
+		r += T.sign * emf(g, Pe, T.e, nb(g, i, j, k, T.o0, T.o1, T.o2));
 	}
 	return r;
 }
@@ -488,102 +553,106 @@ __device__ __forceinline__ double area(const AdjGrid& g, int t)
 	return g.len[(t + 1) % 3] * g.len[(t + 2) % 3];
 }
 
-__global__ __launch_bounds__(ABX * ABY) void adj_mu_kernel(AdjGrid g, const double* __restrict__ s,
-	const double* __restrict__ P, const double* __restrict__ lam, double* __restrict__ mu)
+/* new d-face value (after CT) of the face owned by (i, j, k) */
+__device__ __forceinline__ double new_face(const AdjGrid& g, const double* __restrict__ s,
+	const double* __restrict__ Pe, int d, int i, int j, int k)
 {
-	int i, j, k;
-	if (!thread_cell(g, i, j, k)) return;
+	return s[(5 + d) * g.n + cell_at(g, i, j, k)] + face_dB(g, Pe, d, i, j, k) / area(g, d);
+}
+
+template <int D>
+__device__ __forceinline__ void g_face(const AdjGrid& g, const double* __restrict__ s,
+	const double* __restrict__ Pe, const double* __restrict__ lam, double* __restrict__ G,
+	double* __restrict__ MU, int i, int j, int k)
+{
 	const int n = g.n;
 	const int c = cell_at(g, i, j, k);
 	const double lE = lam[4 * n + c];
+	const int di = D == 0, dj = D == 1, dk = D == 2;
+	const int cp = nb(g, i, j, k, di, dj, dk), cm = nb(g, i, j, k, -di, -dj, -dk);
+	const double f0 = new_face(g, s, Pe, D, i, j, k);
+	const double f1 = new_face(g, s, Pe, D, i + di, j + dj, k + dk);
+	const double fm = new_face(g, s, Pe, D, i - di, j - dj, k - dk);
+	const double mu_c = lE * (0.5 * (f0 + f1)) * g.ph.inv_mu0;
+	const double mu_m = lam[4 * n + cm] * (0.5 * (fm + f0)) * g.ph.inv_mu0;
+	G[D * n + c] = lam[(5 + D) * n + c] + 0.5 * mu_c + 0.5 * mu_m;
+	double vb_flux = 0.5 * (s[(5 + D) * n + c] + s[(5 + D) * n + cp]);
+	for (int e = 0; e < 3; e++) {
+		if (e == D) continue;   /* the normal component has no flux */
+		const int ce = nb(g, i, j, k, e == 0, e == 1, e == 2);
+		const int sl = pe_slot(e, D);
+		vb_flux += (Pe[sl * n + c] - Pe[sl * n + ce]) / g.len[e];
+	}
+	MU[D * n + c] = -lE * vb_flux * g.ph.inv_mu0;
+}
+
+/* G = lambda_face + (mu_new(c) + mu_new(c - e_d)) / 2 and
+   mu_flux = -lambda_E vb_flux / mu0 of each cell, a thread per cell and
+   direction d (blockIdx.z = k + nz d) */
+__global__ __launch_bounds__(ABX * ABY) void adj_g_kernel(AdjGrid g, const double* __restrict__ s,
+	const double* __restrict__ Pe, const double* __restrict__ lam, double* __restrict__ G,
+	double* __restrict__ MU)
+{
+	const int i = blockIdx.x * ABX + threadIdx.x;
+	const int j = blockIdx.y * ABY + threadIdx.y;
+	const int k = blockIdx.z % g.nz, d = blockIdx.z / g.nz;
+	if (i >= g.nx || j >= g.ny) return;
+	if (d == 0) g_face<0>(g, s, Pe, lam, G, MU, i, j, k);
+	else if (d == 1) g_face<1>(g, s, Pe, lam, G, MU, i, j, k);
+	else g_face<2>(g, s, Pe, lam, G, MU, i, j, k);
+}
+
+/* lambda of the flux of the d-face owned by (i, j, k), global frame */
+__device__ __forceinline__ void face_lambda(const AdjGrid& g, const double* __restrict__ lam,
+	const double* __restrict__ MU, const double* __restrict__ G, int d, int i, int j, int k,
+	double lp[8])
+{
+	const int n = g.n;
+	const int c = cell_at(g, i, j, k);
+	const int cm = nb(g, i, j, k, -(d == 0), -(d == 1), -(d == 2));
+	const double il = 1 / g.len[d];
+	lp[RHO] = (lam[c] - lam[cm]) * il;
+	for (int q = 0; q < 3; q++) lp[M0 + q] = (lam[(1 + q) * n + c] - lam[(1 + q) * n + cm]) * il;
+	lp[NRJ] = (lam[4 * n + c] - lam[4 * n + cm]) * il;
+	for (int q = 0; q < 3; q++) lp[B0 + q] = (MU[q * n + c] - MU[q * n + cm]) * il;
+	/* transposed edge sums: E_e(c) enters face t of cell c - offset */
 #pragma unroll
-	for (int d = 0; d < 3; d++) {
-		const int cp = nb(g, i, j, k, d == 0, d == 1, d == 2);
-		const double f0 = s[(5 + d) * n + c] + face_dB(g, P, d, i, j, k) / area(g, d);
-		const double f1 = s[(5 + d) * n + cp]
-			+ face_dB(g, P, d, i + (d == 0), j + (d == 1), k + (d == 2)) / area(g, d);
-		const double vb_new = 0.5 * (f0 + f1);
-		double vb_flux = 0.5 * (s[(5 + d) * n + c] + s[(5 + d) * n + cp]);
-		for (int e = 0; e < 3; e++) {
-			const int ce = nb(g, i, j, k, e == 0, e == 1, e == 2);
-			vb_flux += (P[(e * 8 + 5 + d) * n + c] - P[(e * 8 + 5 + d) * n + ce]) / g.len[e];
+	for (int e = 0; e < 6; e++) {
+		if (CT_EMF[e][0] != d) continue;
+		double lE = 0;
+#pragma unroll
+		for (int q = 0; q < 36; q++) {
+			const CtTerm& T = CT_TERMS[q];
+			if (T.e != e) continue;
+			const int tc = nb(g, i, j, k, -T.o0, -T.o1, -T.o2);
+			lE += T.sign * G[T.t * n + tc] / area(g, T.t);
 		}
-		mu[d * n + c] = lE * vb_new * g.ph.inv_mu0;
-		mu[(3 + d) * n + c] = -lE * vb_flux * g.ph.inv_mu0;
+		lp[CT_EMF[e][1]] += CT_EMF[e][2] * lE * g.len[CT_EMF[e][3]] / 4;
 	}
 }
 
-__global__ __launch_bounds__(ABX * ABY) void adj_lface_kernel(AdjGrid g, const double* __restrict__ lam,
-	const double* __restrict__ mu, double* __restrict__ G)
+/* J^T (dt lambda_P) of the d-face owned by (i, j, k) for its left and right
+   cells, global frame */
+template <bool REVERSE>
+__device__ __forceinline__ void face_jt(const AdjGrid& g, const double* __restrict__ s,
+	const double lp[8], int d, int i, int j, int k, double wl[8], double wr[8])
 {
-	int i, j, k;
-	if (!thread_cell(g, i, j, k)) return;
-	const int n = g.n;
-	const int c = cell_at(g, i, j, k);
-	for (int d = 0; d < 3; d++) {
-		const int cm = nb(g, i, j, k, -(d == 0), -(d == 1), -(d == 2));
-		G[d * n + c] = lam[(5 + d) * n + c] + 0.5 * mu[d * n + c] + 0.5 * mu[d * n + cm];
+	double Ld[8], Rd[8], lF[8], out[2][8];
+	face_state(g, s, d, i - (d == 0), j - (d == 1), k - (d == 2), Ld);
+	face_state(g, s, d, i, j, k, Rd);
+	/* lambda of the face-frame flux: dt * lambda_P, rotated into the frame */
+	lF[RHO] = g.dt * lp[RHO];
+	lF[NRJ] = g.dt * lp[NRJ];
+	for (int q = 0; q < 3; q++) {
+		lF[M0 + q] = g.dt * lp[M0 + (d + q) % 3];
+		lF[B0 + q] = g.dt * lp[B0 + (d + q) % 3];
 	}
-}
-
-/* lambda_P of the three faces owned by the thread's cell */
-__global__ __launch_bounds__(ABX * ABY) void adj_lflux_kernel(AdjGrid g, const double* __restrict__ lam,
-	const double* __restrict__ mu, const double* __restrict__ G, double* __restrict__ LP)
-{
-	int i, j, k;
-	if (!thread_cell(g, i, j, k)) return;
-	const int n = g.n;
-	const int c = cell_at(g, i, j, k);
-#pragma unroll
-	for (int d = 0; d < 3; d++) {
-		const int cm = nb(g, i, j, k, -(d == 0), -(d == 1), -(d == 2));
-		const double il = 1 / g.len[d];
-		double lp[8];
-		lp[RHO] = (lam[c] - lam[cm]) * il;
-		for (int q = 0; q < 3; q++) lp[M0 + q] = (lam[(1 + q) * n + c] - lam[(1 + q) * n + cm]) * il;
-		lp[NRJ] = (lam[4 * n + c] - lam[4 * n + cm]) * il;
-		for (int q = 0; q < 3; q++) lp[B0 + q] = (mu[(3 + q) * n + c] - mu[(3 + q) * n + cm]) * il;
-		/* transposed edge sums: E_e(c) enters face t of cell c - offset */
-#pragma unroll
-		for (int e = 0; e < 6; e++) {
-			if (CT_EMF[e][0] != d) continue;
-			double lE = 0;
-#pragma unroll
-			for (int q = 0; q < 36; q++) {
-				const CtTerm& T = CT_TERMS[q];
-				if (T.e != e) continue;
-				const int tc = nb(g, i, j, k, -T.o0, -T.o1, -T.o2);
-				lE += T.sign * G[T.t * n + tc] / area(g, T.t);
-			}
-			lp[CT_EMF[e][1]] += CT_EMF[e][2] * lE * g.len[CT_EMF[e][3]] / 4;
-		}
-		for (int q = 0; q < 8; q++) LP[(d * 8 + q) * n + c] = lp[q];
-	}
-}
-
-/* lambda_L, lambda_R (global frame) of the faces owned by the cell */
-__global__ __launch_bounds__(ABX * ABY) void adj_jac_kernel(AdjGrid g, const double* __restrict__ s,
-	const double* __restrict__ LP, double* __restrict__ W)
-{
-	int i, j, k;
-	if (!thread_cell(g, i, j, k)) return;
-	const int n = g.n;
-	const int c = cell_at(g, i, j, k);
-	for (int d = 0; d < 3; d++) {
-		double Ld[8], Rd[8], lF[8];
-		face_state(g, s, d, i - (d == 0), j - (d == 1), k - (d == 2), Ld);
-		face_state(g, s, d, i, j, k, Rd);
-		/* lambda of the face-frame flux: dt * lambda_P, rotated into the frame */
-		lF[RHO] = g.dt * LP[(d * 8 + RHO) * n + c];
-		lF[NRJ] = g.dt * LP[(d * 8 + NRJ) * n + c];
-		for (int q = 0; q < 3; q++) {
-			lF[M0 + q] = g.dt * LP[(d * 8 + M0 + (d + q) % 3) * n + c];
-			lF[B0 + q] = g.dt * LP[(d * 8 + B0 + (d + q) % 3) * n + c];
-		}
+	if constexpr (REVERSE) {
+		hll_adjoint(Ld, Rd, g.ph, lF, out[0], out[1]);
+	} else {
 #pragma unroll
 		for (int side = 0; side < 2; side++) {
 			/* J^T lambda_F for the 8 inputs of this side, JN inputs per pass */
-			double out[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
 			for (int a0 = 0; a0 < 8; a0 += JN) {
 				Dual<JN> L[8], R[8], F[8];
@@ -595,95 +664,209 @@ __global__ __launch_bounds__(ABX * ABY) void adj_jac_kernel(AdjGrid g, const dou
 					if (side) R[a0 + q].d[q] = 1;
 					else L[a0 + q].d[q] = 1;
 				}
-				if (hll_dual<JN>(L, R, g.ph, F)) {
-					for (int a = 0; a < JN; a++) {
-						double t = 0;
-						for (int q = 0; q < 8; q++) t += lF[q] * F[q].d[a];
-						out[a0 + a] = t;
-					}
+				const bool ok = hll_dual<JN>(L, R, g.ph, F);
+				for (int a = 0; a < JN; a++) {
+					double t = 0;
+					if (ok) for (int q = 0; q < 8; q++) t += lF[q] * F[q].d[a];
+					out[side][a0 + a] = t;
 				}
 			}
-			/* back to the global frame: rho, m (3), E, B (3) */
-			double* w = W + (size_t)(d * 16 + side * 8) * n;
-			w[RHO * n + c] = out[RHO];
-			w[NRJ * n + c] = out[NRJ];
-			for (int q = 0; q < 3; q++) {
-				w[(M0 + (d + q) % 3) * n + c] = out[M0 + q];
-				w[(B0 + (d + q) % 3) * n + c] = out[B0 + q];
-			}
 		}
 	}
-}
-
-/* the same W by hll_adjoint */
-__global__ __launch_bounds__(ABX * ABY) void adj_jacr_kernel(AdjGrid g, const double* __restrict__ s,
-	const double* __restrict__ LP, double* __restrict__ W)
-{
-	int i, j, k;
-	if (!thread_cell(g, i, j, k)) return;
-	const int n = g.n;
-	const int c = cell_at(g, i, j, k);
-	for (int d = 0; d < 3; d++) {
-		double Ld[8], Rd[8], lF[8], out[2][8];
-		face_state(g, s, d, i - (d == 0), j - (d == 1), k - (d == 2), Ld);
-		face_state(g, s, d, i, j, k, Rd);
-		lF[RHO] = g.dt * LP[(d * 8 + RHO) * n + c];
-		lF[NRJ] = g.dt * LP[(d * 8 + NRJ) * n + c];
+	/* back to the global frame: rho, m (3), E, B (3) */
+	double* w[2] = {wl, wr};
+	for (int side = 0; side < 2; side++) {
+		w[side][RHO] = out[side][RHO];
+		w[side][NRJ] = out[side][NRJ];
 		for (int q = 0; q < 3; q++) {
-			lF[M0 + q] = g.dt * LP[(d * 8 + M0 + (d + q) % 3) * n + c];
-			lF[B0 + q] = g.dt * LP[(d * 8 + B0 + (d + q) % 3) * n + c];
-		}
-		hll_adjoint(Ld, Rd, g.ph, lF, out[0], out[1]);
-		for (int side = 0; side < 2; side++) {
-			double* w = W + (size_t)(d * 16 + side * 8) * n;
-			w[RHO * n + c] = out[side][RHO];
-			w[NRJ * n + c] = out[side][NRJ];
-			for (int q = 0; q < 3; q++) {
-				w[(M0 + (d + q) % 3) * n + c] = out[side][M0 + q];
-				w[(B0 + (d + q) % 3) * n + c] = out[side][B0 + q];
-			}
+			w[side][M0 + (d + q) % 3] = out[side][M0 + q];
+			w[side][B0 + (d + q) % 3] = out[side][B0 + q];
 		}
 	}
 }
 
-/* lambda^n of rho, m, E (in place) and lambda_vb of each cell */
-__global__ __launch_bounds__(ABX * ABY) void adj_cell_kernel(AdjGrid g, double* __restrict__ lam,
-	const double* __restrict__ mu, const double* __restrict__ W, double* __restrict__ V)
+__device__ __forceinline__ void tile_of_cell(const AdjTiles& T, int i, int j, int k, int& ti, int& tj,
+	int& tk, int& li, int& lj, int& lk)
 {
-	int i, j, k;
-	if (!thread_cell(g, i, j, k)) return;
+	ti = i / T.tx;
+	tj = j / T.ty;
+	tk = k / T.tz;
+	li = i - ti * T.tx;
+	lj = j - tj * T.ty;
+	lk = k - tk * T.tz;
+}
+
+/* boundary slot (in its tile) of the d-face owned by a cell on the tile's
+   lower d layer */
+__device__ __forceinline__ int bnd_slot(const AdjTiles& T, int d, int li, int lj, int lk)
+{
+	if (d == 0) return lj + T.ty * lk;
+	if (d == 1) return T.sx + li + T.tx * lk;
+	return T.sx + T.sy + li + T.tx * lj;
+}
+
+/* whether the left cell of a d-face owned by a cell at local (li, lj, lk)
+   is in the same tile */
+__device__ __forceinline__ bool left_in_tile(const AdjTiles& T, int d, int li, int lj, int lk)
+{
+	if (d == 0) return li > 0 || T.ntx == 1;
+	if (d == 1) return lj > 0 || T.nty == 1;
+	return lk > 0 || T.ntz == 1;
+}
+
+/* the d-face owned by cell (i, j, k) at local (li, lj, lk) of tile `tile`:
+   lambda_R into the cell's accumulator (in LDS, which keeps registers for
+   the Jacobian), lambda_L to the left cell's LDS slot or to the boundary
+   buffer */
+template <bool REVERSE, int D>
+__device__ __forceinline__ void tile_face(const AdjGrid& g, const AdjTiles& T,
+	const double* __restrict__ s, const double* __restrict__ lam, const double* __restrict__ MU,
+	const double* __restrict__ G, double* __restrict__ BND, int tile, int i, int j, int k,
+	int li, int lj, int lk, double (*acc_s)[TILE_CELLS], double (*wl_s)[8][TILE_CELLS])
+{
+	double lp[8], wl[8], wr[8];
+	face_lambda(g, lam, MU, G, D, i, j, k, lp);
+	face_jt<REVERSE>(g, s, lp, D, i, j, k, wl, wr);
+	const int t = threadIdx.x;
+	for (int q = 0; q < 8; q++) acc_s[q][t] += wr[q];
+	if (left_in_tile(T, D, li, lj, lk)) {
+		/* the left cell's local index (wrapping inside a tile that spans the
+		   dimension) */
+		int mi = li - (D == 0), mj = lj - (D == 1), mk = lk - (D == 2);
+		if (mi < 0) mi = g.nx - 1;
+		if (mj < 0) mj = g.ny - 1;
+		if (mk < 0) mk = g.nz - 1;
+		const int lm = mi + T.tx * (mj + T.ty * mk);
+		for (int q = 0; q < 8; q++) wl_s[D][q][lm] = wl[q];
+	} else {
+		double* b = BND + ((size_t)tile * T.slots + bnd_slot(T, D, li, lj, lk)) * 8;
+		for (int q = 0; q < 8; q++) b[q] = wl[q];
+	}
+}
+
+template <bool REVERSE>
+__global__ __launch_bounds__(TILE_CELLS) void adj_tile_kernel(AdjGrid g, AdjTiles T,
+	const double* __restrict__ s, const double* __restrict__ lam, const double* __restrict__ MU,
+	const double* __restrict__ G, double* __restrict__ OUT, double* __restrict__ BND)
+{
+	__shared__ double wl_s[3][8][TILE_CELLS];
+	__shared__ double acc_s[8][TILE_CELLS];
+	const int t = threadIdx.x;
+	const int li = t % T.tx, lj = (t / T.tx) % T.ty, lk = t / (T.tx * T.ty);
+	const int i = blockIdx.x * T.tx + li, j = blockIdx.y * T.ty + lj, k = blockIdx.z * T.tz + lk;
+	const bool valid = i < g.nx && j < g.ny && k < g.nz;
 	const int n = g.n;
-	const int c = cell_at(g, i, j, k);
+	const int tile = blockIdx.x + T.ntx * (blockIdx.y + T.nty * blockIdx.z);
+	int c = 0;
+	if (valid) {
+		c = cell_at(g, i, j, k);
+		for (int q = 0; q < 5; q++) acc_s[q][t] = lam[q * n + c];
+		for (int q = 0; q < 3; q++) acc_s[5 + q][t] = MU[q * n + c];
+		tile_face<REVERSE, 0>(g, T, s, lam, MU, G, BND, tile, i, j, k, li, lj, lk, acc_s, wl_s);
+		/* one face at a time: no loads of the next face hoisted above this one */
+		__asm__ volatile("" ::: "memory");
+		tile_face<REVERSE, 1>(g, T, s, lam, MU, G, BND, tile, i, j, k, li, lj, lk, acc_s, wl_s);
+		__asm__ volatile("" ::: "memory");
+		tile_face<REVERSE, 2>(g, T, s, lam, MU, G, BND, tile, i, j, k, li, lj, lk, acc_s, wl_s);
+	}
+	__syncthreads();
+	if (!valid) return;
 	double acc[8];
-	for (int q = 0; q < 5; q++) acc[q] = lam[q * n + c];
-	for (int q = 0; q < 3; q++) acc[5 + q] = mu[(3 + q) * n + c];
+	for (int q = 0; q < 8; q++) acc[q] = acc_s[q][t];
 	for (int d = 0; d < 3; d++) {
-		const int cp = nb(g, i, j, k, d == 0, d == 1, d == 2);
-		const double* wr = W + (size_t)(d * 16 + 8) * n;   /* the cell is R of its own face */
-		const double* wl = W + (size_t)(d * 16) * n;       /* and L of the face owned by cp */
-		for (int q = 0; q < 8; q++) acc[q] += wr[q * n + c] + wl[q * n + cp];
+		/* the face owned by the right neighbour: its left cell is this one */
+		int pi = li + (d == 0), pj = lj + (d == 1), pk = lk + (d == 2);
+		const int ip = i + (d == 0), jp = j + (d == 1), kp = k + (d == 2);
+		if (ip == g.nx) pi = 0;
+		if (jp == g.ny) pj = 0;
+		if (kp == g.nz) pk = 0;
+		if (pi >= T.tx || pj >= T.ty || pk >= T.tz) continue;   /* in another tile */
+		if ((d == 0 && ip == g.nx && T.ntx > 1) || (d == 1 && jp == g.ny && T.nty > 1)
+			|| (d == 2 && kp == g.nz && T.ntz > 1)) continue;
+		for (int q = 0; q < 8; q++) acc[q] += wl_s[d][q][t];
 	}
-	for (int q = 0; q < 5; q++) lam[q * n + c] = acc[q];
-	for (int q = 0; q < 3; q++) V[q * n + c] = acc[5 + q];
+	for (int q = 0; q < 8; q++) OUT[q * n + c] = acc[q];
 }
 
-__global__ __launch_bounds__(ABX * ABY) void adj_face_kernel(AdjGrid g, double* __restrict__ lam,
-	const double* __restrict__ G, const double* __restrict__ V)
+/* the boundary buffer's terms of cell (i, j, k) (in range): for each d
+   whose right neighbour lies in another tile (at that tile's lower d layer),
+   the slot of the neighbour's d-face, else null */
+__device__ __forceinline__ void bnd_terms(const AdjGrid& g, const AdjTiles& T,
+	const double* __restrict__ BND, int i, int j, int k, const double* b[3])
+{
+	const int ip = i + 1 == g.nx ? 0 : i + 1, jp = j + 1 == g.ny ? 0 : j + 1,
+		kp = k + 1 == g.nz ? 0 : k + 1;
+	const int tx = i / T.tx, ty = j / T.ty, tz = k / T.tz;
+	const int li = i - tx * T.tx, lj = j - ty * T.ty, lk = k - tz * T.tz;
+	b[0] = b[1] = b[2] = nullptr;
+	if (T.ntx > 1 && ip % T.tx == 0) {
+		const int tile = ip / T.tx + T.ntx * (ty + T.nty * tz);
+		b[0] = BND + ((size_t)tile * T.slots + bnd_slot(T, 0, 0, lj, lk)) * 8;
+	}
+	if (T.nty > 1 && jp % T.ty == 0) {
+		const int tile = tx + T.ntx * (jp / T.ty + T.nty * tz);
+		b[1] = BND + ((size_t)tile * T.slots + bnd_slot(T, 1, li, 0, lk)) * 8;
+	}
+	if (T.ntz > 1 && kp % T.tz == 0) {
+		const int tile = tx + T.ntx * (ty + T.nty * (kp / T.tz));
+		b[2] = BND + ((size_t)tile * T.slots + bnd_slot(T, 2, li, lj, 0)) * 8;
+	}
+}
+
+/* component q of cell (i, j, k)'s accumulator plus its boundary terms in
+   the order d = 0, 1, 2 */
+__device__ __forceinline__ double final_acc(const AdjGrid& g, const AdjTiles& T,
+	const double* __restrict__ OUT, const double* __restrict__ BND, int i, int j, int k, int q)
+{
+	const double* b[3];
+	bnd_terms(g, T, BND, i, j, k, b);
+	double a = OUT[q * g.n + cell_at(g, i, j, k)];
+	for (int d = 0; d < 3; d++) {
+		if (b[d]) a += b[d][q];
+	}
+	return a;
+}
+
+__global__ __launch_bounds__(ABX * ABY) void adj_final_kernel(AdjGrid g, AdjTiles T,
+	double* __restrict__ lam, const double* __restrict__ G, const double* __restrict__ OUT,
+	const double* __restrict__ BND)
 {
 	int i, j, k;
 	if (!thread_cell(g, i, j, k)) return;
 	const int n = g.n;
 	const int c = cell_at(g, i, j, k);
+	const double* b[3];
+	bnd_terms(g, T, BND, i, j, k, b);
+	double v[3];
+	for (int q = 0; q < 8; q++) {
+		double a = OUT[q * n + c];
+		for (int d = 0; d < 3; d++) {
+			if (b[d]) a += b[d][q];
+		}
+		if (q < 5) lam[q * n + c] = a;
+		else v[q - 5] = a;
+	}
 	for (int d = 0; d < 3; d++) {
-		const int cm = nb(g, i, j, k, -(d == 0), -(d == 1), -(d == 2));
-		lam[(5 + d) * n + c] = G[d * n + c] + 0.5 * V[d * n + c] + 0.5 * V[d * n + cm];
+		const int im = d == 0 ? (i == 0 ? g.nx - 1 : i - 1) : i;
+		const int jm = d == 1 ? (j == 0 ? g.ny - 1 : j - 1) : j;
+		const int km = d == 2 ? (k == 0 ? g.nz - 1 : k - 1) : k;
+		const double vm = final_acc(g, T, OUT, BND, im, jm, km, 5 + d);
+		lam[(5 + d) * n + c] = G[d * n + c] + 0.5 * v[d] + 0.5 * vm;
 	}
 }
 
 } // namespace
 
-/* scratch: 24 n (P) + 6 n (mu) + 3 n (G) + 24 n (lambda_P) + 48 n (W) + 3 n (V) */
-size_t adjoint_scratch_doubles(long long n) { return (size_t)108 * n; }
+/* scratch: G (3 n), mu_flux (3 n), then Pe (6 n) for the first two kernels
+   and the accumulators (8 n) with the tiles' boundary terms for the last two */
+size_t adjoint_scratch_doubles(int nx, int ny, int nz)
+{
+	const size_t n = (size_t)nx * ny * nz;
+	const AdjTiles T = tiles_for(nx, ny, nz);
+	const size_t bnd = (size_t)T.ntx * T.nty * T.ntz * T.slots * 8;
+	const size_t tail = 8 * n + bnd > 6 * n ? 8 * n + bnd : 6 * n;
+	return 6 * n + tail;
+}
 
 void adjoint_launch(int nx, int ny, int nz, const double len[3], double dt, const Phys& ph,
 	const double* state, double* lam, double* scratch, hipStream_t st, int reverse)
@@ -697,21 +880,20 @@ void adjoint_launch(int nx, int ny, int nz, const double len[3], double dt, cons
 	g.dt = dt;
 	g.ph = ph;
 	const size_t n = g.n;
-	double* P = scratch;
-	double* mu = P + 24 * n;
-	double* G = mu + 6 * n;
-	double* LP = G + 3 * n;
-	double* W = LP + 24 * n;
-	double* V = W + 48 * n;
+	const AdjTiles T = tiles_for(nx, ny, nz);
+	double* G = scratch;
+	double* MU = G + 3 * n;
+	double* Pe = MU + 3 * n;
+	double* OUT = Pe;          /* Pe is dead once G and mu_flux are made */
+	double* BND = OUT + 8 * n;
 	const dim3 grid((nx + ABX - 1) / ABX, (ny + ABY - 1) / ABY, nz), block(ABX, ABY, 1);
-	adj_flux_kernel<<<grid, block, 0, st>>>(g, state, P);
-	adj_mu_kernel<<<grid, block, 0, st>>>(g, state, P, lam, mu);
-	adj_lface_kernel<<<grid, block, 0, st>>>(g, lam, mu, G);
-	adj_lflux_kernel<<<grid, block, 0, st>>>(g, lam, mu, G, LP);
-	if (reverse) adj_jacr_kernel<<<grid, block, 0, st>>>(g, state, LP, W);
-	else adj_jac_kernel<<<grid, block, 0, st>>>(g, state, LP, W);
-	adj_cell_kernel<<<grid, block, 0, st>>>(g, lam, mu, W, V);
-	adj_face_kernel<<<grid, block, 0, st>>>(g, lam, G, V);
+	const dim3 tgrid(T.ntx, T.nty, T.ntz);
+	const dim3 fgrid((nx + ABX - 1) / ABX, (ny + ABY - 1) / ABY, 3 * nz);   /* a thread per face */
+	adj_pe_kernel<<<fgrid, block, 0, st>>>(g, state, Pe);
+	adj_g_kernel<<<fgrid, block, 0, st>>>(g, state, Pe, lam, G, MU);
+	if (reverse) adj_tile_kernel<true><<<tgrid, TILE_CELLS, 0, st>>>(g, T, state, lam, MU, G, OUT, BND);
+	else adj_tile_kernel<false><<<tgrid, TILE_CELLS, 0, st>>>(g, T, state, lam, MU, G, OUT, BND);
+	adj_final_kernel<<<grid, block, 0, st>>>(g, T, lam, G, OUT, BND);
 }
 
 } // namespace pamhd_dev

@@ -37,7 +37,7 @@ conserved update, constant-pressure energy refits).  In mixed precision the flux
 #include "swbox.hpp"
 
 namespace pamhd_dev {
-size_t adjoint_scratch_doubles(long long n);
+size_t adjoint_scratch_doubles(int nx, int ny, int nz);
 void adjoint_launch(int nx, int ny, int nz, const double len[3], double dt, const Phys& ph,
 	const double* state, double* lam, double* scratch, hipStream_t st, int reverse);
 }
@@ -4028,7 +4028,7 @@ int pamhd_adjoint_step(pamhd_handle* h, int solver, double dt, const double* sta
 	if (solver != HLL_ATHENA) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint is implemented for hll-athena");
 	if (h->precision != PAMHD_FP64) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint is FP64");
 	if (h->bg) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint has no background field");
-	if (!h->adj) HIP_TRY(h, hipMalloc(&h->adj, sizeof(double) * adjoint_scratch_doubles(h->g.n)));
+	if (!h->adj) HIP_TRY(h, hipMalloc(&h->adj, sizeof(double) * adjoint_scratch_doubles(h->g.nx, h->g.ny, h->g.nz)));
 	const double len[3] = {h->g.dx, h->g.dy, h->g.dz};
 	adjoint_launch(h->g.nx, h->g.ny, h->g.nz, len, dt, h->g.ph, state, lambda, h->adj,
 		(hipStream_t)stream, h->adj_reverse);
