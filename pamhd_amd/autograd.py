@@ -5,8 +5,10 @@ tensor U0 by len(dts) main-loop steps of the given lengths (a frozen time-step
 schedule) on `grid` and returns the final state.  Its backward runs the
 library's hand-written adjoint (pamhd_adjoint_step) step by step in reverse,
 with checkpointing: the forward keeps the state every `every` steps, and the
-backward recomputes the states of one segment at a time from its checkpoint.
-All calls are ordered on torch's current stream.
+backward recomputes the states of one segment at a time from its checkpoint,
+the previous segment's on a second stream while the current segment's
+reverse steps run.  The forward and the reverse steps are ordered on torch's
+current stream.
 """
 import torch
 
@@ -39,22 +41,54 @@ class MHDRollout(torch.autograd.Function):
     @staticmethod
     def backward(ctx, lam_out):
         grid, sid, dts, every = ctx.grid, ctx.sid, ctx.dts, ctx.every
-        stream = torch.cuda.current_stream(lam_out.device).cuda_stream
+        dev = lam_out.device
+        main = torch.cuda.current_stream(dev)
+        # the recomputation of the previous segment's states runs on a side
+        # stream while the current segment's reverse steps run on the main
+        # one: the two only share the checkpoints (read-only), and on a grid
+        # that does not fill the GPU alone they overlap
+        side = torch.cuda.Stream(device=dev)
         lam = lam_out.detach().to(torch.float64).contiguous().clone()
         nsteps = len(dts)
-        for seg in reversed(range(len(ctx.ckpts))):
+        nseg = len(ctx.ckpts)
+        bufs = [[torch.empty_like(ctx.ckpts[0]) for _ in range(max(every - 1, 0))] for _ in range(2)]
+        for b in bufs:
+            for t in b:
+                t.record_stream(side)
+
+        def recompute(seg, stream):
+            """U^start .. U^(end-1) of segment seg: its checkpoint and the
+            states after each of its steps but the last"""
             start = seg * every
             end = min(start + every, nsteps)
-            # states U^start .. U^(end-1) of the segment, recomputed
-            states = [ctx.ckpts[seg]]
-            grid.set_state_device(states[0].data_ptr(), stream)
+            ckpt = ctx.ckpts[seg]
+            buf = bufs[seg % 2]
+            sp = stream.cuda_stream
+            grid.set_state_device(ckpt.data_ptr(), sp)
             for s in range(start, end - 1):
-                grid.step_fixed(sid, dts[s], stream)
-                u = torch.empty_like(states[0])
-                grid.get_state_device(u.data_ptr(), stream)
-                states.append(u)
+                grid.step_fixed(sid, dts[s], sp)
+                grid.get_state_device(buf[s - start].data_ptr(), sp)
+            return [ckpt] + buf[:end - 1 - start]
+
+        side.wait_stream(main)
+        states = recompute(nseg - 1, main)
+        for seg in reversed(range(nseg)):
+            ready = None
+            if seg > 0:
+                # its buffers were last read by the reverse of segment seg + 1
+                side.wait_stream(main)
+                with torch.cuda.stream(side):
+                    prev_states = recompute(seg - 1, side)
+                    ready = torch.cuda.Event()
+                    ready.record(side)
+            start = seg * every
+            end = min(start + every, nsteps)
             for s in reversed(range(start, end)):
-                grid.adjoint_step(sid, dts[s], states[s - start].data_ptr(), lam.data_ptr(), stream)
+                grid.adjoint_step(sid, dts[s], states[s - start].data_ptr(), lam.data_ptr(),
+                                  main.cuda_stream)
+            if ready is not None:
+                main.wait_event(ready)
+                states = prev_states
         ctx.ckpts = None
         return lam, None, None, None, None
 
