@@ -332,6 +332,41 @@ def ensemble_run(nbox, steps, solver="hll-athena", cfl=0.3, seed=2024):
             "results_identical": bool(same), "min_dt": float(dts.min())}
 
 
+def ensemble_imf_run(nbox, max_steps, normal, chunk=10, solver="hll-athena", cfl=0.3, seed=2024):
+    """The ensemble workload with the interplanetary field it specifies
+    (By, Bz uniform in [-10, 10] nT): the boxes are advanced `chunk` steps at
+    a time (graph-replayed), checked after each chunk, until a box reports a
+    physics error or `max_steps` are done.  Returns the steps completed, the
+    throughput over them (`normal` cells per step: the boxes' classification
+    does not depend on the field) and the error that ended the run."""
+    import torch
+    from pamhd_amd import configs
+    from pamhd_amd._lib import PamhdError
+    from pamhd_amd.swbox import Ensemble
+    boxes = configs.solar_wind_ensemble(nbox=nbox, seed=seed, imf=True)
+    stream = torch.cuda.current_stream().cuda_stream
+    done, secs, stop = 0, 0.0, None
+    with Ensemble(boxes) as e:
+        e.timestep(solver, 0.0, cfl, stream=stream)
+        e.check(stream=stream)
+        while done < max_steps:
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            e.run(solver, cfl, chunk, use_graph=True, stream=stream)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            try:
+                e.check(stream=stream)
+            except PamhdError as err:
+                stop = str(err)
+                break
+            done += chunk
+            secs += t1 - t0
+    return {"boxes": nbox, "imf": True, "steps_completed": done, "chunk": chunk,
+            "cell_updates_per_s": (normal * done / secs) if secs > 0 else None,
+            "stopped_by": stop}
+
+
 def adjoint_run(n=512, steps=200, every=20, directions=32, seed=5):
     """The adjoint workload: 2-D periodic n x n vortex with seeded
     random-Fourier perturbations (|k| <= 8, amplitudes 1e-2 on rho, v, p,
@@ -560,6 +595,8 @@ def main():
     ap.add_argument("--no-cleaning", action="store_true",
                     help="skip the divergence-cleaning record")
     ap.add_argument("--ensemble-boxes", type=int, default=64)
+    ap.add_argument("--ensemble-imf-steps", type=int, default=300,
+                    help="most steps of the ensemble with its IMF (it stops at the first physics error)")
     ap.add_argument("--ensemble-steps", type=int, default=500,
                     help="0 skips the ensemble workload")
     ap.add_argument("--precision-study", action="store_true",
@@ -640,6 +677,9 @@ def main():
         log("  ensemble %.3e cell-updates/s, per-box loop %.3e, x%.2f, identical %s"
             % (ens["ensemble_cell_updates_per_s"], ens["per_box_loop_cell_updates_per_s"],
                ens["speedup_vs_per_box_loop"], ens["results_identical"]))
+        imf = ensemble_imf_run(args.ensemble_boxes, args.ensemble_imf_steps, ens["normal_cells"])
+        log("  with the IMF: %d steps completed (%s)" % (imf["steps_completed"], imf["stopped_by"]))
+        ens["with_imf"] = imf
 
     adj = None
     if args.adjoint_steps > 0:

@@ -16,9 +16,10 @@ i + nx (j + ny k), component-major over the N cells of the set):
          29 inflow)
   bg     9 * N: Bg_B(cell)(+d) at the cell's positive face centres
 
-A step is eight launches, each covering every box:
+A step's launches, each covering every box:
   dt            lagged CFL step per box (one thread per box)
-  flux x3       one per face direction; per-box max-speed reduction
+  flux          the three face directions of each cell in one launch
+                (or one launch per direction); per-box max-speed reductions
   update        normal cells: constrained-transport change of their six
                 face copies, conserved update; cells of type >= 0: refit
   sync          each face's two copies averaged (one thread per face)
@@ -80,6 +81,7 @@ struct BoxSet {
 	int* blockbox = nullptr;
 	int nblocks = 0;
 	int tile = 0;
+	int flux3 = 1;   /* the three flux directions in one launch (box_flux3_kernel) */
 	double* S = nullptr;
 	double* P = nullptr;
 	double* bg = nullptr;
@@ -261,6 +263,119 @@ __global__ __launch_bounds__(TX * TY) void box_flux_kernel(const BoxDev* __restr
 		const int bb = blockbox[blockIdx.x];
 		if (m > 0) atomicMax(&status[bb].smax_bits[D], (unsigned long long)__double_as_longlong(m));
 		if (z) atomicAdd(&status[bb].zero[D], z);
+	}
+}
+
+/*
+The three flux passes of box_flux_kernel in one launch: a thread computes
+the fluxes of its cell's -x, -y and -z faces in turn (the cell's state read
+once, the same operations per face as the per-direction kernel) and the
+per-box speed reductions of the three directions.
+*/
+template <int SOLVER, int D>
+__device__ __forceinline__ void box_face(const BoxDev& B, long long N, const double* __restrict__ S,
+	const signed char* __restrict__ ctype, const double* __restrict__ bg, double* __restrict__ P,
+	double dt, BoxStatus* __restrict__ st, int b, long long g, const int id[3], const double own[8],
+	double& sp, unsigned& zero, const Phys& ph)
+{
+	double Fg[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+	if (id[D] >= 1) {
+		const long long gl = D == 0 ? g - 1 : (D == 1 ? g - B.nx : g - (long long)B.nx * B.ny);
+		if (ctype[g] >= 0 && ctype[gl] >= 0) {
+			double U[2][8];
+			const double lo[8] = {S[gl], S[N + gl], S[2 * N + gl], S[3 * N + gl], S[4 * N + gl],
+				S[5 * N + gl], S[6 * N + gl], S[7 * N + gl]};
+			const double* cs[2] = {lo, own};
+			for (int s = 0; s < 2; s++) {
+				const double* c = cs[s];
+				U[s][RHO] = c[0];
+				U[s][NRJ] = c[4];
+				U[s][M0] = c[1 + D];
+				U[s][M1] = c[1 + (D + 1) % 3];
+				U[s][M2] = c[1 + (D + 2) % 3];
+				U[s][B0] = c[5 + D];
+				U[s][B1] = c[5 + (D + 1) % 3];
+				U[s][B2] = c[5 + (D + 2) % 3];
+			}
+			double bb[3] = {0, 0, 0};
+			if (bg) {
+				bb[0] = bg[(3 * D + D) * N + gl];
+				bb[1] = bg[(3 * D + (D + 1) % 3) * N + gl];
+				bb[2] = bg[(3 * D + (D + 2) % 3) * N + gl];
+			}
+			double F[8], s = 0;
+			const int e = riemann<SOLVER, double>(U[0], U[1], bb, ph, F, s);
+			if (e) {
+				box_record_error(st + b, e, (long long)D * B.n + (g - B.off));
+			} else {
+				Fg[RHO] = dt * F[RHO];
+				Fg[NRJ] = dt * F[NRJ];
+				Fg[M0 + D] = dt * F[M0];
+				Fg[M0 + (D + 1) % 3] = dt * F[M1];
+				Fg[M0 + (D + 2) % 3] = dt * F[M2];
+				Fg[B0 + D] = dt * F[B0];
+				Fg[B0 + (D + 1) % 3] = dt * F[B1];
+				Fg[B0 + (D + 2) % 3] = dt * F[B2];
+				sp = s;
+				zero = s <= 0 ? 1u : 0u;
+			}
+		}
+	}
+	double* Pd = P + (long long)D * 8 * N;
+#pragma unroll
+	for (int q = 0; q < 8; q++) Pd[q * N + g] = Fg[q];
+}
+
+template <int SOLVER, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void box_flux3_kernel(const BoxDev* __restrict__ boxes,
+	const int* __restrict__ blockbox, long long N, const double* __restrict__ S,
+	const signed char* __restrict__ ctype, const double* __restrict__ bg, double* __restrict__ P,
+	const double* __restrict__ dts, BoxStatus* __restrict__ status, Phys ph)
+{
+	int b, i, j, k;
+	const bool inside = my_cell<TX, TY>(boxes, blockbox, b, i, j, k);
+	double sp[3] = {0, 0, 0};
+	unsigned zero[3] = {0, 0, 0};
+	if (inside) {
+		const BoxDev& B = boxes[b];
+		const long long g = gidx(B, i, j, k);
+		const int id[3] = {i, j, k};
+		const double own[8] = {S[g], S[N + g], S[2 * N + g], S[3 * N + g], S[4 * N + g],
+			S[5 * N + g], S[6 * N + g], S[7 * N + g]};
+		const double dt = dts[b];
+		box_face<SOLVER, 0>(B, N, S, ctype, bg, P, dt, status, b, g, id, own, sp[0], zero[0], ph);
+		box_face<SOLVER, 1>(B, N, S, ctype, bg, P, dt, status, b, g, id, own, sp[1], zero[1], ph);
+		box_face<SOLVER, 2>(B, N, S, ctype, bg, P, dt, status, b, g, id, own, sp[2], zero[2], ph);
+	}
+	/* per-box max speed and count of non-positive speeds per direction; a
+	   block covers cells of one box only */
+	__shared__ double ms[3][TX * TY / 64];
+	__shared__ unsigned mz[3][TX * TY / 64];
+	const int tid = threadIdx.x + TX * threadIdx.y;
+	for (int d = 0; d < 3; d++) {
+		double v = sp[d];
+		unsigned z = zero[d];
+		for (int off = 32; off > 0; off >>= 1) {
+			v = smax(v, __shfl_xor(v, off, 64));
+			z += __shfl_xor(z, off, 64);
+		}
+		if (tid % 64 == 0) {
+			ms[d][tid / 64] = v;
+			mz[d][tid / 64] = z;
+		}
+	}
+	__syncthreads();
+	if (tid < 3) {
+		const int d = tid;
+		double m = ms[d][0];
+		unsigned z = mz[d][0];
+		for (int w = 1; w < TX * TY / 64; w++) {
+			m = smax(m, ms[d][w]);
+			z += mz[d][w];
+		}
+		const int bb = blockbox[blockIdx.x];
+		if (m > 0) atomicMax(&status[bb].smax_bits[d], (unsigned long long)__double_as_longlong(m));
+		if (z) atomicAdd(&status[bb].zero[d], z);
 	}
 }
 
@@ -635,9 +750,13 @@ template <int TX, int TY> void launch_all(BoxSet* s, int solver, hipStream_t st,
 	switch (stage) {
 	case 0:
 #define FLUX(SV) \
-		box_flux_kernel<SV, 0, TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->bg, s->P, s->dt, stat, s->ph); \
-		box_flux_kernel<SV, 1, TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->bg, s->P, s->dt, stat, s->ph); \
-		box_flux_kernel<SV, 2, TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->bg, s->P, s->dt, stat, s->ph)
+		if (s->flux3) { \
+			box_flux3_kernel<SV, TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->bg, s->P, s->dt, stat, s->ph); \
+		} else { \
+			box_flux_kernel<SV, 0, TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->bg, s->P, s->dt, stat, s->ph); \
+			box_flux_kernel<SV, 1, TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->bg, s->P, s->dt, stat, s->ph); \
+			box_flux_kernel<SV, 2, TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->bg, s->P, s->dt, stat, s->ph); \
+		}
 		switch (solver) {
 		case RUSANOV: FLUX(RUSANOV); break;
 		case HLL_ATHENA: FLUX(HLL_ATHENA); break;
