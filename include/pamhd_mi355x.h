@@ -442,6 +442,39 @@ derivatives.)
 int pamhd_adjoint_step(pamhd_handle* h, int solver, double dt, const double* state,
 	double* lambda, void* stream);
 
+/*
+The magnetic fluxes of the last step, for the adjoint of that step: copies
+the six transverse field components of dt * F (x-faces: B_y, B_z; y-faces:
+B_z, B_x; z-faces: B_x, B_y; 6 * n doubles, the cell owning the face on its
+negative side) into `fluxes` (device memory).  Valid after an FP64 step
+that kept the step's flux array (Rusanov, HLL, Roe without substepping or
+boundaries); PAMHD_E_UNSUPPORTED otherwise.  (No counterpart in the
+reference: an adjoint helper.)
+*/
+int pamhd_get_magnetic_fluxes(pamhd_handle* h, double* fluxes, void* stream);
+
+/*
+pamhd_adjoint_step with the step's magnetic fluxes given (the output of
+pamhd_get_magnetic_fluxes after the forward step from `state`) instead of
+recomputed from the state: the same lambda bit for bit, one kernel fewer.
+(No counterpart in the reference.)
+*/
+int pamhd_adjoint_step_fluxes(pamhd_handle* h, int solver, double dt, const double* state,
+	const double* fluxes, double* lambda, void* stream);
+
+/*
+nsteps reverse steps in one call, in the order they run (the latest step of
+the rollout first): dts, states and fluxes are host arrays of nsteps entries,
+states[q] the device state the step q started from, fluxes[q] its magnetic
+fluxes (pamhd_get_magnetic_fluxes) or NULL (recomputed; fluxes itself may be
+NULL).  Where the next step's fluxes are given, the last kernel of a step
+also makes the next step's first-stage terms, one launch fewer per step.
+The same lambda as nsteps calls of pamhd_adjoint_step, bit for bit.  (No
+counterpart in the reference.)
+*/
+int pamhd_adjoint_steps(pamhd_handle* h, int solver, int nsteps, const double* dts,
+	const double* const* states, const double* const* fluxes, double* lambda, void* stream);
+
 /* ---- solar-wind box (tests/mhd/solar_wind_box.cpp) ---- */
 
 /*

@@ -7,7 +7,8 @@ library's hand-written adjoint (pamhd_adjoint_step) step by step in reverse,
 with checkpointing: the forward keeps the state every `every` steps, and the
 backward recomputes the states of one segment at a time from its checkpoint,
 the previous segment's on a second stream while the current segment's
-reverse steps run.  The forward and the reverse steps are ordered on torch's
+reverse steps run, and keeps each recomputed step's magnetic fluxes for its
+reverse step.  The forward and the reverse steps are ordered on torch's
 current stream.
 """
 import torch
@@ -51,44 +52,56 @@ class MHDRollout(torch.autograd.Function):
         lam = lam_out.detach().to(torch.float64).contiguous().clone()
         nsteps = len(dts)
         nseg = len(ctx.ckpts)
-        bufs = [[torch.empty_like(ctx.ckpts[0]) for _ in range(max(every - 1, 0))] for _ in range(2)]
-        for b in bufs:
+        n = ctx.ckpts[0].shape[1]
+        nb = max(every - 1, 0)
+        bufs = [[torch.empty_like(ctx.ckpts[0]) for _ in range(nb)] for _ in range(2)]
+        # the recomputed steps' magnetic fluxes, which the reverse steps then
+        # need not recompute
+        fbufs = [[torch.empty((6, n), dtype=torch.float64, device=dev) for _ in range(nb)]
+                 for _ in range(2)]
+        for b in bufs + fbufs:
             for t in b:
                 t.record_stream(side)
 
         def recompute(seg, stream):
-            """U^start .. U^(end-1) of segment seg: its checkpoint and the
-            states after each of its steps but the last"""
+            """U^start .. U^(end-1) of segment seg (its checkpoint and the
+            states after each of its steps but the last) and the magnetic
+            fluxes of those steps"""
             start = seg * every
             end = min(start + every, nsteps)
             ckpt = ctx.ckpts[seg]
-            buf = bufs[seg % 2]
+            buf, fbuf = bufs[seg % 2], fbufs[seg % 2]
             sp = stream.cuda_stream
             grid.set_state_device(ckpt.data_ptr(), sp)
             for s in range(start, end - 1):
                 grid.step_fixed(sid, dts[s], sp)
+                grid.get_magnetic_fluxes(fbuf[s - start].data_ptr(), sp)
                 grid.get_state_device(buf[s - start].data_ptr(), sp)
-            return [ckpt] + buf[:end - 1 - start]
+            return [ckpt] + buf[:end - 1 - start], fbuf[:end - 1 - start]
 
         side.wait_stream(main)
-        states = recompute(nseg - 1, main)
+        states, fluxes = recompute(nseg - 1, main)
         for seg in reversed(range(nseg)):
             ready = None
             if seg > 0:
                 # its buffers were last read by the reverse of segment seg + 1
                 side.wait_stream(main)
                 with torch.cuda.stream(side):
-                    prev_states = recompute(seg - 1, side)
+                    prev = recompute(seg - 1, side)
                     ready = torch.cuda.Event()
                     ready.record(side)
             start = seg * every
             end = min(start + every, nsteps)
-            for s in reversed(range(start, end)):
-                grid.adjoint_step(sid, dts[s], states[s - start].data_ptr(), lam.data_ptr(),
-                                  main.cuda_stream)
+            order = list(reversed(range(start, end)))
+            # the segment's last step was not recomputed: its fluxes are not kept
+            grid.adjoint_steps(sid, [dts[s] for s in order],
+                               [states[s - start].data_ptr() for s in order],
+                               [fluxes[s - start].data_ptr() if s - start < len(fluxes) else None
+                                for s in order],
+                               lam.data_ptr(), main.cuda_stream)
             if ready is not None:
                 main.wait_event(ready)
-                states = prev_states
+                states, fluxes = prev
         ctx.ckpts = None
         return lam, None, None, None, None
 

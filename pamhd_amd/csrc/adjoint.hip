@@ -34,7 +34,9 @@ U^{n+1}, same layout as the state) to lambda^n in four kernels, with
                  otherwise; the accumulators (lambda rho, m, E and lambda_vb,
                  8 n) are written out (over Pe, which is dead by then)
   final_kernel   adds the boundary buffer's terms, writes lambda rho, m, E and
-                 lambda_face = G + (lambda_vb(c) + lambda_vb(c - e_d)) / 2
+                 lambda_face = G + (lambda_vb(c) + lambda_vb(c - e_d)) / 2;
+                 in a run of reverse steps whose magnetic fluxes are given it
+                 also makes the next step's G and mu_flux (final_g_kernel)
 Each flux Jacobian is evaluated once per face; every sum has a fixed order,
 so a reverse step is reproducible bit for bit.
 */
@@ -399,6 +401,7 @@ struct AdjGrid {
    whose left cell lies in another tile (none in a dimension one tile spans) */
 struct AdjTiles {
 	int tx, ty, tz;
+	int lx, ly, lz;   /* log2 of tx, ty, tz (powers of two) */
 	int ntx, nty, ntz;
 	int sx, sy, sz, slots;
 };
@@ -409,6 +412,9 @@ AdjTiles tiles_for(int nx, int ny, int nz)
 	T.tx = 64;
 	T.ty = nz > 1 ? 2 : 4;
 	T.tz = nz > 1 ? 2 : 1;
+	T.lx = 6;
+	T.ly = nz > 1 ? 1 : 2;
+	T.lz = nz > 1 ? 1 : 0;
 	T.ntx = (nx + T.tx - 1) / T.tx;
 	T.nty = (ny + T.ty - 1) / T.ty;
 	T.ntz = (nz + T.tz - 1) / T.tz;
@@ -560,22 +566,23 @@ __device__ __forceinline__ double new_face(const AdjGrid& g, const double* __res
 	return s[(5 + d) * g.n + cell_at(g, i, j, k)] + face_dB(g, Pe, d, i, j, k) / area(g, d);
 }
 
+/* G and mu_flux of the D-faces of cell (i, j, k) from lambda_E of the cell
+   and of its lower D neighbour and lambda of the cell's D-face */
 template <int D>
-__device__ __forceinline__ void g_face(const AdjGrid& g, const double* __restrict__ s,
-	const double* __restrict__ Pe, const double* __restrict__ lam, double* __restrict__ G,
+__device__ __forceinline__ void g_face_vals(const AdjGrid& g, const double* __restrict__ s,
+	const double* __restrict__ Pe, double lE, double lEm, double lB, double* __restrict__ G,
 	double* __restrict__ MU, int i, int j, int k)
 {
 	const int n = g.n;
 	const int c = cell_at(g, i, j, k);
-	const double lE = lam[4 * n + c];
 	const int di = D == 0, dj = D == 1, dk = D == 2;
-	const int cp = nb(g, i, j, k, di, dj, dk), cm = nb(g, i, j, k, -di, -dj, -dk);
+	const int cp = nb(g, i, j, k, di, dj, dk);
 	const double f0 = new_face(g, s, Pe, D, i, j, k);
 	const double f1 = new_face(g, s, Pe, D, i + di, j + dj, k + dk);
 	const double fm = new_face(g, s, Pe, D, i - di, j - dj, k - dk);
 	const double mu_c = lE * (0.5 * (f0 + f1)) * g.ph.inv_mu0;
-	const double mu_m = lam[4 * n + cm] * (0.5 * (fm + f0)) * g.ph.inv_mu0;
-	G[D * n + c] = lam[(5 + D) * n + c] + 0.5 * mu_c + 0.5 * mu_m;
+	const double mu_m = lEm * (0.5 * (fm + f0)) * g.ph.inv_mu0;
+	G[D * n + c] = lB + 0.5 * mu_c + 0.5 * mu_m;
 	double vb_flux = 0.5 * (s[(5 + D) * n + c] + s[(5 + D) * n + cp]);
 	for (int e = 0; e < 3; e++) {
 		if (e == D) continue;   /* the normal component has no flux */
@@ -584,6 +591,17 @@ __device__ __forceinline__ void g_face(const AdjGrid& g, const double* __restric
 		vb_flux += (Pe[sl * n + c] - Pe[sl * n + ce]) / g.len[e];
 	}
 	MU[D * n + c] = -lE * vb_flux * g.ph.inv_mu0;
+}
+
+template <int D>
+__device__ __forceinline__ void g_face(const AdjGrid& g, const double* __restrict__ s,
+	const double* __restrict__ Pe, const double* __restrict__ lam, double* __restrict__ G,
+	double* __restrict__ MU, int i, int j, int k)
+{
+	const int n = g.n;
+	const int c = cell_at(g, i, j, k);
+	const int cm = nb(g, i, j, k, -(D == 0), -(D == 1), -(D == 2));
+	g_face_vals<D>(g, s, Pe, lam[4 * n + c], lam[4 * n + cm], lam[(5 + D) * n + c], G, MU, i, j, k);
 }
 
 /* G = lambda_face + (mu_new(c) + mu_new(c - e_d)) / 2 and
@@ -796,19 +814,19 @@ __device__ __forceinline__ void bnd_terms(const AdjGrid& g, const AdjTiles& T,
 {
 	const int ip = i + 1 == g.nx ? 0 : i + 1, jp = j + 1 == g.ny ? 0 : j + 1,
 		kp = k + 1 == g.nz ? 0 : k + 1;
-	const int tx = i / T.tx, ty = j / T.ty, tz = k / T.tz;
-	const int li = i - tx * T.tx, lj = j - ty * T.ty, lk = k - tz * T.tz;
+	const int tx = i >> T.lx, ty = j >> T.ly, tz = k >> T.lz;
+	const int li = i & (T.tx - 1), lj = j & (T.ty - 1), lk = k & (T.tz - 1);
 	b[0] = b[1] = b[2] = nullptr;
-	if (T.ntx > 1 && ip % T.tx == 0) {
-		const int tile = ip / T.tx + T.ntx * (ty + T.nty * tz);
+	if (T.ntx > 1 && (ip & (T.tx - 1)) == 0) {
+		const int tile = (ip >> T.lx) + T.ntx * (ty + T.nty * tz);
 		b[0] = BND + ((size_t)tile * T.slots + bnd_slot(T, 0, 0, lj, lk)) * 8;
 	}
-	if (T.nty > 1 && jp % T.ty == 0) {
-		const int tile = tx + T.ntx * (jp / T.ty + T.nty * tz);
+	if (T.nty > 1 && (jp & (T.ty - 1)) == 0) {
+		const int tile = tx + T.ntx * ((jp >> T.ly) + T.nty * tz);
 		b[1] = BND + ((size_t)tile * T.slots + bnd_slot(T, 1, li, 0, lk)) * 8;
 	}
-	if (T.ntz > 1 && kp % T.tz == 0) {
-		const int tile = tx + T.ntx * (ty + T.nty * (kp / T.tz));
+	if (T.ntz > 1 && (kp & (T.tz - 1)) == 0) {
+		const int tile = tx + T.ntx * (ty + T.nty * (kp >> T.lz));
 		b[2] = BND + ((size_t)tile * T.slots + bnd_slot(T, 2, li, lj, 0)) * 8;
 	}
 }
@@ -855,6 +873,60 @@ __global__ __launch_bounds__(ABX * ABY) void adj_final_kernel(AdjGrid g, AdjTile
 	}
 }
 
+/*
+adj_final_kernel of one reverse step followed by adj_g_kernel of the next
+(the step before it in time, state s2 and magnetic fluxes Pe2): a thread
+finishes lambda of its cell, writes it, and from the values it holds, with
+lambda_E of its lower neighbours recomputed from their accumulators, makes
+the next step's G and mu_flux of its cell.  G of a cell is read only by the
+cell's own thread here, so the next step's G replaces it in place.
+*/
+__global__ __launch_bounds__(ABX * ABY) void adj_final_g_kernel(AdjGrid g, AdjTiles T,
+	double* __restrict__ lam, double* __restrict__ G, const double* __restrict__ OUT,
+	const double* __restrict__ BND, const double* __restrict__ s2, const double* __restrict__ Pe2,
+	double* __restrict__ MU)
+{
+	int i, j, k;
+	if (!thread_cell(g, i, j, k)) return;
+	const int n = g.n;
+	const int c = cell_at(g, i, j, k);
+	const double* b[3];
+	bnd_terms(g, T, BND, i, j, k, b);
+	double v[8];
+	for (int q = 0; q < 8; q++) {
+		double a = OUT[q * n + c];
+		for (int d = 0; d < 3; d++) {
+			if (b[d]) a += b[d][q];
+		}
+		v[q] = a;
+	}
+	for (int q = 0; q < 5; q++) lam[q * n + c] = v[q];
+	double lB[3], lEm[3];
+	for (int d = 0; d < 3; d++) {
+		const int im = d == 0 ? (i == 0 ? g.nx - 1 : i - 1) : i;
+		const int jm = d == 1 ? (j == 0 ? g.ny - 1 : j - 1) : j;
+		const int km = d == 2 ? (k == 0 ? g.nz - 1 : k - 1) : k;
+		/* the lower neighbour's accumulator, components 5 + d and 4, with
+		   its boundary terms in the order of final_acc */
+		const double* bm[3];
+		bnd_terms(g, T, BND, im, jm, km, bm);
+		const int cmi = cell_at(g, im, jm, km);
+		double vm = OUT[(5 + d) * n + cmi], em = OUT[4 * n + cmi];
+		for (int e = 0; e < 3; e++) {
+			if (bm[e]) {
+				vm += bm[e][5 + d];
+				em += bm[e][4];
+			}
+		}
+		lB[d] = G[d * n + c] + 0.5 * v[5 + d] + 0.5 * vm;
+		lam[(5 + d) * n + c] = lB[d];
+		lEm[d] = em;
+	}
+	g_face_vals<0>(g, s2, Pe2, v[4], lEm[0], lB[0], G, MU, i, j, k);
+	g_face_vals<1>(g, s2, Pe2, v[4], lEm[1], lB[1], G, MU, i, j, k);
+	g_face_vals<2>(g, s2, Pe2, v[4], lEm[2], lB[2], G, MU, i, j, k);
+}
+
 } // namespace
 
 /* scratch: G (3 n), mu_flux (3 n), then Pe (6 n) for the first two kernels
@@ -868,8 +940,9 @@ size_t adjoint_scratch_doubles(int nx, int ny, int nz)
 	return 6 * n + tail;
 }
 
-void adjoint_launch(int nx, int ny, int nz, const double len[3], double dt, const Phys& ph,
-	const double* state, double* lam, double* scratch, hipStream_t st, int reverse)
+void adjoint_steps_launch(int nx, int ny, int nz, const double len[3], int nsteps, const double* dts,
+	const double* const* states, const double* const* pes, const Phys& ph, double* lam,
+	double* scratch, hipStream_t st, int reverse)
 {
 	AdjGrid g;
 	g.nx = nx;
@@ -877,23 +950,44 @@ void adjoint_launch(int nx, int ny, int nz, const double len[3], double dt, cons
 	g.nz = nz;
 	g.n = nx * ny * nz;
 	for (int d = 0; d < 3; d++) g.len[d] = len[d];
-	g.dt = dt;
 	g.ph = ph;
 	const size_t n = g.n;
 	const AdjTiles T = tiles_for(nx, ny, nz);
 	double* G = scratch;
 	double* MU = G + 3 * n;
-	double* Pe = MU + 3 * n;
-	double* OUT = Pe;          /* Pe is dead once G and mu_flux are made */
+	double* Pe_own = MU + 3 * n;
+	double* OUT = Pe_own;      /* the scratch Pe is dead once G and mu_flux are made */
 	double* BND = OUT + 8 * n;
 	const dim3 grid((nx + ABX - 1) / ABX, (ny + ABY - 1) / ABY, nz), block(ABX, ABY, 1);
-	const dim3 tgrid(T.ntx, T.nty, T.ntz);
 	const dim3 fgrid((nx + ABX - 1) / ABX, (ny + ABY - 1) / ABY, 3 * nz);   /* a thread per face */
-	adj_pe_kernel<<<fgrid, block, 0, st>>>(g, state, Pe);
-	adj_g_kernel<<<fgrid, block, 0, st>>>(g, state, Pe, lam, G, MU);
-	if (reverse) adj_tile_kernel<true><<<tgrid, TILE_CELLS, 0, st>>>(g, T, state, lam, MU, G, OUT, BND);
-	else adj_tile_kernel<false><<<tgrid, TILE_CELLS, 0, st>>>(g, T, state, lam, MU, G, OUT, BND);
-	adj_final_kernel<<<grid, block, 0, st>>>(g, T, lam, G, OUT, BND);
+	const dim3 tgrid(T.ntx, T.nty, T.ntz);
+	bool have_g = false;   /* G and mu_flux of step q made by the previous step's last kernel */
+	for (int q = 0; q < nsteps; q++) {
+		g.dt = dts[q];
+		const double* state = states[q];
+		if (!have_g) {
+			/* the step's magnetic fluxes: given by the caller (the forward
+			   step's own), or recomputed from the state */
+			const double* Pe = pes && pes[q] ? pes[q] : Pe_own;
+			if (Pe == Pe_own) adj_pe_kernel<<<fgrid, block, 0, st>>>(g, state, Pe_own);
+			adj_g_kernel<<<fgrid, block, 0, st>>>(g, state, Pe, lam, G, MU);
+		}
+		if (reverse) adj_tile_kernel<true><<<tgrid, TILE_CELLS, 0, st>>>(g, T, state, lam, MU, G, OUT, BND);
+		else adj_tile_kernel<false><<<tgrid, TILE_CELLS, 0, st>>>(g, T, state, lam, MU, G, OUT, BND);
+		have_g = q + 1 < nsteps && pes && pes[q + 1];
+		if (have_g) {
+			adj_final_g_kernel<<<grid, block, 0, st>>>(g, T, lam, G, OUT, BND, states[q + 1], pes[q + 1], MU);
+		} else {
+			adj_final_kernel<<<grid, block, 0, st>>>(g, T, lam, G, OUT, BND);
+		}
+	}
+}
+
+void adjoint_launch(int nx, int ny, int nz, const double len[3], double dt, const Phys& ph,
+	const double* state, double* lam, double* scratch, hipStream_t st, int reverse,
+	const double* pe_given)
+{
+	adjoint_steps_launch(nx, ny, nz, len, 1, &dt, &state, &pe_given, ph, lam, scratch, st, reverse);
 }
 
 } // namespace pamhd_dev

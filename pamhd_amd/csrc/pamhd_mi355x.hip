@@ -39,7 +39,11 @@ conserved update, constant-pressure energy refits).  In mixed precision the flux
 namespace pamhd_dev {
 size_t adjoint_scratch_doubles(int nx, int ny, int nz);
 void adjoint_launch(int nx, int ny, int nz, const double len[3], double dt, const Phys& ph,
-	const double* state, double* lam, double* scratch, hipStream_t st, int reverse);
+	const double* state, double* lam, double* scratch, hipStream_t st, int reverse,
+	const double* pe_given);
+void adjoint_steps_launch(int nx, int ny, int nz, const double len[3], int nsteps, const double* dts,
+	const double* const* states, const double* const* pes, const Phys& ph, double* lam,
+	double* scratch, hipStream_t st, int reverse);
 }
 
 using namespace pamhd_dev;
@@ -2068,6 +2072,20 @@ __global__ __launch_bounds__(BX * BY) void project_kernel(GridDev g, const doubl
 	o[7 * n + c] = b[2];
 }
 
+/* the six transverse magnetic fluxes of the flux array: slot 2 d + q - 1
+   (q = 1, 2) holds dt * F of global field component 5 + (d + q) % 3 on the
+   d-faces */
+__global__ __launch_bounds__(256) void magnetic_fluxes_kernel(const double* __restrict__ flux,
+	double* __restrict__ out, long long n)
+{
+	const long long c = (long long)blockIdx.x * 256 + threadIdx.x;
+	if (c >= n) return;
+#pragma unroll
+	for (int d = 0; d < 3; d++)
+#pragma unroll
+	for (int q = 1; q < 3; q++) out[(2 * d + q - 1) * n + c] = flux[(d * 8 + 5 + (d + q) % 3) * n + c];
+}
+
 /* batched Riemann solver for the pamhd_face_flux_batch unit-test surface */
 template <int SOLVER, class T>
 __global__ __launch_bounds__(256) void face_batch_kernel(size_t n, const double* __restrict__ Lp,
@@ -2785,6 +2803,7 @@ struct pamhd_handle {
 	SubstepDev* sub = nullptr; /* per-cell substepping, allocated when enabled */
 	int mg_fused = 0;   /* per-colour passes (faster; see pamhd_set_poisson_smoother) */
 	int adj_reverse = 1;   /* flux Jacobian of the adjoint: 1 reverse accumulation, 0 dual numbers */
+	int flux_step = 0;     /* the last step left its FP64 dt * F in flux (pamhd_get_magnetic_fluxes) */
 	int cur = 0;    /* state buffer holding the current state */
 	int last = 0;   /* status entry of the last flux pass */
 	unsigned long long flux_calcs = 0;   /* faces solved by the last step */
@@ -3521,6 +3540,7 @@ int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
 	}
 	if (!(cfl >= 0)) return fail(h, PAMHD_E_ARG, "time step factor must be >= 0");
 	hipStream_t st = (hipStream_t)stream_;
+	h->flux_step = 0;
 	if (h->box) {
 		/* timestep() alone, or with the main loop's apply_boundaries_sw_box
 		   and sync_magnetic_field (tests/mhd/solar_wind_box.cpp:522-553) */
@@ -3555,6 +3575,7 @@ int do_step(pamhd_handle* h, int solver, double max_dt, double cfl, int refits,
 		: launch_step_body<double, double>(h, solver, st, next, refits);
 	if (e) return e;
 	HIP_TRY(h, hipGetLastError());
+	h->flux_step = (h->precision == PAMHD_FP64 && !fused_step(h, solver)) ? 1 : 0;
 	h->cur = 1 - h->cur;
 	h->last = 1 - h->last;
 	h->flux_calcs = 3ull * (unsigned long long)h->g.n;
@@ -4031,7 +4052,60 @@ int pamhd_adjoint_step(pamhd_handle* h, int solver, double dt, const double* sta
 	if (!h->adj) HIP_TRY(h, hipMalloc(&h->adj, sizeof(double) * adjoint_scratch_doubles(h->g.nx, h->g.ny, h->g.nz)));
 	const double len[3] = {h->g.dx, h->g.dy, h->g.dz};
 	adjoint_launch(h->g.nx, h->g.ny, h->g.nz, len, dt, h->g.ph, state, lambda, h->adj,
-		(hipStream_t)stream, h->adj_reverse);
+		(hipStream_t)stream, h->adj_reverse, nullptr);
+	HIP_TRY(h, hipGetLastError());
+	return PAMHD_OK;
+}
+
+int pamhd_get_magnetic_fluxes(pamhd_handle* h, double* fluxes, void* stream)
+{
+	if (!h || !fluxes) return fail(h, PAMHD_E_ARG, "null argument");
+	if (!h->flux_step) {
+		return fail(h, PAMHD_E_UNSUPPORTED, "the last step kept no FP64 flux array (HLLD's marches, "
+			"mixed precision, substepping and boundaries do not)");
+	}
+	const long long n = h->g.n;
+	magnetic_fluxes_kernel<<<(unsigned)((n + 255) / 256), 256, 0, (hipStream_t)stream>>>(h->flux, fluxes, n);
+	HIP_TRY(h, hipGetLastError());
+	return PAMHD_OK;
+}
+
+int pamhd_adjoint_steps(pamhd_handle* h, int solver, int nsteps, const double* dts,
+	const double* const* states, const double* const* fluxes, double* lambda, void* stream)
+{
+	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
+	if (!h || !dts || !states || !lambda) return fail(h, PAMHD_E_ARG, "null argument");
+	if (nsteps < 0) return fail(h, PAMHD_E_ARG, "nsteps must be >= 0");
+	for (int q = 0; q < nsteps; q++) {
+		if (!states[q]) return fail(h, PAMHD_E_ARG, "null state");
+	}
+	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint needs a periodic grid");
+	if (solver != HLL_ATHENA) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint is implemented for hll-athena");
+	if (h->precision != PAMHD_FP64) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint is FP64");
+	if (h->bg) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint has no background field");
+	if (nsteps == 0) return PAMHD_OK;
+	if (!h->adj) HIP_TRY(h, hipMalloc(&h->adj, sizeof(double) * adjoint_scratch_doubles(h->g.nx, h->g.ny, h->g.nz)));
+	const double len[3] = {h->g.dx, h->g.dy, h->g.dz};
+	adjoint_steps_launch(h->g.nx, h->g.ny, h->g.nz, len, nsteps, dts, states, fluxes, h->g.ph, lambda,
+		h->adj, (hipStream_t)stream, h->adj_reverse);
+	HIP_TRY(h, hipGetLastError());
+	return PAMHD_OK;
+}
+
+int pamhd_adjoint_step_fluxes(pamhd_handle* h, int solver, double dt, const double* state,
+	const double* fluxes, double* lambda, void* stream)
+{
+	if (!fluxes) return fail(h, PAMHD_E_ARG, "null argument");
+	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
+	if (!h || !state || !lambda) return fail(h, PAMHD_E_ARG, "null argument");
+	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint needs a periodic grid");
+	if (solver != HLL_ATHENA) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint is implemented for hll-athena");
+	if (h->precision != PAMHD_FP64) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint is FP64");
+	if (h->bg) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint has no background field");
+	if (!h->adj) HIP_TRY(h, hipMalloc(&h->adj, sizeof(double) * adjoint_scratch_doubles(h->g.nx, h->g.ny, h->g.nz)));
+	const double len[3] = {h->g.dx, h->g.dy, h->g.dz};
+	adjoint_launch(h->g.nx, h->g.ny, h->g.nz, len, dt, h->g.ph, state, lambda, h->adj,
+		(hipStream_t)stream, h->adj_reverse, fluxes);
 	HIP_TRY(h, hipGetLastError());
 	return PAMHD_OK;
 }

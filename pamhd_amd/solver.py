@@ -270,6 +270,29 @@ class Grid:
         self._check(self._L.pamhd_adjoint_step(self._h, solver_id(solver), dt, state_ptr,
                                                lambda_ptr, stream))
 
+    def get_magnetic_fluxes(self, ptr, stream=None):
+        """Copies the last step's six transverse magnetic fluxes (dt * F,
+        6 * n doubles) to a device pointer, for adjoint_step_fluxes."""
+        self._check(self._L.pamhd_get_magnetic_fluxes(self._h, ptr, stream))
+
+    def adjoint_step_fluxes(self, solver, dt, state_ptr, fluxes_ptr, lambda_ptr, stream=None):
+        """adjoint_step with the forward step's magnetic fluxes given
+        (get_magnetic_fluxes after that step): the same lambda."""
+        self._check(self._L.pamhd_adjoint_step_fluxes(self._h, solver_id(solver), dt, state_ptr,
+                                                      fluxes_ptr, lambda_ptr, stream))
+
+    def adjoint_steps(self, solver, dts, state_ptrs, flux_ptrs, lambda_ptr, stream=None):
+        """Several reverse steps in one call, in the order they run (the
+        latest step first); flux_ptrs entries may be None (recomputed)."""
+        k = len(dts)
+        if len(state_ptrs) != k or (flux_ptrs is not None and len(flux_ptrs) != k):
+            raise ValueError("dts, state_ptrs and flux_ptrs differ in length")
+        d = (ctypes.c_double * k)(*dts)
+        sp = (ctypes.c_void_p * k)(*state_ptrs)
+        fp = None if flux_ptrs is None else (ctypes.c_void_p * k)(*flux_ptrs)
+        self._check(self._L.pamhd_adjoint_steps(self._h, solver_id(solver), k, d, sp, fp,
+                                                lambda_ptr, stream))
+
     STEP_MODES = {"auto": -1, "passes": 0, "fused": 1, "accumulate": 2}
 
     def set_fused_step(self, mode="auto", planes_per_chunk=0):

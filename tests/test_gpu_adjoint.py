@@ -197,3 +197,76 @@ def test_reverse_and_dual_jacobians_agree(gpu):
     err = np.abs(outs[0] - outs[1]) / scale
     print("max relative difference per component", err.max(1), flush=True)
     assert np.all(err <= 1e-12)
+
+
+@pytest.mark.parametrize("cells", [(16, 12, 1), (10, 9, 8)])
+def test_adjoint_with_the_forward_steps_fluxes(gpu, cells):
+    """The reverse step fed with the forward step's own magnetic fluxes
+    (pamhd_get_magnetic_fluxes after the step) gives the lambda of the one
+    that recomputes them, bit for bit; the fluxes are refused after a step
+    that kept no FP64 flux array."""
+    import torch
+    from pamhd_amd import _lib, configs
+    from pamhd_amd.solver import Grid
+    state, length, gamma, mu0 = configs.orszag_tang_vortex(cells, noise=0.05, seed=4)
+    n = state.shape[1]
+    dt = 0.2 * min(length) / 3
+    rng = np.random.default_rng(2)
+    lam1 = rng.normal(size=(8, n))
+    stream = torch.cuda.current_stream().cuda_stream
+    with Grid(cells, length, gamma, mu0) as g:
+        U = torch.from_numpy(state).to(gpu).contiguous()
+        g.set_state_device(U.data_ptr(), stream)
+        g.step_fixed("hll-athena", dt, stream)
+        pe = torch.empty((6, n), dtype=torch.float64, device=gpu)
+        g.get_magnetic_fluxes(pe.data_ptr(), stream)
+        a = torch.from_numpy(lam1).to(gpu).contiguous()
+        b = a.clone()
+        g.adjoint_step("hll-athena", dt, U.data_ptr(), a.data_ptr(), stream)
+        g.adjoint_step_fluxes("hll-athena", dt, U.data_ptr(), pe.data_ptr(), b.data_ptr(), stream)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
+        g.set_fused_step("accumulate")
+        g.step_fixed("hlld-athena", dt, stream)
+        with pytest.raises(_lib.PamhdError):
+            g.get_magnetic_fluxes(pe.data_ptr(), stream)
+
+
+@pytest.mark.parametrize("cells", [(70, 9, 1), (66, 5, 6)])
+def test_adjoint_steps_match_single_steps(gpu, cells):
+    """Four reverse steps in one call (the latest step's fluxes recomputed,
+    the others given, so the final gather of each step also makes the next
+    step's G and mu_flux) give the lambda of four pamhd_adjoint_step calls,
+    bit for bit; grids wider than one 64-cell tile exercise the tiles'
+    boundary terms."""
+    import torch
+    from pamhd_amd import configs
+    from pamhd_amd.solver import Grid
+    state, length, gamma, mu0 = configs.orszag_tang_vortex(cells, noise=0.05, seed=6)
+    n = state.shape[1]
+    dts = [0.15 * min(length) / 3, 0.1 * min(length) / 3, 0.12 * min(length) / 3, 0.08 * min(length) / 3]
+    rng = np.random.default_rng(3)
+    lam1 = rng.normal(size=(8, n))
+    stream = torch.cuda.current_stream().cuda_stream
+    with Grid(cells, length, gamma, mu0) as g:
+        U = [torch.from_numpy(state).to(gpu).contiguous()]
+        F = []
+        g.set_state_device(U[0].data_ptr(), stream)
+        for dt in dts:
+            g.step_fixed("hll-athena", dt, stream)
+            f = torch.empty((6, n), dtype=torch.float64, device=gpu)
+            g.get_magnetic_fluxes(f.data_ptr(), stream)
+            F.append(f)
+            u = torch.empty_like(U[0])
+            g.get_state_device(u.data_ptr(), stream)
+            U.append(u)
+        a = torch.from_numpy(lam1).to(gpu).contiguous()
+        b = a.clone()
+        for s in reversed(range(len(dts))):
+            g.adjoint_step("hll-athena", dts[s], U[s].data_ptr(), a.data_ptr(), stream)
+        order = list(reversed(range(len(dts))))
+        g.adjoint_steps("hll-athena", [dts[s] for s in order], [U[s].data_ptr() for s in order],
+                        [None if s == order[0] else F[s].data_ptr() for s in order],
+                        b.data_ptr(), stream)
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(a.cpu().numpy(), b.cpu().numpy())
