@@ -2189,6 +2189,15 @@ struct SubstepDev {
 	unsigned char* bmask = nullptr;
 	double* bval = nullptr;
 	signed char* types = nullptr;   /* allocation behind ct, fi, vt */
+	/* per BX x BY block of the cell grid (nblk of them): [0, 3 nblk) the
+	   smallest owner period m of the block's x-, y-, z-faces (a block has a
+	   face due at substep s exactly when s % m == 0), [3 nblk, 4 nblk) the
+	   smallest period of its cells and their lower neighbours; and
+	   whether a block's last flux pass of a direction left nonzero edge
+	   fields (3 nblk) */
+	int* blk = nullptr;
+	int* dirty = nullptr;
+	int nblk = 0;
 };
 
 /* index of a neighbour coordinate: periodic dimensions wrap (within one
@@ -2435,6 +2444,45 @@ __global__ __launch_bounds__(BX * BY) void ss_expand_kernel(GridDev g, const dou
 	}
 }
 
+/* per block of the cell grid: the smallest owner period of its x-, y- and
+   z-faces and the smallest period of its cells and their lower neighbours
+   (the block-level skip tests of ss_flux_kernel and ss_dbsum_kernel), and
+   every block's edge fields marked as not yet known to be zero */
+__global__ __launch_bounds__(BX * BY) void ss_blockmask_kernel(GridDev g, SubstepDev d)
+{
+	__shared__ int red[4][BX * BY];
+	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	const int tid = threadIdx.x + BX * threadIdx.y;
+	int m[3] = {INT_MAX, INT_MAX, INT_MAX}, p = INT_MAX;
+	if (i < g.nx && j < g.ny) {
+		const int c = cidx(g, i, j, k);
+		p = d.period[c];
+		const int o[3] = {
+			ss_coord(i - 1, g.nx, d.per[0]) < 0 ? -1 : cidx(g, ss_coord(i - 1, g.nx, d.per[0]), j, k),
+			ss_coord(j - 1, g.ny, d.per[1]) < 0 ? -1 : cidx(g, i, ss_coord(j - 1, g.ny, d.per[1]), k),
+			ss_coord(k - 1, g.nz, d.per[2]) < 0 ? -1 : cidx(g, i, j, ss_coord(k - 1, g.nz, d.per[2]))};
+		for (int e = 0; e < 3; e++) {
+			if (o[e] < 0) continue;
+			m[e] = d.minsub[o[e]];
+			p = min(p, d.period[o[e]]);
+		}
+	}
+	for (int e = 0; e < 3; e++) red[e][tid] = m[e];
+	red[3][tid] = p;
+	__syncthreads();
+	for (int off = BX * BY / 2; off > 0; off >>= 1) {
+		if (tid < off) {
+			for (int e = 0; e < 4; e++) red[e][tid] = min(red[e][tid], red[e][tid + off]);
+		}
+		__syncthreads();
+	}
+	if (tid < 4) {
+		const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+		d.blk[tid * d.nblk + b] = red[tid][0];
+		if (tid < 3) d.dirty[tid * d.nblk + b] = 1;
+	}
+}
+
 /*
 Faces of direction D due at substep `sub` (source/mhd/solve.hpp:252-393 with
 substepping): a face between two cells of type >= 0 is solved by its
@@ -2455,6 +2503,16 @@ template <int SOLVER, int D>
 __global__ __launch_bounds__(BX * BY) void ss_flux_kernel(GridDev g, SubstepDev d,
 	const double* __restrict__ bg, double* __restrict__ emf, int sub)
 {
+	/* a block with no face due at this substep writes nothing, unless its
+	   edge fields of the previous substep are not all zero yet */
+	{
+		const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+		const bool blk_due = (sub & (d.blk[D * d.nblk + b] - 1)) == 0;
+		const int was_dirty = d.dirty[D * d.nblk + b];
+		__syncthreads();
+		if (threadIdx.x == 0 && threadIdx.y == 0) d.dirty[D * d.nblk + b] = blk_due ? 1 : 0;
+		if (!blk_due && !was_dirty) return;
+	}
 	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
 	if (i >= g.nx || j >= g.ny) return;
 	const long long n = g.n;
@@ -2568,6 +2626,30 @@ faces) and of its +d faces at a closed end (kept in its end slots).
 __global__ __launch_bounds__(BX * BY) void ss_dbsum_kernel(GridDev g, SubstepDev d,
 	const double* __restrict__ emf, int sub)
 {
+	/* a block none of whose sums restarts at this substep, and around which
+	   no block has a face due (all edge fields its faces use are zero, so
+	   every dB is +0 and no sum changes), has nothing to do */
+	{
+		__shared__ int quiet;
+		const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
+		if (threadIdx.x == 0 && threadIdx.y == 0) {
+			int q = ((sub - 1) & (d.blk[3 * d.nblk + b] - 1)) != 0;
+			const int gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
+			for (int dz = -1; dz <= 1 && q; dz++)
+			for (int dy = -1; dy <= 1 && q; dy++)
+			for (int dx = -1; dx <= 1 && q; dx++) {
+				const int bx = ((int)blockIdx.x + dx + gx) % gx, by = ((int)blockIdx.y + dy + gy) % gy,
+					bz = ((int)blockIdx.z + dz + gz) % gz;
+				const int nbk = bx + gx * (by + gy * bz);
+				for (int e = 0; e < 3; e++) {
+					if ((sub & (d.blk[e * d.nblk + nbk] - 1)) == 0) q = 0;
+				}
+			}
+			quiet = q;
+		}
+		__syncthreads();
+		if (quiet) return;
+	}
 	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
 	if (i >= g.nx || j >= g.ny) return;
 	const long long n = g.n;
@@ -3381,6 +3463,8 @@ void substep_free(SubstepDev* d)
 	(void)hipFree(d->e[1]);
 	(void)hipFree(d->period);
 	(void)hipFree(d->minsub);
+	(void)hipFree(d->blk);
+	(void)hipFree(d->dirty);
 	(void)hipFree(d->opt_dev);
 	(void)hipFree(d->types);
 	(void)hipFree(d->bmask);
@@ -3411,6 +3495,12 @@ int substep_alloc(pamhd_handle* h)
 	al(&d->period, n * sizeof(int));
 	al(&d->minsub, n * sizeof(int));
 	al(&d->opt_dev, 2 * n * sizeof(int));
+	{
+		const dim3 cg = cell_grid(h->g);
+		d->nblk = (int)(cg.x * cg.y * cg.z);
+	}
+	al(&d->blk, 4 * (size_t)d->nblk * sizeof(int));
+	al(&d->dirty, 3 * (size_t)d->nblk * sizeof(int));
 	al(&d->scal, 5 * SS_SLOTS * sizeof(unsigned long long));
 	al(&d->sub_dt, sizeof(double));
 	al(&d->status, sizeof(StepStatus));
@@ -3489,6 +3579,7 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 	}
 	ss_period_kernel<<<grid, block, 0, st>>>(h->g, d, d.e[cur]);
 	ss_fold_kernel<<<1, SS_FOLD, 0, st>>>(d, 1, 0);
+	ss_blockmask_kernel<<<grid, block, 0, st>>>(h->g, d);
 	HIP_TRY(h, hipGetLastError());
 	HIP_TRY(h, hipMemcpyAsync(d.host, d.scal + 1 * SS_SLOTS, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
 	HIP_TRY(h, hipMemcpyAsync(d.host + 1, d.sub_dt, sizeof(double), hipMemcpyDeviceToHost, st));
