@@ -427,6 +427,19 @@ int pamhd_kernel_time(pamhd_handle* h, const char* kernel, double* total_ms,
    (No counterpart in the reference.) */
 int pamhd_step_fixed(pamhd_handle* h, int solver, double dt, void* stream);
 
+/*
+pamhd_step_fixed from the caller's device state state_in (8 * n) into
+state_out (8 * n, not overlapping it); the grid's own state is left as it
+was.  With fluxes non-NULL the step also writes its magnetic fluxes there
+(6 * n, the layout of pamhd_get_magnetic_fluxes) from the update pass, for
+which the step must keep its FP64 flux array (not mixed precision, not
+HLLD's marches).  For recomputing a checkpoint segment in a reverse pass
+without copies.  Not on solar-wind boxes or substepped grids.  (No
+counterpart in the reference.)
+*/
+int pamhd_step_fixed_from(pamhd_handle* h, int solver, double dt, const double* state_in,
+	double* state_out, double* fluxes, void* stream);
+
 /* Stream-ordered device-to-device copies of the 8 * n state. */
 int pamhd_set_state_device(pamhd_handle* h, const double* state, void* stream);
 int pamhd_get_state_device(pamhd_handle* h, double* state, void* stream);
@@ -435,8 +448,8 @@ int pamhd_get_state_device(pamhd_handle* h, double* state, void* stream);
 Reverse step of pamhd_step_fixed: given the state U^n (device, 8 * n) the
 forward step of length dt started from and lambda = dL/dU^{n+1} (device,
 8 * n, same layout), overwrites lambda with dL/dU^n.  HLL fluxes, FP64,
-periodic grid without background field.  Uses 108 * n doubles of scratch
-allocated on first use.  (No counterpart in the reference, which has no
+periodic grid without background field.  Uses about 14 * n doubles of
+scratch (plus the tiles' boundary terms) allocated on first use.  (No counterpart in the reference, which has no
 derivatives.)
 */
 int pamhd_adjoint_step(pamhd_handle* h, int solver, double dt, const double* state,

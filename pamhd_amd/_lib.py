@@ -34,7 +34,7 @@ EXPORTS = (
     "pamhd_ensemble_destroy", "pamhd_ensemble_timestep", "pamhd_ensemble_run",
     "pamhd_ensemble_check", "pamhd_ensemble_download", "pamhd_ensemble_cells",
     "pamhd_ensemble_set_tile", "pamhd_ensemble_autotune", "pamhd_ensemble_last_error",
-    "pamhd_step_fixed", "pamhd_set_state_device", "pamhd_get_state_device",
+    "pamhd_step_fixed", "pamhd_step_fixed_from", "pamhd_set_state_device", "pamhd_get_state_device",
     "pamhd_adjoint_step", "pamhd_get_magnetic_fluxes", "pamhd_adjoint_step_fluxes", "pamhd_adjoint_steps", "pamhd_set_slabs", "pamhd_set_fused_step", "pamhd_set_substepping", "pamhd_set_substepping_cells",
     "pamhd_last_max_substep", "pamhd_last_flux_calcs", "pamhd_last_substep_periods",
 )
@@ -166,6 +166,7 @@ def load():
                                           ctypes.c_size_t, _vp]
     L.pamhd_ensemble_last_error.argtypes = [E, ctypes.c_char_p, ctypes.c_size_t]
     L.pamhd_step_fixed.argtypes = [H, ctypes.c_int, ctypes.c_double, _vp]
+    L.pamhd_step_fixed_from.argtypes = [H, ctypes.c_int, ctypes.c_double, _vp, _vp, _vp, _vp]
     L.pamhd_set_state_device.argtypes = [H, _vp, _vp]
     L.pamhd_get_state_device.argtypes = [H, _vp, _vp]
     L.pamhd_adjoint_step.argtypes = [H, ctypes.c_int, ctypes.c_double, _vp, _vp, _vp]

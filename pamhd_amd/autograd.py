@@ -72,12 +72,14 @@ class MHDRollout(torch.autograd.Function):
             ckpt = ctx.ckpts[seg]
             buf, fbuf = bufs[seg % 2], fbufs[seg % 2]
             sp = stream.cuda_stream
-            grid.set_state_device(ckpt.data_ptr(), sp)
+            # each step from the previous step's buffer straight into the
+            # next, the update pass writing the step's magnetic fluxes
+            states = [ckpt] + buf[:end - 1 - start]
             for s in range(start, end - 1):
-                grid.step_fixed(sid, dts[s], sp)
-                grid.get_magnetic_fluxes(fbuf[s - start].data_ptr(), sp)
-                grid.get_state_device(buf[s - start].data_ptr(), sp)
-            return [ckpt] + buf[:end - 1 - start], fbuf[:end - 1 - start]
+                grid.step_fixed_from(sid, dts[s], states[s - start].data_ptr(),
+                                     states[s - start + 1].data_ptr(),
+                                     fbuf[s - start].data_ptr(), sp)
+            return states, fbuf[:end - 1 - start]
 
         side.wait_stream(main)
         states, fluxes = recompute(nseg - 1, main)

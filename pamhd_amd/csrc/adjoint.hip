@@ -28,7 +28,9 @@ U^{n+1}, same layout as the state) to lambda^n in four kernels, with
                  from the update (conserved and vb_flux) and the transposed
                  12-term CT sums over G, then lambda_L, lambda_R = J^T (dt
                  lambda_P) (the HLL Jacobian by reverse accumulation, or by
-                 forward-mode dual numbers); lambda_R is added to the cell's
+                 forward-mode dual numbers; a face that joins a cell to
+                 itself, in a grid one cell thick, by the physical flux's
+                 Jacobian, both sides at once); lambda_R is added to the cell's
                  accumulator, lambda_L goes to the left cell through LDS when
                  that cell is in the tile and to a per-tile boundary buffer
                  otherwise; the accumulators (lambda rho, m, E and lambda_vb,
@@ -703,6 +705,110 @@ __device__ __forceinline__ void face_jt(const AdjGrid& g, const double* __restri
 	}
 }
 
+/*
+J^T g summed over both sides of a face whose two cells are one cell (the
+d-faces of a grid one cell thick in d, periodic): with L = R = U the HLL
+flux is the physical flux F(U), and the sum of its two side Jacobians is
+F's own (the dissipation's and the wave speeds' derivatives cancel between
+the sides).  Faces where hll_adjoint gives no gradient (a failing pressure
+or fast speed, a zero flux) give none here either.
+*/
+__device__ bool self_face_adjoint(const double u[8], const Phys& ph, const double g[8], double gu[8])
+{
+	for (int q = 0; q < 8; q++) gu[q] = 0;
+	if (u[RHO] <= 0) return false;
+	const double inv_mu0 = ph.inv_mu0, gm1 = ph.gm1, hinv = ph.half_inv_mu0;
+	const double msq = u[M0] * u[M0] + u[M1] * u[M1] + u[M2] * u[M2];
+	const double bsq = u[B0] * u[B0] + u[B1] * u[B1] + u[B2] * u[B2];
+	const double p = (u[NRJ] - 0.5 / u[RHO] * msq - hinv * bsq) * gm1;
+	if (!(p > 0) || !__builtin_isnormal(p)) return false;
+	const double ir = 1 / u[RHO];
+	const double v[3] = {u[M0] * ir, u[M1] * ir, u[M2] * ir};
+	const double mag = __builtin_sqrt(bsq);
+	const double sound = __builtin_sqrt(ph.gamma * p / u[RHO]);
+	double cf = sound;
+	if (mag > 0) {
+		const double alfven = mag / __builtin_sqrt(ph.mu0 * u[RHO]);
+		const double s2 = sound * sound, a2 = alfven * alfven, ratio = u[B0] / mag;
+		const double ts = s2 * s2 + a2 * a2 + 2 * s2 * a2 * (1 - 2 * (ratio * ratio));
+		if (!(ts > 0)) return false;
+		cf = __builtin_sqrt(0.5 * (s2 + a2 + __builtin_sqrt(ts)));
+	}
+	const double msn = v[0] - cf, msp = v[0] + cf;
+	const double den0 = (!(msp < 0.0) ? msp : 0.0) - (!(0.0 < msn) ? msn : 0.0);
+	if (!__builtin_isnormal(den0) || den0 < 0) return true;   /* zero flux */
+	const double* B = u + B0;
+	const double pB1 = 0.5 * inv_mu0 * bsq;
+	const double vB = v[0] * B[0] + v[1] * B[1] + v[2] * B[2];
+	double gv[3] = {0, 0, 0}, gp = 0, gpB1 = 0;
+	gu[M0] += g[RHO];
+	for (int i = 0; i < 3; i++) {
+		gu[M0 + i] += g[M0 + i] * v[0];
+		gv[0] += g[M0 + i] * u[M0 + i];
+		gu[B0 + i] -= g[M0 + i] * B[0] * inv_mu0;
+		gu[B0] -= g[M0 + i] * B[i] * inv_mu0;
+	}
+	gp += g[M0];
+	gpB1 += g[M0];
+	gv[0] += g[NRJ] * (u[NRJ] + p + pB1);
+	gu[NRJ] += g[NRJ] * v[0];
+	gp += g[NRJ] * v[0];
+	gpB1 += g[NRJ] * v[0];
+	gu[B0] -= g[NRJ] * vB * inv_mu0;
+	const double gvB = -g[NRJ] * B[0] * inv_mu0;
+	for (int i = 0; i < 3; i++) {
+		gv[i] += gvB * B[i];
+		gu[B0 + i] += gvB * v[i];
+	}
+	for (int i = 1; i < 3; i++) {
+		gu[B0 + i] += g[B0 + i] * v[0];
+		gv[0] += g[B0 + i] * B[i];
+		gv[i] -= g[B0 + i] * B[0];
+		gu[B0] -= g[B0 + i] * v[i];
+	}
+	for (int i = 0; i < 3; i++) gu[B0 + i] += gpB1 * inv_mu0 * B[i];
+	for (int i = 0; i < 3; i++) {
+		gu[M0 + i] += gv[i] * ir;
+		gu[RHO] -= gv[i] * v[i] * ir;
+	}
+	const double gpg = gp * gm1;
+	gu[NRJ] += gpg;
+	for (int i = 0; i < 3; i++) {
+		gu[M0 + i] -= gpg * v[i];
+		gu[B0 + i] -= gpg * 2 * hinv * u[B0 + i];
+	}
+	gu[RHO] += gpg * 0.5 * msq * ir * ir;
+	return true;
+}
+
+/* J^T (dt lambda_P) of the d-face owned by (i, j, k) when its two cells are
+   the cell itself, global frame */
+__device__ __forceinline__ void self_face_jt(const AdjGrid& g, const double* __restrict__ s,
+	const double lp[8], int d, int i, int j, int k, double w[8])
+{
+	double U[8], lF[8], out[8];
+	face_state(g, s, d, i, j, k, U);
+	lF[RHO] = g.dt * lp[RHO];
+	lF[NRJ] = g.dt * lp[NRJ];
+	for (int q = 0; q < 3; q++) {
+		lF[M0 + q] = g.dt * lp[M0 + (d + q) % 3];
+		lF[B0 + q] = g.dt * lp[B0 + (d + q) % 3];
+	}
+	self_face_adjoint(U, g.ph, lF, out);
+	w[RHO] = out[RHO];
+	w[NRJ] = out[NRJ];
+	for (int q = 0; q < 3; q++) {
+		w[M0 + (d + q) % 3] = out[M0 + q];
+		w[B0 + (d + q) % 3] = out[B0 + q];
+	}
+}
+
+/* whether the grid is one cell thick in d (its d-faces join a cell to itself) */
+__device__ __forceinline__ bool one_thick(const AdjGrid& g, int d)
+{
+	return (d == 0 ? g.nx : d == 1 ? g.ny : g.nz) == 1;
+}
+
 __device__ __forceinline__ void tile_of_cell(const AdjTiles& T, int i, int j, int k, int& ti, int& tj,
 	int& tk, int& li, int& lj, int& lk)
 {
@@ -744,8 +850,16 @@ __device__ __forceinline__ void tile_face(const AdjGrid& g, const AdjTiles& T,
 {
 	double lp[8], wl[8], wr[8];
 	face_lambda(g, lam, MU, G, D, i, j, k, lp);
-	face_jt<REVERSE>(g, s, lp, D, i, j, k, wl, wr);
 	const int t = threadIdx.x;
+	if (one_thick(g, D)) {
+		/* both sides are this cell: the physical flux's Jacobian, no left
+		   term (its hydro lambda_P is zero; its transverse field components
+		   still carry the edge sums) */
+		self_face_jt(g, s, lp, D, i, j, k, wr);
+		for (int q = 0; q < 8; q++) acc_s[q][t] += wr[q];
+		return;
+	}
+	face_jt<REVERSE>(g, s, lp, D, i, j, k, wl, wr);
 	for (int q = 0; q < 8; q++) acc_s[q][t] += wr[q];
 	if (left_in_tile(T, D, li, lj, lk)) {
 		/* the left cell's local index (wrapping inside a tile that spans the
@@ -792,6 +906,7 @@ __global__ __launch_bounds__(TILE_CELLS) void adj_tile_kernel(AdjGrid g, AdjTile
 	double acc[8];
 	for (int q = 0; q < 8; q++) acc[q] = acc_s[q][t];
 	for (int d = 0; d < 3; d++) {
+		if (one_thick(g, d)) continue;   /* that face was summed whole above */
 		/* the face owned by the right neighbour: its left cell is this one */
 		int pi = li + (d == 0), pj = lj + (d == 1), pk = lk + (d == 2);
 		const int ip = i + (d == 0), jp = j + (d == 1), kp = k + (d == 2);

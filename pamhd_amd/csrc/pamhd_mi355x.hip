@@ -888,6 +888,20 @@ template <class ET, class NB> __device__ __forceinline__ double dBz_at(const ET&
 		E.exy(L(i, jp, k)), E.exy(L(i, jp, km)), E.exy(L(i, j, k)), E.exy(L(i, j, km)));
 }
 
+/* the six transverse magnetic fluxes (dt * F) of the three faces cell c
+   owns, in the layout of magnetic_fluxes_kernel: slot 2 d + q - 1 (q = 1, 2)
+   holds global field component 5 + (d + q) % 3 of the d-face; lo(d, comp)
+   reads the flux array's d-face value of component comp at c */
+template <class LO>
+__device__ __forceinline__ void store_magnetic_fluxes(double* __restrict__ pe, long long n, long long c,
+	const LO& lo)
+{
+#pragma unroll
+	for (int d = 0; d < 3; d++)
+#pragma unroll
+	for (int q = 1; q < 3; q++) pe[(2 * d + q - 1) * n + c] = lo(d, B0 + (d + q) % 3);
+}
+
 /*
 Update pass, column-marching variant: update_mhd_state + update_vol_B of
 source/mhd/solve.hpp:1503-1683 with thread (i, j) of a TX x TY column
@@ -903,7 +917,8 @@ update_cell_kernel, so both kernels give identical results.
 template <class PT, int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void update_column_kernel(GridDev g,
 	const double* __restrict__ s, double* __restrict__ out,
-	const PT* __restrict__ P, int refits, int kz, unsigned gx, unsigned gy, int remap)
+	const PT* __restrict__ P, int refits, int kz, unsigned gx, unsigned gy, int remap,
+	double* __restrict__ pe)
 {
 	const unsigned lb = remap ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
 	const unsigned bx = lb % gx, by = (lb / gx) % gy, bz = lb / (gx * gy);
@@ -1018,6 +1033,7 @@ __global__ __launch_bounds__(TX * TY) void update_column_kernel(GridDev g,
 		out[2 * n + c] = m[1];
 		out[3 * n + c] = m[2];
 		out[4 * n + c] = En;
+		if (pe) store_magnetic_fluxes(pe, n, c, lo);
 
 		/* plane k + 1 becomes the current plane */
 		eyx_m = eyx_c; eyx_pm = eyx_pc; exy_m = exy_c; exy_pm = exy_pc;
@@ -1040,7 +1056,8 @@ pipeline.
 template <class PT>
 __global__ __launch_bounds__(BX * BY) void update_cell_kernel(GridDev g,
 	const double* __restrict__ s, double* __restrict__ out,
-	const PT* __restrict__ P, int refits, unsigned gx, unsigned gy, int remap, int k0)
+	const PT* __restrict__ P, int refits, unsigned gx, unsigned gy, int remap, int k0,
+	double* __restrict__ pe)
 {
 	const unsigned lb = (remap & 1) ? logical_block(blockIdx.x, gridDim.x) : blockIdx.x;
 	unsigned bx, by, k;
@@ -1110,6 +1127,11 @@ __global__ __launch_bounds__(BX * BY) void update_cell_kernel(GridDev g,
 	out[2 * n + c] = m[1];
 	out[3 * n + c] = m[2];
 	out[4 * n + c] = En;
+	if (pe) {
+		store_magnetic_fluxes(pe, n, c, [&](int d, int q) -> double {
+			return (double)P[((long long)d * 8 + q) * n + c];
+		});
+	}
 }
 
 /*
@@ -2886,6 +2908,7 @@ struct pamhd_handle {
 	int mg_fused = 0;   /* per-colour passes (faster; see pamhd_set_poisson_smoother) */
 	int adj_reverse = 1;   /* flux Jacobian of the adjoint: 1 reverse accumulation, 0 dual numbers */
 	int flux_step = 0;     /* the last step left its FP64 dt * F in flux (pamhd_get_magnetic_fluxes) */
+	double* pe_out = nullptr;   /* where the update pass also writes the step's magnetic fluxes */
 	int cur = 0;    /* state buffer holding the current state */
 	int last = 0;   /* status entry of the last flux pass */
 	unsigned long long flux_calcs = 0;   /* faces solved by the last step */
@@ -3049,12 +3072,15 @@ template <class PT>
 void launch_update(pamhd_handle* h, hipStream_t st, int refits, int k0 = 0, int nk = -1)
 {
 	const PT* P = reinterpret_cast<const PT*>(h->flux);
+	/* the step's magnetic fluxes, when a caller asked for them (FP64 only) */
+	double* pe = std::is_same<PT, double>::value ? h->pe_out : nullptr;
 	const int kz = h->tune.update_kz < h->g.nz ? h->tune.update_kz : h->g.nz;
 	if (kz <= 0 || nk >= 0) {
 		if (nk < 0) nk = h->g.nz;
 		const unsigned gx = (h->g.nx + BX - 1) / BX, gy = (h->g.ny + BY - 1) / BY;
 		update_cell_kernel<PT><<<dim3(gx * gy * nk), dim3(BX, BY, 1), 0, st>>>(
-			h->g, h->state[h->cur], h->state[1 - h->cur], P, refits, gx, gy, h->tune.update_remap, k0);
+			h->g, h->state[h->cur], h->state[1 - h->cur], P, refits, gx, gy, h->tune.update_remap, k0,
+			pe);
 		return;
 	}
 	const unsigned gz = (h->g.nz + kz - 1) / kz;
@@ -3062,7 +3088,8 @@ void launch_update(pamhd_handle* h, hipStream_t st, int refits, int k0 = 0, int 
 		constexpr int TX = decltype(tx)::value, TY = decltype(ty)::value;
 		const unsigned gx = (h->g.nx + TX - 1) / TX, gy = (h->g.ny + TY - 1) / TY;
 		update_column_kernel<PT, TX, TY><<<dim3(gx * gy * gz), dim3(TX, TY, 1), 0, st>>>(
-			h->g, h->state[h->cur], h->state[1 - h->cur], P, refits, kz, gx, gy, h->tune.update_remap);
+			h->g, h->state[h->cur], h->state[1 - h->cur], P, refits, kz, gx, gy, h->tune.update_remap,
+			pe);
 	};
 	switch (h->tune.update_tile) {
 	case 1: column(std::integral_constant<int, 128>(), std::integral_constant<int, 2>()); break;
@@ -4109,6 +4136,36 @@ int pamhd_step_fixed(pamhd_handle* h, int solver, double dt, void* stream)
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "fixed steps need a periodic grid");
 	if (!(dt >= 0)) return fail(h, PAMHD_E_ARG, "dt must be >= 0");
 	return do_step(h, solver, dt, 0.0, 3, nullptr, stream, 1);
+}
+
+int pamhd_step_fixed_from(pamhd_handle* h, int solver, double dt, const double* state_in,
+	double* state_out, double* fluxes, void* stream)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (!state_in || !state_out) return fail(h, PAMHD_E_ARG, "null argument");
+	if (h->box || h->sub) return fail(h, PAMHD_E_UNSUPPORTED, "steps between caller states need a periodic grid without substepping");
+	if (!(dt >= 0)) return fail(h, PAMHD_E_ARG, "dt must be >= 0");
+	const size_t bytes = sizeof(double) * 8 * (size_t)h->g.n;
+	const char* a = reinterpret_cast<const char*>(state_in);
+	const char* b = reinterpret_cast<const char*>(state_out);
+	if (a < b + bytes && b < a + bytes) return fail(h, PAMHD_E_ARG, "input and output states overlap");
+	if (fluxes && (h->precision != PAMHD_FP64 || fused_step(h, solver))) {
+		return fail(h, PAMHD_E_UNSUPPORTED, "the step keeps no FP64 flux array (HLLD's marches and "
+			"mixed precision do not)");
+	}
+	/* the step reads the current and writes the other state buffer: lend it
+	   the caller's two for this call, then restore the grid's own */
+	double* own[2] = {h->state[0], h->state[1]};
+	const int cur = h->cur;
+	h->state[cur] = const_cast<double*>(state_in);
+	h->state[1 - cur] = state_out;
+	h->pe_out = fluxes;
+	const int e = do_step(h, solver, dt, 0.0, 3, nullptr, stream, 1);
+	h->pe_out = nullptr;
+	h->state[0] = own[0];
+	h->state[1] = own[1];
+	h->cur = cur;
+	return e;
 }
 
 int pamhd_set_state_device(pamhd_handle* h, const double* state, void* stream)

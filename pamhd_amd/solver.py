@@ -256,6 +256,14 @@ class Grid:
         """One main-loop step of exactly dt (no CFL limit)."""
         self._check(self._L.pamhd_step_fixed(self._h, solver_id(solver), dt, stream))
 
+    def step_fixed_from(self, solver, dt, in_ptr, out_ptr, fluxes_ptr=None, stream=None):
+        """step_fixed from the device state at in_ptr into out_ptr (8 * n
+        doubles each, not overlapping), leaving the grid's own state as it
+        was; with fluxes_ptr also the step's magnetic fluxes (6 * n, as
+        get_magnetic_fluxes)."""
+        self._check(self._L.pamhd_step_fixed_from(self._h, solver_id(solver), dt, in_ptr, out_ptr,
+                                                  fluxes_ptr, stream))
+
     def set_state_device(self, ptr, stream=None):
         """Copies 8 * n doubles from a device pointer into the grid."""
         self._check(self._L.pamhd_set_state_device(self._h, ptr, stream))
