@@ -221,43 +221,43 @@ recomputes the intermediate values, the reverse pass carries the adjoint
 g of the face-frame flux back to both sides' inputs (gL, gR).  It returns
 what hll_dual returns (false: no gradient), and the same derivatives up to
 rounding, for about three flux evaluations instead of the 16 seeded
-directions of the dual-number form.
+directions of the dual-number form.  The fast speed is differentiated
+through s2 = gamma p / rho, a2 = |B|^2 / (mu0 rho) and r2 = B_n^2 / |B|^2
+(cf^2 = (s2 + a2 + sqrt(ts)) / 2, ts = s2^2 + a2^2 + 2 s2 a2 (1 - 2 r2)),
+which needs two square roots and two reciprocals per side, and every
+quotient is a product with a reciprocal made once: each differs from the
+solver's own value by a few ulp.
 */
 __device__ bool hll_adjoint(const double L[8], const double R[8], const Phys& ph,
 	const double gF[8], double gL[8], double gR[8])
 {
 	for (int q = 0; q < 8; q++) gL[q] = gR[q] = 0;
 	if (L[RHO] <= 0 || R[RHO] <= 0) return false;
-	const double gamma = ph.gamma, mu0 = ph.mu0, inv_mu0 = ph.inv_mu0;
+	const double gamma = ph.gamma, inv_mu0 = ph.inv_mu0;
 	const double gm1 = ph.gm1, hinv = ph.half_inv_mu0;
 	const double* U[2] = {L, R};
 	double* gU[2] = {gL, gR};
-	double p[2], v[2][3], cf[2], mag[2], sound[2], alfven[2] = {0, 0}, s2[2] = {0, 0};
-	double a2[2] = {0, 0}, ratio[2] = {0, 0}, rts[2] = {1, 1}, bsq[2], msq[2], irho[2];
-	double sqmr[2] = {1, 1};
+	double p[2], v[2][3], cf[2], s2[2], a2[2] = {0, 0}, r2[2] = {0, 0}, rts[2] = {1, 1};
+	double ibsq[2] = {0, 0}, bsq[2], msq[2], irho[2];
 	bool hasB[2];
 	for (int s = 0; s < 2; s++) {
 		const double* u = U[s];
+		const double ir = 1 / u[RHO];
+		irho[s] = ir;
 		msq[s] = u[M0] * u[M0] + u[M1] * u[M1] + u[M2] * u[M2];
 		bsq[s] = u[B0] * u[B0] + u[B1] * u[B1] + u[B2] * u[B2];
-		p[s] = (u[NRJ] - 0.5 / u[RHO] * msq[s] - hinv * bsq[s]) * gm1;
+		p[s] = (u[NRJ] - 0.5 * ir * msq[s] - hinv * bsq[s]) * gm1;
 		if (!(p[s] > 0) || !__builtin_isnormal(p[s])) return false;
-		const double inv = 1 / u[RHO];
-		irho[s] = inv;
-		for (int i = 0; i < 3; i++) v[s][i] = u[M0 + i] * inv;
-		mag[s] = __builtin_sqrt(bsq[s]);
-		sound[s] = __builtin_sqrt(gamma * p[s] / u[RHO]);
-		hasB[s] = mag[s] > 0;
+		for (int i = 0; i < 3; i++) v[s][i] = u[M0 + i] * ir;
+		s2[s] = gamma * p[s] * ir;
+		hasB[s] = bsq[s] > 0;
 		if (!hasB[s]) {
-			cf[s] = sound[s];
+			cf[s] = __builtin_sqrt(s2[s]);
 		} else {
-			sqmr[s] = __builtin_sqrt(mu0 * u[RHO]);
-			alfven[s] = mag[s] / sqmr[s];
-			s2[s] = sound[s] * sound[s];
-			a2[s] = alfven[s] * alfven[s];
-			ratio[s] = u[B0] / mag[s];
-			const double ts = s2[s] * s2[s] + a2[s] * a2[s]
-				+ 2 * s2[s] * a2[s] * (1 - 2 * (ratio[s] * ratio[s]));
+			a2[s] = bsq[s] * inv_mu0 * ir;
+			ibsq[s] = 1 / bsq[s];
+			r2[s] = u[B0] * u[B0] * ibsq[s];
+			const double ts = s2[s] * s2[s] + a2[s] * a2[s] + 2 * s2[s] * a2[s] * (1 - 2 * r2[s]);
 			if (!(ts > 0)) return false;
 			rts[s] = __builtin_sqrt(ts);
 			cf[s] = __builtin_sqrt(0.5 * (s2[s] + a2[s] + rts[s]));
@@ -288,7 +288,7 @@ __device__ bool hll_adjoint(const double L[8], const double R[8], const Phys& ph
 		for (int i = 0; i < 3; i++) f[B0 + i] = B[i] * v[s][0] - v[s][i] * B[0];
 		f[B0] = 0;
 	}
-	const double den = bp - bm, sum = bp + bm, w = sum / den / 2;
+	const double den = bp - bm, sum = bp + bm, rden = 1 / den, i2den = 0.5 * rden, w = sum * i2den;
 	double gFL[8], gFR[8], gw = 0;
 	for (int q = 0; q < 8; q++) {
 		if (q == B0) { gFL[q] = gFR[q] = 0; continue; }
@@ -297,11 +297,7 @@ __device__ bool hll_adjoint(const double L[8], const double R[8], const Phys& ph
 		gFR[q] = gF[q] * (0.5 - w);
 		gw += gF[q] * (fl - fr);
 	}
-	/* the reverse sweep multiplies by reciprocals computed once (1 / rho is
-	   the forward's own); each product differs from the quotient by at most
-	   an ulp */
-	const double i2den = 0.5 / den;
-	const double gsum = gw * i2den, gden = -gw * sum * i2den / den;
+	const double gsum = gw * i2den, gden = -gw * sum * i2den * rden;
 	double gbp = gsum + gden, gbm = gsum - gden;
 	for (int q = 0; q < 8; q++) {
 		if (q == B0) continue;
@@ -352,30 +348,29 @@ __device__ bool hll_adjoint(const double L[8], const double R[8], const Phys& ph
 		const double* u = U[s];
 		double* gu = gU[s];
 		const double ir = irho[s];
-		double gsound = 0, gmag = 0;
+		/* cf = sqrt(Q): gQ = gcf / (2 cf) */
+		const double gQ = gcf[s] * 0.5 / cf[s];
+		double gs2, gbsq = 0;
 		if (!hasB[s]) {
-			gsound += gcf[s];
+			gs2 = gQ;
 		} else {
-			const double gQ = gcf[s] * 0.5 / cf[s];
-			double gs2 = 0.5 * gQ, ga2 = 0.5 * gQ;
+			/* Q = (s2 + a2 + rts) / 2, rts = sqrt(ts) */
+			gs2 = 0.5 * gQ;
+			double ga2 = 0.5 * gQ;
 			const double gts = 0.25 * gQ / rts[s];
-			const double r2 = ratio[s] * ratio[s];
-			gs2 += gts * (2 * s2[s] + 2 * a2[s] * (1 - 2 * r2));
-			ga2 += gts * (2 * a2[s] + 2 * s2[s] * (1 - 2 * r2));
-			const double gratio = gts * 2 * s2[s] * a2[s] * (-4 * ratio[s]);
-			gsound += 2 * sound[s] * gs2;
-			const double galf = 2 * alfven[s] * ga2;
-			const double imag = 1 / mag[s];
-			gu[B0] += gratio * imag;
-			gmag -= gratio * u[B0] * imag * imag;
-			gmag += galf / sqmr[s];
-			gu[RHO] -= 0.5 * alfven[s] * ir * galf;
-			const double gmi = gmag * imag;
-			for (int i = 0; i < 3; i++) gu[B0 + i] += gmi * u[B0 + i];
+			const double t = 1 - 2 * r2[s];
+			gs2 += gts * (2 * s2[s] + 2 * a2[s] * t);
+			ga2 += gts * (2 * a2[s] + 2 * s2[s] * t);
+			const double gr2 = -4 * gts * s2[s] * a2[s];
+			/* r2 = B_n^2 / bsq, a2 = bsq / (mu0 rho) */
+			gu[B0] += gr2 * 2 * u[B0] * ibsq[s];
+			gbsq += ga2 * inv_mu0 * ir - gr2 * r2[s] * ibsq[s];
+			gu[RHO] -= ga2 * a2[s] * ir;
 		}
-		const double gsr = gsound / sound[s] * 0.5 * ir;
-		gp[s] += gsr * gamma;
-		gu[RHO] -= gsound * sound[s] * 0.5 * ir;
+		/* s2 = gamma p / rho */
+		gp[s] += gs2 * gamma * ir;
+		gu[RHO] -= gs2 * s2[s] * ir;
+		for (int i = 0; i < 3; i++) gu[B0 + i] += 2 * u[B0 + i] * gbsq;
 		for (int i = 0; i < 3; i++) {
 			gu[M0 + i] += gv[s][i] * ir;
 			gu[RHO] -= gv[s][i] * v[s][i] * ir;
@@ -394,6 +389,8 @@ __device__ bool hll_adjoint(const double L[8], const double R[8], const Phys& ph
 struct AdjGrid {
 	int nx, ny, nz, n;
 	double len[3];
+	double inv_len[3];    /* 1 / len */
+	double inv_area[3];   /* 1 / area of the d-faces */
 	double dt;
 	Phys ph;
 };
@@ -630,7 +627,7 @@ __device__ __forceinline__ void face_lambda(const AdjGrid& g, const double* __re
 	const int n = g.n;
 	const int c = cell_at(g, i, j, k);
 	const int cm = nb(g, i, j, k, -(d == 0), -(d == 1), -(d == 2));
-	const double il = 1 / g.len[d];
+	const double il = g.inv_len[d];
 	lp[RHO] = (lam[c] - lam[cm]) * il;
 	for (int q = 0; q < 3; q++) lp[M0 + q] = (lam[(1 + q) * n + c] - lam[(1 + q) * n + cm]) * il;
 	lp[NRJ] = (lam[4 * n + c] - lam[4 * n + cm]) * il;
@@ -645,7 +642,7 @@ __device__ __forceinline__ void face_lambda(const AdjGrid& g, const double* __re
 			const CtTerm& T = CT_TERMS[q];
 			if (T.e != e) continue;
 			const int tc = nb(g, i, j, k, -T.o0, -T.o1, -T.o2);
-			lE += T.sign * G[T.t * n + tc] / area(g, T.t);
+			lE += T.sign * G[T.t * n + tc] * g.inv_area[T.t];
 		}
 		lp[CT_EMF[e][1]] += CT_EMF[e][2] * lE * g.len[CT_EMF[e][3]] / 4;
 	}
@@ -1064,7 +1061,11 @@ void adjoint_steps_launch(int nx, int ny, int nz, const double len[3], int nstep
 	g.ny = ny;
 	g.nz = nz;
 	g.n = nx * ny * nz;
-	for (int d = 0; d < 3; d++) g.len[d] = len[d];
+	for (int d = 0; d < 3; d++) {
+		g.len[d] = len[d];
+		g.inv_len[d] = 1 / len[d];
+		g.inv_area[d] = 1 / (len[(d + 1) % 3] * len[(d + 2) % 3]);
+	}
 	g.ph = ph;
 	const size_t n = g.n;
 	const AdjTiles T = tiles_for(nx, ny, nz);
