@@ -480,8 +480,11 @@ nsteps reverse steps in one call, in the order they run (the latest step of
 the rollout first): dts, states and fluxes are host arrays of nsteps entries,
 states[q] the device state the step q started from, fluxes[q] its magnetic
 fluxes (pamhd_get_magnetic_fluxes) or NULL (recomputed; fluxes itself may be
-NULL).  Where the next step's fluxes are given, the last kernel of a step
-also makes the next step's first-stage terms, one launch fewer per step.
+NULL).  The steps are consecutive steps of one rollout: step q + 1 ended at
+states[q], whose face field is that step's constrained-transport result.
+Where the next step's fluxes are given, the last kernel of a step also makes
+the next step's first-stage terms, one launch fewer per step, reading its new
+faces from states[q].
 The same lambda as nsteps calls of pamhd_adjoint_step, bit for bit.  (No
 counterpart in the reference.)
 */

@@ -38,7 +38,8 @@ U^{n+1}, same layout as the state) to lambda^n in four kernels, with
   final_kernel   adds the boundary buffer's terms, writes lambda rho, m, E and
                  lambda_face = G + (lambda_vb(c) + lambda_vb(c - e_d)) / 2;
                  in a run of reverse steps whose magnetic fluxes are given it
-                 also makes the next step's G and mu_flux (final_g_kernel)
+                 also makes the next step's G and mu_flux (final_g_kernel),
+                 taking that step's new faces from the state it ended at
 Each flux Jacobian is evaluated once per face; every sum has a fixed order,
 so a reverse step is reproducible bit for bit.
 */
@@ -566,19 +567,30 @@ __device__ __forceinline__ double new_face(const AdjGrid& g, const double* __res
 }
 
 /* G and mu_flux of the D-faces of cell (i, j, k) from lambda_E of the cell
-   and of its lower D neighbour and lambda of the cell's D-face */
-template <int D>
+   and of its lower D neighbour and lambda of the cell's D-face.  The step's
+   new faces (after CT) are those of s_new, the state the step ended at,
+   when it is given: the forward step sums the same edge terms in the same
+   order, so they equal new_face's bit for bit */
+template <int D, bool GIVEN>
 __device__ __forceinline__ void g_face_vals(const AdjGrid& g, const double* __restrict__ s,
-	const double* __restrict__ Pe, double lE, double lEm, double lB, double* __restrict__ G,
-	double* __restrict__ MU, int i, int j, int k)
+	const double* __restrict__ Pe, const double* __restrict__ s_new, double lE, double lEm,
+	double lB, double* __restrict__ G, double* __restrict__ MU, int i, int j, int k)
 {
 	const int n = g.n;
 	const int c = cell_at(g, i, j, k);
 	const int di = D == 0, dj = D == 1, dk = D == 2;
 	const int cp = nb(g, i, j, k, di, dj, dk);
-	const double f0 = new_face(g, s, Pe, D, i, j, k);
-	const double f1 = new_face(g, s, Pe, D, i + di, j + dj, k + dk);
-	const double fm = new_face(g, s, Pe, D, i - di, j - dj, k - dk);
+	double f0, f1, fm;
+	if constexpr (GIVEN) {
+		const double* F = s_new + (5 + D) * n;
+		f0 = F[c];
+		f1 = F[cp];
+		fm = F[nb(g, i, j, k, -di, -dj, -dk)];
+	} else {
+		f0 = new_face(g, s, Pe, D, i, j, k);
+		f1 = new_face(g, s, Pe, D, i + di, j + dj, k + dk);
+		fm = new_face(g, s, Pe, D, i - di, j - dj, k - dk);
+	}
 	const double mu_c = lE * (0.5 * (f0 + f1)) * g.ph.inv_mu0;
 	const double mu_m = lEm * (0.5 * (fm + f0)) * g.ph.inv_mu0;
 	G[D * n + c] = lB + 0.5 * mu_c + 0.5 * mu_m;
@@ -600,7 +612,8 @@ __device__ __forceinline__ void g_face(const AdjGrid& g, const double* __restric
 	const int n = g.n;
 	const int c = cell_at(g, i, j, k);
 	const int cm = nb(g, i, j, k, -(D == 0), -(D == 1), -(D == 2));
-	g_face_vals<D>(g, s, Pe, lam[4 * n + c], lam[4 * n + cm], lam[(5 + D) * n + c], G, MU, i, j, k);
+	g_face_vals<D, false>(g, s, Pe, nullptr, lam[4 * n + c], lam[4 * n + cm], lam[(5 + D) * n + c], G, MU,
+		i, j, k);
 }
 
 /* G = lambda_face + (mu_new(c) + mu_new(c - e_d)) / 2 and
@@ -991,12 +1004,14 @@ adj_final_kernel of one reverse step followed by adj_g_kernel of the next
 finishes lambda of its cell, writes it, and from the values it holds, with
 lambda_E of its lower neighbours recomputed from their accumulators, makes
 the next step's G and mu_flux of its cell.  G of a cell is read only by the
-cell's own thread here, so the next step's G replaces it in place.
+cell's own thread here, so the next step's G replaces it in place.  The
+next step ended at s_new (the state this reverse step started from), whose
+faces are that step's new faces.
 */
 __global__ __launch_bounds__(ABX * ABY) void adj_final_g_kernel(AdjGrid g, AdjTiles T,
 	double* __restrict__ lam, double* __restrict__ G, const double* __restrict__ OUT,
 	const double* __restrict__ BND, const double* __restrict__ s2, const double* __restrict__ Pe2,
-	double* __restrict__ MU)
+	const double* __restrict__ s_new, double* __restrict__ MU)
 {
 	int i, j, k;
 	if (!thread_cell(g, i, j, k)) return;
@@ -1034,9 +1049,9 @@ __global__ __launch_bounds__(ABX * ABY) void adj_final_g_kernel(AdjGrid g, AdjTi
 		lam[(5 + d) * n + c] = lB[d];
 		lEm[d] = em;
 	}
-	g_face_vals<0>(g, s2, Pe2, v[4], lEm[0], lB[0], G, MU, i, j, k);
-	g_face_vals<1>(g, s2, Pe2, v[4], lEm[1], lB[1], G, MU, i, j, k);
-	g_face_vals<2>(g, s2, Pe2, v[4], lEm[2], lB[2], G, MU, i, j, k);
+	g_face_vals<0, true>(g, s2, Pe2, s_new, v[4], lEm[0], lB[0], G, MU, i, j, k);
+	g_face_vals<1, true>(g, s2, Pe2, s_new, v[4], lEm[1], lB[1], G, MU, i, j, k);
+	g_face_vals<2, true>(g, s2, Pe2, s_new, v[4], lEm[2], lB[2], G, MU, i, j, k);
 }
 
 } // namespace
@@ -1092,7 +1107,8 @@ void adjoint_steps_launch(int nx, int ny, int nz, const double len[3], int nstep
 		else adj_tile_kernel<false><<<tgrid, TILE_CELLS, 0, st>>>(g, T, state, lam, MU, G, OUT, BND);
 		have_g = q + 1 < nsteps && pes && pes[q + 1];
 		if (have_g) {
-			adj_final_g_kernel<<<grid, block, 0, st>>>(g, T, lam, G, OUT, BND, states[q + 1], pes[q + 1], MU);
+			adj_final_g_kernel<<<grid, block, 0, st>>>(g, T, lam, G, OUT, BND, states[q + 1], pes[q + 1],
+				state, MU);
 		} else {
 			adj_final_kernel<<<grid, block, 0, st>>>(g, T, lam, G, OUT, BND);
 		}

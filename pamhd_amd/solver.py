@@ -291,7 +291,9 @@ class Grid:
 
     def adjoint_steps(self, solver, dts, state_ptrs, flux_ptrs, lambda_ptr, stream=None):
         """Several reverse steps in one call, in the order they run (the
-        latest step first); flux_ptrs entries may be None (recomputed)."""
+        latest step first), of consecutive steps of one rollout (step q + 1
+        ended at state_ptrs[q]); flux_ptrs entries may be None
+        (recomputed)."""
         k = len(dts)
         if len(state_ptrs) != k or (flux_ptrs is not None and len(flux_ptrs) != k):
             raise ValueError("dts, state_ptrs and flux_ptrs differ in length")
