@@ -17,7 +17,7 @@ The forward step restated as a map U^n -> U^{n+1} (dt given):
                  kinetic terms cancel exactly)
 The reverse step takes lambda^{n+1} (the loss's gradient with respect to
 U^{n+1}, same layout as the state) to lambda^n in four kernels, with
-20 n doubles of scratch in 2-D (about 22 n in 3-D):
+16 n doubles of scratch in 2-D (about 22 n in 3-D):
   pe_kernel      the forward step's six magnetic-flux (edge field) parts of
                  the three faces of each cell: Pe (6 n); a thread per face
   g_kernel       mu_new = lambda_E vb_new / mu0 at the cell and at its lower
