@@ -2220,7 +2220,18 @@ struct SubstepDev {
 	int* blk = nullptr;
 	int* dirty = nullptr;
 	int nblk = 0;
+	/* per substep, the blocks each pass works on (ss_lists_kernel): lists
+	   SS_LISTS x nblk (the x-, y-, z-flux passes, the face-dB pass, the
+	   update, and the blocks of each direction whose edge fields only need
+	   zeroing), and their lengths, two sets of SS_LISTS used by odd and
+	   even substeps */
+	int* list = nullptr;
+	int* count = nullptr;
 };
+
+/* lists of ss_lists_kernel; the grid of the passes that walk them */
+enum { SS_LIST_DB = 3, SS_LIST_UPDATE = 4, SS_LIST_ZERO = 5, SS_LISTS = 8 };
+constexpr int SS_WALK_BLOCKS = 4096;
 
 /* index of a neighbour coordinate: periodic dimensions wrap (within one
    length), closed ones give -1 beyond their ends */
@@ -2468,7 +2479,7 @@ __global__ __launch_bounds__(BX * BY) void ss_expand_kernel(GridDev g, const dou
 
 /* per block of the cell grid: the smallest owner period of its x-, y- and
    z-faces and the smallest period of its cells and their lower neighbours
-   (the block-level skip tests of ss_flux_kernel and ss_dbsum_kernel), and
+   (the tests of the block lists, ss_lists_kernel), and
    every block's edge fields marked as not yet known to be zero */
 __global__ __launch_bounds__(BX * BY) void ss_blockmask_kernel(GridDev g, SubstepDev d)
 {
@@ -2503,6 +2514,74 @@ __global__ __launch_bounds__(BX * BY) void ss_blockmask_kernel(GridDev g, Subste
 		d.blk[tid * d.nblk + b] = red[tid][0];
 		if (tid < 3) d.dirty[tid * d.nblk + b] = 1;
 	}
+	/* the list lengths of the first substep (odd) start at zero */
+	if (blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && tid < SS_LISTS) d.count[SS_LISTS + tid] = 0;
+}
+
+/*
+The blocks the passes of substep `sub` work on, one thread per block of the
+cell grid, appended in any order (the blocks are independent):
+  flux pass D   a face of the block due at this substep; a block with
+                none whose edge fields of the previous substep are not yet
+                known to be zero goes to the zeroing list of D instead; the
+                flag is updated here
+  face-dB pass  a sum of the block that restarts at this substep, or a
+                block around it (27, three directions) with a face due
+                (else every dB is +0 and no sum changes)
+  update        a cell of the block (or a lower neighbour) due: periods are
+                powers of two, so none is when the smallest is not
+Thread 0 also zeroes the other set of lengths, which the next substep uses
+(the passes of the previous one, which read it, have finished).
+*/
+__global__ __launch_bounds__(256) void ss_lists_kernel(GridDev g, SubstepDev d, int sub)
+{
+	const int b = blockIdx.x * 256 + threadIdx.x;
+	int* cnt = d.count + SS_LISTS * (sub & 1);
+	if (b == 0) {
+		for (int q = 0; q < SS_LISTS; q++) d.count[SS_LISTS * ((sub + 1) & 1) + q] = 0;
+	}
+	bool run[SS_LISTS] = {false, false, false, false, false, false, false, false};
+	if (b < d.nblk) {
+		for (int e = 0; e < 3; e++) {
+			const bool due = (sub & (d.blk[e * d.nblk + b] - 1)) == 0;
+			run[e] = due;
+			run[SS_LIST_ZERO + e] = !due && d.dirty[e * d.nblk + b];
+			d.dirty[e * d.nblk + b] = due ? 1 : 0;
+		}
+		const int gx = (g.nx + BX - 1) / BX, gy = (g.ny + BY - 1) / BY, gz = g.nz;
+		const int bx0 = b % gx, by0 = (b / gx) % gy, bz0 = b / (gx * gy);
+		int q = ((sub - 1) & (d.blk[3 * d.nblk + b] - 1)) != 0;
+		for (int dz = -1; dz <= 1 && q; dz++)
+		for (int dy = -1; dy <= 1 && q; dy++)
+		for (int dx = -1; dx <= 1 && q; dx++) {
+			const int bx = (bx0 + dx + gx) % gx, by = (by0 + dy + gy) % gy, bz = (bz0 + dz + gz) % gz;
+			const int nbk = bx + gx * (by + gy * bz);
+			for (int e = 0; e < 3; e++) {
+				if ((sub & (d.blk[e * d.nblk + nbk] - 1)) == 0) q = 0;
+			}
+		}
+		run[SS_LIST_DB] = !q;
+		run[SS_LIST_UPDATE] = (sub & (d.blk[3 * d.nblk + b] - 1)) == 0;
+	}
+	const unsigned long long lt = (1ull << (threadIdx.x % 64)) - 1;
+	for (int l = 0; l < SS_LISTS; l++) {
+		const unsigned long long m = __ballot(run[l]);
+		if (!m) continue;
+		const int leader = __ffsll((long long)m) - 1;
+		int base = 0;
+		if ((int)(threadIdx.x % 64) == leader) base = atomicAdd(&cnt[l], __popcll(m));
+		base = __shfl(base, leader, 64);
+		if (run[l]) d.list[l * d.nblk + base + __popcll(m & lt)] = b;
+	}
+}
+
+/* block b of the cell grid (cell_grid's order: x fastest, then y, then z) */
+__device__ __forceinline__ void ss_block_coords(const GridDev& g, int b, int& bx, int& by, int& bz)
+{
+	const int gx = (g.nx + BX - 1) / BX, gy = (g.ny + BY - 1) / BY;
+	bx = b % gx;
+	by = (b / gx) % gy;
+	bz = b / (gx * gy);
 }
 
 /*
@@ -2522,20 +2601,10 @@ bit).  The face's two field components the edge terms use go to emf (0
 when the face is not solved at this substep).
 */
 template <int SOLVER, int D>
-__global__ __launch_bounds__(BX * BY) void ss_flux_kernel(GridDev g, SubstepDev d,
-	const double* __restrict__ bg, double* __restrict__ emf, int sub)
+__device__ __forceinline__ void ss_flux_block(const GridDev& g, const SubstepDev& d,
+	const double* __restrict__ bg, double* __restrict__ emf, int sub, int bx, int by, int k)
 {
-	/* a block with no face due at this substep writes nothing, unless its
-	   edge fields of the previous substep are not all zero yet */
-	{
-		const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-		const bool blk_due = (sub & (d.blk[D * d.nblk + b] - 1)) == 0;
-		const int was_dirty = d.dirty[D * d.nblk + b];
-		__syncthreads();
-		if (threadIdx.x == 0 && threadIdx.y == 0) d.dirty[D * d.nblk + b] = blk_due ? 1 : 0;
-		if (!blk_due && !was_dirty) return;
-	}
-	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	const int i = bx * BX + threadIdx.x, j = by * BY + threadIdx.y;
 	if (i >= g.nx || j >= g.ny) return;
 	const long long n = g.n;
 	const int c = cidx(g, i, j, k);
@@ -2621,6 +2690,48 @@ __global__ __launch_bounds__(BX * BY) void ss_flux_kernel(GridDev g, SubstepDev 
 	Em[n + c] = cq[D == 0 ? B0 + 2 : B0];
 }
 
+/* the flux pass of direction D: with WALK over the blocks of its list
+   (those with a face due), else over every block of the cell grid (a
+   substep at which every face is due) */
+template <int SOLVER, int D, bool WALK>
+__global__ __launch_bounds__(BX * BY) void ss_flux_kernel(GridDev g, SubstepDev d,
+	const double* __restrict__ bg, double* __restrict__ emf, int sub)
+{
+	if constexpr (WALK) {
+		const int cnt = d.count[SS_LISTS * (sub & 1) + D];
+		const int* list = d.list + (size_t)D * d.nblk;
+		for (int idx = blockIdx.x; idx < cnt; idx += gridDim.x) {
+			int bx, by, bz;
+			ss_block_coords(g, list[idx], bx, by, bz);
+			ss_flux_block<SOLVER, D>(g, d, bg, emf, sub, bx, by, bz);
+		}
+	} else {
+		ss_flux_block<SOLVER, D>(g, d, bg, emf, sub, blockIdx.x, blockIdx.y, blockIdx.z);
+	}
+}
+
+/* the edge fields of direction D set to zero in the blocks of its zeroing
+   list (no face due, the previous substep's values still there), what the
+   flux pass would write there */
+template <int D>
+__global__ __launch_bounds__(BX * BY) void ss_zero_emf_kernel(GridDev g, SubstepDev d,
+	double* __restrict__ emf, int sub)
+{
+	const int cnt = d.count[SS_LISTS * (sub & 1) + SS_LIST_ZERO + D];
+	const int* list = d.list + (size_t)(SS_LIST_ZERO + D) * d.nblk;
+	const long long n = g.n;
+	double* Em = emf + (long long)(2 * D) * n;
+	for (int idx = blockIdx.x; idx < cnt; idx += gridDim.x) {
+		int bx, by, bz;
+		ss_block_coords(g, list[idx], bx, by, bz);
+		const int i = bx * BX + threadIdx.x, j = by * BY + threadIdx.y;
+		if (i >= g.nx || j >= g.ny) continue;
+		const int c = cidx(g, i, j, bz);
+		Em[c] = 0;
+		Em[n + c] = 0;
+	}
+}
+
 /* constrained-transport edge terms from the substep's 6 field components
    per cell (emf), 0 beyond a closed end */
 struct EdgeTerms6B {
@@ -2645,34 +2756,10 @@ the cells it serves applied (x % p is x & (p - 1): periods are powers of
 two).  A cell computes the sums of the faces it owns (its -x, -y, -z
 faces) and of its +d faces at a closed end (kept in its end slots).
 */
-__global__ __launch_bounds__(BX * BY) void ss_dbsum_kernel(GridDev g, SubstepDev d,
-	const double* __restrict__ emf, int sub)
+__device__ __forceinline__ void ss_dbsum_block(const GridDev& g, const SubstepDev& d,
+	const double* __restrict__ emf, int sub, int bx, int by, int k)
 {
-	/* a block none of whose sums restarts at this substep, and around which
-	   no block has a face due (all edge fields its faces use are zero, so
-	   every dB is +0 and no sum changes), has nothing to do */
-	{
-		__shared__ int quiet;
-		const int b = blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z);
-		if (threadIdx.x == 0 && threadIdx.y == 0) {
-			int q = ((sub - 1) & (d.blk[3 * d.nblk + b] - 1)) != 0;
-			const int gx = gridDim.x, gy = gridDim.y, gz = gridDim.z;
-			for (int dz = -1; dz <= 1 && q; dz++)
-			for (int dy = -1; dy <= 1 && q; dy++)
-			for (int dx = -1; dx <= 1 && q; dx++) {
-				const int bx = ((int)blockIdx.x + dx + gx) % gx, by = ((int)blockIdx.y + dy + gy) % gy,
-					bz = ((int)blockIdx.z + dz + gz) % gz;
-				const int nbk = bx + gx * (by + gy * bz);
-				for (int e = 0; e < 3; e++) {
-					if ((sub & (d.blk[e * d.nblk + nbk] - 1)) == 0) q = 0;
-				}
-			}
-			quiet = q;
-		}
-		__syncthreads();
-		if (quiet) return;
-	}
-	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	const int i = bx * BX + threadIdx.x, j = by * BY + threadIdx.y;
 	if (i >= g.nx || j >= g.ny) return;
 	const long long n = g.n;
 	const int c = cidx(g, i, j, k);
@@ -2697,15 +2784,37 @@ __global__ __launch_bounds__(BX * BY) void ss_dbsum_kernel(GridDev g, SubstepDev
 	if (kp < 0) accumulate(8 * n + c, 8 * n + c, -1, dBz_at(E, L, i, im, ip, j, jm, jp, kp, k));
 }
 
+/* the face-dB pass, with WALK over the blocks of its list (a block none of
+   whose sums restarts at this substep, and around which no block has a
+   face due, is not in it: all edge fields its faces use are zero, so every
+   dB is +0 and no sum changes), else over every block */
+template <bool WALK>
+__global__ __launch_bounds__(BX * BY) void ss_dbsum_kernel(GridDev g, SubstepDev d,
+	const double* __restrict__ emf, int sub)
+{
+	if constexpr (WALK) {
+		const int cnt = d.count[SS_LISTS * (sub & 1) + SS_LIST_DB];
+		const int* list = d.list + (size_t)SS_LIST_DB * d.nblk;
+		for (int idx = blockIdx.x; idx < cnt; idx += gridDim.x) {
+			int bx, by, bz;
+			ss_block_coords(g, list[idx], bx, by, bz);
+			ss_dbsum_block(g, d, emf, sub, bx, by, bz);
+		}
+	} else {
+		ss_dbsum_block(g, d, emf, sub, blockIdx.x, blockIdx.y, blockIdx.z);
+	}
+}
+
 /*
 The update of cells due at substep `sub` (update_mhd_state,
 source/mhd/solve.hpp:1503-1603, and update_vol_B, :1618-1683, from their
 own face copies): a due cell applies the flux and face dB sums of its six
 faces that serve it (the faster cell's where the two periods differ).
 */
-__global__ __launch_bounds__(BX * BY) void ss_update_kernel(GridDev g, SubstepDev d, int sub)
+__device__ __forceinline__ void ss_update_block(const GridDev& g, const SubstepDev& d, int sub,
+	int bx, int by, int k)
 {
-	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+	const int i = bx * BX + threadIdx.x, j = by * BY + threadIdx.y;
 	if (i >= g.nx || j >= g.ny) return;
 	const long long n = g.n;
 	const int c = cidx(g, i, j, k);
@@ -2782,6 +2891,24 @@ __global__ __launch_bounds__(BX * BY) void ss_update_kernel(GridDev g, SubstepDe
 		for (int q = 0; q < 3; q++) d.S[(5 + q) * n + c] = vb[q];
 	}
 	d.S[4 * n + c] = En;
+}
+
+/* the update, with WALK over the blocks of its list (those with a cell
+   that may be due), else over every block */
+template <bool WALK>
+__global__ __launch_bounds__(BX * BY) void ss_update_kernel(GridDev g, SubstepDev d, int sub)
+{
+	if constexpr (WALK) {
+		const int cnt = d.count[SS_LISTS * (sub & 1) + SS_LIST_UPDATE];
+		const int* list = d.list + (size_t)SS_LIST_UPDATE * d.nblk;
+		for (int idx = blockIdx.x; idx < cnt; idx += gridDim.x) {
+			int bx, by, bz;
+			ss_block_coords(g, list[idx], bx, by, bz);
+			ss_update_block(g, d, sub, bx, by, bz);
+		}
+	} else {
+		ss_update_block(g, d, sub, blockIdx.x, blockIdx.y, blockIdx.z);
+	}
 }
 
 /* sync_magnetic_field, common_functions.hpp:184-269: the two visits to a
@@ -3492,6 +3619,8 @@ void substep_free(SubstepDev* d)
 	(void)hipFree(d->minsub);
 	(void)hipFree(d->blk);
 	(void)hipFree(d->dirty);
+	(void)hipFree(d->list);
+	(void)hipFree(d->count);
 	(void)hipFree(d->opt_dev);
 	(void)hipFree(d->types);
 	(void)hipFree(d->bmask);
@@ -3528,6 +3657,8 @@ int substep_alloc(pamhd_handle* h)
 	}
 	al(&d->blk, 4 * (size_t)d->nblk * sizeof(int));
 	al(&d->dirty, 3 * (size_t)d->nblk * sizeof(int));
+	al(&d->list, SS_LISTS * (size_t)d->nblk * sizeof(int));
+	al(&d->count, 2 * SS_LISTS * sizeof(int));
 	al(&d->scal, 5 * SS_SLOTS * sizeof(unsigned long long));
 	al(&d->sub_dt, sizeof(double));
 	al(&d->status, sizeof(StepStatus));
@@ -3549,13 +3680,33 @@ int substep_alloc(pamhd_handle* h)
 	return PAMHD_OK;
 }
 
-template <int SOLVER>
-void ss_launch_flux(pamhd_handle* h, double* P, int sub, hipStream_t st)
+/* grid of the passes that walk a block list */
+dim3 ss_walk_grid(const SubstepDev& d)
 {
-	const dim3 grid = cell_grid(h->g), block(BX, BY, 1);
-	ss_flux_kernel<SOLVER, 0><<<grid, block, 0, st>>>(h->g, *h->sub, h->bg, P, sub);
-	ss_flux_kernel<SOLVER, 1><<<grid, block, 0, st>>>(h->g, *h->sub, h->bg, P, sub);
-	ss_flux_kernel<SOLVER, 2><<<grid, block, 0, st>>>(h->g, *h->sub, h->bg, P, sub);
+	return dim3((unsigned)(d.nblk < SS_WALK_BLOCKS ? d.nblk : SS_WALK_BLOCKS), 1, 1);
+}
+
+/* the flux passes of substep `sub`; at a substep where every face is due
+   (full) they cover the whole cell grid, and no edge field needs zeroing */
+template <int SOLVER>
+void ss_launch_flux(pamhd_handle* h, double* P, int sub, bool full, hipStream_t st)
+{
+	const SubstepDev& d = *h->sub;
+	const dim3 block(BX, BY, 1);
+	if (full) {
+		const dim3 grid = cell_grid(h->g);
+		ss_flux_kernel<SOLVER, 0, false><<<grid, block, 0, st>>>(h->g, d, h->bg, P, sub);
+		ss_flux_kernel<SOLVER, 1, false><<<grid, block, 0, st>>>(h->g, d, h->bg, P, sub);
+		ss_flux_kernel<SOLVER, 2, false><<<grid, block, 0, st>>>(h->g, d, h->bg, P, sub);
+		return;
+	}
+	const dim3 grid = ss_walk_grid(d);
+	ss_zero_emf_kernel<0><<<grid, block, 0, st>>>(h->g, d, P, sub);
+	ss_zero_emf_kernel<1><<<grid, block, 0, st>>>(h->g, d, P, sub);
+	ss_zero_emf_kernel<2><<<grid, block, 0, st>>>(h->g, d, P, sub);
+	ss_flux_kernel<SOLVER, 0, true><<<grid, block, 0, st>>>(h->g, d, h->bg, P, sub);
+	ss_flux_kernel<SOLVER, 1, true><<<grid, block, 0, st>>>(h->g, d, h->bg, P, sub);
+	ss_flux_kernel<SOLVER, 2, true><<<grid, block, 0, st>>>(h->g, d, h->bg, P, sub);
 }
 
 /* apply_boundaries and the main loop's sync_magnetic_field on a grid with
@@ -3620,15 +3771,22 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 	double total = 0;
 	for (int sub = 1; sub <= maxp; sub++) {
 		total += sub_dt;
+		/* the last substep has every face and cell due (periods are powers
+		   of two up to maxp), and at the first every face-dB sum restarts:
+		   those passes cover the whole grid; the others walk their lists */
+		const bool full = sub == maxp;
+		ss_lists_kernel<<<(unsigned)((d.nblk + 255) / 256), 256, 0, st>>>(h->g, d, sub);
 		switch (solver) {
-		case RUSANOV: ss_launch_flux<RUSANOV>(h, P, sub, st); break;
-		case HLL_ATHENA: ss_launch_flux<HLL_ATHENA>(h, P, sub, st); break;
-		case HLLD_ATHENA: ss_launch_flux<HLLD_ATHENA>(h, P, sub, st); break;
-		case ROE_ATHENA: ss_launch_flux<ROE_ATHENA>(h, P, sub, st); break;
+		case RUSANOV: ss_launch_flux<RUSANOV>(h, P, sub, full, st); break;
+		case HLL_ATHENA: ss_launch_flux<HLL_ATHENA>(h, P, sub, full, st); break;
+		case HLLD_ATHENA: ss_launch_flux<HLLD_ATHENA>(h, P, sub, full, st); break;
+		case ROE_ATHENA: ss_launch_flux<ROE_ATHENA>(h, P, sub, full, st); break;
 		default: return fail(h, PAMHD_E_SOLVER, "invalid solver " + std::to_string(solver));
 		}
-		ss_dbsum_kernel<<<grid, block, 0, st>>>(h->g, d, P, sub);
-		ss_update_kernel<<<grid, block, 0, st>>>(h->g, d, sub);
+		if (full || sub == 1) ss_dbsum_kernel<false><<<grid, block, 0, st>>>(h->g, d, P, sub);
+		else ss_dbsum_kernel<true><<<ss_walk_grid(d), block, 0, st>>>(h->g, d, P, sub);
+		if (full) ss_update_kernel<false><<<grid, block, 0, st>>>(h->g, d, sub);
+		else ss_update_kernel<true><<<ss_walk_grid(d), block, 0, st>>>(h->g, d, sub);
 	}
 	ss_sync_kernel<0><<<grid, block, 0, st>>>(h->g, d);
 	ss_sync_kernel<1><<<grid, block, 0, st>>>(h->g, d);
