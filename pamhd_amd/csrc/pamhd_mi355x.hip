@@ -2222,15 +2222,16 @@ struct SubstepDev {
 	int nblk = 0;
 	/* per substep, the blocks each pass works on (ss_lists_kernel): lists
 	   SS_LISTS x nblk (the x-, y-, z-flux passes, the face-dB pass, the
-	   update, and the blocks of each direction whose edge fields only need
-	   zeroing), and their lengths, two sets of SS_LISTS used by odd and
+	   update, the blocks of each direction whose edge fields only need
+	   zeroing, and the face-dB pass's blocks whose dBs are all +0), and
+	   their lengths, two sets of SS_LISTS used by odd and
 	   even substeps */
 	int* list = nullptr;
 	int* count = nullptr;
 };
 
 /* lists of ss_lists_kernel; the grid of the passes that walk them */
-enum { SS_LIST_DB = 3, SS_LIST_UPDATE = 4, SS_LIST_ZERO = 5, SS_LISTS = 8 };
+enum { SS_LIST_DB = 3, SS_LIST_UPDATE = 4, SS_LIST_ZERO = 5, SS_LIST_DB0 = 8, SS_LISTS = 9 };
 constexpr int SS_WALK_BLOCKS = 4096;
 
 /* index of a neighbour coordinate: periodic dimensions wrap (within one
@@ -2525,9 +2526,10 @@ cell grid, appended in any order (the blocks are independent):
                 none whose edge fields of the previous substep are not yet
                 known to be zero goes to the zeroing list of D instead; the
                 flag is updated here
-  face-dB pass  a sum of the block that restarts at this substep, or a
-                block around it (27, three directions) with a face due
-                (else every dB is +0 and no sum changes)
+  face-dB pass  a block around it (27, three directions) with a face due;
+                else, a sum of the block that restarts at this substep goes
+                to the list of blocks whose dBs are all +0 (else no sum
+                changes)
   update        a cell of the block (or a lower neighbour) due: periods are
                 powers of two, so none is when the smallest is not
 Thread 0 also zeroes the other set of lengths, which the next substep uses
@@ -2540,7 +2542,7 @@ __global__ __launch_bounds__(256) void ss_lists_kernel(GridDev g, SubstepDev d, 
 	if (b == 0) {
 		for (int q = 0; q < SS_LISTS; q++) d.count[SS_LISTS * ((sub + 1) & 1) + q] = 0;
 	}
-	bool run[SS_LISTS] = {false, false, false, false, false, false, false, false};
+	bool run[SS_LISTS] = {false, false, false, false, false, false, false, false, false};
 	if (b < d.nblk) {
 		for (int e = 0; e < 3; e++) {
 			const bool due = (sub & (d.blk[e * d.nblk + b] - 1)) == 0;
@@ -2550,17 +2552,19 @@ __global__ __launch_bounds__(256) void ss_lists_kernel(GridDev g, SubstepDev d, 
 		}
 		const int gx = (g.nx + BX - 1) / BX, gy = (g.ny + BY - 1) / BY, gz = g.nz;
 		const int bx0 = b % gx, by0 = (b / gx) % gy, bz0 = b / (gx * gy);
-		int q = ((sub - 1) & (d.blk[3 * d.nblk + b] - 1)) != 0;
-		for (int dz = -1; dz <= 1 && q; dz++)
-		for (int dy = -1; dy <= 1 && q; dy++)
-		for (int dx = -1; dx <= 1 && q; dx++) {
+		const bool restart = ((sub - 1) & (d.blk[3 * d.nblk + b] - 1)) == 0;
+		bool near = false;
+		for (int dz = -1; dz <= 1 && !near; dz++)
+		for (int dy = -1; dy <= 1 && !near; dy++)
+		for (int dx = -1; dx <= 1 && !near; dx++) {
 			const int bx = (bx0 + dx + gx) % gx, by = (by0 + dy + gy) % gy, bz = (bz0 + dz + gz) % gz;
 			const int nbk = bx + gx * (by + gy * bz);
 			for (int e = 0; e < 3; e++) {
-				if ((sub & (d.blk[e * d.nblk + nbk] - 1)) == 0) q = 0;
+				if ((sub & (d.blk[e * d.nblk + nbk] - 1)) == 0) near = true;
 			}
 		}
-		run[SS_LIST_DB] = !q;
+		run[SS_LIST_DB] = near;
+		run[SS_LIST_DB0] = restart && !near;
 		run[SS_LIST_UPDATE] = (sub & (d.blk[3 * d.nblk + b] - 1)) == 0;
 	}
 	const unsigned long long lt = (1ull << (threadIdx.x % 64)) - 1;
@@ -2756,6 +2760,10 @@ the cells it serves applied (x % p is x & (p - 1): periods are powers of
 two).  A cell computes the sums of the faces it owns (its -x, -y, -z
 faces) and of its +d faces at a closed end (kept in its end slots).
 */
+/* ZERO_DB: no face around the block is due, so every edge field its faces
+   use is zero and every dB is +0 (a sum of +-0 terms starting at +0); the
+   sums are written as they would be, without the edge terms */
+template <bool ZERO_DB>
 __device__ __forceinline__ void ss_dbsum_block(const GridDev& g, const SubstepDev& d,
 	const double* __restrict__ emf, int sub, int bx, int by, int k)
 {
@@ -2776,32 +2784,43 @@ __device__ __forceinline__ void ss_dbsum_block(const GridDev& g, const SubstepDe
 		fdb[slow_i] = ((((sub - 1) & (pmax - 1)) == 0) ? 0.0 : fdb[slow_i]) + db;
 		if (pmin < pmax) fdb[fast_i] = ((((sub - 1) & (pmin - 1)) == 0) ? 0.0 : fdb[fast_i]) + db;
 	};
-	accumulate(c, 3 * n + c, L(im, j, k), dBx_at(E, L, i, im, j, jm, jp, k, km, kp));
-	accumulate(n + c, 4 * n + c, L(i, jm, k), dBy_at(E, L, i, im, ip, j, jm, k, km, kp));
-	accumulate(2 * n + c, 5 * n + c, L(i, j, km), dBz_at(E, L, i, im, ip, j, jm, jp, k, km));
-	if (ip < 0) accumulate(6 * n + c, 6 * n + c, -1, dBx_at(E, L, ip, i, j, jm, jp, k, km, kp));
-	if (jp < 0) accumulate(7 * n + c, 7 * n + c, -1, dBy_at(E, L, i, im, ip, jp, j, k, km, kp));
-	if (kp < 0) accumulate(8 * n + c, 8 * n + c, -1, dBz_at(E, L, i, im, ip, j, jm, jp, kp, k));
+	if constexpr (ZERO_DB) {
+		(void)E;
+		accumulate(c, 3 * n + c, L(im, j, k), 0.0);
+		accumulate(n + c, 4 * n + c, L(i, jm, k), 0.0);
+		accumulate(2 * n + c, 5 * n + c, L(i, j, km), 0.0);
+		if (ip < 0) accumulate(6 * n + c, 6 * n + c, -1, 0.0);
+		if (jp < 0) accumulate(7 * n + c, 7 * n + c, -1, 0.0);
+		if (kp < 0) accumulate(8 * n + c, 8 * n + c, -1, 0.0);
+	} else {
+		accumulate(c, 3 * n + c, L(im, j, k), dBx_at(E, L, i, im, j, jm, jp, k, km, kp));
+		accumulate(n + c, 4 * n + c, L(i, jm, k), dBy_at(E, L, i, im, ip, j, jm, k, km, kp));
+		accumulate(2 * n + c, 5 * n + c, L(i, j, km), dBz_at(E, L, i, im, ip, j, jm, jp, k, km));
+		if (ip < 0) accumulate(6 * n + c, 6 * n + c, -1, dBx_at(E, L, ip, i, j, jm, jp, k, km, kp));
+		if (jp < 0) accumulate(7 * n + c, 7 * n + c, -1, dBy_at(E, L, i, im, ip, jp, j, k, km, kp));
+		if (kp < 0) accumulate(8 * n + c, 8 * n + c, -1, dBz_at(E, L, i, im, ip, j, jm, jp, kp, k));
+	}
 }
 
-/* the face-dB pass, with WALK over the blocks of its list (a block none of
-   whose sums restarts at this substep, and around which no block has a
-   face due, is not in it: all edge fields its faces use are zero, so every
-   dB is +0 and no sum changes), else over every block */
-template <bool WALK>
+/* the face-dB pass, with WALK over the blocks of its list (those around
+   which a face is due) and, with ZERO_DB, of its list of blocks with a sum
+   that restarts and no face due around them (a block with neither is in
+   no list: every dB is +0 and no sum changes); else over every block */
+template <bool WALK, bool ZERO_DB>
 __global__ __launch_bounds__(BX * BY) void ss_dbsum_kernel(GridDev g, SubstepDev d,
 	const double* __restrict__ emf, int sub)
 {
 	if constexpr (WALK) {
-		const int cnt = d.count[SS_LISTS * (sub & 1) + SS_LIST_DB];
-		const int* list = d.list + (size_t)SS_LIST_DB * d.nblk;
+		const int l = ZERO_DB ? SS_LIST_DB0 : SS_LIST_DB;
+		const int cnt = d.count[SS_LISTS * (sub & 1) + l];
+		const int* list = d.list + (size_t)l * d.nblk;
 		for (int idx = blockIdx.x; idx < cnt; idx += gridDim.x) {
 			int bx, by, bz;
 			ss_block_coords(g, list[idx], bx, by, bz);
-			ss_dbsum_block(g, d, emf, sub, bx, by, bz);
+			ss_dbsum_block<ZERO_DB>(g, d, emf, sub, bx, by, bz);
 		}
 	} else {
-		ss_dbsum_block(g, d, emf, sub, blockIdx.x, blockIdx.y, blockIdx.z);
+		ss_dbsum_block<false>(g, d, emf, sub, blockIdx.x, blockIdx.y, blockIdx.z);
 	}
 }
 
@@ -3772,8 +3791,8 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 	for (int sub = 1; sub <= maxp; sub++) {
 		total += sub_dt;
 		/* the last substep has every face and cell due (periods are powers
-		   of two up to maxp), and at the first every face-dB sum restarts:
-		   those passes cover the whole grid; the others walk their lists */
+		   of two up to maxp): its passes cover the whole grid; the others
+		   walk their lists */
 		const bool full = sub == maxp;
 		ss_lists_kernel<<<(unsigned)((d.nblk + 255) / 256), 256, 0, st>>>(h->g, d, sub);
 		switch (solver) {
@@ -3783,8 +3802,12 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 		case ROE_ATHENA: ss_launch_flux<ROE_ATHENA>(h, P, sub, full, st); break;
 		default: return fail(h, PAMHD_E_SOLVER, "invalid solver " + std::to_string(solver));
 		}
-		if (full || sub == 1) ss_dbsum_kernel<false><<<grid, block, 0, st>>>(h->g, d, P, sub);
-		else ss_dbsum_kernel<true><<<ss_walk_grid(d), block, 0, st>>>(h->g, d, P, sub);
+		if (full) {
+			ss_dbsum_kernel<false, false><<<grid, block, 0, st>>>(h->g, d, P, sub);
+		} else {
+			ss_dbsum_kernel<true, true><<<ss_walk_grid(d), block, 0, st>>>(h->g, d, P, sub);
+			ss_dbsum_kernel<true, false><<<ss_walk_grid(d), block, 0, st>>>(h->g, d, P, sub);
+		}
 		if (full) ss_update_kernel<false><<<grid, block, 0, st>>>(h->g, d, sub);
 		else ss_update_kernel<true><<<ss_walk_grid(d), block, 0, st>>>(h->g, d, sub);
 	}
