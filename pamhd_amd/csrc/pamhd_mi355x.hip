@@ -2253,26 +2253,6 @@ __device__ __forceinline__ int cidx(const GridDev& g, int i, int j, int k)
 	return i + g.nx * (j + g.ny * k);
 }
 
-/* the face and edge neighbours of (i, j, k) that exist and are of type
-   >= 0, in the order of the neighbourhood */
-template <class F> __device__ __forceinline__ void for_face_edge_neighbours(const GridDev& g,
-	const SubstepDev& d, int i, int j, int k, F&& f)
-{
-	for (int dk = -1; dk <= 1; dk++)
-	for (int dj = -1; dj <= 1; dj++)
-	for (int di = -1; di <= 1; di++) {
-		const int nz_ = (di != 0) + (dj != 0) + (dk != 0);
-		if (nz_ == 0 || nz_ == 3) continue;
-		const int a = ss_coord(i + di, g.nx, d.per[0]);
-		const int b = ss_coord(j + dj, g.ny, d.per[1]);
-		const int c = ss_coord(k + dk, g.nz, d.per[2]);
-		if (a < 0 || b < 0 || c < 0) continue;
-		const int id = cidx(g, a, b, c);
-		if (!ss_active(d, id)) continue;
-		f(id);
-	}
-}
-
 /* the step's five reductions (0: smallest cell time step, 1: largest
    period, 2: smallest time step / 2^substep-min, 3: largest level, 4: faces
    solved) are spread over SS_SLOTS slots each, a wave's atomic going to the
@@ -2424,16 +2404,71 @@ __global__ __launch_bounds__(BX * BY) void ss_level_kernel(GridDev g, SubstepDev
 		atomicMax(&d.scal[3 * SS_SLOTS + ss_slot()], (unsigned long long)lev);
 }
 
-/* one Jacobi pass of restrict_substepping_period, substepping.hpp:99-191 */
+/* the levels of the cells around this BX x BY block of plane k (planes
+   k - 1 .. k + 1, one cell of halo) in LDS, SS_NONE where there is no cell
+   (beyond a closed end), the cell is not solved, or no thread of the block
+   reads the entry */
+constexpr int SS_NONE = INT_MAX / 2;
+struct LevelTile {
+	int v[3][BY + 2][BX + 2];
+};
+
+__device__ __forceinline__ void load_level_tile(const GridDev& g, const SubstepDev& d,
+	const int* __restrict__ e, LevelTile& T)
+{
+	const int k = blockIdx.z;
+	const int i0 = (int)blockIdx.x * BX - 1, j0 = (int)blockIdx.y * BY - 1;
+	const int ihi = min(g.nx - 1, (int)blockIdx.x * BX + BX - 1) + 1;
+	const int jhi = min(g.ny - 1, (int)blockIdx.y * BY + BY - 1) + 1;
+	const int tid = threadIdx.x + BX * threadIdx.y;
+	for (int t = tid; t < 3 * (BY + 2) * (BX + 2); t += BX * BY) {
+		const int ti = t % (BX + 2), tj = (t / (BX + 2)) % (BY + 2), tk = t / ((BX + 2) * (BY + 2));
+		const int vi = i0 + ti, vj = j0 + tj;
+		int val = SS_NONE;
+		if (vi <= ihi && vj <= jhi) {
+			const int a = ss_coord(vi, g.nx, d.per[0]), b = ss_coord(vj, g.ny, d.per[1]);
+			const int c = ss_coord(k + tk - 1, g.nz, d.per[2]);
+			if (a >= 0 && b >= 0 && c >= 0) {
+				const int id = cidx(g, a, b, c);
+				if (ss_active(d, id)) val = e[id];
+			}
+		}
+		T.v[tk][tj][ti] = val;
+	}
+	__syncthreads();
+}
+
+/* f(value) for each of the 18 face and edge neighbours of the thread's cell
+   in the tile (offsets with one or two nonzero coordinates), SS_NONE for
+   those that are no cell or not solved */
+template <class F> __device__ __forceinline__ void for_tile_neighbours(const LevelTile& T, F&& f)
+{
+	const int ti = threadIdx.x + 1, tj = threadIdx.y + 1;
+#pragma unroll
+	for (int dk = -1; dk <= 1; dk++)
+#pragma unroll
+	for (int dj = -1; dj <= 1; dj++)
+#pragma unroll
+	for (int di = -1; di <= 1; di++) {
+		const int nz_ = (di != 0) + (dj != 0) + (dk != 0);
+		if (nz_ == 0 || nz_ == 3) continue;
+		f(T.v[1 + dk][tj + dj][ti + di]);
+	}
+}
+
+/* one Jacobi pass of restrict_substepping_period, substepping.hpp:99-191
+   (a minimum over the neighbours, so their order does not matter) */
 __global__ __launch_bounds__(BX * BY) void ss_restrict_kernel(GridDev g, SubstepDev d,
 	const int* __restrict__ in, int* __restrict__ out)
 {
+	__shared__ LevelTile T;
+	load_level_tile(g, d, in, T);
 	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
 	if (i >= g.nx || j >= g.ny) return;
 	const int c = cidx(g, i, j, k);
 	int v = in[c];
 	if (ss_active(d, c)) {
-		for_face_edge_neighbours(g, d, i, j, k, [&](int nb) { if (v > in[nb] + 1) v = in[nb] + 1; });
+		for_tile_neighbours(T, [&](int w) { if (v > w + 1) v = w + 1; });
 	}
 	out[c] = v;
 }
@@ -2443,13 +2478,15 @@ __global__ __launch_bounds__(BX * BY) void ss_restrict_kernel(GridDev g, Substep
    source/mhd/solve.hpp:252-289) */
 __global__ __launch_bounds__(BX * BY) void ss_period_kernel(GridDev g, SubstepDev d, const int* __restrict__ e)
 {
+	__shared__ LevelTile T;
+	load_level_tile(g, d, e, T);
 	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
 	unsigned long long p = 0;
 	if (i < g.nx && j < g.ny) {
 		const int c = cidx(g, i, j, k);
 		int lo = e[c];
 		const bool act = ss_active(d, c);
-		if (act) for_face_edge_neighbours(g, d, i, j, k, [&](int nb) { if (e[nb] < lo) lo = e[nb]; });
+		if (act) for_tile_neighbours(T, [&](int w) { if (w < lo) lo = w; });
 		d.period[c] = 1 << e[c];
 		d.minsub[c] = 1 << lo;
 		p = act ? 1ull << e[c] : 0ull;
