@@ -373,7 +373,7 @@ def adjoint_run(n=512, steps=200, every=20, directions=32, seed=5):
     A_z), HLL, `steps` steps on the frozen schedule of a CFL-0.3 run; loss =
     magnetic energy of a fixed 64 x 64 window at the end.  Times the forward
     rollout and the backward (hand-written adjoint, checkpoints every
-    `every` steps) and checks the gradient against central differences
+    `every` steps; medians of five repetitions) and checks the gradient against central differences
     (h = 1e-6) along `directions` smooth random directions (random-Fourier
     fields of every state component) and for the 4 amplitudes."""
     import numpy as np
@@ -394,15 +394,21 @@ def adjoint_run(n=512, steps=200, every=20, directions=32, seed=5):
         x = U0.clone().requires_grad_(True)
         loss(rollout(x, g, "hll-athena", dts, every)).backward()
         torch.cuda.synchronize()
-        a = amps.clone().requires_grad_(True)
-        t0 = time.perf_counter()
-        L = loss(rollout(configs.adjoint_initial_state(w, a), g, "hll-athena", dts, every))
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        L.backward()
-        torch.cuda.synchronize()
-        t2 = time.perf_counter()
-        fwd_ms, bwd_ms = (t1 - t0) * 1e3, (t2 - t1) * 1e3
+        # five timed repetitions (one ~25 ms backward is a noisy sample);
+        # the medians are reported, with their ranges
+        fwd, bwd = [], []
+        for _ in range(5):
+            a = amps.clone().requires_grad_(True)
+            t0 = time.perf_counter()
+            L = loss(rollout(configs.adjoint_initial_state(w, a), g, "hll-athena", dts, every))
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            L.backward()
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            fwd.append((t1 - t0) * 1e3)
+            bwd.append((t2 - t1) * 1e3)
+        fwd_ms, bwd_ms = float(np.median(fwd)), float(np.median(bwd))
         x = U0.clone().requires_grad_(True)
         loss(rollout(x, g, "hll-athena", dts, every)).backward()
         lam0 = x.grad.detach()
@@ -428,6 +434,8 @@ def adjoint_run(n=512, steps=200, every=20, directions=32, seed=5):
                 amp_errs.append(abs(fd - ga[q]) / max(abs(fd), abs(ga[q]), 1e-300))
     return {"cells": [n, n, 1], "steps": steps, "checkpoint_every": every, "solver": "hll-athena",
             "loss": float(L.item()), "forward_ms": round(fwd_ms, 3), "backward_ms": round(bwd_ms, 3),
+            "forward_ms_range": [round(min(fwd), 3), round(max(fwd), 3)],
+            "backward_ms_range": [round(min(bwd), 3), round(max(bwd), 3)], "repetitions": len(fwd),
             "backward_over_forward": bwd_ms / fwd_ms, "fd_directions": directions,
             "fd_max_rel_err": max(errs), "amplitude_grad": [float(v) for v in ga],
             "amplitude_fd_rel_err": amp_errs}
