@@ -234,6 +234,12 @@ int pamhd_set_poisson_smoother(pamhd_handle* h, int fused);
    substepping.hpp:376-384), fails the step with PAMHD_E_SUBSTEP. */
 int pamhd_set_substepping(pamhd_handle* h, int substep_min, int substep_max);
 
+/* Launch shape of the substepped step: the passes of a substep walk lists
+   of the 64 x 4-cell blocks that have work from this many workgroups (0:
+   the default, 4096; fewer blocks than that: one workgroup per block).  All
+   give identical results.  (No counterpart in the reference.) */
+int pamhd_set_substep_walk(pamhd_handle* h, int workgroups);
+
 /* The same with each cell's own options (n ints each, x fastest), as the
    reference evaluates string-valued "substep-min" / "substep-max" at the
    cell centres and the current time before every step

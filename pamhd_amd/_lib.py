@@ -35,7 +35,7 @@ EXPORTS = (
     "pamhd_ensemble_check", "pamhd_ensemble_download", "pamhd_ensemble_cells",
     "pamhd_ensemble_set_tile", "pamhd_ensemble_autotune", "pamhd_ensemble_last_error",
     "pamhd_step_fixed", "pamhd_step_fixed_from", "pamhd_set_state_device", "pamhd_get_state_device",
-    "pamhd_adjoint_step", "pamhd_get_magnetic_fluxes", "pamhd_adjoint_step_fluxes", "pamhd_adjoint_steps", "pamhd_set_slabs", "pamhd_set_fused_step", "pamhd_set_substepping", "pamhd_set_substepping_cells",
+    "pamhd_adjoint_step", "pamhd_get_magnetic_fluxes", "pamhd_adjoint_step_fluxes", "pamhd_adjoint_steps", "pamhd_set_slabs", "pamhd_set_fused_step", "pamhd_set_substepping", "pamhd_set_substep_walk", "pamhd_set_substepping_cells",
     "pamhd_last_max_substep", "pamhd_last_flux_calcs", "pamhd_last_substep_periods",
 )
 
@@ -165,6 +165,7 @@ def load():
     L.pamhd_ensemble_autotune.argtypes = [E, ctypes.c_int, ctypes.c_double, ctypes.c_char_p,
                                           ctypes.c_size_t, _vp]
     L.pamhd_ensemble_last_error.argtypes = [E, ctypes.c_char_p, ctypes.c_size_t]
+    L.pamhd_set_substep_walk.argtypes = [H, ctypes.c_int]
     L.pamhd_step_fixed.argtypes = [H, ctypes.c_int, ctypes.c_double, _vp]
     L.pamhd_step_fixed_from.argtypes = [H, ctypes.c_int, ctypes.c_double, _vp, _vp, _vp, _vp]
     L.pamhd_set_state_device.argtypes = [H, _vp, _vp]

@@ -2228,6 +2228,7 @@ struct SubstepDev {
 	   even substeps */
 	int* list = nullptr;
 	int* count = nullptr;
+	int walk = 0;   /* workgroups of the passes that walk a list */
 };
 
 /* lists of ss_lists_kernel; the grid of the passes that walk them */
@@ -3091,6 +3092,7 @@ struct pamhd_handle {
 	int mg_fused = 0;   /* per-colour passes (faster; see pamhd_set_poisson_smoother) */
 	int adj_reverse = 1;   /* flux Jacobian of the adjoint: 1 reverse accumulation, 0 dual numbers */
 	int flux_step = 0;     /* the last step left its FP64 dt * F in flux (pamhd_get_magnetic_fluxes) */
+	int ss_walk = 0;       /* workgroups of the substep passes that walk block lists (0: default) */
 	double* pe_out = nullptr;   /* where the update pass also writes the step's magnetic fluxes */
 	int cur = 0;    /* state buffer holding the current state */
 	int last = 0;   /* status entry of the last flux pass */
@@ -3711,6 +3713,7 @@ int substep_alloc(pamhd_handle* h)
 		const dim3 cg = cell_grid(h->g);
 		d->nblk = (int)(cg.x * cg.y * cg.z);
 	}
+	d->walk = h->ss_walk;
 	al(&d->blk, 4 * (size_t)d->nblk * sizeof(int));
 	al(&d->dirty, 3 * (size_t)d->nblk * sizeof(int));
 	al(&d->list, SS_LISTS * (size_t)d->nblk * sizeof(int));
@@ -3739,7 +3742,8 @@ int substep_alloc(pamhd_handle* h)
 /* grid of the passes that walk a block list */
 dim3 ss_walk_grid(const SubstepDev& d)
 {
-	return dim3((unsigned)(d.nblk < SS_WALK_BLOCKS ? d.nblk : SS_WALK_BLOCKS), 1, 1);
+	const int w = d.walk > 0 ? d.walk : SS_WALK_BLOCKS;
+	return dim3((unsigned)(d.nblk < w ? d.nblk : w), 1, 1);
 }
 
 /* the flux passes of substep `sub`; at a substep where every face is due
@@ -4508,6 +4512,15 @@ int pamhd_set_substepping(pamhd_handle* h, int substep_min, int substep_max)
 		h->sub->opt = nullptr;
 		h->sub->smax = substep_max;
 	}
+	return PAMHD_OK;
+}
+
+int pamhd_set_substep_walk(pamhd_handle* h, int workgroups)
+{
+	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
+	if (workgroups < 0) return fail(h, PAMHD_E_ARG, "workgroups must be >= 0");
+	h->ss_walk = workgroups;
+	if (h->sub) h->sub->walk = workgroups;
 	return PAMHD_OK;
 }
 

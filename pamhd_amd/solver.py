@@ -185,6 +185,11 @@ class Grid:
         self._check(self._L.pamhd_autotune(self._h, solver_id(solver), buf, 4096, stream))
         return buf.value.decode()
 
+    def set_substep_walk(self, workgroups=0):
+        """Workgroups of the substep passes that walk their block lists (0:
+        the default); every value gives the same results."""
+        self._check(self._L.pamhd_set_substep_walk(self._h, int(workgroups)))
+
     def set_substepping(self, substep_min=0, substep_max=0):
         """Per-cell 2^N substepping ("substep-min", "substep-max"): ints for
         every cell, or arrays of each cell's values (x fastest, set again

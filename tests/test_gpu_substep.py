@@ -37,10 +37,13 @@ def test_substepped_run_bit_exact(gpu, name, sid, smax, cells):
     np.testing.assert_array_equal(out, ref)
 
 
-def test_substepped_run_many_blocks_per_reduction_slot(gpu):
+@pytest.mark.parametrize("walk", [0, 7])
+def test_substepped_run_many_blocks_per_reduction_slot(gpu, walk):
     """The step's reductions (smallest time step, largest level and period,
     faces solved) go to 64 slots by block index; a grid of 320 blocks puts
-    several blocks into each slot (the reference's default options 0, 999)."""
+    several blocks into each slot (the reference's default options 0, 999).
+    With 7 workgroups walking the substep passes' block lists, each walks
+    many entries of a list, as on the 256^3 grid's 65 536 blocks."""
     from pamhd_amd.solver import Grid
     cells = (64, 32, 40)
     state, length, gamma, mu0 = configs.hot_blob(cells, contrast=30.0, radius=0.25, seed=6)
@@ -52,6 +55,7 @@ def test_substepped_run_many_blocks_per_reduction_slot(gpu):
     fc_o = O.last_flux_calcs()
     with Grid(cells, length, gamma, mu0) as g:
         g.upload(state)
+        g.set_substep_walk(walk)
         g.set_substepping(0, 999)
         steps_g, t_g = g.run("hlld-athena", 0.3, 0.0, 1e9, max_steps=4)
         mp_g = g.last_max_substep()
