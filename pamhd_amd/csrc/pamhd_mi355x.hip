@@ -2752,6 +2752,108 @@ __global__ __launch_bounds__(BX * BY) void ss_flux_kernel(GridDev g, SubstepDev 
 	}
 }
 
+/*
+The flux pass of a substep at which every face is due, for HLLD's y- (D = 1)
+and z-faces (D = 2) on a periodic grid without cell types: a thread walks
+km faces of a line along D with HLLD's per-cell part (hlld_side) made once
+per cell and carried to the next face, as flux_hlld_march_kernel does
+(flux_hlld is the two parts, the side error order and flux_hlld_face, so
+the fluxes are ss_flux_block's bit for bit); each face's flux then goes
+into the substep's sums and edge fields as in ss_flux_block.  Blocks: x
+tiles fastest, then BY-line groups, then chunks of km along D.
+*/
+template <int D>
+__global__ __launch_bounds__(BX * BY) void ss_flux_march_kernel(GridDev g, SubstepDev d,
+	double* __restrict__ emf, int sub, unsigned gx, int km)
+{
+	static_assert(D == 1 || D == 2, "march along y or z");
+	const int nt = D == 1 ? g.nz : g.ny;                 /* lines across */
+	const unsigned gt = (unsigned)((nt + BY - 1) / BY);
+	const unsigned bx = blockIdx.x % gx, r = blockIdx.x / gx;
+	const int t = (int)(r % gt) * BY + threadIdx.y;
+	const int chunk = (int)(r / gt);
+	const int i = bx * BX + threadIdx.x;
+	const int nd = D == 1 ? g.ny : g.nz;
+	const int a0 = chunk * km;
+	const int a1 = min(a0 + km, nd);
+	const long long n = g.n;
+	unsigned solved = 0;
+	if (i < g.nx && t < nt && a0 < a1) {
+		const int jj = D == 1 ? 0 : t, kk = D == 1 ? t : 0;
+		auto cell = [&](int a) { return D == 1 ? cidx(g, i, a, kk) : cidx(g, i, jj, a); };
+		auto load = [&](int x, double U[8]) {
+			const double mm[3] = {d.S[1 * n + x], d.S[2 * n + x], d.S[3 * n + x]};
+			const double B[3] = {d.S[5 * n + x], d.S[6 * n + x], d.S[7 * n + x]};
+			U[RHO] = d.S[x];
+			U[NRJ] = d.S[4 * n + x];
+			U[M0] = mm[D]; U[M1] = mm[(D + 1) % 3]; U[M2] = mm[(D + 2) % 3];
+			U[B0] = B[D]; U[B1] = B[(D + 1) % 3]; U[B2] = B[(D + 2) % 3];
+		};
+		double* A = d.slot + (long long)D * 8 * n;
+		double* A2 = d.slot + (long long)(3 + D) * 8 * n;
+		double* Em = emf + (long long)(2 * D) * n;
+		int o = cell(wrap_dec(a0, nd));
+		Side<double> Ls;
+		double cfl = 0;
+		int code_l, stage_l;
+		{
+			double U[8];
+			load(o, U);
+			stage_l = hlld_side<double>(U, g.ph, Ls, cfl, code_l);
+		}
+		for (int a = a0; a < a1; a++) {
+			const int c = cell(a);
+			Side<double> Rs;
+			double cfr = 0;
+			int code_r, stage_r;
+			{
+				double U[8];
+				load(c, U);
+				stage_r = hlld_side<double>(U, g.ph, Rs, cfr, code_r);
+			}
+			double F[8], sT = 0;
+			int e;
+			if (stage_l | stage_r) e = first_side_error(stage_l, code_l, stage_r, code_r);
+			else e = flux_hlld_face<double>(Ls, Rs, cfl, cfr, Ls.u, Rs.u, g.ph, F, sT);
+			double Fg[8];
+			if (e) {
+				record_error(d.status, e, (long long)D * n + c);
+				for (int q = 0; q < 8; q++) Fg[q] = 0;
+			} else {
+				d.maxv[(long long)D * n + c] = sT;
+				Fg[RHO] = F[RHO];
+				Fg[NRJ] = F[NRJ];
+				Fg[M0 + D] = F[M0]; Fg[M0 + (D + 1) % 3] = F[M1]; Fg[M0 + (D + 2) % 3] = F[M2];
+				Fg[B0 + D] = F[B0]; Fg[B0 + (D + 1) % 3] = F[B1]; Fg[B0 + (D + 2) % 3] = F[B2];
+			}
+			const int m = d.minsub[o];
+			const double min_dt = *d.sub_dt * m;
+			double cq[8];
+			for (int q = 0; q < 8; q++) cq[q] = min_dt * Fg[q];
+			const int po = d.period[o], pc = d.period[c];
+			const int pmax = po > pc ? po : pc, pmin = po > pc ? pc : po;
+			const bool fresh = ((sub - m) & (pmax - 1)) == 0;
+			for (int q = 0; q < 8; q++) A[q * n + c] = (fresh ? 0.0 : A[q * n + c]) + cq[q];
+			if (pmin < pmax) {
+				const bool fresh2 = ((sub - m) & (pmin - 1)) == 0;
+				for (int q = 0; q < 8; q++) A2[q * n + c] = (fresh2 ? 0.0 : A2[q * n + c]) + cq[q];
+			}
+			Em[c] = cq[D == 1 ? B0 + 2 : B0 + 1];
+			Em[n + c] = cq[B0];
+			solved++;
+			Ls = Rs;
+			cfl = cfr;
+			stage_l = stage_r;
+			code_l = code_r;
+			o = c;
+		}
+	}
+	/* faces solved (timestep()'s flux_calcs), one atomic per wave */
+	for (int off = 32; off > 0; off >>= 1) solved += __shfl_xor(solved, off, 64);
+	if ((threadIdx.x + BX * threadIdx.y) % 64 == 0 && solved)
+		atomicAdd(&d.scal[4 * SS_SLOTS + ss_slot()], (unsigned long long)solved);
+}
+
 /* the edge fields of direction D set to zero in the blocks of its zeroing
    list (no face due, the previous substep's values still there), what the
    flux pass would write there */
@@ -3756,6 +3858,18 @@ void ss_launch_flux(pamhd_handle* h, double* P, int sub, bool full, hipStream_t 
 	if (full) {
 		const dim3 grid = cell_grid(h->g);
 		ss_flux_kernel<SOLVER, 0, false><<<grid, block, 0, st>>>(h->g, d, h->bg, P, sub);
+		if constexpr (SOLVER == HLLD_ATHENA) {
+			if (!d.bounded && !d.ct) {
+				/* HLLD's per-cell part carried along marches of 16 cells */
+				const int km = 16;
+				const unsigned gx = (h->g.nx + BX - 1) / BX;
+				const unsigned g1 = gx * ((h->g.nz + BY - 1) / BY) * ((h->g.ny + km - 1) / km);
+				const unsigned g2 = gx * ((h->g.ny + BY - 1) / BY) * ((h->g.nz + km - 1) / km);
+				ss_flux_march_kernel<1><<<g1, block, 0, st>>>(h->g, d, P, sub, gx, km);
+				ss_flux_march_kernel<2><<<g2, block, 0, st>>>(h->g, d, P, sub, gx, km);
+				return;
+			}
+		}
 		ss_flux_kernel<SOLVER, 1, false><<<grid, block, 0, st>>>(h->g, d, h->bg, P, sub);
 		ss_flux_kernel<SOLVER, 2, false><<<grid, block, 0, st>>>(h->g, d, h->bg, P, sub);
 		return;
