@@ -3099,8 +3099,12 @@ __global__ __launch_bounds__(BX * BY) void ss_sync_kernel(GridDev g, SubstepDev 
 	d.S[(8 + 2 * D) * n + b] = pb;
 }
 
-/* update_vol_B(0) of all cells and the face-once state */
-__global__ __launch_bounds__(BX * BY) void ss_final_kernel(GridDev g, SubstepDev d, double* __restrict__ s)
+/* update_vol_B(0) of all cells and the face-once state; with `refit`
+   also the constant-pressure refit of the main loop's apply_boundaries
+   (refit_kernel's, with the field of the face-once state: the cell's and
+   its +d neighbour's own -d copies, which this kernel does not write) */
+__global__ __launch_bounds__(BX * BY) void ss_final_kernel(GridDev g, SubstepDev d, double* __restrict__ s,
+	int refit)
 {
 	const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
 	if (i >= g.nx || j >= g.ny) return;
@@ -3124,6 +3128,13 @@ __global__ __launch_bounds__(BX * BY) void ss_final_kernel(GridDev g, SubstepDev
 		return;
 	}
 	if (rho > 0) refit_energy(rho, m, En, vb, vb_new, g.ph);
+	if (refit) {
+		const int nb[3] = {cidx(g, wrap_inc(i, g.nx), j, k), cidx(g, i, wrap_inc(j, g.ny), k),
+			cidx(g, i, j, wrap_inc(k, g.nz))};
+		double vf[3];
+		for (int q = 0; q < 3; q++) vf[q] = 0.5 * (d.S[(8 + 2 * q) * n + c] + d.S[(8 + 2 * q) * n + nb[q]]);
+		refit_energy(rho, m, En, vf, vf, g.ph);
+	}
 	s[c] = rho;
 	for (int q = 0; q < 3; q++) s[(1 + q) * n + c] = m[q];
 	s[4 * n + c] = En;
@@ -3969,12 +3980,11 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 	ss_sync_kernel<0><<<grid, block, 0, st>>>(h->g, d);
 	ss_sync_kernel<1><<<grid, block, 0, st>>>(h->g, d);
 	ss_sync_kernel<2><<<grid, block, 0, st>>>(h->g, d);
-	ss_final_kernel<<<grid, block, 0, st>>>(h->g, d, h->state[h->cur]);
+	/* on a periodic grid the final kernel also does apply_boundaries'
+	   refit (refits == 3) */
+	ss_final_kernel<<<grid, block, 0, st>>>(h->g, d, h->state[h->cur], !d.bounded && refits == 3);
 	ss_count_kernel<<<1, SS_FOLD, 0, st>>>(d);
-	if (refits == 3) {
-		if (d.bounded) launch_boundaries(h, st);
-		else refit_kernel<<<grid, block, 0, st>>>(h->g, h->state[h->cur]);
-	}
+	if (refits == 3 && d.bounded) launch_boundaries(h, st);
 	HIP_TRY(h, hipGetLastError());
 	HIP_TRY(h, hipMemcpyAsync(d.host_status, d.status, sizeof(StepStatus), hipMemcpyDeviceToHost, st));
 	HIP_TRY(h, hipStreamSynchronize(st));
