@@ -440,8 +440,9 @@ was.  With fluxes non-NULL the step also writes its magnetic fluxes there
 (6 * n, the layout of pamhd_get_magnetic_fluxes) from the update pass, for
 which the step must keep its FP64 flux array (not mixed precision, not
 HLLD's marches).  For recomputing a checkpoint segment in a reverse pass
-without copies.  Not on solar-wind boxes or substepped grids.  (No
-counterpart in the reference.)
+without copies.  Periodic grids only (not solar-wind boxes or grids with
+boundaries); like pamhd_step_fixed it never substeps.  (No counterpart in
+the reference.)
 */
 int pamhd_step_fixed_from(pamhd_handle* h, int solver, double dt, const double* state_in,
 	double* state_out, double* fluxes, void* stream);

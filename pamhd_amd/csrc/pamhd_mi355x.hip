@@ -4489,7 +4489,10 @@ int pamhd_step_fixed_from(pamhd_handle* h, int solver, double dt, const double* 
 {
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (!state_in || !state_out) return fail(h, PAMHD_E_ARG, "null argument");
-	if (h->box || h->sub) return fail(h, PAMHD_E_UNSUPPORTED, "steps between caller states need a periodic grid without substepping");
+	/* fixed steps never substep; grids with boundaries keep their state in
+	   the substepping layout */
+	if (h->box || (h->sub && h->sub->bounded))
+		return fail(h, PAMHD_E_UNSUPPORTED, "steps between caller states need a periodic grid");
 	if (!(dt >= 0)) return fail(h, PAMHD_E_ARG, "dt must be >= 0");
 	const size_t bytes = sizeof(double) * 8 * (size_t)h->g.n;
 	const char* a = reinterpret_cast<const char*>(state_in);
