@@ -3102,7 +3102,10 @@ __global__ __launch_bounds__(BX * BY) void ss_sync_kernel(GridDev g, SubstepDev 
 /* update_vol_B(0) of all cells and the face-once state; with `refit`
    also the constant-pressure refit of the main loop's apply_boundaries
    (refit_kernel's, with the field of the face-once state: the cell's and
-   its +d neighbour's own -d copies, which this kernel does not write) */
+   its +d neighbour's own -d copies, which this kernel does not write).
+   The working layout is left as ss_expand_kernel would make it from the
+   state written here (energy, face-average field, +d copies from the
+   neighbours' -d copies), so the next step may skip that expansion */
 __global__ __launch_bounds__(BX * BY) void ss_final_kernel(GridDev g, SubstepDev d, double* __restrict__ s,
 	int refit)
 {
@@ -3128,17 +3131,24 @@ __global__ __launch_bounds__(BX * BY) void ss_final_kernel(GridDev g, SubstepDev
 		return;
 	}
 	if (rho > 0) refit_energy(rho, m, En, vb, vb_new, g.ph);
-	if (refit) {
-		const int nb[3] = {cidx(g, wrap_inc(i, g.nx), j, k), cidx(g, i, wrap_inc(j, g.ny), k),
-			cidx(g, i, j, wrap_inc(k, g.nz))};
-		double vf[3];
-		for (int q = 0; q < 3; q++) vf[q] = 0.5 * (d.S[(8 + 2 * q) * n + c] + d.S[(8 + 2 * q) * n + nb[q]]);
-		refit_energy(rho, m, En, vf, vf, g.ph);
+	const int nb[3] = {cidx(g, wrap_inc(i, g.nx), j, k), cidx(g, i, wrap_inc(j, g.ny), k),
+		cidx(g, i, j, wrap_inc(k, g.nz))};
+	double lo[3], up[3], vf[3];
+	for (int q = 0; q < 3; q++) {
+		lo[q] = d.S[(8 + 2 * q) * n + c];
+		up[q] = d.S[(8 + 2 * q) * n + nb[q]];
+		vf[q] = 0.5 * (lo[q] + up[q]);
 	}
+	if (refit) refit_energy(rho, m, En, vf, vf, g.ph);
 	s[c] = rho;
 	for (int q = 0; q < 3; q++) s[(1 + q) * n + c] = m[q];
 	s[4 * n + c] = En;
-	for (int q = 0; q < 3; q++) s[(5 + q) * n + c] = d.S[(8 + 2 * q) * n + c];
+	for (int q = 0; q < 3; q++) s[(5 + q) * n + c] = lo[q];
+	d.S[4 * n + c] = En;
+	for (int q = 0; q < 3; q++) {
+		d.S[(5 + q) * n + c] = vf[q];
+		d.S[(9 + 2 * q) * n + c] = up[q];
+	}
 }
 
 __global__ __launch_bounds__(BX * BY) void ss_fill_kernel(double* __restrict__ a, long long n, double v)
@@ -3210,10 +3220,25 @@ struct pamhd_handle {
 	int cur = 0;    /* state buffer holding the current state */
 	int last = 0;   /* status entry of the last flux pass */
 	unsigned long long flux_calcs = 0;   /* faces solved by the last step */
+	/* the substepped step leaves its working layout as the expansion of the
+	   state it returns, and the next one skips the expansion when nothing
+	   else reached the handle in between: `calls` counts the entry points
+	   that take the handle, ss_at is its value after the last substepped
+	   step (ss_valid), and a state pointer handed out (pamhd_state_device_ptr)
+	   turns the skip off for good */
+	unsigned long long calls = 0, ss_at = 0;
+	bool ss_valid = false;
+	bool state_escaped = false;
 	std::string err;
 };
 
 namespace {
+
+/* an entry point that takes the handle was called */
+inline void touch(pamhd_handle* h)
+{
+	if (h) h->calls++;
+}
 
 enum { K_DT = 0, K_FLUX = 1, K_UPDATE = 2, K_REFIT = 3 };
 const char* const kernel_names[4] = {"dt", "flux_pass", "update", "refit"};
@@ -3952,7 +3977,12 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 	memcpy(&sub_dt, d.host + 1, sizeof(double));
 	if (maxp < 1) return fail(h, PAMHD_E_DEVICE, "substep periods not computed");
 
-	if (!d.bounded) ss_expand_kernel<<<grid, block, 0, st>>>(h->g, h->state[h->cur], d);
+	/* the working layout is still the expansion of the state when the last
+	   substepped step made it and nothing else reached the handle since
+	   (this call and, in pamhd_run's loop, none) */
+	const bool live = !d.bounded && h->ss_valid && !h->state_escaped && h->calls - h->ss_at <= 1;
+	h->ss_valid = false;
+	if (!d.bounded && !live) ss_expand_kernel<<<grid, block, 0, st>>>(h->g, h->state[h->cur], d);
 	double total = 0;
 	for (int sub = 1; sub <= maxp; sub++) {
 		total += sub_dt;
@@ -3983,6 +4013,10 @@ int substep_step(pamhd_handle* h, int solver, double max_dt, double cfl, int ref
 	/* on a periodic grid the final kernel also does apply_boundaries'
 	   refit (refits == 3) */
 	ss_final_kernel<<<grid, block, 0, st>>>(h->g, d, h->state[h->cur], !d.bounded && refits == 3);
+	if (!d.bounded) {
+		h->ss_valid = true;
+		h->ss_at = h->calls;
+	}
 	ss_count_kernel<<<1, SS_FOLD, 0, st>>>(d);
 	if (refits == 3 && d.bounded) launch_boundaries(h, st);
 	HIP_TRY(h, hipGetLastError());
@@ -4190,6 +4224,7 @@ int pamhd_create_bounded(const pamhd_grid_desc* grid, const pamhd_physics* physi
 int pamhd_set_boundary_values(pamhd_handle* h, const unsigned char* mask, const double* number_density,
 	const double* velocity, const double* pressure, const double* face_field)
 {
+	touch(h);
 	if (!h || !mask || !number_density || !velocity || !pressure || !face_field) {
 		return fail(h, PAMHD_E_ARG, "null argument");
 	}
@@ -4206,6 +4241,7 @@ int pamhd_set_boundary_values(pamhd_handle* h, const unsigned char* mask, const 
 
 int pamhd_box_cell_types(pamhd_handle* h, signed char* cell_type, unsigned char* rule)
 {
+	touch(h);
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (!h->box) return fail(h, PAMHD_E_ARG, "not a solar-wind box");
 	HIP_TRY(h, boxset_cell_types(h->box, 0, cell_type, rule));
@@ -4214,6 +4250,7 @@ int pamhd_box_cell_types(pamhd_handle* h, signed char* cell_type, unsigned char*
 
 int pamhd_box_background(pamhd_handle* h, double* bg)
 {
+	touch(h);
 	if (!h || !bg) return fail(h, PAMHD_E_ARG, "null argument");
 	if (!h->box) return fail(h, PAMHD_E_ARG, "not a solar-wind box");
 	HIP_TRY(h, boxset_background(h->box, 0, bg));
@@ -4222,6 +4259,7 @@ int pamhd_box_background(pamhd_handle* h, double* bg)
 
 int pamhd_destroy(pamhd_handle* h)
 {
+	touch(h);
 	if (!h) return PAMHD_OK;
 	collect_times(h);
 	(void)hipFree(h->state[0]);
@@ -4249,6 +4287,7 @@ int pamhd_destroy(pamhd_handle* h)
 
 int pamhd_upload_state(pamhd_handle* h, const double* state)
 {
+	touch(h);
 	if (!h || !state) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h->box) {
 		HIP_TRY(h, boxset_upload(h->box, 0, state));
@@ -4264,6 +4303,7 @@ int pamhd_upload_state(pamhd_handle* h, const double* state)
 
 int pamhd_download_state(pamhd_handle* h, double* state)
 {
+	touch(h);
 	if (!h || !state) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h->box) {
 		HIP_TRY(h, boxset_download(h->box, 0, state));
@@ -4280,6 +4320,7 @@ int pamhd_download_state(pamhd_handle* h, double* state)
 
 int pamhd_state_components(pamhd_handle* h, int* components)
 {
+	touch(h);
 	if (!h || !components) return fail(h, PAMHD_E_ARG, "null argument");
 	*components = (h->box || (h->sub && h->sub->bounded)) ? 14 : 8;
 	return PAMHD_OK;
@@ -4287,6 +4328,7 @@ int pamhd_state_components(pamhd_handle* h, int* components)
 
 int pamhd_grid_kind(pamhd_handle* h, int* kind)
 {
+	touch(h);
 	if (!h || !kind) return fail(h, PAMHD_E_ARG, "null argument");
 	*kind = h->box ? PAMHD_GRID_SOLAR_WIND_BOX : ((h->sub && h->sub->bounded) ? PAMHD_GRID_BOUNDED : PAMHD_GRID_PERIODIC);
 	return PAMHD_OK;
@@ -4294,6 +4336,8 @@ int pamhd_grid_kind(pamhd_handle* h, int* kind)
 
 int pamhd_state_device_ptr(pamhd_handle* h, double** ptr)
 {
+	touch(h);
+	if (h) h->state_escaped = true;
 	if (!h || !ptr) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "solar-wind box state is not a flat array");
 	if (h->sub && h->sub->bounded) {
@@ -4306,6 +4350,7 @@ int pamhd_state_device_ptr(pamhd_handle* h, double** ptr)
 
 int pamhd_set_background_faces(pamhd_handle* h, const double* bg)
 {
+	touch(h);
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "solar-wind boxes compute their background field");
 	if (!bg) {
@@ -4321,17 +4366,20 @@ int pamhd_set_background_faces(pamhd_handle* h, const double* bg)
 int pamhd_timestep(pamhd_handle* h, int solver, double max_time_step, double time_step_factor,
 	double* dt_taken, void* stream)
 {
+	touch(h);
 	return do_step(h, solver, max_time_step, time_step_factor, 2, dt_taken, stream);
 }
 
 int pamhd_step(pamhd_handle* h, int solver, double max_time_step, double time_step_factor,
 	double* dt_taken, void* stream)
 {
+	touch(h);
 	return do_step(h, solver, max_time_step, time_step_factor, 3, dt_taken, stream);
 }
 
 int pamhd_apply_boundaries(pamhd_handle* h, double /*simulation_time*/, void* stream)
 {
+	touch(h);
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (h->box) {
 		if (boxset_launch_boundaries(h->box, (hipStream_t)stream)) return fail(h, PAMHD_E_DEVICE, "boundary launch");
@@ -4358,6 +4406,7 @@ int pamhd_apply_boundaries(pamhd_handle* h, double /*simulation_time*/, void* st
 
 int pamhd_check(pamhd_handle* h)
 {
+	touch(h);
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (h->box) {
 		std::string err;
@@ -4375,6 +4424,7 @@ int pamhd_check(pamhd_handle* h)
 
 int pamhd_max_signal_speeds(pamhd_handle* h, double smax_out[3])
 {
+	touch(h);
 	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h || !smax_out) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "per-box speeds: use the ensemble interface");
@@ -4393,6 +4443,7 @@ int pamhd_max_signal_speeds(pamhd_handle* h, double smax_out[3])
 
 int pamhd_div_face2volume(pamhd_handle* h, double* avg)
 {
+	touch(h);
 	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h || !avg) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h->box) {
@@ -4426,6 +4477,7 @@ int pamhd_div_face2volume(pamhd_handle* h, double* avg)
 int pamhd_clean_divergence(pamhd_handle* h, double tol, int max_cycles, double* div_before,
 	double* rel_residual, int* cycles, void* stream_)
 {
+	touch(h);
 	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "divergence cleaning needs a periodic grid");
@@ -4478,6 +4530,7 @@ int pamhd_clean_divergence(pamhd_handle* h, double tol, int max_cycles, double* 
 
 int pamhd_step_fixed(pamhd_handle* h, int solver, double dt, void* stream)
 {
+	touch(h);
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "fixed steps need a periodic grid");
 	if (!(dt >= 0)) return fail(h, PAMHD_E_ARG, "dt must be >= 0");
@@ -4487,6 +4540,7 @@ int pamhd_step_fixed(pamhd_handle* h, int solver, double dt, void* stream)
 int pamhd_step_fixed_from(pamhd_handle* h, int solver, double dt, const double* state_in,
 	double* state_out, double* fluxes, void* stream)
 {
+	touch(h);
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (!state_in || !state_out) return fail(h, PAMHD_E_ARG, "null argument");
 	/* fixed steps never substep; grids with boundaries keep their state in
@@ -4519,6 +4573,7 @@ int pamhd_step_fixed_from(pamhd_handle* h, int solver, double dt, const double* 
 
 int pamhd_set_state_device(pamhd_handle* h, const double* state, void* stream)
 {
+	touch(h);
 	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h || !state) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "solar-wind box state is not a flat array");
@@ -4529,6 +4584,7 @@ int pamhd_set_state_device(pamhd_handle* h, const double* state, void* stream)
 
 int pamhd_get_state_device(pamhd_handle* h, double* state, void* stream)
 {
+	touch(h);
 	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h || !state) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "solar-wind box state is not a flat array");
@@ -4540,6 +4596,7 @@ int pamhd_get_state_device(pamhd_handle* h, double* state, void* stream)
 int pamhd_adjoint_step(pamhd_handle* h, int solver, double dt, const double* state, double* lambda,
 	void* stream)
 {
+	touch(h);
 	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h || !state || !lambda) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "the adjoint needs a periodic grid");
@@ -4556,6 +4613,7 @@ int pamhd_adjoint_step(pamhd_handle* h, int solver, double dt, const double* sta
 
 int pamhd_get_magnetic_fluxes(pamhd_handle* h, double* fluxes, void* stream)
 {
+	touch(h);
 	if (!h || !fluxes) return fail(h, PAMHD_E_ARG, "null argument");
 	if (!h->flux_step) {
 		return fail(h, PAMHD_E_UNSUPPORTED, "the last step kept no FP64 flux array (HLLD's marches, "
@@ -4570,6 +4628,7 @@ int pamhd_get_magnetic_fluxes(pamhd_handle* h, double* fluxes, void* stream)
 int pamhd_adjoint_steps(pamhd_handle* h, int solver, int nsteps, const double* dts,
 	const double* const* states, const double* const* fluxes, double* lambda, void* stream)
 {
+	touch(h);
 	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h || !dts || !states || !lambda) return fail(h, PAMHD_E_ARG, "null argument");
 	if (nsteps < 0) return fail(h, PAMHD_E_ARG, "nsteps must be >= 0");
@@ -4592,6 +4651,7 @@ int pamhd_adjoint_steps(pamhd_handle* h, int solver, int nsteps, const double* d
 int pamhd_adjoint_step_fluxes(pamhd_handle* h, int solver, double dt, const double* state,
 	const double* fluxes, double* lambda, void* stream)
 {
+	touch(h);
 	if (!fluxes) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h || !state || !lambda) return fail(h, PAMHD_E_ARG, "null argument");
@@ -4609,6 +4669,7 @@ int pamhd_adjoint_step_fluxes(pamhd_handle* h, int solver, double dt, const doub
 
 int pamhd_set_adjoint_jacobian(pamhd_handle* h, int reverse)
 {
+	touch(h);
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	h->adj_reverse = reverse != 0;
 	return PAMHD_OK;
@@ -4616,6 +4677,7 @@ int pamhd_set_adjoint_jacobian(pamhd_handle* h, int reverse)
 
 int pamhd_set_poisson_smoother(pamhd_handle* h, int fused)
 {
+	touch(h);
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	h->mg_fused = fused != 0;
 	return PAMHD_OK;
@@ -4623,6 +4685,7 @@ int pamhd_set_poisson_smoother(pamhd_handle* h, int fused)
 
 int pamhd_set_substepping(pamhd_handle* h, int substep_min, int substep_max)
 {
+	touch(h);
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "substepping needs a periodic grid");
 	if (substep_max < 0) return fail(h, PAMHD_E_ARG, "substep_max must be >= 0");
@@ -4644,6 +4707,7 @@ int pamhd_set_substepping(pamhd_handle* h, int substep_min, int substep_max)
 
 int pamhd_set_substep_walk(pamhd_handle* h, int workgroups)
 {
+	touch(h);
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (workgroups < 0) return fail(h, PAMHD_E_ARG, "workgroups must be >= 0");
 	h->ss_walk = workgroups;
@@ -4653,6 +4717,7 @@ int pamhd_set_substep_walk(pamhd_handle* h, int workgroups)
 
 int pamhd_set_substepping_cells(pamhd_handle* h, const int* substep_min, const int* substep_max)
 {
+	touch(h);
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (!substep_min || !substep_max) return fail(h, PAMHD_E_ARG, "null argument");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "substepping needs a periodic grid");
@@ -4686,6 +4751,7 @@ int pamhd_set_substepping_cells(pamhd_handle* h, const int* substep_min, const i
 
 int pamhd_last_flux_calcs(pamhd_handle* h, unsigned long long* flux_calcs)
 {
+	touch(h);
 	if (!h || !flux_calcs) return fail(h, PAMHD_E_ARG, "null argument");
 	*flux_calcs = h->flux_calcs;
 	return PAMHD_OK;
@@ -4693,6 +4759,7 @@ int pamhd_last_flux_calcs(pamhd_handle* h, unsigned long long* flux_calcs)
 
 int pamhd_last_max_substep(pamhd_handle* h, int* max_substep)
 {
+	touch(h);
 	if (!h || !max_substep) return fail(h, PAMHD_E_ARG, "null argument");
 	*max_substep = h->sub ? h->sub->last_max : 1;
 	return PAMHD_OK;
@@ -4700,6 +4767,7 @@ int pamhd_last_max_substep(pamhd_handle* h, int* max_substep)
 
 int pamhd_last_substep_periods(pamhd_handle* h, int* periods)
 {
+	touch(h);
 	if (!h || !periods) return fail(h, PAMHD_E_ARG, "null argument");
 	if (!h->sub || !h->sub->period) {
 		for (long long c = 0; c < h->g.n; c++) periods[c] = 1;
@@ -4786,6 +4854,7 @@ int pamhd_face_flux_batch(int solver, int precision, size_t n, const double* L, 
 
 int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* stream_)
 {
+	touch(h);
 	if (h && h->sub && h->sub->bounded) return fail(h, PAMHD_E_UNSUPPORTED, "not available on a grid with boundaries");
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (h->box) return fail(h, PAMHD_E_UNSUPPORTED, "solar-wind box: use the ensemble autotuner");
@@ -4971,6 +5040,7 @@ int pamhd_autotune(pamhd_handle* h, int solver, char* buf, size_t buflen, void* 
 
 int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_remap)
 {
+	touch(h);
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (update_kz < 0) return fail(h, PAMHD_E_ARG, "update_kz must be >= 0");
 	h->tune.flux_remap = flux_remap & 3;
@@ -4985,6 +5055,7 @@ int pamhd_set_launch(pamhd_handle* h, int flux_remap, int update_kz, int update_
 
 int pamhd_set_fused_step(pamhd_handle* h, int fused, int planes_per_chunk)
 {
+	touch(h);
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (fused < -1 || fused > 2) return fail(h, PAMHD_E_ARG, "fused must be -1, 0, 1 or 2");
 	const int most = fused == 2 ? ACC_KMAX : FKC_MAX;
@@ -5000,6 +5071,7 @@ int pamhd_set_fused_step(pamhd_handle* h, int fused, int planes_per_chunk)
 
 int pamhd_set_slabs(pamhd_handle* h, int slabs)
 {
+	touch(h);
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	if (slabs < 1) return fail(h, PAMHD_E_ARG, "slabs must be >= 1");
 	h->tune.slabs = slabs;
@@ -5008,6 +5080,7 @@ int pamhd_set_slabs(pamhd_handle* h, int slabs)
 
 int pamhd_enable_timing(pamhd_handle* h, int enable)
 {
+	touch(h);
 	if (!h) return fail(nullptr, PAMHD_E_ARG, "null handle");
 	collect_times(h);
 	h->timing = enable != 0;
@@ -5020,6 +5093,7 @@ int pamhd_enable_timing(pamhd_handle* h, int enable)
 
 int pamhd_kernel_time(pamhd_handle* h, const char* kernel, double* total_ms, long* launches)
 {
+	touch(h);
 	if (!h || !kernel || !total_ms || !launches) return fail(h, PAMHD_E_ARG, "null argument");
 	collect_times(h);
 	for (int k = 0; k < 4; k++) {
@@ -5034,6 +5108,7 @@ int pamhd_kernel_time(pamhd_handle* h, const char* kernel, double* total_ms, lon
 
 int pamhd_last_error(pamhd_handle* h, char* buf, size_t n)
 {
+	touch(h);
 	if (!buf || n == 0) return PAMHD_E_ARG;
 	std::string m;
 	if (h) {

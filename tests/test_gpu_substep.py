@@ -37,6 +37,60 @@ def test_substepped_run_bit_exact(gpu, name, sid, smax, cells):
     np.testing.assert_array_equal(out, ref)
 
 
+def test_state_changed_between_substepped_runs(gpu):
+    """Consecutive substepped steps keep the working layout (the expansion of
+    the state) and skip re-expanding it; any other call on the handle in
+    between makes the next step expand the state again.  Here the state is
+    changed through the ABI between two runs (uploaded with a scaled
+    density, and read back in the middle of a run's worth of single steps),
+    and the result must equal the oracle doing the same, bit for bit."""
+    from pamhd_amd.solver import Grid
+    cells = (16, 12, 10)
+    state, length, gamma, mu0 = configs.hot_blob(cells, contrast=30.0, radius=0.25, seed=5)
+    g_o = O.make_grid(cells, length, gamma, mu0)
+    ref = state.copy()
+    st, steps_o, t1_o, _ = O.run_periodic_substep(g_o, O.HLLD, 0.3, 0, 3, 0.0, 1e9, ref, max_steps=3)
+    assert st == 0 and steps_o == 3
+    ref[0] *= 1.01
+    ref[4] *= 1.01
+    st, steps_o, t2_o, _ = O.run_periodic_substep(g_o, O.HLLD, 0.3, 0, 3, 0.0, 1e9, ref, max_steps=3)
+    assert st == 0 and steps_o == 3
+    with Grid(cells, length, gamma, mu0) as g:
+        g.upload(state)
+        g.set_substepping(0, 3)
+        steps_g, t1_g = g.run("hlld-athena", 0.3, 0.0, 1e9, max_steps=3)
+        mid = g.download()
+        mid[0] *= 1.01
+        mid[4] *= 1.01
+        g.upload(mid)
+        steps_g2, t2_g = g.run("hlld-athena", 0.3, 0.0, 1e9, max_steps=3)
+        out = g.download()
+    assert steps_g == 3 and steps_g2 == 3 and t1_g == t1_o and t2_g == t2_o
+    np.testing.assert_array_equal(out, ref)
+
+
+def test_reads_between_substepped_steps(gpu):
+    """Single substepped steps with a state read-back between some of them
+    (each read makes the next step expand the state again) give the same
+    state as the uninterrupted run, bit for bit."""
+    from pamhd_amd.solver import Grid
+    cells = (16, 12, 10)
+    state, length, gamma, mu0 = configs.hot_blob(cells, contrast=30.0, radius=0.25, seed=7)
+    outs = []
+    for read_between in (False, True):
+        with Grid(cells, length, gamma, mu0) as g:
+            g.upload(state)
+            g.set_substepping(0, 3)
+            g.apply_boundaries(0.0)
+            g.timestep("hlld-athena", 0.0, 0.3)
+            for q in range(5):
+                g.step("hlld-athena", 1e300, 0.3)
+                if read_between and q % 2 == 0:
+                    g.download()
+            outs.append(g.download())
+    np.testing.assert_array_equal(outs[0], outs[1])
+
+
 @pytest.mark.parametrize("walk", [0, 7])
 def test_substepped_run_many_blocks_per_reduction_slot(gpu, walk):
     """The step's reductions (smallest time step, largest level and period,
