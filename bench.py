@@ -279,7 +279,8 @@ def ensemble_run(nbox, steps, solver="hll-athena", cfl=0.3, seed=2024):
     launch per stage for all boxes,
     steps replayed from a captured HIP graph, autotuned tile shape) against a
     loop over the boxes with one handle each (plain launches), on the same
-    stream; results of the two are compared box by box."""
+    stream, and the same steps in one persistent launch; results are
+    compared box by box."""
     import numpy as np
     import torch
     from pamhd_amd import configs
@@ -322,7 +323,22 @@ def ensemble_run(nbox, steps, solver="hll-athena", cfl=0.3, seed=2024):
     same = all(np.array_equal(grids[q].download(), probe[q]) for q in probe)
     for g in grids:
         g.close()
+    # the same steps in one persistent launch (stages separated by grid-wide
+    # barriers), from the same start and tile shape
+    tiles = {"64x4": 0, "32x8": 1, "16x16": 2}
+    with Ensemble(boxes) as e2:
+        e2.set_tile(tiles.get(tune.split("; chosen ")[-1].strip(), 1))
+        e2.timestep(solver, 0.0, cfl, stream=stream)
+        e2.run(solver, cfl, 2, use_graph=False, stream=stream)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        e2.run(solver, cfl, steps, persistent=True, stream=stream)
+        torch.cuda.synchronize()
+        pers_s = time.perf_counter() - t0
+        pers_same = all(np.array_equal(e2.download(q), probe[q]) for q in probe)
     return {"boxes": nbox, "cells": int(sum(b.n for b in boxes)), "normal_cells": normal,
+            "persistent_s": round(pers_s, 4), "persistent_cell_updates_per_s": normal * steps / pers_s,
+            "persistent_identical": bool(pers_same),
             "steps": steps, "solver": solver, "cfl": cfl,
             "ensemble_s": round(ens_s, 4), "per_box_loop_s": round(loop_s, 4),
             "ensemble_cell_updates_per_s": normal * steps / ens_s,
@@ -685,6 +701,8 @@ def main():
         log("  ensemble %.3e cell-updates/s, per-box loop %.3e, x%.2f, identical %s"
             % (ens["ensemble_cell_updates_per_s"], ens["per_box_loop_cell_updates_per_s"],
                ens["speedup_vs_per_box_loop"], ens["results_identical"]))
+        log("  persistent launch %.3e cell-updates/s, identical %s"
+            % (ens["persistent_cell_updates_per_s"], ens["persistent_identical"]))
         imf = ensemble_imf_run(args.ensemble_boxes, args.ensemble_imf_steps, ens["normal_cells"])
         log("  with the IMF: %d steps completed (%s)" % (imf["steps_completed"], imf["stopped_by"]))
         ens["with_imf"] = imf

@@ -541,7 +541,11 @@ program.)
   create       as pamhd_create_solar_wind_box for each box
   timestep     timestep() of every box (the final init uses max_dt = 0)
   run          `steps` main-loop steps of every box (no time limit); with
-               use_graph the steps are replayed from a captured HIP graph
+               use_graph = 1 the steps are replayed from a captured HIP
+               graph, with use_graph = 2 they all run in one persistent
+               cooperative launch (stages separated by grid-wide barriers
+               whose waits are bounded: a timeout returns PAMHD_E_DEVICE;
+               synchronous); every mode gives the same results
   check        synchronises; per-box time step of the last step and the
                first physics error
   autotune     picks the fastest tile shape of the cell kernels (64x4, 32x8,

@@ -3,7 +3,8 @@ Ensemble of solar-wind boxes behind the C ABI: every box of the set is
 advanced by the same launches (one per stage of the step), and a run of
 main-loop steps replays a HIP graph holding two consecutive steps (the
 lagged-step status alternates between two buffers, so two steps bring the
-launch arguments back to where they started).
+launch arguments back to where they started), or runs every step in one
+persistent cooperative launch.
 */
 #include <hip/hip_runtime.h>
 
@@ -94,6 +95,11 @@ int pamhd_ensemble_run(pamhd_ensemble* e, int solver, double time_step_factor, l
 	if (steps < 0) return efail(e, PAMHD_E_ARG, "steps must be >= 0");
 	hipStream_t st = pick(e, stream);
 	int rc = PAMHD_OK;
+	if (use_graph == 2) {
+		/* one persistent launch for all the steps */
+		rc = boxset_run_persistent(e->set, solver, time_step_factor, steps, st, e->err);
+		return rc ? efail(e, rc, e->err) : PAMHD_OK;
+	}
 	if (!use_graph || steps < 4) {
 		for (long s = 0; s < steps && rc == PAMHD_OK; s++) {
 			rc = boxset_launch_step(e->set, solver, 1e300, time_step_factor, true, st, e->err);

@@ -90,6 +90,7 @@ struct BoxSet {
 	BoxStatus* status = nullptr;   /* 2 * nbox */
 	double* dt = nullptr;          /* nbox */
 	double* partial = nullptr;
+	unsigned* bar = nullptr;       /* persistent run: arrival count and error flag */
 	BoxStatus* host_status = nullptr;
 	double* host_dt = nullptr;
 	int last = 0;
@@ -98,14 +99,16 @@ struct BoxSet {
 
 namespace {
 
-/* cell (i, j, k) of box b handled by this thread; false past the box edge */
+/* cell (i, j, k) of box b handled by this thread in tile tl of the set (a
+   workgroup's tile: its block index, or a tile of a persistent run's
+   walk); false past the box edge */
 template <int TX, int TY>
 __device__ __forceinline__ bool my_cell(const BoxDev* __restrict__ boxes, const int* __restrict__ blockbox,
-	int& b, int& i, int& j, int& k)
+	int tl, int& b, int& i, int& j, int& k)
 {
-	b = blockbox[blockIdx.x];
+	b = blockbox[tl];
 	const BoxDev& B = boxes[b];
-	const int lb = (int)blockIdx.x - B.blk0;
+	const int lb = tl - B.blk0;
 	const int tx = lb % B.gx, ty = (lb / B.gx) % B.gy;
 	k = lb / (B.gx * B.gy);
 	i = tx * TX + (int)threadIdx.x;
@@ -147,11 +150,9 @@ __device__ __forceinline__ void box_record_error(BoxStatus* st, int code, long l
 	if (atomicCAS(&st->err, 0, code) == 0) st->err_face = face;
 }
 
-__global__ void box_dt_kernel(const BoxDev* __restrict__ boxes, int nbox, const BoxStatus* __restrict__ prev,
-	BoxStatus* __restrict__ next, double* __restrict__ dt_out, double cfl, double max_dt)
+__device__ __forceinline__ void box_dt_body(const BoxDev* __restrict__ boxes, const BoxStatus* __restrict__ prev,
+	BoxStatus* __restrict__ next, double* __restrict__ dt_out, double cfl, double max_dt, int b)
 {
-	const int b = blockIdx.x * blockDim.x + threadIdx.x;
-	if (b >= nbox) return;
 	const BoxDev& B = boxes[b];
 	const double len[3] = {B.dx, B.dy, B.dz};
 	double cfl_dt = DBL_MAX;
@@ -171,6 +172,13 @@ __global__ void box_dt_kernel(const BoxDev* __restrict__ boxes, int nbox, const 
 	next[b].err_face = -1;
 }
 
+__global__ void box_dt_kernel(const BoxDev* __restrict__ boxes, int nbox, const BoxStatus* __restrict__ prev,
+	BoxStatus* __restrict__ next, double* __restrict__ dt_out, double cfl, double max_dt)
+{
+	const int b = blockIdx.x * blockDim.x + threadIdx.x;
+	if (b < nbox) box_dt_body(boxes, prev, next, dt_out, cfl, max_dt, b);
+}
+
 /*
 Flux pass of one face direction D over all boxes: the thread's cell c owns
 the D-face on its negative side, between L = c - e_D and R = c; the flux is
@@ -178,13 +186,13 @@ computed when both cells have type >= 0 (get_fluxes,
 source/mhd/solve.hpp:262-393) with bg = Bg_B(L)(+D), else the slot is 0.
 */
 template <int SOLVER, int D, int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void box_flux_kernel(const BoxDev* __restrict__ boxes,
+__device__ __forceinline__ void box_flux_body(const BoxDev* __restrict__ boxes,
 	const int* __restrict__ blockbox, long long N, const double* __restrict__ S,
 	const signed char* __restrict__ ctype, const double* __restrict__ bg, double* __restrict__ P,
-	const double* __restrict__ dts, BoxStatus* __restrict__ status, Phys ph)
+	const double* __restrict__ dts, BoxStatus* __restrict__ status, Phys ph, int tl)
 {
 	int b, i, j, k;
-	const bool inside = my_cell<TX, TY>(boxes, blockbox, b, i, j, k);
+	const bool inside = my_cell<TX, TY>(boxes, blockbox, tl, b, i, j, k);
 	double sp = 0;
 	unsigned zero = 0;
 	if (inside) {
@@ -260,10 +268,19 @@ __global__ __launch_bounds__(TX * TY) void box_flux_kernel(const BoxDev* __restr
 			m = smax(m, ms[w]);
 			z += mz[w];
 		}
-		const int bb = blockbox[blockIdx.x];
+		const int bb = blockbox[tl];
 		if (m > 0) atomicMax(&status[bb].smax_bits[D], (unsigned long long)__double_as_longlong(m));
 		if (z) atomicAdd(&status[bb].zero[D], z);
 	}
+}
+
+template <int SOLVER, int D, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void box_flux_kernel(const BoxDev* __restrict__ boxes,
+	const int* __restrict__ blockbox, long long N, const double* __restrict__ S,
+	const signed char* __restrict__ ctype, const double* __restrict__ bg, double* __restrict__ P,
+	const double* __restrict__ dts, BoxStatus* __restrict__ status, Phys ph)
+{
+	box_flux_body<SOLVER, D, TX, TY>(boxes, blockbox, N, S, ctype, bg, P, dts, status, ph, (int)blockIdx.x);
 }
 
 /*
@@ -327,13 +344,13 @@ __device__ __forceinline__ void box_face(const BoxDev& B, long long N, const dou
 }
 
 template <int SOLVER, int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void box_flux3_kernel(const BoxDev* __restrict__ boxes,
+__device__ __forceinline__ void box_flux3_body(const BoxDev* __restrict__ boxes,
 	const int* __restrict__ blockbox, long long N, const double* __restrict__ S,
 	const signed char* __restrict__ ctype, const double* __restrict__ bg, double* __restrict__ P,
-	const double* __restrict__ dts, BoxStatus* __restrict__ status, Phys ph)
+	const double* __restrict__ dts, BoxStatus* __restrict__ status, Phys ph, int tl)
 {
 	int b, i, j, k;
-	const bool inside = my_cell<TX, TY>(boxes, blockbox, b, i, j, k);
+	const bool inside = my_cell<TX, TY>(boxes, blockbox, tl, b, i, j, k);
 	double sp[3] = {0, 0, 0};
 	unsigned zero[3] = {0, 0, 0};
 	if (inside) {
@@ -373,10 +390,19 @@ __global__ __launch_bounds__(TX * TY) void box_flux3_kernel(const BoxDev* __rest
 			m = smax(m, ms[d][w]);
 			z += mz[d][w];
 		}
-		const int bb = blockbox[blockIdx.x];
+		const int bb = blockbox[tl];
 		if (m > 0) atomicMax(&status[bb].smax_bits[d], (unsigned long long)__double_as_longlong(m));
 		if (z) atomicAdd(&status[bb].zero[d], z);
 	}
+}
+
+template <int SOLVER, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void box_flux3_kernel(const BoxDev* __restrict__ boxes,
+	const int* __restrict__ blockbox, long long N, const double* __restrict__ S,
+	const signed char* __restrict__ ctype, const double* __restrict__ bg, double* __restrict__ P,
+	const double* __restrict__ dts, BoxStatus* __restrict__ status, Phys ph)
+{
+	box_flux3_body<SOLVER, TX, TY>(boxes, blockbox, N, S, ctype, bg, P, dts, status, ph, (int)blockIdx.x);
 }
 
 /*
@@ -388,12 +414,12 @@ conserved update of rho, m, E and Vol_B (update_mhd_state, :1551-1593).
 Cells of type >= 0: refit at constant pressure (update_vol_B, :1904-1908).
 */
 template <int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void box_update_kernel(const BoxDev* __restrict__ boxes,
+__device__ __forceinline__ void box_update_body(const BoxDev* __restrict__ boxes,
 	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
-	const signed char* __restrict__ ctype, const double* __restrict__ P, Phys ph)
+	const signed char* __restrict__ ctype, const double* __restrict__ P, Phys ph, int tl)
 {
 	int b, i, j, k;
-	if (!my_cell<TX, TY>(boxes, blockbox, b, i, j, k)) return;
+	if (!my_cell<TX, TY>(boxes, blockbox, tl, b, i, j, k)) return;
 	const BoxDev& B = boxes[b];
 	const long long g = gidx(B, i, j, k);
 	const int ct = ctype[g];
@@ -466,6 +492,14 @@ __global__ __launch_bounds__(TX * TY) void box_update_kernel(const BoxDev* __res
 	refit_cell(S, N, g, ph);
 }
 
+template <int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void box_update_kernel(const BoxDev* __restrict__ boxes,
+	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
+	const signed char* __restrict__ ctype, const double* __restrict__ P, Phys ph)
+{
+	box_update_body<TX, TY>(boxes, blockbox, N, S, ctype, P, ph, (int)blockIdx.x);
+}
+
 /*
 sync_magnetic_field (source/common_functions.hpp:184-269) for the faces on
 each cell's negative sides: the two copies of a face between cells of type
@@ -474,12 +508,12 @@ the smaller index (the negative-side one), then from the other.  One thread
 per face writes both copies.
 */
 template <int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void box_sync_kernel(const BoxDev* __restrict__ boxes,
+__device__ __forceinline__ void box_sync_body(const BoxDev* __restrict__ boxes,
 	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
-	const signed char* __restrict__ ctype)
+	const signed char* __restrict__ ctype, int tl)
 {
 	int b, i, j, k;
-	if (!my_cell<TX, TY>(boxes, blockbox, b, i, j, k)) return;
+	if (!my_cell<TX, TY>(boxes, blockbox, tl, b, i, j, k)) return;
 	const BoxDev& B = boxes[b];
 	const long long g = gidx(B, i, j, k);
 	if (ctype[g] < 0) return;
@@ -503,14 +537,30 @@ __global__ __launch_bounds__(TX * TY) void box_sync_kernel(const BoxDev* __restr
 }
 
 template <int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void box_sync_kernel(const BoxDev* __restrict__ boxes,
+	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
+	const signed char* __restrict__ ctype)
+{
+	box_sync_body<TX, TY>(boxes, blockbox, N, S, ctype, (int)blockIdx.x);
+}
+
+template <int TX, int TY>
+__device__ __forceinline__ void box_refit_body(const BoxDev* __restrict__ boxes,
+	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
+	const signed char* __restrict__ ctype, Phys ph, int tl)
+{
+	int b, i, j, k;
+	if (!my_cell<TX, TY>(boxes, blockbox, tl, b, i, j, k)) return;
+	const long long g = gidx(boxes[b], i, j, k);
+	if (ctype[g] >= 0) refit_cell(S, N, g, ph);
+}
+
+template <int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void box_refit_kernel(const BoxDev* __restrict__ boxes,
 	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
 	const signed char* __restrict__ ctype, Phys ph)
 {
-	int b, i, j, k;
-	if (!my_cell<TX, TY>(boxes, blockbox, b, i, j, k)) return;
-	const long long g = gidx(boxes[b], i, j, k);
-	if (ctype[g] >= 0) refit_cell(S, N, g, ph);
+	box_refit_body<TX, TY>(boxes, blockbox, N, S, ctype, ph, (int)blockIdx.x);
 }
 
 /*
@@ -523,13 +573,13 @@ momentum in place of the velocity, as the reference does); inflow cells the
 solar-wind state.  Copies read only normal cells, which no thread writes.
 */
 template <int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void box_boundary_kernel(const BoxDev* __restrict__ boxes,
+__device__ __forceinline__ void box_boundary_body(const BoxDev* __restrict__ boxes,
 	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
 	const signed char* __restrict__ ctype, const unsigned char* __restrict__ rule,
-	BoxStatus* __restrict__ status, Phys ph)
+	BoxStatus* __restrict__ status, Phys ph, int tl)
 {
 	int b, i, j, k;
-	if (!my_cell<TX, TY>(boxes, blockbox, b, i, j, k)) return;
+	if (!my_cell<TX, TY>(boxes, blockbox, tl, b, i, j, k)) return;
 	const BoxDev& B = boxes[b];
 	const long long g = gidx(B, i, j, k);
 	const int r = rule[g];
@@ -577,16 +627,25 @@ __global__ __launch_bounds__(TX * TY) void box_boundary_kernel(const BoxDev* __r
 	S[4 * N + g] = total_energy64(rho, v, p, vb, ph.gm1, ph.half_inv_mu0);
 }
 
+template <int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void box_boundary_kernel(const BoxDev* __restrict__ boxes,
+	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
+	const signed char* __restrict__ ctype, const unsigned char* __restrict__ rule,
+	BoxStatus* __restrict__ status, Phys ph)
+{
+	box_boundary_body<TX, TY>(boxes, blockbox, N, S, ctype, rule, status, ph, (int)blockIdx.x);
+}
+
 /* the face each inflow cell shares with its interior neighbour gets the
    solar-wind normal field, with that neighbour's Vol_B component and energy
    refit at constant pressure (source/mhd/solar_wind_box.hpp:275-299) */
 template <int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void box_inflow_face_kernel(const BoxDev* __restrict__ boxes,
+__device__ __forceinline__ void box_inflow_face_body(const BoxDev* __restrict__ boxes,
 	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
-	const unsigned char* __restrict__ rule, BoxStatus* __restrict__ status, Phys ph)
+	const unsigned char* __restrict__ rule, BoxStatus* __restrict__ status, Phys ph, int tl)
 {
 	int b, i, j, k;
-	if (!my_cell<TX, TY>(boxes, blockbox, b, i, j, k)) return;
+	if (!my_cell<TX, TY>(boxes, blockbox, tl, b, i, j, k)) return;
 	const BoxDev& B = boxes[b];
 	const long long g = gidx(B, i, j, k);
 	if (rule[g] != R_SW) return;
@@ -611,6 +670,14 @@ __global__ __launch_bounds__(TX * TY) void box_inflow_face_kernel(const BoxDev* 
 	S[4 * N + nb] = total_energy64(rho, v, p, vb, ph.gm1, ph.half_inv_mu0);
 }
 
+template <int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void box_inflow_face_kernel(const BoxDev* __restrict__ boxes,
+	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
+	const unsigned char* __restrict__ rule, BoxStatus* __restrict__ status, Phys ph)
+{
+	box_inflow_face_body<TX, TY>(boxes, blockbox, N, S, rule, status, ph, (int)blockIdx.x);
+}
+
 /* sum of |div B| of normal cells per block, source/math/nabla.hpp:58-113 */
 template <int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void box_div_kernel(const BoxDev* __restrict__ boxes,
@@ -619,7 +686,8 @@ __global__ __launch_bounds__(TX * TY) void box_div_kernel(const BoxDev* __restri
 {
 	int b, i, j, k;
 	double v = 0;
-	if (my_cell<TX, TY>(boxes, blockbox, b, i, j, k) && b == box) {
+	const int tl = (int)blockIdx.x;
+	if (my_cell<TX, TY>(boxes, blockbox, tl, b, i, j, k) && b == box) {
 		const BoxDev& B = boxes[b];
 		const long long g = gidx(B, i, j, k);
 		if (ctype[g] > 0) {
@@ -638,6 +706,116 @@ __global__ __launch_bounds__(TX * TY) void box_div_kernel(const BoxDev* __restri
 		for (int q = 0; q < TX * TY / 64; q++) t += w[q];
 		partial[blockIdx.x] = t;
 	}
+}
+
+/*
+Persistent run: one cooperative launch advances every box by `steps`
+main-loop steps (the stages of boxset_launch_step with boundaries), each
+stage walked by the resident workgroups over the set's tiles and separated
+from the next by a grid-wide barrier.  The barrier counts arrivals in
+bar[0] with release / acquire fences at agent scope (a stage's writes
+reach the other XCDs before the next stage reads them).  Its wait is
+bounded: after RUN_SPIN_LIMIT polls a workgroup sets bar[1] and leaves, and
+every workgroup that sees bar[1] leaves too, so all waves exit and the host
+reports the error.  The per-tile work is the stage kernels' own, so the
+results are theirs bit for bit.
+*/
+constexpr unsigned RUN_SPIN_LIMIT = 1u << 24;
+
+__device__ __forceinline__ bool run_barrier(unsigned* bar, unsigned target)
+{
+	__shared__ int ok;
+	__syncthreads();
+	if (threadIdx.x == 0 && threadIdx.y == 0) {
+		__threadfence();
+		atomicAdd(&bar[0], 1u);
+		int good = 1;
+		for (unsigned spin = 0;; spin++) {
+			if (__hip_atomic_load(&bar[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) >= target) break;
+			if (__hip_atomic_load(&bar[1], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)) {
+				good = 0;
+				break;
+			}
+			if (spin >= RUN_SPIN_LIMIT) {
+				atomicExch(&bar[1], 1u);
+				good = 0;
+				break;
+			}
+			__builtin_amdgcn_s_sleep(2);
+		}
+		__threadfence();
+		ok = good;
+	}
+	__syncthreads();
+	return ok != 0;
+}
+
+struct RunArgs {
+	const BoxDev* boxes;
+	const int* blockbox;
+	long long N;
+	double* S;
+	const signed char* ctype;
+	const unsigned char* rule;
+	const double* bg;
+	double* P;
+	double* dt;
+	BoxStatus* status;
+	int nbox, nblocks;
+	Phys ph;
+	double cfl;
+};
+
+/* the kernel walks stage after stage (8 per step) with the stage picked by
+   a switch, so each stage's values live only in its own case: the
+   kernel's registers are about the largest stage's, not every stage's at
+   once */
+template <int SOLVER, int TX, int TY>
+__global__ __launch_bounds__(TX * TY) void box_run_kernel(RunArgs a, long steps, int last, unsigned* bar)
+{
+	const int tid = threadIdx.x + TX * threadIdx.y;
+	for (long q = 0; q < 8 * steps; q++) {
+		const int lst = (int)((last + q / 8) & 1);
+		BoxStatus* prev = a.status + (long long)lst * a.nbox;
+		BoxStatus* next = a.status + (long long)(1 - lst) * a.nbox;
+		switch ((int)(q % 8)) {
+		case 0:
+			for (int b = (int)blockIdx.x * TX * TY + tid; b < a.nbox; b += (int)gridDim.x * TX * TY)
+				box_dt_body(a.boxes, prev, next, a.dt, a.cfl, 1e300, b);
+			break;
+		case 1:
+			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
+				box_flux3_body<SOLVER, TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.bg, a.P, a.dt, next, a.ph, t);
+			break;
+		case 2:
+			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
+				box_update_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.P, a.ph, t);
+			break;
+		case 3:
+		case 7:
+			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
+				box_sync_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.ctype, t);
+			break;
+		case 4:
+			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
+				box_refit_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.ph, t);
+			break;
+		case 5:
+			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
+				box_boundary_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.rule, next, a.ph, t);
+			break;
+		default:
+			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
+				box_inflow_face_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.rule, next, a.ph, t);
+			break;
+		}
+		if (!run_barrier(bar, (unsigned)(q + 1) * gridDim.x)) return;
+	}
+}
+
+__global__ void run_barrier_clear_kernel(unsigned* bar)
+{
+	if (threadIdx.x < 2) bar[threadIdx.x] = 0;
 }
 
 /* ---------------------------------------------------------------------- */
@@ -861,6 +1039,7 @@ void boxset_destroy(BoxSet* s)
 	(void)hipFree(s->status);
 	(void)hipFree(s->dt);
 	(void)hipFree(s->partial);
+	(void)hipFree(s->bar);
 	(void)hipHostFree(s->host_status);
 	(void)hipHostFree(s->host_dt);
 	delete s;
@@ -939,6 +1118,7 @@ int boxset_create(BoxSet** out, int nbox, const pamhd_grid_desc* grids, const pa
 	if ((e = hipMalloc(&s->rule, N)) != hipSuccess) return bail(e, "rule");
 	if ((e = hipMalloc(&s->status, sizeof(BoxStatus) * 2 * nbox)) != hipSuccess) return bail(e, "status");
 	if ((e = hipMalloc(&s->dt, sizeof(double) * nbox)) != hipSuccess) return bail(e, "dt");
+	if ((e = hipMalloc(&s->bar, sizeof(unsigned) * 2)) != hipSuccess) return bail(e, "barrier");
 	if ((e = hipMalloc(&s->boxes, sizeof(BoxDev) * nbox)) != hipSuccess) return bail(e, "boxes");
 	if ((e = hipHostMalloc(&s->host_status, sizeof(BoxStatus) * nbox)) != hipSuccess) return bail(e, "pinned");
 	if ((e = hipHostMalloc(&s->host_dt, sizeof(double) * nbox)) != hipSuccess) return bail(e, "pinned");
@@ -1075,6 +1255,83 @@ int boxset_launch_step(BoxSet* s, int solver, double max_dt, double cfl, bool wi
 		return PAMHD_E_DEVICE;
 	}
 	return PAMHD_OK;
+}
+
+template <int SOLVER, int TX, int TY>
+int run_persistent(BoxSet* s, double cfl, long steps, hipStream_t st, std::string& err)
+{
+	const void* kern = reinterpret_cast<const void*>(&box_run_kernel<SOLVER, TX, TY>);
+	int per_cu = 0, dev = 0, ncu = 0;
+	hipError_t e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, box_run_kernel<SOLVER, TX, TY>, TX * TY, 0);
+	if (e == hipSuccess) e = hipGetDevice(&dev);
+	if (e == hipSuccess) e = hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+	if (e != hipSuccess || per_cu < 1 || ncu < 1) {
+		err = std::string("persistent run: occupancy query failed: ") + hipGetErrorString(e);
+		return PAMHD_E_DEVICE;
+	}
+	/* every workgroup resident at once (the cooperative launch checks it) */
+	const long long resident = (long long)per_cu * ncu;
+	const unsigned grid = (unsigned)(s->nblocks < resident ? s->nblocks : resident);
+	RunArgs a;
+	a.boxes = s->boxes;
+	a.blockbox = s->blockbox;
+	a.N = s->N;
+	a.S = s->S;
+	a.ctype = s->ctype;
+	a.rule = s->rule;
+	a.bg = s->bg;
+	a.P = s->P;
+	a.dt = s->dt;
+	a.status = s->status;
+	a.nbox = s->nbox;
+	a.nblocks = s->nblocks;
+	a.ph = s->ph;
+	a.cfl = cfl;
+	int last = s->last;
+	unsigned* bar = s->bar;
+	void* args[] = {&a, &steps, &last, &bar};
+	run_barrier_clear_kernel<<<1, 64, 0, st>>>(s->bar);
+	e = hipLaunchCooperativeKernel(kern, dim3(grid), dim3(TX, TY, 1), args, 0, st);
+	if (e != hipSuccess) {
+		err = std::string("persistent run: cooperative launch: ") + hipGetErrorString(e);
+		return PAMHD_E_DEVICE;
+	}
+	s->last = (int)((s->last + steps) & 1);
+	unsigned flag[2] = {0, 0};
+	if ((e = hipMemcpyAsync(flag, s->bar, sizeof(flag), hipMemcpyDeviceToHost, st)) != hipSuccess ||
+		(e = hipStreamSynchronize(st)) != hipSuccess) {
+		err = std::string("persistent run: ") + hipGetErrorString(e);
+		return PAMHD_E_DEVICE;
+	}
+	if (flag[1]) {
+		err = "persistent run: a grid barrier timed out (not every workgroup arrived); the state is incomplete";
+		return PAMHD_E_DEVICE;
+	}
+	return PAMHD_OK;
+}
+
+template <int SOLVER>
+int run_persistent_tile(BoxSet* s, double cfl, long steps, hipStream_t st, std::string& err)
+{
+	switch (s->tile) {
+	case 0: return run_persistent<SOLVER, 64, 4>(s, cfl, steps, st, err);
+	case 1: return run_persistent<SOLVER, 32, 8>(s, cfl, steps, st, err);
+	default: return run_persistent<SOLVER, 16, 16>(s, cfl, steps, st, err);
+	}
+}
+
+int boxset_run_persistent(BoxSet* s, int solver, double cfl, long steps, hipStream_t st, std::string& err)
+{
+	if (steps <= 0) return PAMHD_OK;
+	switch (solver) {
+	case RUSANOV: return run_persistent_tile<RUSANOV>(s, cfl, steps, st, err);
+	case HLL_ATHENA: return run_persistent_tile<HLL_ATHENA>(s, cfl, steps, st, err);
+	case HLLD_ATHENA: return run_persistent_tile<HLLD_ATHENA>(s, cfl, steps, st, err);
+	case ROE_ATHENA: return run_persistent_tile<ROE_ATHENA>(s, cfl, steps, st, err);
+	default:
+		err = "invalid solver " + std::to_string(solver);
+		return PAMHD_E_SOLVER;
+	}
 }
 
 int boxset_launch_boundaries(BoxSet* s, hipStream_t st)

@@ -43,6 +43,15 @@ No synchronisation.
 int boxset_launch_step(BoxSet* s, int solver, double max_dt, double cfl, bool with_boundaries,
 	hipStream_t st, std::string& err);
 
+/*
+`steps` main-loop steps of every box (boxset_launch_step with boundaries,
+max_dt unbounded) in one persistent cooperative launch; the stages are
+separated by grid-wide barriers whose waits are bounded (a timeout returns
+PAMHD_E_DEVICE).  Synchronous.
+*/
+int boxset_run_persistent(BoxSet* s, int solver, double cfl, long steps, hipStream_t st,
+	std::string& err);
+
 /* apply_boundaries_sw_box + sync_magnetic_field only */
 int boxset_launch_boundaries(BoxSet* s, hipStream_t st);
 

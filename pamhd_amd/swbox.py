@@ -218,9 +218,13 @@ class Ensemble:
         self._check(self._L.pamhd_ensemble_timestep(self._e, solver_id(solver), max_time_step,
                                                     time_step_factor, stream))
 
-    def run(self, solver, time_step_factor, steps, use_graph=True, stream=None):
+    def run(self, solver, time_step_factor, steps, use_graph=True, stream=None, persistent=False):
+        """`steps` main-loop steps of every box: one launch per stage, the
+        steps replayed from a HIP graph (use_graph), or all of them in one
+        persistent launch (persistent)."""
+        mode = 2 if persistent else (1 if use_graph else 0)
         self._check(self._L.pamhd_ensemble_run(self._e, solver_id(solver), time_step_factor,
-                                               steps, 1 if use_graph else 0, stream))
+                                               steps, mode, stream))
 
     def check(self, stream=None):
         dts = np.empty(len(self.boxes))

@@ -130,12 +130,13 @@ def _ensemble_boxes():
     return [_test_box(cells=c, seed=10 + q) for q, c in enumerate(sizes)]
 
 
-@pytest.mark.parametrize("use_graph,tile", [(False, 0), (True, 0), (True, 1), (True, 2)])
+@pytest.mark.parametrize("use_graph,tile", [(False, 0), (True, 0), (True, 1), (True, 2),
+                                            ("persistent", 0), ("persistent", 1), ("persistent", 2)])
 def test_ensemble_matches_single_boxes(gpu, use_graph, tile):
     """Boxes of different sizes advanced together (each with its own time
     step) give every box's oracle run bit for bit, with and without the
-    captured graph and for every tile shape; 7 steps exercise the eager step
-    before the graph replays."""
+    captured graph, in one persistent launch, and for every tile shape; 7
+    steps exercise the eager step before the graph replays."""
     from pamhd_amd.swbox import Ensemble
     boxes = _ensemble_boxes()
     refs = []
@@ -148,7 +149,10 @@ def test_ensemble_matches_single_boxes(gpu, use_graph, tile):
     with Ensemble(boxes) as e:
         e.set_tile(tile)
         e.timestep("hll-athena", 0.0, 0.3)
-        e.run("hll-athena", 0.3, 7, use_graph=use_graph)
+        if use_graph == "persistent":
+            e.run("hll-athena", 0.3, 7, persistent=True)
+        else:
+            e.run("hll-athena", 0.3, 7, use_graph=use_graph)
         dts = e.check()
         outs = [e.download(q) for q in range(len(boxes))]
     assert np.all(dts > 0)
