@@ -559,6 +559,13 @@ int pamhd_ensemble_timestep(pamhd_ensemble* e, int solver, double max_time_step,
 int pamhd_ensemble_run(pamhd_ensemble* e, int solver, double time_step_factor, long steps,
 	int use_graph, void* stream);
 int pamhd_ensemble_check(pamhd_ensemble* e, double* dt_per_box, void* stream);
+/* How long a persistent run's barrier waits (use_graph = 2): polls before
+   a workgroup gives up, sets the run's error flag and leaves (0: the
+   default, 2^24 polls, about a second); every workgroup that sees the flag
+   leaves too and the run returns PAMHD_E_DEVICE.  For callers that bound
+   their wait, and for testing that the run gives up.  (No counterpart in
+   the reference.) */
+int pamhd_ensemble_set_barrier_polls(pamhd_ensemble* e, long polls);
 int pamhd_ensemble_download(pamhd_ensemble* e, int box, double* state);
 int pamhd_ensemble_cells(pamhd_ensemble* e, int box, long long* cells);
 int pamhd_ensemble_set_tile(pamhd_ensemble* e, int tile);

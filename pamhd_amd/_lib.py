@@ -32,7 +32,7 @@ EXPORTS = (
     "pamhd_grid_kind", "pamhd_plain1d_run", "pamhd_plain1d_run_noalloc",
     "pamhd_box_cell_types", "pamhd_box_background", "pamhd_ensemble_create",
     "pamhd_ensemble_destroy", "pamhd_ensemble_timestep", "pamhd_ensemble_run",
-    "pamhd_ensemble_check", "pamhd_ensemble_download", "pamhd_ensemble_cells",
+    "pamhd_ensemble_check", "pamhd_ensemble_set_barrier_polls", "pamhd_ensemble_download", "pamhd_ensemble_cells",
     "pamhd_ensemble_set_tile", "pamhd_ensemble_autotune", "pamhd_ensemble_last_error",
     "pamhd_step_fixed", "pamhd_step_fixed_from", "pamhd_set_state_device", "pamhd_get_state_device",
     "pamhd_adjoint_step", "pamhd_get_magnetic_fluxes", "pamhd_adjoint_step_fluxes", "pamhd_adjoint_steps", "pamhd_set_slabs", "pamhd_set_fused_step", "pamhd_set_substepping", "pamhd_set_substep_walk", "pamhd_set_substepping_cells",
@@ -159,6 +159,7 @@ def load():
     L.pamhd_ensemble_run.argtypes = [E, ctypes.c_int, ctypes.c_double, ctypes.c_long,
                                      ctypes.c_int, _vp]
     L.pamhd_ensemble_check.argtypes = [E, _vp, _vp]
+    L.pamhd_ensemble_set_barrier_polls.argtypes = [E, ctypes.c_long]
     L.pamhd_ensemble_download.argtypes = [E, ctypes.c_int, _vp]
     L.pamhd_ensemble_cells.argtypes = [E, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong)]
     L.pamhd_ensemble_set_tile.argtypes = [E, ctypes.c_int]

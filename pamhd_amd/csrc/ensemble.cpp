@@ -136,6 +136,14 @@ int pamhd_ensemble_run(pamhd_ensemble* e, int solver, double time_step_factor, l
 	return PAMHD_OK;
 }
 
+int pamhd_ensemble_set_barrier_polls(pamhd_ensemble* e, long polls)
+{
+	if (!e) return PAMHD_E_ARG;
+	if (polls < 0 || polls > 0xffffffffL) return efail(e, PAMHD_E_ARG, "polls must be in 0..2^32-1");
+	boxset_set_barrier_polls(e->set, (unsigned)polls);
+	return PAMHD_OK;
+}
+
 int pamhd_ensemble_check(pamhd_ensemble* e, double* dt_per_box, void* stream)
 {
 	if (!e) return PAMHD_E_ARG;

@@ -91,6 +91,7 @@ struct BoxSet {
 	double* dt = nullptr;          /* nbox */
 	double* partial = nullptr;
 	unsigned* bar = nullptr;       /* persistent run: arrival count and error flag */
+	unsigned bar_polls = 0;        /* polls before a barrier gives up (0: RUN_SPIN_LIMIT) */
 	BoxStatus* host_status = nullptr;
 	double* host_dt = nullptr;
 	int last = 0;
@@ -722,7 +723,7 @@ results are theirs bit for bit.
 */
 constexpr unsigned RUN_SPIN_LIMIT = 1u << 24;
 
-__device__ __forceinline__ bool run_barrier(unsigned* bar, unsigned target)
+__device__ __forceinline__ bool run_barrier(unsigned* bar, unsigned target, unsigned polls)
 {
 	__shared__ int ok;
 	__syncthreads();
@@ -736,7 +737,7 @@ __device__ __forceinline__ bool run_barrier(unsigned* bar, unsigned target)
 				good = 0;
 				break;
 			}
-			if (spin >= RUN_SPIN_LIMIT) {
+			if (spin >= polls) {
 				atomicExch(&bar[1], 1u);
 				good = 0;
 				break;
@@ -771,7 +772,8 @@ struct RunArgs {
    kernel's registers are about the largest stage's, not every stage's at
    once */
 template <int SOLVER, int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void box_run_kernel(RunArgs a, long steps, int last, unsigned* bar)
+__global__ __launch_bounds__(TX * TY) void box_run_kernel(RunArgs a, long steps, int last, unsigned* bar,
+	unsigned polls)
 {
 	const int tid = threadIdx.x + TX * threadIdx.y;
 	for (long q = 0; q < 8 * steps; q++) {
@@ -809,7 +811,7 @@ __global__ __launch_bounds__(TX * TY) void box_run_kernel(RunArgs a, long steps,
 				box_inflow_face_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.rule, next, a.ph, t);
 			break;
 		}
-		if (!run_barrier(bar, (unsigned)(q + 1) * gridDim.x)) return;
+		if (!run_barrier(bar, (unsigned)(q + 1) * gridDim.x, polls)) return;
 	}
 }
 
@@ -1289,7 +1291,8 @@ int run_persistent(BoxSet* s, double cfl, long steps, hipStream_t st, std::strin
 	a.cfl = cfl;
 	int last = s->last;
 	unsigned* bar = s->bar;
-	void* args[] = {&a, &steps, &last, &bar};
+	unsigned polls = s->bar_polls ? s->bar_polls : RUN_SPIN_LIMIT;
+	void* args[] = {&a, &steps, &last, &bar, &polls};
 	run_barrier_clear_kernel<<<1, 64, 0, st>>>(s->bar);
 	e = hipLaunchCooperativeKernel(kern, dim3(grid), dim3(TX, TY, 1), args, 0, st);
 	if (e != hipSuccess) {
@@ -1332,6 +1335,11 @@ int boxset_run_persistent(BoxSet* s, int solver, double cfl, long steps, hipStre
 		err = "invalid solver " + std::to_string(solver);
 		return PAMHD_E_SOLVER;
 	}
+}
+
+void boxset_set_barrier_polls(BoxSet* s, unsigned polls)
+{
+	s->bar_polls = polls;
 }
 
 int boxset_launch_boundaries(BoxSet* s, hipStream_t st)

@@ -52,6 +52,10 @@ PAMHD_E_DEVICE).  Synchronous.
 int boxset_run_persistent(BoxSet* s, int solver, double cfl, long steps, hipStream_t st,
 	std::string& err);
 
+/* polls of a persistent run's barrier wait before it gives up (0: the
+   default, 2^24, about a second) */
+void boxset_set_barrier_polls(BoxSet* s, unsigned polls);
+
 /* apply_boundaries_sw_box + sync_magnetic_field only */
 int boxset_launch_boundaries(BoxSet* s, hipStream_t st);
 

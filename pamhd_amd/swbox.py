@@ -226,6 +226,11 @@ class Ensemble:
         self._check(self._L.pamhd_ensemble_run(self._e, solver_id(solver), time_step_factor,
                                                steps, mode, stream))
 
+    def set_barrier_polls(self, polls=0):
+        """Polls of a persistent run's barrier wait before it gives up (0:
+        the default, about a second)."""
+        self._check(self._L.pamhd_ensemble_set_barrier_polls(self._e, int(polls)))
+
     def check(self, stream=None):
         dts = np.empty(len(self.boxes))
         self._check(self._L.pamhd_ensemble_check(self._e, dts.ctypes.data, stream))

@@ -160,6 +160,34 @@ def test_ensemble_matches_single_boxes(gpu, use_graph, tile):
         np.testing.assert_array_equal(o, r, err_msg="box %d" % q)
 
 
+def test_persistent_run_gives_up_on_a_barrier_timeout(gpu):
+    """A persistent run whose barrier waits may poll only once: the first
+    workgroup to arrive gives up and sets the error flag, every workgroup
+    leaves, and the run returns an error instead of waiting; the set (and
+    the GPU) then runs again normally."""
+    from pamhd_amd import _lib
+    from pamhd_amd.swbox import Ensemble
+    boxes = _ensemble_boxes()
+    with Ensemble(boxes) as e:
+        e.timestep("hll-athena", 0.0, 0.3)
+        e.set_barrier_polls(1)
+        with pytest.raises(_lib.PamhdError, match="barrier timed out"):
+            e.run("hll-athena", 0.3, 3, persistent=True)
+    refs = []
+    for sb in boxes:
+        ob = O.Box(sb)
+        r = ob.init()
+        st, steps, t = ob.run(O.HLL, 0.3, 0.0, 1e30, r, max_steps=3)
+        assert st == 0
+        refs.append(r)
+    with Ensemble(boxes) as e:
+        e.timestep("hll-athena", 0.0, 0.3)
+        e.run("hll-athena", 0.3, 3, persistent=True)
+        outs = [e.download(q) for q in range(len(boxes))]
+    for q, (o, r) in enumerate(zip(outs, refs)):
+        np.testing.assert_array_equal(o, r, err_msg="box %d" % q)
+
+
 def test_ensemble_autotune_keeps_state(gpu):
     from pamhd_amd.swbox import Ensemble
     boxes = _ensemble_boxes()[:3]
