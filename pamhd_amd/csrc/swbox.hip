@@ -29,7 +29,9 @@ and with the main loop's boundaries:
   inflow face   interior neighbours of the inflow cells
   sync
 Every kernel works in place: no kernel reads what another thread of the
-same launch writes.
+same launch writes.  A persistent run (box_run_kernel) does the same stages
+for all its steps in one cooperative launch, with a grid-wide barrier
+between stages in place of the launch boundaries.
 */
 #include <hip/hip_runtime.h>
 
