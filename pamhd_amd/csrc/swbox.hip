@@ -5,8 +5,9 @@ boxes of different sizes advanced together.
 
 Data layout (all boxes concatenated, cell g = box offset + local index
 i + nx (j + ny k), component-major over the N cells of the set):
-  S      14 * N doubles: rho, m (3), E, Vol_B (3), Face_B copies of the
-         cell's -x, +x, -y, +y, -z, +z faces.  Each cell keeps its own copy
+  S      20 * N doubles: rho, m (3), E, Vol_B (3), Face_B copies of the
+         cell's -x, +x, -y, +y, -z, +z faces, and a second set of the six
+         face copies (scratch within a step).  Each cell keeps its own copy
          of each face, like the reference's Face_Magnetic_Field, because the
          boundary rules and sync_magnetic_field act on copies.
   P      24 * N doubles: dt * F of the d-face on each cell's negative side
@@ -16,22 +17,23 @@ i + nx (j + ny k), component-major over the N cells of the set):
          29 inflow)
   bg     9 * N: Bg_B(cell)(+d) at the cell's positive face centres
 
-A step's launches, each covering every box:
+A main-loop step's launches, each covering every box:
   dt            lagged CFL step per box (one thread per box)
   flux          the three face directions of each cell in one launch
                 (or one launch per direction); per-box max-speed reductions
   update        normal cells: constrained-transport change of their six
                 face copies, conserved update; cells of type >= 0: refit
-  sync          each face's two copies averaged (one thread per face)
-  refit         cells of type >= 0
-and with the main loop's boundaries:
-  boundary      copy / planet / inflow rules (reads only normal cells)
+  sync + refit  each cell's six face copies synced with the other copies
+                into the second set, cells of type >= 0 refit from them
+  boundary      copy / planet / inflow rules on the second set (reads only
+                normal cells)
   inflow face   interior neighbours of the inflow cells
-  sync
-Every kernel works in place: no kernel reads what another thread of the
-same launch writes.  A persistent run (box_run_kernel) does the same stages
-for all its steps in one cooperative launch, with a grid-wide barrier
-between stages in place of the launch boundaries.
+  sync          from the second set back into the first
+A step without boundaries runs update, sync (one thread per face writes
+both copies in place) and refit instead.  No kernel reads what another
+thread of the same launch writes.  A persistent run (box_run_kernel) does
+the main-loop stages for all its steps in one cooperative launch, with a
+grid-wide barrier between stages in place of the launch boundaries.
 */
 #include <hip/hip_runtime.h>
 
@@ -48,7 +50,9 @@ namespace pamhd_dev {
 namespace {
 
 enum { R_NONE = 0, R_PLANET = 28, R_SW = 29 };
-enum { Q_RHO = 0, Q_M = 1, Q_E = 4, Q_VB = 5, Q_F = 8, NQ = 14 };
+/* NQ values per cell; Q_F2: the second set of face copies that a step with
+   boundaries syncs into and back from (scratch between steps) */
+enum { Q_RHO = 0, Q_M = 1, Q_E = 4, Q_VB = 5, Q_F = 8, NQ = 14, Q_F2 = NQ, NQ_ALLOC = NQ + 6 };
 
 struct BoxDev {
 	int nx, ny, nz, n;
@@ -132,20 +136,42 @@ __device__ __forceinline__ double pressure_of(const double* __restrict__ S, long
 	return pressure64(U, ph.gm1, ph.half_inv_mu0);
 }
 
-/* update_vol_B with constant thermal pressure, source/mhd/solve.hpp:1646-1672 */
-__device__ __forceinline__ void refit_cell(double* __restrict__ S, long long N, long long g, const Phys& ph)
+/* update_vol_B with constant thermal pressure, source/mhd/solve.hpp:1646-1672,
+   from the cell's face copies f (-x, +x, -y, +y, -z, +z) */
+__device__ __forceinline__ void refit_cell_faces(double* __restrict__ S, long long N, long long g,
+	const Phys& ph, const double f[6])
 {
 	const double rho = S[g];
 	if (rho <= 0) return;
 	const double p = pressure_of(S, N, g, ph);
 	double vb[3];
 	for (int d = 0; d < 3; d++) {
-		vb[d] = 0.5 * (S[(Q_F + 2 * d) * N + g] + S[(Q_F + 2 * d + 1) * N + g]);
+		vb[d] = 0.5 * (f[2 * d] + f[2 * d + 1]);
 		S[(Q_VB + d) * N + g] = vb[d];
 	}
 	const double inv = 1 / rho;
 	const double v[3] = {S[N + g] * inv, S[2 * N + g] * inv, S[3 * N + g] * inv};
 	S[4 * N + g] = total_energy64(rho, v, p, vb, ph.gm1, ph.half_inv_mu0);
+}
+
+__device__ __forceinline__ void refit_cell(double* __restrict__ S, long long N, long long g, const Phys& ph)
+{
+	double f[6];
+	for (int q = 0; q < 6; q++) f[q] = S[(Q_F + q) * N + g];
+	refit_cell_faces(S, N, g, ph, f);
+}
+
+/* the two copies of one face after sync_magnetic_field's averaging
+   (source/common_functions.hpp:184-269): a is the copy of the cell on the
+   face's negative side, o the other */
+__device__ __forceinline__ void sync_pair(double& a, double& o)
+{
+	const double diff = a - o;
+	a -= diff / 2;
+	o += diff / 2;
+	const double diff2 = o - a;
+	o -= diff2 / 2;
+	a += diff2 / 2;
 }
 
 __device__ __forceinline__ void box_record_error(BoxStatus* st, int code, long long face)
@@ -528,15 +554,62 @@ __device__ __forceinline__ void box_sync_body(const BoxDev* __restrict__ boxes,
 		if (ctype[ga] < 0) continue;
 		double a = S[(Q_F + 2 * d + 1) * N + ga];
 		double o = S[(Q_F + 2 * d) * N + g];
-		const double diff = a - o;
-		a -= diff / 2;
-		o += diff / 2;
-		const double diff2 = o - a;
-		o -= diff2 / 2;
-		a += diff2 / 2;
+		sync_pair(a, o);
 		S[(Q_F + 2 * d + 1) * N + ga] = a;
 		S[(Q_F + 2 * d) * N + g] = o;
 	}
+}
+
+/*
+The same sync from one set of face copies into another: each cell reads
+its six copies and the other copy of each of its faces from the copies at
+component fin and writes its six synced copies at component fout (cells of
+type < 0 copy theirs through), so no thread reads what another writes and
+one pass can also refit every cell of type >= 0 with its synced copies
+(REFIT: the sync and refit stages of a step in one pass).  The two copies
+of a face come out as box_sync_body leaves them: both cells apply
+sync_pair to the same two values.
+*/
+template <int TX, int TY, bool REFIT>
+__device__ __forceinline__ void box_sync_copy_body(const BoxDev* __restrict__ boxes,
+	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
+	const signed char* __restrict__ ctype, Phys ph, int fin, int fout, int tl)
+{
+	int b, i, j, k;
+	if (!my_cell<TX, TY>(boxes, blockbox, tl, b, i, j, k)) return;
+	const BoxDev& B = boxes[b];
+	const long long g = gidx(B, i, j, k);
+	const int ct = ctype[g];
+	const int id[3] = {i, j, k};
+	const int dims[3] = {B.nx, B.ny, B.nz};
+	const long long st[3] = {1, B.nx, (long long)B.nx * B.ny};
+	double f[6];
+	for (int d = 0; d < 3; d++) {
+		double lo = S[(fin + 2 * d) * N + g], hi = S[(fin + 2 * d + 1) * N + g];
+		if (ct >= 0) {
+			if (id[d] >= 1 && ctype[g - st[d]] >= 0) {
+				double a = S[(fin + 2 * d + 1) * N + g - st[d]];
+				sync_pair(a, lo);
+			}
+			if (id[d] + 1 < dims[d] && ctype[g + st[d]] >= 0) {
+				double o = S[(fin + 2 * d) * N + g + st[d]];
+				sync_pair(hi, o);
+			}
+		}
+		f[2 * d] = lo;
+		f[2 * d + 1] = hi;
+		S[(fout + 2 * d) * N + g] = lo;
+		S[(fout + 2 * d + 1) * N + g] = hi;
+	}
+	if (REFIT && ct >= 0) refit_cell_faces(S, N, g, ph, f);
+}
+
+template <int TX, int TY, bool REFIT>
+__global__ __launch_bounds__(TX * TY) void box_sync_copy_kernel(const BoxDev* __restrict__ boxes,
+	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
+	const signed char* __restrict__ ctype, Phys ph, int fin, int fout)
+{
+	box_sync_copy_body<TX, TY, REFIT>(boxes, blockbox, N, S, ctype, ph, fin, fout, (int)blockIdx.x);
 }
 
 template <int TX, int TY>
@@ -579,7 +652,7 @@ template <int TX, int TY>
 __device__ __forceinline__ void box_boundary_body(const BoxDev* __restrict__ boxes,
 	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
 	const signed char* __restrict__ ctype, const unsigned char* __restrict__ rule,
-	BoxStatus* __restrict__ status, Phys ph, int tl)
+	BoxStatus* __restrict__ status, Phys ph, int fb, int tl)
 {
 	int b, i, j, k;
 	if (!my_cell<TX, TY>(boxes, blockbox, tl, b, i, j, k)) return;
@@ -593,8 +666,8 @@ __device__ __forceinline__ void box_boundary_body(const BoxDev* __restrict__ box
 		S[4 * N + g] = B.sw_E;
 		for (int d = 0; d < 3; d++) {
 			S[(Q_VB + d) * N + g] = B.sw_B[d];
-			S[(Q_F + 2 * d) * N + g] = B.sw_B[d];
-			S[(Q_F + 2 * d + 1) * N + g] = B.sw_B[d];
+			S[(fb + 2 * d) * N + g] = B.sw_B[d];
+			S[(fb + 2 * d + 1) * N + g] = B.sw_B[d];
 		}
 		return;
 	}
@@ -603,7 +676,7 @@ __device__ __forceinline__ void box_boundary_body(const BoxDev* __restrict__ box
 		for (int d = 0; d < 3; d++) S[(1 + d) * N + g] = B.in_m[d];
 		S[4 * N + g] = B.in_E;
 		for (int q = 0; q < 3; q++) S[(Q_VB + q) * N + g] = 0;
-		for (int q = 0; q < 6; q++) S[(Q_F + q) * N + g] = 0;
+		for (int q = 0; q < 6; q++) S[(fb + q) * N + g] = 0;
 		return;
 	}
 	const int o[3] = {(r - 1) % 3 - 1, ((r - 1) / 3) % 3 - 1, (r - 1) / 9 - 1};
@@ -617,10 +690,10 @@ __device__ __forceinline__ void box_boundary_body(const BoxDev* __restrict__ box
 	for (int q = 1; q <= 3; q++) S[q * N + g] = S[q * N + src];
 	double vb[3];
 	for (int d = 0; d < 3; d++) {
-		const double lo = S[(Q_F + 2 * d) * N + src], hi = S[(Q_F + 2 * d + 1) * N + src];
+		const double lo = S[(fb + 2 * d) * N + src], hi = S[(fb + 2 * d + 1) * N + src];
 		const double nlo = o[d] ? hi : lo, nhi = o[d] ? lo : hi;
-		S[(Q_F + 2 * d) * N + g] = nlo;
-		S[(Q_F + 2 * d + 1) * N + g] = nhi;
+		S[(fb + 2 * d) * N + g] = nlo;
+		S[(fb + 2 * d + 1) * N + g] = nhi;
 		vb[d] = 0.5 * (nlo + nhi);
 		S[(Q_VB + d) * N + g] = vb[d];
 	}
@@ -634,9 +707,9 @@ template <int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void box_boundary_kernel(const BoxDev* __restrict__ boxes,
 	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
 	const signed char* __restrict__ ctype, const unsigned char* __restrict__ rule,
-	BoxStatus* __restrict__ status, Phys ph)
+	BoxStatus* __restrict__ status, Phys ph, int fb)
 {
-	box_boundary_body<TX, TY>(boxes, blockbox, N, S, ctype, rule, status, ph, (int)blockIdx.x);
+	box_boundary_body<TX, TY>(boxes, blockbox, N, S, ctype, rule, status, ph, fb, (int)blockIdx.x);
 }
 
 /* the face each inflow cell shares with its interior neighbour gets the
@@ -645,7 +718,7 @@ __global__ __launch_bounds__(TX * TY) void box_boundary_kernel(const BoxDev* __r
 template <int TX, int TY>
 __device__ __forceinline__ void box_inflow_face_body(const BoxDev* __restrict__ boxes,
 	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
-	const unsigned char* __restrict__ rule, BoxStatus* __restrict__ status, Phys ph, int tl)
+	const unsigned char* __restrict__ rule, BoxStatus* __restrict__ status, Phys ph, int fb, int tl)
 {
 	int b, i, j, k;
 	if (!my_cell<TX, TY>(boxes, blockbox, tl, b, i, j, k)) return;
@@ -666,7 +739,7 @@ __device__ __forceinline__ void box_inflow_face_body(const BoxDev* __restrict__ 
 	const double inv = 1 / rho;
 	const double v[3] = {S[N + nb] * inv, S[2 * N + nb] * inv, S[3 * N + nb] * inv};
 	const double p = pressure_of(S, N, nb, ph);
-	const int toward = Q_F + 2 * B.sw_dim + (B.sw_sg > 0), away = Q_F + 2 * B.sw_dim + (B.sw_sg < 0);
+	const int toward = fb + 2 * B.sw_dim + (B.sw_sg > 0), away = fb + 2 * B.sw_dim + (B.sw_sg < 0);
 	S[toward * N + nb] = B.sw_B[B.sw_dim];
 	S[(Q_VB + B.sw_dim) * N + nb] = S[toward * N + nb] / 2 + S[away * N + nb] / 2;
 	const double vb[3] = {S[5 * N + nb], S[6 * N + nb], S[7 * N + nb]};
@@ -676,9 +749,9 @@ __device__ __forceinline__ void box_inflow_face_body(const BoxDev* __restrict__ 
 template <int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void box_inflow_face_kernel(const BoxDev* __restrict__ boxes,
 	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
-	const unsigned char* __restrict__ rule, BoxStatus* __restrict__ status, Phys ph)
+	const unsigned char* __restrict__ rule, BoxStatus* __restrict__ status, Phys ph, int fb)
 {
-	box_inflow_face_body<TX, TY>(boxes, blockbox, N, S, rule, status, ph, (int)blockIdx.x);
+	box_inflow_face_body<TX, TY>(boxes, blockbox, N, S, rule, status, ph, fb, (int)blockIdx.x);
 }
 
 /* sum of |div B| of normal cells per block, source/math/nabla.hpp:58-113 */
@@ -769,20 +842,21 @@ struct RunArgs {
 	double cfl;
 };
 
-/* the kernel walks stage after stage (8 per step) with the stage picked by
-   a switch, so each stage's values live only in its own case: the
-   kernel's registers are about the largest stage's, not every stage's at
-   once */
+/* the kernel walks stage after stage (7 per step: dt, flux, update, sync
+   + refit into the second face set, boundaries and inflow faces on it,
+   sync back) with the stage picked by a switch, so each stage's values
+   live only in its own case: the kernel's registers are about the largest
+   stage's, not every stage's at once */
 template <int SOLVER, int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void box_run_kernel(RunArgs a, long steps, int last, unsigned* bar,
 	unsigned polls)
 {
 	const int tid = threadIdx.x + TX * threadIdx.y;
-	for (long q = 0; q < 8 * steps; q++) {
-		const int lst = (int)((last + q / 8) & 1);
+	for (long q = 0; q < 7 * steps; q++) {
+		const int lst = (int)((last + q / 7) & 1);
 		BoxStatus* prev = a.status + (long long)lst * a.nbox;
 		BoxStatus* next = a.status + (long long)(1 - lst) * a.nbox;
-		switch ((int)(q % 8)) {
+		switch ((int)(q % 7)) {
 		case 0:
 			for (int b = (int)blockIdx.x * TX * TY + tid; b < a.nbox; b += (int)gridDim.x * TX * TY)
 				box_dt_body(a.boxes, prev, next, a.dt, a.cfl, 1e300, b);
@@ -796,21 +870,20 @@ __global__ __launch_bounds__(TX * TY) void box_run_kernel(RunArgs a, long steps,
 				box_update_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.P, a.ph, t);
 			break;
 		case 3:
-		case 7:
 			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
-				box_sync_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.ctype, t);
+				box_sync_copy_body<TX, TY, true>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.ph, Q_F, Q_F2, t);
 			break;
 		case 4:
 			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
-				box_refit_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.ph, t);
+				box_boundary_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.rule, next, a.ph, Q_F2, t);
 			break;
 		case 5:
 			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
-				box_boundary_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.rule, next, a.ph, t);
+				box_inflow_face_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.rule, next, a.ph, Q_F2, t);
 			break;
 		default:
 			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
-				box_inflow_face_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.rule, next, a.ph, t);
+				box_sync_copy_body<TX, TY, false>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.ph, Q_F2, Q_F, t);
 			break;
 		}
 		if (!run_barrier(bar, (unsigned)(q + 1) * gridDim.x, polls)) return;
@@ -953,12 +1026,21 @@ template <int TX, int TY> void launch_all(BoxSet* s, int solver, hipStream_t st,
 		box_refit_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->ph);
 		break;
 	case 2:
-		box_boundary_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->rule, stat, s->ph);
-		box_inflow_face_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->rule, stat, s->ph);
+		box_boundary_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->rule, stat, s->ph, Q_F);
+		box_inflow_face_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->rule, stat, s->ph, Q_F);
 		box_sync_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype);
 		break;
 	case 3:
 		box_refit_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->ph);
+		break;
+	case 4:
+		/* stages 1 and 2 with the second face set: sync + refit into it,
+		   the boundaries on it, the last sync back */
+		box_update_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->P, s->ph);
+		box_sync_copy_kernel<TX, TY, true><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->ph, Q_F, Q_F2);
+		box_boundary_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->rule, stat, s->ph, Q_F2);
+		box_inflow_face_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->rule, stat, s->ph, Q_F2);
+		box_sync_copy_kernel<TX, TY, false><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->ph, Q_F2, Q_F);
 		break;
 	}
 }
@@ -1115,7 +1197,7 @@ int boxset_create(BoxSet** out, int nbox, const pamhd_grid_desc* grids, const pa
 		return PAMHD_E_DEVICE;
 	};
 	hipError_t e;
-	if ((e = hipMalloc(&s->S, sizeof(double) * NQ * N)) != hipSuccess) return bail(e, "state");
+	if ((e = hipMalloc(&s->S, sizeof(double) * NQ_ALLOC * N)) != hipSuccess) return bail(e, "state");
 	if ((e = hipMalloc(&s->P, sizeof(double) * 24 * N)) != hipSuccess) return bail(e, "flux");
 	if ((e = hipMalloc(&s->bg, sizeof(double) * 9 * N)) != hipSuccess) return bail(e, "bg");
 	if ((e = hipMalloc(&s->ctype, N)) != hipSuccess) return bail(e, "ctype");
@@ -1250,8 +1332,7 @@ int boxset_launch_step(BoxSet* s, int solver, double max_dt, double cfl, bool wi
 	BoxStatus* next = s->status + (long long)(1 - s->last) * s->nbox;
 	box_dt_kernel<<<(s->nbox + 63) / 64, 64, 0, st>>>(s->boxes, s->nbox, prev, next, s->dt, cfl, max_dt);
 	launch_stage(s, solver, st, 0, next);
-	launch_stage(s, solver, st, 1, next);
-	if (with_boundaries) launch_stage(s, solver, st, 2, next);
+	launch_stage(s, solver, st, with_boundaries ? 4 : 1, next);
 	s->last = 1 - s->last;
 	const hipError_t e = hipGetLastError();
 	if (e != hipSuccess) {
