@@ -1,10 +1,11 @@
 /*
 Ensemble of solar-wind boxes behind the C ABI: every box of the set is
 advanced by the same launches (one per stage of the step), and a run of
-main-loop steps replays a HIP graph holding two consecutive steps (the
-lagged-step status alternates between two buffers, so two steps bring the
-launch arguments back to where they started), or runs every step in one
-persistent cooperative launch.
+main-loop steps replays a HIP graph holding a chain of 16 (or 2)
+consecutive steps (the lagged-step status alternates between two buffers,
+so an even number of steps brings the launch arguments back to where they
+started; inside a chain each step's last face sync is done by the next
+step's update), or runs every step in one persistent cooperative launch.
 */
 #include <hip/hip_runtime.h>
 
@@ -100,22 +101,24 @@ int pamhd_ensemble_run(pamhd_ensemble* e, int solver, double time_step_factor, l
 		rc = boxset_run_persistent(e->set, solver, time_step_factor, steps, st, e->err);
 		return rc ? efail(e, rc, e->err) : PAMHD_OK;
 	}
+	/* chains of at most 1024 steps (the launch count of one call is an int) */
 	if (!use_graph || steps < 4) {
-		for (long s = 0; s < steps && rc == PAMHD_OK; s++) {
-			rc = boxset_launch_step(e->set, solver, 1e300, time_step_factor, true, st, e->err);
+		for (long done = 0; done < steps && rc == PAMHD_OK; done += 1024) {
+			const int n = (int)(steps - done < 1024 ? steps - done : 1024);
+			rc = boxset_launch_steps(e->set, solver, time_step_factor, n, st, e->err);
 		}
-		return rc;
+		return rc ? efail(e, rc, e->err) : PAMHD_OK;
 	}
-	if (steps % 2) rc = boxset_launch_step(e->set, solver, 1e300, time_step_factor, true, st, e->err);
-	if (rc) return rc;
-	/* capture two steps (st is never the default stream here) */
+	const int chain = steps >= 32 ? 16 : 2;
+	if (steps % chain) rc = boxset_launch_steps(e->set, solver, time_step_factor, (int)(steps % chain), st, e->err);
+	if (rc) return efail(e, rc, e->err);
+	/* capture one chain (st is never the default stream here) */
 	hipStream_t cap = st;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
 	hipError_t he = hipStreamBeginCapture(cap, hipStreamCaptureModeThreadLocal);
 	if (he != hipSuccess) return efail(e, PAMHD_E_DEVICE, std::string("begin capture: ") + hipGetErrorString(he));
-	int r1 = boxset_launch_step(e->set, solver, 1e300, time_step_factor, true, cap, e->err);
-	int r2 = r1 ? r1 : boxset_launch_step(e->set, solver, 1e300, time_step_factor, true, cap, e->err);
+	const int r2 = boxset_launch_steps(e->set, solver, time_step_factor, chain, cap, e->err);
 	he = hipStreamEndCapture(cap, &graph);
 	if (r2 || he != hipSuccess) {
 		if (graph) (void)hipGraphDestroy(graph);
@@ -126,7 +129,7 @@ int pamhd_ensemble_run(pamhd_ensemble* e, int solver, double time_step_factor, l
 		(void)hipGraphDestroy(graph);
 		return efail(e, PAMHD_E_DEVICE, std::string("instantiate: ") + hipGetErrorString(he));
 	}
-	const long reps = (steps - steps % 2) / 2;
+	const long reps = steps / chain;
 	for (long r = 0; r < reps && he == hipSuccess; r++) he = hipGraphLaunch(exec, cap);
 	/* the executable graph may be destroyed once its launches complete */
 	(void)hipStreamSynchronize(cap);

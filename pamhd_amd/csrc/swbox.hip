@@ -174,6 +174,32 @@ __device__ __forceinline__ void sync_pair(double& a, double& o)
 	a += diff2 / 2;
 }
 
+/* the six face copies of cell g (type ct) after that sync, from the copies
+   at component fin: each face shared with a cell of the box of type >= 0
+   is synced with the other copy (a cell of type < 0 keeps its copies) */
+__device__ __forceinline__ void synced_faces(const BoxDev& B, long long N, const double* __restrict__ S,
+	const signed char* __restrict__ ctype, long long g, int ct, int i, int j, int k, int fin, double f[6])
+{
+	const int id[3] = {i, j, k};
+	const int dims[3] = {B.nx, B.ny, B.nz};
+	const long long st[3] = {1, B.nx, (long long)B.nx * B.ny};
+	for (int d = 0; d < 3; d++) {
+		double lo = S[(fin + 2 * d) * N + g], hi = S[(fin + 2 * d + 1) * N + g];
+		if (ct >= 0) {
+			if (id[d] >= 1 && ctype[g - st[d]] >= 0) {
+				double a = S[(fin + 2 * d + 1) * N + g - st[d]];
+				sync_pair(a, lo);
+			}
+			if (id[d] + 1 < dims[d] && ctype[g + st[d]] >= 0) {
+				double o = S[(fin + 2 * d) * N + g + st[d]];
+				sync_pair(hi, o);
+			}
+		}
+		f[2 * d] = lo;
+		f[2 * d + 1] = hi;
+	}
+}
+
 __device__ __forceinline__ void box_record_error(BoxStatus* st, int code, long long face)
 {
 	if (atomicCAS(&st->err, 0, code) == 0) st->err_face = face;
@@ -441,18 +467,26 @@ edge terms of assign_face_dBs_f{x,y,z} (source/mhd/solve.hpp:610-613,
 :926-928, :1223-1225) in the order of the periodic step, face += dB / area,
 conserved update of rho, m, E and Vol_B (update_mhd_state, :1551-1593).
 Cells of type >= 0: refit at constant pressure (update_vol_B, :1904-1908).
+With fsrc = Q_F2 the face copies come from the second set, synced here
+(the previous step's last sync, which left them there), and every cell
+writes its copies into the first set.
 */
 template <int TX, int TY>
 __device__ __forceinline__ void box_update_body(const BoxDev* __restrict__ boxes,
 	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
-	const signed char* __restrict__ ctype, const double* __restrict__ P, Phys ph, int tl)
+	const signed char* __restrict__ ctype, const double* __restrict__ P, Phys ph, int fsrc, int tl)
 {
 	int b, i, j, k;
 	if (!my_cell<TX, TY>(boxes, blockbox, tl, b, i, j, k)) return;
 	const BoxDev& B = boxes[b];
 	const long long g = gidx(B, i, j, k);
 	const int ct = ctype[g];
-	if (ct < 0) return;
+	const bool moved = fsrc != Q_F;
+	if (ct < 0 && !moved) return;
+	double f[6];
+	if (moved) synced_faces(B, N, S, ctype, g, ct, i, j, k, fsrc, f);
+	else
+		for (int q = 0; q < 6; q++) f[q] = S[(Q_F + q) * N + g];
 	if (ct > 0) {
 		const double dx = B.dx, dy = B.dy, dz = B.dz;
 		const long long sx = 1, sy = B.nx, sz = (long long)B.nx * B.ny;
@@ -487,10 +521,7 @@ __device__ __forceinline__ void box_update_body(const BoxDev* __restrict__ boxes
 		}
 		const double area[3] = {dy * dz, dx * dz, dx * dy};
 		for (int d = 0; d < 3; d++)
-		for (int side = 0; side < 2; side++) {
-			double& f = S[(Q_F + 2 * d + side) * N + g];
-			f = f + dB[d][side] / area[d];
-		}
+		for (int side = 0; side < 2; side++) f[2 * d + side] = f[2 * d + side] + dB[d][side] / area[d];
 		const double len[3] = {dx, dy, dz};
 		const long long st[3] = {sx, sy, sz};
 		double lo[3][8], hi[3][8];
@@ -518,15 +549,17 @@ __device__ __forceinline__ void box_update_body(const BoxDev* __restrict__ boxes
 		S[4 * N + g] = E;
 		for (int q = 0; q < 3; q++) S[(Q_VB + q) * N + g] = vb[q];
 	}
-	refit_cell(S, N, g, ph);
+	if (ct > 0 || moved)
+		for (int q = 0; q < 6; q++) S[(Q_F + q) * N + g] = f[q];
+	if (ct >= 0) refit_cell_faces(S, N, g, ph, f);
 }
 
 template <int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void box_update_kernel(const BoxDev* __restrict__ boxes,
 	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
-	const signed char* __restrict__ ctype, const double* __restrict__ P, Phys ph)
+	const signed char* __restrict__ ctype, const double* __restrict__ P, Phys ph, int fsrc)
 {
-	box_update_body<TX, TY>(boxes, blockbox, N, S, ctype, P, ph, (int)blockIdx.x);
+	box_update_body<TX, TY>(boxes, blockbox, N, S, ctype, P, ph, fsrc, (int)blockIdx.x);
 }
 
 /*
@@ -580,27 +613,9 @@ __device__ __forceinline__ void box_sync_copy_body(const BoxDev* __restrict__ bo
 	const BoxDev& B = boxes[b];
 	const long long g = gidx(B, i, j, k);
 	const int ct = ctype[g];
-	const int id[3] = {i, j, k};
-	const int dims[3] = {B.nx, B.ny, B.nz};
-	const long long st[3] = {1, B.nx, (long long)B.nx * B.ny};
 	double f[6];
-	for (int d = 0; d < 3; d++) {
-		double lo = S[(fin + 2 * d) * N + g], hi = S[(fin + 2 * d + 1) * N + g];
-		if (ct >= 0) {
-			if (id[d] >= 1 && ctype[g - st[d]] >= 0) {
-				double a = S[(fin + 2 * d + 1) * N + g - st[d]];
-				sync_pair(a, lo);
-			}
-			if (id[d] + 1 < dims[d] && ctype[g + st[d]] >= 0) {
-				double o = S[(fin + 2 * d) * N + g + st[d]];
-				sync_pair(hi, o);
-			}
-		}
-		f[2 * d] = lo;
-		f[2 * d + 1] = hi;
-		S[(fout + 2 * d) * N + g] = lo;
-		S[(fout + 2 * d + 1) * N + g] = hi;
-	}
+	synced_faces(B, N, S, ctype, g, ct, i, j, k, fin, f);
+	for (int q = 0; q < 6; q++) S[(fout + q) * N + g] = f[q];
 	if (REFIT && ct >= 0) refit_cell_faces(S, N, g, ph, f);
 }
 
@@ -842,17 +857,22 @@ struct RunArgs {
 	double cfl;
 };
 
-/* the kernel walks stage after stage (7 per step: dt, flux, update, sync
-   + refit into the second face set, boundaries and inflow faces on it,
-   sync back) with the stage picked by a switch, so each stage's values
-   live only in its own case: the kernel's registers are about the largest
-   stage's, not every stage's at once */
+/* the kernel walks stage after stage (per step: dt, flux, update, sync +
+   refit into the second face set, boundaries and inflow faces on it, and
+   after the last step the sync back, which the next step's update does
+   for the others) with the stage picked by a switch, so each stage's
+   values live only in its own case: the kernel's registers are about the
+   largest stage's, not every stage's at once */
 template <int SOLVER, int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void box_run_kernel(RunArgs a, long steps, int last, unsigned* bar,
 	unsigned polls)
 {
 	const int tid = threadIdx.x + TX * threadIdx.y;
+	unsigned passed = 0;   /* grid barriers passed */
 	for (long q = 0; q < 7 * steps; q++) {
+		/* the last step syncs back into the first set; the others leave
+		   it to the next step's update (no stage, no barrier) */
+		if (q % 7 == 6 && q + 1 != 7 * steps) continue;
 		const int lst = (int)((last + q / 7) & 1);
 		BoxStatus* prev = a.status + (long long)lst * a.nbox;
 		BoxStatus* next = a.status + (long long)(1 - lst) * a.nbox;
@@ -865,10 +885,14 @@ __global__ __launch_bounds__(TX * TY) void box_run_kernel(RunArgs a, long steps,
 			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
 				box_flux3_body<SOLVER, TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.bg, a.P, a.dt, next, a.ph, t);
 			break;
-		case 2:
+		case 2: {
+			/* the steps after the first take the face copies the previous
+			   step left in the second set */
+			const int fsrc = q >= 7 ? Q_F2 : Q_F;
 			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
-				box_update_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.P, a.ph, t);
+				box_update_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.P, a.ph, fsrc, t);
 			break;
+		}
 		case 3:
 			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
 				box_sync_copy_body<TX, TY, true>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.ph, Q_F, Q_F2, t);
@@ -886,7 +910,7 @@ __global__ __launch_bounds__(TX * TY) void box_run_kernel(RunArgs a, long steps,
 				box_sync_copy_body<TX, TY, false>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.ph, Q_F2, Q_F, t);
 			break;
 		}
-		if (!run_barrier(bar, (unsigned)(q + 1) * gridDim.x, polls)) return;
+		if (!run_barrier(bar, ++passed * gridDim.x, polls)) return;
 	}
 }
 
@@ -998,8 +1022,13 @@ void classify(const pamhd_grid_desc& G, const pamhd_solar_wind_desc& w, signed c
 	}
 }
 
+/* stage 4's flags: the update takes the face copies from the second set
+   (the previous step of a chain left them there), and the step leaves its
+   copies in the second set for the next step of the chain */
+enum { CH_FROM2 = 1, CH_KEEP2 = 2 };
+
 template <int TX, int TY> void launch_all(BoxSet* s, int solver, hipStream_t st, int stage,
-	BoxStatus* stat)
+	BoxStatus* stat, int chain)
 {
 	const dim3 grid(s->nblocks), block(TX, TY, 1);
 	switch (stage) {
@@ -1021,7 +1050,7 @@ template <int TX, int TY> void launch_all(BoxSet* s, int solver, hipStream_t st,
 #undef FLUX
 		break;
 	case 1:
-		box_update_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->P, s->ph);
+		box_update_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->P, s->ph, Q_F);
 		box_sync_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype);
 		box_refit_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->ph);
 		break;
@@ -1035,22 +1064,25 @@ template <int TX, int TY> void launch_all(BoxSet* s, int solver, hipStream_t st,
 		break;
 	case 4:
 		/* stages 1 and 2 with the second face set: sync + refit into it,
-		   the boundaries on it, the last sync back */
-		box_update_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->P, s->ph);
+		   the boundaries on it, the last sync back (or, inside a chain,
+		   in the next step's update) */
+		box_update_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->P, s->ph,
+			(chain & CH_FROM2) ? Q_F2 : Q_F);
 		box_sync_copy_kernel<TX, TY, true><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->ph, Q_F, Q_F2);
 		box_boundary_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->rule, stat, s->ph, Q_F2);
 		box_inflow_face_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->rule, stat, s->ph, Q_F2);
-		box_sync_copy_kernel<TX, TY, false><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->ph, Q_F2, Q_F);
+		if (!(chain & CH_KEEP2))
+			box_sync_copy_kernel<TX, TY, false><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->ph, Q_F2, Q_F);
 		break;
 	}
 }
 
-void launch_stage(BoxSet* s, int solver, hipStream_t st, int stage, BoxStatus* stat)
+void launch_stage(BoxSet* s, int solver, hipStream_t st, int stage, BoxStatus* stat, int chain = 0)
 {
 	switch (s->tile) {
-	case 0: launch_all<64, 4>(s, solver, st, stage, stat); break;
-	case 1: launch_all<32, 8>(s, solver, st, stage, stat); break;
-	default: launch_all<16, 16>(s, solver, st, stage, stat); break;
+	case 0: launch_all<64, 4>(s, solver, st, stage, stat, chain); break;
+	case 1: launch_all<32, 8>(s, solver, st, stage, stat, chain); break;
+	default: launch_all<16, 16>(s, solver, st, stage, stat, chain); break;
 	}
 }
 
@@ -1321,19 +1353,47 @@ int boxset_create(BoxSet** out, int nbox, const pamhd_grid_desc* grids, const pa
 	return PAMHD_OK;
 }
 
-int boxset_launch_step(BoxSet* s, int solver, double max_dt, double cfl, bool with_boundaries,
-	hipStream_t st, std::string& err)
+namespace {
+
+/* dt, flux and the update stages of one step (chain: stage 4's flags) */
+void launch_one_step(BoxSet* s, int solver, double max_dt, double cfl, int last_stage, int chain,
+	hipStream_t st)
 {
-	if (solver != RUSANOV && solver != HLL_ATHENA && solver != HLLD_ATHENA && solver != ROE_ATHENA) {
-		err = "invalid solver " + std::to_string(solver);
-		return PAMHD_E_SOLVER;
-	}
 	BoxStatus* prev = s->status + (long long)s->last * s->nbox;
 	BoxStatus* next = s->status + (long long)(1 - s->last) * s->nbox;
 	box_dt_kernel<<<(s->nbox + 63) / 64, 64, 0, st>>>(s->boxes, s->nbox, prev, next, s->dt, cfl, max_dt);
 	launch_stage(s, solver, st, 0, next);
-	launch_stage(s, solver, st, with_boundaries ? 4 : 1, next);
+	launch_stage(s, solver, st, last_stage, next, chain);
 	s->last = 1 - s->last;
+}
+
+bool valid_solver(int solver, std::string& err)
+{
+	if (solver == RUSANOV || solver == HLL_ATHENA || solver == HLLD_ATHENA || solver == ROE_ATHENA) return true;
+	err = "invalid solver " + std::to_string(solver);
+	return false;
+}
+
+} // namespace
+
+int boxset_launch_steps(BoxSet* s, int solver, double cfl, int steps, hipStream_t st, std::string& err)
+{
+	if (!valid_solver(solver, err)) return PAMHD_E_SOLVER;
+	for (int q = 0; q < steps; q++)
+		launch_one_step(s, solver, 1e300, cfl, 4, (q > 0 ? CH_FROM2 : 0) | (q + 1 < steps ? CH_KEEP2 : 0), st);
+	const hipError_t e = hipGetLastError();
+	if (e != hipSuccess) {
+		err = std::string("launch: ") + hipGetErrorString(e);
+		return PAMHD_E_DEVICE;
+	}
+	return PAMHD_OK;
+}
+
+int boxset_launch_step(BoxSet* s, int solver, double max_dt, double cfl, bool with_boundaries,
+	hipStream_t st, std::string& err)
+{
+	if (!valid_solver(solver, err)) return PAMHD_E_SOLVER;
+	launch_one_step(s, solver, max_dt, cfl, with_boundaries ? 4 : 1, 0, st);
 	const hipError_t e = hipGetLastError();
 	if (e != hipSuccess) {
 		err = std::string("launch: ") + hipGetErrorString(e);

@@ -44,6 +44,14 @@ int boxset_launch_step(BoxSet* s, int solver, double max_dt, double cfl, bool wi
 	hipStream_t st, std::string& err);
 
 /*
+`steps` main-loop steps of every box (boxset_launch_step with boundaries
+and max_dt unbounded, each step's last face sync done in the next step's
+update, the last step's in its own pass); results as from that many
+boxset_launch_step calls.  No synchronisation.
+*/
+int boxset_launch_steps(BoxSet* s, int solver, double cfl, int steps, hipStream_t st, std::string& err);
+
+/*
 `steps` main-loop steps of every box (boxset_launch_step with boundaries,
 max_dt unbounded) in one persistent cooperative launch; the stages are
 separated by grid-wide barriers whose waits are bounded (a timeout returns

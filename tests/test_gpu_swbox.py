@@ -130,29 +130,34 @@ def _ensemble_boxes():
     return [_test_box(cells=c, seed=10 + q) for q, c in enumerate(sizes)]
 
 
-@pytest.mark.parametrize("use_graph,tile", [(False, 0), (True, 0), (True, 1), (True, 2),
-                                            ("persistent", 0), ("persistent", 1), ("persistent", 2)])
-def test_ensemble_matches_single_boxes(gpu, use_graph, tile):
+@pytest.mark.parametrize("use_graph,tile,nsteps", [
+    (False, 0, 7), (True, 0, 7), (True, 1, 7), (True, 2, 7),
+    ("persistent", 0, 7), ("persistent", 1, 7), ("persistent", 2, 7),
+    (False, 1, 34), (True, 1, 34), ("persistent", 1, 34)])
+def test_ensemble_matches_single_boxes(gpu, use_graph, tile, nsteps):
     """Boxes of different sizes advanced together (each with its own time
     step) give every box's oracle run bit for bit, with and without the
-    captured graph, in one persistent launch, and for every tile shape; 7
-    steps exercise the eager step before the graph replays."""
+    captured graph, in one persistent launch, and for every tile shape.
+    Consecutive steps of one call leave their last face sync to the next
+    step's update: 7 steps run as a chain of 1 before a graph of 2 steps
+    replayed three times (or as one chain of 7), 34 steps as a chain of 2
+    before a graph of 16 steps replayed twice (or one chain of 34)."""
     from pamhd_amd.swbox import Ensemble
     boxes = _ensemble_boxes()
     refs = []
     for sb in boxes:
         ob = O.Box(sb)
         r = ob.init()
-        st, steps, t = ob.run(O.HLL, 0.3, 0.0, 1e30, r, max_steps=7)
-        assert st == 0
+        st, steps, t = ob.run(O.HLL, 0.3, 0.0, 1e30, r, max_steps=nsteps)
+        assert st == 0 and steps == nsteps
         refs.append(r)
     with Ensemble(boxes) as e:
         e.set_tile(tile)
         e.timestep("hll-athena", 0.0, 0.3)
         if use_graph == "persistent":
-            e.run("hll-athena", 0.3, 7, persistent=True)
+            e.run("hll-athena", 0.3, nsteps, persistent=True)
         else:
-            e.run("hll-athena", 0.3, 7, use_graph=use_graph)
+            e.run("hll-athena", 0.3, nsteps, use_graph=use_graph)
         dts = e.check()
         outs = [e.download(q) for q in range(len(boxes))]
     assert np.all(dts > 0)
