@@ -545,7 +545,9 @@ program.)
                graph, with use_graph = 2 they all run in one persistent
                cooperative launch (stages separated by grid-wide barriers
                whose waits are bounded: a timeout returns PAMHD_E_DEVICE;
-               synchronous); every mode gives the same results
+               synchronous); every mode gives the same results, and
+               the steps of one call run as chains whose steps leave
+               their last face sync to the next step's update
   check        synchronises; per-box time step of the last step and the
                first physics error
   autotune     picks the fastest tile shape of the cell kernels (64x4, 32x8,
