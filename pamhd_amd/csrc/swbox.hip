@@ -26,8 +26,10 @@ A main-loop step's launches, each covering every box:
   sync + refit  each cell's six face copies synced with the other copies
                 into the second set, cells of type >= 0 refit from them
   boundary      copy / planet / inflow rules on the second set (reads only
-                normal cells)
-  inflow face   interior neighbours of the inflow cells
+                normal cells; a thread per cell of the set's list of cells
+                with a rule)
+  inflow face   interior neighbours of the inflow cells (a thread per cell
+                of the list of inflow cells)
   sync          from the second set back into the first
 A step without boundaries runs update, sync (one thread per face writes
 both copies in place) and refit instead.  No kernel reads what another
@@ -72,6 +74,12 @@ struct BoxStatus {
 };
 
 struct Tile { int tx, ty; };
+
+/* a cell of a set's list: global index and box */
+struct CellRef {
+	long long g;
+	int b, pad;
+};
 constexpr Tile TILES[3] = {{64, 4}, {32, 8}, {16, 16}};
 
 } // namespace
@@ -97,6 +105,10 @@ struct BoxSet {
 	double* dt = nullptr;          /* nbox */
 	double* partial = nullptr;
 	unsigned* bar = nullptr;       /* persistent run: arrival count and error flag */
+	CellRef* bnd = nullptr;        /* cells with a boundary rule, in index order */
+	long long nbnd = 0;
+	CellRef* inflow = nullptr;     /* inflow cells, in index order */
+	long long ninflow = 0;
 	unsigned bar_polls = 0;        /* polls before a barrier gives up (0: RUN_SPIN_LIMIT) */
 	BoxStatus* host_status = nullptr;
 	double* host_dt = nullptr;
@@ -662,17 +674,14 @@ neighbour, with the face copies mirrored across the walls they touch;
 planet cells get the inner-boundary state (energy computed with the
 momentum in place of the velocity, as the reference does); inflow cells the
 solar-wind state.  Copies read only normal cells, which no thread writes.
+A thread per entry of the set's list of cells with a rule (cell g of box
+b), the face copies at component fb.
 */
-template <int TX, int TY>
-__device__ __forceinline__ void box_boundary_body(const BoxDev* __restrict__ boxes,
-	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
-	const signed char* __restrict__ ctype, const unsigned char* __restrict__ rule,
-	BoxStatus* __restrict__ status, Phys ph, int fb, int tl)
+__device__ __forceinline__ void box_boundary_cell(const BoxDev* __restrict__ boxes, long long N,
+	double* __restrict__ S, const unsigned char* __restrict__ rule, BoxStatus* __restrict__ status,
+	Phys ph, int fb, int b, long long g)
 {
-	int b, i, j, k;
-	if (!my_cell<TX, TY>(boxes, blockbox, tl, b, i, j, k)) return;
 	const BoxDev& B = boxes[b];
-	const long long g = gidx(B, i, j, k);
 	const int r = rule[g];
 	if (r == R_NONE) return;
 	if (r == R_SW) {
@@ -718,28 +727,28 @@ __device__ __forceinline__ void box_boundary_body(const BoxDev* __restrict__ box
 	S[4 * N + g] = total_energy64(rho, v, p, vb, ph.gm1, ph.half_inv_mu0);
 }
 
-template <int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void box_boundary_kernel(const BoxDev* __restrict__ boxes,
-	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
-	const signed char* __restrict__ ctype, const unsigned char* __restrict__ rule,
-	BoxStatus* __restrict__ status, Phys ph, int fb)
+constexpr int LIST_BLOCK = 256;
+
+__global__ __launch_bounds__(LIST_BLOCK) void box_boundary_kernel(const BoxDev* __restrict__ boxes,
+	const CellRef* __restrict__ list, long long count, long long N, double* __restrict__ S,
+	const unsigned char* __restrict__ rule, BoxStatus* __restrict__ status, Phys ph, int fb)
 {
-	box_boundary_body<TX, TY>(boxes, blockbox, N, S, ctype, rule, status, ph, fb, (int)blockIdx.x);
+	const long long e = (long long)blockIdx.x * LIST_BLOCK + threadIdx.x;
+	if (e < count) box_boundary_cell(boxes, N, S, rule, status, ph, fb, list[e].b, list[e].g);
 }
 
 /* the face each inflow cell shares with its interior neighbour gets the
    solar-wind normal field, with that neighbour's Vol_B component and energy
-   refit at constant pressure (source/mhd/solar_wind_box.hpp:275-299) */
-template <int TX, int TY>
-__device__ __forceinline__ void box_inflow_face_body(const BoxDev* __restrict__ boxes,
-	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
-	const unsigned char* __restrict__ rule, BoxStatus* __restrict__ status, Phys ph, int fb, int tl)
+   refit at constant pressure (source/mhd/solar_wind_box.hpp:275-299); a
+   thread per entry of the set's list of inflow cells */
+__device__ __forceinline__ void box_inflow_face_cell(const BoxDev* __restrict__ boxes, long long N,
+	double* __restrict__ S, const unsigned char* __restrict__ rule, BoxStatus* __restrict__ status,
+	Phys ph, int fb, int b, long long g)
 {
-	int b, i, j, k;
-	if (!my_cell<TX, TY>(boxes, blockbox, tl, b, i, j, k)) return;
 	const BoxDev& B = boxes[b];
-	const long long g = gidx(B, i, j, k);
 	if (rule[g] != R_SW) return;
+	const long long l = g - B.off;
+	const int i = (int)(l % B.nx), j = (int)((l / B.nx) % B.ny), k = (int)(l / ((long long)B.nx * B.ny));
 	const int id[3] = {i, j, k};
 	const int dims[3] = {B.nx, B.ny, B.nz};
 	const int ni = id[B.sw_dim] - B.sw_sg;
@@ -761,12 +770,12 @@ __device__ __forceinline__ void box_inflow_face_body(const BoxDev* __restrict__ 
 	S[4 * N + nb] = total_energy64(rho, v, p, vb, ph.gm1, ph.half_inv_mu0);
 }
 
-template <int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void box_inflow_face_kernel(const BoxDev* __restrict__ boxes,
-	const int* __restrict__ blockbox, long long N, double* __restrict__ S,
+__global__ __launch_bounds__(LIST_BLOCK) void box_inflow_face_kernel(const BoxDev* __restrict__ boxes,
+	const CellRef* __restrict__ list, long long count, long long N, double* __restrict__ S,
 	const unsigned char* __restrict__ rule, BoxStatus* __restrict__ status, Phys ph, int fb)
 {
-	box_inflow_face_body<TX, TY>(boxes, blockbox, N, S, rule, status, ph, fb, (int)blockIdx.x);
+	const long long e = (long long)blockIdx.x * LIST_BLOCK + threadIdx.x;
+	if (e < count) box_inflow_face_cell(boxes, N, S, rule, status, ph, fb, list[e].b, list[e].g);
 }
 
 /* sum of |div B| of normal cells per block, source/math/nabla.hpp:58-113 */
@@ -848,6 +857,9 @@ struct RunArgs {
 	double* S;
 	const signed char* ctype;
 	const unsigned char* rule;
+	const CellRef* bnd;
+	const CellRef* inflow;
+	long long nbnd, ninflow;
 	const double* bg;
 	double* P;
 	double* dt;
@@ -898,12 +910,12 @@ __global__ __launch_bounds__(TX * TY) void box_run_kernel(RunArgs a, long steps,
 				box_sync_copy_body<TX, TY, true>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.ph, Q_F, Q_F2, t);
 			break;
 		case 4:
-			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
-				box_boundary_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.ctype, a.rule, next, a.ph, Q_F2, t);
+			for (long long e = (long long)blockIdx.x * TX * TY + tid; e < a.nbnd; e += (long long)gridDim.x * TX * TY)
+				box_boundary_cell(a.boxes, a.N, a.S, a.rule, next, a.ph, Q_F2, a.bnd[e].b, a.bnd[e].g);
 			break;
 		case 5:
-			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
-				box_inflow_face_body<TX, TY>(a.boxes, a.blockbox, a.N, a.S, a.rule, next, a.ph, Q_F2, t);
+			for (long long e = (long long)blockIdx.x * TX * TY + tid; e < a.ninflow; e += (long long)gridDim.x * TX * TY)
+				box_inflow_face_cell(a.boxes, a.N, a.S, a.rule, next, a.ph, Q_F2, a.inflow[e].b, a.inflow[e].g);
 			break;
 		default:
 			for (int t = blockIdx.x; t < a.nblocks; t += gridDim.x)
@@ -1022,6 +1034,17 @@ void classify(const pamhd_grid_desc& G, const pamhd_solar_wind_desc& w, signed c
 	}
 }
 
+/* the boundary rules, then the inflow faces, a thread per listed cell */
+void launch_boundary_lists(BoxSet* s, hipStream_t st, BoxStatus* stat, int fb)
+{
+	if (s->nbnd > 0)
+		box_boundary_kernel<<<(unsigned)((s->nbnd + LIST_BLOCK - 1) / LIST_BLOCK), LIST_BLOCK, 0, st>>>(
+			s->boxes, s->bnd, s->nbnd, s->N, s->S, s->rule, stat, s->ph, fb);
+	if (s->ninflow > 0)
+		box_inflow_face_kernel<<<(unsigned)((s->ninflow + LIST_BLOCK - 1) / LIST_BLOCK), LIST_BLOCK, 0, st>>>(
+			s->boxes, s->inflow, s->ninflow, s->N, s->S, s->rule, stat, s->ph, fb);
+}
+
 /* stage 4's flags: the update takes the face copies from the second set
    (the previous step of a chain left them there), and the step leaves its
    copies in the second set for the next step of the chain */
@@ -1055,8 +1078,7 @@ template <int TX, int TY> void launch_all(BoxSet* s, int solver, hipStream_t st,
 		box_refit_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->ph);
 		break;
 	case 2:
-		box_boundary_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->rule, stat, s->ph, Q_F);
-		box_inflow_face_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->rule, stat, s->ph, Q_F);
+		launch_boundary_lists(s, st, stat, Q_F);
 		box_sync_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype);
 		break;
 	case 3:
@@ -1069,8 +1091,7 @@ template <int TX, int TY> void launch_all(BoxSet* s, int solver, hipStream_t st,
 		box_update_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->P, s->ph,
 			(chain & CH_FROM2) ? Q_F2 : Q_F);
 		box_sync_copy_kernel<TX, TY, true><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->ph, Q_F, Q_F2);
-		box_boundary_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->rule, stat, s->ph, Q_F2);
-		box_inflow_face_kernel<TX, TY><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->rule, stat, s->ph, Q_F2);
+		launch_boundary_lists(s, st, stat, Q_F2);
 		if (!(chain & CH_KEEP2))
 			box_sync_copy_kernel<TX, TY, false><<<grid, block, 0, st>>>(s->boxes, s->blockbox, s->N, s->S, s->ctype, s->ph, Q_F2, Q_F);
 		break;
@@ -1158,6 +1179,8 @@ void boxset_destroy(BoxSet* s)
 	(void)hipFree(s->dt);
 	(void)hipFree(s->partial);
 	(void)hipFree(s->bar);
+	(void)hipFree(s->bnd);
+	(void)hipFree(s->inflow);
 	(void)hipHostFree(s->host_status);
 	(void)hipHostFree(s->host_dt);
 	delete s;
@@ -1245,6 +1268,7 @@ int boxset_create(BoxSet** out, int nbox, const pamhd_grid_desc* grids, const pa
 	   (source/mhd/solar_wind_box.hpp:62-205) */
 	long long off = 0;
 	const Phys& ph = s->ph;
+	std::vector<CellRef> bnd, inflow;
 	for (int b = 0; b < nbox; b++) {
 		const pamhd_grid_desc& G = grids[b];
 		const pamhd_solar_wind_desc& w = sw[b];
@@ -1275,6 +1299,11 @@ int boxset_create(BoxSet** out, int nbox, const pamhd_grid_desc* grids, const pa
 		std::vector<signed char> ct(n);
 		std::vector<unsigned char> ru(n);
 		classify(G, w, ct.data(), ru.data());
+		for (long c = 0; c < n; c++) {
+			if (ru[c] == R_NONE) continue;
+			bnd.push_back(CellRef{off + c, b, 0});
+			if (ru[c] == R_SW) inflow.push_back(CellRef{off + c, b, 0});
+		}
 		{
 			long long nf = 0;
 			for (long k = 0; k < B.nz; k++)
@@ -1327,6 +1356,18 @@ int boxset_create(BoxSet** out, int nbox, const pamhd_grid_desc* grids, const pa
 		off += n;
 	}
 	if ((e = build_blocks(s)) != hipSuccess) return bail(e, "blocks");
+	s->nbnd = (long long)bnd.size();
+	s->ninflow = (long long)inflow.size();
+	if (s->nbnd > 0) {
+		if ((e = hipMalloc(&s->bnd, sizeof(CellRef) * bnd.size())) != hipSuccess) return bail(e, "boundary list");
+		if ((e = hipMemcpy(s->bnd, bnd.data(), sizeof(CellRef) * bnd.size(), hipMemcpyHostToDevice)) != hipSuccess)
+			return bail(e, "boundary list");
+	}
+	if (s->ninflow > 0) {
+		if ((e = hipMalloc(&s->inflow, sizeof(CellRef) * inflow.size())) != hipSuccess) return bail(e, "inflow list");
+		if ((e = hipMemcpy(s->inflow, inflow.data(), sizeof(CellRef) * inflow.size(), hipMemcpyHostToDevice)) != hipSuccess)
+			return bail(e, "inflow list");
+	}
 	if ((e = hipMemset(s->P, 0, sizeof(double) * 24 * N)) != hipSuccess) return bail(e, "flux");
 	/* before the first flux pass every face's Max_v is -1, so the first
 	   lagged step is 0 */
@@ -1424,6 +1465,10 @@ int run_persistent(BoxSet* s, double cfl, long steps, hipStream_t st, std::strin
 	a.S = s->S;
 	a.ctype = s->ctype;
 	a.rule = s->rule;
+	a.bnd = s->bnd;
+	a.inflow = s->inflow;
+	a.nbnd = s->nbnd;
+	a.ninflow = s->ninflow;
 	a.bg = s->bg;
 	a.P = s->P;
 	a.dt = s->dt;
