@@ -517,6 +517,20 @@ __device__ __forceinline__ unsigned lane_up_u32(unsigned r, unsigned b)
 	return (unsigned)__builtin_amdgcn_update_dpp((int)b, (int)r, DPP_WAVE_SHR1, 0xf, 0xf, false);
 }
 
+/* lane l receives lane l + 1's r, lane 63 receives b: the DPP wavefront
+   shift the other way, for a double */
+__device__ __forceinline__ double lane_down_dpp(double r, double b)
+{
+	constexpr int DPP_WAVE_SHL1 = 0x130;
+	const unsigned long long rv = (unsigned long long)__double_as_longlong(r);
+	const unsigned long long bv = (unsigned long long)__double_as_longlong(b);
+	const unsigned lo = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)bv, (int)(unsigned)rv,
+		DPP_WAVE_SHL1, 0xf, 0xf, false);
+	const unsigned hi = (unsigned)__builtin_amdgcn_update_dpp((int)(unsigned)(bv >> 32), (int)(unsigned)(rv >> 32),
+		DPP_WAVE_SHL1, 0xf, 0xf, false);
+	return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
 __device__ __forceinline__ unsigned lane_up_u32_xbar(unsigned r, unsigned b)
 {
 	const unsigned up = (unsigned)__shfl_up((int)r, 1u, 64);
@@ -1776,6 +1790,35 @@ __global__ __launch_bounds__(BX * BY, MINB) void acc_x_kernel(GridDev g, StateAr
 		for (int a = a0; a < a1; a++) {
 			double s_face;
 			int e;
+			if constexpr (std::is_same_v<T, double>) {
+				/* FP64: the face of the lane above comes down with a DPP
+				   wavefront shift (lane 63 takes the tile's right-edge
+				   face); every lane of the wave is active here */
+				double lo[8], hi[8];
+				{
+					double U[8];
+					load_face_state_d<0>(S, g, i, a, kk, U);
+					P R;
+					apart_make<T>(U, g.ph, R);
+					const P L = apart_up(R, edge_l[threadIdx.y][a - a0]);
+					e = apart_face<T, 0>(L, R, g.ph, dt, lo, s_face);
+				}
+#pragma unroll
+				for (int q = 0; q < 8; q++) hi[q] = lane_down_dpp(lo[q], edge_r[threadIdx.y][a - a0].f[q]);
+				if (own_x) {
+					const int c = i + g.nx * (a + g.ny * kk);
+					double U[8];
+					if (e) record_error(st, e, (long long)c);
+					sp = smax(sp, s_face);
+					zero += s_face <= 0 ? 1u : 0u;
+					store_emf<0>(emf, n, c, lo);
+					load_face_state_d<0>(S, g, i, a, kk, U);
+					add_terms<double>(U, lo, hi, g.dx, il);
+#pragma unroll
+					for (int q = 0; q < 8; q++) out[q * n + c] = U[q];
+				}
+				continue;
+			}
 			{
 				double U[8];
 				load_face_state_d<0>(S, g, i, a, kk, U);
